@@ -1,0 +1,176 @@
+"""Benchmark of the SPH derivative pass on one MI355X.
+
+One step = neighbour build (ConnectivityMap) + computeSPHSumMassDensity +
+SPHHydroBase::evaluateDerivatives over the resident node state, i.e. the
+pass the metric "SPH pair-interactions/s (neighbors+derivs)" times.  Workload:
+the 3-D converging-flow generator at 128^3 nodes (cold gas, v = -r_hat, +-1%
+lattice jitter, cubic B-spline TableKernel, nPerh 1.51, Monaghan-Gingold Q with
+Cl = Cq = 1).  pair-interactions/s = unordered pairs per pass (the reference's
+ncalc) x steps / wall time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--precision fp64|mixed]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "llnl-spheral_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "SPH pair-interactions/s (neighbors+derivs) at 128³ 3D; ensemble p50/p99 solve time"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+IN_BYTES_PER_NODE = 136      # x(3) v(3) m rho P c omega H(6), FP64
+OUT_BYTES_PER_NODE = 576     # the 72 doubles of the derivative fields
+NBR_BYTES_PER_ENTRY = 4      # one int32 neighbour index per directed entry
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(budget_s=10.0, n=48):
+    """The oracle's OpenMP gather-form restatement, timed on this host on a
+    bounded sample (n^3 converging flow)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # test infrastructure, used only as the CPU baseline here
+    from sphgpu import problems
+    st, meta = problems.converging_flow(n)
+    N = meta["n_internal"]
+    T = oracle.Table(3, 1000, fast=True)
+    p = oracle.make_params()
+    threads = oracle.num_threads(fast=True)
+    passes, pairs, t_total = 0, 0, 0.0
+    t_nb = t_rho = t_dv = 0.0
+    while passes == 0 or t_total < budget_s:
+        t0 = time.perf_counter()
+        off, nbr = oracle.connectivity(3, N, st["pos"], st["H"], 2.0, fast=True)
+        t1 = time.perf_counter()
+        oracle.sum_density(T, N, st["pos"], st["mass"], st["H"], off, nbr, gather=True)
+        t2 = time.perf_counter()
+        oracle.eval_derivs(T, p, N, st, off, nbr, gather=True)
+        t3 = time.perf_counter()
+        t_nb += t1 - t0
+        t_rho += t2 - t1
+        t_dv += t3 - t2
+        t_total += t3 - t0
+        passes += 1
+        pairs = int((nbr > np.repeat(np.arange(N), np.diff(off))).sum())   # the reference's ncalc
+    model, ncpu = cpu_info()
+    return {
+        "value": pairs * passes / t_total, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+        "sample": f"converging flow {n}^3 ({N} nodes, {pairs} pairs), {passes} passes of neighbour build + "
+                  f"sum density + evaluateDerivatives (OpenMP gather form), {t_total:.1f} s",
+        "stage_ms": {"neighbors": 1e3 * t_nb / passes, "sum_density": 1e3 * t_rho / passes,
+                     "derivatives": 1e3 * t_dv / passes},
+        "cpu_model": model, "host_cpus": ncpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the pass would shard by spatial domain with one halo exchange per step).", file=sys.stderr)
+        sys.exit(2)
+
+    import sphgpu
+    from sphgpu import problems
+
+    t_setup = time.perf_counter()
+    st, meta = problems.converging_flow(args.n)
+    N = meta["n_internal"]
+    sim = sphgpu.Sim(3)
+    sim.set_table_bspline(1000)
+    sim.set_nodes(N, N)
+    sim.upload_state(st)
+    prec = sphgpu.PRECISION_FP64 if args.precision == "fp64" else sphgpu.PRECISION_MIXED
+    params = sphgpu.hydro_params(precision=prec)
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s N={N}", file=sys.stderr, flush=True)
+
+    def step():
+        sim.build_neighbors(2.0)
+        sim.sum_density()
+        sim.evaluate_derivatives(params)
+
+    for _ in range(args.warmup):
+        step()
+    sim.synchronize()
+    stage = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        t = sim.timings()
+        stage += [t["neighbors_ms"], t["density_ms"], t["derivs_ms"], t["pack_ms"]]
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    stage /= args.steps
+    pairs = sim.pair_count()
+    entries = sim.neighbor_count()
+    ms_step = 1e3 * wall / args.steps
+
+    derivs_ms = stage[2]
+    alg_bytes = N * (IN_BYTES_PER_NODE + OUT_BYTES_PER_NODE) + NBR_BYTES_PER_ENTRY * entries
+    achieved = alg_bytes / (derivs_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": pairs * args.steps / wall,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"converging flow {args.n}^3 3D (cold gas, v=-r_hat, 1% jitter), no boundary ghosts",
+            "nodes": N, "pairs": pairs, "directed_entries": entries,
+            "kernel": "TableKernel(BSpline,1000)", "nPerh": 1.51, "Q": "MonaghanGingold Cl=Cq=1",
+            "options": "XSPH, correctVelocityGradient, no grad-h, no compatible energy",
+            "precision": args.precision,
+            "stage_ms": {"neighbors": stage[0], "sum_density": stage[1], "derivatives": stage[2],
+                         "pack": stage[3]},
+        },
+        "roofline": {
+            "kernel": "k_derivs_fp64 (evaluateDerivatives pair loop + epilogue)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes": alg_bytes,
+            "note": "FP64 VALU-bound kernel; HBM fraction reported from algorithmic bytes",
+        },
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+        except Exception as e:  # the baseline is reported, never required
+            line["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * sphgpu.h -- C-ABI of the MI355X-native SPH derivative pass (libsphgpu.so).
+ *
+ * The reference exposes this path only as C++ virtuals on Physics<Dim>
+ * (src/Physics/Physics.hh:51-75) implemented by SPHHydroBase<Dim>
+ * (src/SPH/SPHHydroBase.hh:53-71), fed by ConnectivityMap / NestedGridNeighbor
+ * (src/Neighbor/ConnectivityMap.hh:46-121, src/Neighbor/NestedGridNeighbor.hh:45-51)
+ * and TableKernel (src/Kernel/TableKernel.hh:29-67).  This header is the plain-C
+ * layer under a GPU hydro package with the same signatures: plain pointers and
+ * sizes, int status codes, no C++ types.
+ *
+ * Conventions
+ *  - One sphgpu_sim per NodeList on one GPU.  Calls on distinct sims are
+ *    thread-safe; calls on one sim are not.  Every sim owns one HIP stream;
+ *    compute calls are asynchronous on it unless stated, downloads synchronise.
+ *  - Node order is the caller's (the reference's Field order): internal nodes
+ *    0..n_internal-1, then ghost nodes up to n_total-1.
+ *  - Per-node values use Spheral's in-memory element layout: Vector = D
+ *    doubles, SymTensor = D(D+1)/2 doubles (xx,xy,xz,yy,yz,zz), Tensor = D*D
+ *    doubles (row major).  Host buffers are caller-owned; device state is owned
+ *    by the sim and stays resident between calls.
+ *  - Every function returns 0 (SPHGPU_OK) or a negative SPHGPU_ERR_* code; the
+ *    message of the last failure is sphgpu_last_error(sim).  No exception crosses
+ *    the ABI.
+ */
+#ifndef SPHGPU_H
+#define SPHGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPHGPU_OK = 0,
+  SPHGPU_ERR_ARG = -1,       /* bad argument (null, size, dimension)          */
+  SPHGPU_ERR_STATE = -2,     /* call out of order (e.g. no neighbours yet)    */
+  SPHGPU_ERR_HIP = -3,       /* HIP runtime failure                           */
+  SPHGPU_ERR_UNSUPPORTED = -4,/* option not implemented on the device path     */
+  SPHGPU_ERR_CAPACITY = -5   /* a row or cell exceeded a device capacity      */
+};
+
+/* Node state fields, named as in src/Hydro/HydroFieldNames.cc:11-63. */
+enum {
+  SPHGPU_F_POSITION = 0,               /* "position"                Vector    */
+  SPHGPU_F_VELOCITY = 1,               /* "velocity"                Vector    */
+  SPHGPU_F_MASS = 2,                   /* "mass"                    Scalar    */
+  SPHGPU_F_MASS_DENSITY = 3,           /* "mass density"            Scalar    */
+  SPHGPU_F_SPECIFIC_THERMAL_ENERGY = 4,/* "specific thermal energy" Scalar    */
+  SPHGPU_F_H = 5,                      /* "H"                       SymTensor */
+  SPHGPU_F_PRESSURE = 6,               /* "pressure"                Scalar    */
+  SPHGPU_F_SOUND_SPEED = 7,            /* "sound speed"             Scalar    */
+  SPHGPU_F_OMEGA_GRADH = 8,            /* "grad h corrections"      Scalar    */
+  SPHGPU_F_CL_MULTIPLIER = 9,          /* Q Cl multiplier           Scalar    */
+  SPHGPU_F_CQ_MULTIPLIER = 10,         /* Q Cq multiplier           Scalar    */
+  SPHGPU_F_SHEAR_CORRECTION = 11,      /* Balsara shear correction  Scalar    */
+  SPHGPU_NUM_STATE_FIELDS = 12
+};
+
+/* Derivative fields written by sphgpu_evaluate_derivatives, in the order
+ * SPHHydroBase::evaluateDerivatives fetches them (src/SPH/SPHHydroBase.cc:521-540). */
+enum {
+  SPHGPU_D_RHO_SUM = 0,                /* "new mass density"              Scalar    */
+  SPHGPU_D_NORMALIZATION = 1,          /* "normalization"                 Scalar    */
+  SPHGPU_D_DXDT = 2,                   /* "delta position"                Vector    */
+  SPHGPU_D_DRHODT = 3,                 /* "delta mass density"            Scalar    */
+  SPHGPU_D_DVDT = 4,                   /* "delta velocity hydro"          Vector    */
+  SPHGPU_D_DEPSDT = 5,                 /* "delta specific thermal energy" Scalar    */
+  SPHGPU_D_DVDX = 6,                   /* "velocity gradient"             Tensor    */
+  SPHGPU_D_LOCAL_DVDX = 7,             /* "internal velocity gradient"    Tensor    */
+  SPHGPU_D_M = 8,                      /* "M SPH gradient correction"     Tensor    */
+  SPHGPU_D_LOCAL_M = 9,                /* "local M ..."                   Tensor    */
+  SPHGPU_D_DHDT = 10,                  /* "delta H"                       SymTensor */
+  SPHGPU_D_HIDEAL = 11,                /* "new H"                         SymTensor */
+  SPHGPU_D_MAX_VISCOUS_PRESSURE = 12,  /* "max viscous pressure"          Scalar    */
+  SPHGPU_D_EFF_VISCOUS_PRESSURE = 13,  /* "effective viscous pressure"    Scalar    */
+  SPHGPU_D_VISCOUS_WORK = 14,          /* "viscous work rate"             Scalar    */
+  SPHGPU_D_XSPH_WEIGHT_SUM = 15,       /* "XSPH weight sum"               Scalar    */
+  SPHGPU_D_XSPH_DELTA_V = 16,          /* "XSPH delta vi"                 Vector    */
+  SPHGPU_D_WEIGHTED_NEIGHBOR_SUM = 17, /* "weighted neighbor sum"         Scalar    */
+  SPHGPU_D_MASS_SECOND_MOMENT = 18,    /* "mass second moment"            SymTensor */
+  SPHGPU_NUM_DERIV_FIELDS = 19
+};
+
+/* Arithmetic of the pair pass. */
+enum {
+  SPHGPU_PRECISION_FP64 = 0,  /* FP64 throughout, reference summation order  */
+  SPHGPU_PRECISION_MIXED = 1  /* FP32 pair kinematics and kernel lookups,
+                                 FP64 positions and compensated accumulators */
+};
+
+/* Options of SPHHydroBase (src/SPH/SPHHydroBase.hh:53-71, defaults of the SPH()
+ * factory src/SPH/SPHHydros.py:317-339) and MonaghanGingoldViscosity
+ * (src/ArtificialViscosity/MonaghanGingoldViscosity.hh:28-31,
+ *  src/ArtificialViscosity/ArtificialViscosity.cc:64-82) that the pass reads,
+ * plus the FluidNodeList smoothing-scale limits (src/NodeList/FluidNodeLists.py:12-31). */
+typedef struct sphgpu_hydro_params {
+  double Cl;                   /* Clinear (1 for the B-spline SPH() default)   */
+  double Cq;                   /* Cquadratic                                   */
+  double eps2;                 /* Q epsilon2 (1e-2)                            */
+  double epsTensile;           /* tensile correction (0)                       */
+  double nPerh;                /* nodes per smoothing scale                    */
+  double hmin, hmax, hminratio;
+  int XSPH;                    /* 1 */
+  int correctVelocityGradient; /* 1 */
+  int evolveTotalEnergy;       /* 0 */
+  int linearInExpansion;       /* 0 */
+  int quadraticInExpansion;    /* 0 */
+  int compatibleEnergyEvolution; /* must be 0: pairAccelerations not produced */
+  int precision;               /* SPHGPU_PRECISION_* */
+} sphgpu_hydro_params;
+
+typedef struct sphgpu_sim sphgpu_sim;
+
+/* Create a simulation context for a Dim<dim> NodeList on HIP device `device`.
+ * Replaces constructing SPHHydroBase<Dim> (src/SPH/SPHHydroBase.cc:68-160). */
+int sphgpu_create(int dim, int device, sphgpu_sim** out);
+/* Release all device memory and the stream. */
+void sphgpu_destroy(sphgpu_sim* sim);
+/* Message of the last failed call on this sim (never null). */
+const char* sphgpu_last_error(const sphgpu_sim* sim);
+
+/* Install a tabulated kernel from the host TableKernel: `coeffs` holds 8 arrays
+ * of npoints doubles, [a b c] of W, [a b c] of dW/deta, the nperh values and the
+ * Wsum values (src/Kernel/TableKernel.hh:44-67, TableKernel.cc:469-548). */
+int sphgpu_set_table(sphgpu_sim* sim, int npoints, double extent, const double* coeffs);
+/* Build TableKernel(BSplineKernel<dim>(), npoints) on the host and install it
+ * (src/Kernel/TableKernel.cc:211-305). */
+int sphgpu_set_table_bspline(sphgpu_sim* sim, int npoints);
+/* Read back the installed table in the sphgpu_set_table layout. */
+int sphgpu_get_table(sphgpu_sim* sim, int npoints, double* coeffs);
+/* W(eta, Hdet) and dW/deta(eta, Hdet) from the installed table on the host
+ * (src/Kernel/TableKernelInline.hh:11-38). */
+int sphgpu_table_values(sphgpu_sim* sim, double eta, double Hdet, double* W, double* gradW);
+
+/* Size the node arrays (NodeList::numInternalNodes / numGhostNodes,
+ * src/NodeList/NodeListInline.hh:28-42).  Invalidates neighbours. */
+int sphgpu_set_nodes(sphgpu_sim* sim, int64_t n_internal, int64_t n_total);
+/* Copy one state field (n_total rows) host -> device. */
+int sphgpu_upload(sphgpu_sim* sim, int field, const double* host, int64_t n_rows);
+/* Copy one state field (n_total rows) device -> host; synchronises. */
+int sphgpu_download(sphgpu_sim* sim, int field, double* host, int64_t n_rows);
+
+/* ConnectivityMap::rebuild (src/Neighbor/ConnectivityMapInline.hh:39-74,
+ * computeConnectivity src/Neighbor/ConnectivityMap.cc:521-890): rows for the
+ * internal nodes, j in N(i) iff |H_i r_ij|^2 <= ext^2 or |H_j r_ij|^2 <= ext^2,
+ * j != i, rows ascending in node index.  Built from resident positions and H. */
+int sphgpu_build_neighbors(sphgpu_sim* sim, double kernel_extent);
+/* Total number of (directed) neighbour entries of the last build
+ * (sum of ConnectivityMap::numNeighborsForNode, ConnectivityMap.hh:86-90). */
+int sphgpu_neighbor_count(sphgpu_sim* sim, int64_t* n_entries);
+/* CSR download in caller indices: offsets[n_internal+1], neighbors[n_entries]
+ * (ConnectivityMap::connectivityForNode, ConnectivityMap.hh:67-79). */
+int sphgpu_download_neighbors(sphgpu_sim* sim, int64_t* offsets, int32_t* neighbors);
+
+/* computeSPHSumMassDensity (src/SPH/computeSPHSumMassDensity.cc:25-107) over the
+ * current neighbours; overwrites the resident "mass density" of internal nodes. */
+int sphgpu_sum_density(sphgpu_sim* sim);
+
+/* SPHHydroBase::evaluateDerivatives (src/SPH/SPHHydroBase.cc:471-928) with
+ * MonaghanGingoldViscosity::Piij fused (MonaghanGingoldViscosity.cc:63-133) and
+ * the SPHSmoothingScale epilogue (src/NodeList/SPHSmoothingScale.cc:102-227).
+ * Results stay resident; fetch them with sphgpu_download_derivative. */
+int sphgpu_evaluate_derivatives(sphgpu_sim* sim, const sphgpu_hydro_params* params);
+/* Copy one derivative field for the internal nodes (n_internal rows). */
+int sphgpu_download_derivative(sphgpu_sim* sim, int field, double* host, int64_t n_rows);
+/* Number of unordered pairs of the last evaluateDerivatives (the reference's
+ * per-node ncalc summed, src/SPH/SPHHydroBase.cc:599, 674). */
+int sphgpu_pair_count(sphgpu_sim* sim, int64_t* n_pairs);
+
+/* Device times in milliseconds of the last call of each stage, measured with
+ * HIP events on the sim's stream: [neighbours, sum density, derivatives, pack]. */
+int sphgpu_last_timings(sphgpu_sim* sim, double* ms4);
+/* Block until all work queued on the sim's stream has finished. */
+int sphgpu_synchronize(sphgpu_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHGPU_H */

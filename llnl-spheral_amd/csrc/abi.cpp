@@ -1,0 +1,304 @@
+// extern "C" entry points of libsphgpu.so (declared in include/sphgpu.h).
+// Each entry point converts exceptions to status codes and records the message
+// on the sim; nothing C++ crosses the ABI.
+#include <cstring>
+#include <algorithm>
+#include "internal.hh"
+
+using namespace sphgpu;
+
+struct sphgpu_sim : Sim {};
+
+namespace {
+
+int fail(sphgpu_sim* s, int code, const std::string& msg) {
+  if (s) s->lastError = msg;
+  return code;
+}
+
+template <class F> int guarded(sphgpu_sim* s, F&& f) {
+  if (!s) return SPHGPU_ERR_ARG;
+  try {
+    SG_CHECK_HIP(hipSetDevice(s->device));
+    f();
+    return SPHGPU_OK;
+  } catch (const Error& e) {
+    return fail(s, e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(s, SPHGPU_ERR_HIP, e.what());
+  }
+}
+
+int stateComponents(int dim, int field) {
+  switch (field) {
+    case SPHGPU_F_POSITION: case SPHGPU_F_VELOCITY: return dim;
+    case SPHGPU_F_H: return nsymOf(dim);
+    default: return 1;
+  }
+}
+
+DBuf& stateBuf(Sim& s, int field) {
+  switch (field) {
+    case SPHGPU_F_POSITION: return s.pos;
+    case SPHGPU_F_VELOCITY: return s.vel;
+    case SPHGPU_F_MASS: return s.mass;
+    case SPHGPU_F_MASS_DENSITY: return s.rho;
+    case SPHGPU_F_SPECIFIC_THERMAL_ENERGY: return s.eps;
+    case SPHGPU_F_H: return s.H;
+    case SPHGPU_F_PRESSURE: return s.P;
+    case SPHGPU_F_SOUND_SPEED: return s.cs;
+    case SPHGPU_F_OMEGA_GRADH: return s.omega;
+    case SPHGPU_F_CL_MULTIPLIER: return s.fCl;
+    case SPHGPU_F_CQ_MULTIPLIER: return s.fCq;
+    case SPHGPU_F_SHEAR_CORRECTION: return s.fshear;
+  }
+  throw Error(SPHGPU_ERR_ARG, "unknown state field id");
+}
+
+void installTable(Sim& s) {
+  const int n = s.table.n;
+  std::vector<double> inter(size_t(6) * n);
+  for (int i = 0; i < n; ++i) {
+    inter[6 * i + 0] = s.table.Ak[i]; inter[6 * i + 1] = s.table.Bk[i]; inter[6 * i + 2] = s.table.Ck[i];
+    inter[6 * i + 3] = s.table.Ag[i]; inter[6 * i + 4] = s.table.Bg[i]; inter[6 * i + 5] = s.table.Cg[i];
+  }
+  s.dTable.ensure(sizeof(double) * inter.size());
+  s.dNperh.ensure(sizeof(double) * n);
+  s.dWsum.ensure(sizeof(double) * n);
+  SG_CHECK_HIP(hipMemcpy(s.dTable.p, inter.data(), sizeof(double) * inter.size(), hipMemcpyHostToDevice));
+  SG_CHECK_HIP(hipMemcpy(s.dNperh.p, s.table.nperh.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  SG_CHECK_HIP(hipMemcpy(s.dWsum.p, s.table.wsum.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+extern "C" {
+
+int sphgpu_create(int dim, int device, sphgpu_sim** out) {
+  if (!out || dim < 1 || dim > 3) return SPHGPU_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SPHGPU_ERR_HIP;
+  if (device < 0 || device >= count) return SPHGPU_ERR_ARG;
+  sphgpu_sim* s = new sphgpu_sim();
+  s->dim = dim;
+  s->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return SPHGPU_ERR_HIP;
+  }
+  for (auto& e : s->ev) {
+    if (hipEventCreate(&e) != hipSuccess) { sphgpu_destroy(s); return SPHGPU_ERR_HIP; }
+  }
+  *out = s;
+  return SPHGPU_OK;
+}
+
+void sphgpu_destroy(sphgpu_sim* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  DBuf* bufs[] = {&s->dTable, &s->dNperh, &s->dWsum, &s->pos, &s->vel, &s->mass, &s->rho, &s->eps, &s->P, &s->cs,
+                  &s->H, &s->omega, &s->fCl, &s->fCq, &s->fshear, &s->cellOf, &s->cellCount, &s->cellStart,
+                  &s->cellFill, &s->perm, &s->iperm, &s->xs, &s->Hs, &s->rowCount, &s->offsets, &s->nbr,
+                  &s->scanTmp, &s->reduceTmp, &s->rec, &s->out};
+  for (DBuf* b : bufs) b->release();
+  for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+const char* sphgpu_last_error(const sphgpu_sim* s) { return s ? s->lastError.c_str() : "null sim"; }
+
+int sphgpu_set_table(sphgpu_sim* s, int npoints, double extent, const double* c) {
+  return guarded(s, [&] {
+    SG_REQUIRE(npoints >= 4 && extent > 0.0 && c, SPHGPU_ERR_ARG, "bad table arguments");
+    TableHost& t = s->table;
+    t.n = npoints;
+    t.extent = extent;
+    t.step = extent / (npoints - 1);
+    auto take = [&](int a, std::vector<double>& v) { v.assign(c + size_t(a) * npoints, c + size_t(a + 1) * npoints); };
+    take(0, t.Ak); take(1, t.Bk); take(2, t.Ck); take(3, t.Ag); take(4, t.Bg); take(5, t.Cg);
+    take(6, t.nperh); take(7, t.wsum);
+    installTable(*s);
+  });
+}
+
+int sphgpu_set_table_bspline(sphgpu_sim* s, int npoints) {
+  return guarded(s, [&] {
+    SG_REQUIRE(npoints >= 4, SPHGPU_ERR_ARG, "table needs at least 4 points");
+    buildBSplineTable(s->dim, npoints, s->table);
+    installTable(*s);
+  });
+}
+
+int sphgpu_get_table(sphgpu_sim* s, int npoints, double* c) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->table.n > 0, SPHGPU_ERR_STATE, "no table installed");
+    SG_REQUIRE(npoints == s->table.n && c, SPHGPU_ERR_ARG, "npoints does not match the installed table");
+    const std::vector<double>* arr[8] = {&s->table.Ak, &s->table.Bk, &s->table.Ck, &s->table.Ag,
+                                         &s->table.Bg, &s->table.Cg, &s->table.nperh, &s->table.wsum};
+    for (int a = 0; a < 8; ++a) std::memcpy(c + size_t(a) * npoints, arr[a]->data(), sizeof(double) * npoints);
+  });
+}
+
+int sphgpu_table_values(sphgpu_sim* s, double eta, double Hdet, double* W, double* gW) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->table.n > 0, SPHGPU_ERR_STATE, "no table installed");
+    SG_REQUIRE(eta >= 0.0 && Hdet >= 0.0, SPHGPU_ERR_ARG, "eta and Hdet must be non-negative");
+    if (W) *W = s->table.kernelValue(eta, Hdet);
+    if (gW) *gW = s->table.gradValue(eta, Hdet);
+  });
+}
+
+int sphgpu_set_nodes(sphgpu_sim* s, int64_t nInt, int64_t nTot) {
+  return guarded(s, [&] {
+    SG_REQUIRE(nInt >= 0 && nTot >= nInt && nTot < (int64_t(1) << 31) - 1, SPHGPU_ERR_ARG, "bad node counts");
+    s->nInt = nInt;
+    s->nTot = nTot;
+    s->neighborsValid = false;
+    s->derivsValid = false;
+    s->hasMultipliers = false;
+    const size_t n = size_t(std::max<int64_t>(nTot, 1));
+    const int D = s->dim, NS = nsymOf(D);
+    s->pos.ensure(sizeof(double) * n * D);
+    s->vel.ensure(sizeof(double) * n * D);
+    s->H.ensure(sizeof(double) * n * NS);
+    DBuf* scal[] = {&s->mass, &s->rho, &s->eps, &s->P, &s->cs, &s->omega};
+    for (DBuf* b : scal) b->ensure(sizeof(double) * n);
+    // omega defaults to 1 (no grad-h correction)
+    std::vector<double> ones(n, 1.0);
+    SG_CHECK_HIP(hipMemcpy(s->omega.p, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  });
+}
+
+int sphgpu_upload(sphgpu_sim* s, int field, const double* host, int64_t nRows) {
+  return guarded(s, [&] {
+    SG_REQUIRE(host && nRows == s->nTot, SPHGPU_ERR_ARG, "upload needs n_total rows");
+    SG_REQUIRE(field >= 0 && field < SPHGPU_NUM_STATE_FIELDS, SPHGPU_ERR_ARG, "unknown state field id");
+    const size_t bytes = sizeof(double) * size_t(nRows) * stateComponents(s->dim, field);
+    DBuf& b = stateBuf(*s, field);
+    if (field == SPHGPU_F_CL_MULTIPLIER || field == SPHGPU_F_CQ_MULTIPLIER || field == SPHGPU_F_SHEAR_CORRECTION) {
+      // first multiplier upload materialises all three with 1.0
+      if (!s->hasMultipliers) {
+        std::vector<double> ones(static_cast<size_t>(std::max<int64_t>(s->nTot, 1)), 1.0);
+        for (DBuf* m : {&s->fCl, &s->fCq, &s->fshear}) {
+          m->ensure(sizeof(double) * ones.size());
+          SG_CHECK_HIP(hipMemcpy(m->p, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
+        }
+        s->hasMultipliers = true;
+      }
+    }
+    b.ensure(std::max<size_t>(bytes, 8));
+    if (bytes) SG_CHECK_HIP(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+    // Neighbours stay those of the last build (the reference's finalize sums density
+    // with the connectivity of the step start); the sorted copies are refreshed lazily.
+    if (field == SPHGPU_F_POSITION || field == SPHGPU_F_H) s->xsStale = true;
+    s->derivsValid = false;
+  });
+}
+
+int sphgpu_download(sphgpu_sim* s, int field, double* host, int64_t nRows) {
+  return guarded(s, [&] {
+    SG_REQUIRE(host && nRows == s->nTot, SPHGPU_ERR_ARG, "download needs n_total rows");
+    SG_REQUIRE(field >= 0 && field < SPHGPU_NUM_STATE_FIELDS, SPHGPU_ERR_ARG, "unknown state field id");
+    const size_t bytes = sizeof(double) * size_t(nRows) * stateComponents(s->dim, field);
+    DBuf& b = stateBuf(*s, field);
+    if ((field == SPHGPU_F_CL_MULTIPLIER || field == SPHGPU_F_CQ_MULTIPLIER || field == SPHGPU_F_SHEAR_CORRECTION) &&
+        !s->hasMultipliers) {
+      std::fill(host, host + nRows, 1.0);
+      return;
+    }
+    if (bytes) SG_CHECK_HIP(hipMemcpyAsync(host, b.p, bytes, hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int sphgpu_build_neighbors(sphgpu_sim* s, double ext) {
+  return guarded(s, [&] { buildNeighbors(*s, ext); });
+}
+
+int sphgpu_neighbor_count(sphgpu_sim* s, int64_t* n) {
+  return guarded(s, [&] {
+    SG_REQUIRE(n, SPHGPU_ERR_ARG, "null output");
+    SG_REQUIRE(s->neighborsValid, SPHGPU_ERR_STATE, "no current neighbour build");
+    *n = s->nEntries;
+  });
+}
+
+int sphgpu_download_neighbors(sphgpu_sim* s, int64_t* offsets, int32_t* nbr) {
+  return guarded(s, [&] {
+    SG_REQUIRE(offsets, SPHGPU_ERR_ARG, "null offsets");
+    SG_REQUIRE(s->neighborsValid, SPHGPU_ERR_STATE, "no current neighbour build");
+    const int64_t n = s->nTot;
+    std::vector<int64_t> off(static_cast<size_t>(n + 1));
+    std::vector<int32_t> perm(static_cast<size_t>(n)), iperm(static_cast<size_t>(n)), rows(static_cast<size_t>(std::max<int64_t>(s->nEntries, 1)));
+    SG_CHECK_HIP(hipMemcpy(off.data(), s->offsets.p, sizeof(int64_t) * size_t(n + 1), hipMemcpyDeviceToHost));
+    SG_CHECK_HIP(hipMemcpy(perm.data(), s->perm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
+    SG_CHECK_HIP(hipMemcpy(iperm.data(), s->iperm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
+    if (s->nEntries)
+      SG_CHECK_HIP(hipMemcpy(rows.data(), s->nbr.p, sizeof(int32_t) * size_t(s->nEntries), hipMemcpyDeviceToHost));
+    int64_t w = 0;
+    for (int64_t i = 0; i < s->nInt; ++i) {
+      offsets[i] = w;
+      const int64_t k = iperm[size_t(i)];
+      for (int64_t q = off[size_t(k)]; q < off[size_t(k + 1)]; ++q) {
+        if (nbr) nbr[w] = perm[size_t(rows[size_t(q)])];
+        ++w;
+      }
+    }
+    offsets[s->nInt] = w;
+  });
+}
+
+int sphgpu_sum_density(sphgpu_sim* s) {
+  return guarded(s, [&] { sumDensity(*s); });
+}
+
+int sphgpu_evaluate_derivatives(sphgpu_sim* s, const sphgpu_hydro_params* p) {
+  return guarded(s, [&] {
+    SG_REQUIRE(p, SPHGPU_ERR_ARG, "null params");
+    evalDerivs(*s, *p);
+  });
+}
+
+int sphgpu_download_derivative(sphgpu_sim* s, int field, double* host, int64_t nRows) {
+  return guarded(s, [&] {
+    SG_REQUIRE(host && nRows == s->nInt, SPHGPU_ERR_ARG, "derivative download needs n_internal rows");
+    SG_REQUIRE(field >= 0 && field < SPHGPU_NUM_DERIV_FIELDS, SPHGPU_ERR_ARG, "unknown derivative field id");
+    SG_REQUIRE(s->derivsValid, SPHGPU_ERR_STATE, "no current derivatives");
+    const int comps = derivComponents(s->dim, field);
+    const int64_t n = s->outStride;
+    std::vector<double> block(static_cast<size_t>(n) * comps);
+    std::vector<int32_t> iperm(static_cast<size_t>(s->nTot));
+    const double* src = s->out.as<double>() + int64_t(derivOffsetDoubles(s->dim, field)) * n;
+    SG_CHECK_HIP(hipMemcpy(block.data(), src, sizeof(double) * block.size(), hipMemcpyDeviceToHost));
+    SG_CHECK_HIP(hipMemcpy(iperm.data(), s->iperm.p, sizeof(int32_t) * iperm.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nRows; ++i) {
+      const int64_t k = iperm[size_t(i)];
+      std::memcpy(host + i * comps, block.data() + k * comps, sizeof(double) * comps);
+    }
+  });
+}
+
+int sphgpu_pair_count(sphgpu_sim* s, int64_t* n) {
+  return guarded(s, [&] {
+    SG_REQUIRE(n, SPHGPU_ERR_ARG, "null output");
+    SG_REQUIRE(s->derivsValid, SPHGPU_ERR_STATE, "no current derivatives");
+    *n = s->ncalc;
+  });
+}
+
+int sphgpu_last_timings(sphgpu_sim* s, double* ms) {
+  return guarded(s, [&] {
+    SG_REQUIRE(ms, SPHGPU_ERR_ARG, "null output");
+    ms[0] = s->timing.neighbors_ms; ms[1] = s->timing.density_ms; ms[2] = s->timing.derivs_ms; ms[3] = s->timing.pack_ms;
+  });
+}
+
+int sphgpu_synchronize(sphgpu_sim* s) {
+  return guarded(s, [&] { SG_CHECK_HIP(hipStreamSynchronize(s->stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// Internal declarations of libsphgpu (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include "../../include/sphgpu.h"
+
+namespace sphgpu {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SG_CHECK_HIP(expr)                                                                     \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      throw ::sphgpu::Error(SPHGPU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define SG_REQUIRE(cond, code, msg)                 \
+  do {                                              \
+    if (!(cond)) throw ::sphgpu::Error(code, msg);  \
+  } while (0)
+
+// Device buffer with grow-only semantics.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t nb) {
+    if (nb <= bytes) return;
+    if (p) SG_CHECK_HIP(hipFree(p));
+    p = nullptr; bytes = 0;
+    if (nb == 0) return;
+    SG_CHECK_HIP(hipMalloc(&p, nb));
+    bytes = nb;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Tabulated kernel as held on the device: per table point i1 the parabola
+// coefficients of W and dW/deta, plus the nperh <-> Wsum tables.
+struct TableHost {
+  int n = 0;
+  double extent = 2.0, step = 0.0;
+  std::vector<double> Ak, Bk, Ck, Ag, Bg, Cg, nperh, wsum;
+  double kernelValue(double eta, double Hdet) const;
+  double gradValue(double eta, double Hdet) const;
+};
+void buildBSplineTable(int dim, int npoints, TableHost& t);
+
+struct Grid {
+  double lo[3] = {0, 0, 0};
+  double cell = 1.0;        // cell edge (>= search radius)
+  double R2 = 0.0;          // squared prefilter radius
+  int nc[3] = {1, 1, 1};    // cells per axis
+  int nb[3] = {1, 1, 1};    // 4-cell bricks per axis
+  int64_t ncells = 0;       // nb[0]*nb[1]*nb[2]*brickVolume
+};
+
+struct Timing {
+  float neighbors_ms = 0, density_ms = 0, derivs_ms = 0, pack_ms = 0;
+};
+
+struct Sim {
+  int dim = 3;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string lastError;
+  int64_t nInt = 0, nTot = 0;
+
+  TableHost table;
+  DBuf dTable;              // 6*n interleaved [Ak Bk Ck Ag Bg Cg] per point
+  DBuf dNperh, dWsum;
+
+  // node state, caller order
+  DBuf pos, vel, mass, rho, eps, P, cs, H, omega, fCl, fCq, fshear;
+  bool hasMultipliers = false;
+
+  // neighbour structure (sorted order)
+  Grid grid;
+  double kernelExtent = 2.0;
+  bool neighborsValid = false;
+  bool xsStale = false;      // positions/H uploaded since the last build
+  DBuf cellOf, cellCount, cellStart, cellFill;
+  DBuf perm;                // sorted slot -> caller index
+  DBuf iperm;               // caller index -> sorted slot
+  DBuf xs, Hs;              // positions / H in sorted order
+  DBuf rowCount, offsets, nbr;
+  int64_t nEntries = 0;
+  DBuf scanTmp, reduceTmp;
+
+  // packed per-node records and outputs (sorted order)
+  DBuf rec;
+  DBuf out;                 // all derivative fields, field-major, sorted order
+  int64_t outStride = 0;    // doubles per field block = nTot rounded
+  bool derivsValid = false;
+  int64_t ncalc = 0;        // unordered pairs evaluated by the last pass
+
+  hipEvent_t ev[8] = {};
+  Timing timing;
+};
+
+// neighbors.hip
+void buildNeighbors(Sim& s, double kernelExtent);
+void regatherSorted(Sim& s);   // refresh xs/Hs from the caller-order state
+// hydro.hip
+void sumDensity(Sim& s);
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p);
+// scan helpers (neighbors.hip)
+void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
+void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);
+
+inline int nsymOf(int dim) { return dim * (dim + 1) / 2; }
+
+// Number of doubles per node of each derivative output field, in the order of
+// sphgpu_field ids SPHGPU_D_* (see sphgpu.h).
+int derivComponents(int dim, int field);
+int derivOffsetDoubles(int dim, int field);   // offset of a field's block, per node
+int derivTotalDoubles(int dim);
+
+}  // namespace sphgpu

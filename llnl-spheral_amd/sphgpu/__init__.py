@@ -1,0 +1,263 @@
+"""Python binding of libsphgpu.so, the MI355X SPH derivative pass.
+
+This is a thin ctypes layer over the C-ABI in include/sphgpu.h; every compute
+call runs the HIP kernels of the library.  There is no CPU fallback: if the
+library is missing, importing works but creating a Sim raises.
+
+Field names follow src/Hydro/HydroFieldNames.cc:11-63 of the reference.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libsphgpu.so")
+
+# state fields (SPHGPU_F_*)
+STATE_FIELDS = {
+    "position": 0, "velocity": 1, "mass": 2, "mass density": 3, "specific thermal energy": 4,
+    "H": 5, "pressure": 6, "sound speed": 7, "grad h corrections": 8,
+    "Cl multiplier": 9, "Cq multiplier": 10, "Balsara shear correction": 11,
+}
+# derivative fields (SPHGPU_D_*), in the order evaluateDerivatives fetches them
+DERIV_FIELDS = [
+    "new mass density", "normalization", "delta position", "delta mass density",
+    "delta velocity hydro", "delta specific thermal energy", "velocity gradient",
+    "internal velocity gradient", "M SPH gradient correction", "local M SPH gradient correction",
+    "delta H", "new H", "max viscous pressure", "effective viscous pressure", "viscous work rate",
+    "XSPH weight sum", "XSPH delta vi", "weighted neighbor sum", "mass second moment",
+]
+# short aliases used by the tests / oracle
+DERIV_ALIASES = [
+    "rhoSum", "normalization", "DxDt", "DrhoDt", "DvDt", "DepsDt", "DvDx", "localDvDx", "M", "localM",
+    "DHDt", "Hideal", "maxViscousPressure", "effViscousPressure", "viscousWork", "XSPHWeightSum",
+    "XSPHDeltaV", "weightedNeighborSum", "massSecondMoment",
+]
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+ERRORS = {-1: "ARG", -2: "STATE", -3: "HIP", -4: "UNSUPPORTED", -5: "CAPACITY"}
+
+
+class SphGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sphgpu error {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class HydroParams(C.Structure):
+    _fields_ = [("Cl", C.c_double), ("Cq", C.c_double), ("eps2", C.c_double), ("epsTensile", C.c_double),
+                ("nPerh", C.c_double), ("hmin", C.c_double), ("hmax", C.c_double), ("hminratio", C.c_double),
+                ("XSPH", C.c_int), ("correctVelocityGradient", C.c_int), ("evolveTotalEnergy", C.c_int),
+                ("linearInExpansion", C.c_int), ("quadraticInExpansion", C.c_int),
+                ("compatibleEnergyEvolution", C.c_int), ("precision", C.c_int)]
+
+
+def hydro_params(**kw):
+    """Defaults of the reference's SPH() factory for the B-spline kernel
+    (src/SPH/SPHHydros.py:317-384) with compatible energy and grad-h off."""
+    d = dict(Cl=1.0, Cq=1.0, eps2=1.0e-2, epsTensile=0.0, nPerh=1.51, hmin=1.0e-20, hmax=1.0e20,
+             hminratio=0.1, XSPH=1, correctVelocityGradient=1, evolveTotalEnergy=0,
+             linearInExpansion=0, quadraticInExpansion=0, compatibleEnergyEvolution=0,
+             precision=PRECISION_FP64)
+    d.update(kw)
+    return HydroParams(**d)
+
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_lib = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsphgpu.so not built ({LIB_PATH}); run llnl-spheral_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "sphgpu_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
+            "sphgpu_destroy": (None, [vp]),
+            "sphgpu_last_error": (C.c_char_p, [vp]),
+            "sphgpu_set_table": (C.c_int, [vp, C.c_int, C.c_double, _dp]),
+            "sphgpu_set_table_bspline": (C.c_int, [vp, C.c_int]),
+            "sphgpu_get_table": (C.c_int, [vp, C.c_int, _dp]),
+            "sphgpu_table_values": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
+            "sphgpu_set_nodes": (C.c_int, [vp, C.c_int64, C.c_int64]),
+            "sphgpu_upload": (C.c_int, [vp, C.c_int, _dp, C.c_int64]),
+            "sphgpu_download": (C.c_int, [vp, C.c_int, _dp, C.c_int64]),
+            "sphgpu_build_neighbors": (C.c_int, [vp, C.c_double]),
+            "sphgpu_neighbor_count": (C.c_int, [vp, _i64p]),
+            "sphgpu_download_neighbors": (C.c_int, [vp, _i64p, _i32p]),
+            "sphgpu_sum_density": (C.c_int, [vp]),
+            "sphgpu_evaluate_derivatives": (C.c_int, [vp, C.POINTER(HydroParams)]),
+            "sphgpu_download_derivative": (C.c_int, [vp, C.c_int, _dp, C.c_int64]),
+            "sphgpu_pair_count": (C.c_int, [vp, _i64p]),
+            "sphgpu_last_timings": (C.c_int, [vp, _dp]),
+            "sphgpu_synchronize": (C.c_int, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, ct=_dp):
+    return a.ctypes.data_as(ct)
+
+
+def state_components(field, dim):
+    if field in ("position", "velocity"):
+        return dim
+    if field == "H":
+        return dim * (dim + 1) // 2
+    return 1
+
+
+def deriv_components(field, dim):
+    i = DERIV_FIELDS.index(field) if field in DERIV_FIELDS else DERIV_ALIASES.index(field)
+    a = DERIV_ALIASES[i]
+    if a in ("DxDt", "DvDt", "XSPHDeltaV"):
+        return dim
+    if a in ("DvDx", "localDvDx", "M", "localM"):
+        return dim * dim
+    if a in ("DHDt", "Hideal", "massSecondMoment"):
+        return dim * (dim + 1) // 2
+    return 1
+
+
+class Sim:
+    """One NodeList resident on one GPU (sphgpu_sim)."""
+
+    def __init__(self, dim, device=0):
+        self.L = lib()
+        self.dim = dim
+        h = C.c_void_p()
+        rc = self.L.sphgpu_create(dim, device, C.byref(h))
+        if rc != 0:
+            raise SphGpuError(rc, "sphgpu_create failed (no HIP device?)")
+        self.h = h
+        self.n_internal = 0
+        self.n_total = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sphgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SphGpuError(rc, self.L.sphgpu_last_error(self.h).decode())
+
+    # kernel table ------------------------------------------------------------
+    def set_table_bspline(self, npoints=1000):
+        self._chk(self.L.sphgpu_set_table_bspline(self.h, npoints))
+        self.npoints = npoints
+
+    def set_table(self, npoints, extent, coeffs):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1)
+        assert c.size == 8 * npoints
+        self._chk(self.L.sphgpu_set_table(self.h, npoints, extent, _p(c)))
+        self.npoints = npoints
+
+    def get_table(self):
+        c = np.zeros(8 * self.npoints)
+        self._chk(self.L.sphgpu_get_table(self.h, self.npoints, _p(c)))
+        names = ["Ak", "Bk", "Ck", "Ag", "Bg", "Cg", "nperh", "wsum"]
+        return {k: c[i * self.npoints:(i + 1) * self.npoints].copy() for i, k in enumerate(names)}
+
+    def table_values(self, eta, Hdet=1.0):
+        W = C.c_double()
+        g = C.c_double()
+        self._chk(self.L.sphgpu_table_values(self.h, eta, Hdet, C.byref(W), C.byref(g)))
+        return W.value, g.value
+
+    # state --------------------------------------------------------------------
+    def set_nodes(self, n_internal, n_total):
+        self._chk(self.L.sphgpu_set_nodes(self.h, n_internal, n_total))
+        self.n_internal = n_internal
+        self.n_total = n_total
+
+    def upload(self, field, arr):
+        fid = STATE_FIELDS[field]
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape[0] == self.n_total, (field, a.shape, self.n_total)
+        assert a.size == self.n_total * state_components(field, self.dim), (field, a.shape)
+        self._chk(self.L.sphgpu_upload(self.h, fid, _p(a), self.n_total))
+
+    def upload_state(self, state):
+        names = {"pos": "position", "vel": "velocity", "mass": "mass", "rho": "mass density",
+                 "eps": "specific thermal energy", "H": "H", "P": "pressure", "cs": "sound speed",
+                 "omega": "grad h corrections", "fCl": "Cl multiplier", "fCq": "Cq multiplier",
+                 "fshear": "Balsara shear correction"}
+        for k, v in state.items():
+            if k in names and v is not None:
+                self.upload(names[k], v)
+
+    def download(self, field):
+        c = state_components(field, self.dim)
+        a = np.zeros((self.n_total, c))
+        self._chk(self.L.sphgpu_download(self.h, STATE_FIELDS[field], _p(a), self.n_total))
+        return a[:, 0] if c == 1 else a
+
+    # passes -------------------------------------------------------------------
+    def build_neighbors(self, kernel_extent=2.0):
+        self._chk(self.L.sphgpu_build_neighbors(self.h, kernel_extent))
+
+    def neighbor_count(self):
+        n = C.c_int64()
+        self._chk(self.L.sphgpu_neighbor_count(self.h, C.byref(n)))
+        return n.value
+
+    def neighbors(self):
+        total = self.neighbor_count()
+        off = np.zeros(self.n_internal + 1, dtype=np.int64)
+        nbr = np.zeros(max(total, 1), dtype=np.int32)
+        self._chk(self.L.sphgpu_download_neighbors(self.h, _p(off, _i64p), _p(nbr, _i32p)))
+        return off, nbr[:total]
+
+    def sum_density(self):
+        self._chk(self.L.sphgpu_sum_density(self.h))
+
+    def evaluate_derivatives(self, params=None, **kw):
+        p = params if params is not None else hydro_params(**kw)
+        self._chk(self.L.sphgpu_evaluate_derivatives(self.h, C.byref(p)))
+
+    def derivative(self, field):
+        i = DERIV_FIELDS.index(field) if field in DERIV_FIELDS else DERIV_ALIASES.index(field)
+        c = deriv_components(field, self.dim)
+        a = np.zeros((self.n_internal, c))
+        self._chk(self.L.sphgpu_download_derivative(self.h, i, _p(a), self.n_internal))
+        return a[:, 0] if c == 1 else a
+
+    def derivatives(self):
+        return {a: self.derivative(a) for a in DERIV_ALIASES}
+
+    def pair_count(self):
+        n = C.c_int64()
+        self._chk(self.L.sphgpu_pair_count(self.h, C.byref(n)))
+        return n.value
+
+    def timings(self):
+        t = np.zeros(4)
+        self._chk(self.L.sphgpu_last_timings(self.h, _p(t)))
+        return dict(neighbors_ms=t[0], density_ms=t[1], derivs_ms=t[2], pack_ms=t[3])
+
+    def synchronize(self):
+        self._chk(self.L.sphgpu_synchronize(self.h))

@@ -1,0 +1,102 @@
+"""Synthetic initial conditions for the SPH derivative pass.
+
+The lattice follows GenerateNodeDistribution3d.latticeDistribution
+(src/NodeGenerators/GenerateNodeDistribution3d.py:573-626): index
+i + nx*(j + ny*k), cell-centred positions xmin + (i + 1/2) dx, mass
+dx*dy*dz*rho, H = diag(1/(nPerh dx)).  Jitter (seeded, uniform in +-jitter*dx
+per coordinate) is added on top, as the benchmark configurations ask.
+"""
+import numpy as np
+
+
+def sym_identity(dim, n, value):
+    """n SymTensors value*I in storage order (xx,xy,xz,yy,yz,zz)."""
+    ns = dim * (dim + 1) // 2
+    H = np.zeros((n, ns))
+    k = 0
+    for a in range(dim):
+        for b in range(a, dim):
+            if a == b:
+                H[:, k] = value
+            k += 1
+    return H
+
+
+def lattice(dim, n, jitter=0.0, seed=0, nPerh=1.51, xmin=0.0, xmax=1.0, rho=1.0):
+    """n^dim nodes in [xmin, xmax]^dim.  Returns pos (N,dim), mass (N,), H (N,nsym)."""
+    N = n ** dim
+    dx = (xmax - xmin) / n
+    idx = np.arange(N, dtype=np.int64)
+    pos = np.zeros((N, dim))
+    rem = idx.copy()
+    for a in range(dim):
+        ia = rem % n
+        rem //= n
+        pos[:, a] = xmin + (ia + 0.5) * dx
+    if jitter:
+        rng = np.random.default_rng(seed)
+        pos += rng.uniform(-jitter, jitter, size=(N, dim)) * dx
+    vol = 1.0
+    for _ in range(dim):
+        vol = vol * dx
+    rho_arr = np.full(N, rho) if np.isscalar(rho) else rho(pos)
+    mass = vol * rho_arr
+    H = sym_identity(dim, N, 1.0 / (nPerh * dx))
+    return pos, mass, H, dx
+
+
+def gamma_law(rho, eps, gamma):
+    """GammaLawGas pressure and sound speed (src/Material/GammaLawGas.cc:163-170, 215-223),
+    no pressure limits active."""
+    g1 = gamma - 1.0
+    P = (g1 * rho) * eps
+    cs = np.sqrt(np.maximum(0.0, (gamma * g1) * eps))
+    return P, cs
+
+
+def converging_flow(n, dim=3, jitter=0.01, seed=12345, nPerh=1.51, eps0=1.0e-6, gamma=5.0 / 3.0):
+    """Cold, uniform-density gas with radial inflow v = -r_hat about the box centre
+    (3-D converging-flow configuration; 32^3 and 128^3 in the benchmarks)."""
+    pos, mass, H, dx = lattice(dim, n, jitter=jitter, seed=seed, nPerh=nPerh)
+    N = pos.shape[0]
+    rho = np.ones(N)
+    eps = np.full(N, eps0)
+    d = pos - 0.5
+    r = np.sqrt((d * d).sum(axis=1))
+    vel = -d / np.maximum(r, 1e-300)[:, None]
+    P, cs = gamma_law(rho, eps, gamma)
+    return dict(pos=pos, vel=vel, mass=mass, rho=rho, eps=eps, P=P, cs=cs, H=H,
+                omega=np.ones(N)), dict(n_internal=N, dx=dx, gamma=gamma, nPerh=nPerh)
+
+
+def random_state(dim, n, seed=7, jitter=0.2, nPerh=1.51, gamma=1.4, vel_scale=0.3, rho_var=0.2,
+                 h_var=0.0):
+    """Perturbed lattice with random velocity, density and energy fields, for
+    parity tests that exercise every branch of the pair loop (compression and
+    expansion, Q on and off).  h_var > 0 gives each node its own h."""
+    pos, mass, H, dx = lattice(dim, n, jitter=jitter, seed=seed, nPerh=nPerh)
+    N = pos.shape[0]
+    rng = np.random.default_rng(seed + 1)
+    rho = 1.0 + rho_var * rng.uniform(-1, 1, N)
+    mass = mass * rho
+    eps = 1.0 + 0.5 * rng.uniform(-1, 1, N)
+    vel = vel_scale * rng.standard_normal((N, dim))
+    if h_var:
+        scale = 1.0 + h_var * rng.uniform(-1, 1, N)
+        H = H * scale[:, None]
+    P, cs = gamma_law(rho, eps, gamma)
+    return dict(pos=pos, vel=vel, mass=mass, rho=rho, eps=eps, P=P, cs=cs, H=H,
+                omega=np.ones(N)), dict(n_internal=N, dx=dx, gamma=gamma, nPerh=nPerh)
+
+
+def linear_velocity(dim, n, A, seed=14892042, jitter=0.2, nPerh=1.51):
+    """v = A.x on a perturbed lattice (src/SPH/tests/testLinearVelocityGradient.py)."""
+    pos, mass, H, dx = lattice(dim, n, jitter=jitter, seed=seed, nPerh=nPerh)
+    N = pos.shape[0]
+    A = np.asarray(A, dtype=np.float64).reshape(dim, dim)
+    vel = pos @ A.T
+    rho = np.ones(N)
+    eps = np.ones(N)
+    P, cs = gamma_law(rho, eps, 5.0 / 3.0)
+    return dict(pos=pos, vel=vel, mass=mass, rho=rho, eps=eps, P=P, cs=cs, H=H,
+                omega=np.ones(N)), dict(n_internal=N, dx=dx, nPerh=nPerh)

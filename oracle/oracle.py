@@ -1,0 +1,210 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module.  It loads oracle/_build/liboracle.so (parity build, no FMA contraction) or
+liboracle_fast.so (timed baseline build) produced by oracle/Makefile.
+
+Array conventions follow Spheral's Field storage: one row per node, Vector =
+D doubles, SymTensor = D(D+1)/2 doubles (xx,xy,xz,yy,yz,zz), Tensor = D*D doubles
+(row major).  Node order is the reference's: internal nodes first, then ghosts.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+
+
+class Params(C.Structure):
+    _fields_ = [("Cl", C.c_double), ("Cq", C.c_double), ("eps2", C.c_double),
+                ("epsTensile", C.c_double),
+                ("nPerh", C.c_double), ("hmin", C.c_double), ("hmax", C.c_double),
+                ("hminratio", C.c_double),
+                ("XSPH", C.c_int), ("correctVelocityGradient", C.c_int),
+                ("evolveTotalEnergy", C.c_int),
+                ("linearInExpansion", C.c_int), ("quadraticInExpansion", C.c_int)]
+
+
+class State(C.Structure):
+    _fields_ = [(n, _dp) for n in ("pos", "vel", "mass", "rho", "eps", "P", "cs", "H", "omega",
+                                   "fCl", "fCq", "fshear")]
+
+
+OUTPUT_FIELDS = ["rhoSum", "normalization", "DxDt", "DrhoDt", "DvDt", "DepsDt", "DvDx",
+                 "localDvDx", "M", "localM", "DHDt", "Hideal", "maxViscousPressure",
+                 "effViscousPressure", "viscousWork", "XSPHWeightSum", "XSPHDeltaV",
+                 "weightedNeighborSum", "massSecondMoment"]
+
+
+class Outputs(C.Structure):
+    _fields_ = [(n, _dp) for n in OUTPUT_FIELDS] + [("ncalc", _i64p)]
+
+
+def ncomp(field, dim):
+    nsym = dim * (dim + 1) // 2
+    return {"DxDt": dim, "DvDt": dim, "XSPHDeltaV": dim,
+            "DvDx": dim * dim, "localDvDx": dim * dim, "M": dim * dim, "localM": dim * dim,
+            "DHDt": nsym, "Hideal": nsym, "massSecondMoment": nsym}.get(field, 1)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_libs = {}
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(BUILD, name)
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_table_create.restype = C.c_void_p
+        L.orc_table_create.argtypes = [C.c_int, C.c_int]
+        L.orc_table_destroy.argtypes = [C.c_void_p]
+        L.orc_table_npoints.argtypes = [C.c_void_p]
+        L.orc_table_export.argtypes = [C.c_void_p, _dp]
+        for f in ("orc_table_kernel", "orc_table_grad", "orc_table_grad2"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_table_nperh_from_wsum.restype = C.c_double
+        L.orc_table_nperh_from_wsum.argtypes = [C.c_void_p, C.c_double]
+        L.orc_bspline.restype = C.c_double
+        L.orc_bspline.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+        L.orc_connectivity.restype = C.c_int64
+        L.orc_connectivity.argtypes = [C.c_int, C.c_int64, C.c_int64, _dp, _dp, C.c_double, C.c_int,
+                                       _i64p, _i32p]
+        L.orc_sum_density.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int64, _dp, _dp, _dp,
+                                      _i64p, _i32p, _dp]
+        L.orc_eval_derivs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params), C.c_int64,
+                                      C.c_int64, C.POINTER(State), _i64p, _i32p, C.POINTER(Outputs)]
+        L.orc_gamma_law.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                    _dp, _dp]
+        L.orc_num_threads.restype = C.c_int
+        _libs[name] = L
+    return _libs[name]
+
+
+def _p(a, ct=_dp):
+    return a.ctypes.data_as(ct) if a is not None else C.cast(None, ct)
+
+
+class Table:
+    """TableKernel(BSplineKernel<dim>, npoints) restated (TableKernel.cc:211-305)."""
+
+    def __init__(self, dim, npoints=1000, fast=False):
+        self.L = lib(fast)
+        self.dim = dim
+        self.h = self.L.orc_table_create(dim, npoints)
+        if not self.h:
+            raise ValueError("bad table arguments")
+        self.n = npoints
+
+    def __del__(self):
+        try:
+            self.L.orc_table_destroy(self.h)
+        except Exception:
+            pass
+
+    def arrays(self):
+        out = np.zeros(11 * self.n)
+        self.L.orc_table_export(self.h, _p(out))
+        names = ["Ak", "Bk", "Ck", "Ag", "Bg", "Cg", "Ag2", "Bg2", "Cg2", "nperh", "wsum"]
+        return {k: out[i * self.n:(i + 1) * self.n].copy() for i, k in enumerate(names)}
+
+    def W(self, eta, Hdet=1.0):
+        return self.L.orc_table_kernel(self.h, eta, Hdet)
+
+    def gradW(self, eta, Hdet=1.0):
+        return self.L.orc_table_grad(self.h, eta, Hdet)
+
+    def grad2W(self, eta, Hdet=1.0):
+        return self.L.orc_table_grad2(self.h, eta, Hdet)
+
+    def nperh_from_wsum(self, w):
+        return self.L.orc_table_nperh_from_wsum(self.h, w)
+
+
+def bspline(dim, which, eta, Hdet=1.0):
+    return lib().orc_bspline(dim, which, eta, Hdet)
+
+
+def connectivity(dim, n_internal, pos, H, ext=2.0, brute=False, fast=False):
+    """Exact ConnectivityMap rows (CSR, ascending neighbour index)."""
+    L = lib(fast)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    n_total = pos.shape[0]
+    off = np.zeros(n_internal + 1, dtype=np.int64)
+    total = L.orc_connectivity(dim, n_internal, n_total, _p(pos), _p(H), ext, int(brute), _p(off, _i64p),
+                               C.cast(None, _i32p))
+    nbr = np.zeros(max(total, 1), dtype=np.int32)
+    L.orc_connectivity(dim, n_internal, n_total, _p(pos), _p(H), ext, int(brute), _p(off, _i64p),
+                       _p(nbr, _i32p))
+    return off, nbr[:total]
+
+
+def sum_density(table, n_internal, pos, mass, H, off, nbr, gather=False):
+    L = table.L
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    rho = np.zeros(n_internal)
+    L.orc_sum_density(table.h, table.dim, int(gather), n_internal, pos.shape[0], _p(pos),
+                      _p(np.ascontiguousarray(mass, dtype=np.float64)),
+                      _p(np.ascontiguousarray(H, dtype=np.float64)), _p(off, _i64p),
+                      _p(nbr, _i32p), _p(rho))
+    return rho
+
+
+def make_params(**kw):
+    d = dict(Cl=1.0, Cq=1.0, eps2=1.0e-2, epsTensile=0.0, nPerh=1.51, hmin=1.0e-20, hmax=1.0e20,
+             hminratio=0.1, XSPH=1, correctVelocityGradient=1, evolveTotalEnergy=0,
+             linearInExpansion=0, quadraticInExpansion=0)
+    d.update(kw)
+    return Params(**d)
+
+
+def eval_derivs(table, params, n_internal, state, off, nbr, gather=False):
+    """SPHHydroBase::evaluateDerivatives restated.  state: dict of numpy arrays
+    (pos, vel, mass, rho, eps, P, cs, H, omega, optional fCl, fCq, fshear) over all
+    nodes.  Returns dict of per-internal-node outputs (+ 'ncalc' for scatter)."""
+    dim = table.dim
+    keep = {}
+    st = State()
+    for name, _ in State._fields_:
+        a = state.get(name)
+        if a is not None:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep[name] = a
+        setattr(st, name, _p(a))
+    outs = {f: np.zeros((n_internal, ncomp(f, dim))) for f in OUTPUT_FIELDS}
+    ncalc = np.zeros(1, dtype=np.int64)
+    o = Outputs(**{f: _p(outs[f]) for f in OUTPUT_FIELDS}, ncalc=_p(ncalc, _i64p))
+    n_total = keep["pos"].shape[0]
+    table.L.orc_eval_derivs(table.h, dim, int(gather), C.byref(params), n_internal, n_total, C.byref(st),
+                            _p(off, _i64p), _p(nbr, _i32p), C.byref(o))
+    res = {f: (v[:, 0] if v.shape[1] == 1 else v) for f, v in outs.items()}
+    if not gather:
+        res["ncalc"] = int(ncalc[0])
+    return res
+
+
+def gamma_law(rho, eps, gamma, Pmin=-1e200, Pmax=1e200, floor=False):
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    P = np.zeros_like(rho)
+    cs = np.zeros_like(rho)
+    lib().orc_gamma_law(rho.size, gamma, Pmin, Pmax, int(floor), _p(rho), _p(eps), _p(P), _p(cs))
+    return P, cs
+
+
+def num_threads(fast=True):
+    return lib(fast).orc_num_threads()
