@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--row-order", choices=["search", "reference"], default="search",
+                    help="search: device search order; reference: rows by node index (bitwise reference sums)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -103,6 +105,7 @@ def main():
     st, meta = problems.converging_flow(args.n)
     N = meta["n_internal"]
     sim = sphgpu.Sim(3)
+    sim.set_row_order(sphgpu.ROW_ORDER_SEARCH if args.row_order == "search" else sphgpu.ROW_ORDER_REFERENCE)
     sim.set_table_bspline(1000)
     sim.set_nodes(N, N)
     sim.upload_state(st)
@@ -153,6 +156,7 @@ def main():
             "kernel": "TableKernel(BSpline,1000)", "nPerh": 1.51, "Q": "MonaghanGingold Cl=Cq=1",
             "options": "XSPH, correctVelocityGradient, no grad-h, no compatible energy",
             "precision": args.precision,
+            "row_order": args.row_order,
             "stage_ms": {"neighbors": stage[0], "sum_density": stage[1], "derivatives": stage[2],
                          "pack": stage[3]},
         },

@@ -142,6 +142,19 @@ int sphgpu_upload(sphgpu_sim* sim, int field, const double* host, int64_t n_rows
 /* Copy one state field (n_total rows) device -> host; synchronises. */
 int sphgpu_download(sphgpu_sim* sim, int field, double* host, int64_t n_rows);
 
+/* Order of each node's neighbour row inside the device passes.
+ *  REFERENCE: ascending node index, the order ConnectivityMap keeps
+ *    (src/Neighbor/ConnectivityMap.cc:735-738); the FP64 pass then sums in the
+ *    reference's order and reproduces its results bit for bit.
+ *  SEARCH: the deterministic order of the device search (no per-row sort);
+ *    FP64 results differ from the reference only by summation order.
+ * sphgpu_download_neighbors returns rows in ascending order in both modes. */
+enum { SPHGPU_ROW_ORDER_REFERENCE = 0, SPHGPU_ROW_ORDER_SEARCH = 1 };
+/* Select the row order of the next neighbour build (default REFERENCE).  Like
+ * ConnectivityMap's domainDecompositionIndependent ordering switch
+ * (src/Neighbor/ConnectivityMap.cc:729-740); invalidates the current build. */
+int sphgpu_set_row_order(sphgpu_sim* sim, int order);
+
 /* ConnectivityMap::rebuild (src/Neighbor/ConnectivityMapInline.hh:39-74,
  * computeConnectivity src/Neighbor/ConnectivityMap.cc:521-890): rows for the
  * internal nodes, j in N(i) iff |H_i r_ij|^2 <= ext^2 or |H_j r_ij|^2 <= ext^2,

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <algorithm>
 #include "internal.hh"
+#include "ell.hh"
 
 using namespace sphgpu;
 
@@ -100,8 +101,8 @@ void sphgpu_destroy(sphgpu_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   DBuf* bufs[] = {&s->dTable, &s->dNperh, &s->dWsum, &s->pos, &s->vel, &s->mass, &s->rho, &s->eps, &s->P, &s->cs,
                   &s->H, &s->omega, &s->fCl, &s->fCq, &s->fshear, &s->cellOf, &s->cellCount, &s->cellStart,
-                  &s->cellFill, &s->perm, &s->iperm, &s->xs, &s->Hs, &s->rowCount, &s->offsets, &s->nbr,
-                  &s->scanTmp, &s->reduceTmp, &s->rec, &s->out};
+                  &s->cellFill, &s->perm, &s->iperm, &s->xs, &s->Hs, &s->rowCount, &s->nbr,
+                  &s->scanTmp, &s->reduceTmp, &s->flags, &s->rec, &s->out};
   for (DBuf* b : bufs) b->release();
   for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -193,8 +194,23 @@ int sphgpu_upload(sphgpu_sim* s, int field, const double* host, int64_t nRows) {
     if (bytes) SG_CHECK_HIP(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, s->stream));
     SG_CHECK_HIP(hipStreamSynchronize(s->stream));
     // Neighbours stay those of the last build (the reference's finalize sums density
-    // with the connectivity of the step start); the sorted copies are refreshed lazily.
-    if (field == SPHGPU_F_POSITION || field == SPHGPU_F_H) s->xsStale = true;
+    // with the connectivity of the step start); the packed records are rebuilt lazily.
+    // Records use the compact isotropic layout only while every H is h*I and no Q
+    // multipliers are present.
+    if (field == SPHGPU_F_H && s->isotropic) {
+      const int D = s->dim, NS = nsymOf(D);
+      for (int64_t i = 0; i < nRows && s->isotropic; ++i) {
+        const double* h = host + i * NS;
+        bool iso = true;
+        int k = 0;
+        for (int a = 0; a < D; ++a)
+          for (int b = a; b < D; ++b, ++k) iso = iso && (a == b ? h[k] == h[0] : h[k] == 0.0);
+        if (!iso) s->isotropic = false;
+      }
+    }
+    if (field == SPHGPU_F_CL_MULTIPLIER || field == SPHGPU_F_CQ_MULTIPLIER || field == SPHGPU_F_SHEAR_CORRECTION)
+      s->isotropic = false;
+    s->recValid = false;
     s->derivsValid = false;
   });
 }
@@ -219,6 +235,15 @@ int sphgpu_build_neighbors(sphgpu_sim* s, double ext) {
   return guarded(s, [&] { buildNeighbors(*s, ext); });
 }
 
+int sphgpu_set_row_order(sphgpu_sim* s, int order) {
+  return guarded(s, [&] {
+    SG_REQUIRE(order == SPHGPU_ROW_ORDER_REFERENCE || order == SPHGPU_ROW_ORDER_SEARCH, SPHGPU_ERR_ARG,
+               "unknown row order");
+    if (order != s->rowOrder) s->neighborsValid = false;
+    s->rowOrder = order;
+  });
+}
+
 int sphgpu_neighbor_count(sphgpu_sim* s, int64_t* n) {
   return guarded(s, [&] {
     SG_REQUIRE(n, SPHGPU_ERR_ARG, "null output");
@@ -231,22 +256,26 @@ int sphgpu_download_neighbors(sphgpu_sim* s, int64_t* offsets, int32_t* nbr) {
   return guarded(s, [&] {
     SG_REQUIRE(offsets, SPHGPU_ERR_ARG, "null offsets");
     SG_REQUIRE(s->neighborsValid, SPHGPU_ERR_STATE, "no current neighbour build");
+    // Rows are held as a transposed ELL over sorted slots; the ConnectivityMap
+    // contract is rows of caller indices in ascending order.
     const int64_t n = s->nTot;
-    std::vector<int64_t> off(static_cast<size_t>(n + 1));
-    std::vector<int32_t> perm(static_cast<size_t>(n)), iperm(static_cast<size_t>(n)), rows(static_cast<size_t>(std::max<int64_t>(s->nEntries, 1)));
-    SG_CHECK_HIP(hipMemcpy(off.data(), s->offsets.p, sizeof(int64_t) * size_t(n + 1), hipMemcpyDeviceToHost));
+    const size_t cap = size_t(s->ellCap);
+    std::vector<int32_t> cnt(static_cast<size_t>(n)), perm(static_cast<size_t>(n)), iperm(static_cast<size_t>(n));
+    std::vector<int32_t> ell(static_cast<size_t>(n) * cap);
+    SG_CHECK_HIP(hipMemcpy(cnt.data(), s->rowCount.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
     SG_CHECK_HIP(hipMemcpy(perm.data(), s->perm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
     SG_CHECK_HIP(hipMemcpy(iperm.data(), s->iperm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
-    if (s->nEntries)
-      SG_CHECK_HIP(hipMemcpy(rows.data(), s->nbr.p, sizeof(int32_t) * size_t(s->nEntries), hipMemcpyDeviceToHost));
+    if (!ell.empty()) SG_CHECK_HIP(hipMemcpy(ell.data(), s->nbr.p, sizeof(int32_t) * ell.size(), hipMemcpyDeviceToHost));
     int64_t w = 0;
     for (int64_t i = 0; i < s->nInt; ++i) {
       offsets[i] = w;
       const int64_t k = iperm[size_t(i)];
-      for (int64_t q = off[size_t(k)]; q < off[size_t(k + 1)]; ++q) {
-        if (nbr) nbr[w] = perm[size_t(rows[size_t(q)])];
-        ++w;
+      const int len = cnt[size_t(k)];
+      if (nbr) {
+        for (int q = 0; q < len; ++q) nbr[w + q] = perm[size_t(ell[size_t(ellIndex(q, n, k))])];
+        std::sort(nbr + w, nbr + w + len);
       }
+      w += len;
     }
     offsets[s->nInt] = w;
   });

@@ -1,13 +1,13 @@
 // Device sum density and evaluateDerivatives.
 //
 // Both passes run in gather form: one lane per internal node (nodes in the
-// neighbour build's sorted order), walking its full row in ascending node index.
-// The reference (src/SPH/SPHHydroBase.cc:575-928,
-// src/SPH/computeSPHSumMassDensity.cc:60-103) evaluates each unordered pair once
-// in the frame of its owner (the lower index, ConnectivityMapInline.hh:182-184)
-// and adds to both ends, visiting owners in ascending order; so every node
-// receives its pair terms in ascending neighbour order.  Walking the row in that
-// order, in the node's own frame, reproduces the reference's FP64 sums exactly:
+// neighbour build's sorted order), walking its full row.  The reference
+// (src/SPH/SPHHydroBase.cc:575-928, src/SPH/computeSPHSumMassDensity.cc:60-103)
+// evaluates each unordered pair once in the frame of its owner (the lower index,
+// ConnectivityMapInline.hh:182-184) and adds to both ends, visiting owners in
+// ascending order; so every node receives its pair terms in ascending neighbour
+// order.  With rows in that order (SPHGPU_ROW_ORDER_REFERENCE), walking a row in
+// the node's own frame reproduces the reference's FP64 sums exactly:
 //   * quantities odd in r_ij (eta, unit vectors, grad W, v_ij, Q accelerations,
 //     the acceleration increment) only flip sign with the frame, exactly;
 //   * products of two odd quantities (mu, v.gradW, dyads, r r) are unchanged;
@@ -16,8 +16,16 @@
 //     factor uses the owner's W.
 // Sum density adds the self term where the reference does: after the
 // contributions of lower-index neighbours, before those of higher ones.
+// With H = h I (every off-diagonal exactly zero) the SymTensor products reduce
+// to h*r componentwise with the same bits, which the isotropic kernels use.
+//
+// Memory: neighbour indices arrive four at a time (one int4 of the blocked ELL),
+// the next neighbour's record is loaded while the current pair is evaluated,
+// and the kernel table sits in LDS.
 #include "internal.hh"
 #include "dgeom.hh"
+#include "records.hh"
+#include "ell.hh"
 
 namespace sphgpu {
 
@@ -26,45 +34,51 @@ namespace {
 constexpr int kBlock = 256;
 inline int nblocks(int64_t n, int bs = kBlock) { return int(std::max<int64_t>(1, (n + bs - 1) / bs)); }
 
-// FP64 node record in sorted order: everything one side of a pair needs.
-template <int D> struct Rec {
-  static constexpr int NS = NSym<D>::value;
-  static constexpr int X = 0, V = D, H = 2 * D, HDET = 2 * D + NS, M = HDET + 1, RHO = M + 1, CS = RHO + 1, P = CS + 1,
-                       SOM = P + 1, PRHO = SOM + 1, MOR = PRHO + 1, FCL = MOR + 1, FCQ = FCL + 1, FSH = FCQ + 1;
-  static constexpr int N = ((FSH + 1) + 1) & ~1;   // even -> 16-byte rows
-};
-
-template <int D>
+// ---------------------------------------------------------------------------
+// Records
+// ---------------------------------------------------------------------------
+template <int D, bool ISO>
 __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double* __restrict__ pos,
                        const double* __restrict__ vel, const double* __restrict__ mass, const double* __restrict__ rho,
                        const double* __restrict__ P, const double* __restrict__ cs, const double* __restrict__ H,
                        const double* __restrict__ omega, const double* __restrict__ fCl, const double* __restrict__ fCq,
                        const double* __restrict__ fsh, double* __restrict__ rec) {
-  using R = Rec<D>;
+  using R = RL<ISO>;
+  constexpr int NS = NSym<D>::value;
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (k >= n) return;
   const int64_t i = perm[k];
-  double* r = rec + k * R::N;
+  double r[R::STRIDE];
+#pragma unroll
+  for (int c = 0; c < R::STRIDE; ++c) r[c] = 0.0;
   for (int c = 0; c < D; ++c) { r[R::X + c] = pos[i * D + c]; r[R::V + c] = vel[i * D + c]; }
   S<D> h;
-  for (int c = 0; c < R::NS; ++c) { h.c[c] = H[i * R::NS + c]; r[R::H + c] = h.c[c]; }
+  for (int c = 0; c < NS; ++c) h.c[c] = H[i * NS + c];
   const double rh = rho[i], m = mass[i], p = P[i];
   const double som = safeInv(omega[i], 1.0e-30);
+  if constexpr (ISO) {
+    r[RL<true>::HINV] = h.c[0];
+  } else {
+    for (int c = 0; c < NS; ++c) r[RL<false>::H + c] = h.c[c];
+    r[RL<false>::FCL] = fCl ? fCl[i] : 1.0;
+    r[RL<false>::FCQ] = fCq ? fCq[i] : 1.0;
+    r[RL<false>::FSH] = fsh ? fsh[i] : 1.0;
+  }
   r[R::HDET] = sdet(h);
   r[R::M] = m;
+  r[R::ORIG] = double(i);
   r[R::RHO] = rh;
   r[R::CS] = cs[i];
   r[R::P] = p;
   r[R::SOM] = som;
   r[R::PRHO] = som * p / (rh * rh);
   r[R::MOR] = m / rh;
-  r[R::FCL] = fCl ? fCl[i] : 1.0;
-  r[R::FCQ] = fCq ? fCq[i] : 1.0;
-  r[R::FSH] = fsh ? fsh[i] : 1.0;
-  for (int c = R::FSH + 1; c < R::N; ++c) r[c] = 0.0;
+  double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
+#pragma unroll
+  for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
 }
 
-// Table in LDS: per point [Ak Bk Ck Ag Bg Cg].
+// Table: per point [Ak Bk Ck Ag Bg Cg].
 struct TabArgs {
   const double* g;     // global copy
   int n;
@@ -99,13 +113,17 @@ __device__ __forceinline__ double lookupW(const double* __restrict__ tab, const 
   return Hdet * (c[0] + c[1] * x + c[2] * x * x);
 }
 
-__device__ __forceinline__ const double* stageTable(double* lds, const TabArgs& t, bool useLds) {
-  if (useLds) {
+// Largest table that is staged in LDS (the 1000-point default takes 48 KB).
+constexpr int kLdsTablePoints = 1365;
+
+template <bool LDS> __device__ __forceinline__ const double* stageTable(double* lds, const TabArgs& t) {
+  if constexpr (LDS) {
     for (int q = threadIdx.x; q < 6 * t.n; q += blockDim.x) lds[q] = t.g[q];
     __syncthreads();
     return lds;
+  } else {
+    return t.g;
   }
-  return t.g;
 }
 
 // TableKernel::equivalentNodesPerSmoothingScale (TableKernel.cc:399-429).
@@ -125,41 +143,123 @@ __device__ double equivalentNperh(const TabArgs& t, double Wsum) {
   return dmin(np[ub], dmax(np[lb], np[lb] + (Wsum - ws[lb]) / (ws[ub] - ws[lb]) * (np[ub] - np[lb])));
 }
 
+// H.v: with isotropic H the product is h*v componentwise (same bits as the
+// SymTensor product with zero off-diagonals).
+template <int D, bool ISO> __device__ __forceinline__ V<D> hdot(const S<D>& H, const V<D>& v) {
+  if constexpr (ISO) {
+    V<D> o;
+    for (int c = 0; c < D; ++c) o.c[c] = H.c[0] * v.c[c];
+    return o;
+  } else {
+    return sdot(H, v);
+  }
+}
+
+// Neighbour indices of one row arrive four at a time (one int4 of the blocked
+// ELL); the next int4 is fetched one chunk ahead.
+__device__ __forceinline__ int32_t pick4(const int4 v, int q) {
+  return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
+}
+// Advance to entry w (w >= 1, called for w = 1, 2, ... in order).
+__device__ __forceinline__ int32_t nextIndex(const int4* __restrict__ nb4, int64_t n, int64_t k, int len, int w,
+                                             int4& cur, int4& nxt) {
+  if ((w & 3) == 0) {
+    cur = nxt;
+    if (w + 4 < len) nxt = nb4[ellChunk((w >> 2) + 1, n, k)];
+  }
+  return pick4(cur, w & 3);
+}
+
+// What one side of a pair reads from a neighbour's record.
+template <int D, bool ISO> struct Nbr {
+  V<D> x, v;
+  S<D> H;
+  double hdet, m, orig, rho, cs, prho, mor, p, som, fcl, fcq, fsh;
+};
+template <int D, bool ISO, bool WANT_P>
+__device__ __forceinline__ Nbr<D, ISO> loadNbr(const double* __restrict__ rec, int32_t j) {
+  using R = RL<ISO>;
+  const double* r = rec + int64_t(j) * R::STRIDE;
+  Nbr<D, ISO> o;
+  for (int c = 0; c < D; ++c) { o.x.c[c] = r[R::X + c]; o.v.c[c] = r[R::V + c]; }
+  if constexpr (ISO) {
+    o.H.c[0] = r[RL<true>::HINV];
+    for (int c = 1; c < NSym<D>::value; ++c) o.H.c[c] = 0.0;
+    for (int c = 1; c < D; ++c) o.H.c[sdiag<D>(c)] = o.H.c[0];
+    o.fcl = o.fcq = o.fsh = 1.0;
+  } else {
+    for (int c = 0; c < NSym<D>::value; ++c) o.H.c[c] = r[RL<false>::H + c];
+    o.fcl = r[RL<false>::FCL]; o.fcq = r[RL<false>::FCQ]; o.fsh = r[RL<false>::FSH];
+  }
+  o.hdet = r[R::HDET]; o.m = r[R::M]; o.orig = r[R::ORIG]; o.rho = r[R::RHO]; o.cs = r[R::CS];
+  o.prho = r[R::PRHO]; o.mor = r[R::MOR];
+  if constexpr (WANT_P) { o.p = r[R::P]; o.som = r[R::SOM]; } else { o.p = 0.0; o.som = 0.0; }
+  return o;
+}
+
 // ---------------------------------------------------------------------------
 // computeSPHSumMassDensity
 // ---------------------------------------------------------------------------
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_sum_density(const double* __restrict__ xs, const double* __restrict__ Hs,
-                                                        const double* __restrict__ mass, const int32_t* __restrict__ perm,
-                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ nbr,
-                                                        int64_t n, int64_t nInt, TabArgs t, bool useLds,
-                                                        double* __restrict__ rhoOut) {
+template <int D, bool ISO> struct DNbr {
+  V<D> x;
+  S<D> H;
+  double hdet, m, orig;
+};
+template <int D, bool ISO> __device__ __forceinline__ DNbr<D, ISO> loadDNbr(const double* __restrict__ rec, int32_t j) {
+  using R = RL<ISO>;
+  const double* r = rec + int64_t(j) * R::STRIDE;
+  DNbr<D, ISO> o;
+  for (int c = 0; c < D; ++c) o.x.c[c] = r[R::X + c];
+  if constexpr (ISO) {
+    o.H.c[0] = r[RL<true>::HINV];
+    for (int c = 1; c < NSym<D>::value; ++c) o.H.c[c] = 0.0;
+    for (int c = 1; c < D; ++c) o.H.c[sdiag<D>(c)] = o.H.c[0];
+  } else {
+    for (int c = 0; c < NSym<D>::value; ++c) o.H.c[c] = r[RL<false>::H + c];
+  }
+  o.hdet = r[R::HDET]; o.m = r[R::M]; o.orig = r[R::ORIG];
+  return o;
+}
+
+template <int D, bool ISO, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec, const int32_t* __restrict__ rowCount,
+                                                        const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
+                                                        TabArgs t, double* __restrict__ rhoOut) {
+  using R = RL<ISO>;
   extern __shared__ double lds[];
-  const double* tab = stageTable(lds, t, useLds);
+  const double* tab = stageTable<LDS>(lds, t);
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (k >= n) return;
-  const int32_t oi = perm[k];
-  if (oi >= nInt) return;
-  constexpr int NS = NSym<D>::value;
-  V<D> ri; S<D> Hi;
-  for (int c = 0; c < D; ++c) ri.c[c] = xs[k * D + c];
-  for (int c = 0; c < NS; ++c) Hi.c[c] = Hs[k * NS + c];
-  const double self = mass[oi] * lookupW(tab, t, 0.0, sdet(Hi));
+  double* ri_ = rec + k * R::STRIDE;
+  const double oi = ri_[R::ORIG];
+  if (oi >= double(nInt)) return;
+  const DNbr<D, ISO> me = loadDNbr<D, ISO>(rec, int32_t(k));
+  const double mi = me.m;
+  const double self = mi * lookupW(tab, t, 0.0, me.hdet);
   double acc = 0.0;
   bool selfDone = false;
-  const int64_t e = off[k + 1];
-  for (int64_t q = off[k]; q < e; ++q) {
-    const int32_t j = nbr[q];
-    const int32_t oj = perm[j];
-    if (!selfDone && oj > oi) { acc += self; selfDone = true; }
-    V<D> rij; S<D> Hj;
-    for (int c = 0; c < D; ++c) rij.c[c] = ri.c[c] - xs[int64_t(j) * D + c];
-    for (int c = 0; c < NS; ++c) Hj.c[c] = Hs[int64_t(j) * NS + c];
-    const double etaj = vmag(sdot(Hj, rij));
-    acc += mass[oj] * lookupW(tab, t, etaj, sdet(Hj));
+  const int len = rowCount[k];
+  const int4* nb4 = reinterpret_cast<const int4*>(nbr);
+  int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
+  int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
+  DNbr<D, ISO> nx;
+  if (len > 0) nx = loadDNbr<D, ISO>(rec, cur.x);
+  for (int w = 0; w < len; ++w) {
+    const DNbr<D, ISO> nb = nx;
+    if (w + 1 < len) nx = loadDNbr<D, ISO>(rec, nextIndex(nb4, n, k, len, w + 1, cur, nxt));
+    if (!selfDone && nb.orig > oi) { acc += self; selfDone = true; }
+    const V<D> rij = vsub(me.x, nb.x);
+    const double etaj = vmag(hdot<D, ISO>(nb.H, rij));
+    acc += nb.m * lookupW(tab, t, etaj, nb.hdet);
   }
   if (!selfDone) acc += self;
-  rhoOut[oi] = acc;
+  rhoOut[int64_t(oi)] = acc;
+  // keep the record consistent with the new density (the pass reads only
+  // positions, H and masses of the neighbours)
+  const double som = ri_[R::SOM], p = ri_[R::P];
+  ri_[R::RHO] = acc;
+  ri_[R::PRHO] = som * p / (acc * acc);
+  ri_[R::MOR] = mi / acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -184,27 +284,24 @@ struct OutArgs {
   int off[SPHGPU_NUM_DERIV_FIELDS];
 };
 
-template <int D>
 __device__ __forceinline__ double* outRow(const OutArgs& o, int field, int64_t k, int comps) {
   return o.base + int64_t(o.off[field]) * o.n + k * comps;
 }
 
-template <int D, bool TENSILE, bool MULT>
-__device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, int32_t oi, const double* __restrict__ rec,
-                                                   const int32_t* __restrict__ perm, const int64_t* __restrict__ off,
-                                                   const int32_t* __restrict__ nbr, const double* __restrict__ tab,
-                                                   const TabArgs& t, const PassArgs& p, const OutArgs& o) {
-  using R = Rec<D>;
+template <int D, bool ISO, bool TENSILE>
+__device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, const double* __restrict__ rec,
+                                                   const int32_t* __restrict__ rowCount, const int32_t* __restrict__ nbr,
+                                                   int64_t n, const double* __restrict__ tab, const TabArgs& t,
+                                                   const PassArgs& p, const OutArgs& o) {
   constexpr int NS = NSym<D>::value;
   const double tiny = 1.0e-30;
 
-  const double* ri_ = rec + k * R::N;
-  V<D> ri, vi; S<D> Hi;
-  for (int c = 0; c < D; ++c) { ri.c[c] = ri_[R::X + c]; vi.c[c] = ri_[R::V + c]; }
-  for (int c = 0; c < NS; ++c) Hi.c[c] = ri_[R::H + c];
-  const double Hdeti = ri_[R::HDET], mi = ri_[R::M], rhoi = ri_[R::RHO], ci = ri_[R::CS], Pi = ri_[R::P];
-  const double somi = ri_[R::SOM], Prhoi0 = ri_[R::PRHO], mori = ri_[R::MOR];
-  const double fcli = ri_[R::FCL], fcqi = ri_[R::FCQ], fshi = ri_[R::FSH];
+  const Nbr<D, ISO> me = loadNbr<D, ISO, true>(rec, int32_t(k));
+  const V<D> ri = me.x, vi = me.v;
+  const S<D> Hi = me.H;
+  const double oi = me.orig;
+  const double Hdeti = me.hdet, mi = me.m, rhoi = me.rho, ci = me.cs, Pi = me.p;
+  const double somi = me.som, Prhoi0 = me.prho, mori = me.mor;
 
   double rhoSum = 0.0, norm = 0.0, DepsDt = 0.0, maxVP = 0.0, effVP = 0.0, viscW = 0.0, xsphW = 0.0, wNS = 0.0;
   V<D> DvDt = vzero<D>(), xsphDV = vzero<D>();
@@ -212,25 +309,28 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, int32_t oi, const 
   S<D> m2nd = szero<D>();
   unsigned pairs = 0;
 
-  const int64_t e = off[k + 1];
-  for (int64_t q = off[k]; q < e; ++q) {
-    const int32_t j = nbr[q];
-    const int32_t oj = perm[j];
-    const bool ownerI = oj > oi;
+  const int len = rowCount[k];
+  const int4* nb4 = reinterpret_cast<const int4*>(nbr);
+  int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
+  int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
+  Nbr<D, ISO> nx;
+  if (len > 0) nx = loadNbr<D, ISO, TENSILE>(rec, cur.x);
+  for (int w = 0; w < len; ++w) {
+    const Nbr<D, ISO> nb = nx;
+    if (w + 1 < len) nx = loadNbr<D, ISO, TENSILE>(rec, nextIndex(nb4, n, k, len, w + 1, cur, nxt));
+    const bool ownerI = nb.orig > oi;
     pairs += ownerI ? 1u : 0u;
-    const double* rj_ = rec + int64_t(j) * R::N;
-    V<D> rij, vij; S<D> Hj;
-    for (int c = 0; c < D; ++c) { rij.c[c] = ri.c[c] - rj_[R::X + c]; vij.c[c] = vi.c[c] - rj_[R::V + c]; }
-    for (int c = 0; c < NS; ++c) Hj.c[c] = rj_[R::H + c];
-    const double Hdetj = rj_[R::HDET], mj = rj_[R::M], rhoj = rj_[R::RHO], cj = rj_[R::CS];
-    const double morj = rj_[R::MOR];
+    const V<D> rij = vsub(ri, nb.x);
+    const V<D> vij = vsub(vi, nb.v);
+    const S<D>& Hj = nb.H;
+    const double Hdetj = nb.hdet, mj = nb.m, rhoj = nb.rho, cj = nb.cs, morj = nb.mor;
 
-    const V<D> etai = sdot(Hi, rij), etaj = sdot(Hj, rij);
+    const V<D> etai = hdot<D, ISO>(Hi, rij), etaj = hdot<D, ISO>(Hj, rij);
     const double eta2i = vmag2(etai), eta2j = vmag2(etaj);
     const double etaMagi = (D == 1) ? fabs(etai.c[0]) : __builtin_sqrt(eta2i);
     const double etaMagj = (D == 1) ? fabs(etaj.c[0]) : __builtin_sqrt(eta2j);
-    const V<D> Hetai = sdot(Hi, vunit(etai, etaMagi));
-    const V<D> Hetaj = sdot(Hj, vunit(etaj, etaMagj));
+    const V<D> Hetai = hdot<D, ISO>(Hi, vunit(etai, etaMagi));
+    const V<D> Hetaj = hdot<D, ISO>(Hj, vunit(etaj, etaMagj));
     double Wi, gWi, Wj, gWj;
     lookupWG(tab, t, etaMagi, Hdeti, Wi, gWi);
     lookupWG(tab, t, etaMagj, Hdetj, Wj, gWj);
@@ -252,10 +352,10 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, int32_t oi, const 
 
     // Monaghan-Gingold Q (MonaghanGingoldViscosity.cc:83-132)
     double Cl = p.Cl, Cq = p.Cq;
-    if (MULT) {
-      const double fsh = dmax(fshi, rj_[R::FSH]);
-      Cl *= 0.5 * (fcli + rj_[R::FCL]) * fsh;
-      Cq *= 0.5 * (fcqi + rj_[R::FCQ]) * fsh;
+    if constexpr (!ISO) {
+      const double fsh = dmax(me.fsh, nb.fsh);
+      Cl *= 0.5 * (me.fcl + nb.fcl) * fsh;
+      Cq *= 0.5 * (me.fcq + nb.fcq) * fsh;
     }
     const double mui = vdot(vij, etai) / (eta2i + p.eps2);
     const double muj = vdot(vij, etaj) / (eta2j + p.eps2);
@@ -269,14 +369,14 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, int32_t oi, const 
     effVP += morj * Qi * Wi;
     viscW += mj * workQi;
 
-    double Prhoi = Prhoi0, Prhoj = rj_[R::PRHO];
-    if (TENSILE) {
+    double Prhoi = Prhoi0, Prhoj = nb.prho;
+    if constexpr (TENSILE) {
       const double Wo = ownerI ? Wi : Wj, Hdeto = ownerI ? Hdeti : Hdetj;
       const double x = Wo / (Hdeto * p.WnPerh);
       const double fij = p.epsTensile * (x * x * x * x);
-      const double Pj = rj_[R::P];
+      const double Pj = nb.p;
       Prhoi = somi * (Pi + fij * (Pi < 0.0 ? -Pi : 0.0)) / (rhoi * rhoi);
-      Prhoj = rj_[R::SOM] * (Pj + fij * (Pj < 0.0 ? -Pj : 0.0)) / (rhoj * rhoj);
+      Prhoj = nb.som * (Pj + fij * (Pj < 0.0 ? -Pj : 0.0)) / (rhoj * rhoj);
     }
     for (int c = 0; c < D; ++c) {
       const double qf = ownerI ? Qacci.c[c] : Qaccj.c[c];
@@ -291,18 +391,17 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, int32_t oi, const 
         M.c[a * D + b] -= (rij.c[a] * gradWi.c[b]) * mj;
       }
     if (p.XSPH) {
-      const double w = 0.5 * (mori * Wi + morj * Wj);
-      xsphW += w;
-      for (int c = 0; c < D; ++c) xsphDV.c[c] -= vij.c[c] * w;
+      const double w2 = 0.5 * (mori * Wi + morj * Wj);
+      xsphW += w2;
+      for (int c = 0; c < D; ++c) xsphDV.c[c] -= vij.c[c] * w2;
     }
   }
 
   // per-node epilogue (SPHHydroBase.cc:834-902)
-  const int64_t numNeighbors = e - off[k];
   rhoSum += mi * p.W0 * Hdeti;
   norm += mori * p.W0 * Hdeti;
   T<D> Mout = M;
-  if (p.correctVelocityGradient && fabs(tdet(M)) > 1.0e-10 && double(numNeighbors) > pownu2<D>()) {
+  if (p.correctVelocityGradient && fabs(tdet(M)) > 1.0e-10 && double(len) > pownu2<D>()) {
     Mout = tinv(M);
     DvDx = tdot(DvDx, Mout);
   } else {
@@ -325,42 +424,42 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, int32_t oi, const 
   S<D> DHDt = sdivs(negH, double(D));
   for (int c = 0; c < NS; ++c) DHDt.c[c] = DHDt.c[c] * trace;
   // SPHSmoothingScale::idealSmoothingScale
-  double cur;
-  if (fuzzyEqual(wNSo, 0.0)) cur = 0.5 * p.nPerh;
-  else cur = equivalentNperh(t, wNSo);
-  const double s = dmin(4.0, dmax(0.25, p.nPerh / (cur + 1.0e-30)));
+  double nperhNow;
+  if (fuzzyEqual(wNSo, 0.0)) nperhNow = 0.5 * p.nPerh;
+  else nperhNow = equivalentNperh(t, wNSo);
+  const double s = dmin(4.0, dmax(0.25, p.nPerh / (nperhNow + 1.0e-30)));
   const double a = (s < 1.0) ? 0.4 * (1.0 + s * s) : 0.4 * (1.0 + 1.0 / (s * s * s));
   const double hi0 = 1.0 / Hi.c[0];
   const double hi1 = dmin(p.hmax, dmax(p.hmin, hi0 * (1.0 - a + a * s)));
   const double hinv = 1.0 / hi1;
 
   // outputs
-  *outRow<D>(o, SPHGPU_D_RHO_SUM, k, 1) = rhoSum;
-  *outRow<D>(o, SPHGPU_D_NORMALIZATION, k, 1) = norm;
-  *outRow<D>(o, SPHGPU_D_DRHODT, k, 1) = DrhoDt;
-  *outRow<D>(o, SPHGPU_D_DEPSDT, k, 1) = DepsDt;
-  *outRow<D>(o, SPHGPU_D_MAX_VISCOUS_PRESSURE, k, 1) = maxVP;
-  *outRow<D>(o, SPHGPU_D_EFF_VISCOUS_PRESSURE, k, 1) = effVP;
-  *outRow<D>(o, SPHGPU_D_VISCOUS_WORK, k, 1) = viscW;
-  *outRow<D>(o, SPHGPU_D_XSPH_WEIGHT_SUM, k, 1) = xsphW;
-  *outRow<D>(o, SPHGPU_D_WEIGHTED_NEIGHBOR_SUM, k, 1) = wNSo;
+  *outRow(o, SPHGPU_D_RHO_SUM, k, 1) = rhoSum;
+  *outRow(o, SPHGPU_D_NORMALIZATION, k, 1) = norm;
+  *outRow(o, SPHGPU_D_DRHODT, k, 1) = DrhoDt;
+  *outRow(o, SPHGPU_D_DEPSDT, k, 1) = DepsDt;
+  *outRow(o, SPHGPU_D_MAX_VISCOUS_PRESSURE, k, 1) = maxVP;
+  *outRow(o, SPHGPU_D_EFF_VISCOUS_PRESSURE, k, 1) = effVP;
+  *outRow(o, SPHGPU_D_VISCOUS_WORK, k, 1) = viscW;
+  *outRow(o, SPHGPU_D_XSPH_WEIGHT_SUM, k, 1) = xsphW;
+  *outRow(o, SPHGPU_D_WEIGHTED_NEIGHBOR_SUM, k, 1) = wNSo;
   {
-    double* dx = outRow<D>(o, SPHGPU_D_DXDT, k, D);
-    double* dv = outRow<D>(o, SPHGPU_D_DVDT, k, D);
-    double* xv = outRow<D>(o, SPHGPU_D_XSPH_DELTA_V, k, D);
+    double* dx = outRow(o, SPHGPU_D_DXDT, k, D);
+    double* dv = outRow(o, SPHGPU_D_DVDT, k, D);
+    double* xv = outRow(o, SPHGPU_D_XSPH_DELTA_V, k, D);
     for (int c = 0; c < D; ++c) { dx[c] = DxDt.c[c]; dv[c] = DvDt.c[c]; xv[c] = xsphDV.c[c]; }
   }
   {
-    double* g = outRow<D>(o, SPHGPU_D_DVDX, k, D * D);
-    double* gl = outRow<D>(o, SPHGPU_D_LOCAL_DVDX, k, D * D);
-    double* m = outRow<D>(o, SPHGPU_D_M, k, D * D);
-    double* ml = outRow<D>(o, SPHGPU_D_LOCAL_M, k, D * D);
+    double* g = outRow(o, SPHGPU_D_DVDX, k, D * D);
+    double* gl = outRow(o, SPHGPU_D_LOCAL_DVDX, k, D * D);
+    double* m = outRow(o, SPHGPU_D_M, k, D * D);
+    double* ml = outRow(o, SPHGPU_D_LOCAL_M, k, D * D);
     for (int c = 0; c < D * D; ++c) { g[c] = DvDx.c[c]; gl[c] = DvDx.c[c]; m[c] = Mout.c[c]; ml[c] = Mout.c[c]; }
   }
   {
-    double* dh = outRow<D>(o, SPHGPU_D_DHDT, k, NS);
-    double* hid = outRow<D>(o, SPHGPU_D_HIDEAL, k, NS);
-    double* m2 = outRow<D>(o, SPHGPU_D_MASS_SECOND_MOMENT, k, NS);
+    double* dh = outRow(o, SPHGPU_D_DHDT, k, NS);
+    double* hid = outRow(o, SPHGPU_D_HIDEAL, k, NS);
+    double* m2 = outRow(o, SPHGPU_D_MASS_SECOND_MOMENT, k, NS);
     int s2 = 0;
     for (int a2 = 0; a2 < D; ++a2)
       for (int b2 = a2; b2 < D; ++b2, ++s2) {
@@ -372,41 +471,65 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, int32_t oi, const 
   return pairs;
 }
 
-template <int D, bool TENSILE, bool MULT>
-__global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict__ rec, const int32_t* __restrict__ perm,
-                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ nbr,
-                                                        int64_t n, int64_t nInt, TabArgs t, bool useLds, PassArgs p,
-                                                        OutArgs o, unsigned long long* __restrict__ ncalc) {
+template <int D, bool ISO, bool TENSILE, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict__ rec, const int32_t* __restrict__ rowCount,
+                                                        const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
+                                                        TabArgs t, PassArgs p, OutArgs o,
+                                                        unsigned long long* __restrict__ ncalc) {
   extern __shared__ double lds[];
-  const double* tab = stageTable(lds, t, useLds);
+  const double* tab = stageTable<LDS>(lds, t);
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  const int32_t oi = k < n ? perm[k] : 0;
   unsigned pairs = 0;
-  if (k < n && oi < nInt) pairs = derivsNodeFP64<D, TENSILE, MULT>(k, oi, rec, perm, off, nbr, tab, t, p, o);
+  if (k < n && rec[k * RL<ISO>::STRIDE + RL<ISO>::ORIG] < double(nInt))
+    pairs = derivsNodeFP64<D, ISO, TENSILE>(k, rec, rowCount, nbr, n, tab, t, p, o);
   for (int sh = 32; sh > 0; sh >>= 1) pairs += __shfl_xor(pairs, sh);
   if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(ncalc, (unsigned long long)pairs);
+}
+
+TabArgs tabArgs(const Sim& s) {
+  return TabArgs{s.dTable.as<double>(), s.table.n, s.table.step, s.table.extent, s.dNperh.as<double>(),
+                 s.dWsum.as<double>()};
+}
+
+template <int D>
+void packD(Sim& s) {
+  const int64_t n = s.nTot;
+  const bool iso = s.isotropic;
+  const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
+  s.rec.ensure(sizeof(double) * size_t(n) * stride);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.perm.as<int32_t>(), n, s.pos.as<double>(),
+                       s.vel.as<double>(), s.mass.as<double>(), s.rho.as<double>(), s.P.as<double>(),
+                       s.cs.as<double>(), s.H.as<double>(), s.omega.as<double>(),
+                       s.hasMultipliers ? s.fCl.as<double>() : nullptr, s.hasMultipliers ? s.fCq.as<double>() : nullptr,
+                       s.hasMultipliers ? s.fshear.as<double>() : nullptr, s.rec.as<double>());
+  };
+  if (iso) go(k_pack<D, true>);
+  else go(k_pack<D, false>);
+  SG_CHECK_HIP(hipGetLastError());
 }
 
 template <int D>
 void sumDensityD(Sim& s) {
   const int64_t n = s.nTot;
-  TabArgs t{s.dTable.as<double>(), s.table.n, s.table.step, s.table.extent, s.dNperh.as<double>(), s.dWsum.as<double>()};
-  const size_t ldsBytes = sizeof(double) * 6 * size_t(s.table.n);
-  const bool useLds = ldsBytes <= 64 * 1024;
+  TabArgs t = tabArgs(s);
+  const bool lds = s.table.n <= kLdsTablePoints;
+  const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
-  hipLaunchKernelGGL((k_sum_density<D>), dim3(nblocks(n)), dim3(kBlock), useLds ? ldsBytes : 0, s.stream,
-                     s.xs.as<double>(), s.Hs.as<double>(), s.mass.as<double>(), s.perm.as<int32_t>(),
-                     s.offsets.as<int64_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, useLds, s.rho.as<double>());
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, s.rho.as<double>());
+  };
+  if (s.isotropic) { if (lds) go(k_sum_density<D, true, true>); else go(k_sum_density<D, true, false>); }
+  else { if (lds) go(k_sum_density<D, false, true>); else go(k_sum_density<D, false, false>); }
   SG_CHECK_HIP(hipGetLastError());
   SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
 }
 
 template <int D>
 void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
-  using R = Rec<D>;
   const int64_t n = s.nTot;
   hipStream_t st = s.stream;
-  s.rec.ensure(sizeof(double) * size_t(n) * R::N);
   const int total = derivTotalDoubles(s.dim);
   s.out.ensure(sizeof(double) * size_t(n) * total);
   s.outStride = n;
@@ -414,17 +537,13 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   unsigned long long* dncalc = reinterpret_cast<unsigned long long*>(s.reduceTmp.as<double>() + 8);
 
   SG_CHECK_HIP(hipEventRecord(s.ev[4], st));
-  hipLaunchKernelGGL((k_pack<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), n, s.pos.as<double>(),
-                     s.vel.as<double>(), s.mass.as<double>(), s.rho.as<double>(), s.P.as<double>(), s.cs.as<double>(),
-                     s.H.as<double>(), s.omega.as<double>(), s.hasMultipliers ? s.fCl.as<double>() : nullptr,
-                     s.hasMultipliers ? s.fCq.as<double>() : nullptr, s.hasMultipliers ? s.fshear.as<double>() : nullptr,
-                     s.rec.as<double>());
+  if (!s.recValid) { packD<D>(s); s.recValid = true; }
   SG_CHECK_HIP(hipMemsetAsync(dncalc, 0, sizeof(unsigned long long), st));
   SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
 
-  TabArgs t{s.dTable.as<double>(), s.table.n, s.table.step, s.table.extent, s.dNperh.as<double>(), s.dWsum.as<double>()};
-  const size_t ldsBytes = sizeof(double) * 6 * size_t(s.table.n);
-  const bool useLds = ldsBytes <= 64 * 1024;
+  TabArgs t = tabArgs(s);
+  const bool lds = s.table.n <= kLdsTablePoints;
+  const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   PassArgs p{};
   p.Cl = hp.Cl; p.Cq = hp.Cq; p.eps2 = hp.eps2; p.epsTensile = hp.epsTensile; p.nPerh = hp.nPerh;
   p.hmin = hp.hmin; p.hmax = hp.hmax;
@@ -437,16 +556,17 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   o.n = n;
   for (int f = 0; f < SPHGPU_NUM_DERIV_FIELDS; ++f) o.off[f] = derivOffsetDoubles(s.dim, f);
   const bool tensile = hp.epsTensile != 0.0;
-  const bool mult = s.hasMultipliers;
   auto launch = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), useLds ? ldsBytes : 0, st, s.rec.as<double>(),
-                       s.perm.as<int32_t>(), s.offsets.as<int64_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, useLds, p, o,
-                       dncalc);
+    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, p, o, dncalc);
   };
-  if (tensile && mult) launch(k_derivs_fp64<D, true, true>);
-  else if (tensile) launch(k_derivs_fp64<D, true, false>);
-  else if (mult) launch(k_derivs_fp64<D, false, true>);
-  else launch(k_derivs_fp64<D, false, false>);
+  if (lds) {
+    if (s.isotropic) { if (tensile) launch(k_derivs_fp64<D, true, true, true>); else launch(k_derivs_fp64<D, true, false, true>); }
+    else { if (tensile) launch(k_derivs_fp64<D, false, true, true>); else launch(k_derivs_fp64<D, false, false, true>); }
+  } else {
+    if (s.isotropic) { if (tensile) launch(k_derivs_fp64<D, true, true, false>); else launch(k_derivs_fp64<D, true, false, false>); }
+    else { if (tensile) launch(k_derivs_fp64<D, false, true, false>); else launch(k_derivs_fp64<D, false, false, false>); }
+  }
   SG_CHECK_HIP(hipGetLastError());
   SG_CHECK_HIP(hipEventRecord(s.ev[6], st));
   unsigned long long nc = 0;
@@ -457,10 +577,17 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
 
 }  // namespace
 
+void packRecords(Sim& s) {
+  if (s.dim == 1) packD<1>(s);
+  else if (s.dim == 2) packD<2>(s);
+  else packD<3>(s);
+  s.recValid = true;
+}
+
 void sumDensity(Sim& s) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "sum density needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
-  if (s.xsStale) regatherSorted(s);
+  if (!s.recValid) packRecords(s);
   if (s.dim == 1) sumDensityD<1>(s);
   else if (s.dim == 2) sumDensityD<2>(s);
   else sumDensityD<3>(s);
@@ -468,6 +595,7 @@ void sumDensity(Sim& s) {
   SG_CHECK_HIP(hipEventSynchronize(s.ev[3]));
   SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[2], s.ev[3]));
   s.timing.density_ms = ms;
+  s.derivsValid = false;
 }
 
 void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
@@ -477,6 +605,7 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
              "compatibleEnergyEvolution (pairAccelerations) is not produced by the device pass");
   SG_REQUIRE(p.precision == SPHGPU_PRECISION_FP64, SPHGPU_ERR_UNSUPPORTED, "precision mode not available");
   SG_REQUIRE(p.nPerh > 0.0, SPHGPU_ERR_ARG, "nPerh must be positive");
+  s.derivsValid = false;
   if (s.dim == 1) evalDerivsD<1>(s, p);
   else if (s.dim == 2) evalDerivsD<2>(s, p);
   else evalDerivsD<3>(s, p);

@@ -42,7 +42,7 @@ struct DBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// Tabulated kernel as held on the device: per table point i1 the parabola
+// Tabulated kernel as held on the host: per table point i1 the parabola
 // coefficients of W and dW/deta, plus the nperh <-> Wsum tables.
 struct TableHost {
   int n = 0;
@@ -55,11 +55,12 @@ void buildBSplineTable(int dim, int npoints, TableHost& t);
 
 struct Grid {
   double lo[3] = {0, 0, 0};
-  double cell = 1.0;        // cell edge (>= search radius)
-  double R2 = 0.0;          // squared prefilter radius
+  double cell = 1.0;        // cell edge (half the search radius on the staged path)
+  double R2 = 0.0;          // squared search radius bound
   int nc[3] = {1, 1, 1};    // cells per axis
-  int nb[3] = {1, 1, 1};    // 4-cell bricks per axis
-  int64_t ncells = 0;       // nb[0]*nb[1]*nb[2]*brickVolume
+  int nb[3] = {1, 1, 1};    // bricks per axis
+  int64_t ncells = 0;
+  int span = 1;             // stencil half-width in cells (1: cell = R, 2: cell = R/2)
 };
 
 struct Timing {
@@ -81,23 +82,29 @@ struct Sim {
   DBuf pos, vel, mass, rho, eps, P, cs, H, omega, fCl, fCq, fshear;
   bool hasMultipliers = false;
 
-  // neighbour structure (sorted order)
+  // neighbour structure (nodes in sorted order; rows in transposed ELL:
+  // entry w of slot k at nbr[w * nTot + k], rowCount[k] entries)
   Grid grid;
   double kernelExtent = 2.0;
+  int rowOrder = SPHGPU_ROW_ORDER_REFERENCE;
   bool neighborsValid = false;
-  bool xsStale = false;      // positions/H uploaded since the last build
+  bool isotropic = false;   // every H is h*I (off-diagonals exactly zero)
   DBuf cellOf, cellCount, cellStart, cellFill;
   DBuf perm;                // sorted slot -> caller index
   DBuf iperm;               // caller index -> sorted slot
   DBuf xs, Hs;              // positions / H in sorted order
-  DBuf rowCount, offsets, nbr;
+  DBuf rowCount, nbr;
+  int ellCap = 0;           // capacity of the blocked-transposed ELL (multiple of 4)
   int64_t nEntries = 0;
-  DBuf scanTmp, reduceTmp;
+  int maxRow = 0;
+  int stagedFallbacks = 0;  // bricks whose candidates exceeded the LDS stage
+  DBuf scanTmp, reduceTmp, flags;
 
-  // packed per-node records and outputs (sorted order)
+  // packed per-node records (sorted order) and outputs (sorted order)
   DBuf rec;
+  bool recValid = false;    // records match the caller-order state
   DBuf out;                 // all derivative fields, field-major, sorted order
-  int64_t outStride = 0;    // doubles per field block = nTot rounded
+  int64_t outStride = 0;
   bool derivsValid = false;
   int64_t ncalc = 0;        // unordered pairs evaluated by the last pass
 
@@ -107,20 +114,25 @@ struct Sim {
 
 // neighbors.hip
 void buildNeighbors(Sim& s, double kernelExtent);
-void regatherSorted(Sim& s);   // refresh xs/Hs from the caller-order state
 // hydro.hip
+void packRecords(Sim& s);
 void sumDensity(Sim& s);
 void evalDerivs(Sim& s, const sphgpu_hydro_params& p);
-// scan helpers (neighbors.hip)
+// scans (neighbors.hip)
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);
 
 inline int nsymOf(int dim) { return dim * (dim + 1) / 2; }
 
 // Number of doubles per node of each derivative output field, in the order of
-// sphgpu_field ids SPHGPU_D_* (see sphgpu.h).
+// the SPHGPU_D_* ids (see sphgpu.h).
 int derivComponents(int dim, int field);
 int derivOffsetDoubles(int dim, int field);   // offset of a field's block, per node
 int derivTotalDoubles(int dim);
+
+// Record layout shared by the record packer and the pair kernels: one 128-byte
+// line per node for the isotropic-H path (H = h I), two lines otherwise.
+// Fields: [x y z | h^-1 or H | m Hdet orig rho] [v | cs Prho m/rho P som fCl fCq fsh]
+template <int D> struct RecLayout;
 
 }  // namespace sphgpu

@@ -2,24 +2,37 @@
 //
 // Contract (src/Neighbor/ConnectivityMap.cc:630-749): for an internal node i, j
 // is a neighbour iff |H_i r_ij|^2 <= ext^2 or |H_j r_ij|^2 <= ext^2 (FP64,
-// r_ij = r_i - r_j, SymTensor.Vector and magnitude2 summed left to right), j != i,
-// and each row is sorted ascending by node index (:735-738).  How candidates are
-// found is free (the reference's NestedGridNeighbor only has to return a
-// superset), so this uses one uniform grid:
+// r_ij = r_i - r_j, SymTensor.Vector and magnitude2 summed left to right), j != i.
+// The reference orders each row by node index (:735-738); that order is kept on
+// request (SPHGPU_ROW_ORDER_REFERENCE, needed for reference-order FP64 sums),
+// otherwise rows keep the deterministic search order.  How candidates are found
+// is free (the reference's NestedGridNeighbor only has to return a superset):
 //
-//  1. bounds: min/max position and the smallest Gershgorin lower bound of
-//     lambda_min(H) over all nodes (internal + ghost).  |H r| <= ext implies
-//     |r| <= ext/lambda_min(H), so a cell edge R = ext/min(lambda) (plus a 2e-6
-//     relative margin) makes the 3^nu cells around a node a superset.
-//  2. counting sort of nodes by cell id.  Cells are numbered brick by brick
-//     (4^nu cells per brick, Z-order inside a brick) so that a wavefront's nodes
-//     and their neighbours are close in memory.  Each cell is then ordered by
-//     node index, so the sorted order depends only on positions and indices.
-//  3. per sorted node: count pass, scan, fill pass over the 3^nu cells with a
-//     |r|^2 <= R^2 prefilter followed by the exact test.
-//  4. per row: bitonic sort in LDS by node index (rows store sorted slots).
+//  1. bounds: min/max position, the smallest Gershgorin lower bound of
+//     lambda_min(H) over all nodes, and whether every H is h I.  |H r| <= ext
+//     implies |r| <= R = ext/min(lambda) (plus a 1e-6 relative margin).
+//  2. counting sort of nodes by cell.  Cells are numbered brick by brick
+//     (4^nu cells per brick, Z-order inside) so a brick's nodes are contiguous;
+//     each cell is then ordered by node index (the order depends only on
+//     positions and indices).  Records of all nodes are packed in this order.
+//  3. rows, one thread per internal node, into a blocked-transposed ELL: entry
+//     w of slot k at nbr[((w/4)*n + k)*4 + w%4], so four entries are one int4
+//     and a wavefront's int4s are contiguous (coalesced writes here, coalesced
+//     int4 reads in the passes):
+//      - 3-D with isotropic H: cells of edge R/2, one workgroup per brick
+//        stages the 8^3 cells around it in LDS (FP32 offsets from the brick
+//        origin), each node scans its 5^3 cells; the FP32 estimate of h^2|r|^2
+//        decides every pair farther than 1e-5 (relative) from the threshold,
+//        the rest take the exact FP64 reference test;
+//      - otherwise: cells of edge R, 3^nu stencil read from global memory.
+//     Rows longer than the ELL capacity are counted, the build grows the
+//     capacity and repeats the row pass.
+//  4. optionally, rows sorted by node index (one wavefront per row, bitonic
+//     sort in registers).
 #include "internal.hh"
 #include "dgeom.hh"
+#include "records.hh"
+#include "ell.hh"
 
 namespace sphgpu {
 
@@ -35,47 +48,71 @@ template <int D> __device__ double gershgorinMin(const double* h) {
   else if constexpr (D == 2) return dmin(h[0] - fabs(h[1]), h[2] - fabs(h[1]));
   else return dmin(h[0] - fabs(h[1]) - fabs(h[2]), dmin(h[3] - fabs(h[1]) - fabs(h[4]), h[5] - fabs(h[2]) - fabs(h[4])));
 }
+template <int D> __device__ bool isIsotropic(const double* h) {
+  if constexpr (D == 1) return true;
+  else if constexpr (D == 2) return h[1] == 0.0 && h[2] == h[0];
+  else return h[1] == 0.0 && h[2] == 0.0 && h[4] == 0.0 && h[3] == h[0] && h[5] == h[0];
+}
 
-// partials[b*7 + {0..2 lo, 3..5 hi, 6 min lambda}]
+// reduced values: lo[3] hi[3] min Gershgorin bound, isotropic flag, max diagonal of H
+constexpr int kNB = 9;
+__device__ __forceinline__ bool isMaxSlot(int k) { return (k >= 3 && k < 6) || k == 8; }
+
+template <int D> __device__ double maxDiag(const double* h) {
+  double m = h[0];
+  for (int a = 1; a < D; ++a) m = dmax(m, h[sdiag<D>(a)]);
+  return m;
+}
+
 template <int D>
 __global__ void k_bounds(const double* __restrict__ pos, const double* __restrict__ H, int64_t n, double* partials) {
-  __shared__ double sh[7][kBlock];
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, lm = 1e300;
+  __shared__ double sh[kNB][kBlock];
+  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300};
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    for (int k = 0; k < D; ++k) { const double x = pos[i * D + k]; lo[k] = dmin(lo[k], x); hi[k] = dmax(hi[k], x); }
-    lm = dmin(lm, gershgorinMin<D>(H + i * NSym<D>::value));
+    for (int k = 0; k < D; ++k) { const double x = pos[i * D + k]; r[k] = dmin(r[k], x); r[3 + k] = dmax(r[3 + k], x); }
+    const double* h = H + i * NSym<D>::value;
+    r[6] = dmin(r[6], gershgorinMin<D>(h));
+    if (!isIsotropic<D>(h)) r[7] = 0.0;
+    r[8] = dmax(r[8], maxDiag<D>(h));
   }
-  for (int k = 0; k < 3; ++k) { sh[k][threadIdx.x] = lo[k]; sh[3 + k][threadIdx.x] = hi[k]; }
-  sh[6][threadIdx.x] = lm;
+  for (int k = 0; k < kNB; ++k) sh[k][threadIdx.x] = r[k];
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      for (int k = 0; k < 3; ++k) {
-        sh[k][threadIdx.x] = dmin(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
-        sh[3 + k][threadIdx.x] = dmax(sh[3 + k][threadIdx.x], sh[3 + k][threadIdx.x + s]);
+    if (threadIdx.x < s)
+      for (int k = 0; k < kNB; ++k) {
+        const double a = sh[k][threadIdx.x], b = sh[k][threadIdx.x + s];
+        sh[k][threadIdx.x] = isMaxSlot(k) ? dmax(a, b) : dmin(a, b);
       }
-      sh[6][threadIdx.x] = dmin(sh[6][threadIdx.x], sh[6][threadIdx.x + s]);
-    }
     __syncthreads();
   }
-  if (threadIdx.x < 7) partials[blockIdx.x * 7 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x < kNB) partials[blockIdx.x * kNB + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ void k_bounds_final(const double* partials, int nb, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double r[7] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300};
-  for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < 7; ++k) {
-      const double v = partials[b * 7 + k];
-      r[k] = (k >= 3 && k < 6) ? dmax(r[k], v) : dmin(r[k], v);
+  __shared__ double sh[kNB][kBlock];
+  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int k = 0; k < kNB; ++k) {
+      const double v = partials[b * kNB + k];
+      r[k] = isMaxSlot(k) ? dmax(r[k], v) : dmin(r[k], v);
     }
-  for (int k = 0; k < 7; ++k) out[k] = r[k];
+  for (int k = 0; k < kNB; ++k) sh[k][threadIdx.x] = r[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < kNB; ++k) {
+        const double a = sh[k][threadIdx.x], b = sh[k][threadIdx.x + s];
+        sh[k][threadIdx.x] = isMaxSlot(k) ? dmax(a, b) : dmin(a, b);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < kNB) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 struct DGrid {
   double lo[3];
+  double cell;
   double invCell;
-  double R2;
   int nc[3];
   int nb[3];
 };
@@ -90,17 +127,17 @@ template <int D> __device__ __forceinline__ void cellCoords(const DGrid& g, cons
 }
 
 // Brick-ordered cell id: bricks of 4^nu cells in row-major order, Z-order inside.
-template <int D> __device__ __forceinline__ int64_t cellId(const DGrid& g, const int* c) {
+template <int D> __device__ __forceinline__ int64_t cellId(const DGrid& g, int cx, int cy, int cz) {
   if constexpr (D == 1) {
-    return c[0];
+    return cx;
   } else if constexpr (D == 2) {
-    const int64_t b = int64_t(c[1] >> 2) * g.nb[0] + (c[0] >> 2);
-    const int lx = c[0] & 3, ly = c[1] & 3;
+    const int64_t b = int64_t(cy >> 2) * g.nb[0] + (cx >> 2);
+    const int lx = cx & 3, ly = cy & 3;
     const int z = (lx & 1) | ((ly & 1) << 1) | ((lx & 2) << 1) | ((ly & 2) << 2);
     return b * 16 + z;
   } else {
-    const int64_t b = (int64_t(c[2] >> 2) * g.nb[1] + (c[1] >> 2)) * g.nb[0] + (c[0] >> 2);
-    const int lx = c[0] & 3, ly = c[1] & 3, lz = c[2] & 3;
+    const int64_t b = (int64_t(cz >> 2) * g.nb[1] + (cy >> 2)) * g.nb[0] + (cx >> 2);
+    const int lx = cx & 3, ly = cy & 3, lz = cz & 3;
     const int z = (lx & 1) | ((ly & 1) << 1) | ((lz & 1) << 2) | ((lx & 2) << 2) | ((ly & 2) << 3) | ((lz & 2) << 4);
     return b * 64 + z;
   }
@@ -113,7 +150,7 @@ __global__ void k_cell_count(const double* __restrict__ pos, int64_t n, DGrid g,
   if (i >= n) return;
   int c[3];
   cellCoords<D>(g, pos + i * D, c);
-  const int32_t id = int32_t(cellId<D>(g, c));
+  const int32_t id = int32_t(cellId<D>(g, c[0], c[1], c[2]));
   cellOf[i] = id;
   atomicAdd(&cellCount[id], 1);
 }
@@ -127,7 +164,8 @@ __global__ void k_cell_scatter(const int32_t* __restrict__ cellOf, const int32_t
   perm[slot] = int32_t(i);
 }
 
-// Order each cell's members by node index (insertion sort; cells hold tens of nodes).
+// Order each cell's members by node index (insertion sort; cells hold a few to
+// tens of nodes).
 __global__ void k_cell_sort(const int32_t* __restrict__ cellStart, int32_t* __restrict__ perm, int64_t ncells) {
   const int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (c >= ncells) return;
@@ -140,121 +178,338 @@ __global__ void k_cell_sort(const int32_t* __restrict__ cellStart, int32_t* __re
   }
 }
 
-template <int D>
-__global__ void k_gather_xh(const int32_t* __restrict__ perm, const double* __restrict__ pos, const double* __restrict__ H,
-                            double* __restrict__ xs, double* __restrict__ Hs, int32_t* __restrict__ iperm, int64_t n) {
+__global__ void k_iperm(const int32_t* __restrict__ perm, int32_t* __restrict__ iperm, int64_t n) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k >= n) return;
-  const int32_t i = perm[k];
-  for (int c = 0; c < D; ++c) xs[k * D + c] = pos[int64_t(i) * D + c];
-  constexpr int NS = NSym<D>::value;
-  for (int c = 0; c < NS; ++c) Hs[k * NS + c] = H[int64_t(i) * NS + c];
-  iperm[i] = int32_t(k);
+  if (k < n) iperm[perm[k]] = int32_t(k);
 }
 
-// Exact ConnectivityMap test for the pair (i, j) in i's frame.
-template <int D> __device__ __forceinline__ bool exactTest(const V<D>& rij, const S<D>& Hi, const S<D>& Hj, double ext2) {
-  const double eta2i = vmag2(sdot(Hi, rij));
-  const double eta2j = vmag2(sdot(Hj, rij));
-  return (eta2i <= ext2 || eta2j <= ext2);
+// Row writer: keeps four entries in registers and stores them as one int4.
+struct RowWriter {
+  int4 buf;
+  int cnt;
+  int cap;
+  int64_t n, k;
+  int4* out;
+  __device__ __forceinline__ RowWriter(int4* o, int64_t n_, int64_t k_, int cap_) : buf{0, 0, 0, 0}, cnt(0), cap(cap_), n(n_), k(k_), out(o) {}
+  __device__ __forceinline__ void push(int32_t v) {
+    const int l = cnt & 3;
+    if (l == 0) buf.x = v; else if (l == 1) buf.y = v; else if (l == 2) buf.z = v; else buf.w = v;
+    if (l == 3 && cnt < cap) out[ellChunk(cnt >> 2, n, k)] = buf;
+    ++cnt;
+  }
+  __device__ __forceinline__ void finish() {
+    if ((cnt & 3) != 0 && cnt <= cap) out[ellChunk(cnt >> 2, n, k)] = buf;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Staged brick search, 3-D, isotropic H.
+// ---------------------------------------------------------------------------
+constexpr int kStageEdge = 8;                       // 4-cell brick + 2 cells each side
+constexpr int kStaged = kStageEdge * kStageEdge * kStageEdge;
+constexpr int kCandCap = 2048;
+
+struct SearchArgs {
+  DGrid g;
+  double ext2;
+  float lo32, hi32;         // ext2*(1 -+ 1e-5): FP32 classification margins
+  int64_t n, nInt;
+  int cap;
+};
+
+// Exact reference test for isotropic H: eta = h r componentwise (the SymTensor
+// product with zero off-diagonals), magnitude2 left to right.
+__device__ __forceinline__ bool exactIso(double rx, double ry, double rz, double hi, double hj, double ext2) {
+  const double ax = hi * rx, ay = hi * ry, az = hi * rz;
+  const double bx = hj * rx, by = hj * ry, bz = hj * rz;
+  const double e2i = ax * ax + ay * ay + az * az;
+  const double e2j = bx * bx + by * by + bz * bz;
+  return (e2i <= ext2 || e2j <= ext2);
 }
 
-template <int D, bool FILL>
-__global__ void k_rows(const double* __restrict__ xs, const double* __restrict__ Hs, const int32_t* __restrict__ perm,
-                       const int32_t* __restrict__ cellStart, DGrid g, double ext2, int64_t n, int64_t nInt,
-                       int32_t* __restrict__ rowCount, const int64_t* __restrict__ offsets, int32_t* __restrict__ nbr) {
-  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k >= n) return;
-  if (perm[k] >= nInt) { if (!FILL) rowCount[k] = 0; return; }
-  constexpr int NS = NSym<D>::value;
-  V<D> ri; S<D> Hi;
-  for (int c = 0; c < D; ++c) ri.c[c] = xs[k * D + c];
-  for (int c = 0; c < NS; ++c) Hi.c[c] = Hs[k * NS + c];
-  int cc[3];
-  cellCoords<D>(g, ri.c, cc);
-  int32_t count = 0;
-  int64_t w = FILL ? offsets[k] : 0;
-  const int64_t wEnd = FILL ? offsets[k + 1] : 0;   // the count pass sized this row
-  const int dz0 = D >= 3 ? -1 : 0, dz1 = D >= 3 ? 1 : 0;
-  const int dy0 = D >= 2 ? -1 : 0, dy1 = D >= 2 ? 1 : 0;
-  for (int dz = dz0; dz <= dz1; ++dz) {
-    const int z = cc[2] + dz;
-    if (z < 0 || z >= g.nc[2]) continue;
-    for (int dy = dy0; dy <= dy1; ++dy) {
-      const int y = cc[1] + dy;
-      if (y < 0 || y >= g.nc[1]) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = cc[0] + dx;
-        if (x < 0 || x >= g.nc[0]) continue;
-        const int c3[3] = {x, y, z};
-        const int64_t id = cellId<D>(g, c3);
-        const int32_t b = cellStart[id], e = cellStart[id + 1];
-        for (int32_t j = b; j < e; ++j) {
-          if (j == k) continue;
-          V<D> rij;
-          for (int c = 0; c < D; ++c) rij.c[c] = ri.c[c] - xs[int64_t(j) * D + c];
-          if (vmag2(rij) > g.R2) continue;
-          S<D> Hj;
-          for (int c = 0; c < NS; ++c) Hj.c[c] = Hs[int64_t(j) * NS + c];
-          if (exactTest<D>(rij, Hi, Hj, ext2)) {
-            if (FILL && w < wEnd) nbr[w++] = j;
-            ++count;
+__global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restrict__ rec, const int32_t* __restrict__ cellStart,
+                                                         SearchArgs a, int32_t* __restrict__ rowCount,
+                                                         int4* __restrict__ nbr, int32_t* __restrict__ flags) {
+  using R = RL<true>;
+  __shared__ int32_t sOff[kStaged + 1];
+  __shared__ int32_t sStart[kStaged];
+  __shared__ float4 sCand[kCandCap];               // x,y,z offsets from the brick origin (FP32), 1/h
+  __shared__ int32_t sSlot[kCandCap];
+  __shared__ int32_t sc[kBlock];
+  const DGrid& g = a.g;
+  const int b = blockIdx.x;
+  const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
+  const int64_t h0 = cellStart[int64_t(b) * 64], h1 = cellStart[int64_t(b) * 64 + 64];
+  if (h0 == h1) return;                           // uniform over the block
+  const int cx0 = bx * 4 - 2, cy0 = by * 4 - 2, cz0 = bz * 4 - 2;
+  const double ox = g.lo[0] + (bx * 4) * g.cell, oy = g.lo[1] + (by * 4) * g.cell, oz = g.lo[2] + (bz * 4) * g.cell;
+  // staged cell table (x fastest)
+  for (int t = threadIdx.x; t < kStaged; t += blockDim.x) {
+    const int sx = t % kStageEdge, sy = (t / kStageEdge) % kStageEdge, sz = t / (kStageEdge * kStageEdge);
+    const int cx = cx0 + sx, cy = cy0 + sy, cz = cz0 + sz;
+    int32_t st = 0, cnt = 0;
+    if (cx >= 0 && cx < g.nc[0] && cy >= 0 && cy < g.nc[1] && cz >= 0 && cz < g.nc[2]) {
+      const int64_t id = cellId<3>(g, cx, cy, cz);
+      st = cellStart[id];
+      cnt = cellStart[id + 1] - st;
+    }
+    sStart[t] = st;
+    sOff[t + 1] = cnt;
+  }
+  if (threadIdx.x == 0) sOff[0] = 0;
+  __syncthreads();
+  // inclusive scan of sOff[1..512] (two elements per thread)
+  {
+    const int t0 = 2 * threadIdx.x + 1, t1 = t0 + 1;
+    const int32_t v0 = sOff[t0], v1 = sOff[t1];
+    const int32_t pair = v0 + v1;
+    sc[threadIdx.x] = pair;
+    __syncthreads();
+    for (int s = 1; s < kBlock; s <<= 1) {
+      const int32_t add = threadIdx.x >= s ? sc[threadIdx.x - s] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int32_t excl = sc[threadIdx.x] - pair;
+    sOff[t0] = excl + v0;
+    sOff[t1] = excl + v0 + v1;
+  }
+  __syncthreads();
+  const int32_t total = sOff[kStaged];
+  const bool staged = total <= kCandCap;
+  if (staged) {
+    for (int t = threadIdx.x; t < kStaged; t += blockDim.x) {
+      const int32_t o = sOff[t], cnt = sOff[t + 1] - o, st = sStart[t];
+      for (int q = 0; q < cnt; ++q) {
+        const double4 r = *reinterpret_cast<const double4*>(rec + int64_t(st + q) * R::STRIDE);
+        sCand[o + q] = make_float4(float(r.x - ox), float(r.y - oy), float(r.z - oz), float(r.w));
+        sSlot[o + q] = st + q;
+      }
+    }
+  } else if (threadIdx.x == 0) {
+    atomicAdd(&flags[1], 1);                      // bricks that read candidates from global memory
+  }
+  __syncthreads();
+
+  for (int64_t k = h0 + threadIdx.x; k < h1; k += blockDim.x) {
+    const double* ri = rec + k * R::STRIDE;
+    if (int64_t(ri[R::ORIG]) >= a.nInt) { rowCount[k] = 0; continue; }
+    const double xi = ri[R::X], yi = ri[R::X + 1], zi = ri[R::X + 2], hi = ri[R::HINV];
+    const float fx = float(xi - ox), fy = float(yi - oy), fz = float(zi - oz);
+    const float hi2 = float(hi) * float(hi);
+    int c[3];
+    const double xv[3] = {xi, yi, zi};
+    cellCoords<3>(g, xv, c);
+    const int sx = c[0] - cx0, sy = c[1] - cy0, sz = c[2] - cz0;   // in [2, 5]
+    RowWriter row(nbr, a.n, k, a.cap);
+    for (int dz = -2; dz <= 2; ++dz) {
+      for (int dy = -2; dy <= 2; ++dy) {
+        const int base = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge + (sx - 2);
+        if (staged) {
+          const int32_t t0 = sOff[base], t1 = sOff[base + 5];
+          for (int32_t t = t0; t < t1; ++t) {
+            const float4 cj = sCand[t];
+            const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
+            const float q = rx * rx + ry * ry + rz * rz;
+            const float ti = hi2 * q, tj = (cj.w * cj.w) * q;
+            const float tmin = ti < tj ? ti : tj;
+            bool in = tmin < a.lo32;
+            if (!in && tmin <= a.hi32) {
+              const int32_t sj = sSlot[t];
+              const double* rj = rec + int64_t(sj) * R::STRIDE;
+              in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
+            }
+            if (in) {
+              const int32_t sj = sSlot[t];
+              if (sj != int32_t(k)) row.push(sj);
+            }
+          }
+        } else {
+          for (int dx = 0; dx < 5; ++dx) {
+            const int cidx = base + dx;
+            const int32_t st = sStart[cidx], ncell = sOff[cidx + 1] - sOff[cidx];
+            for (int32_t q2 = 0; q2 < ncell; ++q2) {
+              const int32_t sj = st + q2;
+              if (sj == int32_t(k)) continue;
+              const double* rj = rec + int64_t(sj) * R::STRIDE;
+              if (exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2)) row.push(sj);
+            }
           }
         }
       }
     }
+    row.finish();
+    rowCount[k] = row.cnt;
   }
-  if (!FILL) rowCount[k] = count;
 }
 
-// Sort each row (sorted slots) ascending by node index.  One 64-lane block per
-// row; rows up to kRowCap entries are sorted in LDS, longer ones by lane 0.
-constexpr int kRowCap = 1024;
-__global__ void __launch_bounds__(64) k_row_sort(const int64_t* __restrict__ offsets, const int32_t* __restrict__ perm,
-                                                 int32_t* __restrict__ nbr, int64_t n) {
-  __shared__ uint64_t key[kRowCap];
-  const int64_t row = blockIdx.x;
-  if (row >= n) return;
-  const int64_t b = offsets[row], e = offsets[row + 1];
-  const int len = int(e - b);
-  if (len <= 1) return;
-  if (len > kRowCap) {
-    if (threadIdx.x == 0) {
-      for (int64_t a = b + 1; a < e; ++a) {
-        const int32_t v = nbr[a];
-        const int32_t pv = perm[v];
-        int64_t q = a - 1;
-        while (q >= b && perm[nbr[q]] > pv) { nbr[q + 1] = nbr[q]; --q; }
-        nbr[q + 1] = v;
-      }
-    }
-    return;
+// ---------------------------------------------------------------------------
+// Global search: any dimension, any H (cell edge = R, 3^nu stencil).
+// ---------------------------------------------------------------------------
+template <int D, bool ISO>
+__global__ void k_search_global(const double* __restrict__ rec, const int32_t* __restrict__ cellStart, SearchArgs a,
+                                int32_t* __restrict__ rowCount, int4* __restrict__ nbr) {
+  using R = RL<ISO>;
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k >= a.n) return;
+  const double* ri_ = rec + k * R::STRIDE;
+  if (int64_t(ri_[R::ORIG]) >= a.nInt) { rowCount[k] = 0; return; }
+  constexpr int NS = NSym<D>::value;
+  V<D> ri; S<D> Hi;
+  for (int c = 0; c < D; ++c) ri.c[c] = ri_[R::X + c];
+  if constexpr (ISO) {
+    for (int c = 0; c < NS; ++c) Hi.c[c] = 0.0;
+    for (int c = 0; c < D; ++c) Hi.c[sdiag<D>(c)] = ri_[RL<true>::HINV];
+  } else {
+    for (int c = 0; c < NS; ++c) Hi.c[c] = ri_[RL<false>::H + c];
   }
-  int np2 = 1;
-  while (np2 < len) np2 <<= 1;
-  for (int t = threadIdx.x; t < np2; t += 64) {
-    if (t < len) {
-      const uint32_t slot = uint32_t(nbr[b + t]);
-      key[t] = (uint64_t(uint32_t(perm[slot])) << 32) | slot;
-    } else {
-      key[t] = ~uint64_t(0);
-    }
-  }
-  __syncthreads();
-  for (int kk = 2; kk <= np2; kk <<= 1) {
-    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-      for (int t = threadIdx.x; t < np2; t += 64) {
-        const int p = t ^ jj;
-        if (p > t) {
-          const uint64_t a = key[t], c = key[p];
-          const bool up = (t & kk) == 0;
-          if ((a > c) == up) { key[t] = c; key[p] = a; }
+  int cc[3];
+  cellCoords<D>(a.g, ri.c, cc);
+  RowWriter row(nbr, a.n, k, a.cap);
+  const int dz0 = D >= 3 ? -1 : 0, dz1 = D >= 3 ? 1 : 0;
+  const int dy0 = D >= 2 ? -1 : 0, dy1 = D >= 2 ? 1 : 0;
+  for (int dz = dz0; dz <= dz1; ++dz) {
+    const int z = cc[2] + dz;
+    if (z < 0 || z >= a.g.nc[2]) continue;
+    for (int dy = dy0; dy <= dy1; ++dy) {
+      const int y = cc[1] + dy;
+      if (y < 0 || y >= a.g.nc[1]) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = cc[0] + dx;
+        if (x < 0 || x >= a.g.nc[0]) continue;
+        const int64_t id = cellId<D>(a.g, x, y, z);
+        const int32_t b = cellStart[id], e = cellStart[id + 1];
+        for (int32_t j = b; j < e; ++j) {
+          if (j == k) continue;
+          const double* rj_ = rec + int64_t(j) * R::STRIDE;
+          V<D> rij;
+          for (int c = 0; c < D; ++c) rij.c[c] = ri.c[c] - rj_[R::X + c];
+          S<D> Hj;
+          if constexpr (ISO) {
+            for (int c = 0; c < NS; ++c) Hj.c[c] = 0.0;
+            for (int c = 0; c < D; ++c) Hj.c[sdiag<D>(c)] = rj_[RL<true>::HINV];
+          } else {
+            for (int c = 0; c < NS; ++c) Hj.c[c] = rj_[RL<false>::H + c];
+          }
+          const double eta2i = vmag2(sdot(Hi, rij));
+          const double eta2j = vmag2(sdot(Hj, rij));
+          if (eta2i <= a.ext2 || eta2j <= a.ext2) row.push(j);
         }
       }
-      __syncthreads();
     }
   }
-  for (int t = threadIdx.x; t < len; t += 64) nbr[b + t] = int32_t(uint32_t(key[t] & 0xffffffffu));
+  row.finish();
+  rowCount[k] = row.cnt;
+}
+
+// Total entries and longest row.
+__global__ void k_row_stats(const int32_t* __restrict__ rowCount, int64_t n, unsigned long long* __restrict__ total,
+                            int32_t* __restrict__ maxRow) {
+  __shared__ unsigned long long st[kBlock];
+  __shared__ int32_t sm[kBlock];
+  unsigned long long s = 0;
+  int32_t m = 0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t c = rowCount[i];
+    s += unsigned(c);
+    m = c > m ? c : m;
+  }
+  st[threadIdx.x] = s;
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      st[threadIdx.x] += st[threadIdx.x + w];
+      sm[threadIdx.x] = sm[threadIdx.x] > sm[threadIdx.x + w] ? sm[threadIdx.x] : sm[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(total, st[0]);
+    atomicMax(maxRow, sm[0]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reference row order: sort each row by caller index.  One wavefront per row,
+// K keys per lane (rows up to 256 entries), bitonic network in registers;
+// longer rows are sorted by a single lane.
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void waveBitonic(int32_t (&key)[K], int lane) {
+  constexpr int N = 64 * K;
+#pragma unroll
+  for (int kk = 2; kk <= N; kk <<= 1) {
+#pragma unroll
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      if (jj >= 64) {
+        const int qj = jj / 64;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          const int p = q ^ qj;
+          if (p > q) {
+            const int e = q * 64 + lane;
+            const bool asc = (e & kk) == 0;
+            const int32_t a = key[q], b = key[p];
+            const bool sw = asc ? (a > b) : (a < b);
+            key[q] = sw ? b : a;
+            key[p] = sw ? a : b;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          const int e = q * 64 + lane;
+          const int32_t other = __shfl_xor(key[q], jj);
+          const bool asc = (e & kk) == 0;
+          const bool lower = (lane & jj) == 0;
+          const int32_t mn = key[q] < other ? key[q] : other;
+          const int32_t mx = key[q] < other ? other : key[q];
+          key[q] = (lower == asc) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+template <int K>
+__device__ void sortRowWave(int32_t* __restrict__ nbr, int64_t n, int64_t k, int len, const int32_t* __restrict__ perm,
+                            const int32_t* __restrict__ iperm, int lane) {
+  int32_t key[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int e = q * 64 + lane;
+    key[q] = e < len ? perm[nbr[ellIndex(e, n, k)]] : 0x7fffffff;
+  }
+  waveBitonic<K>(key, lane);
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int e = q * 64 + lane;
+    if (e < len) nbr[ellIndex(e, n, k)] = iperm[key[q]];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_row_sort(const int32_t* __restrict__ rowCount, int32_t* __restrict__ nbr,
+                                                    int64_t n, const int32_t* __restrict__ perm,
+                                                    const int32_t* __restrict__ iperm) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6;
+  if (k >= n) return;                              // uniform per wavefront
+  const int len = rowCount[k];
+  if (len <= 1) return;
+  if (len <= 64) sortRowWave<1>(nbr, n, k, len, perm, iperm, lane);
+  else if (len <= 128) sortRowWave<2>(nbr, n, k, len, perm, iperm, lane);
+  else if (len <= 256) sortRowWave<4>(nbr, n, k, len, perm, iperm, lane);
+  else if (lane == 0) {
+    for (int a2 = 1; a2 < len; ++a2) {
+      const int32_t v = nbr[ellIndex(a2, n, k)];
+      const int32_t pv = perm[v];
+      int q = a2 - 1;
+      while (q >= 0 && perm[nbr[ellIndex(q, n, k)]] > pv) { nbr[ellIndex(q + 1, n, k)] = nbr[ellIndex(q, n, k)]; --q; }
+      nbr[ellIndex(q + 1, n, k)] = v;
+    }
+  }
 }
 
 // ---- exclusive scans -------------------------------------------------------
@@ -280,7 +535,7 @@ __global__ void k_scan_tiles(const TI* __restrict__ in, TO* __restrict__ out, TO
     sh[threadIdx.x] += t;
     __syncthreads();
   }
-  TO run = sh[threadIdx.x] - local;   // exclusive prefix of this thread
+  TO run = sh[threadIdx.x] - local;
   for (int q = 0; q < kScanItems; ++q) {
     const int64_t i = base + q;
     if (i < n) out[i] = run;
@@ -289,7 +544,6 @@ __global__ void k_scan_tiles(const TI* __restrict__ in, TO* __restrict__ out, TO
   if (threadIdx.x == kBlock - 1) tileSums[blockIdx.x] = sh[kBlock - 1];
 }
 
-// Single-block exclusive scan of the tile sums (in place); writes the total.
 template <class TO>
 __global__ void k_scan_sums(TO* __restrict__ sums, int64_t m, TO* __restrict__ total) {
   __shared__ TO sh[kBlock];
@@ -336,38 +590,46 @@ void exclusiveScan(const TI* in, TO* out, int64_t n, Sim& s) {
 template <int D>
 void buildNeighborsD(Sim& s, double ext) {
   const int64_t n = s.nTot;
-  constexpr int NS = NSym<D>::value;
   hipStream_t st = s.stream;
   SG_CHECK_HIP(hipEventRecord(s.ev[0], st));
 
   // 1. bounds
   const int nbB = int(std::min<int64_t>(1024, nblocks(n)));
-  s.reduceTmp.ensure(sizeof(double) * (size_t(nbB) * 7 + 8));
+  s.reduceTmp.ensure(sizeof(double) * (size_t(nbB) * kNB + 16));
   double* partials = s.reduceTmp.as<double>();
-  double* bounds = partials + size_t(nbB) * 7;
+  double* bounds = partials + size_t(nbB) * kNB;
   hipLaunchKernelGGL((k_bounds<D>), dim3(nbB), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(), n, partials);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, partials, nbB, bounds);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, st, partials, nbB, bounds);
   SG_CHECK_HIP(hipGetLastError());
-  double hb[7];
+  double hb[kNB];
   SG_CHECK_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
   SG_CHECK_HIP(hipStreamSynchronize(st));
   SG_REQUIRE(hb[6] > 0.0 && std::isfinite(hb[6]), SPHGPU_ERR_UNSUPPORTED,
              "neighbour search needs H with a positive Gershgorin bound on every node");
   for (int k = 0; k < D; ++k)
     SG_REQUIRE(std::isfinite(hb[k]) && std::isfinite(hb[3 + k]), SPHGPU_ERR_ARG, "non-finite node position");
+  s.isotropic = hb[7] > 0.5 && !s.hasMultipliers;
 
   Grid& G = s.grid;
   const double R = ext / hb[6] * (1.0 + 1.0e-6);
-  double cell = R * (1.0 + 1.0e-9);
-  double span = 0.0;
-  for (int k = 0; k < D; ++k) span = std::max(span, hb[3 + k] - hb[k]);
+  double spanX = 0.0;
+  for (int k = 0; k < D; ++k) spanX = std::max(spanX, hb[3 + k] - hb[k]);
   const int maxCells = D == 3 ? 1024 : (D == 2 ? 32768 : (1 << 26));
-  if (span / cell > maxCells - 2) cell = span / (maxCells - 2);
+  // Staged path: cells of edge R/2 (5^3 stencil).  Global path: cells of edge
+  // >= R (3^nu stencil), enlarged if the grid would be too fine.
+  bool stagedPath = (D == 3) && s.isotropic;
+  double cell = 0.5 * R * (1.0 + 1.0e-9);
+  if (stagedPath && spanX / cell > maxCells - 2) stagedPath = false;
+  if (!stagedPath) {
+    cell = R * (1.0 + 1.0e-9);
+    if (spanX / cell > maxCells - 2) cell = spanX / (maxCells - 2);
+  }
+  G.span = stagedPath ? 2 : 1;
   G.cell = cell;
   G.R2 = R * R;
   DGrid dg{};
+  dg.cell = cell;
   dg.invCell = 1.0 / cell;
-  dg.R2 = G.R2;
   int64_t bricks = 1;
   const int bw = (D == 1) ? 1 : 4;
   for (int k = 0; k < 3; ++k) {
@@ -387,8 +649,6 @@ void buildNeighborsD(Sim& s, double ext) {
   s.cellStart.ensure(sizeof(int32_t) * size_t(G.ncells + 2));
   s.perm.ensure(sizeof(int32_t) * size_t(n));
   s.iperm.ensure(sizeof(int32_t) * size_t(n));
-  s.xs.ensure(sizeof(double) * size_t(n) * D);
-  s.Hs.ensure(sizeof(double) * size_t(n) * NS);
   SG_CHECK_HIP(hipMemsetAsync(s.cellCount.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
   SG_CHECK_HIP(hipMemsetAsync(s.cellFill.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
   hipLaunchKernelGGL((k_cell_count<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), n, dg,
@@ -398,31 +658,61 @@ void buildNeighborsD(Sim& s, double ext) {
                      s.cellStart.as<int32_t>(), s.cellFill.as<int32_t>(), s.perm.as<int32_t>(), n);
   hipLaunchKernelGGL(k_cell_sort, dim3(nblocks(G.ncells)), dim3(kBlock), 0, st, s.cellStart.as<int32_t>(),
                      s.perm.as<int32_t>(), G.ncells);
-  hipLaunchKernelGGL((k_gather_xh<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.perm.as<int32_t>(),
-                     s.pos.as<double>(), s.H.as<double>(), s.xs.as<double>(), s.Hs.as<double>(),
-                     s.iperm.as<int32_t>(), n);
+  hipLaunchKernelGGL(k_iperm, dim3(nblocks(n)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), s.iperm.as<int32_t>(), n);
   SG_CHECK_HIP(hipGetLastError());
+  packRecords(s);
 
-  // 3. rows: count, scan, fill
-  const double ext2 = ext * ext;
+  // 3. rows
+  SearchArgs sa{};
+  sa.g = dg;
+  sa.ext2 = ext * ext;
+  // FP32 classification band.  Offsets from the brick origin are at most 3R, so
+  // |r| carries an absolute error below 7.2e-7 R; near the threshold of the
+  // smallest smoothing scale (|r| = ext/hmax_inv >= R*hinv_min/hinv_max) that is
+  // a relative error of 1.44e-6*ratio in |r|^2, plus ~5e-7 from the FP32
+  // products.  The band is four times that, and never narrower than 1e-5.
+  const double ratio = hb[8] / hb[6];
+  const double band = std::max(1.0e-5, 4.0 * (1.44e-6 * ratio + 5.0e-7));
+  sa.lo32 = float(sa.ext2 * (1.0 - band));
+  sa.hi32 = float(sa.ext2 * (1.0 + band));
+  sa.n = n;
+  sa.nInt = s.nInt;
+  if (s.ellCap <= 0) s.ellCap = D == 3 ? 160 : (D == 2 ? 48 : 16);
   s.rowCount.ensure(sizeof(int32_t) * size_t(n));
-  s.offsets.ensure(sizeof(int64_t) * size_t(n + 1));
-  hipLaunchKernelGGL((k_rows<D, false>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.xs.as<double>(), s.Hs.as<double>(),
-                     s.perm.as<int32_t>(), s.cellStart.as<int32_t>(), dg, ext2, n, s.nInt,
-                     s.rowCount.as<int32_t>(), (const int64_t*)nullptr, (int32_t*)nullptr);
-  exclusiveScan<int32_t, int64_t>(s.rowCount.as<int32_t>(), s.offsets.as<int64_t>(), n, s);
-  int64_t total = 0;
-  SG_CHECK_HIP(hipMemcpyAsync(&total, s.offsets.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  SG_CHECK_HIP(hipStreamSynchronize(st));
-  s.nEntries = total;
-  s.nbr.ensure(sizeof(int32_t) * size_t(std::max<int64_t>(total, 1)));
-  hipLaunchKernelGGL((k_rows<D, true>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.xs.as<double>(), s.Hs.as<double>(),
-                     s.perm.as<int32_t>(), s.cellStart.as<int32_t>(), dg, ext2, n, s.nInt,
-                     s.rowCount.as<int32_t>(), s.offsets.as<int64_t>(), s.nbr.as<int32_t>());
-  // 4. order rows by node index
-  if (n > 0)
-    hipLaunchKernelGGL(k_row_sort, dim3(n), dim3(64), 0, st, s.offsets.as<int64_t>(), s.perm.as<int32_t>(),
-                       s.nbr.as<int32_t>(), n);
+  s.flags.ensure(sizeof(int32_t) * 8 + sizeof(unsigned long long) * 2);
+  unsigned long long* dTotal = reinterpret_cast<unsigned long long*>(s.flags.as<int32_t>() + 4);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    sa.cap = s.ellCap;
+    s.nbr.ensure(ellBytes(n, s.ellCap));
+    SG_CHECK_HIP(hipMemsetAsync(s.flags.p, 0, s.flags.bytes, st));
+    if (stagedPath) {
+      hipLaunchKernelGGL(k_search_staged, dim3(int(bricks)), dim3(kBlock), 0, st, s.rec.as<double>(),
+                         s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>(),
+                         s.flags.as<int32_t>());
+    } else if (s.isotropic) {
+      hipLaunchKernelGGL((k_search_global<D, true>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.rec.as<double>(),
+                         s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
+    } else {
+      hipLaunchKernelGGL((k_search_global<D, false>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.rec.as<double>(),
+                         s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
+    }
+    hipLaunchKernelGGL(k_row_stats, dim3(int(std::min<int64_t>(1024, nblocks(n)))), dim3(kBlock), 0, st,
+                       s.rowCount.as<int32_t>(), n, dTotal, s.flags.as<int32_t>());
+    SG_CHECK_HIP(hipGetLastError());
+    struct { int32_t f[4]; unsigned long long total; } hf;
+    SG_CHECK_HIP(hipMemcpyAsync(&hf, s.flags.p, sizeof(hf), hipMemcpyDeviceToHost, st));
+    SG_CHECK_HIP(hipStreamSynchronize(st));
+    s.maxRow = hf.f[0];
+    s.stagedFallbacks = hf.f[1];
+    s.nEntries = int64_t(hf.total);
+    if (s.maxRow <= s.ellCap) break;
+    SG_REQUIRE(attempt == 0, SPHGPU_ERR_CAPACITY, "neighbour row capacity exceeded twice");
+    s.ellCap = ((s.maxRow + 15) / 16) * 16 + 16;
+  }
+  // 4. reference row order
+  if (s.rowOrder == SPHGPU_ROW_ORDER_REFERENCE && n > 0)
+    hipLaunchKernelGGL(k_row_sort, dim3(nblocks(n * 64)), dim3(kBlock), 0, st, s.rowCount.as<int32_t>(),
+                       s.nbr.as<int32_t>(), n, s.perm.as<int32_t>(), s.iperm.as<int32_t>());
   SG_CHECK_HIP(hipGetLastError());
   SG_CHECK_HIP(hipEventRecord(s.ev[1], st));
 }
@@ -431,19 +721,6 @@ void buildNeighborsD(Sim& s, double ext) {
 
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s) { exclusiveScan<int32_t, int64_t>(in, out, n, s); }
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s) { exclusiveScan<int32_t, int32_t>(in, out, n, s); }
-
-void regatherSorted(Sim& s) {
-  const int64_t n = s.nTot;
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.perm.as<int32_t>(), s.pos.as<double>(),
-                       s.H.as<double>(), s.xs.as<double>(), s.Hs.as<double>(), s.iperm.as<int32_t>(), n);
-  };
-  if (s.dim == 1) go(k_gather_xh<1>);
-  else if (s.dim == 2) go(k_gather_xh<2>);
-  else go(k_gather_xh<3>);
-  SG_CHECK_HIP(hipGetLastError());
-  s.xsStale = false;
-}
 
 void buildNeighbors(Sim& s, double ext) {
   SG_REQUIRE(s.nTot > 0, SPHGPU_ERR_STATE, "no nodes");
@@ -459,7 +736,6 @@ void buildNeighbors(Sim& s, double ext) {
   SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[0], s.ev[1]));
   s.timing.neighbors_ms = ms;
   s.neighborsValid = true;
-  s.xsStale = false;
 }
 
 }  // namespace sphgpu

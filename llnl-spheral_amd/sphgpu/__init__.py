@@ -37,6 +37,8 @@ DERIV_ALIASES = [
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+ROW_ORDER_REFERENCE = 0   # rows ascending in node index (reference summation order)
+ROW_ORDER_SEARCH = 1      # deterministic device search order (no per-row sort)
 
 ERRORS = {-1: "ARG", -2: "STATE", -3: "HIP", -4: "UNSUPPORTED", -5: "CAPACITY"}
 
@@ -94,6 +96,7 @@ def lib():
             "sphgpu_set_nodes": (C.c_int, [vp, C.c_int64, C.c_int64]),
             "sphgpu_upload": (C.c_int, [vp, C.c_int, _dp, C.c_int64]),
             "sphgpu_download": (C.c_int, [vp, C.c_int, _dp, C.c_int64]),
+            "sphgpu_set_row_order": (C.c_int, [vp, C.c_int]),
             "sphgpu_build_neighbors": (C.c_int, [vp, C.c_double]),
             "sphgpu_neighbor_count": (C.c_int, [vp, _i64p]),
             "sphgpu_download_neighbors": (C.c_int, [vp, _i64p, _i32p]),
@@ -217,6 +220,9 @@ class Sim:
         return a[:, 0] if c == 1 else a
 
     # passes -------------------------------------------------------------------
+    def set_row_order(self, order):
+        self._chk(self.L.sphgpu_set_row_order(self.h, order))
+
     def build_neighbors(self, kernel_extent=2.0):
         self._chk(self.L.sphgpu_build_neighbors(self.h, kernel_extent))
 

@@ -89,6 +89,23 @@ def random_state(dim, n, seed=7, jitter=0.2, nPerh=1.51, gamma=1.4, vel_scale=0.
                 omega=np.ones(N)), dict(n_internal=N, dx=dx, gamma=gamma, nPerh=nPerh)
 
 
+def anisotropic_H(dim, n, h0, seed=3, spread=0.3):
+    """SymTensors R diag(1/h) R^T with random rotations and +-spread anisotropy."""
+    rng = np.random.default_rng(seed)
+    ns = dim * (dim + 1) // 2
+    H = np.zeros((n, ns))
+    for i in range(n):
+        h = h0 * (1.0 + spread * rng.uniform(-1, 1, dim))
+        Q, _ = np.linalg.qr(rng.standard_normal((dim, dim)))
+        Mx = Q @ np.diag(1.0 / h) @ Q.T
+        k = 0
+        for a in range(dim):
+            for b in range(a, dim):
+                H[i, k] = 0.5 * (Mx[a, b] + Mx[b, a])
+                k += 1
+    return H
+
+
 def linear_velocity(dim, n, A, seed=14892042, jitter=0.2, nPerh=1.51):
     """v = A.x on a perturbed lattice (src/SPH/tests/testLinearVelocityGradient.py)."""
     pos, mass, H, dx = lattice(dim, n, jitter=jitter, seed=seed, nPerh=nPerh)

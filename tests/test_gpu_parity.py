@@ -3,7 +3,11 @@
 Bars: neighbour CSR byte-identical to the oracle's ConnectivityMap restatement;
 sum density and every evaluateDerivatives field of the FP64 path equal to the
 oracle's reference-order scatter loop value for value (np.array_equal), which
-is stricter than the 1e-12 relative bound the FP64 path is held to.
+is stricter than the 1e-12 relative bound the FP64 path is held to.  The one
+exception: in 3-D the weighted neighbour sum is a cube root taken with
+pow(x, 0.3333333333333333) (Dim<3>::rootnu, src/Utilities/FastMath.hh:148-156)
+and the device pow and the host libm pow may differ in the last bit; that
+field and the ideal H computed from it are held to 1e-14 relative.
 """
 import numpy as np
 import pytest
@@ -13,9 +17,10 @@ from sphgpu import problems
 pytestmark = pytest.mark.gpu
 
 
-def make_sim(sg, dim, st, n_int, npoints=1000):
+def make_sim(sg, dim, st, n_int, npoints=1000, order=0):
     N = st["pos"].shape[0]
     s = sg.Sim(dim)
+    s.set_row_order(order)
     s.set_table_bspline(npoints)
     s.set_nodes(n_int, N)
     s.upload_state(st)
@@ -37,17 +42,45 @@ def cases():
     N = meta["n_internal"]
     st.update(fCl=rng.uniform(0.5, 1.5, N), fCq=rng.uniform(0.5, 1.5, N), fshear=rng.uniform(0.1, 1, N))
     out.append(("multipliers2d", 2, st, N, dict(linearInExpansion=1)))
+    # anisotropic, rotated H: the general-H search and pair kernels
+    st, meta = problems.random_state(3, 8, seed=8)
+    st["H"] = problems.anisotropic_H(3, meta["n_internal"], 1.51 * meta["dx"])
+    out.append(("aniso3d", 3, st, meta["n_internal"] - 50, {}))
+    # one node with a 4x larger smoothing scale: coarse cells, staged-search
+    # fallback to global reads, and a row longer than the default capacity
+    st, meta = problems.converging_flow(14)
+    st["H"] = st["H"].copy()
+    st["H"][5] *= 0.25
+    out.append(("bighalo3d", 3, st, meta["n_internal"], {}))
     return out
 
 
 CASES = cases()
 
+POW_FIELDS = ("weightedNeighborSum", "Hideal")
 
+
+def same(field, dim, a, b, order=0):
+    """REFERENCE row order: equal value for value (pow-derived 3-D fields to
+    1e-14).  SEARCH row order: relative L-inf error against the field's largest
+    magnitude <= 1e-12 (the FP64 bar; only the summation order differs)."""
+    scale = max(np.abs(b).max(), 1e-300)
+    if order == 1:
+        return np.abs(a - b).max() <= 1e-12 * scale
+    if dim == 3 and field in POW_FIELDS:
+        return np.abs(a - b).max() <= 1e-14 * scale
+    return np.array_equal(a, b)
+
+
+ORDERS = [0, 1]
+
+
+@pytest.mark.parametrize("order", ORDERS, ids=["refo", "searcho"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_neighbors_bitexact(sphgpu, oracle, case):
+def test_neighbors_bitexact(sphgpu, oracle, case, order):
     name, dim, st, n_int, kw = case
-    print(f"[gpu] neighbors {name} dim={dim} N={st['pos'].shape[0]}", flush=True)
-    s = make_sim(sphgpu, dim, st, n_int)
+    print(f"[gpu] neighbors {name} dim={dim} N={st['pos'].shape[0]} order={order}", flush=True)
+    s = make_sim(sphgpu, dim, st, n_int, order=order)
     s.build_neighbors(2.0)
     off, nbr = s.neighbors()
     off_o, nbr_o = oracle.connectivity(dim, n_int, st["pos"], st["H"], 2.0)
@@ -55,28 +88,30 @@ def test_neighbors_bitexact(sphgpu, oracle, case):
     assert np.array_equal(nbr, nbr_o)
 
 
+@pytest.mark.parametrize("order", ORDERS, ids=["refo", "searcho"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_sum_density_exact(sphgpu, oracle, case):
+def test_sum_density_exact(sphgpu, oracle, case, order):
     name, dim, st, n_int, kw = case
-    print(f"[gpu] sum density {name}", flush=True)
-    s = make_sim(sphgpu, dim, st, n_int)
+    print(f"[gpu] sum density {name} order={order}", flush=True)
+    s = make_sim(sphgpu, dim, st, n_int, order=order)
     s.build_neighbors(2.0)
     s.sum_density()
     rho = s.download("mass density")[:n_int]
     T = oracle.Table(dim, 1000)
     off, nbr = oracle.connectivity(dim, n_int, st["pos"], st["H"], 2.0)
     ref = oracle.sum_density(T, n_int, st["pos"], st["mass"], st["H"], off, nbr)
-    assert np.array_equal(rho, ref), np.abs(rho - ref).max()
+    assert same("rho", dim, rho, ref, order), np.abs(rho - ref).max()
     # ghosts keep their uploaded density
     N = st["pos"].shape[0]
     assert np.array_equal(s.download("mass density")[n_int:], st["rho"][n_int:N])
 
 
+@pytest.mark.parametrize("order", ORDERS, ids=["refo", "searcho"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_derivatives_exact(sphgpu, oracle, case):
+def test_derivatives_exact(sphgpu, oracle, case, order):
     name, dim, st, n_int, kw = case
-    print(f"[gpu] derivatives {name}", flush=True)
-    s = make_sim(sphgpu, dim, st, n_int)
+    print(f"[gpu] derivatives {name} order={order}", flush=True)
+    s = make_sim(sphgpu, dim, st, n_int, order=order)
     s.build_neighbors(2.0)
     s.evaluate_derivatives(sphgpu.hydro_params(**kw))
     got = s.derivatives()
@@ -87,7 +122,7 @@ def test_derivatives_exact(sphgpu, oracle, case):
     for f in oracle.OUTPUT_FIELDS:
         a = got[f].reshape(ref[f].shape)
         b = ref[f]
-        if not np.array_equal(a, b):
+        if not same(f, dim, a, b, order):
             scale = max(np.abs(b).max(), 1e-300)
             bad.append((f, float(np.abs(a - b).max() / scale)))
     assert not bad, bad
@@ -121,7 +156,7 @@ def test_config1_converging_32cube(sphgpu, oracle):
     ref = oracle.eval_derivs(T, oracle.make_params(), N, st, off_o, nbr_o, gather=True)
     for f in oracle.OUTPUT_FIELDS:
         a = got[f].reshape(ref[f].shape)
-        assert np.array_equal(a, ref[f]), f
+        assert same(f, 3, a, ref[f]), f
     s.sum_density()
     rho = s.download("mass density")
     ref_rho = oracle.sum_density(T, N, st["pos"], st["mass"], st["H"], off_o, nbr_o, gather=True)
