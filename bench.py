@@ -92,6 +92,8 @@ def main():
                     help="search: device search order; reference: rows by node index (bitwise reference sums)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-mib", type=int, default=256,
+                    help="size of the plain copy kernel run after the timed steps (counter calibration); 0 = off")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -168,6 +170,12 @@ def main():
             "note": "FP64 VALU-bound kernel; HBM fraction reported from algorithmic bytes",
         },
     }
+    if args.probe_mib > 0:
+        nb = args.probe_mib << 20
+        ms = sim.copy_probe(nb, 5)
+        line["copy_probe"] = {"bytes_read": nb, "bytes_written": nb, "ms": ms,
+                              "GBps": 2 * nb / (ms * 1e-3) / 1e9,
+                              "kernel": "k_copy_probe (double4 streaming copy)"}
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args.cpu_budget)

@@ -188,6 +188,12 @@ int sphgpu_last_timings(sphgpu_sim* sim, double* ms4);
 /* Block until all work queued on the sim's stream has finished. */
 int sphgpu_synchronize(sphgpu_sim* sim);
 
+/* Measurement helper with no counterpart in the reference: copy `bytes` bytes
+ * between two device buffers with a plain streaming kernel `repeats` times and
+ * return the average kernel time (ms).  Used to calibrate HBM traffic counters
+ * and to state the achievable copy bandwidth beside the roofline figures. */
+int sphgpu_copy_probe(sphgpu_sim* sim, int64_t bytes, int repeats, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -330,4 +330,11 @@ int sphgpu_synchronize(sphgpu_sim* s) {
   return guarded(s, [&] { SG_CHECK_HIP(hipStreamSynchronize(s->stream)); });
 }
 
+int sphgpu_copy_probe(sphgpu_sim* s, int64_t bytes, int repeats, double* ms) {
+  return guarded(s, [&] {
+    SG_REQUIRE(ms, SPHGPU_ERR_ARG, "null output");
+    *ms = copyProbe(*s, bytes, repeats);
+  });
+}
+
 }  // extern "C"

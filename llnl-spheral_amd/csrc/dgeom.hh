@@ -137,6 +137,18 @@ template <int D> __device__ __forceinline__ double rootnu(double x) {
 }
 template <int D> SG_HD constexpr double pownu2() { return D == 1 ? 2.0 : (D == 2 ? 4.0 : 8.0); }
 
+// XCD-aware block order.  The dispatcher deals workgroups round-robin to the 8
+// XCDs of an MI355X, each with its own L2; remapping so that XCD x runs the
+// logical blocks [x*per, (x+1)*per) keeps spatially adjacent blocks (and their
+// shared neighbours) in one L2.  Launch kXcds*per blocks; logical ids >= nb
+// have no work.
+constexpr int kXcds = 8;
+__host__ __forceinline__ int64_t xcdGrid(int64_t nb) { return kXcds * ((nb + kXcds - 1) / kXcds); }
+__device__ __forceinline__ int64_t xcdBlock(int64_t nb) {
+  const int64_t per = (nb + kXcds - 1) / kXcds;
+  return int64_t(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 // src/Utilities/SpheralFunctions.hh:20-23; src/Utilities/safeInv.hh:16-27.
 SG_HD bool fuzzyEqual(double l, double r, double fuzz = 1.0e-15) {
   return dabs(l - r) / dmax(1.0, dabs(l) + dabs(r)) < fuzz;

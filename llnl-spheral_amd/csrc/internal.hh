@@ -118,6 +118,8 @@ void buildNeighbors(Sim& s, double kernelExtent);
 void packRecords(Sim& s);
 void sumDensity(Sim& s);
 void evalDerivs(Sim& s, const sphgpu_hydro_params& p);
+// probe.hip
+double copyProbe(Sim& s, int64_t bytes, int repeats);
 // scans (neighbors.hip)
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);

@@ -106,6 +106,7 @@ def lib():
             "sphgpu_pair_count": (C.c_int, [vp, _i64p]),
             "sphgpu_last_timings": (C.c_int, [vp, _dp]),
             "sphgpu_synchronize": (C.c_int, [vp]),
+            "sphgpu_copy_probe": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -267,3 +268,9 @@ class Sim:
 
     def synchronize(self):
         self._chk(self.L.sphgpu_synchronize(self.h))
+
+    def copy_probe(self, nbytes, repeats=5):
+        """Average time (ms) of a plain device-to-device streaming copy kernel."""
+        ms = C.c_double()
+        self._chk(self.L.sphgpu_copy_probe(self.h, nbytes, repeats, C.byref(ms)))
+        return ms.value
