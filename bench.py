@@ -29,6 +29,18 @@ FP64_PEAK_TFLOPS = 78.6
 IN_BYTES_PER_NODE = 136      # x(3) v(3) m rho P c omega H(6), FP64
 OUT_BYTES_PER_NODE = 576     # the 72 doubles of the derivative fields
 NBR_BYTES_PER_ENTRY = 4      # one int32 neighbour index per directed entry
+# HBM bytes per pair-kernel dispatch, from calibrated FETCH_SIZE / WRITE_SIZE runs
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_derivs_fp64_converging_flow_128_traffic.json")
+
+
+def measured_traffic(n, precision, row_order):
+    """HBM bytes per launch of the pair kernel as the counter runs recorded
+    them, for the workload they were taken on; None for any other workload."""
+    if (n, precision, row_order) != (128, "fp64", "search") or not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    return t["traffic_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
 def cpu_info():
@@ -139,6 +151,7 @@ def main():
     derivs_ms = stage[2]
     alg_bytes = N * (IN_BYTES_PER_NODE + OUT_BYTES_PER_NODE) + NBR_BYTES_PER_ENTRY * entries
     achieved = alg_bytes / (derivs_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args.n, args.precision, args.row_order)
     line = {
         "metric": METRIC,
         "value": pairs * args.steps / wall,
@@ -165,7 +178,8 @@ def main():
         "roofline": {
             "kernel": "k_derivs_fp64 (evaluateDerivatives pair loop + epilogue)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes": alg_bytes,
             "note": "FP64 VALU-bound kernel; HBM fraction reported from algorithmic bytes",
         },

@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
   using R = RL<ISO>;
   extern __shared__ double lds[];
   const double* tab = stageTable<LDS>(lds, t);
-  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int64_t k = xcdBlock((n + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   if (k >= n) return;
   double* ri_ = rec + k * R::STRIDE;
   const double oi = ri_[R::ORIG];
@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict
                                                         unsigned long long* __restrict__ ncalc) {
   extern __shared__ double lds[];
   const double* tab = stageTable<LDS>(lds, t);
-  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int64_t k = xcdBlock((n + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   unsigned pairs = 0;
   if (k < n && rec[k * RL<ISO>::STRIDE + RL<ISO>::ORIG] < double(nInt))
     pairs = derivsNodeFP64<D, ISO, TENSILE>(k, rec, rowCount, nbr, n, tab, t, p, o);
@@ -517,7 +517,7 @@ void sumDensityD(Sim& s) {
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
+    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(n))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, s.rho.as<double>());
   };
   if (s.isotropic) { if (lds) go(k_sum_density<D, true, true>); else go(k_sum_density<D, true, false>); }
@@ -557,7 +557,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   for (int f = 0; f < SPHGPU_NUM_DERIV_FIELDS; ++f) o.off[f] = derivOffsetDoubles(s.dim, f);
   const bool tensile = hp.epsTensile != 0.0;
   auto launch = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
+    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(n))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, p, o, dncalc);
   };
   if (lds) {

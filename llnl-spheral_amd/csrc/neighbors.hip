@@ -237,7 +237,8 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
   __shared__ int32_t sSlot[kCandCap];
   __shared__ int32_t sc[kBlock];
   const DGrid& g = a.g;
-  const int b = blockIdx.x;
+  const int b = int(xcdBlock(int64_t(g.nb[0]) * g.nb[1] * g.nb[2]));
+  if (b >= g.nb[0] * g.nb[1] * g.nb[2]) return;   // uniform over the block
   const int bx = b % g.nb[0], by = (b / g.nb[0]) % g.nb[1], bz = b / (g.nb[0] * g.nb[1]);
   const int64_t h0 = cellStart[int64_t(b) * 64], h1 = cellStart[int64_t(b) * 64 + 64];
   if (h0 == h1) return;                           // uniform over the block
@@ -351,7 +352,7 @@ template <int D, bool ISO>
 __global__ void k_search_global(const double* __restrict__ rec, const int32_t* __restrict__ cellStart, SearchArgs a,
                                 int32_t* __restrict__ rowCount, int4* __restrict__ nbr) {
   using R = RL<ISO>;
-  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int64_t k = xcdBlock((a.n + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   if (k >= a.n) return;
   const double* ri_ = rec + k * R::STRIDE;
   if (int64_t(ri_[R::ORIG]) >= a.nInt) { rowCount[k] = 0; return; }
@@ -686,14 +687,14 @@ void buildNeighborsD(Sim& s, double ext) {
     s.nbr.ensure(ellBytes(n, s.ellCap));
     SG_CHECK_HIP(hipMemsetAsync(s.flags.p, 0, s.flags.bytes, st));
     if (stagedPath) {
-      hipLaunchKernelGGL(k_search_staged, dim3(int(bricks)), dim3(kBlock), 0, st, s.rec.as<double>(),
+      hipLaunchKernelGGL(k_search_staged, dim3(int(xcdGrid(bricks))), dim3(kBlock), 0, st, s.rec.as<double>(),
                          s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>(),
                          s.flags.as<int32_t>());
     } else if (s.isotropic) {
-      hipLaunchKernelGGL((k_search_global<D, true>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.rec.as<double>(),
+      hipLaunchKernelGGL((k_search_global<D, true>), dim3(int(xcdGrid(nblocks(n)))), dim3(kBlock), 0, st, s.rec.as<double>(),
                          s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
     } else {
-      hipLaunchKernelGGL((k_search_global<D, false>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.rec.as<double>(),
+      hipLaunchKernelGGL((k_search_global<D, false>), dim3(int(xcdGrid(nblocks(n)))), dim3(kBlock), 0, st, s.rec.as<double>(),
                          s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
     }
     hipLaunchKernelGGL(k_row_stats, dim3(int(std::min<int64_t>(1024, nblocks(n)))), dim3(kBlock), 0, st,
