@@ -122,6 +122,22 @@ def test_uniform_lattice_interior(oracle):
     eta = np.sqrt((g ** 2).sum(1)) / (1.51 * dx)
     ref = sum(mass[0] * T.W(e, Hdet) for e in eta)
     assert np.allclose(out["rhoSum"][interior], ref, rtol=1e-13, atol=0)
+    # weighted neighbour sum: (sum_j |W'(eta_j)| Hdet / Hdet)^(1/3) over the lattice shell
+    nz = eta > 0
+    gw = np.array([abs(T.gradW(e, Hdet)) for e in eta[nz]])
+    wns = (gw.sum() / Hdet) ** (1 / 3)
+    assert np.allclose(out["weightedNeighborSum"][interior], wns, rtol=1e-13, atol=0)
+    # second moment: sum_j (r r)/r^6 |gradW_j|^2 / Hdet^2 (diagonal, equal entries by symmetry)
+    r2 = (g[nz] ** 2).sum(1)
+    m2 = ((g[nz, 0] ** 2) / r2 ** 3 * (gw / (1.51 * dx)) ** 2).sum() / Hdet ** 2
+    m2nd = out["massSecondMoment"].reshape(N, 6)[interior]
+    assert np.allclose(m2nd[:, [0, 3, 5]], m2, rtol=1e-12, atol=0)
+    assert np.abs(m2nd[:, [1, 2, 4]]).max() < 1e-12 * m2
+    # on the lattice the table's own nPerh is recovered, so the ideal H is H
+    # (to the nperh table's interpolation) and DHDt vanishes with tr DvDx
+    hid = out["Hideal"].reshape(N, 6)[interior]
+    assert np.allclose(hid[:, [0, 3, 5]], 1 / (1.51 * dx), rtol=2e-3, atol=0)
+    assert np.all(out["DHDt"].reshape(N, 6)[interior] == 0)
 
 
 @pytest.mark.parametrize("dim", [1, 2, 3])
