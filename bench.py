@@ -1,12 +1,14 @@
 """Benchmark of the SPH derivative pass on one MI355X.
 
-One step = neighbour build (ConnectivityMap) + computeSPHSumMassDensity +
+One step = Integrator::setGhostNodes (reflecting-box ghosts, neighbour build,
+ghost culling) + computeSPHSumMassDensity (with the ghost density refresh) +
 SPHHydroBase::evaluateDerivatives over the resident node state, i.e. the
 pass the metric "SPH pair-interactions/s (neighbors+derivs)" times.  Workload:
-the 3-D converging-flow generator at 128^3 nodes (cold gas, v = -r_hat, +-1%
-lattice jitter, cubic B-spline TableKernel, nPerh 1.51, Monaghan-Gingold Q with
-Cl = Cq = 1).  pair-interactions/s = unordered pairs per pass (the reference's
-ncalc) x steps / wall time.
+the 3-D converging-flow generator at 128^3 nodes in the unit box with 6
+reflecting planes (cold gas, v = -r_hat, +-1% lattice jitter, cubic B-spline
+TableKernel, nPerh 1.51, Monaghan-Gingold Q with Cl = Cq = 1).
+pair-interactions/s = unordered pairs per pass (the reference's ncalc,
+internal-internal and internal-ghost) x steps / wall time.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--precision fp64|mixed]
 """
@@ -33,13 +35,15 @@ NBR_BYTES_PER_ENTRY = 4      # one int32 neighbour index per directed entry
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_derivs_fp64_converging_flow_128_traffic.json")
 
 
-def measured_traffic(n, precision, row_order):
+def measured_traffic(n, precision, row_order, boundaries):
     """HBM bytes per launch of the pair kernel as the counter runs recorded
     them, for the workload they were taken on; None for any other workload."""
-    if (n, precision, row_order) != (128, "fp64", "search") or not os.path.exists(TRAFFIC_FILE):
+    if not os.path.exists(TRAFFIC_FILE):
         return None, None
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
+    if t.get("key") != [n, precision, row_order, boundaries]:
+        return None, None
     return t["traffic_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
@@ -55,9 +59,9 @@ def cpu_info():
     return model, os.cpu_count()
 
 
-def cpu_baseline(budget_s=10.0, n=48):
+def cpu_baseline(budget_s=10.0, n=48, boundaries="reflecting"):
     """The oracle's OpenMP gather-form restatement, timed on this host on a
-    bounded sample (n^3 converging flow)."""
+    bounded sample (n^3 converging flow, same boundaries as the GPU run)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # test infrastructure, used only as the CPU baseline here
     from sphgpu import problems
@@ -68,13 +72,18 @@ def cpu_baseline(budget_s=10.0, n=48):
     threads = oracle.num_threads(fast=True)
     passes, pairs, t_total = 0, 0, 0.0
     t_nb = t_rho = t_dv = 0.0
+    bounds = oracle.box_boundaries(3) if boundaries == "reflecting" else []
     while passes == 0 or t_total < budget_s:
         t0 = time.perf_counter()
-        off, nbr = oracle.connectivity(3, N, st["pos"], st["H"], 2.0, fast=True)
+        if bounds:
+            full, off, nbr, info = oracle.ghost_pipeline(3, st, bounds, fast=True)
+        else:
+            full = st
+            off, nbr = oracle.connectivity(3, N, st["pos"], st["H"], 2.0, fast=True)
         t1 = time.perf_counter()
-        oracle.sum_density(T, N, st["pos"], st["mass"], st["H"], off, nbr, gather=True)
+        oracle.sum_density(T, N, full["pos"], full["mass"], full["H"], off, nbr, gather=True)
         t2 = time.perf_counter()
-        oracle.eval_derivs(T, p, N, st, off, nbr, gather=True)
+        oracle.eval_derivs(T, p, N, full, off, nbr, gather=True)
         t3 = time.perf_counter()
         t_nb += t1 - t0
         t_rho += t2 - t1
@@ -85,8 +94,9 @@ def cpu_baseline(budget_s=10.0, n=48):
     model, ncpu = cpu_info()
     return {
         "value": pairs * passes / t_total, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
-        "sample": f"converging flow {n}^3 ({N} nodes, {pairs} pairs), {passes} passes of neighbour build + "
-                  f"sum density + evaluateDerivatives (OpenMP gather form), {t_total:.1f} s",
+        "sample": f"converging flow {n}^3 ({N} nodes, {pairs} pairs, boundaries: {boundaries}), {passes} passes "
+                  f"of ghosts + neighbour build + sum density + evaluateDerivatives (OpenMP gather form; the "
+                  f"ghost generation is the sequential restatement), {t_total:.1f} s",
         "stage_ms": {"neighbors": 1e3 * t_nb / passes, "sum_density": 1e3 * t_rho / passes,
                      "derivatives": 1e3 * t_dv / passes},
         "cpu_model": model, "host_cpus": ncpu,
@@ -102,6 +112,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--row-order", choices=["search", "reference"], default="search",
                     help="search: device search order; reference: rows by node index (bitwise reference sums)")
+    ap.add_argument("--boundaries", choices=["reflecting", "none"], default="reflecting",
+                    help="reflecting: 6 ReflectingBoundary planes on the unit box, ghosts made on the device "
+                         "every step (the benchmark configuration); none: free surface")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-mib", type=int, default=256,
@@ -123,24 +136,31 @@ def main():
     sim.set_table_bspline(1000)
     sim.set_nodes(N, N)
     sim.upload_state(st)
+    ghosts = args.boundaries == "reflecting"
+    if ghosts:
+        sim.set_boundaries(sphgpu.box_boundaries(3))
     prec = sphgpu.PRECISION_FP64 if args.precision == "fp64" else sphgpu.PRECISION_MIXED
     params = sphgpu.hydro_params(precision=prec)
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s N={N}", file=sys.stderr, flush=True)
 
     def step():
-        sim.build_neighbors(2.0)
+        if ghosts:
+            sim.set_ghost_nodes(2.0, cull=True)   # ghosts + neighbour build + culling
+        else:
+            sim.build_neighbors(2.0)
         sim.sum_density()
         sim.evaluate_derivatives(params)
 
     for _ in range(args.warmup):
         step()
     sim.synchronize()
-    stage = np.zeros(4)
+    stage = np.zeros(6)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
         t = sim.timings()
-        stage += [t["neighbors_ms"], t["density_ms"], t["derivs_ms"], t["pack_ms"]]
+        stage += [t["neighbors_ms"], t["density_ms"], t["derivs_ms"], t["pack_ms"],
+                  t["ghosts_ms"] if ghosts else 0.0, t["cull_ms"] if ghosts else 0.0]
     sim.synchronize()
     wall = time.perf_counter() - t0
     stage /= args.steps
@@ -149,9 +169,10 @@ def main():
     ms_step = 1e3 * wall / args.steps
 
     derivs_ms = stage[2]
-    alg_bytes = N * (IN_BYTES_PER_NODE + OUT_BYTES_PER_NODE) + NBR_BYTES_PER_ENTRY * entries
+    n_tot = sim.node_counts()[1]
+    alg_bytes = N * OUT_BYTES_PER_NODE + n_tot * IN_BYTES_PER_NODE + NBR_BYTES_PER_ENTRY * entries
     achieved = alg_bytes / (derivs_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(args.n, args.precision, args.row_order)
+    traffic, traffic_src = measured_traffic(args.n, args.precision, args.row_order, args.boundaries)
     line = {
         "metric": METRIC,
         "value": pairs * args.steps / wall,
@@ -166,14 +187,16 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"converging flow {args.n}^3 3D (cold gas, v=-r_hat, 1% jitter), no boundary ghosts",
-            "nodes": N, "pairs": pairs, "directed_entries": entries,
+            "workload": f"converging flow {args.n}^3 3D (cold gas, v=-r_hat, 1% jitter), "
+                        + ("reflecting box: 6 ReflectingBoundary planes, ghosts rebuilt on the device every step"
+                           if ghosts else "free surface (no boundaries)"),
+            "nodes": N, "ghost_nodes": n_tot - N, "pairs": pairs, "directed_entries": entries,
             "kernel": "TableKernel(BSpline,1000)", "nPerh": 1.51, "Q": "MonaghanGingold Cl=Cq=1",
             "options": "XSPH, correctVelocityGradient, no grad-h, no compatible energy",
             "precision": args.precision,
             "row_order": args.row_order,
-            "stage_ms": {"neighbors": stage[0], "sum_density": stage[1], "derivatives": stage[2],
-                         "pack": stage[3]},
+            "stage_ms": {"ghosts": stage[4], "neighbors": stage[0], "cull": stage[5], "sum_density": stage[1],
+                         "derivatives": stage[2], "pack": stage[3]},
         },
         "roofline": {
             "kernel": "k_derivs_fp64 (evaluateDerivatives pair loop + epilogue)",
@@ -192,7 +215,7 @@ def main():
                               "kernel": "k_copy_probe (double4 streaming copy)"}
     if not args.no_cpu:
         try:
-            line["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+            line["cpu_baseline"] = cpu_baseline(args.cpu_budget, boundaries=args.boundaries)
         except Exception as e:  # the baseline is reported, never required
             line["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(line), flush=True)

@@ -137,10 +137,49 @@ int sphgpu_table_values(sphgpu_sim* sim, double eta, double Hdet, double* W, dou
 /* Size the node arrays (NodeList::numInternalNodes / numGhostNodes,
  * src/NodeList/NodeListInline.hh:28-42).  Invalidates neighbours. */
 int sphgpu_set_nodes(sphgpu_sim* sim, int64_t n_internal, int64_t n_total);
-/* Copy one state field (n_total rows) host -> device. */
+/* Copy one state field host -> device: n_total rows, or, when the ghosts were
+ * made by sphgpu_set_ghost_nodes, n_internal rows (ghost rows then keep their
+ * values until sphgpu_apply_ghost_boundaries). */
 int sphgpu_upload(sphgpu_sim* sim, int field, const double* host, int64_t n_rows);
 /* Copy one state field (n_total rows) device -> host; synchronises. */
 int sphgpu_download(sphgpu_sim* sim, int field, double* host, int64_t n_rows);
+
+/* Planar ghost boundaries generated on the device.
+ * SPHGPU_BOUNDARY_REFLECTING: ReflectingBoundary(plane1)
+ *   (src/Boundary/ReflectingBoundary.cc:40-175): ghosts mirrored through the
+ *   plane; vectors and tensors mapped by R = I - 2 n n
+ *   (src/Utilities/planarReflectingOperator.hh), scalars copied.
+ * SPHGPU_BOUNDARY_PERIODIC: PeriodicBoundary(plane1, plane2)
+ *   (src/Boundary/PeriodicBoundary.cc:66-135): two planar halves, plane1 ->
+ *   plane2 then plane2 -> plane1, values copied.
+ * Normals point into the domain and are normalised as GeomPlane does
+ * (src/Geometry/GeomPlane.cc:41-46).  Components beyond dim are ignored. */
+enum { SPHGPU_BOUNDARY_REFLECTING = 0, SPHGPU_BOUNDARY_PERIODIC = 1 };
+typedef struct sphgpu_plane { double point[3]; double normal[3]; } sphgpu_plane;
+typedef struct sphgpu_boundary { int kind; sphgpu_plane plane1; sphgpu_plane plane2; } sphgpu_boundary;
+
+/* Register the boundaries in the order the physics package holds them
+ * (Physics::boundaryBegin/End, src/Physics/Physics.hh:98-110); n = 0 clears. */
+int sphgpu_set_boundaries(sphgpu_sim* sim, int n, const sphgpu_boundary* boundaries);
+/* Integrator::setGhostNodes (src/Integrator/Integrator.cc:443-595): drop old
+ * ghosts, let each boundary in order select its controls among all current
+ * nodes (PlanarBoundary::setGhostNodes, src/Boundary/PlanarBoundary.cc:112-182)
+ * and append their ghosts, fill every ghost state field
+ * (applyGhostBoundaries), build the neighbours of the internal nodes
+ * (as sphgpu_build_neighbors), and, if `cull`, remove the ghosts that no
+ * internal node sees and that control no other ghost, keeping the order of the
+ * rest (src/Boundary/Boundary.cc:182-233).  The internal rows of every state
+ * field must be uploaded first (n_total = n_internal). */
+int sphgpu_set_ghost_nodes(sphgpu_sim* sim, double kernel_extent, int cull);
+/* Integrator::applyGhostBoundaries (src/Integrator/Integrator.cc:600-650):
+ * refresh positions, H and every state field of the ghosts from their control
+ * nodes, boundary by boundary, with the current ghost set. */
+int sphgpu_apply_ghost_boundaries(sphgpu_sim* sim);
+/* Current node counts (NodeList::numInternalNodes / numNodes). */
+int sphgpu_node_counts(sphgpu_sim* sim, int64_t* n_internal, int64_t* n_total);
+/* Control node (caller index) and planar sub-boundary of every ghost
+ * (n_total - n_internal entries each; Boundary::controlNodes, ghostNodes). */
+int sphgpu_download_ghost_controls(sphgpu_sim* sim, int32_t* control, int32_t* sub_boundary);
 
 /* Order of each node's neighbour row inside the device passes.
  *  REFERENCE: ascending node index, the order ConnectivityMap keeps
@@ -183,8 +222,10 @@ int sphgpu_download_derivative(sphgpu_sim* sim, int field, double* host, int64_t
 int sphgpu_pair_count(sphgpu_sim* sim, int64_t* n_pairs);
 
 /* Device times in milliseconds of the last call of each stage, measured with
- * HIP events on the sim's stream: [neighbours, sum density, derivatives, pack]. */
-int sphgpu_last_timings(sphgpu_sim* sim, double* ms4);
+ * HIP events on the sim's stream: [neighbours, sum density, derivatives, pack,
+ * ghost generation, ghost culling]. */
+enum { SPHGPU_NUM_TIMINGS = 6 };
+int sphgpu_last_timings(sphgpu_sim* sim, double* ms);
 /* Block until all work queued on the sim's stream has finished. */
 int sphgpu_synchronize(sphgpu_sim* sim);
 

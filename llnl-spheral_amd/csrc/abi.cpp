@@ -157,6 +157,7 @@ int sphgpu_set_nodes(sphgpu_sim* s, int64_t nInt, int64_t nTot) {
     SG_REQUIRE(nInt >= 0 && nTot >= nInt && nTot < (int64_t(1) << 31) - 1, SPHGPU_ERR_ARG, "bad node counts");
     s->nInt = nInt;
     s->nTot = nTot;
+    s->deviceGhosts = false;
     s->neighborsValid = false;
     s->derivsValid = false;
     s->hasMultipliers = false;
@@ -175,14 +176,15 @@ int sphgpu_set_nodes(sphgpu_sim* s, int64_t nInt, int64_t nTot) {
 
 int sphgpu_upload(sphgpu_sim* s, int field, const double* host, int64_t nRows) {
   return guarded(s, [&] {
-    SG_REQUIRE(host && nRows == s->nTot, SPHGPU_ERR_ARG, "upload needs n_total rows");
+    SG_REQUIRE(host && (nRows == s->nTot || (s->deviceGhosts && nRows == s->nInt)), SPHGPU_ERR_ARG,
+               "upload needs n_total rows (or n_internal rows once the device made the ghosts)");
     SG_REQUIRE(field >= 0 && field < SPHGPU_NUM_STATE_FIELDS, SPHGPU_ERR_ARG, "unknown state field id");
     const size_t bytes = sizeof(double) * size_t(nRows) * stateComponents(s->dim, field);
     DBuf& b = stateBuf(*s, field);
     if (field == SPHGPU_F_CL_MULTIPLIER || field == SPHGPU_F_CQ_MULTIPLIER || field == SPHGPU_F_SHEAR_CORRECTION) {
       // first multiplier upload materialises all three with 1.0
       if (!s->hasMultipliers) {
-        std::vector<double> ones(static_cast<size_t>(std::max<int64_t>(s->nTot, 1)), 1.0);
+        std::vector<double> ones(std::max<size_t>(s->pos.bytes / (sizeof(double) * size_t(s->dim)), 1), 1.0);
         for (DBuf* m : {&s->fCl, &s->fCq, &s->fshear}) {
           m->ensure(sizeof(double) * ones.size());
           SG_CHECK_HIP(hipMemcpy(m->p, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
@@ -217,7 +219,8 @@ int sphgpu_upload(sphgpu_sim* s, int field, const double* host, int64_t nRows) {
 
 int sphgpu_download(sphgpu_sim* s, int field, double* host, int64_t nRows) {
   return guarded(s, [&] {
-    SG_REQUIRE(host && nRows == s->nTot, SPHGPU_ERR_ARG, "download needs n_total rows");
+    SG_REQUIRE(host && (nRows == s->nTot || nRows == s->nInt), SPHGPU_ERR_ARG,
+               "download needs n_total or n_internal rows");
     SG_REQUIRE(field >= 0 && field < SPHGPU_NUM_STATE_FIELDS, SPHGPU_ERR_ARG, "unknown state field id");
     const size_t bytes = sizeof(double) * size_t(nRows) * stateComponents(s->dim, field);
     DBuf& b = stateBuf(*s, field);
@@ -233,6 +236,44 @@ int sphgpu_download(sphgpu_sim* s, int field, double* host, int64_t nRows) {
 
 int sphgpu_build_neighbors(sphgpu_sim* s, double ext) {
   return guarded(s, [&] { buildNeighbors(*s, ext); });
+}
+
+int sphgpu_set_boundaries(sphgpu_sim* s, int n, const sphgpu_boundary* b) {
+  return guarded(s, [&] {
+    setBoundaries(*s, n, b);
+    if (s->deviceGhosts) { s->nTot = s->nInt; s->deviceGhosts = false; }
+    s->neighborsValid = false;
+    s->recValid = false;
+    s->derivsValid = false;
+  });
+}
+
+int sphgpu_set_ghost_nodes(sphgpu_sim* s, double ext, int cull) {
+  return guarded(s, [&] { setGhostNodes(*s, ext, cull != 0); });
+}
+
+int sphgpu_apply_ghost_boundaries(sphgpu_sim* s) {
+  return guarded(s, [&] { applyGhostBoundaries(*s); });
+}
+
+int sphgpu_node_counts(sphgpu_sim* s, int64_t* nInt, int64_t* nTot) {
+  return guarded(s, [&] {
+    SG_REQUIRE(nInt && nTot, SPHGPU_ERR_ARG, "null output");
+    *nInt = s->nInt;
+    *nTot = s->nTot;
+  });
+}
+
+int sphgpu_download_ghost_controls(sphgpu_sim* s, int32_t* control, int32_t* sub) {
+  return guarded(s, [&] {
+    SG_REQUIRE(control && sub, SPHGPU_ERR_ARG, "null output");
+    const int64_t ng = s->deviceGhosts ? s->nTot - s->nInt : 0;
+    if (ng == 0) return;
+    SG_CHECK_HIP(hipMemcpyAsync(control, s->ghostControl.p, sizeof(int32_t) * size_t(ng), hipMemcpyDeviceToHost,
+                                s->stream));
+    SG_CHECK_HIP(hipMemcpyAsync(sub, s->ghostSub.p, sizeof(int32_t) * size_t(ng), hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+  });
 }
 
 int sphgpu_set_row_order(sphgpu_sim* s, int order) {
@@ -258,13 +299,14 @@ int sphgpu_download_neighbors(sphgpu_sim* s, int64_t* offsets, int32_t* nbr) {
     SG_REQUIRE(s->neighborsValid, SPHGPU_ERR_STATE, "no current neighbour build");
     // Rows are held as a transposed ELL over sorted slots; the ConnectivityMap
     // contract is rows of caller indices in ascending order.
-    const int64_t n = s->nTot;
+    const int64_t n = s->nSlots;
     const size_t cap = size_t(s->ellCap);
-    std::vector<int32_t> cnt(static_cast<size_t>(n)), perm(static_cast<size_t>(n)), iperm(static_cast<size_t>(n));
+    std::vector<int32_t> cnt(static_cast<size_t>(n)), perm(static_cast<size_t>(n));
+    std::vector<int32_t> iperm(static_cast<size_t>(s->nTot));
     std::vector<int32_t> ell(static_cast<size_t>(n) * cap);
     SG_CHECK_HIP(hipMemcpy(cnt.data(), s->rowCount.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
     SG_CHECK_HIP(hipMemcpy(perm.data(), s->perm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
-    SG_CHECK_HIP(hipMemcpy(iperm.data(), s->iperm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
+    SG_CHECK_HIP(hipMemcpy(iperm.data(), s->iperm.p, sizeof(int32_t) * iperm.size(), hipMemcpyDeviceToHost));
     if (!ell.empty()) SG_CHECK_HIP(hipMemcpy(ell.data(), s->nbr.p, sizeof(int32_t) * ell.size(), hipMemcpyDeviceToHost));
     int64_t w = 0;
     for (int64_t i = 0; i < s->nInt; ++i) {
@@ -323,6 +365,7 @@ int sphgpu_last_timings(sphgpu_sim* s, double* ms) {
   return guarded(s, [&] {
     SG_REQUIRE(ms, SPHGPU_ERR_ARG, "null output");
     ms[0] = s->timing.neighbors_ms; ms[1] = s->timing.density_ms; ms[2] = s->timing.derivs_ms; ms[3] = s->timing.pack_ms;
+    ms[4] = s->timing.ghosts_ms; ms[5] = s->timing.cull_ms;
   });
 }
 

@@ -1,0 +1,599 @@
+// Planar ghost boundaries on the device.
+//
+// The reference (src/Integrator/Integrator.cc:443-595) rebuilds its ghosts once
+// per step: every boundary, in order, looks at all nodes that exist when it
+// runs (internal nodes and the ghosts of the boundaries before it), selects
+// its control nodes and appends one ghost per control
+// (src/Boundary/PlanarBoundary.cc:112-182, 329-400):
+//   hmax     = max over nodes i with min(|d_exit(x_i)|, |d_enter(x_i)|) < ext*h_i
+//              of h_i = 1/lambda_min(H_i)
+//   controls = nodes with 0 <= d_exit(x_i)/hmax <= ext, in index order
+//   ghost    = mapPositionThroughPlanes(x_c, exit, enter); H, v by R = I - 2nn
+//              (reflecting) or copied (periodic); scalars copied.
+// After the connectivity is built, ghosts that no internal node sees and that
+// control no other ghost are culled, the rest keep their order
+// (src/Integrator/Integrator.cc:491-570, src/Boundary/Boundary.cc:182-233).
+//
+// Device form: per sub-boundary one pass over the current nodes computes hmax
+// (wavefront max + one 64-bit atomic max per wavefront on the bit pattern of a
+// non-negative double), a second flags the controls, an exclusive scan places
+// the ghosts, and a third writes them after the current rows.  Ghost fields
+// are filled boundary by boundary (a ghost's control always precedes it).
+// Culling: the row search marks every ghost slot it puts in a row; a scan of
+// the keep flags in caller order gives the old -> new index map; the state is
+// compacted and the slot permutation renumbered.  Slots themselves stay where
+// they are (culled ghosts become dead slots that no row holds), so the rows,
+// their entries and their order are untouched and no second build is needed.
+//
+// Floating-point order follows the reference exactly: plane distances
+// n.(x - p), closest points x + ((p - x).n) n (3-D), the 2-D and 1-D special
+// cases, and the closed-form eigenvalues of SymTensor::eigenValues
+// (src/Geometry/GeomSymmetricTensorInline_default.hh:2053-2110).  The 3-D
+// eigenvalues call atan2, cos and sin; their device and host results may
+// differ in the last bit, which moves hmax by far less than one ulp except in
+// rare rounding ties.
+#include "internal.hh"
+#include "dgeom.hh"
+#include "records.hh"
+#include "ell.hh"
+#include <cmath>
+#include <algorithm>
+
+namespace sphgpu {
+
+void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);
+
+namespace {
+
+constexpr int kBlock = 256;
+inline int nblocks(int64_t n, int bs = kBlock) { return int(std::max<int64_t>(1, (n + bs - 1) / bs)); }
+
+struct DSub {
+  double enterP[3], enterN[3], exitP[3], exitN[3];
+  int reflect;
+  double R[9];
+};
+
+DSub toDevice(const SubBoundary& b) {
+  DSub d;
+  for (int k = 0; k < 3; ++k) {
+    d.enterP[k] = b.enterP[k]; d.enterN[k] = b.enterN[k]; d.exitP[k] = b.exitP[k]; d.exitN[k] = b.exitN[k];
+  }
+  d.reflect = b.reflect;
+  for (int k = 0; k < 9; ++k) d.R[k] = b.R[k];
+  return d;
+}
+
+template <int D> __device__ __forceinline__ V<D> loadV(const double* __restrict__ a, int64_t i) {
+  V<D> v;
+  for (int k = 0; k < D; ++k) v.c[k] = a[i * D + k];
+  return v;
+}
+template <int D> __device__ __forceinline__ V<D> arr(const double* a) {
+  V<D> v;
+  for (int k = 0; k < D; ++k) v.c[k] = a[k];
+  return v;
+}
+// GeomPlane::signedDistance: normal.dot(point - mPoint)
+template <int D> __device__ __forceinline__ double signedDist(const double* P, const double* N, const V<D>& x) {
+  return vdot(arr<D>(N), vsub(x, arr<D>(P)));
+}
+// GeomPlane::closestPointOnPlane (GeomPlane.cc:237-259, pointDistances.hh:53-57)
+template <int D> __device__ __forceinline__ V<D> closestPoint(const double* P, const double* N, const V<D>& x) {
+  if constexpr (D == 1) {
+    return arr<1>(P);
+  } else if constexpr (D == 2) {
+    const V<2> a0p = vsub(x, arr<2>(P));
+    V<2> dir;
+    dir.c[0] = -N[1]; dir.c[1] = N[0];
+    const double s = vdot(a0p, dir);
+    V<2> o;
+    for (int k = 0; k < 2; ++k) o.c[k] = P[k] + dir.c[k] * s;
+    return o;
+  } else {
+    const double s = vdot(vsub(arr<3>(P), x), arr<3>(N));
+    V<3> o;
+    for (int k = 0; k < 3; ++k) o.c[k] = x.c[k] + N[k] * s;
+    return o;
+  }
+}
+// PlanarBoundary::updateGhostNodes: mapPositionThroughPlanes(x, exitPlane, enterPlane)
+//   = closestPoint(enter, x) - signedDistance(exit, x) * enter.normal
+template <int D> __device__ __forceinline__ V<D> ghostPosition(const DSub& b, const V<D>& x) {
+  const V<D> p0 = closestPoint<D>(b.enterP, b.enterN, x);
+  const double s = signedDist<D>(b.exitP, b.exitN, x);
+  V<D> o;
+  for (int k = 0; k < D; ++k) o.c[k] = p0.c[k] - b.enterN[k] * s;
+  return o;
+}
+
+// SymTensor::eigenValues().minElement()
+template <int D> __device__ double minEigen(const double* h) {
+  if constexpr (D == 1) {
+    return h[0];
+  } else if constexpr (D == 2) {
+    if (dabs(h[1]) < 1.0e-50) return dmin(h[0], h[2]);
+    const double b = h[0] + h[2];
+    const double c = h[0] * h[2] - h[1] * h[1];
+    const double q = 0.5 * (b + (b < 0.0 ? -1.0 : 1.0) * sqrt(dmax(0.0, b * b - 4.0 * c)));
+    return dmin(q, c / q);
+  } else {
+    const double onethird = 1.0 / 3.0;
+    const double sqrt3 = 1.7320508075688772;   // std::sqrt(3.0), correctly rounded
+    const double fscale = dmax(10.0 * 2.220446049250313e-16,
+                               dmax(dabs(h[0]), dmax(dabs(h[1]), dmax(dabs(h[2]), dmax(dabs(h[3]),
+                                    dmax(dabs(h[4]), dabs(h[5])))))));
+    const double fscalei = 1.0 / fscale;
+    const double a00 = h[0] * fscalei, a01 = h[1] * fscalei, a02 = h[2] * fscalei;
+    const double a11 = h[3] * fscalei, a12 = h[4] * fscalei, a22 = h[5] * fscalei;
+    const double c0 = a00 * a11 * a22 + 2.0 * a01 * a02 * a12 - a00 * a12 * a12 - a11 * a02 * a02 - a22 * a01 * a01;
+    const double c1 = a00 * a11 - a01 * a01 + a00 * a22 - a02 * a02 + a11 * a22 - a12 * a12;
+    const double c2 = a00 + a11 + a22;
+    const double c2Div3 = c2 * onethird;
+    const double aDiv3 = dmin(0.0, onethird * (c1 - c2 * c2Div3));
+    const double mbDiv2 = 0.5 * (c0 + c2Div3 * (2.0 * c2Div3 * c2Div3 - c1));
+    const double q = dmin(0.0, mbDiv2 * mbDiv2 + aDiv3 * aDiv3 * aDiv3);
+    const double mag = sqrt(-aDiv3);
+    const double angle = atan2(sqrt(-q), mbDiv2) * onethird;
+    const double cs = cos(angle), sn = sin(angle);
+    const double e0 = fscale * (c2Div3 + 2.0 * mag * cs);
+    const double e1 = fscale * (c2Div3 - mag * (cs + sqrt3 * sn));
+    const double e2 = fscale * (c2Div3 - mag * (cs - sqrt3 * sn));
+    return dmin(e0, dmin(e1, e2));
+  }
+}
+
+// R.v, rows left to right (GeomTensorInline_default.hh:1918-1922)
+template <int D> __device__ __forceinline__ V<D> reflectV(const double* R, const V<D>& v) {
+  V<D> o;
+  for (int i = 0; i < D; ++i) {
+    double s = R[i * D] * v.c[0];
+    for (int k = 1; k < D; ++k) s = s + R[i * D + k] * v.c[k];
+    o.c[i] = s;
+  }
+  return o;
+}
+// (R*(H*R)).Symmetric() (ReflectingBoundary.cc:155-175)
+template <int D> __device__ __forceinline__ void reflectH(const double* R, const double* h, double* out) {
+  double full[D * D], t1[D * D], t2[D * D];
+  int c = 0;
+  for (int i = 0; i < D; ++i)
+    for (int j = i; j < D; ++j, ++c) { full[i * D + j] = h[c]; full[j * D + i] = h[c]; }
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      double s = full[i * D] * R[j];
+      for (int k = 1; k < D; ++k) s = s + full[i * D + k] * R[k * D + j];
+      t1[i * D + j] = s;
+    }
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      double s = R[i * D] * t1[j];
+      for (int k = 1; k < D; ++k) s = s + R[i * D + k] * t1[k * D + j];
+      t2[i * D + j] = s;
+    }
+  c = 0;
+  for (int i = 0; i < D; ++i)
+    for (int j = i; j < D; ++j, ++c) out[c] = (i == j) ? t2[i * D + i] : 0.5 * (t2[i * D + j] + t2[j * D + i]);
+}
+
+// 1/lambda_min(H) of the internal nodes (a ghost's is computed when it is made).
+template <int D>
+__global__ void k_hinv_min(const double* __restrict__ H, int64_t n, double* __restrict__ hmaxOf) {
+  constexpr int NS = NSym<D>::value;
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) hmaxOf[i] = 1.0 / minEigen<D>(H + i * NS);
+}
+
+template <int D>
+__global__ void k_ghost_hmax(const double* __restrict__ pos, const double* __restrict__ hmaxOf, int64_t n, DSub b,
+                             double ext, unsigned long long* __restrict__ hmaxBits) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    const V<D> x = loadV<D>(pos, i);
+    const double hmaxi = hmaxOf[i];
+    const double d = dmin(dabs(signedDist<D>(b.exitP, b.exitN, x)), dabs(signedDist<D>(b.enterP, b.enterN, x)));
+    if (hmaxi > 0.0 && d < ext * hmaxi) v = hmaxi;
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) v = dmax(v, __shfl_xor(v, sh));
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(hmaxBits, (unsigned long long)__double_as_longlong(v));
+}
+
+template <int D>
+__global__ void k_ghost_flag(const double* __restrict__ pos, int64_t n, DSub b, double ext,
+                             const unsigned long long* __restrict__ hmaxBits, int32_t* __restrict__ flags) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const double hmax = __longlong_as_double((long long)*hmaxBits);
+  int32_t f = 0;
+  if (hmax > 0.0) {
+    const double disti = signedDist<D>(b.exitP, b.exitN, loadV<D>(pos, i)) / hmax;
+    f = (disti >= 0.0 && disti <= ext) ? 1 : 0;
+  }
+  flags[i] = f;
+}
+
+template <int D>
+__global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, int64_t n, int64_t nInt,
+                             const int32_t* __restrict__ flags, const int32_t* __restrict__ offs, DSub b, int subIdx,
+                             int32_t* __restrict__ control, int32_t* __restrict__ subOf, double* __restrict__ hmaxOf) {
+  constexpr int NS = NSym<D>::value;
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n || !flags[i]) return;
+  const int64_t g = n + offs[i];
+  const V<D> xg = ghostPosition<D>(b, loadV<D>(pos, i));
+  for (int k = 0; k < D; ++k) pos[g * D + k] = xg.c[k];
+  if (b.reflect) {
+    double hg[NS];
+    reflectH<D>(b.R, H + i * NS, hg);
+    for (int k = 0; k < NS; ++k) H[g * NS + k] = hg[k];
+    hmaxOf[g] = 1.0 / minEigen<D>(hg);
+  } else {
+    for (int k = 0; k < NS; ++k) H[g * NS + k] = H[i * NS + k];
+    hmaxOf[g] = hmaxOf[i];
+  }
+  control[g - nInt] = int32_t(i);
+  subOf[g - nInt] = subIdx;
+}
+
+struct FieldPtrs {
+  double *pos, *H, *vel, *mass, *rho, *eps, *P, *cs, *omega, *fCl, *fCq, *fsh;
+};
+
+// Ghost rows [g0, g1) of one sub-boundary from their controls.
+template <int D>
+__global__ void k_ghost_fields(FieldPtrs f, int64_t g0, int64_t g1, int64_t nInt, const int32_t* __restrict__ control,
+                               DSub b, int geometry) {
+  constexpr int NS = NSym<D>::value;
+  const int64_t g = g0 + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (g >= g1) return;
+  const int64_t c = control[g - nInt];
+  if (geometry) {
+    const V<D> xg = ghostPosition<D>(b, loadV<D>(f.pos, c));
+    for (int k = 0; k < D; ++k) f.pos[g * D + k] = xg.c[k];
+    if (b.reflect) {
+      double hg[NS];
+      reflectH<D>(b.R, f.H + c * NS, hg);
+      for (int k = 0; k < NS; ++k) f.H[g * NS + k] = hg[k];
+    } else {
+      for (int k = 0; k < NS; ++k) f.H[g * NS + k] = f.H[c * NS + k];
+    }
+  }
+  const V<D> v = loadV<D>(f.vel, c);
+  const V<D> vg = b.reflect ? reflectV<D>(b.R, v) : v;
+  for (int k = 0; k < D; ++k) f.vel[g * D + k] = vg.c[k];
+  f.mass[g] = f.mass[c];
+  f.rho[g] = f.rho[c];
+  f.eps[g] = f.eps[c];
+  f.P[g] = f.P[c];
+  f.cs[g] = f.cs[c];
+  f.omega[g] = f.omega[c];
+  if (f.fCl) { f.fCl[g] = f.fCl[c]; f.fCq[g] = f.fCq[c]; f.fsh[g] = f.fsh[c]; }
+}
+
+__global__ void k_ghost_scalar(double* __restrict__ a, int64_t g0, int64_t g1, int64_t nInt,
+                               const int32_t* __restrict__ control) {
+  const int64_t g = g0 + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (g < g1) a[g] = a[control[g - nInt]];
+}
+
+// Ghost records after sum density: the same expressions as the internal update.
+template <bool ISO>
+__global__ void k_ghost_records(double* __restrict__ rec, const int32_t* __restrict__ iperm,
+                                const double* __restrict__ rho, int64_t nInt, int64_t nTot) {
+  using R = RL<ISO>;
+  const int64_t g = nInt + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (g >= nTot) return;
+  double* r = rec + int64_t(iperm[g]) * R::STRIDE;
+  const double acc = rho[g];
+  const double som = r[R::SOM], p = r[R::P], m = r[R::M];
+  r[R::RHO] = acc;
+  r[R::PRHO] = som * p / (acc * acc);
+  r[R::MOR] = m / acc;
+}
+
+// ---- culling ---------------------------------------------------------------
+// keep[c]: internal nodes, ghosts some row holds (marked by the search in slot
+// space), then every ghost that controls another ghost.
+__global__ void k_keep_gather(int32_t* __restrict__ keep, const int32_t* __restrict__ iperm,
+                              const int32_t* __restrict__ slotKeep, int64_t nInt, int64_t nTot) {
+  const int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (c < nInt) keep[c] = 1;
+  else if (c < nTot) keep[c] = slotKeep[iperm[c]];
+}
+
+__global__ void k_keep_controls(int32_t* __restrict__ keep, int64_t nInt, int64_t nTot,
+                                const int32_t* __restrict__ control) {
+  const int64_t g = nInt + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (g >= nTot) return;
+  const int32_t c = control[g - nInt];
+  if (c >= nInt) keep[c] = 1;
+}
+
+// Compact one caller-order field of `comps` doubles per row.
+__global__ void k_compact_rows(const double* __restrict__ src, double* __restrict__ dst, int comps,
+                               const int32_t* __restrict__ keep, const int32_t* __restrict__ scan, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  const int64_t o = scan[i];
+  for (int c = 0; c < comps; ++c) dst[o * comps + c] = src[i * comps + c];
+}
+
+__global__ void k_compact_ghosts(const int32_t* __restrict__ control, const int32_t* __restrict__ subOf,
+                                 int32_t* __restrict__ newControl, int32_t* __restrict__ newSubOf,
+                                 const int32_t* __restrict__ keep, const int32_t* __restrict__ scan,
+                                 int64_t nInt, int64_t nTot) {
+  const int64_t g = nInt + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (g >= nTot || !keep[g]) return;
+  const int64_t o = scan[g] - nInt;
+  newControl[o] = scan[control[g - nInt]];
+  newSubOf[o] = subOf[g - nInt];
+}
+
+// Slots keep their place; their caller index is renumbered, culled ones die.
+__global__ void k_perm_remap(int32_t* __restrict__ perm, const int32_t* __restrict__ keep,
+                             const int32_t* __restrict__ scan, int64_t nSlots) {
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k >= nSlots) return;
+  const int32_t c = perm[k];
+  if (c >= 0) perm[k] = keep[c] ? scan[c] : -1;
+}
+
+__global__ void k_iperm_remap(const int32_t* __restrict__ iperm, const int32_t* __restrict__ keep,
+                              const int32_t* __restrict__ scan, int64_t n, int32_t* __restrict__ out) {
+  const int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (c < n && keep[c]) out[scan[c]] = iperm[c];
+}
+
+std::vector<DBuf*> stateBufs(Sim& s) {
+  return {&s.pos, &s.vel, &s.H, &s.mass, &s.rho, &s.eps, &s.P, &s.cs, &s.omega, &s.fCl, &s.fCq, &s.fshear};
+}
+int stateComps(const Sim& s, int which) {
+  const int D = s.dim, NS = nsymOf(D);
+  return which == 0 || which == 1 ? D : (which == 2 ? NS : 1);
+}
+
+// Make room for `rows` rows in every caller-order field, keeping nTot rows.
+void ensureRows(Sim& s, int64_t rows) {
+  auto bufs = stateBufs(s);
+  for (int w = 0; w < int(bufs.size()); ++w) {
+    if (w >= 9 && !s.hasMultipliers) continue;
+    const size_t per = sizeof(double) * size_t(stateComps(s, w));
+    if (bufs[size_t(w)]->bytes >= per * size_t(rows)) continue;
+    bufs[size_t(w)]->grow(per * size_t(rows + rows / 4), per * size_t(s.nTot), s.stream);
+  }
+}
+
+FieldPtrs fieldPtrs(Sim& s) {
+  FieldPtrs f;
+  f.pos = s.pos.as<double>(); f.H = s.H.as<double>(); f.vel = s.vel.as<double>(); f.mass = s.mass.as<double>();
+  f.rho = s.rho.as<double>(); f.eps = s.eps.as<double>(); f.P = s.P.as<double>(); f.cs = s.cs.as<double>();
+  f.omega = s.omega.as<double>();
+  f.fCl = s.hasMultipliers ? s.fCl.as<double>() : nullptr;
+  f.fCq = s.hasMultipliers ? s.fCq.as<double>() : nullptr;
+  f.fsh = s.hasMultipliers ? s.fshear.as<double>() : nullptr;
+  return f;
+}
+
+template <int D> void applyFieldsD(Sim& s, bool geometry) {
+  const FieldPtrs f = fieldPtrs(s);
+  for (size_t b = 0; b < s.subs.size(); ++b) {
+    const int64_t g0 = s.nInt + s.subStart[b], g1 = s.nInt + s.subStart[b + 1];
+    if (g1 <= g0) continue;
+    hipLaunchKernelGGL(k_ghost_fields<D>, dim3(nblocks(g1 - g0)), dim3(kBlock), 0, s.stream, f, g0, g1, s.nInt,
+                       s.ghostControl.as<int32_t>(), toDevice(s.subs[b]), geometry ? 1 : 0);
+  }
+  SG_CHECK_HIP(hipGetLastError());
+}
+
+template <int D> void generateD(Sim& s, double ext) {
+  hipStream_t st = s.stream;
+  constexpr int NS = NSym<D>::value;
+  (void)NS;
+  s.subStart.assign(s.subs.size() + 1, 0);
+  s.flags.ensure(sizeof(int32_t) * 8 + sizeof(unsigned long long) * 2);
+  unsigned long long* hbits = reinterpret_cast<unsigned long long*>(s.flags.as<int32_t>() + 4);
+  s.hmaxOf.ensure(sizeof(double) * size_t(std::max<int64_t>(s.nInt, 1)));
+  if (s.nInt > 0)
+    hipLaunchKernelGGL(k_hinv_min<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.H.as<double>(), s.nInt,
+                       s.hmaxOf.as<double>());
+  for (size_t b = 0; b < s.subs.size(); ++b) {
+    const int64_t n = s.nTot;
+    s.subStart[b] = n - s.nInt;
+    const DSub d = toDevice(s.subs[b]);
+    s.keep.ensure(sizeof(int32_t) * size_t(n + 1));
+    s.keepScan.ensure(sizeof(int32_t) * size_t(n + 1));
+    SG_CHECK_HIP(hipMemsetAsync(hbits, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_ghost_hmax<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
+                       s.hmaxOf.as<double>(), n, d, ext, hbits);
+    hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), n, d, ext, hbits,
+                       s.keep.as<int32_t>());
+    exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), n, s);
+    int32_t count = 0;
+    SG_CHECK_HIP(hipMemcpyAsync(&count, s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SG_CHECK_HIP(hipStreamSynchronize(st));
+    SG_REQUIRE(n + int64_t(count) < (int64_t(1) << 31) - 1, SPHGPU_ERR_CAPACITY, "too many ghost nodes");
+    if (count == 0) continue;
+    ensureRows(s, n + count);
+    s.hmaxOf.grow(sizeof(double) * size_t(n + count + (n + count) / 4), sizeof(double) * size_t(n), st);
+    const int64_t ng = n + count - s.nInt;
+    s.ghostControl.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
+    s.ghostSub.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
+    hipLaunchKernelGGL(k_ghost_make<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
+                       n, s.nInt, s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), d, int(b),
+                       s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>());
+    SG_CHECK_HIP(hipGetLastError());
+    s.nTot = n + count;
+  }
+  s.subStart[s.subs.size()] = s.nTot - s.nInt;
+  s.ghostsGenerated = s.nTot - s.nInt;
+  applyFieldsD<D>(s, false);
+}
+
+void cull(Sim& s) {
+  hipStream_t st = s.stream;
+  const int64_t n = s.nTot, nInt = s.nInt;
+  if (n == nInt) return;
+  s.keep.ensure(sizeof(int32_t) * size_t(n + 1));
+  s.keepScan.ensure(sizeof(int32_t) * size_t(n + 1));
+  int32_t* keep = s.keep.as<int32_t>();
+  hipLaunchKernelGGL(k_keep_gather, dim3(nblocks(n)), dim3(kBlock), 0, st, keep, s.iperm.as<int32_t>(),
+                     s.slotKeep.as<int32_t>(), nInt, n);
+  hipLaunchKernelGGL(k_keep_controls, dim3(nblocks(n - nInt)), dim3(kBlock), 0, st, keep, nInt, n,
+                     s.ghostControl.as<int32_t>());
+  exclusiveScanI32(keep, s.keepScan.as<int32_t>(), n, s);
+  // new total and the new first ghost of every sub-boundary
+  std::vector<int32_t> starts(s.subStart.size() + 1);
+  for (size_t b = 0; b < s.subStart.size(); ++b)
+    SG_CHECK_HIP(hipMemcpyAsync(&starts[b], s.keepScan.as<int32_t>() + nInt + s.subStart[b], sizeof(int32_t),
+                                hipMemcpyDeviceToHost, st));
+  SG_CHECK_HIP(hipMemcpyAsync(&starts.back(), s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              st));
+  SG_CHECK_HIP(hipStreamSynchronize(st));
+  const int64_t nNew = starts.back();
+  if (nNew == n) return;
+  const int32_t* scan = s.keepScan.as<int32_t>();
+
+  // caller-order fields
+  auto bufs = stateBufs(s);
+  for (int w = 0; w < int(bufs.size()); ++w) {
+    if (w >= 9 && !s.hasMultipliers) continue;
+    const int comps = stateComps(s, w);
+    DBuf& spare = s.spare[w];
+    spare.ensure(bufs[size_t(w)]->bytes);
+    hipLaunchKernelGGL(k_compact_rows, dim3(nblocks(n)), dim3(kBlock), 0, st, bufs[size_t(w)]->as<double>(),
+                       spare.as<double>(), comps, keep, scan, n);
+    bufs[size_t(w)]->swap(spare);
+  }
+  // ghost controls (control and sub-boundary arrays share the spare pair)
+  s.ghostControlSpare.ensure(s.ghostControl.bytes);
+  s.ghostSubSpare.ensure(s.ghostSub.bytes);
+  hipLaunchKernelGGL(k_compact_ghosts, dim3(nblocks(n - nInt)), dim3(kBlock), 0, st, s.ghostControl.as<int32_t>(),
+                     s.ghostSub.as<int32_t>(), s.ghostControlSpare.as<int32_t>(), s.ghostSubSpare.as<int32_t>(), keep,
+                     scan, nInt, n);
+  s.ghostControl.swap(s.ghostControlSpare);
+  s.ghostSub.swap(s.ghostSubSpare);
+  // slot <-> caller maps
+  hipLaunchKernelGGL(k_perm_remap, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), keep, scan,
+                     s.nSlots);
+  s.ipermSpare.ensure(s.iperm.bytes);
+  hipLaunchKernelGGL(k_iperm_remap, dim3(nblocks(n)), dim3(kBlock), 0, st, s.iperm.as<int32_t>(), keep, scan, n,
+                     s.ipermSpare.as<int32_t>());
+  s.iperm.swap(s.ipermSpare);
+  SG_CHECK_HIP(hipGetLastError());
+  for (size_t b = 0; b < s.subStart.size(); ++b) s.subStart[b] = int64_t(starts[b]) - nInt;
+  s.nTot = nNew;
+  s.recValid = false;
+  s.derivsValid = false;
+}
+
+}  // namespace
+
+void setBoundaries(Sim& s, int n, const sphgpu_boundary* b) {
+  SG_REQUIRE(n >= 0 && (n == 0 || b), SPHGPU_ERR_ARG, "bad boundary list");
+  const int D = s.dim;
+  std::vector<SubBoundary> subs;
+  auto unit = [&](const double* nin, double* nout) {
+    // GeomPlane: normal.unitVector() (1-D divides, else multiplies by 1/|n|)
+    double m2 = nin[0] * nin[0];
+    for (int k = 1; k < D; ++k) m2 = m2 + nin[k] * nin[k];
+    const double m = D == 1 ? std::fabs(nin[0]) : std::sqrt(m2);
+    SG_REQUIRE(m > 1.0e-50, SPHGPU_ERR_ARG, "boundary plane normal is zero");
+    for (int k = 0; k < 3; ++k) nout[k] = 0.0;
+    if (D == 1) nout[0] = nin[0] / m;
+    else { const double inv = 1.0 / m; for (int k = 0; k < D; ++k) nout[k] = nin[k] * inv; }
+  };
+  for (int q = 0; q < n; ++q) {
+    SG_REQUIRE(b[q].kind == SPHGPU_BOUNDARY_REFLECTING || b[q].kind == SPHGPU_BOUNDARY_PERIODIC, SPHGPU_ERR_ARG,
+               "unknown boundary kind");
+    double n1[3], n2[3], p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
+    unit(b[q].plane1.normal, n1);
+    for (int k = 0; k < D; ++k) p1[k] = b[q].plane1.point[k];
+    SubBoundary a{};
+    if (b[q].kind == SPHGPU_BOUNDARY_REFLECTING) {
+      for (int k = 0; k < 3; ++k) { a.enterP[k] = a.exitP[k] = p1[k]; a.enterN[k] = a.exitN[k] = n1[k]; }
+      a.reflect = 1;
+      // Tensor::one - 2.0*normal.selfdyad()
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) a.R[i * D + j] = (i == j ? 1.0 : 0.0) - 2.0 * (n1[std::min(i, j)] * n1[std::max(i, j)]);
+      subs.push_back(a);
+    } else {
+      unit(b[q].plane2.normal, n2);
+      for (int k = 0; k < D; ++k) p2[k] = b[q].plane2.point[k];
+      SubBoundary c{}, d{};
+      for (int k = 0; k < 3; ++k) {
+        c.enterP[k] = p1[k]; c.enterN[k] = n1[k]; c.exitP[k] = p2[k]; c.exitN[k] = n2[k];
+        d.enterP[k] = p2[k]; d.enterN[k] = n2[k]; d.exitP[k] = p1[k]; d.exitN[k] = n1[k];
+      }
+      c.reflect = d.reflect = 0;
+      subs.push_back(c);
+      subs.push_back(d);
+    }
+  }
+  s.subs = subs;
+  s.subStart.clear();
+}
+
+void setGhostNodes(Sim& s, double ext, bool doCull) {
+  SG_REQUIRE(ext > 0.0, SPHGPU_ERR_ARG, "kernel extent must be positive");
+  SG_REQUIRE(s.nTot == s.nInt || s.deviceGhosts, SPHGPU_ERR_STATE,
+             "sphgpu_set_ghost_nodes needs n_total = n_internal (ghosts are made on the device)");
+  hipStream_t st = s.stream;
+  SG_CHECK_HIP(hipEventRecord(s.ev[8], st));
+  s.nTot = s.nInt;                       // drop the previous ghosts
+  s.neighborsValid = false;
+  s.recValid = false;
+  s.derivsValid = false;
+  if (s.dim == 1) generateD<1>(s, ext);
+  else if (s.dim == 2) generateD<2>(s, ext);
+  else generateD<3>(s, ext);
+  s.deviceGhosts = true;
+  SG_CHECK_HIP(hipEventRecord(s.ev[9], st));
+  s.markGhostUse = doCull;
+  try {
+    buildNeighbors(s, ext);
+  } catch (...) {
+    s.markGhostUse = false;
+    throw;
+  }
+  s.markGhostUse = false;
+  SG_CHECK_HIP(hipEventRecord(s.ev[10], st));
+  if (doCull) cull(s);
+  SG_CHECK_HIP(hipEventRecord(s.ev[11], st));
+  SG_CHECK_HIP(hipEventSynchronize(s.ev[11]));
+  float a = 0.f, c = 0.f;
+  SG_CHECK_HIP(hipEventElapsedTime(&a, s.ev[8], s.ev[9]));
+  SG_CHECK_HIP(hipEventElapsedTime(&c, s.ev[10], s.ev[11]));
+  s.timing.ghosts_ms = a;
+  s.timing.cull_ms = c;
+}
+
+void applyGhostBoundaries(Sim& s) {
+  if (!s.deviceGhosts || s.nTot == s.nInt) return;
+  if (s.dim == 1) applyFieldsD<1>(s, true);
+  else if (s.dim == 2) applyFieldsD<2>(s, true);
+  else applyFieldsD<3>(s, true);
+  s.recValid = false;
+  s.derivsValid = false;
+}
+
+void applyGhostDensity(Sim& s) {
+  if (!s.deviceGhosts || s.nTot == s.nInt) return;
+  hipStream_t st = s.stream;
+  for (size_t b = 0; b < s.subs.size(); ++b) {
+    const int64_t g0 = s.nInt + s.subStart[b], g1 = s.nInt + s.subStart[b + 1];
+    if (g1 <= g0) continue;
+    hipLaunchKernelGGL(k_ghost_scalar, dim3(nblocks(g1 - g0)), dim3(kBlock), 0, st, s.rho.as<double>(), g0, g1,
+                       s.nInt, s.ghostControl.as<int32_t>());
+  }
+  const int64_t ng = s.nTot - s.nInt;
+  if (s.isotropic)
+    hipLaunchKernelGGL(k_ghost_records<true>, dim3(nblocks(ng)), dim3(kBlock), 0, st, s.rec.as<double>(),
+                       s.iperm.as<int32_t>(), s.rho.as<double>(), s.nInt, s.nTot);
+  else
+    hipLaunchKernelGGL(k_ghost_records<false>, dim3(nblocks(ng)), dim3(kBlock), 0, st, s.rec.as<double>(),
+                       s.iperm.as<int32_t>(), s.rho.as<double>(), s.nInt, s.nTot);
+  SG_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace sphgpu

@@ -51,6 +51,13 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
   double r[R::STRIDE];
 #pragma unroll
   for (int c = 0; c < R::STRIDE; ++c) r[c] = 0.0;
+  if (i < 0) {                      // dead slot (culled ghost): no row holds it
+    r[R::ORIG] = 2147483648.0;
+    double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
+#pragma unroll
+    for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
+    return;
+  }
   for (int c = 0; c < D; ++c) { r[R::X + c] = pos[i * D + c]; r[R::V + c] = vel[i * D + c]; }
   S<D> h;
   for (int c = 0; c < NS; ++c) h.c[c] = H[i * NS + c];
@@ -224,12 +231,14 @@ template <int D, bool ISO> __device__ __forceinline__ DNbr<D, ISO> loadDNbr(cons
 template <int D, bool ISO, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec, const int32_t* __restrict__ rowCount,
                                                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
-                                                        TabArgs t, double* __restrict__ rhoOut) {
+                                                        const int32_t* __restrict__ work, TabArgs t,
+                                                        double* __restrict__ rhoOut) {
   using R = RL<ISO>;
   extern __shared__ double lds[];
   const double* tab = stageTable<LDS>(lds, t);
-  const int64_t k = xcdBlock((n + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
-  if (k >= n) return;
+  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  if (tw >= nInt) return;
+  const int64_t k = work[tw];
   double* ri_ = rec + k * R::STRIDE;
   const double oi = ri_[R::ORIG];
   if (oi >= double(nInt)) return;
@@ -474,14 +483,13 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, const double* __re
 template <int D, bool ISO, bool TENSILE, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict__ rec, const int32_t* __restrict__ rowCount,
                                                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
-                                                        TabArgs t, PassArgs p, OutArgs o,
-                                                        unsigned long long* __restrict__ ncalc) {
+                                                        const int32_t* __restrict__ work, TabArgs t, PassArgs p,
+                                                        OutArgs o, unsigned long long* __restrict__ ncalc) {
   extern __shared__ double lds[];
   const double* tab = stageTable<LDS>(lds, t);
-  const int64_t k = xcdBlock((n + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   unsigned pairs = 0;
-  if (k < n && rec[k * RL<ISO>::STRIDE + RL<ISO>::ORIG] < double(nInt))
-    pairs = derivsNodeFP64<D, ISO, TENSILE>(k, rec, rowCount, nbr, n, tab, t, p, o);
+  if (tw < nInt) pairs = derivsNodeFP64<D, ISO, TENSILE>(work[tw], rec, rowCount, nbr, n, tab, t, p, o);
   for (int sh = 32; sh > 0; sh >>= 1) pairs += __shfl_xor(pairs, sh);
   if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(ncalc, (unsigned long long)pairs);
 }
@@ -493,7 +501,7 @@ TabArgs tabArgs(const Sim& s) {
 
 template <int D>
 void packD(Sim& s) {
-  const int64_t n = s.nTot;
+  const int64_t n = s.nSlots;
   const bool iso = s.isotropic;
   const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
   s.rec.ensure(sizeof(double) * size_t(n) * stride);
@@ -511,24 +519,27 @@ void packD(Sim& s) {
 
 template <int D>
 void sumDensityD(Sim& s) {
-  const int64_t n = s.nTot;
+  const int64_t n = s.nSlots;
   TabArgs t = tabArgs(s);
   const bool lds = s.table.n <= kLdsTablePoints;
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(n))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
-                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, s.rho.as<double>());
+    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
+                       s.rho.as<double>());
   };
   if (s.isotropic) { if (lds) go(k_sum_density<D, true, true>); else go(k_sum_density<D, true, false>); }
   else { if (lds) go(k_sum_density<D, false, true>); else go(k_sum_density<D, false, false>); }
   SG_CHECK_HIP(hipGetLastError());
+  // SPHHydroBase::finalize then applies the boundaries to the mass density
+  applyGhostDensity(s);
   SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
 }
 
 template <int D>
 void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
-  const int64_t n = s.nTot;
+  const int64_t n = s.nSlots;
   hipStream_t st = s.stream;
   const int total = derivTotalDoubles(s.dim);
   s.out.ensure(sizeof(double) * size_t(n) * total);
@@ -557,8 +568,9 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   for (int f = 0; f < SPHGPU_NUM_DERIV_FIELDS; ++f) o.off[f] = derivOffsetDoubles(s.dim, f);
   const bool tensile = hp.epsTensile != 0.0;
   auto launch = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(n))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
-                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, t, p, o, dncalc);
+    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t, p, o,
+                       dncalc);
   };
   if (lds) {
     if (s.isotropic) { if (tensile) launch(k_derivs_fp64<D, true, true, true>); else launch(k_derivs_fp64<D, true, false, true>); }

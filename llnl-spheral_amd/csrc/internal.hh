@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include <stdexcept>
+#include <utility>
 #include "../../include/sphgpu.h"
 
 namespace sphgpu {
@@ -38,6 +39,20 @@ struct DBuf {
     SG_CHECK_HIP(hipMalloc(&p, nb));
     bytes = nb;
   }
+  // Grow to at least nb bytes keeping the first `keep` bytes (stream-ordered copy).
+  void grow(size_t nb, size_t keep, hipStream_t st) {
+    if (nb <= bytes) return;
+    void* q = nullptr;
+    SG_CHECK_HIP(hipMalloc(&q, nb));
+    if (p && keep) SG_CHECK_HIP(hipMemcpyAsync(q, p, keep, hipMemcpyDeviceToDevice, st));
+    if (p) {
+      SG_CHECK_HIP(hipStreamSynchronize(st));
+      SG_CHECK_HIP(hipFree(p));
+    }
+    p = q;
+    bytes = nb;
+  }
+  void swap(DBuf& o) { std::swap(p, o.p); std::swap(bytes, o.bytes); }
   void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
@@ -64,7 +79,16 @@ struct Grid {
 };
 
 struct Timing {
-  float neighbors_ms = 0, density_ms = 0, derivs_ms = 0, pack_ms = 0;
+  float neighbors_ms = 0, density_ms = 0, derivs_ms = 0, pack_ms = 0, ghosts_ms = 0, cull_ms = 0;
+};
+
+// One planar sub-boundary: the reference's PlanarBoundary with its enter and
+// exit planes (a ReflectingBoundary is one with enter = exit; a
+// PeriodicBoundary is two, plane1 -> plane2 then plane2 -> plane1).
+struct SubBoundary {
+  double enterP[3], enterN[3], exitP[3], exitN[3];   // unit normals
+  int reflect;                                       // 1: fields mapped by R = I - 2 n n, 0: copied
+  double R[9];                                       // row major, D x D used
 };
 
 struct Sim {
@@ -108,7 +132,23 @@ struct Sim {
   bool derivsValid = false;
   int64_t ncalc = 0;        // unordered pairs evaluated by the last pass
 
-  hipEvent_t ev[8] = {};
+  // planar ghost boundaries generated on the device
+  std::vector<SubBoundary> subs;
+  bool deviceGhosts = false;     // ghost rows nInt..nTot-1 were made by setGhostNodes
+  DBuf ghostControl, ghostSub;   // per ghost: control node (caller index), sub-boundary
+  std::vector<int64_t> subStart; // first ghost (offset from nInt) of each sub-boundary, +1 entry
+  int64_t ghostsGenerated = 0;   // before culling
+  DBuf keep, keepScan, slotKeep, ghostControlSpare, ghostSubSpare, ipermSpare, hmaxOf;
+  DBuf spare[12];                // one per caller-order state field (culling compacts into it)
+  bool markGhostUse = false;     // the next build marks the ghost slots rows hold (for culling)
+
+  // Slots of the current build.  Culling renumbers the caller-order rows but
+  // keeps the slot space: culled ghosts become dead slots (perm = -1) that no
+  // row holds, so rows and their order are untouched.
+  int64_t nSlots = 0;
+  DBuf work, workFlag, workScan; // internal slots in slot order (n_internal entries)
+
+  hipEvent_t ev[12] = {};
   Timing timing;
 };
 
@@ -120,6 +160,11 @@ void sumDensity(Sim& s);
 void evalDerivs(Sim& s, const sphgpu_hydro_params& p);
 // probe.hip
 double copyProbe(Sim& s, int64_t bytes, int repeats);
+// ghosts.hip
+void setBoundaries(Sim& s, int n, const sphgpu_boundary* b);
+void setGhostNodes(Sim& s, double kernelExtent, bool cull);
+void applyGhostBoundaries(Sim& s);
+void applyGhostDensity(Sim& s);   // after sum density: ghost rho and their records
 // scans (neighbors.hip)
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);

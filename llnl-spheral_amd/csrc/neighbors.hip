@@ -215,6 +215,7 @@ struct SearchArgs {
   float lo32, hi32;         // ext2*(1 -+ 1e-5): FP32 classification margins
   int64_t n, nInt;
   int cap;
+  int32_t* keepSlot;        // optional: set to 1 for every ghost slot some row holds
 };
 
 // Exact reference test for isotropic H: eta = h r componentwise (the SymTensor
@@ -285,7 +286,9 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
       for (int q = 0; q < cnt; ++q) {
         const double4 r = *reinterpret_cast<const double4*>(rec + int64_t(st + q) * R::STRIDE);
         sCand[o + q] = make_float4(float(r.x - ox), float(r.y - oy), float(r.z - oz), float(r.w));
-        sSlot[o + q] = st + q;
+        // bit 31 flags a ghost (slots stay below 2^31)
+        const bool ghost = rec[int64_t(st + q) * R::STRIDE + R::ORIG] >= double(a.nInt);
+        sSlot[o + q] = int32_t(uint32_t(st + q) | (ghost ? 0x80000000u : 0u));
       }
     }
   } else if (threadIdx.x == 0) {
@@ -317,13 +320,17 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
             const float tmin = ti < tj ? ti : tj;
             bool in = tmin < a.lo32;
             if (!in && tmin <= a.hi32) {
-              const int32_t sj = sSlot[t];
+              const int32_t sj = sSlot[t] & 0x7fffffff;
               const double* rj = rec + int64_t(sj) * R::STRIDE;
               in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
             }
             if (in) {
-              const int32_t sj = sSlot[t];
-              if (sj != int32_t(k)) row.push(sj);
+              const int32_t raw = sSlot[t];
+              const int32_t sj = raw & 0x7fffffff;
+              if (sj != int32_t(k)) {
+                row.push(sj);
+                if (raw < 0 && a.keepSlot) a.keepSlot[sj] = 1;
+              }
             }
           }
         } else {
@@ -334,7 +341,10 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
               const int32_t sj = st + q2;
               if (sj == int32_t(k)) continue;
               const double* rj = rec + int64_t(sj) * R::STRIDE;
-              if (exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2)) row.push(sj);
+              if (exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2)) {
+                row.push(sj);
+                if (a.keepSlot && rj[R::ORIG] >= double(a.nInt)) a.keepSlot[sj] = 1;
+              }
             }
           }
         }
@@ -395,13 +405,27 @@ __global__ void k_search_global(const double* __restrict__ rec, const int32_t* _
           }
           const double eta2i = vmag2(sdot(Hi, rij));
           const double eta2j = vmag2(sdot(Hj, rij));
-          if (eta2i <= a.ext2 || eta2j <= a.ext2) row.push(j);
+          if (eta2i <= a.ext2 || eta2j <= a.ext2) {
+            row.push(j);
+            if (a.keepSlot && rj_[R::ORIG] >= double(a.nInt)) a.keepSlot[j] = 1;
+          }
         }
       }
     }
   }
   row.finish();
   rowCount[k] = row.cnt;
+}
+
+// Internal slots in slot order: the work list of the density and pair passes.
+__global__ void k_work_flag(const int32_t* __restrict__ perm, int64_t n, int64_t nInt, int32_t* __restrict__ flag) {
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k < n) flag[k] = (perm[k] >= 0 && perm[k] < nInt) ? 1 : 0;
+}
+__global__ void k_work_scatter(const int32_t* __restrict__ flag, const int32_t* __restrict__ scan, int64_t n,
+                               int32_t* __restrict__ work) {
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k < n && flag[k]) work[scan[k]] = int32_t(k);
 }
 
 // Total entries and longest row.
@@ -661,6 +685,7 @@ void buildNeighborsD(Sim& s, double ext) {
                      s.perm.as<int32_t>(), G.ncells);
   hipLaunchKernelGGL(k_iperm, dim3(nblocks(n)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), s.iperm.as<int32_t>(), n);
   SG_CHECK_HIP(hipGetLastError());
+  s.nSlots = n;
   packRecords(s);
 
   // 3. rows
@@ -678,6 +703,12 @@ void buildNeighborsD(Sim& s, double ext) {
   sa.hi32 = float(sa.ext2 * (1.0 + band));
   sa.n = n;
   sa.nInt = s.nInt;
+  sa.keepSlot = nullptr;
+  if (s.markGhostUse && s.nTot > s.nInt) {
+    s.slotKeep.ensure(sizeof(int32_t) * size_t(n + 1));
+    SG_CHECK_HIP(hipMemsetAsync(s.slotKeep.p, 0, sizeof(int32_t) * size_t(n), st));
+    sa.keepSlot = s.slotKeep.as<int32_t>();
+  }
   if (s.ellCap <= 0) s.ellCap = D == 3 ? 160 : (D == 2 ? 48 : 16);
   s.rowCount.ensure(sizeof(int32_t) * size_t(n));
   s.flags.ensure(sizeof(int32_t) * 8 + sizeof(unsigned long long) * 2);
@@ -714,6 +745,15 @@ void buildNeighborsD(Sim& s, double ext) {
   if (s.rowOrder == SPHGPU_ROW_ORDER_REFERENCE && n > 0)
     hipLaunchKernelGGL(k_row_sort, dim3(nblocks(n * 64)), dim3(kBlock), 0, st, s.rowCount.as<int32_t>(),
                        s.nbr.as<int32_t>(), n, s.perm.as<int32_t>(), s.iperm.as<int32_t>());
+  // 5. work list of internal slots
+  s.workFlag.ensure(sizeof(int32_t) * size_t(n + 1));
+  s.workScan.ensure(sizeof(int32_t) * size_t(n + 1));
+  s.work.ensure(sizeof(int32_t) * size_t(std::max<int64_t>(s.nInt, 1)));
+  hipLaunchKernelGGL(k_work_flag, dim3(nblocks(n)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), n, s.nInt,
+                     s.workFlag.as<int32_t>());
+  exclusiveScan<int32_t, int32_t>(s.workFlag.as<int32_t>(), s.workScan.as<int32_t>(), n, s);
+  hipLaunchKernelGGL(k_work_scatter, dim3(nblocks(n)), dim3(kBlock), 0, st, s.workFlag.as<int32_t>(),
+                     s.workScan.as<int32_t>(), n, s.work.as<int32_t>());
   SG_CHECK_HIP(hipGetLastError());
   SG_CHECK_HIP(hipEventRecord(s.ev[1], st));
 }

@@ -57,6 +57,53 @@ class HydroParams(C.Structure):
                 ("compatibleEnergyEvolution", C.c_int), ("precision", C.c_int)]
 
 
+BOUNDARY_REFLECTING = 0
+BOUNDARY_PERIODIC = 1
+
+
+class Plane(C.Structure):
+    _fields_ = [("point", C.c_double * 3), ("normal", C.c_double * 3)]
+
+
+class Boundary(C.Structure):
+    _fields_ = [("kind", C.c_int), ("plane1", Plane), ("plane2", Plane)]
+
+
+def plane(point, normal):
+    pt = list(point) + [0.0] * (3 - len(point))
+    nm = list(normal) + [0.0] * (3 - len(normal))
+    return Plane((C.c_double * 3)(*pt), (C.c_double * 3)(*nm))
+
+
+def reflecting(point, normal):
+    """ReflectingBoundary(GeomPlane(point, normal)); normal points inwards."""
+    return Boundary(BOUNDARY_REFLECTING, plane(point, normal), plane(point, normal))
+
+
+def periodic(point1, normal1, point2, normal2):
+    """PeriodicBoundary(GeomPlane(point1, normal1), GeomPlane(point2, normal2))."""
+    return Boundary(BOUNDARY_PERIODIC, plane(point1, normal1), plane(point2, normal2))
+
+
+def box_boundaries(dim, lo=0.0, hi=1.0, kind="reflecting"):
+    """Planes at lo and hi on every axis with inward normals, x then y then z
+    (reflecting: lo plane then hi plane per axis; periodic: one pair per axis)."""
+    out = []
+    for a in range(dim):
+        e = [0.0] * dim
+        e[a] = 1.0
+        p_lo = [0.0] * dim
+        p_lo[a] = lo
+        p_hi = [0.0] * dim
+        p_hi[a] = hi
+        if kind == "reflecting":
+            out.append(reflecting(p_lo, e))
+            out.append(reflecting(p_hi, [-x for x in e]))
+        else:
+            out.append(periodic(p_lo, e, p_hi, [-x for x in e]))
+    return out
+
+
 def hydro_params(**kw):
     """Defaults of the reference's SPH() factory for the B-spline kernel
     (src/SPH/SPHHydros.py:317-384) with compatible energy and grad-h off."""
@@ -107,6 +154,11 @@ def lib():
             "sphgpu_last_timings": (C.c_int, [vp, _dp]),
             "sphgpu_synchronize": (C.c_int, [vp]),
             "sphgpu_copy_probe": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
+            "sphgpu_set_boundaries": (C.c_int, [vp, C.c_int, C.POINTER(Boundary)]),
+            "sphgpu_set_ghost_nodes": (C.c_int, [vp, C.c_double, C.c_int]),
+            "sphgpu_apply_ghost_boundaries": (C.c_int, [vp]),
+            "sphgpu_node_counts": (C.c_int, [vp, _i64p, _i64p]),
+            "sphgpu_download_ghost_controls": (C.c_int, [vp, _i32p, _i32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -199,11 +251,12 @@ class Sim:
         self.n_total = n_total
 
     def upload(self, field, arr):
+        """Upload n_total rows, or n_internal rows once the device made the ghosts."""
         fid = STATE_FIELDS[field]
         a = np.ascontiguousarray(arr, dtype=np.float64)
-        assert a.shape[0] == self.n_total, (field, a.shape, self.n_total)
-        assert a.size == self.n_total * state_components(field, self.dim), (field, a.shape)
-        self._chk(self.L.sphgpu_upload(self.h, fid, _p(a), self.n_total))
+        rows = a.shape[0]
+        assert a.size == rows * state_components(field, self.dim), (field, a.shape)
+        self._chk(self.L.sphgpu_upload(self.h, fid, _p(a), rows))
 
     def upload_state(self, state):
         names = {"pos": "position", "vel": "velocity", "mass": "mass", "rho": "mass density",
@@ -214,11 +267,41 @@ class Sim:
             if k in names and v is not None:
                 self.upload(names[k], v)
 
-    def download(self, field):
+    def download(self, field, rows=None):
         c = state_components(field, self.dim)
-        a = np.zeros((self.n_total, c))
-        self._chk(self.L.sphgpu_download(self.h, STATE_FIELDS[field], _p(a), self.n_total))
+        n = self.n_total if rows is None else rows
+        a = np.zeros((n, c))
+        self._chk(self.L.sphgpu_download(self.h, STATE_FIELDS[field], _p(a), n))
         return a[:, 0] if c == 1 else a
+
+    # ghost boundaries ---------------------------------------------------------
+    def set_boundaries(self, boundaries):
+        arr = (Boundary * max(1, len(boundaries)))(*boundaries)
+        self._chk(self.L.sphgpu_set_boundaries(self.h, len(boundaries), arr))
+        self._sync_counts()
+
+    def set_ghost_nodes(self, kernel_extent=2.0, cull=True):
+        self._chk(self.L.sphgpu_set_ghost_nodes(self.h, kernel_extent, int(cull)))
+        self._sync_counts()
+
+    def apply_ghost_boundaries(self):
+        self._chk(self.L.sphgpu_apply_ghost_boundaries(self.h))
+
+    def _sync_counts(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._chk(self.L.sphgpu_node_counts(self.h, C.byref(a), C.byref(b)))
+        self.n_internal, self.n_total = a.value, b.value
+
+    def node_counts(self):
+        self._sync_counts()
+        return self.n_internal, self.n_total
+
+    def ghost_controls(self):
+        ng = self.n_total - self.n_internal
+        c = np.zeros(max(ng, 1), dtype=np.int32)
+        b = np.zeros(max(ng, 1), dtype=np.int32)
+        self._chk(self.L.sphgpu_download_ghost_controls(self.h, _p(c, _i32p), _p(b, _i32p)))
+        return c[:ng], b[:ng]
 
     # passes -------------------------------------------------------------------
     def set_row_order(self, order):
@@ -262,9 +345,10 @@ class Sim:
         return n.value
 
     def timings(self):
-        t = np.zeros(4)
+        t = np.zeros(6)
         self._chk(self.L.sphgpu_last_timings(self.h, _p(t)))
-        return dict(neighbors_ms=t[0], density_ms=t[1], derivs_ms=t[2], pack_ms=t[3])
+        return dict(neighbors_ms=t[0], density_ms=t[1], derivs_ms=t[2], pack_ms=t[3], ghosts_ms=t[4],
+                    cull_ms=t[5])
 
     def synchronize(self):
         self._chk(self.L.sphgpu_synchronize(self.h))

@@ -1,0 +1,285 @@
+// Test infrastructure: CPU oracle for planar ghost boundaries (FP64).
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// this library; the product (llnl-spheral_amd) never links or calls it.
+//
+// Restated in the reference's floating-point operation order:
+//   * GeomPlane(point, normal): normal.unitVector()        src/Geometry/GeomPlane.cc:41-46
+//     signedDistance / minimumDistance                    src/Geometry/GeomPlane.cc:218-232
+//     closestPointOnPlane (1-D, 2-D, 3-D)                 src/Geometry/GeomPlane.cc:237-259,
+//                                                         src/Utilities/pointDistances.hh:53-57
+//   * mapPositionThroughPlanes                            src/Boundary/mapPositionThroughPlanes.hh:12-26
+//   * PlanarBoundary::setGhostNodes, O(N) branch          src/Boundary/PlanarBoundary.cc:112-182
+//     setGhostNodeIndices / updateGhostNodes              src/Boundary/PlanarBoundary.cc:329-400
+//   * SymTensor::eigenValues (hmax = 1/min eigenvalue)    src/Geometry/GeomSymmetricTensorInline_default.hh:2053-2110
+//   * ReflectingBoundary field maps (I - 2 n n)           src/Boundary/ReflectingBoundary.cc:40-175,
+//                                                         src/Utilities/planarReflectingOperator.hh
+//   * PeriodicBoundary = two PeriodicPlanarBoundary halves (plane1 -> plane2,
+//     then plane2 -> plane1), values copied              src/Boundary/PeriodicBoundary.cc:66-135,
+//                                                         src/Boundary/PeriodicBoundary.cc (PeriodicPlanarBoundary)
+//   * ghost culling and stable renumbering                src/Integrator/Integrator.cc:491-570,
+//                                                         src/Boundary/Boundary.cc:182-233
+// Each boundary sees every node that exists when it runs, including the ghosts
+// of the boundaries before it; its ghosts are appended in control order.
+#include <cstdint>
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include <limits>
+#include "geom.hh"
+
+namespace orc {
+
+struct Plane3 { double point[3]; double normal[3]; };
+
+}  // namespace orc
+
+extern "C" {
+struct orc_boundary {
+  int kind;                      // 0 reflecting (plane1), 1 periodic (plane1, plane2)
+  orc::Plane3 p1, p2;
+};
+}
+
+namespace orc {
+
+template <int D> struct GPlane {
+  Vec<D> p, n;
+};
+
+template <int D> GPlane<D> makePlane(const Plane3& q) {
+  GPlane<D> g;
+  Vec<D> nn;
+  for (int k = 0; k < D; ++k) { g.p.c[k] = q.point[k]; nn.c[k] = q.normal[k]; }
+  g.n = unitVector(nn);
+  return g;
+}
+template <int D> double signedDistance(const GPlane<D>& P, const Vec<D>& x) { return dot(P.n, sub(x, P.p)); }
+template <int D> double minimumDistance(const GPlane<D>& P, const Vec<D>& x) { return std::fabs(signedDistance(P, x)); }
+inline Vec<1> closestPoint(const GPlane<1>& P, const Vec<1>&) { return P.p; }
+inline Vec<2> closestPoint(const GPlane<2>& P, const Vec<2>& x) {
+  const Vec<2> a0p = sub(x, P.p);
+  const Vec<2> dir{{-P.n.c[1], P.n.c[0]}};
+  const double s = dot(a0p, dir);
+  return add(P.p, scale(dir, s));
+}
+inline Vec<3> closestPoint(const GPlane<3>& P, const Vec<3>& x) {
+  return add(x, scale(P.n, dot(sub(P.p, x), P.n)));
+}
+// mapPositionThroughPlanes(position, enterPlane, exitPlane)
+template <int D> Vec<D> mapThrough(const Vec<D>& x, const GPlane<D>& enter, const GPlane<D>& exit) {
+  const Vec<D> p0 = closestPoint(exit, x);
+  const double s = signedDistance(enter, x);
+  return sub(p0, scale(exit.n, s));
+}
+
+// Smallest eigenvalue of a SymTensor, as SymTensor::eigenValues().minElement().
+inline double minEigen(const Sym<1>& H) { return H.xx; }
+inline double minEigen(const Sym<2>& H) {
+  if (std::fabs(H.xy) < 1.0e-50) return std::min(H.xx, H.yy);
+  const double b = H.xx + H.yy;
+  const double c = sdet(H);
+  const double q = 0.5 * (b + sgn(b) * std::sqrt(std::max(0.0, b * b - 4.0 * c)));
+  return std::min(q, c / q);
+}
+inline double minEigen(const Sym<3>& H) {
+  const double onethird = 1.0 / 3.0, sqrt3 = std::sqrt(3.0);
+  const double fscale = std::max(10.0 * std::numeric_limits<double>::epsilon(),
+                                 std::max(std::fabs(H.xx), std::max(std::fabs(H.xy), std::max(std::fabs(H.xz),
+                                          std::max(std::fabs(H.yy), std::max(std::fabs(H.yz), std::fabs(H.zz)))))));
+  const double fscalei = 1.0 / fscale;
+  const double a00 = H.xx * fscalei, a01 = H.xy * fscalei, a02 = H.xz * fscalei;
+  const double a11 = H.yy * fscalei, a12 = H.yz * fscalei, a22 = H.zz * fscalei;
+  const double c0 = a00 * a11 * a22 + 2.0 * a01 * a02 * a12 - a00 * a12 * a12 - a11 * a02 * a02 - a22 * a01 * a01;
+  const double c1 = a00 * a11 - a01 * a01 + a00 * a22 - a02 * a02 + a11 * a22 - a12 * a12;
+  const double c2 = a00 + a11 + a22;
+  const double c2Div3 = c2 * onethird;
+  const double aDiv3 = std::min(0.0, onethird * (c1 - c2 * c2Div3));
+  const double mbDiv2 = 0.5 * (c0 + c2Div3 * (2.0 * c2Div3 * c2Div3 - c1));
+  const double q = std::min(0.0, mbDiv2 * mbDiv2 + aDiv3 * aDiv3 * aDiv3);
+  const double mag = std::sqrt(-aDiv3);
+  const double angle = std::atan2(std::sqrt(-q), mbDiv2) * onethird;
+  const double cs = std::cos(angle), sn = std::sin(angle);
+  const double e0 = fscale * (c2Div3 + 2.0 * mag * cs);
+  const double e1 = fscale * (c2Div3 - mag * (cs + sqrt3 * sn));
+  const double e2 = fscale * (c2Div3 - mag * (cs - sqrt3 * sn));
+  return std::min(e0, std::min(e1, e2));
+}
+
+// Planar sub-boundary: enter / exit planes and whether fields are reflected.
+template <int D> struct Sub {
+  GPlane<D> enter, exit;
+  bool reflect;
+  Ten<D> R;
+};
+template <int D> std::vector<Sub<D>> expand(int nb, const orc_boundary* b) {
+  std::vector<Sub<D>> out;
+  for (int k = 0; k < nb; ++k) {
+    const GPlane<D> p1 = makePlane<D>(b[k].p1);
+    if (b[k].kind == 0) {
+      Sub<D> s{p1, p1, true, tzero<D>()};
+      // Tensor::one - 2.0*normal.selfdyad()
+      Sym<D> nn = selfdyad(p1.n);
+      const double* q = sptr(nn);
+      int c = 0;
+      for (int i = 0; i < D; ++i)
+        for (int j = i; j < D; ++j, ++c) {
+          const double v = 2.0 * q[c];
+          s.R.c[i * D + j] = (i == j ? 1.0 : 0.0) - v;
+          s.R.c[j * D + i] = (i == j ? 1.0 : 0.0) - v;
+        }
+      out.push_back(s);
+    } else {
+      const GPlane<D> p2 = makePlane<D>(b[k].p2);
+      out.push_back(Sub<D>{p1, p2, false, tzero<D>()});
+      out.push_back(Sub<D>{p2, p1, false, tzero<D>()});
+    }
+  }
+  return out;
+}
+
+template <int D> Vec<D> tdotv(const Ten<D>& R, const Vec<D>& v) {
+  Vec<D> o;
+  for (int i = 0; i < D; ++i) {
+    double s = R.c[i * D] * v.c[0];
+    for (int k = 1; k < D; ++k) s = s + R.c[i * D + k] * v.c[k];
+    o.c[i] = s;
+  }
+  return o;
+}
+// SymTensor.Tensor (GeomSymmetricTensorInline_default.hh:1687-1697 and lower dims)
+template <int D> Ten<D> sdott(const Sym<D>& H, const Ten<D>& T) {
+  Ten<D> full;
+  const double* h = sptr(H);
+  int c = 0;
+  for (int i = 0; i < D; ++i)
+    for (int j = i; j < D; ++j, ++c) { full.c[i * D + j] = h[c]; full.c[j * D + i] = h[c]; }
+  return tdot(full, T);
+}
+// (R*(H*R)).Symmetric()
+template <int D> Sym<D> reflectSym(const Ten<D>& R, const Sym<D>& H) {
+  const Ten<D> t = tdot(R, sdott(H, R));
+  Sym<D> o;
+  double* p = sptr(o);
+  int c = 0;
+  for (int i = 0; i < D; ++i)
+    for (int j = i; j < D; ++j, ++c) p[c] = (i == j) ? t.c[i * D + i] : 0.5 * (t.c[i * D + j] + t.c[j * D + i]);
+  return o;
+}
+
+template <int D>
+int64_t setGhosts(int64_t nInt, double* pos, double* H, int nb, const orc_boundary* b, double ext, int64_t maxGhost,
+                  int32_t* control, int32_t* subOf, int32_t* subStart) {
+  constexpr int NS = nsym<D>();
+  const auto subs = expand<D>(nb, b);
+  int64_t n = nInt;
+  auto X = [&](int64_t i) { Vec<D> v; for (int k = 0; k < D; ++k) v.c[k] = pos[i * D + k]; return v; };
+  auto HH = [&](int64_t i) { Sym<D> s; double* p = sptr(s); for (int k = 0; k < NS; ++k) p[k] = H[i * NS + k]; return s; };
+  for (size_t si = 0; si < subs.size(); ++si) {
+    const Sub<D>& S = subs[si];
+    subStart[si] = int32_t(n - nInt);
+    double hmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const Vec<D> ri = X(i);
+      const double hmaxi = 1.0 / minEigen(HH(i));
+      if (hmaxi > hmax && std::min(minimumDistance(S.exit, ri), minimumDistance(S.enter, ri)) < ext * hmaxi) hmax = hmaxi;
+    }
+    std::vector<int64_t> ctrl;
+    if (hmax > 0.0) {
+      for (int64_t i = 0; i < n; ++i) {
+        const double disti = signedDistance(S.exit, X(i)) / hmax;
+        if (disti >= 0.0 && disti <= ext) ctrl.push_back(i);
+      }
+    }
+    if (n - nInt + int64_t(ctrl.size()) > maxGhost) return -1;   // later boundaries need these rows
+    for (size_t k = 0; k < ctrl.size(); ++k) {
+      const int64_t c = ctrl[k], g = n + int64_t(k);
+      // updateGhostNodes: mapPosition(x, exitPlane, enterPlane)
+      const Vec<D> xg = mapThrough(X(c), S.exit, S.enter);
+      for (int d = 0; d < D; ++d) pos[g * D + d] = xg.c[d];
+      const Sym<D> hg = S.reflect ? reflectSym(S.R, HH(c)) : HH(c);
+      const double* hp = sptr(hg);
+      for (int d = 0; d < NS; ++d) H[g * NS + d] = hp[d];
+      control[g - nInt] = int32_t(c);
+      subOf[g - nInt] = int32_t(si);
+    }
+    n += int64_t(ctrl.size());
+  }
+  subStart[subs.size()] = int32_t(n - nInt);
+  return n - nInt;
+}
+
+// field kinds: 0 scalar (copied), 1 Vector, 2 SymTensor
+template <int D>
+void applyGhosts(int64_t nInt, int64_t nTot, int nb, const orc_boundary* b, const int32_t* control,
+                 const int32_t* subOf, int kind, double* f) {
+  constexpr int NS = nsym<D>();
+  const auto subs = expand<D>(nb, b);
+  for (int64_t g = nInt; g < nTot; ++g) {
+    const int64_t c = control[g - nInt];
+    const Sub<D>& S = subs[size_t(subOf[g - nInt])];
+    if (kind == 0) {
+      f[g] = f[c];
+    } else if (kind == 1) {
+      Vec<D> v; for (int d = 0; d < D; ++d) v.c[d] = f[c * D + d];
+      const Vec<D> o = S.reflect ? tdotv(S.R, v) : v;
+      for (int d = 0; d < D; ++d) f[g * D + d] = o.c[d];
+    } else {
+      Sym<D> h; double* hp = sptr(h);
+      for (int d = 0; d < NS; ++d) hp[d] = f[c * NS + d];
+      const Sym<D> o = S.reflect ? reflectSym(S.R, h) : h;
+      const double* op = sptr(o);
+      for (int d = 0; d < NS; ++d) f[g * NS + d] = op[d];
+    }
+  }
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Integrator::setGhostNodes without culling: pos / H hold n_internal rows on
+// entry and room for max_ghost more; ghost rows are appended.  control[g] and
+// sub_of[g] give each ghost's control node and planar sub-boundary, sub_start
+// (n_sub+1 entries) the first ghost of each sub-boundary.  Returns the number
+// of ghosts, or -1 when max_ghost rows are not enough.
+int64_t orc_set_ghosts(int dim, int64_t nInt, double* pos, double* H, int nb, const orc_boundary* b, double ext,
+                       int64_t maxGhost, int32_t* control, int32_t* subOf, int32_t* subStart) {
+  if (dim == 1) return orc::setGhosts<1>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart);
+  if (dim == 2) return orc::setGhosts<2>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart);
+  return orc::setGhosts<3>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart);
+}
+
+// SymTensor::eigenValues().minElement() of one packed SymTensor.
+double orc_min_eigen(int dim, const double* h) {
+  if (dim == 1) return orc::minEigen(orc::Sym<1>{h[0]});
+  if (dim == 2) return orc::minEigen(orc::Sym<2>{h[0], h[1], h[2]});
+  return orc::minEigen(orc::Sym<3>{h[0], h[1], h[2], h[3], h[4], h[5]});
+}
+
+// Boundary::applyGhostBoundary for one field, every ghost in order.
+void orc_apply_ghosts(int dim, int64_t nInt, int64_t nTot, int nb, const orc_boundary* b, const int32_t* control,
+                      const int32_t* subOf, int kind, double* f) {
+  if (dim == 1) orc::applyGhosts<1>(nInt, nTot, nb, b, control, subOf, kind, f);
+  else if (dim == 2) orc::applyGhosts<2>(nInt, nTot, nb, b, control, subOf, kind, f);
+  else orc::applyGhosts<3>(nInt, nTot, nb, b, control, subOf, kind, f);
+}
+
+// Ghost culling flags and the old -> new index map (Integrator::setGhostNodes
+// culling branch): internal nodes, every neighbour of an internal node, and
+// every ghost that is the control of another ghost are kept; the kept nodes
+// keep their relative order.  Returns the new node count.
+int64_t orc_cull_map(int64_t nInt, int64_t nTot, const int64_t* off, const int32_t* nbr, const int32_t* control,
+                     int32_t* old2new) {
+  std::vector<char> keep(size_t(nTot), 0);
+  for (int64_t i = 0; i < nInt; ++i) {
+    keep[size_t(i)] = 1;
+    for (int64_t e = off[i]; e < off[i + 1]; ++e) keep[size_t(nbr[e])] = 1;
+  }
+  for (int64_t g = nInt; g < nTot; ++g)
+    if (control[g - nInt] >= nInt) keep[size_t(control[g - nInt])] = 1;
+  int64_t m = 0;
+  for (int64_t i = 0; i < nTot; ++i) old2new[i] = keep[size_t(i)] ? int32_t(m++) : -1;
+  return m;
+}
+
+}  // extern "C"

@@ -43,6 +43,47 @@ OUTPUT_FIELDS = ["rhoSum", "normalization", "DxDt", "DrhoDt", "DvDt", "DepsDt", 
                  "weightedNeighborSum", "massSecondMoment"]
 
 
+class Plane(C.Structure):
+    _fields_ = [("point", C.c_double * 3), ("normal", C.c_double * 3)]
+
+
+class Boundary(C.Structure):
+    """kind 0: ReflectingBoundary(p1); kind 1: PeriodicBoundary(p1, p2)."""
+    _fields_ = [("kind", C.c_int), ("p1", Plane), ("p2", Plane)]
+
+
+def plane(point, normal):
+    pt = list(point) + [0.0] * (3 - len(point))
+    nm = list(normal) + [0.0] * (3 - len(normal))
+    return Plane((C.c_double * 3)(*pt), (C.c_double * 3)(*nm))
+
+
+def reflecting(point, normal):
+    return Boundary(0, plane(point, normal), plane(point, normal))
+
+
+def periodic(point1, normal1, point2, normal2):
+    return Boundary(1, plane(point1, normal1), plane(point2, normal2))
+
+
+def box_boundaries(dim, lo=0.0, hi=1.0, kind="reflecting"):
+    """Planes at lo and hi on every axis, inward normals, in x, y, z order."""
+    out = []
+    for a in range(dim):
+        e = [0.0] * dim
+        e[a] = 1.0
+        p_lo = [0.0] * dim
+        p_lo[a] = lo
+        p_hi = [0.0] * dim
+        p_hi[a] = hi
+        if kind == "reflecting":
+            out.append(reflecting(p_lo, e))
+            out.append(reflecting(p_hi, [-x for x in e]))
+        else:
+            out.append(periodic(p_lo, e, p_hi, [-x for x in e]))
+    return out
+
+
 class Outputs(C.Structure):
     _fields_ = [(n, _dp) for n in OUTPUT_FIELDS] + [("ncalc", _i64p)]
 
@@ -90,6 +131,15 @@ def lib(fast=False):
         L.orc_gamma_law.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     _dp, _dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_ghosts.restype = C.c_int64
+        L.orc_set_ghosts.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int, C.POINTER(Boundary), C.c_double,
+                                     C.c_int64, _i32p, _i32p, _i32p]
+        L.orc_apply_ghosts.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int, C.POINTER(Boundary), _i32p, _i32p,
+                                       C.c_int, _dp]
+        L.orc_min_eigen.restype = C.c_double
+        L.orc_min_eigen.argtypes = [C.c_int, _dp]
+        L.orc_cull_map.restype = C.c_int64
+        L.orc_cull_map.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _i32p, _i32p]
         _libs[name] = L
     return _libs[name]
 
@@ -208,3 +258,95 @@ def gamma_law(rho, eps, gamma, Pmin=-1e200, Pmax=1e200, floor=False):
 
 def num_threads(fast=True):
     return lib(fast).orc_num_threads()
+
+
+def _barr(boundaries):
+    arr = (Boundary * max(1, len(boundaries)))(*boundaries)
+    return arr
+
+
+def n_sub(boundaries):
+    return sum(1 if b.kind == 0 else 2 for b in boundaries)
+
+
+def set_ghosts(dim, pos, H, boundaries, ext=2.0):
+    """PlanarBoundary::setGhostNodes for each boundary in order over internal
+    rows pos/H.  Returns (pos_all, H_all, control, sub_of, sub_start)."""
+    L = lib()
+    nsym = dim * (dim + 1) // 2
+    n = pos.shape[0]
+    cap = max(64, n)
+    barr = _barr(boundaries)
+    while True:
+        P = np.zeros((n + cap, dim))
+        Hh = np.zeros((n + cap, nsym))
+        P[:n] = pos
+        Hh[:n] = H
+        control = np.zeros(cap, dtype=np.int32)
+        sub_of = np.zeros(cap, dtype=np.int32)
+        sub_start = np.zeros(n_sub(boundaries) + 1, dtype=np.int32)
+        ng = L.orc_set_ghosts(dim, n, _p(P), _p(Hh), len(boundaries), barr, ext, cap, _p(control, _i32p),
+                              _p(sub_of, _i32p), _p(sub_start, _i32p))
+        if ng >= 0:
+            return P[:n + ng], Hh[:n + ng], control[:ng], sub_of[:ng], sub_start
+        cap *= 2
+
+
+FIELD_KIND = {"pos": 1, "vel": 1, "H": 2}
+
+
+def apply_ghosts(dim, n_internal, boundaries, control, sub_of, field, kind):
+    """Boundary::applyGhostBoundary over every ghost row of `field` (in place).
+    kind 0 scalar, 1 Vector, 2 SymTensor."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    lib().orc_apply_ghosts(dim, n_internal, f.shape[0], len(boundaries), _barr(boundaries),
+                           _p(np.ascontiguousarray(control, dtype=np.int32), _i32p),
+                           _p(np.ascontiguousarray(sub_of, dtype=np.int32), _i32p), kind, _p(f))
+    return f
+
+
+def min_eigen(dim, h):
+    """SymTensor::eigenValues().minElement() of a packed SymTensor."""
+    a = np.ascontiguousarray(h, dtype=np.float64)
+    return lib().orc_min_eigen(dim, _p(a))
+
+
+def cull_map(n_internal, n_total, off, nbr, control):
+    old2new = np.zeros(n_total, dtype=np.int32)
+    m = lib().orc_cull_map(n_internal, n_total, _p(off, _i64p), _p(nbr, _i32p),
+                           _p(np.ascontiguousarray(control, dtype=np.int32), _i32p), _p(old2new, _i32p))
+    return int(m), old2new
+
+
+STATE_KINDS = {"vel": 1, "mass": 0, "rho": 0, "eps": 0, "P": 0, "cs": 0, "omega": 0,
+               "fCl": 0, "fCq": 0, "fshear": 0}
+
+
+def ghost_pipeline(dim, state, boundaries, ext=2.0, cull=True, fast=False):
+    """Integrator::setGhostNodes + ConnectivityMap + culling + applyGhostBoundaries
+    on internal-node state.  Returns (state_all, off, nbr, info) with ghosts
+    appended after the internal rows and rows in the culled numbering."""
+    n = state["pos"].shape[0]
+    pos, H, control, sub_of, sub_start = set_ghosts(dim, state["pos"], state["H"], boundaries, ext)
+    off, nbr = connectivity(dim, n, pos, H, ext, fast=fast)
+    out = {"pos": pos, "H": H}
+    for k, kind in STATE_KINDS.items():
+        if state.get(k) is None:
+            continue
+        a = state[k]
+        full = np.zeros((pos.shape[0],) + a.shape[1:])
+        full[:n] = a
+        out[k] = apply_ghosts(dim, n, boundaries, control, sub_of, full, kind)
+    info = {"n_ghost_generated": pos.shape[0] - n, "control": control, "sub_of": sub_of,
+            "sub_start": sub_start}
+    if cull:
+        m, old2new = cull_map(n, pos.shape[0], off, nbr, control)
+        keep = old2new >= 0
+        for k in list(out):
+            out[k] = out[k][keep]
+        gk = keep[n:]
+        info["control"] = old2new[control[gk]]
+        info["sub_of"] = sub_of[gk]
+        info["old2new"] = old2new
+        nbr = old2new[nbr]
+    return out, off, nbr, info
