@@ -141,6 +141,7 @@ def main():
         sim.set_boundaries(sphgpu.box_boundaries(3))
     prec = sphgpu.PRECISION_FP64 if args.precision == "fp64" else sphgpu.PRECISION_MIXED
     params = sphgpu.hydro_params(precision=prec)
+    sim.set_precision(prec)
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s N={N}", file=sys.stderr, flush=True)
 
     def step():

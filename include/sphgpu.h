@@ -86,8 +86,9 @@ enum {
 /* Arithmetic of the pair pass. */
 enum {
   SPHGPU_PRECISION_FP64 = 0,  /* FP64 throughout, reference summation order  */
-  SPHGPU_PRECISION_MIXED = 1  /* FP32 pair kinematics and kernel lookups,
-                                 FP64 positions and compensated accumulators */
+  SPHGPU_PRECISION_MIXED = 1  /* FP32 pair kinematics and kernel lookups, FP64
+                                 positions and per-node accumulators; needs
+                                 H = h I, no Q multipliers, no tensile term */
 };
 
 /* Options of SPHHydroBase (src/SPH/SPHHydroBase.hh:53-71, defaults of the SPH()
@@ -206,8 +207,14 @@ int sphgpu_neighbor_count(sphgpu_sim* sim, int64_t* n_entries);
  * (ConnectivityMap::connectivityForNode, ConnectivityMap.hh:67-79). */
 int sphgpu_download_neighbors(sphgpu_sim* sim, int64_t* offsets, int32_t* neighbors);
 
+/* Arithmetic of sphgpu_sum_density (SPHGPU_PRECISION_*, default FP64).  No
+ * counterpart in the reference, which is FP64 only; evaluate_derivatives takes
+ * its precision from sphgpu_hydro_params. */
+int sphgpu_set_precision(sphgpu_sim* sim, int precision);
+
 /* computeSPHSumMassDensity (src/SPH/computeSPHSumMassDensity.cc:25-107) over the
- * current neighbours; overwrites the resident "mass density" of internal nodes. */
+ * current neighbours; overwrites the resident "mass density" of internal nodes.
+ * MIXED: FP32 kernel lookups, FP64 positions and sums (isotropic H only). */
 int sphgpu_sum_density(sphgpu_sim* sim);
 
 /* SPHHydroBase::evaluateDerivatives (src/SPH/SPHHydroBase.cc:471-928) with

@@ -67,6 +67,13 @@ void installTable(Sim& s) {
   s.dNperh.ensure(sizeof(double) * n);
   s.dWsum.ensure(sizeof(double) * n);
   SG_CHECK_HIP(hipMemcpy(s.dTable.p, inter.data(), sizeof(double) * inter.size(), hipMemcpyHostToDevice));
+  std::vector<float> t32(size_t(8) * n, 0.0f);
+  for (int i = 0; i < n; ++i) {
+    t32[8 * i + 0] = float(s.table.Ak[i]); t32[8 * i + 1] = float(s.table.Bk[i]); t32[8 * i + 2] = float(s.table.Ck[i]);
+    t32[8 * i + 4] = float(s.table.Ag[i]); t32[8 * i + 5] = float(s.table.Bg[i]); t32[8 * i + 6] = float(s.table.Cg[i]);
+  }
+  s.dTable32.ensure(sizeof(float) * t32.size());
+  SG_CHECK_HIP(hipMemcpy(s.dTable32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
   SG_CHECK_HIP(hipMemcpy(s.dNperh.p, s.table.nperh.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   SG_CHECK_HIP(hipMemcpy(s.dWsum.p, s.table.wsum.data(), sizeof(double) * n, hipMemcpyHostToDevice));
 }
@@ -320,6 +327,14 @@ int sphgpu_download_neighbors(sphgpu_sim* s, int64_t* offsets, int32_t* nbr) {
       w += len;
     }
     offsets[s->nInt] = w;
+  });
+}
+
+int sphgpu_set_precision(sphgpu_sim* s, int precision) {
+  return guarded(s, [&] {
+    SG_REQUIRE(precision == SPHGPU_PRECISION_FP64 || precision == SPHGPU_PRECISION_MIXED, SPHGPU_ERR_ARG,
+               "unknown precision");
+    s->precision = precision;
   });
 }
 

@@ -88,6 +88,7 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
 // Table: per point [Ak Bk Ck Ag Bg Cg].
 struct TabArgs {
   const double* g;     // global copy
+  const float* g32;    // FP32 copy, 8 floats per point
   int n;
   double step, extent;
   const double* nperh;
@@ -297,14 +298,115 @@ __device__ __forceinline__ double* outRow(const OutArgs& o, int field, int64_t k
   return o.base + int64_t(o.off[field]) * o.n + k * comps;
 }
 
+// Per-node accumulators of the pair loop (FP64 in both precisions).
+template <int D> struct Acc {
+  double rhoSum, norm, DepsDt, maxVP, effVP, viscW, xsphW, wNS;
+  V<D> DvDt, xsphDV;
+  T<D> DvDx, M;
+  S<D> m2nd;
+  __device__ __forceinline__ void zero() {
+    rhoSum = norm = DepsDt = maxVP = effVP = viscW = xsphW = wNS = 0.0;
+    DvDt = vzero<D>(); xsphDV = vzero<D>(); DvDx = tzero<D>(); M = tzero<D>(); m2nd = szero<D>();
+  }
+};
+
+// Own-node values the epilogue reads.
+template <int D> struct Self {
+  V<D> v;
+  S<D> H;
+  double Hdet, m, rho, mor;
+};
+
+// Per-node epilogue (SPHHydroBase.cc:834-902) and the output stores.
+template <int D>
+__device__ __forceinline__ void epilogue(int64_t k, int len, const Self<D>& me, Acc<D>& a, const TabArgs& t,
+                                         const PassArgs& p, const OutArgs& o) {
+  constexpr int NS = NSym<D>::value;
+  const double tiny = 1.0e-30;
+  const V<D> vi = me.v;
+  const S<D> Hi = me.H;
+  const double Hdeti = me.Hdet, mi = me.m, rhoi = me.rho, mori = me.mor;
+  double rhoSum = a.rhoSum, norm = a.norm, DepsDt = a.DepsDt, xsphW = a.xsphW;
+  V<D> DvDt = a.DvDt, xsphDV = a.xsphDV;
+  T<D> DvDx = a.DvDx, M = a.M;
+  S<D> m2nd = a.m2nd;
+  rhoSum += mi * p.W0 * Hdeti;
+  norm += mori * p.W0 * Hdeti;
+  T<D> Mout = M;
+  if (p.correctVelocityGradient && fabs(tdet(M)) > 1.0e-10 && double(len) > pownu2<D>()) {
+    Mout = tinv(M);
+    DvDx = tdot(DvDx, Mout);
+  } else {
+    tdivEq(DvDx, rhoi);
+  }
+  const double trace = ttrace(DvDx);
+  const double DrhoDt = -rhoi * trace;
+  if (p.evolveTotalEnergy) DepsDt = mi * (vdot(vi, DvDt) + DepsDt);
+  const double wNSo = rootnu<D>(dmax(0.0, a.wNS / Hdeti));
+  m2nd = sdivs(m2nd, Hdeti * Hdeti);
+  V<D> DxDt = vi;
+  if (p.XSPH) {
+    xsphW += Hdeti * mi / rhoi * p.W0;
+    const V<D> corr = vdivs(xsphDV, dmax(tiny, xsphW));
+    for (int c = 0; c < D; ++c) DxDt.c[c] = vi.c[c] + corr.c[c];
+  }
+  // SPHSmoothingScale::smoothingScaleDerivative: -H/nu * tr(DvDx)
+  S<D> negH;
+  for (int c = 0; c < NS; ++c) negH.c[c] = -Hi.c[c];
+  S<D> DHDt = sdivs(negH, double(D));
+  for (int c = 0; c < NS; ++c) DHDt.c[c] = DHDt.c[c] * trace;
+  // SPHSmoothingScale::idealSmoothingScale
+  double nperhNow;
+  if (fuzzyEqual(wNSo, 0.0)) nperhNow = 0.5 * p.nPerh;
+  else nperhNow = equivalentNperh(t, wNSo);
+  const double s = dmin(4.0, dmax(0.25, p.nPerh / (nperhNow + 1.0e-30)));
+  const double aa = (s < 1.0) ? 0.4 * (1.0 + s * s) : 0.4 * (1.0 + 1.0 / (s * s * s));
+  const double hi0 = 1.0 / Hi.c[0];
+  const double hi1 = dmin(p.hmax, dmax(p.hmin, hi0 * (1.0 - aa + aa * s)));
+  const double hinv = 1.0 / hi1;
+
+  // outputs
+  *outRow(o, SPHGPU_D_RHO_SUM, k, 1) = rhoSum;
+  *outRow(o, SPHGPU_D_NORMALIZATION, k, 1) = norm;
+  *outRow(o, SPHGPU_D_DRHODT, k, 1) = DrhoDt;
+  *outRow(o, SPHGPU_D_DEPSDT, k, 1) = DepsDt;
+  *outRow(o, SPHGPU_D_MAX_VISCOUS_PRESSURE, k, 1) = a.maxVP;
+  *outRow(o, SPHGPU_D_EFF_VISCOUS_PRESSURE, k, 1) = a.effVP;
+  *outRow(o, SPHGPU_D_VISCOUS_WORK, k, 1) = a.viscW;
+  *outRow(o, SPHGPU_D_XSPH_WEIGHT_SUM, k, 1) = xsphW;
+  *outRow(o, SPHGPU_D_WEIGHTED_NEIGHBOR_SUM, k, 1) = wNSo;
+  {
+    double* dx = outRow(o, SPHGPU_D_DXDT, k, D);
+    double* dv = outRow(o, SPHGPU_D_DVDT, k, D);
+    double* xv = outRow(o, SPHGPU_D_XSPH_DELTA_V, k, D);
+    for (int c = 0; c < D; ++c) { dx[c] = DxDt.c[c]; dv[c] = DvDt.c[c]; xv[c] = xsphDV.c[c]; }
+  }
+  {
+    double* g = outRow(o, SPHGPU_D_DVDX, k, D * D);
+    double* gl = outRow(o, SPHGPU_D_LOCAL_DVDX, k, D * D);
+    double* m = outRow(o, SPHGPU_D_M, k, D * D);
+    double* ml = outRow(o, SPHGPU_D_LOCAL_M, k, D * D);
+    for (int c = 0; c < D * D; ++c) { g[c] = DvDx.c[c]; gl[c] = DvDx.c[c]; m[c] = Mout.c[c]; ml[c] = Mout.c[c]; }
+  }
+  {
+    double* dh = outRow(o, SPHGPU_D_DHDT, k, NS);
+    double* hid = outRow(o, SPHGPU_D_HIDEAL, k, NS);
+    double* m2 = outRow(o, SPHGPU_D_MASS_SECOND_MOMENT, k, NS);
+    int s2 = 0;
+    for (int a2 = 0; a2 < D; ++a2)
+      for (int b2 = a2; b2 < D; ++b2, ++s2) {
+        dh[s2] = DHDt.c[s2];
+        hid[s2] = (a2 == b2 ? 1.0 : 0.0) * hinv;
+        m2[s2] = m2nd.c[s2];
+      }
+  }
+}
+
 template <int D, bool ISO, bool TENSILE>
 __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, const double* __restrict__ rec,
                                                    const int32_t* __restrict__ rowCount, const int32_t* __restrict__ nbr,
                                                    int64_t n, const double* __restrict__ tab, const TabArgs& t,
                                                    const PassArgs& p, const OutArgs& o) {
-  constexpr int NS = NSym<D>::value;
-  const double tiny = 1.0e-30;
-
   const Nbr<D, ISO> me = loadNbr<D, ISO, true>(rec, int32_t(k));
   const V<D> ri = me.x, vi = me.v;
   const S<D> Hi = me.H;
@@ -407,76 +509,12 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, const double* __re
   }
 
   // per-node epilogue (SPHHydroBase.cc:834-902)
-  rhoSum += mi * p.W0 * Hdeti;
-  norm += mori * p.W0 * Hdeti;
-  T<D> Mout = M;
-  if (p.correctVelocityGradient && fabs(tdet(M)) > 1.0e-10 && double(len) > pownu2<D>()) {
-    Mout = tinv(M);
-    DvDx = tdot(DvDx, Mout);
-  } else {
-    tdivEq(DvDx, rhoi);
-  }
-  const double trace = ttrace(DvDx);
-  const double DrhoDt = -rhoi * trace;
-  if (p.evolveTotalEnergy) DepsDt = mi * (vdot(vi, DvDt) + DepsDt);
-  const double wNSo = rootnu<D>(dmax(0.0, wNS / Hdeti));
-  m2nd = sdivs(m2nd, Hdeti * Hdeti);
-  V<D> DxDt = vi;
-  if (p.XSPH) {
-    xsphW += Hdeti * mi / rhoi * p.W0;
-    const V<D> corr = vdivs(xsphDV, dmax(tiny, xsphW));
-    for (int c = 0; c < D; ++c) DxDt.c[c] = vi.c[c] + corr.c[c];
-  }
-  // SPHSmoothingScale::smoothingScaleDerivative: -H/nu * tr(DvDx)
-  S<D> negH;
-  for (int c = 0; c < NS; ++c) negH.c[c] = -Hi.c[c];
-  S<D> DHDt = sdivs(negH, double(D));
-  for (int c = 0; c < NS; ++c) DHDt.c[c] = DHDt.c[c] * trace;
-  // SPHSmoothingScale::idealSmoothingScale
-  double nperhNow;
-  if (fuzzyEqual(wNSo, 0.0)) nperhNow = 0.5 * p.nPerh;
-  else nperhNow = equivalentNperh(t, wNSo);
-  const double s = dmin(4.0, dmax(0.25, p.nPerh / (nperhNow + 1.0e-30)));
-  const double a = (s < 1.0) ? 0.4 * (1.0 + s * s) : 0.4 * (1.0 + 1.0 / (s * s * s));
-  const double hi0 = 1.0 / Hi.c[0];
-  const double hi1 = dmin(p.hmax, dmax(p.hmin, hi0 * (1.0 - a + a * s)));
-  const double hinv = 1.0 / hi1;
-
-  // outputs
-  *outRow(o, SPHGPU_D_RHO_SUM, k, 1) = rhoSum;
-  *outRow(o, SPHGPU_D_NORMALIZATION, k, 1) = norm;
-  *outRow(o, SPHGPU_D_DRHODT, k, 1) = DrhoDt;
-  *outRow(o, SPHGPU_D_DEPSDT, k, 1) = DepsDt;
-  *outRow(o, SPHGPU_D_MAX_VISCOUS_PRESSURE, k, 1) = maxVP;
-  *outRow(o, SPHGPU_D_EFF_VISCOUS_PRESSURE, k, 1) = effVP;
-  *outRow(o, SPHGPU_D_VISCOUS_WORK, k, 1) = viscW;
-  *outRow(o, SPHGPU_D_XSPH_WEIGHT_SUM, k, 1) = xsphW;
-  *outRow(o, SPHGPU_D_WEIGHTED_NEIGHBOR_SUM, k, 1) = wNSo;
-  {
-    double* dx = outRow(o, SPHGPU_D_DXDT, k, D);
-    double* dv = outRow(o, SPHGPU_D_DVDT, k, D);
-    double* xv = outRow(o, SPHGPU_D_XSPH_DELTA_V, k, D);
-    for (int c = 0; c < D; ++c) { dx[c] = DxDt.c[c]; dv[c] = DvDt.c[c]; xv[c] = xsphDV.c[c]; }
-  }
-  {
-    double* g = outRow(o, SPHGPU_D_DVDX, k, D * D);
-    double* gl = outRow(o, SPHGPU_D_LOCAL_DVDX, k, D * D);
-    double* m = outRow(o, SPHGPU_D_M, k, D * D);
-    double* ml = outRow(o, SPHGPU_D_LOCAL_M, k, D * D);
-    for (int c = 0; c < D * D; ++c) { g[c] = DvDx.c[c]; gl[c] = DvDx.c[c]; m[c] = Mout.c[c]; ml[c] = Mout.c[c]; }
-  }
-  {
-    double* dh = outRow(o, SPHGPU_D_DHDT, k, NS);
-    double* hid = outRow(o, SPHGPU_D_HIDEAL, k, NS);
-    double* m2 = outRow(o, SPHGPU_D_MASS_SECOND_MOMENT, k, NS);
-    int s2 = 0;
-    for (int a2 = 0; a2 < D; ++a2)
-      for (int b2 = a2; b2 < D; ++b2, ++s2) {
-        dh[s2] = DHDt.c[s2];
-        hid[s2] = (a2 == b2 ? 1.0 : 0.0) * hinv;
-        m2[s2] = m2nd.c[s2];
-      }
-  }
+  Acc<D> a;
+  a.rhoSum = rhoSum; a.norm = norm; a.DepsDt = DepsDt; a.maxVP = maxVP; a.effVP = effVP; a.viscW = viscW;
+  a.xsphW = xsphW; a.wNS = wNS; a.DvDt = DvDt; a.xsphDV = xsphDV; a.DvDx = DvDx; a.M = M; a.m2nd = m2nd;
+  Self<D> self;
+  self.v = vi; self.H = Hi; self.Hdet = Hdeti; self.m = mi; self.rho = rhoi; self.mor = mori;
+  epilogue<D>(k, len, self, a, t, p, o);
   return pairs;
 }
 
@@ -494,9 +532,266 @@ __global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict
   if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(ncalc, (unsigned long long)pairs);
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision (isotropic H): FP64 positions and per-node accumulators,
+// FP32 pair kinematics, kernel lookups and pair terms.
+//
+// With H = h I both unit vectors of a pair are r_hat = r_ij/|r_ij| and
+// grad W = r_hat * h * dW/deta * det H, so the pair body reduces to scalars
+// along r_hat: the acceleration increment is r_hat * m_j * (g_i (P_i/rho_i^2 +
+// Q_i/2) + g_j (P_j/rho_j^2 + Q_j/2)) with g = h W' det H.  The two ends of a
+// pair compute the same scalar from the same FP32 inputs (r_ij is the negated
+// FP64 difference, r_hat flips sign exactly), so the pair forces are
+// antisymmetric up to the final product with the partner mass.
+//
+// Neighbour record, 64 B (16 floats):
+//   [0..5] x, y, z (FP64)  [6] 1/h  [7] det H (negative for ghost nodes)
+//   [8] m  [9] 1/rho  [10..12] v  [13] c  [14] P/rho^2/omega  [15] m/rho
+// ---------------------------------------------------------------------------
+struct MNbr {
+  double x[3];
+  float hinv, hdet, m, invRho, v[3], cs, prho, mor;
+  bool ghost;
+};
+__device__ __forceinline__ void loadM(const float4* __restrict__ recM, int32_t j, MNbr& o) {
+  const float4* p = recM + int64_t(j) * 4;
+  const float4 f0 = p[0], f1 = p[1], f2 = p[2], f3 = p[3];
+  o.x[0] = __hiloint2double(__float_as_int(f0.y), __float_as_int(f0.x));
+  o.x[1] = __hiloint2double(__float_as_int(f0.w), __float_as_int(f0.z));
+  o.x[2] = __hiloint2double(__float_as_int(f1.y), __float_as_int(f1.x));
+  o.hinv = f1.z;
+  o.ghost = f1.w < 0.0f;
+  o.hdet = fabsf(f1.w);
+  o.m = f2.x; o.invRho = f2.y; o.v[0] = f2.z; o.v[1] = f2.w;
+  o.v[2] = f3.x; o.cs = f3.y; o.prho = f3.z; o.mor = f3.w;
+}
+
+template <int D>
+__global__ void k_pack_mixed(const double* __restrict__ rec, int64_t n, int64_t nInt, float4* __restrict__ recM) {
+  using R = RL<true>;
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k >= n) return;
+  const double* r = rec + k * R::STRIDE;
+  const double x0 = r[R::X], x1 = D > 1 ? r[R::X + 1] : 0.0, x2 = D > 2 ? r[R::X + 2] : 0.0;
+  const bool ghost = r[R::ORIG] >= double(nInt);
+  const float hdet = float(r[R::HDET]);
+  float4* p = recM + k * 4;
+  p[0] = make_float4(__int_as_float(__double2loint(x0)), __int_as_float(__double2hiint(x0)),
+                     __int_as_float(__double2loint(x1)), __int_as_float(__double2hiint(x1)));
+  p[1] = make_float4(__int_as_float(__double2loint(x2)), __int_as_float(__double2hiint(x2)), float(r[R::HINV]),
+                     ghost ? -hdet : hdet);
+  const double rho = r[R::RHO];
+  p[2] = make_float4(float(r[R::M]), rho != 0.0 ? float(1.0 / rho) : 0.0f, float(r[R::V]),
+                     D > 1 ? float(r[R::V + 1]) : 0.0f);
+  p[3] = make_float4(D > 2 ? float(r[R::V + 2]) : 0.0f, float(r[R::CS]), float(r[R::PRHO]), float(r[R::MOR]));
+}
+
+// FP32 table in LDS: 8 floats per point.
+template <bool LDS> __device__ __forceinline__ const float* stageTable32(float* lds, const TabArgs& t) {
+  if constexpr (LDS) {
+    for (int q = threadIdx.x; q < 8 * t.n; q += blockDim.x) lds[q] = t.g32[q];
+    __syncthreads();
+    return lds;
+  } else {
+    return t.g32;
+  }
+}
+struct Tab32 {
+  float invStep, extent;
+  int n;
+};
+__device__ __forceinline__ void lookupWG32(const float* __restrict__ tab, const Tab32& t, float eta, float hdet,
+                                           float& W, float& gW) {
+  if (eta < t.extent) {
+    const float q = eta * t.invStep;
+    int lb = int(q);
+    lb = lb < t.n - 1 ? lb : t.n - 1;
+    const int i0 = lb < t.n - 3 ? lb : t.n - 3;
+    const float x = q - float(i0);
+    const float4 a = *reinterpret_cast<const float4*>(tab + 8 * (i0 + 1));
+    const float4 g = *reinterpret_cast<const float4*>(tab + 8 * (i0 + 1) + 4);
+    W = hdet * __builtin_fmaf(__builtin_fmaf(a.z, x, a.y), x, a.x);
+    gW = hdet * __builtin_fmaf(__builtin_fmaf(g.z, x, g.y), x, g.x);
+  } else {
+    W = 0.0f; gW = 0.0f;
+  }
+}
+__device__ __forceinline__ float lookupW32(const float* __restrict__ tab, const Tab32& t, float eta, float hdet) {
+  if (!(eta < t.extent)) return 0.0f;
+  const float q = eta * t.invStep;
+  int lb = int(q);
+  lb = lb < t.n - 1 ? lb : t.n - 1;
+  const int i0 = lb < t.n - 3 ? lb : t.n - 3;
+  const float x = q - float(i0);
+  const float4 a = *reinterpret_cast<const float4*>(tab + 8 * (i0 + 1));
+  return hdet * __builtin_fmaf(__builtin_fmaf(a.z, x, a.y), x, a.x);
+}
+__device__ __forceinline__ float muQ32(float mu, float Cl, float Cq, float cs, int linear, int quadratic) {
+  const float m0 = fminf(0.0f, mu);
+  const float lin = linear ? mu : m0;
+  const float quad = quadratic ? -(mu < 0.0f ? -1.0f : 1.0f) * mu * mu : m0 * m0;
+  return -Cl * cs * lin + Cq * quad;
+}
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <int D, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict__ rec, const float4* __restrict__ recM,
+                                                              const int32_t* __restrict__ rowCount,
+                                                              const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
+                                                              const int32_t* __restrict__ work, TabArgs t, Tab32 t32,
+                                                              double W0, double* __restrict__ rhoOut) {
+  using R = RL<true>;
+  extern __shared__ float lds32[];
+  const float* tab = stageTable32<LDS>(lds32, t);
+  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  if (tw >= nInt) return;
+  const int64_t k = work[tw];
+  double* ri_ = rec + k * R::STRIDE;
+  double xi[3] = {ri_[R::X], D > 1 ? ri_[R::X + 1] : 0.0, D > 2 ? ri_[R::X + 2] : 0.0};
+  const double mi = ri_[R::M], hdeti = ri_[R::HDET];
+  double acc = mi * (W0 * hdeti);
+  const int len = rowCount[k];
+  const int4* nb4 = reinterpret_cast<const int4*>(nbr);
+  int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
+  int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
+  MNbr nx;
+  if (len > 0) loadM(recM, cur.x, nx);
+  for (int w = 0; w < len; ++w) {
+    const MNbr nb = nx;
+    if (w + 1 < len) loadM(recM, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nx);
+    float r2 = 0.0f;
+    for (int c = 0; c < D; ++c) {
+      const float rc = float(xi[c] - nb.x[c]);
+      r2 = __builtin_fmaf(rc, rc, r2);
+    }
+    const float eta = nb.hinv * __builtin_sqrtf(r2);
+    acc += double(nb.m * lookupW32(tab, t32, eta, nb.hdet));
+  }
+  rhoOut[int64_t(ri_[R::ORIG])] = acc;
+  const double som = ri_[R::SOM], p = ri_[R::P];
+  ri_[R::RHO] = acc;
+  ri_[R::PRHO] = som * p / (acc * acc);
+  ri_[R::MOR] = mi / acc;
+}
+
+template <int D, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_derivs_mixed(const double* __restrict__ rec, const float4* __restrict__ recM,
+                                                         const int32_t* __restrict__ rowCount,
+                                                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
+                                                         const int32_t* __restrict__ work, TabArgs t, Tab32 t32,
+                                                         PassArgs p, OutArgs o, unsigned long long* __restrict__ ncalc) {
+  using R = RL<true>;
+  extern __shared__ float lds32[];
+  const float* tab = stageTable32<LDS>(lds32, t);
+  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  unsigned pairs = 0;
+  if (tw < nInt) {
+    const int64_t k = work[tw];
+    const double* ri_ = rec + k * R::STRIDE;
+    double xi[3];
+    float vi[3];
+    Self<D> me;
+    for (int c = 0; c < 3; ++c) {
+      xi[c] = c < D ? ri_[R::X + c] : 0.0;
+      vi[c] = c < D ? float(ri_[R::V + c]) : 0.0f;
+    }
+    for (int c = 0; c < D; ++c) me.v.c[c] = ri_[R::V + c];
+    const double hinvD = ri_[RL<true>::HINV];
+    for (int c = 0; c < NSym<D>::value; ++c) me.H.c[c] = 0.0;
+    for (int c = 0; c < D; ++c) me.H.c[sdiag<D>(c)] = hinvD;
+    me.Hdet = ri_[R::HDET]; me.m = ri_[R::M]; me.rho = ri_[R::RHO]; me.mor = ri_[R::MOR];
+    const float hi = float(hinvD), hdeti = float(me.Hdet), ci = float(ri_[R::CS]);
+    const float invRhoi = float(1.0 / me.rho), rhoi2 = float(me.rho * me.rho);
+    const float Prhoi = float(ri_[R::PRHO]), mori = float(me.mor);
+    const float Cl = float(p.Cl), Cq = float(p.Cq), eps2 = float(p.eps2);
+
+    Acc<D> a;
+    a.zero();
+    float maxVP = 0.0f;
+    const int len = rowCount[k];
+    const int4* nb4 = reinterpret_cast<const int4*>(nbr);
+    int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
+    int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
+    MNbr nx;
+    int32_t jn = cur.x;
+    if (len > 0) loadM(recM, jn, nx);
+    for (int w = 0; w < len; ++w) {
+      const MNbr nb = nx;
+      const int32_t j = jn;
+      if (w + 1 < len) {
+        jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
+        loadM(recM, jn, nx);
+      }
+      pairs += (nb.ghost || j > int32_t(k)) ? 1u : 0u;
+      float rij[3], vij[3];
+      float r2 = 0.0f, vdotr = 0.0f;
+      for (int c = 0; c < D; ++c) {
+        rij[c] = float(xi[c] - nb.x[c]);
+        vij[c] = vi[c] - nb.v[c];
+        r2 = __builtin_fmaf(rij[c], rij[c], r2);
+        vdotr = __builtin_fmaf(vij[c], rij[c], vdotr);
+      }
+      const float rinv = r2 > 1.0e-30f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+      const float rmag = r2 * rinv;
+      const float hj = nb.hinv;
+      float Wi, gWi, Wj, gWj;
+      lookupWG32(tab, t32, hi * rmag, hdeti, Wi, gWi);
+      lookupWG32(tab, t32, hj * rmag, nb.hdet, Wj, gWj);
+      const float gi = hi * gWi, gj = hj * gWj;     // grad W = r_hat * g
+      // Monaghan-Gingold Q: mu = v_ij . eta / (eta^2 + eps2), eta = h r_ij
+      const float mui = hi * vdotr * rcp32(__builtin_fmaf(hi * hi, r2, eps2));
+      const float muj = hj * vdotr * rcp32(__builtin_fmaf(hj * hj, r2, eps2));
+      const float QPi = muQ32(mui, Cl, Cq, ci, p.linearInExpansion, p.quadraticInExpansion) * invRhoi;
+      const float QPj = muQ32(muj, Cl, Cq, nb.cs, p.linearInExpansion, p.quadraticInExpansion) * nb.invRho;
+      const float Qi = rhoi2 * fabsf(QPi);
+      maxVP = fmaxf(maxVP, Qi);
+      const float vdr = vdotr * rinv;                // v_ij . r_hat
+      const float halfQi = 0.5f * QPi * gi;
+      const float pi = __builtin_fmaf(gi, Prhoi, halfQi);
+      const float pj = __builtin_fmaf(gj, nb.prho, 0.5f * QPj * gj);
+      const float mj = nb.m;
+      const float fsr = mj * (pi + pj) * rinv;       // acceleration along r_ij
+      for (int c = 0; c < D; ++c) a.DvDt.c[c] -= double(rij[c] * fsr);
+      a.DepsDt += double(mj * vdr * pi);
+      a.viscW += double(mj * vdr * halfQi);
+      a.effVP += double(nb.mor * Qi * Wi);
+      a.rhoSum += double(mj * Wi);
+      a.norm += double(mori * Wi);
+      a.wNS += double(fabsf(gWi));
+      const float wg = mj * gi * rinv;              // grad W_i * m_j = r_ij * wg
+      for (int c = 0; c < D; ++c) {
+        const float u = vij[c] * wg;
+        for (int b = 0; b < D; ++b) a.DvDx.c[c * D + b] -= double(u * rij[b]);
+      }
+      const float f2 = gi * gi * rcp32(fmaxf(1.0e-30f, r2 * r2 * r2));
+      {
+        int s2 = 0;
+        for (int c = 0; c < D; ++c) {
+          const float rw = rij[c] * wg, rf = rij[c] * f2;
+          for (int b = c; b < D; ++b, ++s2) {
+            a.M.c[c * D + b] -= double(rw * rij[b]);
+            a.m2nd.c[s2] += double(rf * rij[b]);
+          }
+        }
+      }
+      if (p.XSPH) {
+        const float w2 = 0.5f * __builtin_fmaf(mori, Wi, nb.mor * Wj);
+        a.xsphW += double(w2);
+        for (int c = 0; c < D; ++c) a.xsphDV.c[c] -= double(vij[c] * w2);
+      }
+    }
+    for (int c = 0; c < D; ++c)
+      for (int b = 0; b < c; ++b) a.M.c[c * D + b] = a.M.c[b * D + c];
+    a.maxVP = double(maxVP);
+    epilogue<D>(k, len, me, a, t, p, o);
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) pairs += __shfl_xor(pairs, sh);
+  if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(ncalc, (unsigned long long)pairs);
+}
+
 TabArgs tabArgs(const Sim& s) {
-  return TabArgs{s.dTable.as<double>(), s.table.n, s.table.step, s.table.extent, s.dNperh.as<double>(),
-                 s.dWsum.as<double>()};
+  return TabArgs{s.dTable.as<double>(), s.dTable32.as<float>(), s.table.n, s.table.step, s.table.extent,
+                 s.dNperh.as<double>(), s.dWsum.as<double>()};
 }
 
 template <int D>
@@ -517,6 +812,16 @@ void packD(Sim& s) {
   SG_CHECK_HIP(hipGetLastError());
 }
 
+Tab32 tab32(const Sim& s) { return Tab32{float(1.0 / s.table.step), float(s.table.extent), s.table.n}; }
+
+template <int D>
+void packMixed(Sim& s) {
+  s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(s.nSlots, 1)));
+  hipLaunchKernelGGL(k_pack_mixed<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), s.nSlots,
+                     s.nInt, s.recM.as<float4>());
+  SG_CHECK_HIP(hipGetLastError());
+}
+
 template <int D>
 void sumDensityD(Sim& s) {
   const int64_t n = s.nSlots;
@@ -524,6 +829,22 @@ void sumDensityD(Sim& s) {
   const bool lds = s.table.n <= kLdsTablePoints;
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
+  if (s.precision == SPHGPU_PRECISION_MIXED) {
+    SG_REQUIRE(s.isotropic, SPHGPU_ERR_UNSUPPORTED, "mixed precision needs H = h I and no Q multipliers");
+    packMixed<D>(s);
+    const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
+    const double W0 = s.table.kernelValue(0.0, 1.0);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), lds32, s.stream, s.rec.as<double>(),
+                         s.recM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
+                         s.work.as<int32_t>(), t, tab32(s), W0, s.rho.as<double>());
+    };
+    if (lds) go(k_sum_density_mixed<D, true>); else go(k_sum_density_mixed<D, false>);
+    SG_CHECK_HIP(hipGetLastError());
+    applyGhostDensity(s);
+    SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
+    return;
+  }
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
@@ -567,17 +888,31 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   o.n = n;
   for (int f = 0; f < SPHGPU_NUM_DERIV_FIELDS; ++f) o.off[f] = derivOffsetDoubles(s.dim, f);
   const bool tensile = hp.epsTensile != 0.0;
-  auto launch = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
-                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t, p, o,
-                       dncalc);
-  };
-  if (lds) {
-    if (s.isotropic) { if (tensile) launch(k_derivs_fp64<D, true, true, true>); else launch(k_derivs_fp64<D, true, false, true>); }
-    else { if (tensile) launch(k_derivs_fp64<D, false, true, true>); else launch(k_derivs_fp64<D, false, false, true>); }
+  if (hp.precision == SPHGPU_PRECISION_MIXED) {
+    SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
+               "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
+    packMixed<D>(s);
+    SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
+    const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), lds32, st, s.rec.as<double>(),
+                         s.recM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
+                         s.work.as<int32_t>(), t, tab32(s), p, o, dncalc);
+    };
+    if (lds) go(k_derivs_mixed<D, true>); else go(k_derivs_mixed<D, false>);
   } else {
-    if (s.isotropic) { if (tensile) launch(k_derivs_fp64<D, true, true, false>); else launch(k_derivs_fp64<D, true, false, false>); }
-    else { if (tensile) launch(k_derivs_fp64<D, false, true, false>); else launch(k_derivs_fp64<D, false, false, false>); }
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
+                         s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t, p, o,
+                         dncalc);
+    };
+    if (lds) {
+      if (s.isotropic) { if (tensile) launch(k_derivs_fp64<D, true, true, true>); else launch(k_derivs_fp64<D, true, false, true>); }
+      else { if (tensile) launch(k_derivs_fp64<D, false, true, true>); else launch(k_derivs_fp64<D, false, false, true>); }
+    } else {
+      if (s.isotropic) { if (tensile) launch(k_derivs_fp64<D, true, true, false>); else launch(k_derivs_fp64<D, true, false, false>); }
+      else { if (tensile) launch(k_derivs_fp64<D, false, true, false>); else launch(k_derivs_fp64<D, false, false, false>); }
+    }
   }
   SG_CHECK_HIP(hipGetLastError());
   SG_CHECK_HIP(hipEventRecord(s.ev[6], st));
@@ -615,7 +950,8 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   SG_REQUIRE(p.compatibleEnergyEvolution == 0, SPHGPU_ERR_UNSUPPORTED,
              "compatibleEnergyEvolution (pairAccelerations) is not produced by the device pass");
-  SG_REQUIRE(p.precision == SPHGPU_PRECISION_FP64, SPHGPU_ERR_UNSUPPORTED, "precision mode not available");
+  SG_REQUIRE(p.precision == SPHGPU_PRECISION_FP64 || p.precision == SPHGPU_PRECISION_MIXED, SPHGPU_ERR_ARG,
+             "unknown precision");
   SG_REQUIRE(p.nPerh > 0.0, SPHGPU_ERR_ARG, "nPerh must be positive");
   s.derivsValid = false;
   if (s.dim == 1) evalDerivsD<1>(s, p);

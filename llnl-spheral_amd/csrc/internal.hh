@@ -100,6 +100,8 @@ struct Sim {
 
   TableHost table;
   DBuf dTable;              // 6*n interleaved [Ak Bk Ck Ag Bg Cg] per point
+  DBuf dTable32;            // FP32 copy, 8 floats per point [Ak Bk Ck 0 Ag Bg Cg 0]
+  int precision = SPHGPU_PRECISION_FP64;   // of sum density (evaluate_derivatives takes its own)
   DBuf dNperh, dWsum;
 
   // node state, caller order
@@ -127,6 +129,7 @@ struct Sim {
   // packed per-node records (sorted order) and outputs (sorted order)
   DBuf rec;
   bool recValid = false;    // records match the caller-order state
+  DBuf recM;                // mixed-precision neighbour records (64 B per slot)
   DBuf out;                 // all derivative fields, field-major, sorted order
   int64_t outStride = 0;
   bool derivsValid = false;

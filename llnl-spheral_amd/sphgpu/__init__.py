@@ -148,6 +148,7 @@ def lib():
             "sphgpu_neighbor_count": (C.c_int, [vp, _i64p]),
             "sphgpu_download_neighbors": (C.c_int, [vp, _i64p, _i32p]),
             "sphgpu_sum_density": (C.c_int, [vp]),
+            "sphgpu_set_precision": (C.c_int, [vp, C.c_int]),
             "sphgpu_evaluate_derivatives": (C.c_int, [vp, C.POINTER(HydroParams)]),
             "sphgpu_download_derivative": (C.c_int, [vp, C.c_int, _dp, C.c_int64]),
             "sphgpu_pair_count": (C.c_int, [vp, _i64p]),
@@ -321,6 +322,10 @@ class Sim:
         nbr = np.zeros(max(total, 1), dtype=np.int32)
         self._chk(self.L.sphgpu_download_neighbors(self.h, _p(off, _i64p), _p(nbr, _i32p)))
         return off, nbr[:total]
+
+    def set_precision(self, precision):
+        """Arithmetic of sum_density (evaluate_derivatives takes params.precision)."""
+        self._chk(self.L.sphgpu_set_precision(self.h, precision))
 
     def sum_density(self):
         self._chk(self.L.sphgpu_sum_density(self.h))

@@ -117,3 +117,40 @@ def linear_velocity(dim, n, A, seed=14892042, jitter=0.2, nPerh=1.51):
     P, cs = gamma_law(rho, eps, 5.0 / 3.0)
     return dict(pos=pos, vel=vel, mass=mass, rho=rho, eps=eps, P=P, cs=cs, H=H,
                 omega=np.ones(N)), dict(n_internal=N, dx=dx, nPerh=nPerh)
+
+
+def gaussian_velocity_field(n, dim=3, sigma=0.1, slope=-2.0, seed=2024):
+    """Seeded Gaussian random velocity field on the n^dim lattice: white noise
+    shaped in Fourier space to a power spectrum ~ k^slope (|v_k| ~ k^(slope/2)),
+    zero mean, each component scaled to RMS sigma."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n) * n
+    grids = np.meshgrid(*([k1] * dim), indexing="ij")
+    kmag = np.sqrt(sum(g * g for g in grids))
+    amp = np.zeros_like(kmag)
+    nz = kmag > 0
+    amp[nz] = kmag[nz] ** (0.5 * slope)
+    vel = np.zeros((n ** dim, dim))
+    for a in range(dim):
+        noise = rng.standard_normal([n] * dim)
+        f = np.fft.ifftn(np.fft.fftn(noise) * amp).real
+        f -= f.mean()
+        f *= sigma / max(f.std(), 1e-300)
+        # lattice() numbers nodes x-fastest: index = ix + n*iy + n^2*iz
+        vel[:, a] = f.transpose(*reversed(range(dim))).reshape(-1)
+    return vel
+
+
+def periodic_sinusoid(n, dim=3, amp=0.1, sigma=0.1, gamma=1.4, u=1.0, seed=2024, nPerh=1.51):
+    """Periodic-box workload: unit lattice, rho = 1 + amp * prod sin(2 pi x_a),
+    masses m = rho(x) dx^dim, Gaussian random velocity (sigma, k^-2), gamma-law
+    gas with specific energy u."""
+    pos, mass, H, dx = lattice(dim, n, nPerh=nPerh)
+    N = pos.shape[0]
+    rho = 1.0 + amp * np.prod(np.sin(2.0 * np.pi * pos), axis=1)
+    mass = rho * dx ** dim
+    eps = np.full(N, u)
+    vel = gaussian_velocity_field(n, dim, sigma=sigma, seed=seed)
+    P, cs = gamma_law(rho, eps, gamma)
+    return dict(pos=pos, vel=vel, mass=mass, rho=rho, eps=eps, P=P, cs=cs, H=H,
+                omega=np.ones(N)), dict(n_internal=N, dx=dx, gamma=gamma, nPerh=nPerh)
