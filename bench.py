@@ -32,19 +32,22 @@ IN_BYTES_PER_NODE = 136      # x(3) v(3) m rho P c omega H(6), FP64
 OUT_BYTES_PER_NODE = 576     # the 72 doubles of the derivative fields
 NBR_BYTES_PER_ENTRY = 4      # one int32 neighbour index per directed entry
 # HBM bytes per pair-kernel dispatch, from calibrated FETCH_SIZE / WRITE_SIZE runs
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_derivs_fp64_converging_flow_128_traffic.json")
+PROFILES = os.path.join(ROOT, "profiles")
+VALU_ISSUE_HZ = 1024 * 2.4e9 / 4     # wavefront instructions per second: 1024 SIMDs, 4 cycles each
 
 
-def measured_traffic(n, precision, row_order, boundaries):
-    """HBM bytes per launch of the pair kernel as the counter runs recorded
-    them, for the workload they were taken on; None for any other workload."""
-    if not os.path.exists(TRAFFIC_FILE):
-        return None, None
-    with open(TRAFFIC_FILE) as f:
-        t = json.load(f)
-    if t.get("key") != [n, precision, row_order, boundaries]:
-        return None, None
-    return t["traffic_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
+def measured_counters(n, precision, row_order, boundaries):
+    """Counter data of the pair kernel recorded for this workload (HBM traffic
+    per launch, VALU wavefront instructions per launch), or None."""
+    key = [n, precision, row_order, boundaries]
+    for name in sorted(os.listdir(PROFILES)) if os.path.isdir(PROFILES) else []:
+        if not name.endswith("_traffic.json"):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            t = json.load(f)
+        if t.get("key") == key:
+            return t, os.path.join("profiles", name)
+    return None, None
 
 
 def cpu_info():
@@ -173,7 +176,9 @@ def main():
     n_tot = sim.node_counts()[1]
     alg_bytes = N * OUT_BYTES_PER_NODE + n_tot * IN_BYTES_PER_NODE + NBR_BYTES_PER_ENTRY * entries
     achieved = alg_bytes / (derivs_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(args.n, args.precision, args.row_order, args.boundaries)
+    counters, counters_src = measured_counters(args.n, args.precision, args.row_order, args.boundaries)
+    traffic = counters["traffic_bytes"] if counters else None
+    kernel = "k_derivs_fp64" if args.precision == "fp64" else "k_derivs_mixed"
     line = {
         "metric": METRIC,
         "value": pairs * args.steps / wall,
@@ -200,14 +205,18 @@ def main():
                          "derivatives": stage[2], "pack": stage[3]},
         },
         "roofline": {
-            "kernel": "k_derivs_fp64 (evaluateDerivatives pair loop + epilogue)",
+            "kernel": f"{kernel} (evaluateDerivatives pair loop + epilogue)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "traffic_source": traffic_src,
+            "traffic_source": counters_src,
             "algorithmic_bytes": alg_bytes,
-            "note": "FP64 VALU-bound kernel; HBM fraction reported from algorithmic bytes",
+            "note": "the pair kernel is VALU-issue bound, not HBM bound: see valu",
         },
     }
+    if counters and counters.get("valu_wave_instructions"):
+        issue_ms = 1e3 * counters["valu_wave_instructions"] / VALU_ISSUE_HZ
+        line["roofline"]["valu"] = {"wave_instructions": counters["valu_wave_instructions"], "issue_ms": issue_ms,
+                                    "kernel_ms": derivs_ms, "busy_frac": issue_ms / derivs_ms}
     if args.probe_mib > 0:
         nb = args.probe_mib << 20
         ms = sim.copy_probe(nb, 5)
