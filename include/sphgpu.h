@@ -228,6 +228,45 @@ int sphgpu_download_derivative(sphgpu_sim* sim, int field, double* host, int64_t
  * per-node ncalc summed, src/SPH/SPHHydroBase.cc:599, 674). */
 int sphgpu_pair_count(sphgpu_sim* sim, int64_t* n_pairs);
 
+/* Time integration.  Options of SynchronousRK2 / Integrator
+ * (src/Integrator/Integrator.cc:44-62: dtMin 0, dtMax FLT_MAX, dtGrowth 2),
+ * GenericHydro (cfl 0.25, useVelocityMagnitudeForDt 0; src/SPH/SPHHydros.py:17-47),
+ * GammaLawGas (src/Material/GammaLawGas.cc:163-223, pressure limits -1e200 /
+ * 1e200, src/Material/MaterialEquationsOfState.py:20-30) and the FluidNodeList
+ * density limits (rhoMin 1e-10, rhoMax 1e10, src/NodeList/FluidNodeLists.py:12-31). */
+typedef struct sphgpu_integrator_params {
+  double cfl;
+  double dtMin, dtMax, dtGrowth;
+  double gamma;                /* GammaLawGas gamma                                   */
+  double Pmin, Pmax;           /* applyPressureLimits bounds                         */
+  int pressureFloor;           /* 1: P < Pmin -> Pmin (PressureFloor), 0: -> 0       */
+  int cullGhostNodes;          /* Integrator::cullGhostNodes (1)                     */
+  double rhoMin, rhoMax;       /* IncrementBoundedState limits of the mass density   */
+  double kernelExtent;         /* TableKernel extent (2)                             */
+  int useVelocityMagnitudeForDt;
+} sphgpu_integrator_params;
+
+/* GammaLawGas::setPressure / setSoundSpeed on the internal rows from the
+ * resident mass density and specific thermal energy
+ * (src/Material/GammaLawGas.cc:163-223, EquationOfStateInline.hh:71-78). */
+int sphgpu_eos_gamma_law(sphgpu_sim* sim, double gamma, double Pmin, double Pmax, int pressureFloor);
+/* GenericHydro::dt (src/Physics/GenericHydro.cc:114-331): cfl times the
+ * smallest sound-speed, viscous, divergence, pairwise-velocity and
+ * acceleration limit over the internal nodes, with the current neighbours and
+ * the derivatives of the last step (zero before the first). */
+int sphgpu_hydro_dt(sphgpu_sim* sim, double nPerh, double cfl, int useVelocityMagnitudeForDt, double* dt);
+/* SynchronousRK2::step (src/Integrator/SynchronousRK2.cc:78-144) with the
+ * SPH state policies (src/SPH/SPHHydroBase.cc:238-336): ghosts and
+ * neighbours at the step start, dt selection, derivatives at t, update to
+ * t + dt/2, ghosts, derivatives at t + dt/2, update from the saved state,
+ * ghosts, sum density (SPHHydroBase::finalize), boundary enforcement.
+ * Advances the sim's time; returns the step taken in *dt. */
+int sphgpu_rk2_step(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, const sphgpu_integrator_params* integ,
+                    double max_time, double* dt);
+/* Integrator clock: current time, last step, cycle count (and setter). */
+int sphgpu_get_time(sphgpu_sim* sim, double* time, double* last_dt, int64_t* cycle);
+int sphgpu_set_time(sphgpu_sim* sim, double time, double last_dt, int64_t cycle);
+
 /* Device times in milliseconds of the last call of each stage, measured with
  * HIP events on the sim's stream: [neighbours, sum density, derivatives, pack,
  * ghost generation, ghost culling]. */

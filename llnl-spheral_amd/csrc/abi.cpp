@@ -283,6 +283,39 @@ int sphgpu_download_ghost_controls(sphgpu_sim* s, int32_t* control, int32_t* sub
   });
 }
 
+int sphgpu_eos_gamma_law(sphgpu_sim* s, double gamma, double Pmin, double Pmax, int floorType) {
+  return guarded(s, [&] { eosGammaLaw(*s, gamma, Pmin, Pmax, floorType); });
+}
+
+int sphgpu_hydro_dt(sphgpu_sim* s, double nPerh, double cfl, int useVelMag, double* dt) {
+  return guarded(s, [&] {
+    SG_REQUIRE(dt, SPHGPU_ERR_ARG, "null output");
+    *dt = hydroDt(*s, nPerh, cfl, useVelMag);
+  });
+}
+
+int sphgpu_rk2_step(sphgpu_sim* s, const sphgpu_hydro_params* hp, const sphgpu_integrator_params* ip, double maxTime,
+                    double* dt) {
+  return guarded(s, [&] {
+    SG_REQUIRE(hp && ip && dt, SPHGPU_ERR_ARG, "null argument");
+    *dt = rk2Step(*s, *hp, *ip, maxTime);
+  });
+}
+
+int sphgpu_get_time(sphgpu_sim* s, double* t, double* lastDt, int64_t* cycle) {
+  return guarded(s, [&] {
+    SG_REQUIRE(t && lastDt && cycle, SPHGPU_ERR_ARG, "null output");
+    *t = s->time; *lastDt = s->lastDt; *cycle = s->cycle;
+  });
+}
+
+int sphgpu_set_time(sphgpu_sim* s, double t, double lastDt, int64_t cycle) {
+  return guarded(s, [&] {
+    SG_REQUIRE(lastDt > 0.0 && cycle >= 0, SPHGPU_ERR_ARG, "bad clock");
+    s->time = t; s->lastDt = lastDt; s->cycle = cycle;
+  });
+}
+
 int sphgpu_set_row_order(sphgpu_sim* s, int order) {
   return guarded(s, [&] {
     SG_REQUIRE(order == SPHGPU_ROW_ORDER_REFERENCE || order == SPHGPU_ROW_ORDER_SEARCH, SPHGPU_ERR_ARG,

@@ -36,6 +36,7 @@
 #include "dgeom.hh"
 #include "records.hh"
 #include "ell.hh"
+#include "eigen.hh"
 #include <cmath>
 #include <algorithm>
 
@@ -105,42 +106,6 @@ template <int D> __device__ __forceinline__ V<D> ghostPosition(const DSub& b, co
   V<D> o;
   for (int k = 0; k < D; ++k) o.c[k] = p0.c[k] - b.enterN[k] * s;
   return o;
-}
-
-// SymTensor::eigenValues().minElement()
-template <int D> __device__ double minEigen(const double* h) {
-  if constexpr (D == 1) {
-    return h[0];
-  } else if constexpr (D == 2) {
-    if (dabs(h[1]) < 1.0e-50) return dmin(h[0], h[2]);
-    const double b = h[0] + h[2];
-    const double c = h[0] * h[2] - h[1] * h[1];
-    const double q = 0.5 * (b + (b < 0.0 ? -1.0 : 1.0) * sqrt(dmax(0.0, b * b - 4.0 * c)));
-    return dmin(q, c / q);
-  } else {
-    const double onethird = 1.0 / 3.0;
-    const double sqrt3 = 1.7320508075688772;   // std::sqrt(3.0), correctly rounded
-    const double fscale = dmax(10.0 * 2.220446049250313e-16,
-                               dmax(dabs(h[0]), dmax(dabs(h[1]), dmax(dabs(h[2]), dmax(dabs(h[3]),
-                                    dmax(dabs(h[4]), dabs(h[5])))))));
-    const double fscalei = 1.0 / fscale;
-    const double a00 = h[0] * fscalei, a01 = h[1] * fscalei, a02 = h[2] * fscalei;
-    const double a11 = h[3] * fscalei, a12 = h[4] * fscalei, a22 = h[5] * fscalei;
-    const double c0 = a00 * a11 * a22 + 2.0 * a01 * a02 * a12 - a00 * a12 * a12 - a11 * a02 * a02 - a22 * a01 * a01;
-    const double c1 = a00 * a11 - a01 * a01 + a00 * a22 - a02 * a02 + a11 * a22 - a12 * a12;
-    const double c2 = a00 + a11 + a22;
-    const double c2Div3 = c2 * onethird;
-    const double aDiv3 = dmin(0.0, onethird * (c1 - c2 * c2Div3));
-    const double mbDiv2 = 0.5 * (c0 + c2Div3 * (2.0 * c2Div3 * c2Div3 - c1));
-    const double q = dmin(0.0, mbDiv2 * mbDiv2 + aDiv3 * aDiv3 * aDiv3);
-    const double mag = sqrt(-aDiv3);
-    const double angle = atan2(sqrt(-q), mbDiv2) * onethird;
-    const double cs = cos(angle), sn = sin(angle);
-    const double e0 = fscale * (c2Div3 + 2.0 * mag * cs);
-    const double e1 = fscale * (c2Div3 - mag * (cs + sqrt3 * sn));
-    const double e2 = fscale * (c2Div3 - mag * (cs - sqrt3 * sn));
-    return dmin(e0, dmin(e1, e2));
-  }
 }
 
 // R.v, rows left to right (GeomTensorInline_default.hh:1918-1922)
@@ -290,6 +255,70 @@ __global__ void k_ghost_records(double* __restrict__ rec, const int32_t* __restr
   r[R::RHO] = acc;
   r[R::PRHO] = som * p / (acc * acc);
   r[R::MOR] = m / acc;
+}
+
+// ---- boundary enforcement ---------------------------------------------------
+// position < plane (GeomPlane::compare == 1): n.(x - p) < 0 and not within
+// fuzzyEqual of 0 (src/Geometry/GeomPlane.cc:306-341).
+template <int D> __device__ __forceinline__ bool behind(const double* P, const double* N, const V<D>& x) {
+  const double d = signedDist<D>(P, N, x);
+  return !fuzzyEqual(d, 0.0) && !(d > 0.0);
+}
+
+// PlanarBoundary::setViolationNodes / updateViolationNodes over the internal
+// nodes, with ReflectingBoundary::enforceBoundary of v and H
+// ((R*H)*R).Symmetric(), src/Boundary/ReflectingBoundary.cc:240-282).
+template <int D>
+__global__ void k_enforce(double* __restrict__ pos, double* __restrict__ vel, double* __restrict__ H, int64_t nInt,
+                          DSub b) {
+  constexpr int NS = NSym<D>::value;
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= nInt) return;
+  const V<D> x = loadV<D>(pos, i);
+  if (!behind<D>(b.enterP, b.enterN, x)) return;
+  // mapPosition(x, enterPlane, exitPlane)
+  const V<D> p0 = closestPoint<D>(b.exitP, b.exitN, x);
+  const double sd = signedDist<D>(b.enterP, b.enterN, x);
+  for (int k = 0; k < D; ++k) pos[i * D + k] = p0.c[k] - b.exitN[k] * sd;
+  if (b.reflect) {
+    const V<D> v = reflectV<D>(b.R, loadV<D>(vel, i));
+    for (int k = 0; k < D; ++k) vel[i * D + k] = v.c[k];
+    double full[D * D], t1[D * D], t2[D * D];
+    int c = 0;
+    for (int a = 0; a < D; ++a)
+      for (int q = a; q < D; ++q, ++c) { full[a * D + q] = H[i * NS + c]; full[q * D + a] = H[i * NS + c]; }
+    for (int a = 0; a < D; ++a)
+      for (int q = 0; q < D; ++q) {
+        double sum = b.R[a * D] * full[q];
+        for (int k = 1; k < D; ++k) sum = sum + b.R[a * D + k] * full[k * D + q];
+        t1[a * D + q] = sum;
+      }
+    for (int a = 0; a < D; ++a)
+      for (int q = 0; q < D; ++q) {
+        double sum = t1[a * D] * b.R[q];
+        for (int k = 1; k < D; ++k) sum = sum + t1[a * D + k] * b.R[k * D + q];
+        t2[a * D + q] = sum;
+      }
+    c = 0;
+    for (int a = 0; a < D; ++a)
+      for (int q = a; q < D; ++q, ++c) H[i * NS + c] = (a == q) ? t2[a * D + a] : 0.5 * (t2[a * D + q] + t2[q * D + a]);
+  }
+}
+
+// PeriodicBoundary::updateViolationNodes: nodes still outside either plane
+// are wrapped by the fmod rule (src/Boundary/PeriodicBoundary.cc:176-207).
+template <int D>
+__global__ void k_periodic_wrap(double* __restrict__ pos, int64_t nInt, DSub b) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= nInt) return;
+  const V<D> x = loadV<D>(pos, i);
+  // plane1 = enter, plane2 = exit of the first half of the pair
+  if (!(behind<D>(b.enterP, b.enterN, x) || behind<D>(b.exitP, b.exitN, x))) return;
+  const V<D> p2 = arr<D>(b.exitP);
+  const double L = dabs(signedDist<D>(b.enterP, b.enterN, p2));
+  const double s = vdot(vsub(arr<D>(b.enterP), x), arr<D>(b.enterN));
+  const double f = dmax(-1.0, dmin(1.0, fmod(signedDist<D>(b.enterP, b.enterN, x) / L, 1.0)));
+  for (int k = 0; k < D; ++k) pos[i * D + k] = (x.c[k] + b.enterN[k] * s) + b.enterN[k] * (f * L);
 }
 
 // ---- culling ---------------------------------------------------------------
@@ -513,6 +542,7 @@ void setBoundaries(Sim& s, int n, const sphgpu_boundary* b) {
     if (b[q].kind == SPHGPU_BOUNDARY_REFLECTING) {
       for (int k = 0; k < 3; ++k) { a.enterP[k] = a.exitP[k] = p1[k]; a.enterN[k] = a.exitN[k] = n1[k]; }
       a.reflect = 1;
+      a.periodicSecond = 0;
       // Tensor::one - 2.0*normal.selfdyad()
       for (int i = 0; i < D; ++i)
         for (int j = 0; j < D; ++j) a.R[i * D + j] = (i == j ? 1.0 : 0.0) - 2.0 * (n1[std::min(i, j)] * n1[std::max(i, j)]);
@@ -526,6 +556,8 @@ void setBoundaries(Sim& s, int n, const sphgpu_boundary* b) {
         d.enterP[k] = p2[k]; d.enterN[k] = n2[k]; d.exitP[k] = p1[k]; d.exitN[k] = n1[k];
       }
       c.reflect = d.reflect = 0;
+      c.periodicSecond = 0;
+      d.periodicSecond = 1;
       subs.push_back(c);
       subs.push_back(d);
     }
@@ -573,6 +605,36 @@ void applyGhostBoundaries(Sim& s) {
   if (s.dim == 1) applyFieldsD<1>(s, true);
   else if (s.dim == 2) applyFieldsD<2>(s, true);
   else applyFieldsD<3>(s, true);
+  s.recValid = false;
+  s.derivsValid = false;
+}
+
+void enforceBoundaries(Sim& s) {
+  if (s.subs.empty() || s.nInt == 0) return;
+  hipStream_t st = s.stream;
+  for (size_t b = 0; b < s.subs.size(); ++b) {
+    const DSub d = toDevice(s.subs[b]);
+    if (s.dim == 1)
+      hipLaunchKernelGGL(k_enforce<1>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.pos.as<double>(), s.vel.as<double>(),
+                         s.H.as<double>(), s.nInt, d);
+    else if (s.dim == 2)
+      hipLaunchKernelGGL(k_enforce<2>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.pos.as<double>(), s.vel.as<double>(),
+                         s.H.as<double>(), s.nInt, d);
+    else
+      hipLaunchKernelGGL(k_enforce<3>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.pos.as<double>(), s.vel.as<double>(),
+                         s.H.as<double>(), s.nInt, d);
+    // the second half of a periodic pair closes it: wrap what is still outside
+    if (s.subs[b].periodicSecond) {
+      const DSub first = toDevice(s.subs[b - 1]);
+      if (s.dim == 1)
+        hipLaunchKernelGGL(k_periodic_wrap<1>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.pos.as<double>(), s.nInt, first);
+      else if (s.dim == 2)
+        hipLaunchKernelGGL(k_periodic_wrap<2>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.pos.as<double>(), s.nInt, first);
+      else
+        hipLaunchKernelGGL(k_periodic_wrap<3>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.pos.as<double>(), s.nInt, first);
+    }
+  }
+  SG_CHECK_HIP(hipGetLastError());
   s.recValid = false;
   s.derivsValid = false;
 }

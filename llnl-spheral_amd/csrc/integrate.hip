@@ -1,0 +1,295 @@
+// Time integration on the device: GammaLawGas, GenericHydro::dt and
+// SynchronousRK2::step with the SPH state policies.
+//
+// Step (src/Integrator/SynchronousRK2.cc:78-144):
+//   ghosts + neighbours (Integrator::preStepInitialize -> setGhostNodes),
+//   dt = selectDt(GenericHydro::dt) with dtGrowth and the dtMin/dtMax clamps,
+//   derivatives at t, state0 <- state, update by dt/2, ghosts,
+//   derivatives at t + dt/2, state <- state0 updated by dt with them, ghosts,
+//   SPHHydroBase::finalize (sum density with the step's neighbour rows),
+//   Integrator::enforceBoundaries.
+// State::update with the policies of SPHHydroBase::registerState
+// (src/SPH/SPHHydroBase.cc:238-336) on the internal rows:
+//   x += m*DxDt, v += m*DvDt, eps += m*DepsDt          (IncrementFieldList)
+//   rho = min(rhoMax, max(rhoMin, rho + m*DrhoDt))      (IncrementBoundedState)
+//   H = ideal H with eigenvalues bounded by 1/hmax, 1/hmin (ReplaceBoundedState;
+//       the SPH ideal H is h1^-1 I, so the bound is a clamp of the diagonal)
+//   P, c from the gamma-law EOS                         (Pressure/SoundSpeedPolicy)
+// The derivative fields are read in slot order (one lane per internal slot of
+// the work list) and the state is written at the caller index of the slot.
+#include "internal.hh"
+#include "dgeom.hh"
+#include "eigen.hh"
+#include "ell.hh"
+#include <cfloat>
+#include <cstring>
+#include <algorithm>
+
+namespace sphgpu {
+
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p);
+
+namespace {
+
+constexpr int kBlock = 256;
+inline int nblocks(int64_t n, int bs = kBlock) { return int(std::max<int64_t>(1, (n + bs - 1) / bs)); }
+
+struct Eos {
+  double gamma, gamma1, Pmin, Pmax;
+  int floor;
+};
+// GammaLawGas::pressure with applyPressureLimits, soundSpeed
+__device__ __forceinline__ double eosP(const Eos& e, double rho, double eps) {
+  const double P = e.gamma1 * rho * eps;
+  return P < e.Pmin ? (e.floor ? e.Pmin : 0.0) : (P > e.Pmax ? e.Pmax : P);
+}
+__device__ __forceinline__ double eosCs(const Eos& e, double eps) { return sqrt(dmax(0.0, e.gamma * e.gamma1 * eps)); }
+
+__global__ void k_eos(const double* __restrict__ rho, const double* __restrict__ eps, double* __restrict__ P,
+                      double* __restrict__ cs, int64_t n, Eos e) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  P[i] = eosP(e, rho[i], eps[i]);
+  cs[i] = eosCs(e, eps[i]);
+}
+
+struct Fields {
+  double *pos, *vel, *eps, *rho, *H, *P, *cs;
+};
+struct Base {
+  const double *pos, *vel, *eps, *rho;
+};
+struct DerivIn {
+  const double* base;  // field-major blocks of the last evaluation, slot rows
+  int64_t n;
+  int oDx, oDv, oDeps, oDrho, oHid;
+};
+struct Bounds {
+  double rhoMin, rhoMax, hmaxInv, hminInv;
+};
+
+template <int D>
+__global__ void k_update(const int32_t* __restrict__ work, const int32_t* __restrict__ perm, int64_t nInt, Fields f,
+                         Base b, DerivIn d, double mult, Bounds bd, Eos e) {
+  constexpr int NS = NSym<D>::value;
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= nInt) return;
+  const int64_t k = work[t];
+  const int64_t i = perm[k];
+  const double* dx = d.base + int64_t(d.oDx) * d.n + k * D;
+  const double* dv = d.base + int64_t(d.oDv) * d.n + k * D;
+  for (int c = 0; c < D; ++c) {
+    f.pos[i * D + c] = b.pos[i * D + c] + mult * dx[c];
+    f.vel[i * D + c] = b.vel[i * D + c] + mult * dv[c];
+  }
+  const double eps = b.eps[i] + mult * d.base[int64_t(d.oDeps) * d.n + k];
+  const double rho = dmin(bd.rhoMax, dmax(bd.rhoMin, b.rho[i] + mult * d.base[int64_t(d.oDrho) * d.n + k]));
+  f.eps[i] = eps;
+  f.rho[i] = rho;
+  const double* hid = d.base + int64_t(d.oHid) * d.n + k * NS;
+  int c = 0;
+  for (int a = 0; a < D; ++a)
+    for (int q = a; q < D; ++q, ++c) f.H[i * NS + c] = (a == q) ? dmin(bd.hminInv, dmax(bd.hmaxInv, hid[c])) : hid[c];
+  f.P[i] = eosP(e, rho, eps);
+  f.cs[i] = eosCs(e, eps);
+}
+
+// maxViscousPressure, tr DvDx and DvDt of the last evaluation, by caller index.
+template <int D>
+__global__ void k_dt_inputs(const int32_t* __restrict__ work, const int32_t* __restrict__ perm, int64_t nInt,
+                            const double* __restrict__ out, int64_t n, int oVP, int oDvDx, int oDvDt,
+                            double* __restrict__ dtIn) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= nInt) return;
+  const int64_t k = work[t];
+  const int64_t i = perm[k];
+  constexpr int W = 2 + D;
+  dtIn[i * W] = out[int64_t(oVP) * n + k];
+  const double* g = out + int64_t(oDvDx) * n + k * D * D;
+  double tr = g[0];
+  for (int a = 1; a < D; ++a) tr = tr + g[a * D + a];
+  dtIn[i * W + 1] = tr;
+  for (int c = 0; c < D; ++c) dtIn[i * W + 2 + c] = out[int64_t(oDvDt) * n + k * D + c];
+}
+
+template <int D> __device__ __forceinline__ double vmagArr(const double* a) {
+  if constexpr (D == 1) return dabs(a[0]);
+  double s = a[0] * a[0];
+  for (int c = 1; c < D; ++c) s = s + a[c] * a[c];
+  return sqrt(s);
+}
+
+// GenericHydro::dt candidates of one internal node; the minimum over all
+// nodes is reduced with one 64-bit atomic min per wavefront on the bit
+// pattern of the (positive) candidate.
+template <int D>
+__global__ void k_dt(const int32_t* __restrict__ work, const int32_t* __restrict__ perm, int64_t nInt,
+                     const int32_t* __restrict__ rowCount, const int32_t* __restrict__ nbr, int64_t n,
+                     const double* __restrict__ H, const double* __restrict__ cs, const double* __restrict__ rho,
+                     const double* __restrict__ vel, const double* __restrict__ dtIn, int haveDtIn, double nPerh,
+                     int useVelMag, unsigned long long* __restrict__ minBits) {
+  constexpr int NS = NSym<D>::value;
+  const int64_t t = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  double m = double(FLT_MAX);
+  if (t < nInt) {
+    const int64_t k = work[t];
+    const int64_t i = perm[k];
+    const double nodeScale = 1.0 / maxEigen<D>(H + i * NS) / nPerh;
+    m = dmin(m, nodeScale / (cs[i] + FLT_MIN));
+    constexpr int W = 2 + D;
+    double vp = 0.0, divv = 0.0, acc[3] = {0.0, 0.0, 0.0};
+    if (haveDtIn) {
+      vp = dtIn[i * W];
+      divv = dtIn[i * W + 1];
+      for (int c = 0; c < D; ++c) acc[c] = dtIn[i * W + 2 + c];
+    }
+    const double csq = sqrt(vp / rho[i]);
+    m = dmin(m, nodeScale / (csq + FLT_MIN));
+    m = dmin(m, 1.0 / (dabs(divv) + FLT_MIN));
+    double vi[3];
+    for (int c = 0; c < D; ++c) vi[c] = vel[i * D + c];
+    const int len = rowCount[k];
+    for (int w = 0; w < len; ++w) {
+      const int64_t j = perm[nbr[ellIndex(w, n, k)]];
+      double d[3];
+      for (int c = 0; c < D; ++c) d[c] = vi[c] - vel[j * D + c];
+      m = dmin(m, nodeScale * (1.0 / dmax(1.0e-30, vmagArr<D>(d))));
+    }
+    m = dmin(m, sqrt(nodeScale / (vmagArr<D>(acc) + FLT_MIN)));
+    if (useVelMag) m = dmin(m, nodeScale / (vmagArr<D>(vi) + 1.0e-10));
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) m = dmin(m, __shfl_xor(m, sh));
+  if ((threadIdx.x & 63) == 0) atomicMin(minBits, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+Eos makeEos(double gamma, double Pmin, double Pmax, int floor) { return Eos{gamma, gamma - 1.0, Pmin, Pmax, floor}; }
+
+template <int D> void updateD(Sim& s, const Base& b, double mult, const Bounds& bd, const Eos& e) {
+  Fields f{s.pos.as<double>(), s.vel.as<double>(), s.eps.as<double>(), s.rho.as<double>(), s.H.as<double>(),
+           s.P.as<double>(), s.cs.as<double>()};
+  DerivIn d{s.out.as<double>(), s.outStride, derivOffsetDoubles(D, SPHGPU_D_DXDT), derivOffsetDoubles(D, SPHGPU_D_DVDT),
+            derivOffsetDoubles(D, SPHGPU_D_DEPSDT), derivOffsetDoubles(D, SPHGPU_D_DRHODT),
+            derivOffsetDoubles(D, SPHGPU_D_HIDEAL)};
+  hipLaunchKernelGGL(k_update<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
+                     s.perm.as<int32_t>(), s.nInt, f, b, d, mult, bd, e);
+  SG_CHECK_HIP(hipGetLastError());
+  s.recValid = false;
+}
+
+template <int D> void dtInputsD(Sim& s) {
+  s.dtIn.ensure(sizeof(double) * size_t(2 + D) * size_t(std::max<int64_t>(s.nInt, 1)));
+  hipLaunchKernelGGL(k_dt_inputs<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
+                     s.perm.as<int32_t>(), s.nInt, s.out.as<double>(), s.outStride,
+                     derivOffsetDoubles(D, SPHGPU_D_MAX_VISCOUS_PRESSURE), derivOffsetDoubles(D, SPHGPU_D_DVDX),
+                     derivOffsetDoubles(D, SPHGPU_D_DVDT), s.dtIn.as<double>());
+  SG_CHECK_HIP(hipGetLastError());
+  s.haveDtIn = true;
+}
+
+template <int D> double hydroDtD(Sim& s, double nPerh, double cfl, int useVelMag) {
+  s.reduceTmp.ensure(sizeof(double) * 16);
+  unsigned long long* bits = reinterpret_cast<unsigned long long*>(s.reduceTmp.as<double>() + 9);
+  const double init = double(FLT_MAX);
+  unsigned long long ib;
+  std::memcpy(&ib, &init, sizeof(ib));
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s.stream, bits, ib);
+  if (!s.haveDtIn) s.dtIn.ensure(sizeof(double) * 8);
+  hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
+                     s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
+                     s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(), s.dtIn.as<double>(),
+                     s.haveDtIn ? 1 : 0, nPerh, useVelMag, bits);
+  SG_CHECK_HIP(hipGetLastError());
+  unsigned long long hb = 0;
+  SG_CHECK_HIP(hipMemcpyAsync(&hb, bits, sizeof(hb), hipMemcpyDeviceToHost, s.stream));
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+  double minDt;
+  std::memcpy(&minDt, &hb, sizeof(minDt));
+  return cfl * minDt;
+}
+
+template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
+                             double maxTime) {
+  constexpr int NS = NSym<D>::value;
+  hipStream_t st = s.stream;
+  const double t = s.time;
+  // Integrator::preStepInitialize: ghosts and connectivity
+  if (!s.subs.empty()) setGhostNodes(s, ip.kernelExtent, ip.cullGhostNodes != 0);
+  else buildNeighbors(s, ip.kernelExtent);
+  // selectDt
+  const double dtMin = std::min(ip.dtMin, maxTime - t), dtMax = std::min(ip.dtMax, maxTime - t);
+  const double vote = hydroDtD<D>(s, hp.nPerh, ip.cfl, ip.useVelocityMagnitudeForDt);
+  double dt = dtMax;
+  if (vote > 0.0 && vote < dt) dt = vote;
+  dt = std::min(dt, ip.dtGrowth * s.lastDt);
+  dt = std::min(dtMax, std::max(dtMin, dt));
+  const double hdt = 0.5 * dt;
+  const Bounds bd{ip.rhoMin, ip.rhoMax, 1.0 / hp.hmax, 1.0 / hp.hmin};
+  const Eos e = makeEos(ip.gamma, ip.Pmin, ip.Pmax, ip.pressureFloor);
+
+  // derivatives at t
+  evalDerivs(s, hp);
+  // state0
+  const size_t n = size_t(s.nInt);
+  const size_t bytes[5] = {sizeof(double) * n * D, sizeof(double) * n * D, sizeof(double) * n, sizeof(double) * n,
+                           sizeof(double) * n * NS};
+  DBuf* src[5] = {&s.pos, &s.vel, &s.eps, &s.rho, &s.H};
+  for (int q = 0; q < 5; ++q) {
+    s.state0[q].ensure(std::max<size_t>(bytes[q], 8));
+    if (bytes[q]) SG_CHECK_HIP(hipMemcpyAsync(s.state0[q].p, src[q]->p, bytes[q], hipMemcpyDeviceToDevice, st));
+  }
+  const Base b0{s.state0[0].as<double>(), s.state0[1].as<double>(), s.state0[2].as<double>(), s.state0[3].as<double>()};
+  // trial advance to the midpoint
+  updateD<D>(s, b0, hdt, bd, e);
+  applyGhostBoundaries(s);
+  // derivatives at t + dt/2 with the step's neighbour rows
+  evalDerivs(s, hp);
+  // advance from state0 with the midpoint derivatives
+  updateD<D>(s, b0, dt, bd, e);
+  applyGhostBoundaries(s);
+  dtInputsD<D>(s);
+  // SPHHydroBase::finalize: sum density
+  sumDensity(s);
+  // Integrator::enforceBoundaries
+  enforceBoundaries(s);
+  SG_CHECK_HIP(hipStreamSynchronize(st));
+  s.time = t + dt;
+  s.lastDt = dt;
+  s.cycle += 1;
+  s.derivsValid = false;
+  return dt;
+}
+
+}  // namespace
+
+void eosGammaLaw(Sim& s, double gamma, double Pmin, double Pmax, int floorType) {
+  if (s.nInt == 0) return;
+  hipLaunchKernelGGL(k_eos, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.rho.as<double>(), s.eps.as<double>(),
+                     s.P.as<double>(), s.cs.as<double>(), s.nInt, makeEos(gamma, Pmin, Pmax, floorType));
+  SG_CHECK_HIP(hipGetLastError());
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+  s.recValid = false;
+  s.derivsValid = false;
+}
+
+double hydroDt(Sim& s, double nPerh, double cfl, int useVelMag) {
+  SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "the dt vote needs a neighbour build");
+  SG_REQUIRE(nPerh > 0.0, SPHGPU_ERR_ARG, "nPerh must be positive");
+  if (s.dim == 1) return hydroDtD<1>(s, nPerh, cfl, useVelMag);
+  if (s.dim == 2) return hydroDtD<2>(s, nPerh, cfl, useVelMag);
+  return hydroDtD<3>(s, nPerh, cfl, useVelMag);
+}
+
+double rk2Step(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip, double maxTime) {
+  SG_REQUIRE(s.nInt > 0, SPHGPU_ERR_STATE, "no nodes");
+  SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
+  SG_REQUIRE(ip.kernelExtent > 0.0 && ip.cfl > 0.0 && ip.dtGrowth > 0.0, SPHGPU_ERR_ARG, "bad integrator parameters");
+  SG_REQUIRE(s.nTot == s.nInt || s.deviceGhosts, SPHGPU_ERR_STATE,
+             "the integrator makes its own ghosts: set_nodes(n_internal, n_internal)");
+  if (s.dim == 1) return rk2D<1>(s, hp, ip, maxTime);
+  if (s.dim == 2) return rk2D<2>(s, hp, ip, maxTime);
+  return rk2D<3>(s, hp, ip, maxTime);
+}
+
+}  // namespace sphgpu

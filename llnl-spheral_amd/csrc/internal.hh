@@ -88,6 +88,7 @@ struct Timing {
 struct SubBoundary {
   double enterP[3], enterN[3], exitP[3], exitN[3];   // unit normals
   int reflect;                                       // 1: fields mapped by R = I - 2 n n, 0: copied
+  int periodicSecond;                                // second half of a periodic pair
   double R[9];                                       // row major, D x D used
 };
 
@@ -151,6 +152,14 @@ struct Sim {
   int64_t nSlots = 0;
   DBuf work, workFlag, workScan; // internal slots in slot order (n_internal entries)
 
+  // time integration (SynchronousRK2)
+  double time = 0.0;
+  double lastDt = 1.0e-5;        // Integrator::mLastDt default
+  int64_t cycle = 0;
+  DBuf state0[5];                // pos, vel, eps, rho, H of the internal rows at the step start
+  DBuf dtIn;                     // per internal node: maxViscousPressure, tr DvDx, DvDt (caller order)
+  bool haveDtIn = false;
+
   hipEvent_t ev[12] = {};
   Timing timing;
 };
@@ -168,6 +177,11 @@ void setBoundaries(Sim& s, int n, const sphgpu_boundary* b);
 void setGhostNodes(Sim& s, double kernelExtent, bool cull);
 void applyGhostBoundaries(Sim& s);
 void applyGhostDensity(Sim& s);   // after sum density: ghost rho and their records
+void enforceBoundaries(Sim& s);   // internal nodes behind a plane are mapped back
+// integrate.hip
+void eosGammaLaw(Sim& s, double gamma, double Pmin, double Pmax, int floorType);
+double hydroDt(Sim& s, double nPerh, double cfl, int useVelocityMagnitude);
+double rk2Step(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip, double maxTime);
 // scans (neighbors.hip)
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);

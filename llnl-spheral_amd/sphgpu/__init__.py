@@ -104,6 +104,26 @@ def box_boundaries(dim, lo=0.0, hi=1.0, kind="reflecting"):
     return out
 
 
+class IntegratorParams(C.Structure):
+    _fields_ = [("cfl", C.c_double), ("dtMin", C.c_double), ("dtMax", C.c_double), ("dtGrowth", C.c_double),
+                ("gamma", C.c_double), ("Pmin", C.c_double), ("Pmax", C.c_double), ("pressureFloor", C.c_int),
+                ("cullGhostNodes", C.c_int), ("rhoMin", C.c_double), ("rhoMax", C.c_double),
+                ("kernelExtent", C.c_double), ("useVelocityMagnitudeForDt", C.c_int)]
+
+
+FLT_MAX = 3.4028234663852886e38
+
+
+def integrator_params(**kw):
+    """SynchronousRK2 / Integrator, GenericHydro, GammaLawGas and FluidNodeList
+    defaults of the reference (see include/sphgpu.h)."""
+    d = dict(cfl=0.25, dtMin=0.0, dtMax=FLT_MAX, dtGrowth=2.0, gamma=5.0 / 3.0, Pmin=-1.0e200, Pmax=1.0e200,
+             pressureFloor=1, cullGhostNodes=1, rhoMin=1.0e-10, rhoMax=1.0e10, kernelExtent=2.0,
+             useVelocityMagnitudeForDt=0)
+    d.update(kw)
+    return IntegratorParams(**d)
+
+
 def hydro_params(**kw):
     """Defaults of the reference's SPH() factory for the B-spline kernel
     (src/SPH/SPHHydros.py:317-384) with compatible energy and grad-h off."""
@@ -156,6 +176,11 @@ def lib():
             "sphgpu_synchronize": (C.c_int, [vp]),
             "sphgpu_copy_probe": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
             "sphgpu_set_boundaries": (C.c_int, [vp, C.c_int, C.POINTER(Boundary)]),
+            "sphgpu_eos_gamma_law": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int]),
+            "sphgpu_hydro_dt": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, _dp]),
+            "sphgpu_rk2_step": (C.c_int, [vp, C.POINTER(HydroParams), C.POINTER(IntegratorParams), C.c_double, _dp]),
+            "sphgpu_get_time": (C.c_int, [vp, _dp, _dp, _i64p]),
+            "sphgpu_set_time": (C.c_int, [vp, C.c_double, C.c_double, C.c_int64]),
             "sphgpu_set_ghost_nodes": (C.c_int, [vp, C.c_double, C.c_int]),
             "sphgpu_apply_ghost_boundaries": (C.c_int, [vp]),
             "sphgpu_node_counts": (C.c_int, [vp, _i64p, _i64p]),
@@ -348,6 +373,31 @@ class Sim:
         n = C.c_int64()
         self._chk(self.L.sphgpu_pair_count(self.h, C.byref(n)))
         return n.value
+
+    # time integration ---------------------------------------------------------
+    def eos_gamma_law(self, gamma, Pmin=-1.0e200, Pmax=1.0e200, floor=True):
+        self._chk(self.L.sphgpu_eos_gamma_law(self.h, gamma, Pmin, Pmax, int(floor)))
+
+    def hydro_dt(self, nPerh=1.51, cfl=0.25, use_velocity_magnitude=False):
+        dt = C.c_double()
+        self._chk(self.L.sphgpu_hydro_dt(self.h, nPerh, cfl, int(use_velocity_magnitude), C.byref(dt)))
+        return dt.value
+
+    def rk2_step(self, hydro=None, integ=None, max_time=FLT_MAX):
+        hp = hydro if hydro is not None else hydro_params()
+        ip = integ if integ is not None else integrator_params()
+        dt = C.c_double()
+        self._chk(self.L.sphgpu_rk2_step(self.h, C.byref(hp), C.byref(ip), max_time, C.byref(dt)))
+        self._sync_counts()
+        return dt.value
+
+    def clock(self):
+        t, ldt, cyc = C.c_double(), C.c_double(), C.c_int64()
+        self._chk(self.L.sphgpu_get_time(self.h, C.byref(t), C.byref(ldt), C.byref(cyc)))
+        return t.value, ldt.value, cyc.value
+
+    def set_clock(self, t=0.0, last_dt=1.0e-5, cycle=0):
+        self._chk(self.L.sphgpu_set_time(self.h, t, last_dt, cycle))
 
     def timings(self):
         t = np.zeros(6)

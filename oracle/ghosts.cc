@@ -81,6 +81,39 @@ inline double minEigen(const Sym<2>& H) {
   const double q = 0.5 * (b + sgn(b) * std::sqrt(std::max(0.0, b * b - 4.0 * c)));
   return std::min(q, c / q);
 }
+template <int D> double maxEigen(const Sym<D>& H);
+// Eigenvalues of a SymTensor as SymTensor::eigenValues()
+// (GeomSymmetricTensorInline_default.hh:2053-2110).
+inline void eigenValues(const Sym<1>& H, double* e) { e[0] = H.xx; }
+inline void eigenValues(const Sym<2>& H, double* e) {
+  if (std::fabs(H.xy) < 1.0e-50) { e[0] = H.xx; e[1] = H.yy; return; }
+  const double b = H.xx + H.yy;
+  const double c = sdet(H);
+  const double q = 0.5 * (b + sgn(b) * std::sqrt(std::max(0.0, b * b - 4.0 * c)));
+  e[0] = q; e[1] = c / q;
+}
+inline void eigenValues(const Sym<3>& H, double* e) {
+  const double onethird = 1.0 / 3.0, sqrt3 = std::sqrt(3.0);
+  const double fscale = std::max(10.0 * std::numeric_limits<double>::epsilon(),
+                                 std::max(std::fabs(H.xx), std::max(std::fabs(H.xy), std::max(std::fabs(H.xz),
+                                          std::max(std::fabs(H.yy), std::max(std::fabs(H.yz), std::fabs(H.zz)))))));
+  const double fscalei = 1.0 / fscale;
+  const double a00 = H.xx * fscalei, a01 = H.xy * fscalei, a02 = H.xz * fscalei;
+  const double a11 = H.yy * fscalei, a12 = H.yz * fscalei, a22 = H.zz * fscalei;
+  const double c0 = a00 * a11 * a22 + 2.0 * a01 * a02 * a12 - a00 * a12 * a12 - a11 * a02 * a02 - a22 * a01 * a01;
+  const double c1 = a00 * a11 - a01 * a01 + a00 * a22 - a02 * a02 + a11 * a22 - a12 * a12;
+  const double c2 = a00 + a11 + a22;
+  const double c2Div3 = c2 * onethird;
+  const double aDiv3 = std::min(0.0, onethird * (c1 - c2 * c2Div3));
+  const double mbDiv2 = 0.5 * (c0 + c2Div3 * (2.0 * c2Div3 * c2Div3 - c1));
+  const double q = std::min(0.0, mbDiv2 * mbDiv2 + aDiv3 * aDiv3 * aDiv3);
+  const double mag = std::sqrt(-aDiv3);
+  const double angle = std::atan2(std::sqrt(-q), mbDiv2) * onethird;
+  const double cs = std::cos(angle), sn = std::sin(angle);
+  e[0] = fscale * (c2Div3 + 2.0 * mag * cs);
+  e[1] = fscale * (c2Div3 - mag * (cs + sqrt3 * sn));
+  e[2] = fscale * (c2Div3 - mag * (cs - sqrt3 * sn));
+}
 inline double minEigen(const Sym<3>& H) {
   const double onethird = 1.0 / 3.0, sqrt3 = std::sqrt(3.0);
   const double fscale = std::max(10.0 * std::numeric_limits<double>::epsilon(),
@@ -219,6 +252,10 @@ void applyGhosts(int64_t nInt, int64_t nTot, int nb, const orc_boundary* b, cons
     const Sub<D>& S = subs[size_t(subOf[g - nInt])];
     if (kind == 0) {
       f[g] = f[c];
+    } else if (kind == 3) {      // positions: PlanarBoundary::updateGhostNodes
+      Vec<D> x; for (int d = 0; d < D; ++d) x.c[d] = f[c * D + d];
+      const Vec<D> xg = mapThrough(x, S.exit, S.enter);
+      for (int d = 0; d < D; ++d) f[g * D + d] = xg.c[d];
     } else if (kind == 1) {
       Vec<D> v; for (int d = 0; d < D; ++d) v.c[d] = f[c * D + d];
       const Vec<D> o = S.reflect ? tdotv(S.R, v) : v;
@@ -231,6 +268,70 @@ void applyGhosts(int64_t nInt, int64_t nTot, int nb, const orc_boundary* b, cons
       for (int d = 0; d < NS; ++d) f[g * NS + d] = op[d];
     }
   }
+}
+
+// position < plane: GeomPlane::compare == 1, i.e. n.(x - p) < 0 and not
+// fuzzyEqual to 0 (src/Geometry/GeomPlane.cc:306-341).
+template <int D> bool behind(const GPlane<D>& P, const Vec<D>& x) {
+  const double d = signedDistance(P, x);
+  return !fuzzyEqual(d, 0.0) && !(d > 0.0);
+}
+
+// (R*H*R).Symmetric() as enforceBoundary writes it: (R*H)*R.
+template <int D> Sym<D> reflectSymLeft(const Ten<D>& R, const Sym<D>& H) {
+  Ten<D> full;
+  const double* h = sptr(H);
+  int c = 0;
+  for (int i = 0; i < D; ++i)
+    for (int j = i; j < D; ++j, ++c) { full.c[i * D + j] = h[c]; full.c[j * D + i] = h[c]; }
+  const Ten<D> t = tdot(tdot(R, full), R);
+  Sym<D> o;
+  double* p = sptr(o);
+  c = 0;
+  for (int i = 0; i < D; ++i)
+    for (int j = i; j < D; ++j, ++c) p[c] = (i == j) ? t.c[i * D + i] : 0.5 * (t.c[i * D + j] + t.c[j * D + i]);
+  return o;
+}
+
+template <int D>
+void enforce(int64_t nInt, int nb, const orc_boundary* b, double* pos, double* vel, double* H) {
+  constexpr int NS = nsym<D>();
+  size_t sb = 0;
+  for (int q = 0; q < nb; ++q) {
+    const auto subs = expand<D>(1, b + q);
+    for (const Sub<D>& S : subs) {
+      ++sb;
+      for (int64_t i = 0; i < nInt; ++i) {
+        Vec<D> x; for (int d = 0; d < D; ++d) x.c[d] = pos[i * D + d];
+        if (!behind(S.enter, x)) continue;
+        const Vec<D> xn = mapThrough(x, S.enter, S.exit);
+        for (int d = 0; d < D; ++d) pos[i * D + d] = xn.c[d];
+        if (S.reflect) {
+          Vec<D> v; for (int d = 0; d < D; ++d) v.c[d] = vel[i * D + d];
+          const Vec<D> vn = tdotv(S.R, v);
+          for (int d = 0; d < D; ++d) vel[i * D + d] = vn.c[d];
+          Sym<D> h; double* hp = sptr(h); for (int d = 0; d < NS; ++d) hp[d] = H[i * NS + d];
+          const Sym<D> hn = reflectSymLeft(S.R, h);
+          const double* hq = sptr(hn); for (int d = 0; d < NS; ++d) H[i * NS + d] = hq[d];
+        }
+      }
+    }
+    if (b[q].kind == 1) {
+      // PeriodicBoundary::updateViolationNodes: wrap what is still outside
+      const GPlane<D> p1 = makePlane<D>(b[q].p1), p2 = makePlane<D>(b[q].p2);
+      const double L = minimumDistance(p1, p2.p);
+      for (int64_t i = 0; i < nInt; ++i) {
+        Vec<D> x; for (int d = 0; d < D; ++d) x.c[d] = pos[i * D + d];
+        if (behind(p1, x) || behind(p2, x)) {
+          const Vec<D> p0 = add(x, scale(p1.n, dot(sub(p1.p, x), p1.n)));
+          const double f = std::max(-1.0, std::min(1.0, std::fmod(signedDistance(p1, x) / L, 1.0)));
+          const Vec<D> xn = add(p0, scale(p1.n, f * L));
+          for (int d = 0; d < D; ++d) pos[i * D + d] = xn.c[d];
+        }
+      }
+    }
+  }
+  (void)sb;
 }
 
 }  // namespace orc
@@ -256,12 +357,40 @@ double orc_min_eigen(int dim, const double* h) {
   return orc::minEigen(orc::Sym<3>{h[0], h[1], h[2], h[3], h[4], h[5]});
 }
 
+// SymTensor::eigenValues().maxElement() of one packed SymTensor.
+double orc_max_eigen(int dim, const double* h) {
+  double e[3];
+  if (dim == 1) { orc::eigenValues(orc::Sym<1>{h[0]}, e); return e[0]; }
+  if (dim == 2) { orc::eigenValues(orc::Sym<2>{h[0], h[1], h[2]}, e); return std::max(e[0], e[1]); }
+  orc::eigenValues(orc::Sym<3>{h[0], h[1], h[2], h[3], h[4], h[5]}, e);
+  return std::max(e[0], std::max(e[1], e[2]));
+}
+
+void orc_max_eigen_n(int dim, int64_t n, const double* H, double* out) {
+  const int ns = dim * (dim + 1) / 2;
+  for (int64_t i = 0; i < n; ++i) out[i] = orc_max_eigen(dim, H + i * ns);
+}
+
 // Boundary::applyGhostBoundary for one field, every ghost in order.
 void orc_apply_ghosts(int dim, int64_t nInt, int64_t nTot, int nb, const orc_boundary* b, const int32_t* control,
                       const int32_t* subOf, int kind, double* f) {
   if (dim == 1) orc::applyGhosts<1>(nInt, nTot, nb, b, control, subOf, kind, f);
   else if (dim == 2) orc::applyGhosts<2>(nInt, nTot, nb, b, control, subOf, kind, f);
   else orc::applyGhosts<3>(nInt, nTot, nb, b, control, subOf, kind, f);
+}
+
+// Integrator::enforceBoundaries over the internal nodes, boundary by boundary:
+// nodes behind a sub-boundary's enter plane are mapped through it
+// (PlanarBoundary::setViolationNodes/updateViolationNodes,
+// src/Boundary/PlanarBoundary.cc:219-270); a reflecting boundary maps v by R
+// and H by (R H R).Symmetric() (ReflectingBoundary.cc:240-282); a periodic
+// boundary then wraps nodes still outside by the fmod rule
+// (src/Boundary/PeriodicBoundary.cc:176-207).
+void orc_enforce_boundaries(int dim, int64_t nInt, int nb, const orc_boundary* b, double* pos, double* vel,
+                            double* H) {
+  if (dim == 1) orc::enforce<1>(nInt, nb, b, pos, vel, H);
+  else if (dim == 2) orc::enforce<2>(nInt, nb, b, pos, vel, H);
+  else orc::enforce<3>(nInt, nb, b, pos, vel, H);
 }
 
 // Ghost culling flags and the old -> new index map (Integrator::setGhostNodes

@@ -136,6 +136,8 @@ def lib(fast=False):
                                      C.c_int64, _i32p, _i32p, _i32p]
         L.orc_apply_ghosts.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int, C.POINTER(Boundary), _i32p, _i32p,
                                        C.c_int, _dp]
+        L.orc_max_eigen_n.argtypes = [C.c_int, C.c_int64, _dp, _dp]
+        L.orc_enforce_boundaries.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(Boundary), _dp, _dp, _dp]
         L.orc_min_eigen.restype = C.c_double
         L.orc_min_eigen.argtypes = [C.c_int, _dp]
         L.orc_cull_map.restype = C.c_int64
@@ -292,12 +294,25 @@ def set_ghosts(dim, pos, H, boundaries, ext=2.0):
         cap *= 2
 
 
-FIELD_KIND = {"pos": 1, "vel": 1, "H": 2}
+FIELD_KIND = {"pos": 3, "vel": 1, "H": 2}
+
+
+def max_eigen_n(dim, H):
+    """SymTensor::eigenValues().maxElement() of every row."""
+    a = np.ascontiguousarray(H, dtype=np.float64)
+    out = np.zeros(a.shape[0])
+    lib().orc_max_eigen_n(dim, a.shape[0], _p(a), _p(out))
+    return out
+
+
+def enforce_boundaries(dim, n_internal, boundaries, pos, vel, H):
+    """Integrator::enforceBoundaries on the internal rows (in place)."""
+    lib().orc_enforce_boundaries(dim, n_internal, len(boundaries), _barr(boundaries), _p(pos), _p(vel), _p(H))
 
 
 def apply_ghosts(dim, n_internal, boundaries, control, sub_of, field, kind):
     """Boundary::applyGhostBoundary over every ghost row of `field` (in place).
-    kind 0 scalar, 1 Vector, 2 SymTensor."""
+    kind 0 scalar, 1 Vector, 2 SymTensor, 3 positions (updateGhostNodes)."""
     f = np.ascontiguousarray(field, dtype=np.float64)
     lib().orc_apply_ghosts(dim, n_internal, f.shape[0], len(boundaries), _barr(boundaries),
                            _p(np.ascontiguousarray(control, dtype=np.int32), _i32p),
