@@ -120,6 +120,8 @@ def main():
                          "every step (the benchmark configuration); none: free surface")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rk2-steps", type=int, default=5,
+                    help="SynchronousRK2 steps timed after the pass measurement (reported under rk2; 0 = off)")
     ap.add_argument("--probe-mib", type=int, default=256,
                     help="size of the plain copy kernel run after the timed steps (counter calibration); 0 = off")
     args = ap.parse_args()
@@ -217,6 +219,22 @@ def main():
         issue_ms = 1e3 * counters["valu_wave_instructions"] / VALU_ISSUE_HZ
         line["roofline"]["valu"] = {"wave_instructions": counters["valu_wave_instructions"], "issue_ms": issue_ms,
                                     "kernel_ms": derivs_ms, "busy_frac": issue_ms / derivs_ms}
+    if args.rk2_steps > 0:
+        # full time steps on the same resident state: ghosts + neighbours, dt
+        # vote, two derivative passes, updates, sum density, enforcement
+        hp = sphgpu.hydro_params(precision=prec)
+        ip = sphgpu.integrator_params(gamma=meta["gamma"])
+        sim.eos_gamma_law(meta["gamma"])
+        sim.rk2_step(hp, ip)                      # warm-up step
+        sim.synchronize()
+        t0 = time.perf_counter()
+        dts = [sim.rk2_step(hp, ip) for _ in range(args.rk2_steps)]
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        line["rk2"] = {"steps": args.rk2_steps, "ms_per_step": 1e3 * wall / args.rk2_steps, "dt": dts,
+                       "step": "SynchronousRK2: ghosts + neighbours, dt vote, 2 evaluateDerivatives, updates, "
+                               "sum density, boundary enforcement",
+                       "pairs_per_s": 2 * pairs * args.rk2_steps / wall}
     if args.probe_mib > 0:
         nb = args.probe_mib << 20
         ms = sim.copy_probe(nb, 5)

@@ -22,6 +22,7 @@
 // Memory: neighbour indices arrive four at a time (one int4 of the blocked ELL),
 // the next neighbour's record is loaded while the current pair is evaluated,
 // and the kernel table sits in LDS.
+#include <type_traits>
 #include "internal.hh"
 #include "dgeom.hh"
 #include "records.hh"
@@ -533,27 +534,33 @@ __global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision (isotropic H): FP64 positions and per-node accumulators,
-// FP32 pair kinematics, kernel lookups and pair terms.
+// Isotropic pair loop in collapsed form, FP32 (mixed precision) or FP64 (the
+// search-order FP64 path).  Positions and per-node accumulators are FP64.
 //
 // With H = h I both unit vectors of a pair are r_hat = r_ij/|r_ij| and
 // grad W = r_hat * h * dW/deta * det H, so the pair body reduces to scalars
 // along r_hat: the acceleration increment is r_hat * m_j * (g_i (P_i/rho_i^2 +
 // Q_i/2) + g_j (P_j/rho_j^2 + Q_j/2)) with g = h W' det H.  The two ends of a
-// pair compute the same scalar from the same FP32 inputs (r_ij is the negated
-// FP64 difference, r_hat flips sign exactly), so the pair forces are
-// antisymmetric up to the final product with the partner mass.
+// pair compute the same scalar from the same inputs (r_ij is the negated FP64
+// difference, r_hat flips sign exactly), so the pair forces are antisymmetric
+// up to the final product with the partner mass.  This reorders the
+// reference's arithmetic: in FP64 the results agree with the reference-order
+// kernel to round-off (the bar for search row order is 1e-12), in FP32 to the
+// mixed-precision bar (1e-5).
 //
-// Neighbour record, 64 B (16 floats):
-//   [0..5] x, y, z (FP64)  [6] 1/h  [7] det H (negative for ghost nodes)
-//   [8] m  [9] 1/rho  [10..12] v  [13] c  [14] P/rho^2/omega  [15] m/rho
+// Neighbour records:
+//  FP32, 64 B (16 floats): [0..5] x, y, z (FP64)  [6] 1/h  [7] det H (negative
+//    for ghost nodes)  [8] m  [9] 1/rho  [10..12] v  [13] c  [14] P/rho^2/omega
+//    [15] m/rho
+//  FP64, 128 B (16 doubles): [0..2] x  [3] 1/h  [4] det H (negative for
+//    ghosts)  [5] m  [6] 1/rho  [7..9] v  [10] c  [11] P/rho^2/omega  [12] m/rho
 // ---------------------------------------------------------------------------
-struct MNbr {
+template <typename Real> struct INbr {
   double x[3];
-  float hinv, hdet, m, invRho, v[3], cs, prho, mor;
+  Real hinv, hdet, m, invRho, v[3], cs, prho, mor;
   bool ghost;
 };
-__device__ __forceinline__ void loadM(const float4* __restrict__ recM, int32_t j, MNbr& o) {
+__device__ __forceinline__ void loadIso(const float4* __restrict__ recM, int32_t j, INbr<float>& o) {
   const float4* p = recM + int64_t(j) * 4;
   const float4 f0 = p[0], f1 = p[1], f2 = p[2], f3 = p[3];
   o.x[0] = __hiloint2double(__float_as_int(f0.y), __float_as_int(f0.x));
@@ -564,6 +571,15 @@ __device__ __forceinline__ void loadM(const float4* __restrict__ recM, int32_t j
   o.hdet = fabsf(f1.w);
   o.m = f2.x; o.invRho = f2.y; o.v[0] = f2.z; o.v[1] = f2.w;
   o.v[2] = f3.x; o.cs = f3.y; o.prho = f3.z; o.mor = f3.w;
+}
+__device__ __forceinline__ void loadIso(const double2* __restrict__ recD, int32_t j, INbr<double>& o) {
+  const double2* p = recD + int64_t(j) * 8;
+  const double2 d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3], d4 = p[4], d5 = p[5], d6 = p[6];
+  o.x[0] = d0.x; o.x[1] = d0.y; o.x[2] = d1.x; o.hinv = d1.y;
+  o.ghost = d2.x < 0.0;
+  o.hdet = fabs(d2.x);
+  o.m = d2.y; o.invRho = d3.x; o.v[0] = d3.y; o.v[1] = d4.x; o.v[2] = d4.y;
+  o.cs = d5.x; o.prho = d5.y; o.mor = d6.x;
 }
 
 template <int D>
@@ -586,6 +602,25 @@ __global__ void k_pack_mixed(const double* __restrict__ rec, int64_t n, int64_t 
   p[3] = make_float4(D > 2 ? float(r[R::V + 2]) : 0.0f, float(r[R::CS]), float(r[R::PRHO]), float(r[R::MOR]));
 }
 
+template <int D>
+__global__ void k_pack_fast64(const double* __restrict__ rec, int64_t n, int64_t nInt, double2* __restrict__ recD) {
+  using R = RL<true>;
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k >= n) return;
+  const double* r = rec + k * R::STRIDE;
+  const bool ghost = r[R::ORIG] >= double(nInt);
+  const double rho = r[R::RHO];
+  double2* p = recD + k * 8;
+  p[0] = make_double2(r[R::X], D > 1 ? r[R::X + 1] : 0.0);
+  p[1] = make_double2(D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
+  p[2] = make_double2(ghost ? -r[R::HDET] : r[R::HDET], r[R::M]);
+  p[3] = make_double2(rho != 0.0 ? 1.0 / rho : 0.0, r[R::V]);
+  p[4] = make_double2(D > 1 ? r[R::V + 1] : 0.0, D > 2 ? r[R::V + 2] : 0.0);
+  p[5] = make_double2(r[R::CS], r[R::PRHO]);
+  p[6] = make_double2(r[R::MOR], 0.0);
+  p[7] = make_double2(0.0, 0.0);
+}
+
 // FP32 table in LDS: 8 floats per point.
 template <bool LDS> __device__ __forceinline__ const float* stageTable32(float* lds, const TabArgs& t) {
   if constexpr (LDS) {
@@ -600,8 +635,34 @@ struct Tab32 {
   float invStep, extent;
   int n;
 };
-__device__ __forceinline__ void lookupWG32(const float* __restrict__ tab, const Tab32& t, float eta, float hdet,
-                                           float& W, float& gW) {
+struct Tab64 {
+  double invStep, extent;
+  int n;
+};
+
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// reciprocal and reciprocal square root: the hardware approximations in FP32,
+// refined by two Newton steps (error squared each step) in FP64
+__device__ __forceinline__ float rcpR(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpR(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = __builtin_fma(y, __builtin_fma(-x, y, 1.0), y);
+  return __builtin_fma(y, __builtin_fma(-x, y, 1.0), y);
+}
+__device__ __forceinline__ float rsqR(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqR(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = __builtin_fma(0.5 * y, __builtin_fma(-(x * y), y, 1.0), y);
+  return __builtin_fma(0.5 * y, __builtin_fma(-(x * y), y, 1.0), y);
+}
+__device__ __forceinline__ float absR(float x) { return fabsf(x); }
+__device__ __forceinline__ double absR(double x) { return fabs(x); }
+__device__ __forceinline__ float maxR(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double maxR(double a, double b) { return fmax(a, b); }
+
+__device__ __forceinline__ void lookupWGR(const float* __restrict__ tab, const Tab32& t, float eta, float hdet,
+                                          float& W, float& gW) {
   if (eta < t.extent) {
     const float q = eta * t.invStep;
     int lb = int(q);
@@ -616,6 +677,21 @@ __device__ __forceinline__ void lookupWG32(const float* __restrict__ tab, const 
     W = 0.0f; gW = 0.0f;
   }
 }
+__device__ __forceinline__ void lookupWGR(const double* __restrict__ tab, const Tab64& t, double eta, double hdet,
+                                          double& W, double& gW) {
+  if (eta < t.extent) {
+    const double q = eta * t.invStep;
+    int lb = int(q);
+    lb = lb < t.n - 1 ? lb : t.n - 1;
+    const int i0 = lb < t.n - 3 ? lb : t.n - 3;
+    const double x = q - double(i0);
+    const double* c = tab + 6 * (i0 + 1);
+    W = hdet * __builtin_fma(__builtin_fma(c[2], x, c[1]), x, c[0]);
+    gW = hdet * __builtin_fma(__builtin_fma(c[5], x, c[4]), x, c[3]);
+  } else {
+    W = 0.0; gW = 0.0;
+  }
+}
 __device__ __forceinline__ float lookupW32(const float* __restrict__ tab, const Tab32& t, float eta, float hdet) {
   if (!(eta < t.extent)) return 0.0f;
   const float q = eta * t.invStep;
@@ -626,13 +702,13 @@ __device__ __forceinline__ float lookupW32(const float* __restrict__ tab, const 
   const float4 a = *reinterpret_cast<const float4*>(tab + 8 * (i0 + 1));
   return hdet * __builtin_fmaf(__builtin_fmaf(a.z, x, a.y), x, a.x);
 }
-__device__ __forceinline__ float muQ32(float mu, float Cl, float Cq, float cs, int linear, int quadratic) {
-  const float m0 = fminf(0.0f, mu);
-  const float lin = linear ? mu : m0;
-  const float quad = quadratic ? -(mu < 0.0f ? -1.0f : 1.0f) * mu * mu : m0 * m0;
+template <typename Real>
+__device__ __forceinline__ Real muQR(Real mu, Real Cl, Real Cq, Real cs, int linear, int quadratic) {
+  const Real m0 = mu < Real(0) ? mu : Real(0);
+  const Real lin = linear ? mu : m0;
+  const Real quad = quadratic ? -(mu < Real(0) ? Real(-1) : Real(1)) * mu * mu : m0 * m0;
   return -Cl * cs * lin + Cq * quad;
 }
-__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <int D, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict__ rec, const float4* __restrict__ recM,
@@ -654,11 +730,11 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
   const int4* nb4 = reinterpret_cast<const int4*>(nbr);
   int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
   int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
-  MNbr nx;
-  if (len > 0) loadM(recM, cur.x, nx);
+  INbr<float> nx;
+  if (len > 0) loadIso(recM, cur.x, nx);
   for (int w = 0; w < len; ++w) {
-    const MNbr nb = nx;
-    if (w + 1 < len) loadM(recM, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nx);
+    const INbr<float> nb = nx;
+    if (w + 1 < len) loadIso(recM, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nx);
     float r2 = 0.0f;
     for (int c = 0; c < D; ++c) {
       const float rc = float(xi[c] - nb.x[c]);
@@ -674,100 +750,128 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
   ri_[R::MOR] = mi / acc;
 }
 
-template <int D, bool LDS>
-__global__ void __launch_bounds__(kBlock) k_derivs_mixed(const double* __restrict__ rec, const float4* __restrict__ recM,
-                                                         const int32_t* __restrict__ rowCount,
-                                                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
-                                                         const int32_t* __restrict__ work, TabArgs t, Tab32 t32,
-                                                         PassArgs p, OutArgs o, unsigned long long* __restrict__ ncalc) {
+// The table of the collapsed pair loop: FP32 (8 floats per point) or FP64 (6
+// doubles per point), staged in LDS.
+template <typename Real> struct IsoTab;
+template <> struct IsoTab<float> {
+  using T = Tab32;
+  using Rec = float4;
+  using Elem = float;
+  static constexpr int PER = 8;
+};
+template <> struct IsoTab<double> {
+  using T = Tab64;
+  using Rec = double2;
+  using Elem = double;
+  static constexpr int PER = 6;
+};
+
+template <typename Real, int D, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict__ rec,
+                                                       const typename IsoTab<Real>::Rec* __restrict__ recN,
+                                                       const int32_t* __restrict__ rowCount,
+                                                       const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
+                                                       const int32_t* __restrict__ work, TabArgs t,
+                                                       typename IsoTab<Real>::T tt, PassArgs p, OutArgs o,
+                                                       unsigned long long* __restrict__ ncalc) {
   using R = RL<true>;
-  extern __shared__ float lds32[];
-  const float* tab = stageTable32<LDS>(lds32, t);
+  using E = typename IsoTab<Real>::Elem;
+  extern __shared__ double ldsIso[];
+  const E* tab;
+  if constexpr (LDS) {
+    E* l = reinterpret_cast<E*>(ldsIso);
+    const E* g = std::is_same<Real, float>::value ? reinterpret_cast<const E*>(t.g32) : reinterpret_cast<const E*>(t.g);
+    for (int q = threadIdx.x; q < IsoTab<Real>::PER * t.n; q += blockDim.x) l[q] = g[q];
+    __syncthreads();
+    tab = l;
+  } else {
+    tab = std::is_same<Real, float>::value ? reinterpret_cast<const E*>(t.g32) : reinterpret_cast<const E*>(t.g);
+  }
   const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   unsigned pairs = 0;
   if (tw < nInt) {
     const int64_t k = work[tw];
     const double* ri_ = rec + k * R::STRIDE;
     double xi[3];
-    float vi[3];
+    Real vi[3];
     Self<D> me;
     for (int c = 0; c < 3; ++c) {
       xi[c] = c < D ? ri_[R::X + c] : 0.0;
-      vi[c] = c < D ? float(ri_[R::V + c]) : 0.0f;
+      vi[c] = c < D ? Real(ri_[R::V + c]) : Real(0);
     }
     for (int c = 0; c < D; ++c) me.v.c[c] = ri_[R::V + c];
     const double hinvD = ri_[RL<true>::HINV];
     for (int c = 0; c < NSym<D>::value; ++c) me.H.c[c] = 0.0;
     for (int c = 0; c < D; ++c) me.H.c[sdiag<D>(c)] = hinvD;
     me.Hdet = ri_[R::HDET]; me.m = ri_[R::M]; me.rho = ri_[R::RHO]; me.mor = ri_[R::MOR];
-    const float hi = float(hinvD), hdeti = float(me.Hdet), ci = float(ri_[R::CS]);
-    const float invRhoi = float(1.0 / me.rho), rhoi2 = float(me.rho * me.rho);
-    const float Prhoi = float(ri_[R::PRHO]), mori = float(me.mor);
-    const float Cl = float(p.Cl), Cq = float(p.Cq), eps2 = float(p.eps2);
+    const Real hi = Real(hinvD), hdeti = Real(me.Hdet), ci = Real(ri_[R::CS]);
+    const Real invRhoi = Real(1.0 / me.rho), rhoi2 = Real(me.rho * me.rho);
+    const Real Prhoi = Real(ri_[R::PRHO]), mori = Real(me.mor);
+    const Real Cl = Real(p.Cl), Cq = Real(p.Cq), eps2 = Real(p.eps2);
 
     Acc<D> a;
     a.zero();
-    float maxVP = 0.0f;
+    Real maxVP = Real(0);
     const int len = rowCount[k];
     const int4* nb4 = reinterpret_cast<const int4*>(nbr);
     int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
     int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
-    MNbr nx;
+    INbr<Real> nx;
     int32_t jn = cur.x;
-    if (len > 0) loadM(recM, jn, nx);
+    if (len > 0) loadIso(recN, jn, nx);
     for (int w = 0; w < len; ++w) {
-      const MNbr nb = nx;
+      const INbr<Real> nb = nx;
       const int32_t j = jn;
       if (w + 1 < len) {
         jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
-        loadM(recM, jn, nx);
+        loadIso(recN, jn, nx);
       }
       pairs += (nb.ghost || j > int32_t(k)) ? 1u : 0u;
-      float rij[3], vij[3];
-      float r2 = 0.0f, vdotr = 0.0f;
+      Real rij[3], vij[3];
+      Real r2 = Real(0), vdotr = Real(0);
       for (int c = 0; c < D; ++c) {
-        rij[c] = float(xi[c] - nb.x[c]);
+        rij[c] = Real(xi[c] - nb.x[c]);
         vij[c] = vi[c] - nb.v[c];
-        r2 = __builtin_fmaf(rij[c], rij[c], r2);
-        vdotr = __builtin_fmaf(vij[c], rij[c], vdotr);
+        r2 = fmaR(rij[c], rij[c], r2);
+        vdotr = fmaR(vij[c], rij[c], vdotr);
       }
-      const float rinv = r2 > 1.0e-30f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-      const float rmag = r2 * rinv;
-      const float hj = nb.hinv;
-      float Wi, gWi, Wj, gWj;
-      lookupWG32(tab, t32, hi * rmag, hdeti, Wi, gWi);
-      lookupWG32(tab, t32, hj * rmag, nb.hdet, Wj, gWj);
-      const float gi = hi * gWi, gj = hj * gWj;     // grad W = r_hat * g
+      const Real rinv = r2 > Real(1.0e-30) ? rsqR(r2) : Real(0);
+      const Real rmag = r2 * rinv;
+      const Real hj = nb.hinv;
+      Real Wi, gWi, Wj, gWj;
+      lookupWGR(tab, tt, hi * rmag, hdeti, Wi, gWi);
+      lookupWGR(tab, tt, hj * rmag, nb.hdet, Wj, gWj);
+      const Real gi = hi * gWi, gj = hj * gWj;     // grad W = r_hat * g
       // Monaghan-Gingold Q: mu = v_ij . eta / (eta^2 + eps2), eta = h r_ij
-      const float mui = hi * vdotr * rcp32(__builtin_fmaf(hi * hi, r2, eps2));
-      const float muj = hj * vdotr * rcp32(__builtin_fmaf(hj * hj, r2, eps2));
-      const float QPi = muQ32(mui, Cl, Cq, ci, p.linearInExpansion, p.quadraticInExpansion) * invRhoi;
-      const float QPj = muQ32(muj, Cl, Cq, nb.cs, p.linearInExpansion, p.quadraticInExpansion) * nb.invRho;
-      const float Qi = rhoi2 * fabsf(QPi);
-      maxVP = fmaxf(maxVP, Qi);
-      const float vdr = vdotr * rinv;                // v_ij . r_hat
-      const float halfQi = 0.5f * QPi * gi;
-      const float pi = __builtin_fmaf(gi, Prhoi, halfQi);
-      const float pj = __builtin_fmaf(gj, nb.prho, 0.5f * QPj * gj);
-      const float mj = nb.m;
-      const float fsr = mj * (pi + pj) * rinv;       // acceleration along r_ij
+      const Real mui = hi * vdotr * rcpR(fmaR(hi * hi, r2, eps2));
+      const Real muj = hj * vdotr * rcpR(fmaR(hj * hj, r2, eps2));
+      const Real QPi = muQR(mui, Cl, Cq, ci, p.linearInExpansion, p.quadraticInExpansion) * invRhoi;
+      const Real QPj = muQR(muj, Cl, Cq, nb.cs, p.linearInExpansion, p.quadraticInExpansion) * nb.invRho;
+      const Real Qi = rhoi2 * absR(QPi);
+      maxVP = maxR(maxVP, Qi);
+      const Real vdr = vdotr * rinv;                 // v_ij . r_hat
+      const Real halfQi = Real(0.5) * QPi * gi;
+      const Real pi = fmaR(gi, Prhoi, halfQi);
+      const Real pj = fmaR(gj, nb.prho, Real(0.5) * QPj * gj);
+      const Real mj = nb.m;
+      const Real fsr = mj * (pi + pj) * rinv;        // acceleration along r_ij
       for (int c = 0; c < D; ++c) a.DvDt.c[c] -= double(rij[c] * fsr);
       a.DepsDt += double(mj * vdr * pi);
       a.viscW += double(mj * vdr * halfQi);
       a.effVP += double(nb.mor * Qi * Wi);
       a.rhoSum += double(mj * Wi);
       a.norm += double(mori * Wi);
-      a.wNS += double(fabsf(gWi));
-      const float wg = mj * gi * rinv;              // grad W_i * m_j = r_ij * wg
+      a.wNS += double(absR(gWi));
+      const Real wg = mj * gi * rinv;               // grad W_i * m_j = r_ij * wg
       for (int c = 0; c < D; ++c) {
-        const float u = vij[c] * wg;
+        const Real u = vij[c] * wg;
         for (int b = 0; b < D; ++b) a.DvDx.c[c * D + b] -= double(u * rij[b]);
       }
-      const float f2 = gi * gi * rcp32(fmaxf(1.0e-30f, r2 * r2 * r2));
+      const Real f2 = gi * gi * rcpR(maxR(Real(1.0e-30), r2 * r2 * r2));
       {
         int s2 = 0;
         for (int c = 0; c < D; ++c) {
-          const float rw = rij[c] * wg, rf = rij[c] * f2;
+          const Real rw = rij[c] * wg, rf = rij[c] * f2;
           for (int b = c; b < D; ++b, ++s2) {
             a.M.c[c * D + b] -= double(rw * rij[b]);
             a.m2nd.c[s2] += double(rf * rij[b]);
@@ -775,7 +879,7 @@ __global__ void __launch_bounds__(kBlock) k_derivs_mixed(const double* __restric
         }
       }
       if (p.XSPH) {
-        const float w2 = 0.5f * __builtin_fmaf(mori, Wi, nb.mor * Wj);
+        const Real w2 = Real(0.5) * fmaR(mori, Wi, nb.mor * Wj);
         a.xsphW += double(w2);
         for (int c = 0; c < D; ++c) a.xsphDV.c[c] -= double(vij[c] * w2);
       }
@@ -821,6 +925,16 @@ void packMixed(Sim& s) {
                      s.nInt, s.recM.as<float4>());
   SG_CHECK_HIP(hipGetLastError());
 }
+
+template <int D>
+void packFast64(Sim& s) {
+  s.recD.ensure(sizeof(double) * 16 * size_t(std::max<int64_t>(s.nSlots, 1)));
+  hipLaunchKernelGGL(k_pack_fast64<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, s.stream, s.rec.as<double>(),
+                     s.nSlots, s.nInt, s.recD.as<double2>());
+  SG_CHECK_HIP(hipGetLastError());
+}
+
+Tab64 tab64(const Sim& s) { return Tab64{1.0 / s.table.step, s.table.extent, s.table.n}; }
 
 template <int D>
 void sumDensityD(Sim& s) {
@@ -899,7 +1013,18 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
                          s.recM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
                          s.work.as<int32_t>(), t, tab32(s), p, o, dncalc);
     };
-    if (lds) go(k_derivs_mixed<D, true>); else go(k_derivs_mixed<D, false>);
+    if (lds) go(k_derivs_iso<float, D, true>); else go(k_derivs_iso<float, D, false>);
+  } else if (s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile) {
+    // search row order is held to round-off of the reference sums, not to
+    // their bits: the collapsed isotropic algebra in FP64
+    packFast64<D>(s);
+    SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
+                         s.recD.as<double2>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
+                         s.work.as<int32_t>(), t, tab64(s), p, o, dncalc);
+    };
+    if (lds) go(k_derivs_iso<double, D, true>); else go(k_derivs_iso<double, D, false>);
   } else {
     auto launch = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),

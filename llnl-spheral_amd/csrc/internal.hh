@@ -131,6 +131,7 @@ struct Sim {
   DBuf rec;
   bool recValid = false;    // records match the caller-order state
   DBuf recM;                // mixed-precision neighbour records (64 B per slot)
+  DBuf recD;                // FP64 records of the collapsed isotropic pair loop (128 B per slot)
   DBuf out;                 // all derivative fields, field-major, sorted order
   int64_t outStride = 0;
   bool derivsValid = false;
