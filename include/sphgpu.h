@@ -244,7 +244,14 @@ typedef struct sphgpu_integrator_params {
   double rhoMin, rhoMax;       /* IncrementBoundedState limits of the mass density   */
   double kernelExtent;         /* TableKernel extent (2)                             */
   int useVelocityMagnitudeForDt;
+  int gradhCorrection;         /* SPHHydroBase gradhCorrection: omega before each derivative pass */
 } sphgpu_integrator_params;
+
+/* computeSPHOmegaGradhCorrection (src/SPH/computeSPHOmegaGradhCorrection.cc:26-133)
+ * over the current neighbours into the resident "grad h corrections" field,
+ * ghosts refreshed from their controls (SPHHydroBase::initialize,
+ * src/SPH/SPHHydroBase.cc:433-455). */
+int sphgpu_omega_gradh(sphgpu_sim* sim);
 
 /* GammaLawGas::setPressure / setSoundSpeed on the internal rows from the
  * resident mass density and specific thermal energy

@@ -283,6 +283,10 @@ int sphgpu_download_ghost_controls(sphgpu_sim* s, int32_t* control, int32_t* sub
   });
 }
 
+int sphgpu_omega_gradh(sphgpu_sim* s) {
+  return guarded(s, [&] { omegaGradh(*s); });
+}
+
 int sphgpu_eos_gamma_law(sphgpu_sim* s, double gamma, double Pmin, double Pmax, int floorType) {
   return guarded(s, [&] { eosGammaLaw(*s, gamma, Pmin, Pmax, floorType); });
 }

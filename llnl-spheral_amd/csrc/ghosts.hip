@@ -639,6 +639,17 @@ void enforceBoundaries(Sim& s) {
   s.derivsValid = false;
 }
 
+void applyGhostScalar(Sim& s, DBuf& field) {
+  if (!s.deviceGhosts || s.nTot == s.nInt) return;
+  for (size_t b = 0; b < s.subs.size(); ++b) {
+    const int64_t g0 = s.nInt + s.subStart[b], g1 = s.nInt + s.subStart[b + 1];
+    if (g1 <= g0) continue;
+    hipLaunchKernelGGL(k_ghost_scalar, dim3(nblocks(g1 - g0)), dim3(kBlock), 0, s.stream, field.as<double>(), g0, g1,
+                       s.nInt, s.ghostControl.as<int32_t>());
+  }
+  SG_CHECK_HIP(hipGetLastError());
+}
+
 void applyGhostDensity(Sim& s) {
   if (!s.deviceGhosts || s.nTot == s.nInt) return;
   hipStream_t st = s.stream;

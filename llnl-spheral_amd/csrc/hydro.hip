@@ -274,6 +274,52 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
 }
 
 // ---------------------------------------------------------------------------
+// computeSPHOmegaGradhCorrection (src/SPH/computeSPHOmegaGradhCorrection.cc:26-133)
+// Gather form of the owner loop: node i receives W(|H_i r_ij|) det H_i and
+// |H_i r_ij| W' from every neighbour in row order, its self term W(0) det H_i
+// after the lower-index neighbours, and omega_i = -sum(eta W')/(nu sum W); a
+// node without neighbours gets 1.  With index-ordered rows this is the
+// reference's sum order.
+// ---------------------------------------------------------------------------
+template <int D, bool ISO, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_omega(const double* __restrict__ rec, const int32_t* __restrict__ rowCount,
+                                                  const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
+                                                  const int32_t* __restrict__ work, TabArgs t,
+                                                  double* __restrict__ omegaOut) {
+  extern __shared__ double lds[];
+  const double* tab = stageTable<LDS>(lds, t);
+  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  if (tw >= nInt) return;
+  const int64_t k = work[tw];
+  const DNbr<D, ISO> me = loadDNbr<D, ISO>(rec, int32_t(k));
+  const int len = rowCount[k];
+  if (len == 0) {
+    omegaOut[int64_t(me.orig)] = 1.0;
+    return;
+  }
+  const double self = lookupW(tab, t, 0.0, me.hdet);
+  double om = 0.0, gs = 0.0;
+  bool selfDone = false;
+  const int4* nb4 = reinterpret_cast<const int4*>(nbr);
+  int4 cur = nb4[ellChunk(0, n, k)];
+  int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
+  DNbr<D, ISO> nx = loadDNbr<D, ISO>(rec, cur.x);
+  for (int w = 0; w < len; ++w) {
+    const DNbr<D, ISO> nb = nx;
+    if (w + 1 < len) nx = loadDNbr<D, ISO>(rec, nextIndex(nb4, n, k, len, w + 1, cur, nxt));
+    if (!selfDone && nb.orig > me.orig) { om += self; selfDone = true; }
+    const V<D> rij = vsub(me.x, nb.x);
+    const double eta = vmag(hdot<D, ISO>(me.H, rij));
+    double Wv, gW;
+    lookupWG(tab, t, eta, me.hdet, Wv, gW);
+    om += Wv;
+    gs += eta * gW;
+  }
+  if (!selfDone) om += self;
+  omegaOut[int64_t(me.orig)] = -gs / (double(D) * om);
+}
+
+// ---------------------------------------------------------------------------
 // evaluateDerivatives, FP64
 // ---------------------------------------------------------------------------
 struct PassArgs {
@@ -1048,6 +1094,35 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
 }
 
 }  // namespace
+
+template <int D>
+void omegaGradhD(Sim& s) {
+  const int64_t n = s.nSlots;
+  TabArgs t = tabArgs(s);
+  const bool lds = s.table.n <= kLdsTablePoints;
+  const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
+                       s.omega.as<double>());
+  };
+  if (s.isotropic) { if (lds) go(k_omega<D, true, true>); else go(k_omega<D, true, false>); }
+  else { if (lds) go(k_omega<D, false, true>); else go(k_omega<D, false, false>); }
+  SG_CHECK_HIP(hipGetLastError());
+}
+
+void omegaGradh(Sim& s) {
+  SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "the grad-h correction needs a neighbour build");
+  SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
+  if (!s.recValid) packRecords(s);
+  if (s.dim == 1) omegaGradhD<1>(s);
+  else if (s.dim == 2) omegaGradhD<2>(s);
+  else omegaGradhD<3>(s);
+  // SPHHydroBase::initialize applies the boundaries to omega
+  applyGhostScalar(s, s.omega);
+  s.recValid = false;         // records hold safeInv(omega)
+  s.derivsValid = false;
+}
 
 void packRecords(Sim& s) {
   if (s.dim == 1) packD<1>(s);

@@ -228,7 +228,8 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
   const Bounds bd{ip.rhoMin, ip.rhoMax, 1.0 / hp.hmax, 1.0 / hp.hmin};
   const Eos e = makeEos(ip.gamma, ip.Pmin, ip.Pmax, ip.pressureFloor);
 
-  // derivatives at t
+  // derivatives at t (SPHHydroBase::initialize first: the grad-h correction)
+  if (ip.gradhCorrection) omegaGradh(s);
   evalDerivs(s, hp);
   // state0
   const size_t n = size_t(s.nInt);
@@ -244,6 +245,7 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
   updateD<D>(s, b0, hdt, bd, e);
   applyGhostBoundaries(s);
   // derivatives at t + dt/2 with the step's neighbour rows
+  if (ip.gradhCorrection) omegaGradh(s);
   evalDerivs(s, hp);
   // advance from state0 with the midpoint derivatives
   updateD<D>(s, b0, dt, bd, e);

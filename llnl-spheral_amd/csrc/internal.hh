@@ -178,6 +178,9 @@ void setBoundaries(Sim& s, int n, const sphgpu_boundary* b);
 void setGhostNodes(Sim& s, double kernelExtent, bool cull);
 void applyGhostBoundaries(Sim& s);
 void applyGhostDensity(Sim& s);   // after sum density: ghost rho and their records
+void applyGhostScalar(Sim& s, DBuf& field);   // ghost rows of one scalar field from their controls
+// hydro.hip
+void omegaGradh(Sim& s);          // computeSPHOmegaGradhCorrection into the omega field
 void enforceBoundaries(Sim& s);   // internal nodes behind a plane are mapped back
 // integrate.hip
 void eosGammaLaw(Sim& s, double gamma, double Pmin, double Pmax, int floorType);

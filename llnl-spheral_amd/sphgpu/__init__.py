@@ -108,7 +108,8 @@ class IntegratorParams(C.Structure):
     _fields_ = [("cfl", C.c_double), ("dtMin", C.c_double), ("dtMax", C.c_double), ("dtGrowth", C.c_double),
                 ("gamma", C.c_double), ("Pmin", C.c_double), ("Pmax", C.c_double), ("pressureFloor", C.c_int),
                 ("cullGhostNodes", C.c_int), ("rhoMin", C.c_double), ("rhoMax", C.c_double),
-                ("kernelExtent", C.c_double), ("useVelocityMagnitudeForDt", C.c_int)]
+                ("kernelExtent", C.c_double), ("useVelocityMagnitudeForDt", C.c_int),
+                ("gradhCorrection", C.c_int)]
 
 
 FLT_MAX = 3.4028234663852886e38
@@ -119,7 +120,7 @@ def integrator_params(**kw):
     defaults of the reference (see include/sphgpu.h)."""
     d = dict(cfl=0.25, dtMin=0.0, dtMax=FLT_MAX, dtGrowth=2.0, gamma=5.0 / 3.0, Pmin=-1.0e200, Pmax=1.0e200,
              pressureFloor=1, cullGhostNodes=1, rhoMin=1.0e-10, rhoMax=1.0e10, kernelExtent=2.0,
-             useVelocityMagnitudeForDt=0)
+             useVelocityMagnitudeForDt=0, gradhCorrection=0)
     d.update(kw)
     return IntegratorParams(**d)
 
@@ -177,6 +178,7 @@ def lib():
             "sphgpu_copy_probe": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
             "sphgpu_set_boundaries": (C.c_int, [vp, C.c_int, C.POINTER(Boundary)]),
             "sphgpu_eos_gamma_law": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int]),
+            "sphgpu_omega_gradh": (C.c_int, [vp]),
             "sphgpu_hydro_dt": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, _dp]),
             "sphgpu_rk2_step": (C.c_int, [vp, C.POINTER(HydroParams), C.POINTER(IntegratorParams), C.c_double, _dp]),
             "sphgpu_get_time": (C.c_int, [vp, _dp, _dp, _i64p]),
@@ -373,6 +375,10 @@ class Sim:
         n = C.c_int64()
         self._chk(self.L.sphgpu_pair_count(self.h, C.byref(n)))
         return n.value
+
+    def omega_gradh(self):
+        """computeSPHOmegaGradhCorrection into the resident omega field."""
+        self._chk(self.L.sphgpu_omega_gradh(self.h))
 
     # time integration ---------------------------------------------------------
     def eos_gamma_law(self, gamma, Pmin=-1.0e200, Pmax=1.0e200, floor=True):

@@ -101,7 +101,7 @@ class RK2:
 
     def __init__(self, dim, state, boundaries, gamma, params=None, table_points=1000, cfl=0.25, dtMin=0.0,
                  dtMax=FLT_MAX, dtGrowth=2.0, rhoMin=1.0e-10, rhoMax=1.0e10, hmin=1.0e-20, hmax=1.0e20,
-                 ext=2.0, cull=True, Pmin=-1.0e200, Pmax=1.0e200):
+                 ext=2.0, cull=True, Pmin=-1.0e200, Pmax=1.0e200, gradh=False):
         self.dim = dim
         self.n = state["pos"].shape[0]
         self.st = {k: np.array(v, dtype=np.float64) for k, v in state.items() if v is not None}
@@ -115,6 +115,7 @@ class RK2:
         self.hmaxInv, self.hminInv = 1.0 / hmax, 1.0 / hmin
         self.ext, self.cull = ext, cull
         self.Pmin, self.Pmax = Pmin, Pmax
+        self.gradh = gradh
         self.st["P"], self.st["cs"] = gamma_law_eos(self.st["rho"], self.st["eps"], gamma, Pmin, Pmax)
         self.t = 0.0
         self.lastDt = 1.0e-5
@@ -159,7 +160,13 @@ class RK2:
         full["H"][:n] = np.minimum(self.hminInv, np.maximum(self.hmaxInv, hid)) * (hid != 0.0) + hid * (hid == 0.0)
         full["P"][:n], full["cs"][:n] = gamma_law_eos(full["rho"][:n], full["eps"][:n], self.gamma, self.Pmin, self.Pmax)
 
-    def _derivs(self, full, off, nbr):
+    def _derivs(self, full, off, nbr, info=None):
+        if self.gradh:
+            # SPHHydroBase::initialize: grad-h correction, then its ghost boundaries
+            full["omega"][:self.n] = oracle.omega_gradh(self.T, self.n, full["pos"], full["H"], off, nbr)
+            if info is not None:
+                full["omega"] = oracle.apply_ghosts(self.dim, self.n, self.b, info["control"], info["sub_of"],
+                                                    full["omega"], 0)
         return oracle.eval_derivs(self.T, self.p, self.n, full, off, nbr)
 
     # -- the step ----------------------------------------------------------------
@@ -176,11 +183,11 @@ class RK2:
         dt = min(dt, self.dtGrowth * self.lastDt)
         dt = min(dtMax, max(dtMin, dt))
         hdt = 0.5 * dt
-        d1 = self._derivs(full, off, nbr)
+        d1 = self._derivs(full, off, nbr, info)
         state0 = {k: full[k][:n].copy() for k in ("pos", "vel", "eps", "rho", "H", "P", "cs")}
         self._update(full, {k: full[k][:n].copy() for k in state0}, d1, hdt)
         self._apply_ghosts(full, info)
-        d2 = self._derivs(full, off, nbr)
+        d2 = self._derivs(full, off, nbr, info)
         for k, v in state0.items():
             full[k][:n] = v
         self._update(full, state0, d2, dt)

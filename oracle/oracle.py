@@ -128,6 +128,7 @@ def lib(fast=False):
                                       _i64p, _i32p, _dp]
         L.orc_eval_derivs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params), C.c_int64,
                                       C.c_int64, C.POINTER(State), _i64p, _i32p, C.POINTER(Outputs)]
+        L.orc_omega_gradh.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int64, _dp, _dp, _i64p, _i32p, _dp]
         L.orc_gamma_law.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     _dp, _dp]
         L.orc_num_threads.restype = C.c_int
@@ -214,6 +215,15 @@ def sum_density(table, n_internal, pos, mass, H, off, nbr, gather=False):
                       _p(np.ascontiguousarray(H, dtype=np.float64)), _p(off, _i64p),
                       _p(nbr, _i32p), _p(rho))
     return rho
+
+
+def omega_gradh(table, n_internal, pos, H, off, nbr):
+    """computeSPHOmegaGradhCorrection restated (internal nodes)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    om = np.zeros(n_internal)
+    table.L.orc_omega_gradh(table.h, table.dim, n_internal, pos.shape[0], _p(pos),
+                            _p(np.ascontiguousarray(H, dtype=np.float64)), _p(off, _i64p), _p(nbr, _i32p), _p(om))
+    return om
 
 
 def make_params(**kw):

@@ -174,6 +174,40 @@ void sumDensity(const Table& W, int64_t nInt, int64_t nTot, const double* pos, c
   for (int64_t i = 0; i < nInt; ++i) rhoOut[i] = rho[i];
 }
 
+// computeSPHOmegaGradhCorrection (src/SPH/computeSPHOmegaGradhCorrection.cc:26-133):
+// each pair once by its owner, W and eta*W' added to both ends, the self term
+// W(0) when the owner's turn comes, then omega_i = -sum(eta W')/(nu sum W);
+// nodes without neighbours get 1.
+template <int D>
+void omegaGradh(const Table& W, int64_t nInt, int64_t nTot, const double* pos, const double* H, const int64_t* off,
+                const int32_t* nbr, double* out) {
+  NodeState<D> s{}; s.pos = pos; s.H = H;
+  std::vector<double> om(static_cast<size_t>(nTot), 0.0), gs(static_cast<size_t>(nTot), 0.0);
+  for (int64_t i = 0; i < nInt; ++i) {
+    if (off[i + 1] == off[i]) { om[size_t(i)] = 1.0; continue; }
+    const Vec<D> ri = s.r(i); const Sym<D> Hi = s.h(i);
+    const double Hdeti = sdet(Hi);
+    om[size_t(i)] += W.kernelValue(0.0, Hdeti);
+    for (int64_t k = off[i]; k < off[i + 1]; ++k) {
+      const int64_t j = nbr[k];
+      if (!(j > i)) continue;
+      const Vec<D> rj = s.r(j); const Sym<D> Hj = s.h(j);
+      const double Hdetj = sdet(Hj);
+      const Vec<D> rij = sub(ri, rj);
+      const double etai = mag(sdot(Hi, rij));
+      const double etaj = mag(sdot(Hj, rij));
+      const double Wi = W.kernelValue(etai, Hdeti), gWi = W.gradValue(etai, Hdeti);
+      const double Wj = W.kernelValue(etaj, Hdetj), gWj = W.gradValue(etaj, Hdetj);
+      om[size_t(i)] += Wi;
+      gs[size_t(i)] += etai * gWi;
+      om[size_t(j)] += Wj;
+      gs[size_t(j)] += etaj * gWj;
+    }
+    om[size_t(i)] = -gs[size_t(i)] / (double(D) * om[size_t(i)]);
+  }
+  for (int64_t i = 0; i < nInt; ++i) out[i] = om[size_t(i)];
+}
+
 // Gather form: node i walks its row; the self term is inserted where the
 // reference adds it (after all j < i, before all j > i).
 template <int D>
@@ -555,6 +589,14 @@ void orc_sum_density(void* tp, int dim, int gather, int64_t nInt, int64_t nTot, 
   if (dim == 1) sumDensity<1>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
   else if (dim == 2) sumDensity<2>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
   else sumDensity<3>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
+}
+
+void orc_omega_gradh(void* tp, int dim, int64_t nInt, int64_t nTot, const double* pos, const double* H,
+                     const int64_t* off, const int32_t* nbr, double* omega) {
+  const Table& W = *static_cast<Table*>(tp);
+  if (dim == 1) orc::omegaGradh<1>(W, nInt, nTot, pos, H, off, nbr, omega);
+  else if (dim == 2) orc::omegaGradh<2>(W, nInt, nTot, pos, H, off, nbr, omega);
+  else orc::omegaGradh<3>(W, nInt, nTot, pos, H, off, nbr, omega);
 }
 
 // ---- evaluateDerivatives ---------------------------------------------------

@@ -110,3 +110,42 @@ def test_hydro_dt_vote(sphgpu, oracle):
     prev = {"maxViscousPressure": np.zeros(N), "DvDx": np.zeros((N, 9)), "DvDt": np.zeros((N, 3))}
     ref = integrator.hydro_dt(3, N, st, prev, off, nbr, 1.51, 0.25)
     assert s.hydro_dt(1.51, 0.25) == ref
+
+
+@pytest.mark.parametrize("dim,n", [(1, 120), (2, 18), (3, 8)])
+def test_omega_gradh_matches_oracle(sphgpu, oracle, dim, n):
+    """computeSPHOmegaGradhCorrection on the device, reference row order:
+    equal value for value (ghost rows copied from their controls)."""
+    st, _ = problems.random_state(dim, n, seed=70 + dim, jitter=0.2, h_var=0.15)
+    N = st["pos"].shape[0]
+    bo = oracle.box_boundaries(dim)
+    full, off, nbr, info = oracle.ghost_pipeline(dim, st, bo)
+    T = oracle.Table(dim, 1000)
+    om = oracle.omega_gradh(T, N, full["pos"], full["H"], off, nbr)
+    s = sphgpu.Sim(dim)
+    s.set_table_bspline(1000)
+    s.set_nodes(N, N)
+    s.upload_state(st)
+    s.set_boundaries(sphgpu.box_boundaries(dim))
+    s.set_ghost_nodes(2.0)
+    s.omega_gradh()
+    got = s.download("grad h corrections")
+    om_all = np.array(full["omega"])
+    om_all[:N] = om
+    om_all = oracle.apply_ghosts(dim, N, bo, info["control"], info["sub_of"], om_all, 0)
+    assert np.array_equal(got, om_all)
+
+
+def test_rk2_with_gradh(sphgpu, oracle):
+    import integrator
+    st, _ = problems.random_state(1, 100, seed=63, jitter=0.1, vel_scale=0.5)
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(1, base, oracle.box_boundaries(1), 1.4, gradh=True)
+    s = device(sphgpu, 1, st, "reflecting", 1.4)
+    hp = sphgpu.hydro_params()
+    ip = sphgpu.integrator_params(gamma=1.4, gradhCorrection=1)
+    for k in range(4):
+        assert s.rk2_step(hp, ip) == ref.step()
+    N = st["pos"].shape[0]
+    for key in ("pos", "vel", "eps", "rho", "H"):
+        assert np.array_equal(s.download(FIELDS[key], rows=N).reshape(ref.st[key].shape), ref.st[key]), key

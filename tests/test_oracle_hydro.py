@@ -196,3 +196,30 @@ def test_gamma_law(oracle):
     assert cs == pytest.approx(np.sqrt(1.4 * 0.4 * eps), rel=1e-15)
     P2, _ = oracle.gamma_law(np.array([1.0]), np.array([-1.0]), 1.4, Pmin=0.0)
     assert P2[0] == 0.0
+
+
+def test_omega_gradh_lattice(oracle):
+    """computeSPHOmegaGradhCorrection on a uniform lattice interior equals the
+    independent lattice sum -sum(eta W')/(nu (W(0) + sum W)); a node without
+    neighbours gets 1."""
+    dim, n = 3, 12
+    pos, mass, H, dx = problems.lattice(dim, n)
+    N = pos.shape[0]
+    off, nbr = oracle.connectivity(dim, N, pos, H)
+    T = oracle.Table(dim, 1000)
+    om = oracle.omega_gradh(T, N, pos, H, off, nbr)
+    interior = np.all((pos > 3.5 * dx) & (pos < 1 - 3.5 * dx), axis=1)
+    Hdet = (1 / (1.51 * dx)) ** 3
+    r = np.arange(-4, 5)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3) * dx
+    eta = np.sqrt((g ** 2).sum(1)) / (1.51 * dx)
+    nz = (eta > 0) & (eta < 2.0)
+    Wsum = T.W(0.0, Hdet) + sum(T.W(e, Hdet) for e in eta[nz])
+    gsum = sum(e * T.gradW(e, Hdet) for e in eta[nz])
+    assert np.allclose(om[interior], -gsum / (3 * Wsum), rtol=1e-12, atol=0)
+    assert np.all(om > 0)
+    # an isolated node
+    p1 = np.array([[0.0, 0.0, 0.0], [5.0, 5.0, 5.0]])
+    H1 = problems.sym_identity(3, 2, 1.0)
+    o1, n1 = oracle.connectivity(3, 2, p1, H1)
+    assert np.array_equal(oracle.omega_gradh(T, 2, p1, H1, o1, n1), [1.0, 1.0])
