@@ -108,7 +108,9 @@ typedef struct sphgpu_hydro_params {
   int evolveTotalEnergy;       /* 0 */
   int linearInExpansion;       /* 0 */
   int quadraticInExpansion;    /* 0 */
-  int compatibleEnergyEvolution; /* must be 0: pairAccelerations not produced */
+  int compatibleEnergyEvolution; /* 0; 1 (FP64 only): the pass keeps every row
+                                    entry's pair term for the RK2 step's
+                                    SpecificThermalEnergyPolicy */
   int precision;               /* SPHGPU_PRECISION_* */
 } sphgpu_hydro_params;
 
@@ -267,6 +269,8 @@ int sphgpu_hydro_dt(sphgpu_sim* sim, double nPerh, double cfl, int useVelocityMa
  * neighbours at the step start, dt selection, derivatives at t, update to
  * t + dt/2, ghosts, derivatives at t + dt/2, update from the saved state,
  * ghosts, sum density (SPHHydroBase::finalize), boundary enforcement.
+ * hydro->compatibleEnergyEvolution selects SpecificThermalEnergyPolicy for
+ * eps (src/Hydro/SpecificThermalEnergyPolicy.cc:64-189).
  * Advances the sim's time; returns the step taken in *dt. */
 int sphgpu_rk2_step(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, const sphgpu_integrator_params* integ,
                     double max_time, double* dt);

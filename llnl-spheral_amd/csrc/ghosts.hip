@@ -242,6 +242,18 @@ __global__ void k_ghost_scalar(double* __restrict__ a, int64_t g0, int64_t g1, i
   if (g < g1) a[g] = a[control[g - nInt]];
 }
 
+// A Vector field on ghost rows [g0, g1): R.a of the control for a reflecting
+// plane, a copy for a periodic one (ReflectingBoundary.cc:106-125).
+template <int D>
+__global__ void k_ghost_vector(double* __restrict__ a, int64_t g0, int64_t g1, int64_t nInt,
+                               const int32_t* __restrict__ control, DSub b) {
+  const int64_t g = g0 + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (g >= g1) return;
+  const V<D> v = loadV<D>(a, control[g - nInt]);
+  const V<D> o = b.reflect ? reflectV<D>(b.R, v) : v;
+  for (int k = 0; k < D; ++k) a[g * D + k] = o.c[k];
+}
+
 // Ghost records after sum density: the same expressions as the internal update.
 template <bool ISO>
 __global__ void k_ghost_records(double* __restrict__ rec, const int32_t* __restrict__ iperm,
@@ -646,6 +658,26 @@ void applyGhostScalar(Sim& s, DBuf& field) {
     if (g1 <= g0) continue;
     hipLaunchKernelGGL(k_ghost_scalar, dim3(nblocks(g1 - g0)), dim3(kBlock), 0, s.stream, field.as<double>(), g0, g1,
                        s.nInt, s.ghostControl.as<int32_t>());
+  }
+  SG_CHECK_HIP(hipGetLastError());
+}
+
+void applyGhostVector(Sim& s, DBuf& field) {
+  if (!s.deviceGhosts || s.nTot == s.nInt) return;
+  for (size_t b = 0; b < s.subs.size(); ++b) {
+    const int64_t g0 = s.nInt + s.subStart[b], g1 = s.nInt + s.subStart[b + 1];
+    if (g1 <= g0) continue;
+    const DSub d = toDevice(s.subs[b]);
+    const int32_t* ctl = s.ghostControl.as<int32_t>();
+    if (s.dim == 1)
+      hipLaunchKernelGGL(k_ghost_vector<1>, dim3(nblocks(g1 - g0)), dim3(kBlock), 0, s.stream, field.as<double>(), g0,
+                         g1, s.nInt, ctl, d);
+    else if (s.dim == 2)
+      hipLaunchKernelGGL(k_ghost_vector<2>, dim3(nblocks(g1 - g0)), dim3(kBlock), 0, s.stream, field.as<double>(), g0,
+                         g1, s.nInt, ctl, d);
+    else
+      hipLaunchKernelGGL(k_ghost_vector<3>, dim3(nblocks(g1 - g0)), dim3(kBlock), 0, s.stream, field.as<double>(), g0,
+                         g1, s.nInt, ctl, d);
   }
   SG_CHECK_HIP(hipGetLastError());
 }

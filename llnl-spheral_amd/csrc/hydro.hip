@@ -339,6 +339,9 @@ struct OutArgs {
   double* base;      // field-major blocks, each n rows
   int64_t n;         // rows per block (sorted slots)
   int off[SPHGPU_NUM_DERIV_FIELDS];
+  double* pairDv;    // compatible energy: the pair term of every row entry in the node's own
+                     // frame, component c at pairDv[c*plane + ellIndex(w, n, k)]; null otherwise
+  int64_t plane;
 };
 
 __device__ __forceinline__ double* outRow(const OutArgs& o, int field, int64_t k, int comps) {
@@ -541,6 +544,7 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, const double* __re
       const double qs = ownerI ? Qaccj.c[c] : Qacci.c[c];
       const double dv = gradWi.c[c] * Prhoi + gradWj.c[c] * Prhoj + qf + qs;
       DvDt.c[c] += dv * (-mj);
+      if (o.pairDv) o.pairDv[int64_t(c) * o.plane + ellIndex(w, n, k)] = dv;
     }
     DepsDt += mj * (Prhoi * vdot(vij, gradWi) + workQi);
     for (int a = 0; a < D; ++a)
@@ -1048,6 +1052,13 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   o.n = n;
   for (int f = 0; f < SPHGPU_NUM_DERIV_FIELDS; ++f) o.off[f] = derivOffsetDoubles(s.dim, f);
   const bool tensile = hp.epsTensile != 0.0;
+  const bool compat = hp.compatibleEnergyEvolution != 0;
+  if (compat) {
+    // pairAccelerations: the own-frame pair term of every row entry
+    o.plane = n * int64_t(s.ellCap);
+    s.pairDv.ensure(sizeof(double) * size_t(D) * size_t(std::max<int64_t>(o.plane, 1)));
+    o.pairDv = s.pairDv.as<double>();
+  }
   if (hp.precision == SPHGPU_PRECISION_MIXED) {
     SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
                "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
@@ -1060,7 +1071,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
                          s.work.as<int32_t>(), t, tab32(s), p, o, dncalc);
     };
     if (lds) go(k_derivs_iso<float, D, true>); else go(k_derivs_iso<float, D, false>);
-  } else if (s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile) {
+  } else if (s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile && !compat) {
     // search row order is held to round-off of the reference sums, not to
     // their bits: the collapsed isotropic algebra in FP64
     packFast64<D>(s);
@@ -1148,8 +1159,10 @@ void sumDensity(Sim& s) {
 void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "evaluateDerivatives needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
-  SG_REQUIRE(p.compatibleEnergyEvolution == 0, SPHGPU_ERR_UNSUPPORTED,
-             "compatibleEnergyEvolution (pairAccelerations) is not produced by the device pass");
+  SG_REQUIRE(p.compatibleEnergyEvolution == 0 || p.precision == SPHGPU_PRECISION_FP64, SPHGPU_ERR_UNSUPPORTED,
+             "compatibleEnergyEvolution (pairAccelerations) needs the FP64 pass");
+  SG_REQUIRE(!(p.compatibleEnergyEvolution && p.evolveTotalEnergy), SPHGPU_ERR_ARG,
+             "compatibleEnergyEvolution and evolveTotalEnergy exclude each other");
   SG_REQUIRE(p.precision == SPHGPU_PRECISION_FP64 || p.precision == SPHGPU_PRECISION_MIXED, SPHGPU_ERR_ARG,
              "unknown precision");
   SG_REQUIRE(p.nPerh > 0.0, SPHGPU_ERR_ARG, "nPerh must be positive");

@@ -15,6 +15,10 @@
 //   H = ideal H with eigenvalues bounded by 1/hmax, 1/hmin (ReplaceBoundedState;
 //       the SPH ideal H is h1^-1 I, so the bound is a clamp of the diagonal)
 //   P, c from the gamma-law EOS                         (Pressure/SoundSpeedPolicy)
+// With compatibleEnergyEvolution, eps follows SpecificThermalEnergyPolicy
+// (src/Hydro/SpecificThermalEnergyPolicy.cc:64-189): eps += m*DepsDt with
+// DepsDt rebuilt from the pair accelerations of the pass and the step-start
+// velocities.
 // The derivative fields are read in slot order (one lane per internal slot of
 // the work list) and the state is written at the caller index of the slot.
 #include "internal.hh"
@@ -63,6 +67,7 @@ struct DerivIn {
   const double* base;  // field-major blocks of the last evaluation, slot rows
   int64_t n;
   int oDx, oDv, oDeps, oDrho, oHid;
+  const double* compDe;  // compatible energy: DepsDt of the policy per slot, else null
 };
 struct Bounds {
   double rhoMin, rhoMax, hmaxInv, hminInv;
@@ -82,7 +87,7 @@ __global__ void k_update(const int32_t* __restrict__ work, const int32_t* __rest
     f.pos[i * D + c] = b.pos[i * D + c] + mult * dx[c];
     f.vel[i * D + c] = b.vel[i * D + c] + mult * dv[c];
   }
-  const double eps = b.eps[i] + mult * d.base[int64_t(d.oDeps) * d.n + k];
+  const double eps = d.compDe ? b.eps[i] + d.compDe[k] * mult : b.eps[i] + mult * d.base[int64_t(d.oDeps) * d.n + k];
   const double rho = dmin(bd.rhoMax, dmax(bd.rhoMin, b.rho[i] + mult * d.base[int64_t(d.oDrho) * d.n + k]));
   f.eps[i] = eps;
   f.rho[i] = rho;
@@ -164,14 +169,125 @@ __global__ void k_dt(const int32_t* __restrict__ work, const int32_t* __restrict
 
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// ---- SpecificThermalEnergyPolicy ------------------------------------------
+// DvDt and DepsDt of the internal slots to caller order; the ghost rows are
+// then filled by the boundaries (SPHHydroBase::finalizeDerivatives).
+template <int D>
+__global__ void k_compat_scatter(const int32_t* __restrict__ work, const int32_t* __restrict__ perm, int64_t nInt,
+                                 const double* __restrict__ out, int64_t n, int oDv, int oDeps,
+                                 double* __restrict__ acc, double* __restrict__ de0) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= nInt) return;
+  const int64_t k = work[t];
+  const int64_t i = perm[k];
+  for (int c = 0; c < D; ++c) acc[i * D + c] = out[int64_t(oDv) * n + k * D + c];
+  de0[i] = out[int64_t(oDeps) * n + k];
+}
+
+// Per slot: v + a hdt, |DepsDt0| + epsilon, m and the caller index (the pair
+// owner is the node of smaller caller index).  Dead slots are in no row.
+template <int D>
+__global__ void k_compat_rec(const int32_t* __restrict__ perm, int64_t nSlots, const double* __restrict__ vel0,
+                             const double* __restrict__ acc, const double* __restrict__ de0,
+                             const double* __restrict__ mass, double hdt, double* __restrict__ crec) {
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k >= nSlots) return;
+  const int64_t i = perm[k];
+  double r[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i >= 0) {
+    for (int c = 0; c < D; ++c) r[c] = vel0[i * D + c] + acc[i * D + c] * hdt;
+    r[3] = dabs(de0[i]) + DBL_EPSILON;
+    r[4] = mass[i];
+    r[5] = double(i);
+  }
+  double2* o = reinterpret_cast<double2*>(crec + k * 8);
+  o[0] = make_double2(r[0], r[1]);
+  o[1] = make_double2(r[2], r[3]);
+  o[2] = make_double2(r[4], r[5]);
+}
+
+// DepsDt of one internal node, gathered along its row.  The row holds the
+// pass's own-frame pair term dv of each neighbour j; the reference's pair
+// accelerations are -m_j dv for the node's own pairs (i owns when its caller
+// index is the smaller) and m_i dv for the neighbour's end of a pair j owns.
+// The owner end receives w_i du, the other end (1 - w_i) du m_i/m_j, written
+// here from the receiving node with the owner's operands in the reference's
+// order.  With rows in caller-index order the sum runs in the reference's
+// order (owners j < i ascending, then the node's own pairs).
+template <int D>
+__global__ void k_compat(const int32_t* __restrict__ work, const int32_t* __restrict__ rowCount,
+                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt, const double* __restrict__ crec,
+                         const double* __restrict__ pairDv, int64_t plane, double* __restrict__ compDe) {
+  const int64_t t = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  if (t >= nInt) return;
+  const int64_t k = work[t];
+  const double2* mr = reinterpret_cast<const double2*>(crec + k * 8);
+  const double2 a0 = mr[0], a1 = mr[1], a2 = mr[2];
+  const double vi[3] = {a0.x, a0.y, a1.x};
+  const double wti = a1.y, mi = a2.x, oi = a2.y;
+  double de = 0.0;
+  const int len = rowCount[k];
+  for (int w = 0; w < len; ++w) {
+    const int64_t e = ellIndex(w, n, k);
+    const double2* nr = reinterpret_cast<const double2*>(crec + int64_t(nbr[e]) * 8);
+    const double2 b0 = nr[0], b1 = nr[1], b2 = nr[2];
+    const double vj[3] = {b0.x, b0.y, b1.x};
+    const double wtj = b1.y, mj = b2.x, oj = b2.y;
+    double du = 0.0;
+    if (oj > oi) {
+      for (int c = 0; c < D; ++c) {
+        const double pa = pairDv[int64_t(c) * plane + e] * (-mj);
+        const double tc = (vj[c] - vi[c]) * pa;
+        du = (c == 0) ? tc : du + tc;
+      }
+      const double wi = wti / (wti + wtj);
+      de += wi * du;
+    } else {
+      for (int c = 0; c < D; ++c) {
+        const double pa = pairDv[int64_t(c) * plane + e] * mi;
+        const double tc = (vi[c] - vj[c]) * pa;
+        du = (c == 0) ? tc : du + tc;
+      }
+      const double wj = wtj / (wtj + wti);
+      de += (1.0 - wj) * du * mj / mi;
+    }
+  }
+  compDe[k] = de;
+}
+
 Eos makeEos(double gamma, double Pmin, double Pmax, int floor) { return Eos{gamma, gamma - 1.0, Pmin, Pmax, floor}; }
 
-template <int D> void updateD(Sim& s, const Base& b, double mult, const Bounds& bd, const Eos& e) {
+// SpecificThermalEnergyPolicy::update's DepsDt into s.compDe (slot rows) for
+// an update by `mult`, with vel0 the velocities the state holds before it.
+template <int D> void compatD(Sim& s, const double* vel0, double mult) {
+  hipStream_t st = s.stream;
+  const size_t nt = size_t(std::max<int64_t>(s.nTot, 1)), ns = size_t(std::max<int64_t>(s.nSlots, 1));
+  s.compAcc.ensure(sizeof(double) * D * nt);
+  s.compDe0.ensure(sizeof(double) * nt);
+  s.compRec.ensure(sizeof(double) * 8 * ns);
+  s.compDe.ensure(sizeof(double) * ns);
+  hipLaunchKernelGGL(k_compat_scatter<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.work.as<int32_t>(),
+                     s.perm.as<int32_t>(), s.nInt, s.out.as<double>(), s.outStride,
+                     derivOffsetDoubles(D, SPHGPU_D_DVDT), derivOffsetDoubles(D, SPHGPU_D_DEPSDT),
+                     s.compAcc.as<double>(), s.compDe0.as<double>());
+  applyGhostVector(s, s.compAcc);
+  applyGhostScalar(s, s.compDe0);
+  hipLaunchKernelGGL(k_compat_rec<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), s.nSlots,
+                     vel0, s.compAcc.as<double>(), s.compDe0.as<double>(), s.mass.as<double>(), 0.5 * mult,
+                     s.compRec.as<double>());
+  hipLaunchKernelGGL(k_compat<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
+                     s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots, s.nInt, s.compRec.as<double>(),
+                     s.pairDv.as<double>(), s.nSlots * int64_t(s.ellCap), s.compDe.as<double>());
+  SG_CHECK_HIP(hipGetLastError());
+}
+
+template <int D> void updateD(Sim& s, const Base& b, double mult, const Bounds& bd, const Eos& e,
+                              const double* compDe = nullptr) {
   Fields f{s.pos.as<double>(), s.vel.as<double>(), s.eps.as<double>(), s.rho.as<double>(), s.H.as<double>(),
            s.P.as<double>(), s.cs.as<double>()};
   DerivIn d{s.out.as<double>(), s.outStride, derivOffsetDoubles(D, SPHGPU_D_DXDT), derivOffsetDoubles(D, SPHGPU_D_DVDT),
             derivOffsetDoubles(D, SPHGPU_D_DEPSDT), derivOffsetDoubles(D, SPHGPU_D_DRHODT),
-            derivOffsetDoubles(D, SPHGPU_D_HIDEAL)};
+            derivOffsetDoubles(D, SPHGPU_D_HIDEAL), compDe};
   hipLaunchKernelGGL(k_update<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
                      s.perm.as<int32_t>(), s.nInt, f, b, d, mult, bd, e);
   SG_CHECK_HIP(hipGetLastError());
@@ -225,15 +341,18 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
   dt = std::min(dt, ip.dtGrowth * s.lastDt);
   dt = std::min(dtMax, std::max(dtMin, dt));
   const double hdt = 0.5 * dt;
+  const bool compat = hp.compatibleEnergyEvolution != 0;
+  SG_REQUIRE(!compat || s.deviceGhosts || s.nTot == s.nInt, SPHGPU_ERR_UNSUPPORTED,
+             "compatible energy with caller-supplied ghost rows: the ghost derivatives need the device boundaries");
   const Bounds bd{ip.rhoMin, ip.rhoMax, 1.0 / hp.hmax, 1.0 / hp.hmin};
   const Eos e = makeEos(ip.gamma, ip.Pmin, ip.Pmax, ip.pressureFloor);
 
   // derivatives at t (SPHHydroBase::initialize first: the grad-h correction)
   if (ip.gradhCorrection) omegaGradh(s);
   evalDerivs(s, hp);
-  // state0
-  const size_t n = size_t(s.nInt);
-  const size_t bytes[5] = {sizeof(double) * n * D, sizeof(double) * n * D, sizeof(double) * n, sizeof(double) * n,
+  // state0 (the policy reads every row's step-start velocity, ghosts included)
+  const size_t n = size_t(s.nInt), nv = compat ? size_t(s.nTot) : n;
+  const size_t bytes[5] = {sizeof(double) * n * D, sizeof(double) * nv * D, sizeof(double) * n, sizeof(double) * n,
                            sizeof(double) * n * NS};
   DBuf* src[5] = {&s.pos, &s.vel, &s.eps, &s.rho, &s.H};
   for (int q = 0; q < 5; ++q) {
@@ -242,13 +361,15 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
   }
   const Base b0{s.state0[0].as<double>(), s.state0[1].as<double>(), s.state0[2].as<double>(), s.state0[3].as<double>()};
   // trial advance to the midpoint
-  updateD<D>(s, b0, hdt, bd, e);
+  if (compat) compatD<D>(s, b0.vel, hdt);
+  updateD<D>(s, b0, hdt, bd, e, compat ? s.compDe.as<double>() : nullptr);
   applyGhostBoundaries(s);
   // derivatives at t + dt/2 with the step's neighbour rows
   if (ip.gradhCorrection) omegaGradh(s);
   evalDerivs(s, hp);
   // advance from state0 with the midpoint derivatives
-  updateD<D>(s, b0, dt, bd, e);
+  if (compat) compatD<D>(s, b0.vel, dt);
+  updateD<D>(s, b0, dt, bd, e, compat ? s.compDe.as<double>() : nullptr);
   applyGhostBoundaries(s);
   dtInputsD<D>(s);
   // SPHHydroBase::finalize: sum density

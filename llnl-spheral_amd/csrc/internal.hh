@@ -132,6 +132,10 @@ struct Sim {
   bool recValid = false;    // records match the caller-order state
   DBuf recM;                // mixed-precision neighbour records (64 B per slot)
   DBuf recD;                // FP64 records of the collapsed isotropic pair loop (128 B per slot)
+  DBuf pairDv;              // compatible energy: own-frame pair term per row entry (D planes)
+  DBuf compDe;              // compatible energy: DepsDt of the policy, per slot
+  DBuf compAcc, compDe0;    // compatible energy: DvDt and DepsDt by caller index, ghost rows from the boundaries
+  DBuf compRec;             // compatible energy: per slot [v + a hdt (3), |DepsDt0| + epsilon, m, caller index]
   DBuf out;                 // all derivative fields, field-major, sorted order
   int64_t outStride = 0;
   bool derivsValid = false;
@@ -179,6 +183,7 @@ void setGhostNodes(Sim& s, double kernelExtent, bool cull);
 void applyGhostBoundaries(Sim& s);
 void applyGhostDensity(Sim& s);   // after sum density: ghost rho and their records
 void applyGhostScalar(Sim& s, DBuf& field);   // ghost rows of one scalar field from their controls
+void applyGhostVector(Sim& s, DBuf& field);   // ghost rows of one Vector field (R.a or a copy)
 // hydro.hip
 void omegaGradh(Sim& s);          // computeSPHOmegaGradhCorrection into the omega field
 void enforceBoundaries(Sim& s);   // internal nodes behind a plane are mapped back

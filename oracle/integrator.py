@@ -101,7 +101,7 @@ class RK2:
 
     def __init__(self, dim, state, boundaries, gamma, params=None, table_points=1000, cfl=0.25, dtMin=0.0,
                  dtMax=FLT_MAX, dtGrowth=2.0, rhoMin=1.0e-10, rhoMax=1.0e10, hmin=1.0e-20, hmax=1.0e20,
-                 ext=2.0, cull=True, Pmin=-1.0e200, Pmax=1.0e200, gradh=False):
+                 ext=2.0, cull=True, Pmin=-1.0e200, Pmax=1.0e200, gradh=False, compatible=False):
         self.dim = dim
         self.n = state["pos"].shape[0]
         self.st = {k: np.array(v, dtype=np.float64) for k, v in state.items() if v is not None}
@@ -116,6 +116,7 @@ class RK2:
         self.ext, self.cull = ext, cull
         self.Pmin, self.Pmax = Pmin, Pmax
         self.gradh = gradh
+        self.compatible = compatible
         self.st["P"], self.st["cs"] = gamma_law_eos(self.st["rho"], self.st["eps"], gamma, Pmin, Pmax)
         self.t = 0.0
         self.lastDt = 1.0e-5
@@ -152,9 +153,13 @@ class RK2:
         """State::update on the internal rows of `full` from `base` (the
         current state or the saved state0)."""
         n, dim = self.n, self.dim
+        if self.compatible:
+            eps = self._compatible_eps(full, base, d, mult)
+        else:
+            eps = base["eps"] + mult * d["DepsDt"]
         full["pos"][:n] = base["pos"] + mult * d["DxDt"].reshape(n, dim)
         full["vel"][:n] = base["vel"] + mult * d["DvDt"].reshape(n, dim)
-        full["eps"][:n] = base["eps"] + mult * d["DepsDt"]
+        full["eps"][:n] = eps
         full["rho"][:n] = np.minimum(self.rhoMax, np.maximum(self.rhoMin, base["rho"] + mult * d["DrhoDt"]))
         hid = d["Hideal"].reshape(n, self.nsym)
         full["H"][:n] = np.minimum(self.hminInv, np.maximum(self.hmaxInv, hid)) * (hid != 0.0) + hid * (hid == 0.0)
@@ -167,7 +172,29 @@ class RK2:
             if info is not None:
                 full["omega"] = oracle.apply_ghosts(self.dim, self.n, self.b, info["control"], info["sub_of"],
                                                     full["omega"], 0)
-        return oracle.eval_derivs(self.T, self.p, self.n, full, off, nbr)
+        d = oracle.eval_derivs(self.T, self.p, self.n, full, off, nbr, pair_acc=self.compatible)
+        if self.compatible:
+            # SPHHydroBase::finalizeDerivatives: boundaries on DvDt and DepsDt
+            n, dim = self.n, self.dim
+            nt = full["pos"].shape[0]
+            acc = np.zeros((nt, dim))
+            acc[:n] = d["DvDt"].reshape(n, dim)
+            de0 = np.zeros(nt)
+            de0[:n] = d["DepsDt"]
+            if info is not None:
+                acc = oracle.apply_ghosts(dim, n, self.b, info["control"], info["sub_of"], acc, 1)
+                de0 = oracle.apply_ghosts(dim, n, self.b, info["control"], info["sub_of"], de0, 0)
+            d["_acc_all"], d["_depsdt0_all"] = acc, de0
+            d["_rows"] = (off, nbr)
+        return d
+
+    def _compatible_eps(self, full, base, d, mult):
+        """SpecificThermalEnergyPolicy: eps from the pair accelerations, with
+        the velocities the state holds before the update."""
+        off, nbr = d["_rows"]
+        de = oracle.compatible_depsdt(self.dim, self.n, off, nbr, full["mass"], full["vel"], d["_acc_all"],
+                                      d["_depsdt0_all"], d["pairAcc"], mult)
+        return base["eps"] + de * mult
 
     # -- the step ----------------------------------------------------------------
     def step(self, maxTime=FLT_MAX):
@@ -184,13 +211,14 @@ class RK2:
         dt = min(dtMax, max(dtMin, dt))
         hdt = 0.5 * dt
         d1 = self._derivs(full, off, nbr, info)
-        state0 = {k: full[k][:n].copy() for k in ("pos", "vel", "eps", "rho", "H", "P", "cs")}
-        self._update(full, {k: full[k][:n].copy() for k in state0}, d1, hdt)
+        # State state0(state): every registered field, ghost rows included
+        state0 = {k: full[k].copy() for k in ("pos", "vel", "mass", "eps", "rho", "H", "P", "cs", "omega")}
+        self._update(full, {k: v[:n].copy() for k, v in state0.items()}, d1, hdt)
         self._apply_ghosts(full, info)
         d2 = self._derivs(full, off, nbr, info)
-        for k, v in state0.items():
-            full[k][:n] = v
-        self._update(full, state0, d2, dt)
+        for k, v in state0.items():       # state.assign(state0)
+            full[k][:] = v
+        self._update(full, {k: v[:n] for k, v in state0.items()}, d2, dt)
         self._apply_ghosts(full, info)
         # SPHHydroBase::finalize: sum density with the step's connectivity
         full["rho"][:n] = oracle.sum_density(self.T, n, full["pos"], full["mass"], full["H"], off, nbr)

@@ -85,7 +85,7 @@ def box_boundaries(dim, lo=0.0, hi=1.0, kind="reflecting"):
 
 
 class Outputs(C.Structure):
-    _fields_ = [(n, _dp) for n in OUTPUT_FIELDS] + [("ncalc", _i64p)]
+    _fields_ = [(n, _dp) for n in OUTPUT_FIELDS] + [("ncalc", _i64p), ("pairAcc", _dp)]
 
 
 def ncomp(field, dim):
@@ -128,6 +128,8 @@ def lib(fast=False):
                                       _i64p, _i32p, _dp]
         L.orc_eval_derivs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params), C.c_int64,
                                       C.c_int64, C.POINTER(State), _i64p, _i32p, C.POINTER(Outputs)]
+        L.orc_compatible_depsdt.argtypes = [C.c_int, C.c_int64, C.c_int64, _i64p, _i32p, _dp, _dp, _dp, _dp, _dp,
+                                            C.c_double, _dp]
         L.orc_omega_gradh.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int64, _dp, _dp, _i64p, _i32p, _dp]
         L.orc_gamma_law.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     _dp, _dp]
@@ -234,7 +236,7 @@ def make_params(**kw):
     return Params(**d)
 
 
-def eval_derivs(table, params, n_internal, state, off, nbr, gather=False):
+def eval_derivs(table, params, n_internal, state, off, nbr, gather=False, pair_acc=False):
     """SPHHydroBase::evaluateDerivatives restated.  state: dict of numpy arrays
     (pos, vel, mass, rho, eps, P, cs, H, omega, optional fCl, fCq, fshear) over all
     nodes.  Returns dict of per-internal-node outputs (+ 'ncalc' for scatter)."""
@@ -249,14 +251,27 @@ def eval_derivs(table, params, n_internal, state, off, nbr, gather=False):
         setattr(st, name, _p(a))
     outs = {f: np.zeros((n_internal, ncomp(f, dim))) for f in OUTPUT_FIELDS}
     ncalc = np.zeros(1, dtype=np.int64)
-    o = Outputs(**{f: _p(outs[f]) for f in OUTPUT_FIELDS}, ncalc=_p(ncalc, _i64p))
+    pa = np.zeros((max(int(off[-1]), 1), dim)) if pair_acc else None
+    o = Outputs(**{f: _p(outs[f]) for f in OUTPUT_FIELDS}, ncalc=_p(ncalc, _i64p), pairAcc=_p(pa))
     n_total = keep["pos"].shape[0]
     table.L.orc_eval_derivs(table.h, dim, int(gather), C.byref(params), n_internal, n_total, C.byref(st),
                             _p(off, _i64p), _p(nbr, _i32p), C.byref(o))
     res = {f: (v[:, 0] if v.shape[1] == 1 else v) for f, v in outs.items()}
     if not gather:
         res["ncalc"] = int(ncalc[0])
+    if pair_acc:
+        res["pairAcc"] = pa[:int(off[-1])]
     return res
+
+
+def compatible_depsdt(dim, n_internal, off, nbr, mass, vel, acc, DepsDt0, pair_acc, multiplier):
+    """SpecificThermalEnergyPolicy restated: DepsDt of the internal nodes."""
+    out = np.zeros(n_internal)
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    mass, vel, acc, DepsDt0, pair_acc = f(mass), f(vel), f(acc), f(DepsDt0), f(pair_acc)
+    lib().orc_compatible_depsdt(dim, n_internal, mass.shape[0], _p(off, _i64p), _p(nbr, _i32p), _p(mass), _p(vel),
+                                _p(acc), _p(DepsDt0), _p(pair_acc), multiplier, _p(out))
+    return out
 
 
 def gamma_law(rho, eps, gamma, Pmin=-1e200, Pmax=1e200, floor=False):

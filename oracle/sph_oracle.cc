@@ -46,6 +46,9 @@ struct Outputs {            // every pointer sized n_internal * components
       *DHDt, *Hideal, *maxViscousPressure, *effViscousPressure, *viscousWork, *XSPHWeightSum,
       *XSPHDeltaV, *weightedNeighborSum, *massSecondMoment;
   int64_t *ncalc;           // number of pair evaluations (scatter form), may be null
+  double *pairAcc;          // compatible energy: per row entry of the internal nodes, the pair
+                            // acceleration the node receives from that neighbour (D per entry,
+                            // rows in index order); may be null (scatter form only)
 };
 
 template <int D> struct Acc {
@@ -332,6 +335,7 @@ void evalScatter(const Table& W, const Params& p, int64_t nInt, int64_t nTot, co
   const double tiny = 1.0e-30;
   const double WnPerh = W.kernelValue(1.0 / p.nPerh, 1.0);
   std::vector<Acc<D>> acc(static_cast<size_t>(nTot));
+  std::vector<int64_t> pushed(static_cast<size_t>(nInt), 0);   // pairAccelerations push counts
   int64_t ncalc = 0;
   for (int64_t i = 0; i < nInt; ++i) {
     const Vec<D> ri = s.r(i); const double mi = s.mass[i]; const Vec<D> vi = s.v(i);
@@ -411,6 +415,15 @@ void evalScatter(const Table& W, const Params& p, int64_t nInt, int64_t nTot, co
       for (int k = 0; k < D; ++k) delta.c[k] = gradWi.c[k] * Prhoi + gradWj.c[k] * Prhoj + Qacci.c[k] + Qaccj.c[k];
       addTo(ai.DvDt, scale(delta, -mj));
       addTo(aj.DvDt, scale(delta, mi));
+      if (o.pairAcc) {   // pairAccelerations: -mj*delta to i, mi*delta to j (SPHHydroBase.cc:794-797)
+        const Vec<D> pai = scale(delta, -mj), paj = scale(delta, mi);
+        const int64_t ei = off[i] + pushed[size_t(i)]++;
+        for (int k = 0; k < D; ++k) o.pairAcc[ei * D + k] = pai.c[k];
+        if (j < nInt) {
+          const int64_t ej = off[j] + pushed[size_t(j)]++;
+          for (int k = 0; k < D; ++k) o.pairAcc[ej * D + k] = paj.c[k];
+        }
+      }
       ai.DepsDt += mj * (Prhoi * dot(vij, gradWi) + workQi);
       aj.DepsDt += mi * (Prhoj * dot(vij, gradWj) + workQj);
 
@@ -597,6 +610,43 @@ void orc_omega_gradh(void* tp, int dim, int64_t nInt, int64_t nTot, const double
   if (dim == 1) orc::omegaGradh<1>(W, nInt, nTot, pos, H, off, nbr, omega);
   else if (dim == 2) orc::omegaGradh<2>(W, nInt, nTot, pos, H, off, nbr, omega);
   else orc::omegaGradh<3>(W, nInt, nTot, pos, H, off, nbr, omega);
+}
+
+// SpecificThermalEnergyPolicy::update (src/Hydro/SpecificThermalEnergyPolicy.cc:64-189):
+// for each pair once by its owner i, du = (v_j + a_j hdt - (v_i + a_i hdt)).pa_i
+// with hdt = multiplier/2 and pa_i the owner's pair acceleration; the owner
+// receives w_i du, the other end (1 - w_i) du m_i/m_j, with
+// w_i = weight_i/(weight_i + weight_j), weight = |DepsDt0| + epsilon.
+// vel, acc (DvDt), DepsDt0 and mass cover every node (ghost rows included);
+// pairAcc is the internal rows' pair accelerations as orc_eval_derivs writes them.
+void orc_compatible_depsdt(int dim, int64_t nInt, int64_t nTot, const int64_t* off, const int32_t* nbr,
+                           const double* mass, const double* vel, const double* acc, const double* DepsDt0,
+                           const double* pairAcc, double multiplier, double* DepsDtOut) {
+  std::vector<double> de(static_cast<size_t>(nTot), 0.0);
+  const double hdt = 0.5 * multiplier;
+  const double eps = 2.220446049250313e-16;
+  for (int64_t i = 0; i < nInt; ++i) {
+    const double weighti = std::fabs(DepsDt0[i]) + eps;
+    const double mi = mass[i];
+    double vi12[3];
+    for (int k = 0; k < dim; ++k) vi12[k] = vel[i * dim + k] + acc[i * dim + k] * hdt;
+    for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+      const int64_t j = nbr[e];
+      if (!(j > i)) continue;
+      const double weightj = std::fabs(DepsDt0[j]) + eps;
+      const double mj = mass[j];
+      double duij = 0.0;
+      for (int k = 0; k < dim; ++k) {
+        const double vj12 = vel[j * dim + k] + acc[j * dim + k] * hdt;
+        const double t = (vj12 - vi12[k]) * pairAcc[e * dim + k];
+        duij = (k == 0) ? t : duij + t;
+      }
+      const double wi = weighti / (weighti + weightj);
+      de[size_t(i)] += wi * duij;
+      de[size_t(j)] += (1.0 - wi) * duij * mi / mj;
+    }
+  }
+  for (int64_t i = 0; i < nInt; ++i) DepsDtOut[i] = de[size_t(i)];
 }
 
 // ---- evaluateDerivatives ---------------------------------------------------

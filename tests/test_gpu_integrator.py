@@ -149,3 +149,63 @@ def test_rk2_with_gradh(sphgpu, oracle):
     N = st["pos"].shape[0]
     for key in ("pos", "vel", "eps", "rho", "H"):
         assert np.array_equal(s.download(FIELDS[key], rows=N).reshape(ref.st[key].shape), ref.st[key]), key
+
+
+COMPAT = [c for c in CASES if c[0] in ("converging3d", "periodic2d", "reflect1d")]
+
+
+@pytest.mark.parametrize("case", COMPAT, ids=[c[0] for c in COMPAT])
+def test_rk2_compatible_energy_matches_oracle(sphgpu, oracle, case):
+    """compatibleEnergyEvolution: eps from SpecificThermalEnergyPolicy with the
+    pass's pair terms, against the oracle's RK2 with the reference's scatter
+    pairAccelerations.  Same bars as the plain step."""
+    import integrator
+    name, dim, st, kind, gamma, steps = case
+    print(f"[rk2 compatible] {name}", flush=True)
+    N = st["pos"].shape[0]
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(dim, base, oracle.box_boundaries(dim, kind=kind), gamma, compatible=True)
+    s = device(sphgpu, dim, st, kind, gamma)
+    hp = sphgpu.hydro_params(compatibleEnergyEvolution=1)
+    ip = sphgpu.integrator_params(gamma=gamma)
+    for k in range(min(steps, 4)):
+        dref = ref.step()
+        ddev = s.rk2_step(hp, ip)
+        if dim < 3:
+            assert ddev == dref, (k, ddev, dref)
+        else:
+            assert ddev == pytest.approx(dref, rel=1e-12), (k, ddev, dref)
+    for key, f in FIELDS.items():
+        a = s.download(f, rows=N).reshape(ref.st[key].shape)
+        b = ref.st[key]
+        if dim < 3:
+            assert np.array_equal(a, b), key
+        else:
+            assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), key
+
+
+def test_rk2_compatible_energy_conserved(sphgpu):
+    """Periodic box, compatible energy on the device: sum m (v^2/2 + eps) holds
+    to round-off over the steps."""
+    st, _ = problems.periodic_sinusoid(10)
+    m = st["mass"]
+    N = m.shape[0]
+    s = device(sphgpu, 3, st, "periodic", 1.4)
+    s.set_row_order(sphgpu.ROW_ORDER_SEARCH)
+    E0 = (0.5 * m * (st["vel"] ** 2).sum(1) + m * st["eps"]).sum()
+    hp = sphgpu.hydro_params(compatibleEnergyEvolution=1)
+    ip = sphgpu.integrator_params(gamma=1.4)
+    for _ in range(5):
+        s.rk2_step(hp, ip)
+    v = s.download("velocity", rows=N).reshape(N, 3)
+    eps = s.download("specific thermal energy", rows=N)
+    E1 = (0.5 * m * (v ** 2).sum(1) + m * eps).sum()
+    assert abs(E1 - E0) <= 1e-13 * E0
+
+
+def test_compatible_energy_needs_fp64(sphgpu):
+    st, _ = problems.periodic_sinusoid(6)
+    s = device(sphgpu, 3, st, "periodic", 1.4, precision=sphgpu.PRECISION_MIXED)
+    s.set_ghost_nodes(2.0)
+    with pytest.raises(sphgpu.SphGpuError):
+        s.evaluate_derivatives(sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED, compatibleEnergyEvolution=1))

@@ -223,3 +223,53 @@ def test_omega_gradh_lattice(oracle):
     H1 = problems.sym_identity(3, 2, 1.0)
     o1, n1 = oracle.connectivity(3, 2, p1, H1)
     assert np.array_equal(oracle.omega_gradh(T, 2, p1, H1, o1, n1), [1.0, 1.0])
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_pair_accelerations(oracle, dim):
+    """pairAccelerations (SPHHydroBase.cc:794-797): each node's entries sum to
+    its DvDt in the order they were pushed (the same bits), and the two ends of
+    a pair carry equal and opposite momentum, m_i pa_i = -m_j pa_j."""
+    n = {1: 60, 2: 14, 3: 6}[dim]
+    st, meta = problems.random_state(dim, n, seed=19, h_var=0.1, vel_scale=0.3)
+    N = meta["n_internal"]
+    T = oracle.Table(dim, 1000)
+    off, nbr = oracle.connectivity(dim, N, st["pos"], st["H"], 2.0)
+    out = oracle.eval_derivs(T, oracle.make_params(), N, st, off, nbr, pair_acc=True)
+    pa, m = out["pairAcc"], st["mass"]
+    dv = out["DvDt"].reshape(N, dim)
+    for i in range(N):
+        acc = np.zeros(dim)
+        for e in range(off[i], off[i + 1]):
+            acc = acc + pa[e]
+        assert np.array_equal(acc, dv[i]), i
+    where = {(i, int(nbr[e])): e for i in range(N) for e in range(off[i], off[i + 1])}
+    checked = 0
+    for (i, j), e in where.items():
+        if j > i and (j, i) in where:
+            a, b = m[i] * pa[e], -m[j] * pa[where[(j, i)]]
+            assert np.allclose(a, b, rtol=1e-14, atol=1e-14 * np.abs(a).max()), (i, j)
+            checked += 1
+    assert checked > 0
+
+
+def test_compatible_depsdt_split(oracle):
+    """SpecificThermalEnergyPolicy on one pair: the work du = (v_j12 - v_i12).pa_i
+    is shared by w_i = |DepsDt0_i|/(|DepsDt0_i| + |DepsDt0_j|), the owner's
+    share specific to its mass and the rest to the other node's; the total
+    m_i de_i + m_j de_j is the pair's work m_i du whatever the split."""
+    dim = 3
+    off = np.array([0, 1, 2], dtype=np.int64)
+    nbr = np.array([1, 0], dtype=np.int32)
+    mass = np.array([2.0, 0.5])
+    vel = np.array([[0.3, -0.1, 0.2], [-0.4, 0.25, 0.0]])
+    acc = np.array([[1.0, 0.5, -2.0], [-4.0, -2.0, 8.0]])
+    pa = np.array([[1.0, 0.5, -2.0], [-4.0, -2.0, 8.0]])      # m0 pa0 = -m1 pa1
+    mult = 0.02
+    v12 = vel + acc * (0.5 * mult)
+    du = ((v12[1] - v12[0]) * pa[0]).sum()
+    for d0, d1, w0 in [(3.0, 1.0, 0.75), (1.0, 0.0, 1.0), (0.0, 2.0, 0.0), (-2.0, 2.0, 0.5)]:
+        de = oracle.compatible_depsdt(dim, 2, off, nbr, mass, vel, acc, np.array([d0, d1]), pa, mult)
+        assert de[0] == pytest.approx(w0 * du, rel=1e-14, abs=1e-15 * abs(du))
+        assert de[1] == pytest.approx((1.0 - w0) * du * mass[0] / mass[1], rel=1e-14, abs=1e-15 * abs(du))
+        assert mass[0] * de[0] + mass[1] * de[1] == pytest.approx(mass[0] * du, rel=1e-14)

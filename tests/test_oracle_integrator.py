@@ -91,3 +91,28 @@ def test_converging_flow_steps(oracle, integ):
     E1 = (0.5 * m * (r.st["vel"] ** 2).sum(1) + m * r.st["eps"]).sum()
     assert abs(E1 - E0) < 1e-6 * E0
     assert r.t == pytest.approx(sum(dts), rel=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["periodic", "none"])
+def test_compatible_energy_conserved(oracle, integ, kind):
+    """compatibleEnergyEvolution: the thermal energy takes exactly the work of
+    the pair forces, so sum m (v^2/2 + eps) holds to round-off over the steps
+    where the plain update drifts at the truncation level."""
+    if kind == "periodic":
+        st, _ = problems.periodic_sinusoid(6)
+        b = oracle.box_boundaries(3, kind="periodic")
+    else:
+        st, _ = problems.random_state(3, 6, seed=23, jitter=0.1, vel_scale=0.5, h_var=0.0)
+        b = []
+    m = st["mass"]
+    E0 = (0.5 * m * (st["vel"] ** 2).sum(1) + m * st["eps"]).sum()
+    drift = {}
+    for compatible in (False, True):
+        r = integ.RK2(3, dict(st), b, 1.4, compatible=compatible)
+        for _ in range(4):
+            r.step()
+        v = r.st["vel"]
+        E1 = (0.5 * m * (v ** 2).sum(1) + m * r.st["eps"]).sum()
+        drift[compatible] = abs(E1 - E0) / E0
+    assert drift[True] <= 1e-14
+    assert drift[False] > 10 * drift[True]
