@@ -286,6 +286,83 @@ int sphgpu_last_timings(sphgpu_sim* sim, double* ms);
 /* Block until all work queued on the sim's stream has finished. */
 int sphgpu_synchronize(sphgpu_sim* sim);
 
+/* ---- Ensembles: many independent simulations in one node set --------------
+ * No counterpart in the reference, which runs one simulation per process
+ * (src/SimulationControl/SpheralController.py:303-336 drives one
+ * SynchronousRK2).  Each member is advanced by exactly that step with its own
+ * dt, time and gamma; the members share the kernel table, the hydro and
+ * integrator parameters, the row order and the boundaries (planes in each
+ * member's own coordinates).  A member's rows are contiguous, in member order,
+ * and its results do not depend on which other members share the set.
+ *
+ * Usage: sphgpu_create, table, rows order, sphgpu_set_boundaries, then
+ * sphgpu_ensemble_begin; add members; sphgpu_ensemble_step until a member
+ * reports its end time; download and remove it; add more at any time. */
+typedef struct sphgpu_member {
+  int64_t n;                   /* internal nodes (> 0)                                   */
+  const double* pos;           /* n*dim  (caller layout of sphgpu_upload)                */
+  const double* vel;           /* n*dim                                                  */
+  const double* mass;          /* n                                                      */
+  const double* rho;           /* n                                                      */
+  const double* eps;           /* n     specific thermal energy                          */
+  const double* H;             /* n*dim(dim+1)/2                                         */
+  double gamma;                /* GammaLawGas gamma (P and c are set from rho, eps)      */
+  double tEnd;                 /* the member runs to this time                           */
+} sphgpu_member;
+
+/* Switch a sim with no nodes to ensemble mode. */
+int sphgpu_ensemble_begin(sphgpu_sim* sim);
+/* Append a member (t = 0, last dt 1e-5, cycle 0).  integ supplies the pressure
+ * limits of its first EOS evaluation.  *id identifies it from now on. */
+int sphgpu_ensemble_add(sphgpu_sim* sim, const sphgpu_member* member, const sphgpu_integrator_params* integ,
+                        int64_t* id);
+/* One SynchronousRK2 step of every member (each with its own dt, clamped to
+ * its end time).  *finished = number of members that reached their end time;
+ * they must be downloaded and removed before the next step. */
+int sphgpu_ensemble_step(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, const sphgpu_integrator_params* integ,
+                         int* finished);
+/* Members in row order: id, time, last dt, cycle, first row, rows (arrays of
+ * length cap, any may be null); *count = number of members. */
+int sphgpu_ensemble_members(sphgpu_sim* sim, int cap, int64_t* ids, double* time, double* last_dt, int64_t* cycle,
+                            int64_t* first_row, int64_t* rows, int* count);
+/* A member's internal rows of one state field (SPHGPU_F_*). */
+int sphgpu_ensemble_download(sphgpu_sim* sim, int64_t id, int field, double* host);
+/* Remove members; the others keep their state and results. */
+int sphgpu_ensemble_remove(sphgpu_sim* sim, int n_ids, const int64_t* ids);
+/* Device bytes per node the library holds for a member of this dimension
+ * (state, neighbour rows, records and derivative fields; ghosts not included):
+ * the unit of admission by HBM footprint. */
+int64_t sphgpu_ensemble_bytes_per_node(int dim);
+
+/* ---- Serving: continuous batching of arriving simulations -----------------
+ * No counterpart in the reference.  Requests become visible at `arrival`
+ * seconds after the call starts; they are admitted first come first served
+ * while the resident footprint (sphgpu_ensemble_bytes_per_node * n *
+ * ghostFactor per member) fits hbmBudget and fewer than maxMembers are
+ * resident; all resident members advance together (sphgpu_ensemble_step) and
+ * each leaves when it reaches tEnd, with its final state copied to the out
+ * arrays.  Returns when every request has finished.  The sim must be an empty
+ * ensemble (sphgpu_ensemble_begin). */
+typedef struct sphgpu_request {
+  sphgpu_member member;        /* the simulation                                          */
+  double arrival;              /* seconds after the start of the call                     */
+  double* outPos;              /* n*dim, or null  (final state)                           */
+  double* outVel;              /* n*dim, or null                                          */
+  double* outRho;              /* n, or null                                              */
+  double* outEps;              /* n, or null                                              */
+  double* outH;                /* n*dim(dim+1)/2, or null                                 */
+  double admitted;             /* out: seconds after the start when it joined             */
+  double finished;             /* out: seconds after the start when it reached tEnd       */
+  int64_t steps;               /* out: RK2 steps taken                                    */
+} sphgpu_request;
+typedef struct sphgpu_serve_params {
+  int64_t hbmBudget;           /* bytes; <= 0: 80% of the device memory free at the start */
+  int maxMembers;              /* resident members at most                                */
+  double ghostFactor;          /* footprint rows per internal node (>= 1)                 */
+} sphgpu_serve_params;
+int sphgpu_serve(sphgpu_sim* sim, sphgpu_request* requests, int n_requests, const sphgpu_hydro_params* hydro,
+                 const sphgpu_integrator_params* integ, const sphgpu_serve_params* serve);
+
 /* Measurement helper with no counterpart in the reference: copy `bytes` bytes
  * between two device buffers with a plain streaming kernel `repeats` times and
  * return the average kernel time (ms).  Used to calibrate HBM traffic counters

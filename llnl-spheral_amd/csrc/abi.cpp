@@ -162,6 +162,7 @@ int sphgpu_table_values(sphgpu_sim* s, double eta, double Hdet, double* W, doubl
 int sphgpu_set_nodes(sphgpu_sim* s, int64_t nInt, int64_t nTot) {
   return guarded(s, [&] {
     SG_REQUIRE(nInt >= 0 && nTot >= nInt && nTot < (int64_t(1) << 31) - 1, SPHGPU_ERR_ARG, "bad node counts");
+    SG_REQUIRE(!s->ensemble, SPHGPU_ERR_STATE, "an ensemble takes its nodes with sphgpu_ensemble_add");
     s->nInt = nInt;
     s->nTot = nTot;
     s->deviceGhosts = false;
@@ -185,6 +186,7 @@ int sphgpu_upload(sphgpu_sim* s, int field, const double* host, int64_t nRows) {
   return guarded(s, [&] {
     SG_REQUIRE(host && (nRows == s->nTot || (s->deviceGhosts && nRows == s->nInt)), SPHGPU_ERR_ARG,
                "upload needs n_total rows (or n_internal rows once the device made the ghosts)");
+    SG_REQUIRE(!s->ensemble, SPHGPU_ERR_STATE, "an ensemble takes its nodes with sphgpu_ensemble_add");
     SG_REQUIRE(field >= 0 && field < SPHGPU_NUM_STATE_FIELDS, SPHGPU_ERR_ARG, "unknown state field id");
     const size_t bytes = sizeof(double) * size_t(nRows) * stateComponents(s->dim, field);
     DBuf& b = stateBuf(*s, field);
@@ -423,6 +425,84 @@ int sphgpu_last_timings(sphgpu_sim* s, double* ms) {
 
 int sphgpu_synchronize(sphgpu_sim* s) {
   return guarded(s, [&] { SG_CHECK_HIP(hipStreamSynchronize(s->stream)); });
+}
+
+int sphgpu_ensemble_begin(sphgpu_sim* s) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->nTot == 0 && !s->ensemble, SPHGPU_ERR_STATE, "ensemble mode needs a sim with no nodes");
+    SG_REQUIRE(!s->hasMultipliers, SPHGPU_ERR_UNSUPPORTED, "ensembles take no Q multiplier fields");
+    s->ensemble = true;
+    s->nSeg = 0;
+    s->deviceGhosts = true;    // every ghost row is made by the device boundaries
+  });
+}
+
+int sphgpu_ensemble_add(sphgpu_sim* s, const sphgpu_member* m, const sphgpu_integrator_params* ip, int64_t* id) {
+  return guarded(s, [&] {
+    SG_REQUIRE(m && ip && id, SPHGPU_ERR_ARG, "null argument");
+    *id = ensembleAdd(*s, *m, *ip);
+  });
+}
+
+int sphgpu_ensemble_step(sphgpu_sim* s, const sphgpu_hydro_params* hp, const sphgpu_integrator_params* ip,
+                         int* finished) {
+  return guarded(s, [&] {
+    SG_REQUIRE(hp && ip, SPHGPU_ERR_ARG, "null parameters");
+    const int done = ensembleStep(*s, *hp, *ip);
+    if (finished) *finished = done;
+  });
+}
+
+int sphgpu_ensemble_members(sphgpu_sim* s, int cap, int64_t* ids, double* time, double* lastDt, int64_t* cycle,
+                            int64_t* first, int64_t* rows, int* count) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->ensemble, SPHGPU_ERR_STATE, "not an ensemble");
+    SG_REQUIRE(cap >= 0, SPHGPU_ERR_ARG, "negative capacity");
+    const int nm = int(s->members.size());
+    if (count) *count = nm;
+    for (int q = 0; q < std::min(cap, nm); ++q) {
+      const Member& mb = s->members[size_t(q)];
+      const SegState& g = s->segHost[size_t(q)];
+      if (ids) ids[q] = mb.id;
+      if (time) time[q] = g.t;
+      if (lastDt) lastDt[q] = g.lastDt;
+      if (cycle) cycle[q] = int64_t(g.cycle);
+      if (first) first[q] = mb.first;
+      if (rows) rows[q] = mb.rows;
+    }
+  });
+}
+
+int sphgpu_ensemble_download(sphgpu_sim* s, int64_t id, int field, double* host) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->ensemble, SPHGPU_ERR_STATE, "not an ensemble");
+    SG_REQUIRE(host, SPHGPU_ERR_ARG, "null buffer");
+    auto it = std::find_if(s->members.begin(), s->members.end(), [&](const Member& mb) { return mb.id == id; });
+    SG_REQUIRE(it != s->members.end(), SPHGPU_ERR_ARG, "no member with that id");
+    SG_REQUIRE(field >= 0 && field <= SPHGPU_F_OMEGA_GRADH, SPHGPU_ERR_ARG, "unknown state field id");
+    const int comps = stateComponents(s->dim, field);
+    DBuf& b = stateBuf(*s, field);
+    SG_CHECK_HIP(hipMemcpyAsync(host, b.as<double>() + it->first * comps, sizeof(double) * size_t(it->rows) * comps,
+                                hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int sphgpu_ensemble_remove(sphgpu_sim* s, int nIds, const int64_t* ids) {
+  return guarded(s, [&] { ensembleRemove(*s, nIds, ids); });
+}
+
+int sphgpu_serve(sphgpu_sim* s, sphgpu_request* req, int n, const sphgpu_hydro_params* hp,
+                 const sphgpu_integrator_params* ip, const sphgpu_serve_params* sp) {
+  return guarded(s, [&] {
+    SG_REQUIRE(hp && ip && sp, SPHGPU_ERR_ARG, "null parameters");
+    serve(*s, req, n, *hp, *ip, *sp);
+  });
+}
+
+int64_t sphgpu_ensemble_bytes_per_node(int dim) {
+  if (dim < 1 || dim > 3) return -1;
+  return ensembleBytesPerNode(dim);
 }
 
 int sphgpu_copy_probe(sphgpu_sim* s, int64_t bytes, int repeats, double* ms) {

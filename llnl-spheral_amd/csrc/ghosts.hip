@@ -149,27 +149,37 @@ __global__ void k_hinv_min(const double* __restrict__ H, int64_t n, double* __re
   if (i < n) hmaxOf[i] = 1.0 / minEigen<D>(H + i * NS);
 }
 
+// hmax of the nodes near either plane; per ensemble member when seg is given
+// (hmaxBits[member]), else one value.
 template <int D>
-__global__ void k_ghost_hmax(const double* __restrict__ pos, const double* __restrict__ hmaxOf, int64_t n, DSub b,
-                             double ext, unsigned long long* __restrict__ hmaxBits) {
+__global__ void k_ghost_hmax(const double* __restrict__ pos, const double* __restrict__ hmaxOf,
+                             const double* __restrict__ seg, int64_t n, DSub b, double ext,
+                             unsigned long long* __restrict__ hmaxBits) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   double v = 0.0;
+  int m = 0;
   if (i < n) {
     const V<D> x = loadV<D>(pos, i);
     const double hmaxi = hmaxOf[i];
     const double d = dmin(dabs(signedDist<D>(b.exitP, b.exitN, x)), dabs(signedDist<D>(b.enterP, b.enterN, x)));
     if (hmaxi > 0.0 && d < ext * hmaxi) v = hmaxi;
+    if (seg) m = int(seg[i]);
   }
-  for (int sh = 32; sh > 0; sh >>= 1) v = dmax(v, __shfl_xor(v, sh));
-  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(hmaxBits, (unsigned long long)__double_as_longlong(v));
+  const int m0 = __shfl(m, 0);
+  if (__all(m == m0)) {
+    for (int sh = 32; sh > 0; sh >>= 1) v = dmax(v, __shfl_xor(v, sh));
+    if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(hmaxBits + m0, (unsigned long long)__double_as_longlong(v));
+  } else if (v > 0.0) {
+    atomicMax(hmaxBits + m, (unsigned long long)__double_as_longlong(v));
+  }
 }
 
 template <int D>
-__global__ void k_ghost_flag(const double* __restrict__ pos, int64_t n, DSub b, double ext,
-                             const unsigned long long* __restrict__ hmaxBits, int32_t* __restrict__ flags) {
+__global__ void k_ghost_flag(const double* __restrict__ pos, const double* __restrict__ seg, int64_t n, DSub b,
+                             double ext, const unsigned long long* __restrict__ hmaxBits, int32_t* __restrict__ flags) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
-  const double hmax = __longlong_as_double((long long)*hmaxBits);
+  const double hmax = __longlong_as_double((long long)hmaxBits[seg ? int(seg[i]) : 0]);
   int32_t f = 0;
   if (hmax > 0.0) {
     const double disti = signedDist<D>(b.exitP, b.exitN, loadV<D>(pos, i)) / hmax;
@@ -179,9 +189,10 @@ __global__ void k_ghost_flag(const double* __restrict__ pos, int64_t n, DSub b, 
 }
 
 template <int D>
-__global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, int64_t n, int64_t nInt,
-                             const int32_t* __restrict__ flags, const int32_t* __restrict__ offs, DSub b, int subIdx,
-                             int32_t* __restrict__ control, int32_t* __restrict__ subOf, double* __restrict__ hmaxOf) {
+__global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, double* __restrict__ seg, int64_t n,
+                             int64_t nInt, const int32_t* __restrict__ flags, const int32_t* __restrict__ offs, DSub b,
+                             int subIdx, int32_t* __restrict__ control, int32_t* __restrict__ subOf,
+                             double* __restrict__ hmaxOf) {
   constexpr int NS = NSym<D>::value;
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n || !flags[i]) return;
@@ -199,6 +210,7 @@ __global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, i
   }
   control[g - nInt] = int32_t(i);
   subOf[g - nInt] = subIdx;
+  if (seg) seg[g] = seg[i];
 }
 
 struct FieldPtrs {
@@ -387,7 +399,13 @@ __global__ void k_iperm_remap(const int32_t* __restrict__ iperm, const int32_t* 
 }
 
 std::vector<DBuf*> stateBufs(Sim& s) {
-  return {&s.pos, &s.vel, &s.H, &s.mass, &s.rho, &s.eps, &s.P, &s.cs, &s.omega, &s.fCl, &s.fCq, &s.fshear};
+  return {&s.pos, &s.vel, &s.H, &s.mass, &s.rho, &s.eps, &s.P, &s.cs, &s.omega, &s.fCl, &s.fCq, &s.fshear, &s.seg};
+}
+// fields 9-11 exist with Q multipliers, field 12 in an ensemble
+bool stateUsed(const Sim& s, int which) {
+  if (which >= 9 && which < 12) return s.hasMultipliers;
+  if (which == 12) return s.ensemble;
+  return true;
 }
 int stateComps(const Sim& s, int which) {
   const int D = s.dim, NS = nsymOf(D);
@@ -398,7 +416,7 @@ int stateComps(const Sim& s, int which) {
 void ensureRows(Sim& s, int64_t rows) {
   auto bufs = stateBufs(s);
   for (int w = 0; w < int(bufs.size()); ++w) {
-    if (w >= 9 && !s.hasMultipliers) continue;
+    if (!stateUsed(s, w)) continue;
     const size_t per = sizeof(double) * size_t(stateComps(s, w));
     if (bufs[size_t(w)]->bytes >= per * size_t(rows)) continue;
     bufs[size_t(w)]->grow(per * size_t(rows + rows / 4), per * size_t(s.nTot), s.stream);
@@ -434,6 +452,11 @@ template <int D> void generateD(Sim& s, double ext) {
   s.subStart.assign(s.subs.size() + 1, 0);
   s.flags.ensure(sizeof(int32_t) * 8 + sizeof(unsigned long long) * 2);
   unsigned long long* hbits = reinterpret_cast<unsigned long long*>(s.flags.as<int32_t>() + 4);
+  const int nh = std::max(1, s.nSeg);
+  if (s.nSeg > 0) {
+    s.segHmaxBits.ensure(sizeof(unsigned long long) * size_t(nh));
+    hbits = s.segHmaxBits.as<unsigned long long>();
+  }
   s.hmaxOf.ensure(sizeof(double) * size_t(std::max<int64_t>(s.nInt, 1)));
   if (s.nInt > 0)
     hipLaunchKernelGGL(k_hinv_min<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.H.as<double>(), s.nInt,
@@ -444,11 +467,12 @@ template <int D> void generateD(Sim& s, double ext) {
     const DSub d = toDevice(s.subs[b]);
     s.keep.ensure(sizeof(int32_t) * size_t(n + 1));
     s.keepScan.ensure(sizeof(int32_t) * size_t(n + 1));
-    SG_CHECK_HIP(hipMemsetAsync(hbits, 0, sizeof(unsigned long long), st));
+    const double* segp = s.nSeg > 0 ? s.seg.as<double>() : nullptr;
+    SG_CHECK_HIP(hipMemsetAsync(hbits, 0, sizeof(unsigned long long) * size_t(nh), st));
     hipLaunchKernelGGL(k_ghost_hmax<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
-                       s.hmaxOf.as<double>(), n, d, ext, hbits);
-    hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), n, d, ext, hbits,
-                       s.keep.as<int32_t>());
+                       s.hmaxOf.as<double>(), segp, n, d, ext, hbits);
+    hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), segp, n, d, ext,
+                       hbits, s.keep.as<int32_t>());
     exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), n, s);
     int32_t count = 0;
     SG_CHECK_HIP(hipMemcpyAsync(&count, s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -461,7 +485,7 @@ template <int D> void generateD(Sim& s, double ext) {
     s.ghostControl.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
     s.ghostSub.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
     hipLaunchKernelGGL(k_ghost_make<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
-                       n, s.nInt, s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), d, int(b),
+                       s.nSeg > 0 ? s.seg.as<double>() : nullptr, n, s.nInt, s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), d, int(b),
                        s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>());
     SG_CHECK_HIP(hipGetLastError());
     s.nTot = n + count;
@@ -498,7 +522,7 @@ void cull(Sim& s) {
   // caller-order fields
   auto bufs = stateBufs(s);
   for (int w = 0; w < int(bufs.size()); ++w) {
-    if (w >= 9 && !s.hasMultipliers) continue;
+    if (!stateUsed(s, w)) continue;
     const int comps = stateComps(s, w);
     DBuf& spare = s.spare[w];
     spare.ensure(bufs[size_t(w)]->bytes);
@@ -661,6 +685,11 @@ void applyGhostScalar(Sim& s, DBuf& field) {
   }
   SG_CHECK_HIP(hipGetLastError());
 }
+
+void ensureStateRows(Sim& s, int64_t rows) { ensureRows(s, rows); }
+bool stateFieldUsed(const Sim& s, int which) { return stateUsed(s, which); }
+DBuf& stateField(Sim& s, int which) { return *stateBufs(s)[size_t(which)]; }
+int stateFieldComps(const Sim& s, int which) { return stateComps(s, which); }
 
 void applyGhostVector(Sim& s, DBuf& field) {
   if (!s.deviceGhosts || s.nTot == s.nInt) return;

@@ -72,15 +72,29 @@ struct DerivIn {
 struct Bounds {
   double rhoMin, rhoMax, hmaxInv, hminInv;
 };
+// Ensemble members: the update's multiplier (dt or dt/2) and gamma per member.
+struct SegArgs {
+  const double* seg;        // null: the scalar multiplier and EOS apply to every row
+  const SegState* st;
+  int half;                 // 1: multiply by dt/2
+};
+__device__ __forceinline__ void rowMultEos(const SegArgs& sa, int64_t i, double& mult, Eos& e) {
+  if (!sa.seg) return;
+  const SegState& g = sa.st[int(sa.seg[i])];
+  mult = sa.half ? g.hdt : g.dt;
+  e.gamma = g.gamma;
+  e.gamma1 = g.gamma - 1.0;
+}
 
 template <int D>
 __global__ void k_update(const int32_t* __restrict__ work, const int32_t* __restrict__ perm, int64_t nInt, Fields f,
-                         Base b, DerivIn d, double mult, Bounds bd, Eos e) {
+                         Base b, DerivIn d, double mult, Bounds bd, Eos e, SegArgs sa) {
   constexpr int NS = NSym<D>::value;
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (t >= nInt) return;
   const int64_t k = work[t];
   const int64_t i = perm[k];
+  rowMultEos(sa, i, mult, e);
   const double* dx = d.base + int64_t(d.oDx) * d.n + k * D;
   const double* dv = d.base + int64_t(d.oDv) * d.n + k * D;
   for (int c = 0; c < D; ++c) {
@@ -132,13 +146,16 @@ __global__ void k_dt(const int32_t* __restrict__ work, const int32_t* __restrict
                      const int32_t* __restrict__ rowCount, const int32_t* __restrict__ nbr, int64_t n,
                      const double* __restrict__ H, const double* __restrict__ cs, const double* __restrict__ rho,
                      const double* __restrict__ vel, const double* __restrict__ dtIn, int haveDtIn, double nPerh,
-                     int useVelMag, unsigned long long* __restrict__ minBits) {
+                     int useVelMag, const double* __restrict__ seg, unsigned long long* __restrict__ minBits,
+                     int64_t minStride) {
   constexpr int NS = NSym<D>::value;
   const int64_t t = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   double m = double(FLT_MAX);
+  int sg = 0;
   if (t < nInt) {
     const int64_t k = work[t];
     const int64_t i = perm[k];
+    if (seg) sg = int(seg[i]);
     const double nodeScale = 1.0 / maxEigen<D>(H + i * NS) / nPerh;
     m = dmin(m, nodeScale / (cs[i] + FLT_MIN));
     constexpr int W = 2 + D;
@@ -163,8 +180,13 @@ __global__ void k_dt(const int32_t* __restrict__ work, const int32_t* __restrict
     m = dmin(m, sqrt(nodeScale / (vmagArr<D>(acc) + FLT_MIN)));
     if (useVelMag) m = dmin(m, nodeScale / (vmagArr<D>(vi) + 1.0e-10));
   }
-  for (int sh = 32; sh > 0; sh >>= 1) m = dmin(m, __shfl_xor(m, sh));
-  if ((threadIdx.x & 63) == 0) atomicMin(minBits, (unsigned long long)__double_as_longlong(m));
+  const int s0 = __shfl(sg, 0);
+  if (__all(sg == s0)) {
+    for (int sh = 32; sh > 0; sh >>= 1) m = dmin(m, __shfl_xor(m, sh));
+    if ((threadIdx.x & 63) == 0) atomicMin(minBits + s0 * minStride, (unsigned long long)__double_as_longlong(m));
+  } else if (t < nInt) {
+    atomicMin(minBits + sg * minStride, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
@@ -189,12 +211,15 @@ __global__ void k_compat_scatter(const int32_t* __restrict__ work, const int32_t
 template <int D>
 __global__ void k_compat_rec(const int32_t* __restrict__ perm, int64_t nSlots, const double* __restrict__ vel0,
                              const double* __restrict__ acc, const double* __restrict__ de0,
-                             const double* __restrict__ mass, double hdt, double* __restrict__ crec) {
+                             const double* __restrict__ mass, double mult, SegArgs sa, double* __restrict__ crec) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (k >= nSlots) return;
   const int64_t i = perm[k];
   double r[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i >= 0) {
+    Eos unused{};
+    rowMultEos(sa, i, mult, unused);
+    const double hdt = 0.5 * mult;
     for (int c = 0; c < D; ++c) r[c] = vel0[i * D + c] + acc[i * D + c] * hdt;
     r[3] = dabs(de0[i]) + DBL_EPSILON;
     r[4] = mass[i];
@@ -259,7 +284,7 @@ Eos makeEos(double gamma, double Pmin, double Pmax, int floor) { return Eos{gamm
 
 // SpecificThermalEnergyPolicy::update's DepsDt into s.compDe (slot rows) for
 // an update by `mult`, with vel0 the velocities the state holds before it.
-template <int D> void compatD(Sim& s, const double* vel0, double mult) {
+template <int D> void compatD(Sim& s, const double* vel0, double mult, const SegArgs& sa) {
   hipStream_t st = s.stream;
   const size_t nt = size_t(std::max<int64_t>(s.nTot, 1)), ns = size_t(std::max<int64_t>(s.nSlots, 1));
   s.compAcc.ensure(sizeof(double) * D * nt);
@@ -273,7 +298,7 @@ template <int D> void compatD(Sim& s, const double* vel0, double mult) {
   applyGhostVector(s, s.compAcc);
   applyGhostScalar(s, s.compDe0);
   hipLaunchKernelGGL(k_compat_rec<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), s.nSlots,
-                     vel0, s.compAcc.as<double>(), s.compDe0.as<double>(), s.mass.as<double>(), 0.5 * mult,
+                     vel0, s.compAcc.as<double>(), s.compDe0.as<double>(), s.mass.as<double>(), mult, sa,
                      s.compRec.as<double>());
   hipLaunchKernelGGL(k_compat<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
                      s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots, s.nInt, s.compRec.as<double>(),
@@ -282,14 +307,14 @@ template <int D> void compatD(Sim& s, const double* vel0, double mult) {
 }
 
 template <int D> void updateD(Sim& s, const Base& b, double mult, const Bounds& bd, const Eos& e,
-                              const double* compDe = nullptr) {
+                              const double* compDe, const SegArgs& sa) {
   Fields f{s.pos.as<double>(), s.vel.as<double>(), s.eps.as<double>(), s.rho.as<double>(), s.H.as<double>(),
            s.P.as<double>(), s.cs.as<double>()};
   DerivIn d{s.out.as<double>(), s.outStride, derivOffsetDoubles(D, SPHGPU_D_DXDT), derivOffsetDoubles(D, SPHGPU_D_DVDT),
             derivOffsetDoubles(D, SPHGPU_D_DEPSDT), derivOffsetDoubles(D, SPHGPU_D_DRHODT),
             derivOffsetDoubles(D, SPHGPU_D_HIDEAL), compDe};
   hipLaunchKernelGGL(k_update<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
-                     s.perm.as<int32_t>(), s.nInt, f, b, d, mult, bd, e);
+                     s.perm.as<int32_t>(), s.nInt, f, b, d, mult, bd, e, sa);
   SG_CHECK_HIP(hipGetLastError());
   s.recValid = false;
 }
@@ -315,7 +340,7 @@ template <int D> double hydroDtD(Sim& s, double nPerh, double cfl, int useVelMag
   hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
                      s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
                      s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(), s.dtIn.as<double>(),
-                     s.haveDtIn ? 1 : 0, nPerh, useVelMag, bits);
+                     s.haveDtIn ? 1 : 0, nPerh, useVelMag, nullptr, bits, int64_t(0));
   SG_CHECK_HIP(hipGetLastError());
   unsigned long long hb = 0;
   SG_CHECK_HIP(hipMemcpyAsync(&hb, bits, sizeof(hb), hipMemcpyDeviceToHost, s.stream));
@@ -325,21 +350,47 @@ template <int D> double hydroDtD(Sim& s, double nPerh, double cfl, int useVelMag
   return cfl * minDt;
 }
 
-template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
-                             double maxTime) {
-  constexpr int NS = NSym<D>::value;
-  hipStream_t st = s.stream;
-  const double t = s.time;
-  // Integrator::preStepInitialize: ghosts and connectivity
-  if (!s.subs.empty()) setGhostNodes(s, ip.kernelExtent, ip.cullGhostNodes != 0);
-  else buildNeighbors(s, ip.kernelExtent);
-  // selectDt
-  const double dtMin = std::min(ip.dtMin, maxTime - t), dtMax = std::min(ip.dtMax, maxTime - t);
-  const double vote = hydroDtD<D>(s, hp.nPerh, ip.cfl, ip.useVelocityMagnitudeForDt);
+// Per-member dt selection, the same expressions as rk2D's host selection.
+__global__ void k_seg_vote_init(SegState* __restrict__ st, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) st[i].voteBits = v;
+}
+__global__ void k_seg_select_dt(SegState* __restrict__ st, int n, double cfl, double dtMinIn, double dtMaxIn,
+                                double dtGrowth) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  SegState& g = st[i];
+  const double dtMin = dmin(dtMinIn, g.tEnd - g.t), dtMax = dmin(dtMaxIn, g.tEnd - g.t);
+  const double vote = cfl * __longlong_as_double((long long)g.voteBits);
   double dt = dtMax;
   if (vote > 0.0 && vote < dt) dt = vote;
-  dt = std::min(dt, ip.dtGrowth * s.lastDt);
-  dt = std::min(dtMax, std::max(dtMin, dt));
+  dt = dmin(dt, dtGrowth * g.lastDt);
+  dt = dmin(dtMax, dmax(dtMin, dt));
+  g.dt = dt;
+  g.hdt = 0.5 * dt;
+}
+__global__ void k_seg_advance(SegState* __restrict__ st, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  SegState& g = st[i];
+  g.t = g.t + g.dt;
+  g.lastDt = g.dt;
+  g.cycle = g.cycle + 1.0;
+}
+
+// Integrator::preStepInitialize: ghosts and connectivity
+void preStep(Sim& s, const sphgpu_integrator_params& ip) {
+  if (!s.subs.empty()) setGhostNodes(s, ip.kernelExtent, ip.cullGhostNodes != 0);
+  else buildNeighbors(s, ip.kernelExtent);
+}
+
+// The two derivative passes and updates of SynchronousRK2::step, then
+// SPHHydroBase::finalize and Integrator::enforceBoundaries.  Multipliers are
+// dt/2, dt (plain simulation) or per member (half, full).
+template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip, double dt,
+                               const SegArgs& half, const SegArgs& full) {
+  constexpr int NS = NSym<D>::value;
+  hipStream_t st = s.stream;
   const double hdt = 0.5 * dt;
   const bool compat = hp.compatibleEnergyEvolution != 0;
   SG_REQUIRE(!compat || s.deviceGhosts || s.nTot == s.nInt, SPHGPU_ERR_UNSUPPORTED,
@@ -361,27 +412,81 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
   }
   const Base b0{s.state0[0].as<double>(), s.state0[1].as<double>(), s.state0[2].as<double>(), s.state0[3].as<double>()};
   // trial advance to the midpoint
-  if (compat) compatD<D>(s, b0.vel, hdt);
-  updateD<D>(s, b0, hdt, bd, e, compat ? s.compDe.as<double>() : nullptr);
+  if (compat) compatD<D>(s, b0.vel, hdt, half);
+  updateD<D>(s, b0, hdt, bd, e, compat ? s.compDe.as<double>() : nullptr, half);
   applyGhostBoundaries(s);
   // derivatives at t + dt/2 with the step's neighbour rows
   if (ip.gradhCorrection) omegaGradh(s);
   evalDerivs(s, hp);
   // advance from state0 with the midpoint derivatives
-  if (compat) compatD<D>(s, b0.vel, dt);
-  updateD<D>(s, b0, dt, bd, e, compat ? s.compDe.as<double>() : nullptr);
+  if (compat) compatD<D>(s, b0.vel, dt, full);
+  updateD<D>(s, b0, dt, bd, e, compat ? s.compDe.as<double>() : nullptr, full);
   applyGhostBoundaries(s);
   dtInputsD<D>(s);
   // SPHHydroBase::finalize: sum density
   sumDensity(s);
   // Integrator::enforceBoundaries
   enforceBoundaries(s);
-  SG_CHECK_HIP(hipStreamSynchronize(st));
+}
+
+template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
+                             double maxTime) {
+  const double t = s.time;
+  preStep(s, ip);
+  // selectDt
+  const double dtMin = std::min(ip.dtMin, maxTime - t), dtMax = std::min(ip.dtMax, maxTime - t);
+  const double vote = hydroDtD<D>(s, hp.nPerh, ip.cfl, ip.useVelocityMagnitudeForDt);
+  double dt = dtMax;
+  if (vote > 0.0 && vote < dt) dt = vote;
+  dt = std::min(dt, ip.dtGrowth * s.lastDt);
+  dt = std::min(dtMax, std::max(dtMin, dt));
+  const SegArgs none{nullptr, nullptr, 0};
+  stepBody<D>(s, hp, ip, dt, none, none);
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
   s.time = t + dt;
   s.lastDt = dt;
   s.cycle += 1;
   s.derivsValid = false;
   return dt;
+}
+
+// One SynchronousRK2 step of every ensemble member, each with its own dt.
+template <int D> void ensembleRk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip) {
+  hipStream_t st = s.stream;
+  const int ns = s.nSeg;
+  SegState* sd = s.segState.as<SegState>();
+  preStep(s, ip);
+  // GenericHydro::dt per member, then selectDt per member
+  const double init = double(FLT_MAX);
+  unsigned long long ib;
+  std::memcpy(&ib, &init, sizeof(ib));
+  hipLaunchKernelGGL(k_seg_vote_init, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns, ib);
+  if (!s.haveDtIn) s.dtIn.ensure(sizeof(double) * 8);
+  hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
+                     s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
+                     s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(), s.dtIn.as<double>(),
+                     s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt, s.seg.as<double>(), &sd->voteBits,
+                     int64_t(sizeof(SegState) / sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_seg_select_dt, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns, ip.cfl, ip.dtMin, ip.dtMax,
+                     ip.dtGrowth);
+  SG_CHECK_HIP(hipGetLastError());
+  const SegArgs half{s.seg.as<double>(), sd, 1}, full{s.seg.as<double>(), sd, 0};
+  stepBody<D>(s, hp, ip, 0.0, half, full);
+  hipLaunchKernelGGL(k_seg_advance, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns);
+  SG_CHECK_HIP(hipGetLastError());
+  s.segHost.resize(size_t(ns));
+  SG_CHECK_HIP(hipMemcpyAsync(s.segHost.data(), sd, sizeof(SegState) * size_t(ns), hipMemcpyDeviceToHost, st));
+  SG_CHECK_HIP(hipStreamSynchronize(st));
+  s.derivsValid = false;
+}
+
+__global__ void k_eos_rows(const double* __restrict__ rho, const double* __restrict__ eps, double* __restrict__ P,
+                           double* __restrict__ cs, int64_t first, int64_t rows, Eos e) {
+  const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (r >= rows) return;
+  const int64_t i = first + r;
+  P[i] = eosP(e, rho[i], eps[i]);
+  cs[i] = eosCs(e, eps[i]);
 }
 
 }  // namespace
@@ -404,7 +509,22 @@ double hydroDt(Sim& s, double nPerh, double cfl, int useVelMag) {
   return hydroDtD<3>(s, nPerh, cfl, useVelMag);
 }
 
+void ensembleRk2(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip) {
+  if (s.dim == 1) ensembleRk2D<1>(s, hp, ip);
+  else if (s.dim == 2) ensembleRk2D<2>(s, hp, ip);
+  else ensembleRk2D<3>(s, hp, ip);
+}
+
+void eosRows(Sim& s, int64_t first, int64_t rows, double gamma, double Pmin, double Pmax, int floorType) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(k_eos_rows, dim3(nblocks(rows)), dim3(kBlock), 0, s.stream, s.rho.as<double>(),
+                     s.eps.as<double>(), s.P.as<double>(), s.cs.as<double>(), first, rows,
+                     makeEos(gamma, Pmin, Pmax, floorType));
+  SG_CHECK_HIP(hipGetLastError());
+}
+
 double rk2Step(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip, double maxTime) {
+  SG_REQUIRE(!s.ensemble, SPHGPU_ERR_STATE, "an ensemble steps with sphgpu_ensemble_step");
   SG_REQUIRE(s.nInt > 0, SPHGPU_ERR_STATE, "no nodes");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   SG_REQUIRE(ip.kernelExtent > 0.0 && ip.cfl > 0.0 && ip.dtGrowth > 0.0, SPHGPU_ERR_ARG, "bad integrator parameters");

@@ -31,6 +31,10 @@ struct Error : std::runtime_error {
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
   void ensure(size_t nb) {
     if (nb <= bytes) return;
     if (p) SG_CHECK_HIP(hipFree(p));
@@ -92,6 +96,23 @@ struct SubBoundary {
   double R[9];                                       // row major, D x D used
 };
 
+// Per-member integrator state of an ensemble (device array, host mirror).
+struct SegState {
+  double t;                 // member time
+  double lastDt;            // Integrator::mLastDt
+  double tEnd;              // the member runs to this time
+  double gamma;             // GammaLawGas gamma
+  double dt, hdt;           // this step's dt and dt/2
+  unsigned long long voteBits;   // GenericHydro::dt minimum (bits of a positive double)
+  double cycle;
+};
+
+// One member of an ensemble: a contiguous range of internal caller rows.
+struct Member {
+  int64_t id;
+  int64_t first, rows;
+};
+
 struct Sim {
   int dim = 3;
   int device = 0;
@@ -148,7 +169,7 @@ struct Sim {
   std::vector<int64_t> subStart; // first ghost (offset from nInt) of each sub-boundary, +1 entry
   int64_t ghostsGenerated = 0;   // before culling
   DBuf keep, keepScan, slotKeep, ghostControlSpare, ghostSubSpare, ipermSpare, hmaxOf;
-  DBuf spare[12];                // one per caller-order state field (culling compacts into it)
+  DBuf spare[13];                // one per caller-order state field (culling compacts into it)
   bool markGhostUse = false;     // the next build marks the ghost slots rows hold (for culling)
 
   // Slots of the current build.  Culling renumbers the caller-order rows but
@@ -164,6 +185,23 @@ struct Sim {
   DBuf state0[5];                // pos, vel, eps, rho, H of the internal rows at the step start
   DBuf dtIn;                     // per internal node: maxViscousPressure, tr DvDx, DvDt (caller order)
   bool haveDtIn = false;
+
+  // Ensemble: independent simulations packed in one node set.  Each member's
+  // internal rows are contiguous and in member order; every row (ghosts too)
+  // carries its member index in `seg`.  Neighbour grids, ghost selection, the
+  // time step and the EOS are per member; the density and pair passes run
+  // over all members in one launch.  nSeg = 0: a plain simulation.
+  bool ensemble = false;
+  int nSeg = 0;
+  DBuf seg;                      // member index per caller row (as a double)
+  DBuf segGrid, segCellBase;     // per member: neighbour grid, first global cell (+1 entry)
+  DBuf segBoundsBits;            // per member: 9 ordered-bit bounds of the neighbour build
+  DBuf segHmaxBits;              // per member: ghost-selection hmax bits
+  DBuf segState;                 // SegState per member
+  std::vector<SegState> segHost;
+  std::vector<Member> members;
+  int64_t nextMemberId = 0;
+  DBuf segSpare, dtInSpare;
 
   hipEvent_t ev[12] = {};
   Timing timing;
@@ -184,6 +222,11 @@ void applyGhostBoundaries(Sim& s);
 void applyGhostDensity(Sim& s);   // after sum density: ghost rho and their records
 void applyGhostScalar(Sim& s, DBuf& field);   // ghost rows of one scalar field from their controls
 void applyGhostVector(Sim& s, DBuf& field);   // ghost rows of one Vector field (R.a or a copy)
+void ensureStateRows(Sim& s, int64_t rows);   // room for `rows` rows in every caller-order field (keeps nTot)
+bool stateFieldUsed(const Sim& s, int which); // caller-order fields, in culling order
+DBuf& stateField(Sim& s, int which);
+int stateFieldComps(const Sim& s, int which);
+constexpr int kStateFields = 13;
 // hydro.hip
 void omegaGradh(Sim& s);          // computeSPHOmegaGradhCorrection into the omega field
 void enforceBoundaries(Sim& s);   // internal nodes behind a plane are mapped back
@@ -191,6 +234,17 @@ void enforceBoundaries(Sim& s);   // internal nodes behind a plane are mapped ba
 void eosGammaLaw(Sim& s, double gamma, double Pmin, double Pmax, int floorType);
 double hydroDt(Sim& s, double nPerh, double cfl, int useVelocityMagnitude);
 double rk2Step(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip, double maxTime);
+// ensemble.hip
+int64_t ensembleAdd(Sim& s, const sphgpu_member& m, const sphgpu_integrator_params& ip);
+void ensembleRemove(Sim& s, int nIds, const int64_t* ids);
+int64_t ensembleBytesPerNode(int dim);
+// serve.cpp
+void serve(Sim& s, sphgpu_request* req, int n, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
+           const sphgpu_serve_params& sp);
+int ensembleStep(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip);
+// ensemble update / dt (integrate.hip)
+void ensembleRk2(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip);
+void eosRows(Sim& s, int64_t first, int64_t rows, double gamma, double Pmin, double Pmax, int floorType);
 // scans (neighbors.hip)
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);

@@ -29,10 +29,15 @@
 //     capacity and repeats the row pass.
 //  4. optionally, rows sorted by node index (one wavefront per row, bitonic
 //     sort in registers).
+// Ensembles: bounds and grid per member (each member's grid is the one it
+// would get alone); a member's cells are numbered after those of the members
+// before it, so the sort groups slots by member, and a row only looks at its
+// own member's cells.
 #include "internal.hh"
 #include "dgeom.hh"
 #include "records.hh"
 #include "ell.hh"
+#include <cstring>
 
 namespace sphgpu {
 
@@ -117,6 +122,67 @@ struct DGrid {
   int nb[3];
 };
 
+// Ensemble grids: member index per caller row, grid and first global cell per member.
+struct SegGrids {
+  const double* seg;        // null: one grid for all nodes
+  const DGrid* g;
+  const int64_t* base;
+};
+
+// Order-preserving map of a double onto uint64 (for atomic min/max).
+__device__ __forceinline__ unsigned long long orderedBits(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+inline double fromOrderedBits(unsigned long long o) {
+  const unsigned long long b = (o >> 63) ? (o & 0x7FFFFFFFFFFFFFFFull) : ~o;
+  double x;
+  std::memcpy(&x, &b, sizeof(x));
+  return x;
+}
+
+__global__ void k_seg_bounds_init(unsigned long long* __restrict__ bits, int nSeg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nSeg * kNB) bits[i] = isMaxSlot(i % kNB) ? 0ull : ~0ull;
+}
+
+// The bounds of k_bounds per member: a wavefront whose nodes share one member
+// reduces in registers and makes one atomic per value.
+template <int D>
+__global__ void k_seg_bounds(const double* __restrict__ pos, const double* __restrict__ H,
+                             const double* __restrict__ seg, int64_t n, unsigned long long* __restrict__ bits) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300};
+  int m = -1;
+  if (i < n) {
+    m = int(seg[i]);
+    for (int k = 0; k < D; ++k) { const double x = pos[i * D + k]; r[k] = x; r[3 + k] = x; }
+    const double* h = H + i * NSym<D>::value;
+    r[6] = gershgorinMin<D>(h);
+    r[7] = isIsotropic<D>(h) ? 1.0 : 0.0;
+    r[8] = maxDiag<D>(h);
+  }
+  const int m0 = __shfl(m, 0);
+  if (m0 < 0) return;                      // lanes past n come last: the whole wavefront is idle
+  if (__all(m == m0 || m < 0)) {
+    for (int k = 0; k < kNB; ++k)
+      for (int sh = 32; sh > 0; sh >>= 1) {
+        const double o = __shfl_xor(r[k], sh);
+        r[k] = isMaxSlot(k) ? dmax(r[k], o) : dmin(r[k], o);
+      }
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < kNB; ++k) {
+        if (isMaxSlot(k)) atomicMax(bits + int64_t(m0) * kNB + k, orderedBits(r[k]));
+        else atomicMin(bits + int64_t(m0) * kNB + k, orderedBits(r[k]));
+      }
+  } else if (m >= 0) {
+    for (int k = 0; k < kNB; ++k) {
+      if (isMaxSlot(k)) atomicMax(bits + int64_t(m) * kNB + k, orderedBits(r[k]));
+      else atomicMin(bits + int64_t(m) * kNB + k, orderedBits(r[k]));
+    }
+  }
+}
+
 template <int D> __device__ __forceinline__ void cellCoords(const DGrid& g, const double* x, int* c) {
   c[0] = c[1] = c[2] = 0;
   for (int k = 0; k < D; ++k) {
@@ -144,13 +210,19 @@ template <int D> __device__ __forceinline__ int64_t cellId(const DGrid& g, int c
 }
 
 template <int D>
-__global__ void k_cell_count(const double* __restrict__ pos, int64_t n, DGrid g, int32_t* __restrict__ cellOf,
-                             int32_t* __restrict__ cellCount) {
+__global__ void k_cell_count(const double* __restrict__ pos, int64_t n, DGrid g, SegGrids sg,
+                             int32_t* __restrict__ cellOf, int32_t* __restrict__ cellCount) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
+  int64_t base = 0;
+  if (sg.seg) {
+    const int m = int(sg.seg[i]);
+    g = sg.g[m];
+    base = sg.base[m];
+  }
   int c[3];
   cellCoords<D>(g, pos + i * D, c);
-  const int32_t id = int32_t(cellId<D>(g, c[0], c[1], c[2]));
+  const int32_t id = int32_t(base + cellId<D>(g, c[0], c[1], c[2]));
   cellOf[i] = id;
   atomicAdd(&cellCount[id], 1);
 }
@@ -407,12 +479,20 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
 // ---------------------------------------------------------------------------
 template <int D, bool ISO>
 __global__ void k_search_global(const double* __restrict__ rec, const int32_t* __restrict__ cellStart, SearchArgs a,
-                                int32_t* __restrict__ rowCount, int4* __restrict__ nbr) {
+                                SegGrids sg, int32_t* __restrict__ rowCount, int4* __restrict__ nbr) {
   using R = RL<ISO>;
   const int64_t k = xcdBlock((a.n + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   if (k >= a.n) return;
   const double* ri_ = rec + k * R::STRIDE;
-  if (int64_t(ri_[R::ORIG]) >= a.nInt) { rowCount[k] = 0; return; }
+  const int64_t oi = int64_t(ri_[R::ORIG]);
+  if (oi >= a.nInt) { rowCount[k] = 0; return; }
+  DGrid g = a.g;
+  int64_t base = 0;
+  if (sg.seg) {
+    const int m = int(sg.seg[oi]);
+    g = sg.g[m];
+    base = sg.base[m];
+  }
   constexpr int NS = NSym<D>::value;
   V<D> ri; S<D> Hi;
   for (int c = 0; c < D; ++c) ri.c[c] = ri_[R::X + c];
@@ -423,20 +503,20 @@ __global__ void k_search_global(const double* __restrict__ rec, const int32_t* _
     for (int c = 0; c < NS; ++c) Hi.c[c] = ri_[RL<false>::H + c];
   }
   int cc[3];
-  cellCoords<D>(a.g, ri.c, cc);
+  cellCoords<D>(g, ri.c, cc);
   RowWriter row(nbr, a.n, k, a.cap);
   const int dz0 = D >= 3 ? -1 : 0, dz1 = D >= 3 ? 1 : 0;
   const int dy0 = D >= 2 ? -1 : 0, dy1 = D >= 2 ? 1 : 0;
   for (int dz = dz0; dz <= dz1; ++dz) {
     const int z = cc[2] + dz;
-    if (z < 0 || z >= a.g.nc[2]) continue;
+    if (z < 0 || z >= g.nc[2]) continue;
     for (int dy = dy0; dy <= dy1; ++dy) {
       const int y = cc[1] + dy;
-      if (y < 0 || y >= a.g.nc[1]) continue;
+      if (y < 0 || y >= g.nc[1]) continue;
       for (int dx = -1; dx <= 1; ++dx) {
         const int x = cc[0] + dx;
-        if (x < 0 || x >= a.g.nc[0]) continue;
-        const int64_t id = cellId<D>(a.g, x, y, z);
+        if (x < 0 || x >= g.nc[0]) continue;
+        const int64_t id = base + cellId<D>(g, x, y, z);
         const int32_t b = cellStart[id], e = cellStart[id + 1];
         for (int32_t j = b; j < e; ++j) {
           if (j == k) continue;
@@ -659,37 +739,21 @@ void exclusiveScan(const TI* in, TO* out, int64_t n, Sim& s) {
   SG_CHECK_HIP(hipGetLastError());
 }
 
+// Search grid from the bounds hb (k_bounds layout).  Staged path: cells of
+// edge R/2 (5^3 stencil).  Global path: cells of edge >= R (3^nu stencil),
+// enlarged if the grid would be too fine.
 template <int D>
-void buildNeighborsD(Sim& s, double ext) {
-  const int64_t n = s.nTot;
-  hipStream_t st = s.stream;
-  SG_CHECK_HIP(hipEventRecord(s.ev[0], st));
-
-  // 1. bounds
-  const int nbB = int(std::min<int64_t>(1024, nblocks(n)));
-  s.reduceTmp.ensure(sizeof(double) * (size_t(nbB) * kNB + 16));
-  double* partials = s.reduceTmp.as<double>();
-  double* bounds = partials + size_t(nbB) * kNB;
-  hipLaunchKernelGGL((k_bounds<D>), dim3(nbB), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(), n, partials);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, st, partials, nbB, bounds);
-  SG_CHECK_HIP(hipGetLastError());
-  double hb[kNB];
-  SG_CHECK_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
-  SG_CHECK_HIP(hipStreamSynchronize(st));
+void gridFromBounds(const double* hb, double ext, bool allowStaged, Grid& G, DGrid& dg, int64_t& bricks,
+                    bool& stagedPath) {
   SG_REQUIRE(hb[6] > 0.0 && std::isfinite(hb[6]), SPHGPU_ERR_UNSUPPORTED,
              "neighbour search needs H with a positive Gershgorin bound on every node");
   for (int k = 0; k < D; ++k)
     SG_REQUIRE(std::isfinite(hb[k]) && std::isfinite(hb[3 + k]), SPHGPU_ERR_ARG, "non-finite node position");
-  s.isotropic = hb[7] > 0.5 && !s.hasMultipliers;
-
-  Grid& G = s.grid;
   const double R = ext / hb[6] * (1.0 + 1.0e-6);
   double spanX = 0.0;
   for (int k = 0; k < D; ++k) spanX = std::max(spanX, hb[3 + k] - hb[k]);
   const int maxCells = D == 3 ? 1024 : (D == 2 ? 32768 : (1 << 26));
-  // Staged path: cells of edge R/2 (5^3 stencil).  Global path: cells of edge
-  // >= R (3^nu stencil), enlarged if the grid would be too fine.
-  bool stagedPath = (D == 3) && s.isotropic;
+  stagedPath = (D == 3) && allowStaged;
   double cell = 0.5 * R * (1.0 + 1.0e-9);
   if (stagedPath && spanX / cell > maxCells - 2) stagedPath = false;
   if (!stagedPath) {
@@ -699,10 +763,10 @@ void buildNeighborsD(Sim& s, double ext) {
   G.span = stagedPath ? 2 : 1;
   G.cell = cell;
   G.R2 = R * R;
-  DGrid dg{};
+  dg = DGrid{};
   dg.cell = cell;
   dg.invCell = 1.0 / cell;
-  int64_t bricks = 1;
+  bricks = 1;
   const int bw = (D == 1) ? 1 : 4;
   for (int k = 0; k < 3; ++k) {
     G.lo[k] = k < D ? hb[k] : 0.0;
@@ -713,6 +777,73 @@ void buildNeighborsD(Sim& s, double ext) {
   }
   G.ncells = (D == 1) ? int64_t(G.nc[0]) : bricks * (D == 2 ? 16 : 64);
   SG_REQUIRE(G.ncells < (int64_t(1) << 31) - 2, SPHGPU_ERR_CAPACITY, "neighbour grid too large");
+}
+
+template <int D>
+void buildNeighborsD(Sim& s, double ext) {
+  const int64_t n = s.nTot;
+  hipStream_t st = s.stream;
+  SG_CHECK_HIP(hipEventRecord(s.ev[0], st));
+
+  // 1. bounds
+  Grid& G = s.grid;
+  DGrid dg{};
+  int64_t bricks = 1;
+  bool stagedPath = false;
+  double hb[kNB];
+  SegGrids sgr{nullptr, nullptr, nullptr};
+  if (s.nSeg > 0) {
+    const int ns = s.nSeg;
+    s.segBoundsBits.ensure(sizeof(unsigned long long) * size_t(ns) * kNB);
+    unsigned long long* bits = s.segBoundsBits.as<unsigned long long>();
+    hipLaunchKernelGGL(k_seg_bounds_init, dim3(nblocks(int64_t(ns) * kNB)), dim3(kBlock), 0, st, bits, ns);
+    hipLaunchKernelGGL((k_seg_bounds<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
+                       s.H.as<double>(), s.seg.as<double>(), n, bits);
+    SG_CHECK_HIP(hipGetLastError());
+    std::vector<unsigned long long> hbits(size_t(ns) * kNB);
+    SG_CHECK_HIP(hipMemcpyAsync(hbits.data(), bits, hbits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    SG_CHECK_HIP(hipStreamSynchronize(st));
+    std::vector<DGrid> grids(static_cast<size_t>(ns));
+    std::vector<int64_t> base(size_t(ns) + 1, 0);
+    bool iso = true;
+    for (int m = 0; m < ns; ++m) {
+      double hm[kNB];
+      for (int k = 0; k < kNB; ++k) hm[k] = fromOrderedBits(hbits[size_t(m) * kNB + k]);
+      iso = iso && hm[7] > 0.5;
+      Grid Gm;
+      int64_t bm;
+      bool staged;
+      gridFromBounds<D>(hm, ext, false, Gm, grids[size_t(m)], bm, staged);
+      base[size_t(m) + 1] = base[size_t(m)] + Gm.ncells;
+      if (m == 0) for (int k = 0; k < kNB; ++k) hb[k] = hm[k];
+      hb[6] = std::min(hb[6], hm[6]);
+      hb[8] = std::max(hb[8], hm[8]);
+    }
+    SG_REQUIRE(base.back() < (int64_t(1) << 31) - 2, SPHGPU_ERR_CAPACITY, "neighbour grid too large");
+    s.segGrid.ensure(sizeof(DGrid) * size_t(ns));
+    s.segCellBase.ensure(sizeof(int64_t) * size_t(ns + 1));
+    SG_CHECK_HIP(hipMemcpyAsync(s.segGrid.p, grids.data(), sizeof(DGrid) * size_t(ns), hipMemcpyHostToDevice, st));
+    SG_CHECK_HIP(hipMemcpyAsync(s.segCellBase.p, base.data(), sizeof(int64_t) * size_t(ns + 1), hipMemcpyHostToDevice,
+                                st));
+    // the host vectors must outlive the copies
+    SG_CHECK_HIP(hipStreamSynchronize(st));
+    s.isotropic = iso && !s.hasMultipliers;
+    G = Grid{};
+    G.ncells = base.back();
+    sgr = SegGrids{s.seg.as<double>(), s.segGrid.as<DGrid>(), s.segCellBase.as<int64_t>()};
+  } else {
+    const int nbB = int(std::min<int64_t>(1024, nblocks(n)));
+    s.reduceTmp.ensure(sizeof(double) * (size_t(nbB) * kNB + 16));
+    double* partials = s.reduceTmp.as<double>();
+    double* bounds = partials + size_t(nbB) * kNB;
+    hipLaunchKernelGGL((k_bounds<D>), dim3(nbB), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(), n, partials);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, st, partials, nbB, bounds);
+    SG_CHECK_HIP(hipGetLastError());
+    SG_CHECK_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
+    SG_CHECK_HIP(hipStreamSynchronize(st));
+    s.isotropic = hb[7] > 0.5 && !s.hasMultipliers;
+    gridFromBounds<D>(hb, ext, s.isotropic, G, dg, bricks, stagedPath);
+  }
 
   // 2. counting sort by cell, then by node index inside a cell
   s.cellOf.ensure(sizeof(int32_t) * size_t(n));
@@ -723,7 +854,7 @@ void buildNeighborsD(Sim& s, double ext) {
   s.iperm.ensure(sizeof(int32_t) * size_t(n));
   SG_CHECK_HIP(hipMemsetAsync(s.cellCount.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
   SG_CHECK_HIP(hipMemsetAsync(s.cellFill.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
-  hipLaunchKernelGGL((k_cell_count<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), n, dg,
+  hipLaunchKernelGGL((k_cell_count<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), n, dg, sgr,
                      s.cellOf.as<int32_t>(), s.cellCount.as<int32_t>());
   exclusiveScan<int32_t, int32_t>(s.cellCount.as<int32_t>(), s.cellStart.as<int32_t>(), G.ncells, s);
   hipLaunchKernelGGL(k_cell_scatter, dim3(nblocks(n)), dim3(kBlock), 0, st, s.cellOf.as<int32_t>(),
@@ -770,10 +901,10 @@ void buildNeighborsD(Sim& s, double ext) {
                          s.flags.as<int32_t>());
     } else if (s.isotropic) {
       hipLaunchKernelGGL((k_search_global<D, true>), dim3(int(xcdGrid(nblocks(n)))), dim3(kBlock), 0, st, s.rec.as<double>(),
-                         s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
+                         s.cellStart.as<int32_t>(), sa, sgr, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
     } else {
       hipLaunchKernelGGL((k_search_global<D, false>), dim3(int(xcdGrid(nblocks(n)))), dim3(kBlock), 0, st, s.rec.as<double>(),
-                         s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
+                         s.cellStart.as<int32_t>(), sa, sgr, s.rowCount.as<int32_t>(), s.nbr.as<int4>());
     }
     hipLaunchKernelGGL(k_row_stats, dim3(int(std::min<int64_t>(1024, nblocks(n)))), dim3(kBlock), 0, st,
                        s.rowCount.as<int32_t>(), n, dTotal, s.flags.as<int32_t>());

@@ -11,6 +11,10 @@ import os
 
 import numpy as np
 
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "lib", "libsphgpu.so")
 
@@ -59,6 +63,23 @@ class HydroParams(C.Structure):
 
 BOUNDARY_REFLECTING = 0
 BOUNDARY_PERIODIC = 1
+
+
+class Member(C.Structure):
+    """sphgpu_member: one simulation of an ensemble."""
+    _fields_ = [("n", C.c_int64), ("pos", _dp), ("vel", _dp), ("mass", _dp), ("rho", _dp), ("eps", _dp),
+                ("H", _dp), ("gamma", C.c_double), ("tEnd", C.c_double)]
+
+
+class Request(C.Structure):
+    """sphgpu_request: a simulation arriving at a serving loop."""
+    _fields_ = [("member", Member), ("arrival", C.c_double), ("outPos", _dp), ("outVel", _dp), ("outRho", _dp),
+                ("outEps", _dp), ("outH", _dp), ("admitted", C.c_double), ("finished", C.c_double),
+                ("steps", C.c_int64)]
+
+
+class ServeParams(C.Structure):
+    _fields_ = [("hbmBudget", C.c_int64), ("maxMembers", C.c_int), ("ghostFactor", C.c_double)]
 
 
 class Plane(C.Structure):
@@ -136,9 +157,6 @@ def hydro_params(**kw):
     return HydroParams(**d)
 
 
-_dp = C.POINTER(C.c_double)
-_i64p = C.POINTER(C.c_int64)
-_i32p = C.POINTER(C.c_int32)
 _lib = None
 
 
@@ -187,6 +205,17 @@ def lib():
             "sphgpu_apply_ghost_boundaries": (C.c_int, [vp]),
             "sphgpu_node_counts": (C.c_int, [vp, _i64p, _i64p]),
             "sphgpu_download_ghost_controls": (C.c_int, [vp, _i32p, _i32p]),
+            "sphgpu_ensemble_begin": (C.c_int, [vp]),
+            "sphgpu_ensemble_add": (C.c_int, [vp, C.POINTER(Member), C.POINTER(IntegratorParams), _i64p]),
+            "sphgpu_ensemble_step": (C.c_int, [vp, C.POINTER(HydroParams), C.POINTER(IntegratorParams),
+                                               C.POINTER(C.c_int)]),
+            "sphgpu_ensemble_members": (C.c_int, [vp, C.c_int, _i64p, _dp, _dp, _i64p, _i64p, _i64p,
+                                                  C.POINTER(C.c_int)]),
+            "sphgpu_ensemble_download": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
+            "sphgpu_ensemble_remove": (C.c_int, [vp, C.c_int, _i64p]),
+            "sphgpu_ensemble_bytes_per_node": (C.c_int64, [C.c_int]),
+            "sphgpu_serve": (C.c_int, [vp, C.POINTER(Request), C.c_int, C.POINTER(HydroParams),
+                                       C.POINTER(IntegratorParams), C.POINTER(ServeParams)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -413,6 +442,87 @@ class Sim:
 
     def synchronize(self):
         self._chk(self.L.sphgpu_synchronize(self.h))
+
+    # ensembles -------------------------------------------------------------------
+    def ensemble_begin(self):
+        """Switch this (empty) sim to an ensemble of independent members."""
+        self._chk(self.L.sphgpu_ensemble_begin(self.h))
+        self._members = {}
+
+    def ensemble_add(self, state, gamma, t_end, integ=None):
+        """Append a member from a state dict (pos, vel, mass, rho, eps, H);
+        returns its id."""
+        ip = integ if integ is not None else integrator_params(gamma=gamma)
+        f = lambda k: np.ascontiguousarray(state[k], dtype=np.float64)
+        arrs = {k: f(k) for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+        n = arrs["mass"].shape[0]
+        m = Member(n=n, gamma=gamma, tEnd=t_end, **{k: _p(v) for k, v in arrs.items()})
+        mid = C.c_int64()
+        self._chk(self.L.sphgpu_ensemble_add(self.h, C.byref(m), C.byref(ip), C.byref(mid)))
+        self._members[mid.value] = n
+        return mid.value
+
+    def ensemble_step(self, hydro=None, integ=None):
+        """One RK2 step of every member; returns how many reached their end time."""
+        hp = hydro if hydro is not None else hydro_params()
+        ip = integ if integ is not None else integrator_params()
+        done = C.c_int()
+        self._chk(self.L.sphgpu_ensemble_step(self.h, C.byref(hp), C.byref(ip), C.byref(done)))
+        return done.value
+
+    def ensemble_members(self):
+        """List of dicts (id, t, last_dt, cycle, first, rows) in row order."""
+        cnt = C.c_int()
+        self._chk(self.L.sphgpu_ensemble_members(self.h, 0, None, None, None, None, None, None, C.byref(cnt)))
+        n = cnt.value
+        ids, cyc, first, rows = (np.zeros(max(n, 1), dtype=np.int64) for _ in range(4))
+        t, ldt = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        self._chk(self.L.sphgpu_ensemble_members(self.h, n, _p(ids, _i64p), _p(t), _p(ldt), _p(cyc, _i64p),
+                                                 _p(first, _i64p), _p(rows, _i64p), C.byref(cnt)))
+        return [dict(id=int(ids[q]), t=float(t[q]), last_dt=float(ldt[q]), cycle=int(cyc[q]), first=int(first[q]),
+                     rows=int(rows[q])) for q in range(n)]
+
+    def ensemble_download(self, member_id, field):
+        c = state_components(field, self.dim)
+        a = np.zeros((self._members[member_id], c))
+        self._chk(self.L.sphgpu_ensemble_download(self.h, member_id, STATE_FIELDS[field], _p(a)))
+        return a[:, 0] if c == 1 else a
+
+    def ensemble_remove(self, ids):
+        ids = np.ascontiguousarray(list(ids), dtype=np.int64)
+        self._chk(self.L.sphgpu_ensemble_remove(self.h, len(ids), _p(ids, _i64p)))
+        for q in ids:
+            self._members.pop(int(q), None)
+
+    def serve(self, requests, hydro=None, integ=None, budget=0, max_members=1024, ghost_factor=1.5):
+        """Run the request loop (sphgpu_serve) on this empty ensemble.
+        requests: dicts with state, gamma, t_end, arrival (seconds).  Returns
+        per request: admitted, finished, steps and the final pos, vel, rho,
+        eps, H."""
+        hp = hydro if hydro is not None else hydro_params()
+        ip = integ if integ is not None else integrator_params()
+        dim, ns = self.dim, self.dim * (self.dim + 1) // 2
+        keep, outs = [], []
+        reqs = (Request * max(1, len(requests)))()
+        for q, r in enumerate(requests):
+            st = r["state"]
+            arrs = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+            n = arrs["mass"].shape[0]
+            o = dict(pos=np.zeros((n, dim)), vel=np.zeros((n, dim)), rho=np.zeros(n), eps=np.zeros(n),
+                     H=np.zeros((n, ns)))
+            keep.append(arrs)
+            outs.append(o)
+            reqs[q].member = Member(n=n, gamma=r["gamma"], tEnd=r["t_end"], **{k: _p(v) for k, v in arrs.items()})
+            reqs[q].arrival = r.get("arrival", 0.0)
+            reqs[q].outPos, reqs[q].outVel = _p(o["pos"]), _p(o["vel"])
+            reqs[q].outRho, reqs[q].outEps, reqs[q].outH = _p(o["rho"]), _p(o["eps"]), _p(o["H"])
+        sp = ServeParams(hbmBudget=int(budget), maxMembers=int(max_members), ghostFactor=float(ghost_factor))
+        self._chk(self.L.sphgpu_serve(self.h, reqs, len(requests), C.byref(hp), C.byref(ip), C.byref(sp)))
+        res = []
+        for q, o in enumerate(outs):
+            o.update(admitted=reqs[q].admitted, finished=reqs[q].finished, steps=int(reqs[q].steps))
+            res.append(o)
+        return res
 
     def copy_probe(self, nbytes, repeats=5):
         """Average time (ms) of a plain device-to-device streaming copy kernel."""

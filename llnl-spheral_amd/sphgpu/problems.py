@@ -154,3 +154,80 @@ def periodic_sinusoid(n, dim=3, amp=0.1, sigma=0.1, gamma=1.4, u=1.0, seed=2024,
     P, cs = gamma_law(rho, eps, gamma)
     return dict(pos=pos, vel=vel, mass=mass, rho=rho, eps=eps, P=P, cs=cs, H=H,
                 omega=np.ones(N)), dict(n_internal=N, dx=dx, gamma=gamma, nPerh=nPerh)
+
+
+def shock_tube(n, rhoL=1.0, PL=1.0, rhoR=0.125, PR=0.1, gamma=1.4, nPerh=1.51):
+    """Planar two-state tube on [0, 1] split at 0.5, equal node masses, the
+    nodes on each side in proportion to its mass (the layout of
+    tests/Hydro/Sod/Sod-planar-1d.py:14-32, 189-231), v = 0."""
+    mtot = 0.5 * rhoL + 0.5 * rhoR
+    m = mtot / n
+    nL = int(round(n * 0.5 * rhoL / mtot))
+    nL = min(max(nL, 1), n - 1)
+    nR = n - nL
+    xL = (np.arange(nL) + 0.5) * (0.5 / nL)
+    xR = 0.5 + (np.arange(nR) + 0.5) * (0.5 / nR)
+    pos = np.concatenate([xL, xR])[:, None]
+    rho = np.concatenate([np.full(nL, rhoL), np.full(nR, rhoR)])
+    P = np.concatenate([np.full(nL, PL), np.full(nR, PR)])
+    eps = P / ((gamma - 1.0) * rho)
+    dx = np.concatenate([np.full(nL, 0.5 / nL), np.full(nR, 0.5 / nR)])
+    H = (1.0 / (nPerh * dx))[:, None]
+    return dict(pos=pos, vel=np.zeros((n, 1)), mass=np.full(n, m), rho=rho, eps=eps, H=H)
+
+
+def tube_ensemble(count=512, nmin=200, nmax=20000, t_end=0.2, seed=4242):
+    """Batched shock tubes: node counts log-uniform in [nmin, nmax], left
+    rho, P ~ U[0.5, 2], right rho, P ~ U[0.05, 0.5], v = 0, gamma in
+    {1.4, 5/3}, each run to t_end.  Returns a list of dicts (state, gamma,
+    t_end, n)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).round().astype(np.int64)
+    rhoL, PL = rng.uniform(0.5, 2.0, count), rng.uniform(0.5, 2.0, count)
+    rhoR, PR = rng.uniform(0.05, 0.5, count), rng.uniform(0.05, 0.5, count)
+    gam = np.where(rng.uniform(0.0, 1.0, count) < 0.5, 1.4, 5.0 / 3.0)
+    out = []
+    for q in range(count):
+        st = shock_tube(int(ns[q]), rhoL[q], PL[q], rhoR[q], PR[q], gam[q])
+        out.append(dict(state=st, gamma=float(gam[q]), t_end=float(t_end), n=int(ns[q])))
+    return out
+
+
+def bspline_w(dim, eta):
+    """Cubic B-spline kernel W(eta) with unit det H (BSplineKernelInline.hh:9-100)."""
+    A = {1: 2.0 / 3.0, 2: 10.0 / (7.0 * np.pi), 3: 1.0 / np.pi}[dim]
+    eta = np.asarray(eta, dtype=np.float64)
+    w = np.where(eta < 1.0, 1.0 - 1.5 * eta ** 2 + 0.75 * eta ** 3,
+                 np.where(eta < 2.0, 0.25 * (2.0 - eta) ** 3, 0.0))
+    return A * w
+
+
+def point_blast(n, gamma=5.0 / 3.0, E=1.0, P0=1.0e-6, jitter=0.01, seed=77, spike_scale=0.5, nPerh=1.51):
+    """2-D point blast: n x n jittered lattice on [0, 1]^2, rho = 1, P = P0,
+    v = 0, energy E deposited with the smoothed-spike rule of
+    tests/Hydro/Sedov/Sedov-cylindrical-2d.py:240-262 about the box centre:
+    eps_i = eps0 + E W(|H_i (x_i - c)|/spike_scale) / (m_i sum W)."""
+    pos, mass, H, dx = lattice(2, n, jitter=jitter, seed=seed, nPerh=nPerh)
+    N = pos.shape[0]
+    eps0 = P0 / ((gamma - 1.0) * 1.0)
+    eta = np.sqrt((((pos - 0.5) * H[:, [0]]) ** 2).sum(1))
+    W = bspline_w(2, eta / spike_scale)
+    eps = eps0 + E * W / mass / W.sum()
+    return dict(pos=pos, vel=np.zeros((N, 2)), mass=mass, rho=np.ones(N), eps=eps, H=H)
+
+
+def blast_requests(count, rate, nmin=1000, nmax=250000, tmin=0.05, tmax=0.2, seed=99):
+    """Poisson arrivals (rate per second) of 2-D point blasts with node counts
+    log-uniform in [nmin, nmax] (rounded to squares) and end times U[tmin,
+    tmax].  Returns a list of dicts (state, gamma, t_end, arrival, n)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, count)
+    arrival = np.cumsum(gaps) - gaps[0]
+    side = np.sqrt(np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))).round().astype(np.int64)
+    tend = rng.uniform(tmin, tmax, count)
+    out = []
+    for q in range(count):
+        st = point_blast(int(side[q]), seed=int(rng.integers(1 << 30)))
+        out.append(dict(state=st, gamma=5.0 / 3.0, t_end=float(tend[q]), arrival=float(arrival[q]),
+                        n=int(side[q]) ** 2))
+    return out
