@@ -106,6 +106,156 @@ def cpu_baseline(budget_s=10.0, n=48, boundaries="reflecting"):
     }
 
 
+def plain_solve(sphgpu, dim, st, gamma, t_end, hp, prec, order):
+    """One problem alone as a plain simulation (sphgpu_rk2_step loop); returns
+    (wall seconds, steps, final fields)."""
+    N = st["mass"].shape[0]
+    s = sphgpu.Sim(dim)
+    s.set_row_order(order)
+    s.set_table_bspline(1000)
+    s.set_precision(prec)
+    s.set_nodes(N, N)
+    s.upload_state({k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")})
+    s.set_boundaries(sphgpu.box_boundaries(dim))
+    t0 = time.perf_counter()
+    s.eos_gamma_law(gamma)
+    ip = sphgpu.integrator_params(gamma=gamma)
+    steps = 0
+    while s.clock()[0] < t_end:
+        s.rk2_step(hp, ip, max_time=t_end)
+        steps += 1
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    out = {f: s.download(f, rows=N) for f in ("velocity", "mass density", "specific thermal energy")}
+    s.close()
+    return wall, steps, out
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else None
+
+
+def ensemble_bench(sphgpu, problems, count, precision, sample):
+    """Batched shock tubes (count tubes, 200-20000 nodes, to t = 0.2): every
+    member advanced by one launch sequence per step; time-to-solution of each
+    tube from the common start.  The one-at-a-time baseline solves `sample`
+    tubes spread over the sizes alone and scales their per-step time to all
+    tubes by node count (stated as an estimate)."""
+    prec = sphgpu.PRECISION_FP64 if precision == "fp64" else sphgpu.PRECISION_MIXED
+    order = sphgpu.ROW_ORDER_SEARCH
+    members = problems.tube_ensemble(count)
+    hp = sphgpu.hydro_params(precision=prec)
+    ip = sphgpu.integrator_params()
+    e = sphgpu.Sim(1)
+    e.set_row_order(order)
+    e.set_table_bspline(1000)
+    e.set_precision(prec)
+    e.set_boundaries(sphgpu.box_boundaries(1))
+    e.ensemble_begin()
+    print(f"[bench] ensemble of {count} tubes, {sum(m['n'] for m in members)} nodes", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ids = {}
+    for q, m in enumerate(members):
+        ids[e.ensemble_add(m["state"], m["gamma"], m["t_end"], sphgpu.integrator_params(gamma=m["gamma"]))] = q
+    rows = {i: members[q]["n"] for i, q in ids.items()}
+    finish, cycles = {}, {}
+    steps = node_steps = 0
+    last_report = t0
+    while ids:
+        done = e.ensemble_step(hp, ip)
+        steps += 1
+        node_steps += sum(rows[i] for i in ids)
+        if done:
+            now = time.perf_counter() - t0
+            gone = [mb["id"] for mb in e.ensemble_members() if mb["t"] >= members[ids[mb["id"]]]["t_end"]]
+            for mb in e.ensemble_members():
+                if mb["id"] in gone:
+                    cycles[ids[mb["id"]]] = mb["cycle"]
+            e.ensemble_remove(gone)
+            for g in gone:
+                finish[ids.pop(g)] = now
+        if time.perf_counter() - last_report > 20.0:
+            last_report = time.perf_counter()
+            print(f"[bench] ensemble step {steps}: {len(ids)} tubes left", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    e.close()
+    tts = [finish[q] for q in range(count)]
+    # one-at-a-time plain solves of a sample spread over the sizes
+    by_n = sorted(range(count), key=lambda q: members[q]["n"])
+    pick = [by_n[int(round(k * (count - 1) / max(1, sample - 1)))] for k in range(sample)] if sample > 0 else []
+    samp = []
+    for q in pick:
+        m = members[q]
+        w, s, _ = plain_solve(sphgpu, 1, m["state"], m["gamma"], m["t_end"], hp, prec, order)
+        samp.append((m["n"], w / s, s))
+    est = None
+    if samp:
+        ln = np.log([x[0] for x in samp])
+        lt = np.log([x[1] for x in samp])
+        per_step = lambda n: float(np.exp(np.interp(np.log(n), ln, lt)))
+        est = float(sum(cycles[q] * per_step(members[q]["n"]) for q in range(count)))
+    return {
+        "workload": f"{count} planar shock tubes, n log-uniform in [200, 20000] (total {sum(m['n'] for m in members)}), "
+                    "rho_L,P_L~U[0.5,2], rho_R,P_R~U[0.05,0.5], gamma in {1.4,5/3}, reflecting walls, t_end 0.2",
+        "precision": precision,
+        "wall_s": wall, "steps": steps, "node_steps": node_steps, "node_steps_per_s": node_steps / wall,
+        "time_to_solution_s": {"p50": pct(tts, 50), "p99": pct(tts, 99), "max": max(tts)},
+        "member_steps": {"p50": pct(list(cycles.values()), 50), "max": max(cycles.values())},
+        "sequential_plain": {"sample": [{"n": n, "ms_per_step": 1e3 * t, "steps": s} for n, t, s in samp],
+                             "estimated_total_s": est,
+                             "method": "per-step time of the sampled tubes alone, interpolated in log n, times each "
+                                       "tube's step count from the ensemble run"},
+    }
+
+
+def serve_bench(sphgpu, problems, count, rate, precision, check):
+    """Poisson arrivals of 2-D point blasts through the request loop
+    (continuous batching); time-to-solution = finish - arrival."""
+    prec = sphgpu.PRECISION_FP64 if precision == "fp64" else sphgpu.PRECISION_MIXED
+    order = sphgpu.ROW_ORDER_SEARCH
+    reqs = problems.blast_requests(count, rate)
+    hp = sphgpu.hydro_params(precision=prec)
+    ip = sphgpu.integrator_params()
+    print(f"[bench] serving {count} blasts at {rate}/s, {sum(r['n'] for r in reqs)} nodes", file=sys.stderr,
+          flush=True)
+
+    def run(max_members):
+        e = sphgpu.Sim(2)
+        e.set_row_order(order)
+        e.set_table_bspline(1000)
+        e.set_precision(prec)
+        e.set_boundaries(sphgpu.box_boundaries(2))
+        e.ensemble_begin()
+        t0 = time.perf_counter()
+        out = e.serve(reqs, hp, ip, max_members=max_members)
+        wall = time.perf_counter() - t0
+        e.close()
+        tts = [o["finished"] - r["arrival"] for o, r in zip(out, reqs)]
+        ns = sum(o["steps"] * r["n"] for o, r in zip(out, reqs))
+        return out, {"wall_s": wall, "time_to_solution_s": {"p50": pct(tts, 50), "p99": pct(tts, 99),
+                                                             "max": max(tts)},
+                     "node_steps_per_s": ns / wall, "max_members": max_members}
+
+    out, batched = run(1024)
+    print(f"[bench] serving batched: {batched['wall_s']:.1f}s", file=sys.stderr, flush=True)
+    _, single = run(1)
+    res = {"workload": f"{count} 2-D point blasts (E=1, rho=1, P=1e-6, gamma 5/3, n log-uniform in [1k, 250k], "
+                       f"t_end~U[0.05,0.2], reflecting box), Poisson arrivals at {rate}/s",
+           "precision": precision, "batched": batched, "one_at_a_time": single}
+    if check > 0 and prec == sphgpu.PRECISION_MIXED:
+        # error of the mixed-precision results vs FP64 solves alone, smallest requests
+        errs = {}
+        for q in sorted(range(count), key=lambda q: reqs[q]["n"])[:check]:
+            r = reqs[q]
+            _, _, ref = plain_solve(sphgpu, 2, r["state"], r["gamma"], r["t_end"], sphgpu.hydro_params(),
+                                    sphgpu.PRECISION_FP64, order)
+            for f, k in (("mass density", "rho"), ("specific thermal energy", "eps"), ("velocity", "vel")):
+                a, b = np.asarray(out[q][k]).ravel(), np.asarray(ref[f]).ravel()
+                errs[k] = max(errs.get(k, 0.0), float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)))
+        res["error_vs_fp64"] = {"rel_linf": errs, "requests": check}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +274,15 @@ def main():
                     help="SynchronousRK2 steps timed after the pass measurement (reported under rk2; 0 = off)")
     ap.add_argument("--probe-mib", type=int, default=256,
                     help="size of the plain copy kernel run after the timed steps (counter calibration); 0 = off")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="also run the batched shock-tube ensemble with this many tubes (512 = the benchmark)")
+    ap.add_argument("--ensemble-sample", type=int, default=8,
+                    help="tubes solved alone for the one-at-a-time estimate")
+    ap.add_argument("--serve", type=int, default=0, help="also run the serving loop with this many blast requests")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-check", type=int, default=3, help="requests re-solved alone in FP64 for the error")
+    ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--only-batch", action="store_true", help="skip the 128^3 pass measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -132,6 +291,17 @@ def main():
 
     import sphgpu
     from sphgpu import problems
+
+    if args.only_batch:
+        line = {"metric": METRIC, "n_gpus": 1}
+        if args.ensemble:
+            line["ensemble"] = ensemble_bench(sphgpu, problems, args.ensemble, args.batch_precision,
+                                              args.ensemble_sample)
+        if args.serve:
+            line["serving"] = serve_bench(sphgpu, problems, args.serve, args.serve_rate, args.batch_precision,
+                                          args.serve_check)
+        print(json.dumps(line), flush=True)
+        return
 
     t_setup = time.perf_counter()
     st, meta = problems.converging_flow(args.n)
@@ -241,6 +411,11 @@ def main():
         line["copy_probe"] = {"bytes_read": nb, "bytes_written": nb, "ms": ms,
                               "GBps": 2 * nb / (ms * 1e-3) / 1e9,
                               "kernel": "k_copy_probe (double4 streaming copy)"}
+    if args.ensemble:
+        line["ensemble"] = ensemble_bench(sphgpu, problems, args.ensemble, args.batch_precision, args.ensemble_sample)
+    if args.serve:
+        line["serving"] = serve_bench(sphgpu, problems, args.serve, args.serve_rate, args.batch_precision,
+                                      args.serve_check)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args.cpu_budget, boundaries=args.boundaries)
