@@ -80,6 +80,23 @@ void installTable(Sim& s) {
 
 }  // namespace
 
+namespace sphgpu {
+// Stage times of the last call of each stage (events recorded on the stream).
+void resolveTimings(Sim& s) {
+  auto el = [&](int a, int b) {
+    float ms = 0.f;
+    SG_CHECK_HIP(hipEventSynchronize(s.ev[b]));
+    SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[a], s.ev[b]));
+    return ms;
+  };
+  if (s.timingPending & 1u) s.timing.neighbors_ms = el(0, 1);
+  if (s.timingPending & 2u) s.timing.density_ms = el(2, 3);
+  if (s.timingPending & 4u) { s.timing.derivs_ms = el(5, 6); s.timing.pack_ms = el(4, 5); }
+  if (s.timingPending & 8u) { s.timing.ghosts_ms = el(8, 9); s.timing.cull_ms = el(10, 11); }
+  s.timingPending = 0;
+}
+}  // namespace sphgpu
+
 extern "C" {
 
 int sphgpu_create(int dim, int device, sphgpu_sim** out) {
@@ -98,6 +115,13 @@ int sphgpu_create(int dim, int device, sphgpu_sim** out) {
   for (auto& e : s->ev) {
     if (hipEventCreate(&e) != hipSuccess) { sphgpu_destroy(s); return SPHGPU_ERR_HIP; }
   }
+  void* pinned = nullptr;
+  if (hipHostMalloc(&pinned, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    sphgpu_destroy(s);
+    return SPHGPU_ERR_HIP;
+  }
+  s->ncalcHost = static_cast<unsigned long long*>(pinned);
+  *s->ncalcHost = 0;
   *out = s;
   return SPHGPU_OK;
 }
@@ -112,6 +136,7 @@ void sphgpu_destroy(sphgpu_sim* s) {
                   &s->scanTmp, &s->reduceTmp, &s->flags, &s->rec, &s->out};
   for (DBuf* b : bufs) b->release();
   for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
+  if (s->ncalcHost) (void)hipHostFree(s->ncalcHost);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -411,13 +436,15 @@ int sphgpu_pair_count(sphgpu_sim* s, int64_t* n) {
   return guarded(s, [&] {
     SG_REQUIRE(n, SPHGPU_ERR_ARG, "null output");
     SG_REQUIRE(s->derivsValid, SPHGPU_ERR_STATE, "no current derivatives");
-    *n = s->ncalc;
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+    *n = int64_t(*s->ncalcHost);
   });
 }
 
 int sphgpu_last_timings(sphgpu_sim* s, double* ms) {
   return guarded(s, [&] {
     SG_REQUIRE(ms, SPHGPU_ERR_ARG, "null output");
+    resolveTimings(*s);
     ms[0] = s->timing.neighbors_ms; ms[1] = s->timing.density_ms; ms[2] = s->timing.derivs_ms; ms[3] = s->timing.pack_ms;
     ms[4] = s->timing.ghosts_ms; ms[5] = s->timing.cull_ms;
   });

@@ -1098,10 +1098,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   }
   SG_CHECK_HIP(hipGetLastError());
   SG_CHECK_HIP(hipEventRecord(s.ev[6], st));
-  unsigned long long nc = 0;
-  SG_CHECK_HIP(hipMemcpyAsync(&nc, dncalc, sizeof(nc), hipMemcpyDeviceToHost, st));
-  SG_CHECK_HIP(hipStreamSynchronize(st));
-  s.ncalc = int64_t(nc);
+  SG_CHECK_HIP(hipMemcpyAsync(s.ncalcHost, dncalc, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 }
 
 }  // namespace
@@ -1149,10 +1146,7 @@ void sumDensity(Sim& s) {
   if (s.dim == 1) sumDensityD<1>(s);
   else if (s.dim == 2) sumDensityD<2>(s);
   else sumDensityD<3>(s);
-  float ms = 0.f;
-  SG_CHECK_HIP(hipEventSynchronize(s.ev[3]));
-  SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[2], s.ev[3]));
-  s.timing.density_ms = ms;
+  s.timingPending |= 2u;
   s.derivsValid = false;
 }
 
@@ -1170,11 +1164,7 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
   if (s.dim == 1) evalDerivsD<1>(s, p);
   else if (s.dim == 2) evalDerivsD<2>(s, p);
   else evalDerivsD<3>(s, p);
-  float ms = 0.f, msp = 0.f;
-  SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[5], s.ev[6]));
-  SG_CHECK_HIP(hipEventElapsedTime(&msp, s.ev[4], s.ev[5]));
-  s.timing.derivs_ms = ms;
-  s.timing.pack_ms = msp;
+  s.timingPending |= 4u;
   s.derivsValid = true;
 }
 

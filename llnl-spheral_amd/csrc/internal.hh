@@ -194,7 +194,8 @@ struct Sim {
   bool ensemble = false;
   int nSeg = 0;
   DBuf seg;                      // member index per caller row (as a double)
-  DBuf segGrid, segCellBase;     // per member: neighbour grid, first global cell (+1 entry)
+  DBuf segGrid, segCellBase;     // neighbour grid list (levels, members): grid, first global cell (+1 entry)
+  DBuf gridFirst;                // first grid of each member (+1 entry)
   DBuf segBoundsBits;            // per member: 9 ordered-bit bounds of the neighbour build
   DBuf segHmaxBits;              // per member: ghost-selection hmax bits
   DBuf segState;                 // SegState per member
@@ -203,9 +204,14 @@ struct Sim {
   int64_t nextMemberId = 0;
   DBuf segSpare, dtInSpare;
 
-  hipEvent_t ev[12] = {};
+  hipEvent_t ev[14] = {};          // 0-11 stage timings, 12-13 the copy probe
   Timing timing;
+  // Stage timings and the pair count are read back lazily (no host wait
+  // inside a step): bit 0 neighbours, 1 density, 2 derivatives, 3 ghosts.
+  unsigned timingPending = 0;
+  unsigned long long* ncalcHost = nullptr;   // pinned; written by the stream after each pass
 };
+void resolveTimings(Sim& s);
 
 // neighbors.hip
 void buildNeighbors(Sim& s, double kernelExtent);

@@ -59,9 +59,10 @@ template <int D> __device__ bool isIsotropic(const double* h) {
   else return h[1] == 0.0 && h[2] == 0.0 && h[4] == 0.0 && h[3] == h[0] && h[5] == h[0];
 }
 
-// reduced values: lo[3] hi[3] min Gershgorin bound, isotropic flag, max diagonal of H
-constexpr int kNB = 9;
-__device__ __forceinline__ bool isMaxSlot(int k) { return (k >= 3 && k < 6) || k == 8; }
+// reduced values: lo[3] hi[3] min Gershgorin bound, isotropic flag, max diagonal
+// of H, max Gershgorin bound
+constexpr int kNB = 10;
+__host__ __device__ __forceinline__ bool isMaxSlot(int k) { return (k >= 3 && k < 6) || k == 8 || k == 9; }
 
 template <int D> __device__ double maxDiag(const double* h) {
   double m = h[0];
@@ -72,13 +73,15 @@ template <int D> __device__ double maxDiag(const double* h) {
 template <int D>
 __global__ void k_bounds(const double* __restrict__ pos, const double* __restrict__ H, int64_t n, double* partials) {
   __shared__ double sh[kNB][kBlock];
-  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300};
+  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
     for (int k = 0; k < D; ++k) { const double x = pos[i * D + k]; r[k] = dmin(r[k], x); r[3 + k] = dmax(r[3 + k], x); }
     const double* h = H + i * NSym<D>::value;
-    r[6] = dmin(r[6], gershgorinMin<D>(h));
+    const double gm = gershgorinMin<D>(h);
+    r[6] = dmin(r[6], gm);
     if (!isIsotropic<D>(h)) r[7] = 0.0;
     r[8] = dmax(r[8], maxDiag<D>(h));
+    r[9] = dmax(r[9], gm);
   }
   for (int k = 0; k < kNB; ++k) sh[k][threadIdx.x] = r[k];
   __syncthreads();
@@ -95,7 +98,7 @@ __global__ void k_bounds(const double* __restrict__ pos, const double* __restric
 
 __global__ void k_bounds_final(const double* partials, int nb, double* out) {
   __shared__ double sh[kNB][kBlock];
-  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300};
+  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int k = 0; k < kNB; ++k) {
       const double v = partials[b * kNB + k];
@@ -122,12 +125,22 @@ struct DGrid {
   int nb[3];
 };
 
-// Ensemble grids: member index per caller row, grid and first global cell per member.
+// Grid lists.  Grids [first[m], first[m+1]) serve ensemble member m (m = 0
+// without an ensemble), in increasing cell size: the levels of a node set
+// whose search radii span more than a factor 2.  A node lives in the first of
+// its member's grids whose cell is at least its search radius bound
+// ext/gershgorin(H)(1 + 1e-6).  g == null: one grid, passed by value.
 struct SegGrids {
-  const double* seg;        // null: one grid for all nodes
+  const double* seg;        // member index per caller row, or null
   const DGrid* g;
-  const int64_t* base;
+  const int64_t* base;      // first global cell of each grid
+  const int32_t* first;     // grids of each member, +1 entry
+  double ext;
 };
+
+template <int D> __device__ __forceinline__ double searchRadius(const double* h, double ext) {
+  return ext / gershgorinMin<D>(h) * (1.0 + 1.0e-6);
+}
 
 // Order-preserving map of a double onto uint64 (for atomic min/max).
 __device__ __forceinline__ unsigned long long orderedBits(double x) {
@@ -152,7 +165,7 @@ template <int D>
 __global__ void k_seg_bounds(const double* __restrict__ pos, const double* __restrict__ H,
                              const double* __restrict__ seg, int64_t n, unsigned long long* __restrict__ bits) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300};
+  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
   int m = -1;
   if (i < n) {
     m = int(seg[i]);
@@ -161,6 +174,7 @@ __global__ void k_seg_bounds(const double* __restrict__ pos, const double* __res
     r[6] = gershgorinMin<D>(h);
     r[7] = isIsotropic<D>(h) ? 1.0 : 0.0;
     r[8] = maxDiag<D>(h);
+    r[9] = r[6];
   }
   const int m0 = __shfl(m, 0);
   if (m0 < 0) return;                      // lanes past n come last: the whole wavefront is idle
@@ -210,15 +224,21 @@ template <int D> __device__ __forceinline__ int64_t cellId(const DGrid& g, int c
 }
 
 template <int D>
-__global__ void k_cell_count(const double* __restrict__ pos, int64_t n, DGrid g, SegGrids sg,
-                             int32_t* __restrict__ cellOf, int32_t* __restrict__ cellCount) {
+__global__ void k_cell_count(const double* __restrict__ pos, const double* __restrict__ H, int64_t n, DGrid g,
+                             SegGrids sg, int32_t* __restrict__ cellOf, int32_t* __restrict__ cellCount) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   int64_t base = 0;
-  if (sg.seg) {
-    const int m = int(sg.seg[i]);
-    g = sg.g[m];
-    base = sg.base[m];
+  if (sg.g) {
+    const int m = sg.seg ? int(sg.seg[i]) : 0;
+    int q = sg.first[m];
+    const int e = sg.first[m + 1];
+    if (e - q > 1) {
+      const double R = searchRadius<D>(H + i * NSym<D>::value, sg.ext);
+      while (q < e - 1 && R > sg.g[q].cell) ++q;
+    }
+    g = sg.g[q];
+    base = sg.base[q];
   }
   int c[3];
   cellCoords<D>(g, pos + i * D, c);
@@ -283,7 +303,7 @@ constexpr int kCandCap = 2048;
 
 struct SearchArgs {
   DGrid g;
-  double ext2;
+  double ext, ext2;
   float lo32, hi32;         // ext2*(1 -+ 1e-5): FP32 classification margins
   int64_t n, nInt;
   int cap;
@@ -486,12 +506,11 @@ __global__ void k_search_global(const double* __restrict__ rec, const int32_t* _
   const double* ri_ = rec + k * R::STRIDE;
   const int64_t oi = int64_t(ri_[R::ORIG]);
   if (oi >= a.nInt) { rowCount[k] = 0; return; }
-  DGrid g = a.g;
-  int64_t base = 0;
-  if (sg.seg) {
-    const int m = int(sg.seg[oi]);
-    g = sg.g[m];
-    base = sg.base[m];
+  int ga = 0, gb = 1;
+  if (sg.g) {
+    const int m = sg.seg ? int(sg.seg[oi]) : 0;
+    ga = sg.first[m];
+    gb = sg.first[m + 1];
   }
   constexpr int NS = NSym<D>::value;
   V<D> ri; S<D> Hi;
@@ -502,20 +521,37 @@ __global__ void k_search_global(const double* __restrict__ rec, const int32_t* _
   } else {
     for (int c = 0; c < NS; ++c) Hi.c[c] = ri_[RL<false>::H + c];
   }
-  int cc[3];
-  cellCoords<D>(g, ri.c, cc);
+  // own search radius bound (only needed with several levels)
+  double Ri = 0.0;
+  if (gb - ga > 1) {
+    if constexpr (ISO) Ri = a.ext / ri_[RL<true>::HINV] * (1.0 + 1.0e-6);
+    else Ri = a.ext / gershgorinMin<D>(Hi.c) * (1.0 + 1.0e-6);
+  }
   RowWriter row(nbr, a.n, k, a.cap);
-  const int dz0 = D >= 3 ? -1 : 0, dz1 = D >= 3 ? 1 : 0;
-  const int dy0 = D >= 2 ? -1 : 0, dy1 = D >= 2 ? 1 : 0;
-  for (int dz = dz0; dz <= dz1; ++dz) {
-    const int z = cc[2] + dz;
-    if (z < 0 || z >= g.nc[2]) continue;
-    for (int dy = dy0; dy <= dy1; ++dy) {
-      const int y = cc[1] + dy;
-      if (y < 0 || y >= g.nc[1]) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = cc[0] + dx;
-        if (x < 0 || x >= g.nc[0]) continue;
+  for (int q = ga; q < gb; ++q) {
+  const DGrid g = sg.g ? sg.g[q] : a.g;
+  const int64_t base = sg.g ? sg.base[q] : 0;
+  // cells that can hold a neighbour: the 3^nu stencil when the grid's cell
+  // covers both radii (nodes of a grid have radius <= its cell), else every
+  // cell within Ri
+  int c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0};
+  {
+    int cc[3];
+    cellCoords<D>(g, ri.c, cc);
+    for (int d = 0; d < D; ++d) {
+      if (Ri <= g.cell) {
+        c0[d] = cc[d] - 1 < 0 ? 0 : cc[d] - 1;
+        c1[d] = cc[d] + 1 >= g.nc[d] ? g.nc[d] - 1 : cc[d] + 1;
+      } else {
+        const double t0 = floor((ri.c[d] - Ri - g.lo[d]) * g.invCell), t1 = floor((ri.c[d] + Ri - g.lo[d]) * g.invCell);
+        c0[d] = t0 < 0.0 ? 0 : (t0 > double(g.nc[d] - 1) ? g.nc[d] - 1 : int(t0));
+        c1[d] = t1 < 0.0 ? 0 : (t1 > double(g.nc[d] - 1) ? g.nc[d] - 1 : int(t1));
+      }
+    }
+  }
+  for (int z = c0[2]; z <= c1[2]; ++z) {
+    for (int y = c0[1]; y <= c1[1]; ++y) {
+      for (int x = c0[0]; x <= c1[0]; ++x) {
         const int64_t id = base + cellId<D>(g, x, y, z);
         const int32_t b = cellStart[id], e = cellStart[id + 1];
         for (int32_t j = b; j < e; ++j) {
@@ -539,6 +575,7 @@ __global__ void k_search_global(const double* __restrict__ rec, const int32_t* _
         }
       }
     }
+  }
   }
   row.finish();
   rowCount[k] = row.cnt;
@@ -739,6 +776,26 @@ void exclusiveScan(const TI* in, TO* out, int64_t n, Sim& s) {
   SG_CHECK_HIP(hipGetLastError());
 }
 
+// A grid of cell edge `cell` over the bounds hb: cells per axis, bricks, cell count.
+template <int D>
+void gridWithCell(const double* hb, double cell, Grid& G, DGrid& dg, int64_t& bricks) {
+  G.cell = cell;
+  dg = DGrid{};
+  dg.cell = cell;
+  dg.invCell = 1.0 / cell;
+  bricks = 1;
+  const int bw = (D == 1) ? 1 : 4;
+  for (int k = 0; k < 3; ++k) {
+    G.lo[k] = k < D ? hb[k] : 0.0;
+    G.nc[k] = k < D ? int(std::floor((hb[3 + k] - hb[k]) / cell)) + 1 : 1;
+    G.nb[k] = k < D ? (G.nc[k] + bw - 1) / bw : 1;
+    dg.lo[k] = G.lo[k]; dg.nc[k] = G.nc[k]; dg.nb[k] = G.nb[k];
+    if (k < D) bricks *= G.nb[k];
+  }
+  G.ncells = (D == 1) ? int64_t(G.nc[0]) : bricks * (D == 2 ? 16 : 64);
+  SG_REQUIRE(G.ncells < (int64_t(1) << 31) - 2, SPHGPU_ERR_CAPACITY, "neighbour grid too large");
+}
+
 // Search grid from the bounds hb (k_bounds layout).  Staged path: cells of
 // edge R/2 (5^3 stencil).  Global path: cells of edge >= R (3^nu stencil),
 // enlarged if the grid would be too fine.
@@ -761,22 +818,59 @@ void gridFromBounds(const double* hb, double ext, bool allowStaged, Grid& G, DGr
     if (spanX / cell > maxCells - 2) cell = spanX / (maxCells - 2);
   }
   G.span = stagedPath ? 2 : 1;
-  G.cell = cell;
   G.R2 = R * R;
-  dg = DGrid{};
-  dg.cell = cell;
-  dg.invCell = 1.0 / cell;
-  bricks = 1;
-  const int bw = (D == 1) ? 1 : 4;
-  for (int k = 0; k < 3; ++k) {
-    G.lo[k] = k < D ? hb[k] : 0.0;
-    G.nc[k] = k < D ? int(std::floor((hb[3 + k] - hb[k]) / cell)) + 1 : 1;
-    G.nb[k] = k < D ? (G.nc[k] + bw - 1) / bw : 1;
-    dg.lo[k] = G.lo[k]; dg.nc[k] = G.nc[k]; dg.nb[k] = G.nb[k];
-    if (k < D) bricks *= G.nb[k];
+  gridWithCell<D>(hb, cell, G, dg, bricks);
+}
+
+// Levels of a node set whose search radii span more than a factor 2: cells
+// of edge Rmin 2^L (>= R at the top level), each grid enlarged if too fine.
+// One grid when the radii are within a factor 2.
+template <int D>
+void levelGrids(const double* hb, double ext, std::vector<DGrid>& out, std::vector<int64_t>& cells) {
+  const double R = ext / hb[6] * (1.0 + 1.0e-6);
+  const double Rmin = ext / hb[9] * (1.0 + 1.0e-6);
+  Grid G;
+  DGrid dg;
+  int64_t bricks;
+  if (!(R > 2.0 * Rmin)) {
+    bool staged;
+    gridFromBounds<D>(hb, ext, false, G, dg, bricks, staged);
+    out.push_back(dg);
+    cells.push_back(G.ncells);
+    return;
   }
-  G.ncells = (D == 1) ? int64_t(G.nc[0]) : bricks * (D == 2 ? 16 : 64);
-  SG_REQUIRE(G.ncells < (int64_t(1) << 31) - 2, SPHGPU_ERR_CAPACITY, "neighbour grid too large");
+  double spanX = 0.0;
+  for (int k = 0; k < D; ++k) spanX = std::max(spanX, hb[3 + k] - hb[k]);
+  const int maxCells = D == 3 ? 1024 : (D == 2 ? 32768 : (1 << 26));
+  double c = Rmin * (1.0 + 1.0e-9);
+  for (int L = 0; L < 40; ++L) {
+    const bool top = c >= R * (1.0 + 1.0e-9);
+    double cell = c;
+    if (spanX / cell > maxCells - 2) cell = spanX / (maxCells - 2);
+    gridWithCell<D>(hb, cell, G, dg, bricks);
+    out.push_back(dg);
+    cells.push_back(G.ncells);
+    if (top) break;
+    c *= 2.0;
+  }
+}
+
+// Grid lists to the device; returns the total cell count.
+int64_t uploadGrids(Sim& s, const std::vector<DGrid>& grids, const std::vector<int64_t>& cells,
+                    const std::vector<int32_t>& first) {
+  hipStream_t st = s.stream;
+  const size_t ng = grids.size();
+  std::vector<int64_t> base(ng + 1, 0);
+  for (size_t q = 0; q < ng; ++q) base[q + 1] = base[q] + cells[q];
+  SG_REQUIRE(base.back() < (int64_t(1) << 31) - 2, SPHGPU_ERR_CAPACITY, "neighbour grid too large");
+  s.segGrid.ensure(sizeof(DGrid) * ng);
+  s.segCellBase.ensure(sizeof(int64_t) * (ng + 1));
+  s.gridFirst.ensure(sizeof(int32_t) * first.size());
+  SG_CHECK_HIP(hipMemcpyAsync(s.segGrid.p, grids.data(), sizeof(DGrid) * ng, hipMemcpyHostToDevice, st));
+  SG_CHECK_HIP(hipMemcpyAsync(s.segCellBase.p, base.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st));
+  SG_CHECK_HIP(hipMemcpyAsync(s.gridFirst.p, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice, st));
+  SG_CHECK_HIP(hipStreamSynchronize(st));   // the host vectors must outlive the copies
+  return base.back();
 }
 
 template <int D>
@@ -791,7 +885,7 @@ void buildNeighborsD(Sim& s, double ext) {
   int64_t bricks = 1;
   bool stagedPath = false;
   double hb[kNB];
-  SegGrids sgr{nullptr, nullptr, nullptr};
+  SegGrids sgr{nullptr, nullptr, nullptr, nullptr, ext};
   if (s.nSeg > 0) {
     const int ns = s.nSeg;
     s.segBoundsBits.ensure(sizeof(unsigned long long) * size_t(ns) * kNB);
@@ -803,34 +897,25 @@ void buildNeighborsD(Sim& s, double ext) {
     std::vector<unsigned long long> hbits(size_t(ns) * kNB);
     SG_CHECK_HIP(hipMemcpyAsync(hbits.data(), bits, hbits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
-    std::vector<DGrid> grids(static_cast<size_t>(ns));
-    std::vector<int64_t> base(size_t(ns) + 1, 0);
+    std::vector<DGrid> grids;
+    std::vector<int64_t> cells;
+    std::vector<int32_t> first(size_t(ns) + 1, 0);
     bool iso = true;
     for (int m = 0; m < ns; ++m) {
       double hm[kNB];
       for (int k = 0; k < kNB; ++k) hm[k] = fromOrderedBits(hbits[size_t(m) * kNB + k]);
       iso = iso && hm[7] > 0.5;
-      Grid Gm;
-      int64_t bm;
-      bool staged;
-      gridFromBounds<D>(hm, ext, false, Gm, grids[size_t(m)], bm, staged);
-      base[size_t(m) + 1] = base[size_t(m)] + Gm.ncells;
+      levelGrids<D>(hm, ext, grids, cells);
+      first[size_t(m) + 1] = int32_t(grids.size());
       if (m == 0) for (int k = 0; k < kNB; ++k) hb[k] = hm[k];
       hb[6] = std::min(hb[6], hm[6]);
       hb[8] = std::max(hb[8], hm[8]);
     }
-    SG_REQUIRE(base.back() < (int64_t(1) << 31) - 2, SPHGPU_ERR_CAPACITY, "neighbour grid too large");
-    s.segGrid.ensure(sizeof(DGrid) * size_t(ns));
-    s.segCellBase.ensure(sizeof(int64_t) * size_t(ns + 1));
-    SG_CHECK_HIP(hipMemcpyAsync(s.segGrid.p, grids.data(), sizeof(DGrid) * size_t(ns), hipMemcpyHostToDevice, st));
-    SG_CHECK_HIP(hipMemcpyAsync(s.segCellBase.p, base.data(), sizeof(int64_t) * size_t(ns + 1), hipMemcpyHostToDevice,
-                                st));
-    // the host vectors must outlive the copies
-    SG_CHECK_HIP(hipStreamSynchronize(st));
     s.isotropic = iso && !s.hasMultipliers;
     G = Grid{};
-    G.ncells = base.back();
-    sgr = SegGrids{s.seg.as<double>(), s.segGrid.as<DGrid>(), s.segCellBase.as<int64_t>()};
+    G.ncells = uploadGrids(s, grids, cells, first);
+    sgr = SegGrids{s.seg.as<double>(), s.segGrid.as<DGrid>(), s.segCellBase.as<int64_t>(),
+                   s.gridFirst.as<int32_t>(), ext};
   } else {
     const int nbB = int(std::min<int64_t>(1024, nblocks(n)));
     s.reduceTmp.ensure(sizeof(double) * (size_t(nbB) * kNB + 16));
@@ -842,7 +927,18 @@ void buildNeighborsD(Sim& s, double ext) {
     SG_CHECK_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
     s.isotropic = hb[7] > 0.5 && !s.hasMultipliers;
-    gridFromBounds<D>(hb, ext, s.isotropic, G, dg, bricks, stagedPath);
+    std::vector<DGrid> grids;
+    std::vector<int64_t> cells;
+    levelGrids<D>(hb, ext, grids, cells);
+    if (grids.size() == 1) {
+      gridFromBounds<D>(hb, ext, s.isotropic, G, dg, bricks, stagedPath);
+    } else {
+      // smoothing scales spanning more than a factor 2: levels
+      std::vector<int32_t> first = {0, int32_t(grids.size())};
+      G = Grid{};
+      G.ncells = uploadGrids(s, grids, cells, first);
+      sgr = SegGrids{nullptr, s.segGrid.as<DGrid>(), s.segCellBase.as<int64_t>(), s.gridFirst.as<int32_t>(), ext};
+    }
   }
 
   // 2. counting sort by cell, then by node index inside a cell
@@ -854,7 +950,8 @@ void buildNeighborsD(Sim& s, double ext) {
   s.iperm.ensure(sizeof(int32_t) * size_t(n));
   SG_CHECK_HIP(hipMemsetAsync(s.cellCount.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
   SG_CHECK_HIP(hipMemsetAsync(s.cellFill.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
-  hipLaunchKernelGGL((k_cell_count<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), n, dg, sgr,
+  hipLaunchKernelGGL((k_cell_count<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
+                     n, dg, sgr,
                      s.cellOf.as<int32_t>(), s.cellCount.as<int32_t>());
   exclusiveScan<int32_t, int32_t>(s.cellCount.as<int32_t>(), s.cellStart.as<int32_t>(), G.ncells, s);
   hipLaunchKernelGGL(k_cell_scatter, dim3(nblocks(n)), dim3(kBlock), 0, st, s.cellOf.as<int32_t>(),
@@ -869,6 +966,7 @@ void buildNeighborsD(Sim& s, double ext) {
   // 3. rows
   SearchArgs sa{};
   sa.g = dg;
+  sa.ext = ext;
   sa.ext2 = ext * ext;
   // FP32 classification band.  Offsets from the brick origin are at most 3R, so
   // |r| carries an absolute error below 7.2e-7 R; near the threshold of the
@@ -950,10 +1048,7 @@ void buildNeighbors(Sim& s, double ext) {
   if (s.dim == 1) buildNeighborsD<1>(s, ext);
   else if (s.dim == 2) buildNeighborsD<2>(s, ext);
   else buildNeighborsD<3>(s, ext);
-  float ms = 0.f;
-  SG_CHECK_HIP(hipEventSynchronize(s.ev[1]));
-  SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[0], s.ev[1]));
-  s.timing.neighbors_ms = ms;
+  s.timingPending |= 1u;
   s.neighborsValid = true;
 }
 

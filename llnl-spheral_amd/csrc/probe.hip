@@ -23,14 +23,14 @@ double copyProbe(Sim& s, int64_t bytes, int repeats) {
   SG_CHECK_HIP(hipMemsetAsync(a.p, 0, size_t(bytes), s.stream));
   const int grid = int(std::min<int64_t>((n + 255) / 256, 256 * 64));
   hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s.stream, a.as<double4>(), b.as<double4>(), n);
-  SG_CHECK_HIP(hipEventRecord(s.ev[6], s.stream));
+  SG_CHECK_HIP(hipEventRecord(s.ev[12], s.stream));
   for (int r = 0; r < repeats; ++r)
     hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s.stream, a.as<double4>(), b.as<double4>(), n);
   SG_CHECK_HIP(hipGetLastError());
-  SG_CHECK_HIP(hipEventRecord(s.ev[7], s.stream));
-  SG_CHECK_HIP(hipEventSynchronize(s.ev[7]));
+  SG_CHECK_HIP(hipEventRecord(s.ev[13], s.stream));
+  SG_CHECK_HIP(hipEventSynchronize(s.ev[13]));
   float ms = 0.f;
-  SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[6], s.ev[7]));
+  SG_CHECK_HIP(hipEventElapsedTime(&ms, s.ev[12], s.ev[13]));
   a.release();
   b.release();
   return double(ms) / repeats;

@@ -52,6 +52,14 @@ def cases():
     st["H"] = st["H"].copy()
     st["H"][5] *= 0.25
     out.append(("bighalo3d", 3, st, meta["n_internal"], {}))
+    # smoothing scales spread over a factor ~12 (a rarefied blast core): the
+    # multi-level search grids
+    for dim, n in ((1, 160), (2, 28), (3, 9)):
+        st, meta = problems.random_state(dim, n, seed=40 + dim, jitter=0.1)
+        r = np.sqrt(((st["pos"] - 0.5) ** 2).sum(1))
+        scale = 1.0 + 11.0 * np.exp(-(r / 0.12) ** 2)
+        st["H"] = st["H"] / scale[:, None]
+        out.append((f"levels{dim}d", dim, st, meta["n_internal"], {}))
     return out
 
 
