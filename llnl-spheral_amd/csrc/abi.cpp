@@ -66,16 +66,17 @@ void installTable(Sim& s) {
   s.dTable.ensure(sizeof(double) * inter.size());
   s.dNperh.ensure(sizeof(double) * n);
   s.dWsum.ensure(sizeof(double) * n);
-  SG_CHECK_HIP(hipMemcpy(s.dTable.p, inter.data(), sizeof(double) * inter.size(), hipMemcpyHostToDevice));
+  SG_CHECK_HIP(hipMemcpyAsync(s.dTable.p, inter.data(), sizeof(double) * inter.size(), hipMemcpyHostToDevice, s.stream));
   std::vector<float> t32(size_t(8) * n, 0.0f);
   for (int i = 0; i < n; ++i) {
     t32[8 * i + 0] = float(s.table.Ak[i]); t32[8 * i + 1] = float(s.table.Bk[i]); t32[8 * i + 2] = float(s.table.Ck[i]);
     t32[8 * i + 4] = float(s.table.Ag[i]); t32[8 * i + 5] = float(s.table.Bg[i]); t32[8 * i + 6] = float(s.table.Cg[i]);
   }
   s.dTable32.ensure(sizeof(float) * t32.size());
-  SG_CHECK_HIP(hipMemcpy(s.dTable32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
-  SG_CHECK_HIP(hipMemcpy(s.dNperh.p, s.table.nperh.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  SG_CHECK_HIP(hipMemcpy(s.dWsum.p, s.table.wsum.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  SG_CHECK_HIP(hipMemcpyAsync(s.dTable32.p, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice, s.stream));
+  SG_CHECK_HIP(hipMemcpyAsync(s.dNperh.p, s.table.nperh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
+  SG_CHECK_HIP(hipMemcpyAsync(s.dWsum.p, s.table.wsum.data(), sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));   // the host vectors die on return
 }
 
 }  // namespace
@@ -203,7 +204,8 @@ int sphgpu_set_nodes(sphgpu_sim* s, int64_t nInt, int64_t nTot) {
     for (DBuf* b : scal) b->ensure(sizeof(double) * n);
     // omega defaults to 1 (no grad-h correction)
     std::vector<double> ones(n, 1.0);
-    SG_CHECK_HIP(hipMemcpy(s->omega.p, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    SG_CHECK_HIP(hipMemcpyAsync(s->omega.p, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));   // `ones` dies on return
   });
 }
 
@@ -221,7 +223,7 @@ int sphgpu_upload(sphgpu_sim* s, int field, const double* host, int64_t nRows) {
         std::vector<double> ones(std::max<size_t>(s->pos.bytes / (sizeof(double) * size_t(s->dim)), 1), 1.0);
         for (DBuf* m : {&s->fCl, &s->fCq, &s->fshear}) {
           m->ensure(sizeof(double) * ones.size());
-          SG_CHECK_HIP(hipMemcpy(m->p, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
+          SG_CHECK_HIP(hipMemcpyAsync(m->p, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice, s->stream));
         }
         s->hasMultipliers = true;
       }
@@ -375,10 +377,11 @@ int sphgpu_download_neighbors(sphgpu_sim* s, int64_t* offsets, int32_t* nbr) {
     std::vector<int32_t> cnt(static_cast<size_t>(n)), perm(static_cast<size_t>(n));
     std::vector<int32_t> iperm(static_cast<size_t>(s->nTot));
     std::vector<int32_t> ell(static_cast<size_t>(n) * cap);
-    SG_CHECK_HIP(hipMemcpy(cnt.data(), s->rowCount.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
-    SG_CHECK_HIP(hipMemcpy(perm.data(), s->perm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost));
-    SG_CHECK_HIP(hipMemcpy(iperm.data(), s->iperm.p, sizeof(int32_t) * iperm.size(), hipMemcpyDeviceToHost));
-    if (!ell.empty()) SG_CHECK_HIP(hipMemcpy(ell.data(), s->nbr.p, sizeof(int32_t) * ell.size(), hipMemcpyDeviceToHost));
+    SG_CHECK_HIP(hipMemcpyAsync(cnt.data(), s->rowCount.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipMemcpyAsync(perm.data(), s->perm.p, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipMemcpyAsync(iperm.data(), s->iperm.p, sizeof(int32_t) * iperm.size(), hipMemcpyDeviceToHost, s->stream));
+    if (!ell.empty()) SG_CHECK_HIP(hipMemcpyAsync(ell.data(), s->nbr.p, sizeof(int32_t) * ell.size(), hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
     int64_t w = 0;
     for (int64_t i = 0; i < s->nInt; ++i) {
       offsets[i] = w;
@@ -423,8 +426,9 @@ int sphgpu_download_derivative(sphgpu_sim* s, int field, double* host, int64_t n
     std::vector<double> block(static_cast<size_t>(n) * comps);
     std::vector<int32_t> iperm(static_cast<size_t>(s->nTot));
     const double* src = s->out.as<double>() + int64_t(derivOffsetDoubles(s->dim, field)) * n;
-    SG_CHECK_HIP(hipMemcpy(block.data(), src, sizeof(double) * block.size(), hipMemcpyDeviceToHost));
-    SG_CHECK_HIP(hipMemcpy(iperm.data(), s->iperm.p, sizeof(int32_t) * iperm.size(), hipMemcpyDeviceToHost));
+    SG_CHECK_HIP(hipMemcpyAsync(block.data(), src, sizeof(double) * block.size(), hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipMemcpyAsync(iperm.data(), s->iperm.p, sizeof(int32_t) * iperm.size(), hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
     for (int64_t i = 0; i < nRows; ++i) {
       const int64_t k = iperm[size_t(i)];
       std::memcpy(host + i * comps, block.data() + k * comps, sizeof(double) * comps);
