@@ -273,94 +273,6 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
   ri_[R::MOR] = mi / acc;
 }
 
-// Isotropic H: the same sum with the neighbour records fetched a group of
-// four ahead (one int4 of row indices per group), so each lane keeps up to
-// eight record loads in flight while it adds the current group in row order.
-struct DIso {
-  double x[3], hinv, hdet, m, orig;
-};
-template <int D> __device__ __forceinline__ DIso loadDIso(const double* __restrict__ rec, int32_t j) {
-  using R = RL<true>;
-  const double* r = rec + int64_t(j) * R::STRIDE;
-  DIso o;
-  const double4 a = *reinterpret_cast<const double4*>(r);          // x y z 1/h
-  const double4 b = *reinterpret_cast<const double4*>(r + 4);      // m det H orig rho
-  o.x[0] = a.x; o.x[1] = a.y; o.x[2] = a.z; o.hinv = a.w;
-  o.m = b.x; o.hdet = b.y; o.orig = b.z;
-  return o;
-}
-
-template <int D, bool LDS>
-__global__ void __launch_bounds__(kBlock) k_sum_density_iso(double* __restrict__ rec, const int32_t* __restrict__ rowCount,
-                                                            const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
-                                                            const int32_t* __restrict__ work, TabArgs t,
-                                                            double* __restrict__ rhoOut) {
-  using R = RL<true>;
-  extern __shared__ double lds[];
-  const double* tab = stageTable<LDS>(lds, t);
-  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
-  if (tw >= nInt) return;
-  const int64_t k = work[tw];
-  double* ri_ = rec + k * R::STRIDE;
-  const DIso me = loadDIso<D>(rec, int32_t(k));
-  const double oi = me.orig;
-  const double mi = me.m;
-  const double self = mi * lookupW(tab, t, 0.0, me.hdet);
-  double acc = 0.0;
-  bool selfDone = false;
-  const int len = rowCount[k];
-  const int4* nb4 = reinterpret_cast<const int4*>(nbr);
-  const int ng = (len + 3) >> 2;
-  const int32_t kk = int32_t(k);
-  // entries past the row end read the node's own record (always valid)
-  auto fix = [&](int4 c, int g) {
-    const int m4 = len - 4 * g;
-    if (m4 < 4) c.w = kk;
-    if (m4 < 3) c.z = kk;
-    if (m4 < 2) c.y = kk;
-    return c;
-  };
-  int4 c0 = ng > 0 ? fix(nb4[ellChunk(0, n, k)], 0) : make_int4(kk, kk, kk, kk);
-  int4 c1 = ng > 1 ? fix(nb4[ellChunk(1, n, k)], 1) : make_int4(kk, kk, kk, kk);
-  DIso a0 = loadDIso<D>(rec, c0.x), a1 = loadDIso<D>(rec, c0.y), a2 = loadDIso<D>(rec, c0.z), a3 = loadDIso<D>(rec, c0.w);
-  for (int g = 0; g < ng; ++g) {
-    // next group's records, and the row indices of the one after
-    const int4 c2 = g + 2 < ng ? fix(nb4[ellChunk(g + 2, n, k)], g + 2) : make_int4(kk, kk, kk, kk);
-    DIso b0, b1, b2, b3;
-    if (g + 1 < ng) {
-      b0 = loadDIso<D>(rec, c1.x); b1 = loadDIso<D>(rec, c1.y); b2 = loadDIso<D>(rec, c1.z); b3 = loadDIso<D>(rec, c1.w);
-    }
-    const int m4 = len - 4 * g;
-    const DIso* q[4] = {&a0, &a1, &a2, &a3};
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (u < m4) {
-        const DIso& nb = *q[u];
-        if (!selfDone && nb.orig > oi) { acc += self; selfDone = true; }
-        double e2;
-        {
-          // |H_j r_ij| with H_j = h_j I: the SymTensor product row sums, magnitude
-          const double r0 = me.x[0] - nb.x[0];
-          const double h0 = nb.hinv * r0;
-          e2 = h0 * h0;
-          if constexpr (D >= 2) { const double r1 = me.x[1] - nb.x[1]; const double h1 = nb.hinv * r1; e2 = e2 + h1 * h1; }
-          if constexpr (D >= 3) { const double r2 = me.x[2] - nb.x[2]; const double h2 = nb.hinv * r2; e2 = e2 + h2 * h2; }
-        }
-        const double etaj = (D == 1) ? fabs(nb.hinv * (me.x[0] - nb.x[0])) : __builtin_sqrt(e2);
-        acc += nb.m * lookupW(tab, t, etaj, nb.hdet);
-      }
-    }
-    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-    c1 = c2;
-  }
-  if (!selfDone) acc += self;
-  rhoOut[int64_t(oi)] = acc;
-  const double som = ri_[R::SOM], p = ri_[R::P];
-  ri_[R::RHO] = acc;
-  ri_[R::PRHO] = som * p / (acc * acc);
-  ri_[R::MOR] = mi / acc;
-}
-
 // ---------------------------------------------------------------------------
 // computeSPHOmegaGradhCorrection (src/SPH/computeSPHOmegaGradhCorrection.cc:26-133)
 // Gather form of the owner loop: node i receives W(|H_i r_ij|) det H_i and
@@ -1102,7 +1014,7 @@ void sumDensityD(Sim& s) {
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
                        s.rho.as<double>());
   };
-  if (s.isotropic) { if (lds) go(k_sum_density_iso<D, true>); else go(k_sum_density_iso<D, false>); }
+  if (s.isotropic) { if (lds) go(k_sum_density<D, true, true>); else go(k_sum_density<D, true, false>); }
   else { if (lds) go(k_sum_density<D, false, true>); else go(k_sum_density<D, false, false>); }
   SG_CHECK_HIP(hipGetLastError());
   // SPHHydroBase::finalize then applies the boundaries to the mass density
