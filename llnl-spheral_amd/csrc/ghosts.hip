@@ -149,8 +149,29 @@ __global__ void k_hinv_min(const double* __restrict__ H, int64_t n, double* __re
   if (i < n) hmaxOf[i] = 1.0 / minEigen<D>(H + i * NS);
 }
 
-// hmax of the nodes near either plane; per ensemble member when seg is given
-// (hmaxBits[member]), else one value.
+// hmax of the nodes near either plane, one value: grid-stride loop, a block
+// maximum and one 64-bit atomic max per block.
+template <int D>
+__global__ void k_ghost_hmax_one(const double* __restrict__ pos, const double* __restrict__ hmaxOf, int64_t n, DSub b,
+                                 double ext, unsigned long long* __restrict__ hmaxBits) {
+  __shared__ double wmax[kBlock / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const V<D> x = loadV<D>(pos, i);
+    const double hmaxi = hmaxOf[i];
+    const double d = dmin(dabs(signedDist<D>(b.exitP, b.exitN, x)), dabs(signedDist<D>(b.enterP, b.enterN, x)));
+    if (hmaxi > 0.0 && d < ext * hmaxi) v = dmax(v, hmaxi);
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) v = dmax(v, __shfl_xor(v, sh));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) v = dmax(v, wmax[w]);
+    if (v > 0.0) atomicMax(hmaxBits, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+// hmax of the nodes near either plane per ensemble member (hmaxBits[member]).
 template <int D>
 __global__ void k_ghost_hmax(const double* __restrict__ pos, const double* __restrict__ hmaxOf,
                              const double* __restrict__ seg, int64_t n, DSub b, double ext,
@@ -363,10 +384,12 @@ __global__ void k_keep_controls(int32_t* __restrict__ keep, int64_t nInt, int64_
   if (c >= nInt) keep[c] = 1;
 }
 
-// Compact one caller-order field of `comps` doubles per row.
+// Compact the ghost rows [i0, n) of one caller-order field of `comps`
+// doubles per row (internal rows keep their place).
 __global__ void k_compact_rows(const double* __restrict__ src, double* __restrict__ dst, int comps,
-                               const int32_t* __restrict__ keep, const int32_t* __restrict__ scan, int64_t n) {
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+                               const int32_t* __restrict__ keep, const int32_t* __restrict__ scan, int64_t i0,
+                               int64_t n) {
+  const int64_t i = i0 + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n || !keep[i]) return;
   const int64_t o = scan[i];
   for (int c = 0; c < comps; ++c) dst[o * comps + c] = src[i * comps + c];
@@ -384,12 +407,18 @@ __global__ void k_compact_ghosts(const int32_t* __restrict__ control, const int3
 }
 
 // Slots keep their place; their caller index is renumbered, culled ones die.
+// The records (when current) take the new caller index; nothing else in a
+// record depends on the numbering, so they stay valid.
 __global__ void k_perm_remap(int32_t* __restrict__ perm, const int32_t* __restrict__ keep,
-                             const int32_t* __restrict__ scan, int64_t nSlots) {
+                             const int32_t* __restrict__ scan, int64_t nSlots, double* __restrict__ rec, int stride,
+                             int origOff) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (k >= nSlots) return;
   const int32_t c = perm[k];
-  if (c >= 0) perm[k] = keep[c] ? scan[c] : -1;
+  if (c < 0) return;
+  const int32_t nc = keep[c] ? scan[c] : -1;
+  perm[k] = nc;
+  if (rec) rec[k * stride + origOff] = nc >= 0 ? double(nc) : 2147483648.0;
 }
 
 __global__ void k_iperm_remap(const int32_t* __restrict__ iperm, const int32_t* __restrict__ keep,
@@ -469,8 +498,12 @@ template <int D> void generateD(Sim& s, double ext) {
     s.keepScan.ensure(sizeof(int32_t) * size_t(n + 1));
     const double* segp = s.nSeg > 0 ? s.seg.as<double>() : nullptr;
     SG_CHECK_HIP(hipMemsetAsync(hbits, 0, sizeof(unsigned long long) * size_t(nh), st));
-    hipLaunchKernelGGL(k_ghost_hmax<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
-                       s.hmaxOf.as<double>(), segp, n, d, ext, hbits);
+    if (segp)
+      hipLaunchKernelGGL(k_ghost_hmax<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
+                         s.hmaxOf.as<double>(), segp, n, d, ext, hbits);
+    else
+      hipLaunchKernelGGL(k_ghost_hmax_one<D>, dim3(std::min(nblocks(n), 1024)), dim3(kBlock), 0, st,
+                         s.pos.as<double>(), s.hmaxOf.as<double>(), n, d, ext, hbits);
     hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), segp, n, d, ext,
                        hbits, s.keep.as<int32_t>());
     exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), n, s);
@@ -519,16 +552,19 @@ void cull(Sim& s) {
   if (nNew == n) return;
   const int32_t* scan = s.keepScan.as<int32_t>();
 
-  // caller-order fields
+  // caller-order fields: kept ghost rows compacted through the spare and
+  // copied back (internal rows stay where they are)
   auto bufs = stateBufs(s);
   for (int w = 0; w < int(bufs.size()); ++w) {
     if (!stateUsed(s, w)) continue;
     const int comps = stateComps(s, w);
     DBuf& spare = s.spare[w];
     spare.ensure(bufs[size_t(w)]->bytes);
-    hipLaunchKernelGGL(k_compact_rows, dim3(nblocks(n)), dim3(kBlock), 0, st, bufs[size_t(w)]->as<double>(),
-                       spare.as<double>(), comps, keep, scan, n);
-    bufs[size_t(w)]->swap(spare);
+    hipLaunchKernelGGL(k_compact_rows, dim3(nblocks(n - nInt)), dim3(kBlock), 0, st, bufs[size_t(w)]->as<double>(),
+                       spare.as<double>(), comps, keep, scan, nInt, n);
+    if (nNew > nInt)
+      SG_CHECK_HIP(hipMemcpyAsync(bufs[size_t(w)]->as<double>() + nInt * comps, spare.as<double>() + nInt * comps,
+                                  sizeof(double) * size_t(nNew - nInt) * comps, hipMemcpyDeviceToDevice, st));
   }
   // ghost controls (control and sub-boundary arrays share the spare pair)
   s.ghostControlSpare.ensure(s.ghostControl.bytes);
@@ -539,8 +575,10 @@ void cull(Sim& s) {
   s.ghostControl.swap(s.ghostControlSpare);
   s.ghostSub.swap(s.ghostSubSpare);
   // slot <-> caller maps
+  const bool recs = s.recValid;
   hipLaunchKernelGGL(k_perm_remap, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), keep, scan,
-                     s.nSlots);
+                     s.nSlots, recs ? s.rec.as<double>() : nullptr, s.isotropic ? RL<true>::STRIDE : RL<false>::STRIDE,
+                     s.isotropic ? RL<true>::ORIG : RL<false>::ORIG);
   s.ipermSpare.ensure(s.iperm.bytes);
   hipLaunchKernelGGL(k_iperm_remap, dim3(nblocks(n)), dim3(kBlock), 0, st, s.iperm.as<int32_t>(), keep, scan, n,
                      s.ipermSpare.as<int32_t>());
@@ -548,7 +586,7 @@ void cull(Sim& s) {
   SG_CHECK_HIP(hipGetLastError());
   for (size_t b = 0; b < s.subStart.size(); ++b) s.subStart[b] = int64_t(starts[b]) - nInt;
   s.nTot = nNew;
-  s.recValid = false;
+  s.recValid = recs;
   s.derivsValid = false;
 }
 

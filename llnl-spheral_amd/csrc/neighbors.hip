@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
   using R = RL<true>;
   __shared__ int32_t sOff[kStaged + 1];
   __shared__ int32_t sStart[kStaged];
-  __shared__ float4 sCand[kCandCap];               // x,y,z offsets from the brick origin (FP32), 1/h
+  __shared__ float4 sCand[kCandCap];               // x,y,z offsets from the brick origin, (1/h)^2 (FP32)
   __shared__ int32_t sSlot[kCandCap];
   __shared__ int32_t sc[kBlock];
   const DGrid& g = a.g;
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
       const int32_t o = sOff[t], cnt = sOff[t + 1] - o, st = sStart[t];
       for (int q = 0; q < cnt; ++q) {
         const double4 r = *reinterpret_cast<const double4*>(rec + int64_t(st + q) * R::STRIDE);
-        sCand[o + q] = make_float4(float(r.x - ox), float(r.y - oy), float(r.z - oz), float(r.w));
+        sCand[o + q] = make_float4(float(r.x - ox), float(r.y - oy), float(r.z - oz), float(r.w) * float(r.w));
         // bit 31 flags a ghost (slots stay below 2^31)
         const bool ghost = rec[int64_t(st + q) * R::STRIDE + R::ORIG] >= double(a.nInt);
         sSlot[o + q] = int32_t(uint32_t(st + q) | (ghost ? 0x80000000u : 0u));
@@ -388,84 +388,16 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
   }
   __syncthreads();
 
-  if (staged) {
-    // One wavefront per node: the node's 25 (dz, dy) runs of 5 staged cells are
-    // contiguous LDS segments; a wavefront prefix sum flattens them, each lane
-    // tests one candidate per round, and a ballot compacts the accepted ones
-    // into the row in candidate order.
-    const int lane = threadIdx.x & 63;
-    int32_t* nbrI = reinterpret_cast<int32_t*>(nbr);
-    for (int64_t k = h0 + (threadIdx.x >> 6); k < h1; k += kBlock / 64) {
-      const double* ri = rec + k * R::STRIDE;
-      if (int64_t(ri[R::ORIG]) >= a.nInt) {
-        if (lane == 0) rowCount[k] = 0;
-        continue;
-      }
-      const double xi = ri[R::X], yi = ri[R::X + 1], zi = ri[R::X + 2], hi = ri[R::HINV];
-      const float fx = float(xi - ox), fy = float(yi - oy), fz = float(zi - oz);
-      const float hi2 = float(hi) * float(hi);
-      int c[3];
-      const double xv[3] = {xi, yi, zi};
-      cellCoords<3>(g, xv, c);
-      const int sx = c[0] - cx0, sy = c[1] - cy0, sz = c[2] - cz0;   // in [2, 5]
-      int32_t segLo = 0, segLen = 0;
-      if (lane < 25) {
-        const int base = ((sz + lane / 5 - 2) * kStageEdge + (sy + lane % 5 - 2)) * kStageEdge + (sx - 2);
-        segLo = sOff[base];
-        segLen = sOff[base + 5] - segLo;
-      }
-      int32_t incl = segLen;
-      for (int d = 1; d < 64; d <<= 1) {
-        const int32_t u = __shfl_up(incl, d);
-        if (lane >= d) incl += u;
-      }
-      const int32_t excl = incl - segLen;
-      const int32_t total = __shfl(incl, 24);
-      int cnt = 0;
-      for (int32_t v0 = 0; v0 < total; v0 += 64) {
-        const int32_t v = v0 + lane;
-        int lo = 0, up = 24;                       // last segment starting at or before v
-        for (int step = 0; step < 5; ++step) {
-          const int mid = (lo + up + 1) >> 1;
-          if (__shfl(excl, mid) <= v) lo = mid; else up = mid - 1;
-        }
-        const int32_t t = __shfl(segLo, lo) + (v - __shfl(excl, lo));
-        bool in = false;
-        int32_t raw = 0, sj = 0;
-        if (v < total) {
-          const float4 cj = sCand[t];
-          const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
-          const float q = rx * rx + ry * ry + rz * rz;
-          const float ti = hi2 * q, tj = (cj.w * cj.w) * q;
-          const float tmin = ti < tj ? ti : tj;
-          raw = sSlot[t];
-          sj = raw & 0x7fffffff;
-          in = tmin < a.lo32;
-          if (!in && tmin <= a.hi32) {
-            const double* rj = rec + int64_t(sj) * R::STRIDE;
-            in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
-          }
-          in = in && sj != int32_t(k);
-        }
-        const unsigned long long m = __ballot(in);
-        if (in) {
-          const int w = cnt + __popcll(m & ((1ull << lane) - 1ull));
-          if (w < a.cap) nbrI[ellIndex(w, a.n, k)] = sj;
-          if (raw < 0 && a.keepSlot) a.keepSlot[sj] = 1;
-        }
-        cnt += __popcll(m);
-      }
-      if (lane == 0) rowCount[k] = cnt;
-    }
-    return;
-  }
-
-  // Bricks whose candidates exceed the stage: one lane per node, candidates
-  // read from the records in global memory.
+  // One lane per node; the node's 25 (dz, dy) runs of 5 staged cells are
+  // contiguous candidate ranges in LDS.  Rows leave as int4 chunks, one per
+  // lane, contiguous across a wavefront's slots.  Bricks whose candidates
+  // exceed the stage read them from the records in global memory.
   for (int64_t k = h0 + threadIdx.x; k < h1; k += blockDim.x) {
     const double* ri = rec + k * R::STRIDE;
     if (int64_t(ri[R::ORIG]) >= a.nInt) { rowCount[k] = 0; continue; }
     const double xi = ri[R::X], yi = ri[R::X + 1], zi = ri[R::X + 2], hi = ri[R::HINV];
+    const float fx = float(xi - ox), fy = float(yi - oy), fz = float(zi - oz);
+    const float hi2 = float(hi) * float(hi);
     int c[3];
     const double xv[3] = {xi, yi, zi};
     cellCoords<3>(g, xv, c);
@@ -474,16 +406,41 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
     for (int dz = -2; dz <= 2; ++dz) {
       for (int dy = -2; dy <= 2; ++dy) {
         const int base = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge + (sx - 2);
-        for (int dx = 0; dx < 5; ++dx) {
-          const int cidx = base + dx;
-          const int32_t st = sStart[cidx], ncell = sOff[cidx + 1] - sOff[cidx];
-          for (int32_t q2 = 0; q2 < ncell; ++q2) {
-            const int32_t sj = st + q2;
-            if (sj == int32_t(k)) continue;
-            const double* rj = rec + int64_t(sj) * R::STRIDE;
-            if (exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2)) {
-              row.push(sj);
-              if (a.keepSlot && rj[R::ORIG] >= double(a.nInt)) a.keepSlot[sj] = 1;
+        if (staged) {
+          const int32_t t0 = sOff[base], t1 = sOff[base + 5];
+          for (int32_t t = t0; t < t1; ++t) {
+            const float4 cj = sCand[t];
+            const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
+            const float q = rx * rx + ry * ry + rz * rz;
+            const float ti = hi2 * q, tj = cj.w * q;
+            const float tmin = ti < tj ? ti : tj;
+            bool in = tmin < a.lo32;
+            if (!in && tmin <= a.hi32) {
+              const int32_t sj = sSlot[t] & 0x7fffffff;
+              const double* rj = rec + int64_t(sj) * R::STRIDE;
+              in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
+            }
+            if (in) {
+              const int32_t raw = sSlot[t];
+              const int32_t sj = raw & 0x7fffffff;
+              if (sj != int32_t(k)) {
+                row.push(sj);
+                if (raw < 0 && a.keepSlot) a.keepSlot[sj] = 1;
+              }
+            }
+          }
+        } else {
+          for (int dx = 0; dx < 5; ++dx) {
+            const int cidx = base + dx;
+            const int32_t st = sStart[cidx], ncell = sOff[cidx + 1] - sOff[cidx];
+            for (int32_t q2 = 0; q2 < ncell; ++q2) {
+              const int32_t sj = st + q2;
+              if (sj == int32_t(k)) continue;
+              const double* rj = rec + int64_t(sj) * R::STRIDE;
+              if (exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2)) {
+                row.push(sj);
+                if (a.keepSlot && rj[R::ORIG] >= double(a.nInt)) a.keepSlot[sj] = 1;
+              }
             }
           }
         }
