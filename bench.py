@@ -256,6 +256,80 @@ def serve_bench(sphgpu, problems, count, rate, precision, check):
     return res
 
 
+def periodic_bench(sphgpu, problems, n, steps):
+    """Periodic box (density 1 + 0.1 sin sin sin, Gaussian random velocity
+    sigma 0.1 with a k^-2 spectrum, gamma 1.4, u = 1): mixed precision against
+    the FP64 path, after one pass (sum density + evaluateDerivatives on the
+    same state) and after `steps` SynchronousRK2 steps, plus the drift of
+    total momentum and energy of both."""
+    st, meta = problems.periodic_sinusoid(n)
+    N = st["mass"].shape[0]
+    print(f"[bench] periodic {n}^3: {N} nodes, {steps} steps", file=sys.stderr, flush=True)
+    m = st["mass"]
+
+    def totals(v, eps):
+        p = (m[:, None] * v).sum(0)
+        return p, float((0.5 * m * (v * v).sum(1) + m * eps).sum())
+
+    def rel(a, b):
+        a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+        scale = max(np.abs(b).max(), 1e-300)
+        return {"linf": float(np.abs(a - b).max() / scale),
+                "l2": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300 * np.sqrt(a.size)))}
+
+    out = {}
+    for prec_name in ("fp64", "mixed"):
+        prec = sphgpu.PRECISION_FP64 if prec_name == "fp64" else sphgpu.PRECISION_MIXED
+        s = sphgpu.Sim(3)
+        s.set_row_order(sphgpu.ROW_ORDER_SEARCH)
+        s.set_table_bspline(1000)
+        s.set_precision(prec)
+        s.set_nodes(N, N)
+        s.upload_state({k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")})
+        s.set_boundaries(sphgpu.box_boundaries(3, kind="periodic"))
+        s.eos_gamma_law(1.4)
+        hp = sphgpu.hydro_params(precision=prec)
+        ip = sphgpu.integrator_params(gamma=1.4)
+        # one pass on the initial state
+        s.set_ghost_nodes(2.0)
+        s.sum_density()
+        s.evaluate_derivatives(hp)
+        pas = {"rho": s.download("mass density", rows=N), "DvDt": s.derivative("DvDt"),
+               "DepsDt": s.derivative("DepsDt")}
+        s.upload("mass density", st["rho"])
+        s.eos_gamma_law(1.4)
+        p0, e0 = totals(st["vel"], st["eps"])
+        t0 = time.perf_counter()
+        drift = []
+        for k in range(steps):
+            s.rk2_step(hp, ip)
+            if (k + 1) % max(1, steps // 10) == 0 or k + 1 == steps:
+                v = s.download("velocity", rows=N).reshape(N, 3)
+                eps = s.download("specific thermal energy", rows=N)
+                p, e = totals(v, eps)
+                drift.append({"step": k + 1, "momentum": [float(x) for x in (p - p0)], "energy_rel": (e - e0) / e0})
+                print(f"[bench] periodic {prec_name} step {k + 1}", file=sys.stderr, flush=True)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        fin = {"rho": s.download("mass density", rows=N), "eps": s.download("specific thermal energy", rows=N),
+               "vel": s.download("velocity", rows=N)}
+        mom_scale = float((m[:, None] * np.abs(st["vel"])).sum())
+        out[prec_name] = {"pass": pas, "final": fin, "wall_s": wall, "drift": drift, "ms_per_step": 1e3 * wall / steps,
+                          "momentum_scale": mom_scale}
+        s.close()
+    res = {"workload": f"periodic box {n}^3 ({N} nodes): density 1+0.1 sin(2pi x)sin(2pi y)sin(2pi z), Gaussian random "
+                       "velocity sigma 0.1 (k^-2), gamma 1.4, u 1; 3 PeriodicBoundary pairs; search row order",
+           "steps": steps,
+           "pass_error_mixed_vs_fp64": {k: rel(out["mixed"]["pass"][k], out["fp64"]["pass"][k])
+                                        for k in ("rho", "DvDt", "DepsDt")},
+           "final_error_mixed_vs_fp64": {k: rel(out["mixed"]["final"][k], out["fp64"]["final"][k])
+                                         for k in ("rho", "eps", "vel")}}
+    for p in ("fp64", "mixed"):
+        res[p] = {"ms_per_step": out[p]["ms_per_step"], "wall_s": out[p]["wall_s"],
+                  "momentum_scale": out[p]["momentum_scale"], "drift": out[p]["drift"]}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +357,9 @@ def main():
     ap.add_argument("--serve-check", type=int, default=3, help="requests re-solved alone in FP64 for the error")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--only-batch", action="store_true", help="skip the 128^3 pass measurement")
+    ap.add_argument("--periodic", type=int, default=0,
+                    help="also run the periodic mixed-vs-FP64 comparison at this lattice size (256 = the benchmark)")
+    ap.add_argument("--periodic-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -294,6 +371,8 @@ def main():
 
     if args.only_batch:
         line = {"metric": METRIC, "n_gpus": 1}
+        if args.periodic:
+            line["periodic"] = periodic_bench(sphgpu, problems, args.periodic, args.periodic_steps)
         if args.ensemble:
             line["ensemble"] = ensemble_bench(sphgpu, problems, args.ensemble, args.batch_precision,
                                               args.ensemble_sample)
@@ -350,7 +429,12 @@ def main():
     achieved = alg_bytes / (derivs_ms * 1e-3) / 1e9
     counters, counters_src = measured_counters(args.n, args.precision, args.row_order, args.boundaries)
     traffic = counters["traffic_bytes"] if counters else None
-    kernel = "k_derivs_fp64" if args.precision == "fp64" else "k_derivs_mixed"
+    if counters and counters.get("kernel"):
+        kernel = counters["kernel"]
+    elif args.precision == "mixed":
+        kernel = "k_derivs_iso<float, 3, true>"
+    else:
+        kernel = "k_derivs_iso<double, 3, true>" if args.row_order == "search" else "k_derivs_fp64<3, true, false, true>"
     line = {
         "metric": METRIC,
         "value": pairs * args.steps / wall,
@@ -382,7 +466,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": counters_src,
             "algorithmic_bytes": alg_bytes,
-            "note": "the pair kernel is VALU-issue bound, not HBM bound: see valu",
+            "note": "the pair kernel is bound by the latency of its neighbour gathers at 2-3 waves/SIMD, "
+                    "not by HBM bandwidth: see valu for the issue-time fraction",
         },
     }
     if counters and counters.get("valu_wave_instructions"):

@@ -209,3 +209,26 @@ def test_compatible_energy_needs_fp64(sphgpu):
     s.set_ghost_nodes(2.0)
     with pytest.raises(sphgpu.SphGpuError):
         s.evaluate_derivatives(sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED, compatibleEnergyEvolution=1))
+
+
+def test_config1_converging_32cube_rk2(sphgpu, oracle):
+    """The CPU-runnable benchmark configuration: 32^3 converging flow in a
+    reflecting box, 20 SynchronousRK2 steps on the device and in the oracle
+    (reference row order), same dt sequence and state to 1e-12."""
+    import integrator
+    st, meta = problems.converging_flow(32)
+    N = st["pos"].shape[0]
+    print(f"[rk2] config1 32^3 N={N}", flush=True)
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(3, base, oracle.box_boundaries(3), 5.0 / 3.0)
+    s = device(sphgpu, 3, st, "reflecting", 5.0 / 3.0)
+    hp = sphgpu.hydro_params()
+    ip = sphgpu.integrator_params(gamma=5.0 / 3.0)
+    for k in range(20):
+        dref = ref.step()
+        ddev = s.rk2_step(hp, ip)
+        assert ddev == pytest.approx(dref, rel=1e-12), (k, ddev, dref)
+    for key, f in FIELDS.items():
+        a = s.download(f, rows=N).reshape(ref.st[key].shape)
+        b = ref.st[key]
+        assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), key
