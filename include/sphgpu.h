@@ -274,6 +274,17 @@ int sphgpu_hydro_dt(sphgpu_sim* sim, double nPerh, double cfl, int useVelocityMa
  * Advances the sim's time; returns the step taken in *dt. */
 int sphgpu_rk2_step(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, const sphgpu_integrator_params* integ,
                     double max_time, double* dt);
+/* iterateIdealH (src/Utilities/iterateIdealH.cc:30-310) with the SPH smoothing
+ * scale (SPHSmoothingScale::idealSmoothingScale, src/NodeList/SPHSmoothingScale.cc:120-197;
+ * nPerh, hmin, hmax from `hydro`), nPerhForIteration = 0, no spherical start, no
+ * fixed determinant: each iteration makes the ghosts of the sim's boundaries and
+ * the connectivity from the current H, then replaces H of every internal node
+ * whose last change exceeds `tolerance`; ends after max_iterations or when the
+ * largest change (max |phi - 1| over the eigenvalues of H1^1/2 H^-1 H1^1/2) is
+ * within the tolerance (reference defaults: 100, 1e-10).  The ghosts are made
+ * from the final H.  *iterations and *max_delta report the loop (may be null). */
+int sphgpu_iterate_ideal_h(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, int max_iterations, double tolerance,
+                           double kernel_extent, int cull, int* iterations, double* max_delta);
 /* Integrator clock: current time, last step, cycle count (and setter). */
 int sphgpu_get_time(sphgpu_sim* sim, double* time, double* last_dt, int64_t* cycle);
 int sphgpu_set_time(sphgpu_sim* sim, double time, double last_dt, int64_t cycle);

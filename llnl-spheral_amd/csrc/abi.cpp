@@ -458,6 +458,15 @@ int sphgpu_synchronize(sphgpu_sim* s) {
   return guarded(s, [&] { SG_CHECK_HIP(hipStreamSynchronize(s->stream)); });
 }
 
+int sphgpu_iterate_ideal_h(sphgpu_sim* s, const sphgpu_hydro_params* hp, int maxIterations, double tolerance,
+                           double kernelExtent, int cull, int* iterations, double* maxDelta) {
+  return guarded(s, [&] {
+    SG_REQUIRE(hp, SPHGPU_ERR_ARG, "null parameters");
+    const int it = iterateIdealH(*s, *hp, maxIterations, tolerance, kernelExtent, cull != 0, maxDelta);
+    if (iterations) *iterations = it;
+  });
+}
+
 int sphgpu_ensemble_begin(sphgpu_sim* s) {
   return guarded(s, [&] {
     SG_REQUIRE(s->nTot == 0 && !s->ensemble, SPHGPU_ERR_STATE, "ensemble mode needs a sim with no nodes");

@@ -27,6 +27,8 @@
 #include "dgeom.hh"
 #include "records.hh"
 #include "ell.hh"
+#include "eigen.hh"
+#include <cstring>
 
 namespace sphgpu {
 
@@ -150,6 +152,20 @@ __device__ double equivalentNperh(const TabArgs& t, double Wsum) {
   if (lb == -1) return np[0];
   if (ub == n) return np[n - 1];
   return dmin(np[ub], dmax(np[lb], np[lb] + (Wsum - ws[lb]) / (ws[ub] - ws[lb]) * (np[ub] - np[lb])));
+}
+
+// SPHSmoothingScale::idealSmoothingScale (src/NodeList/SPHSmoothingScale.cc:120-197):
+// 1/h1 of the ideal H = h1^-1 I from the root-nu zeroth moment and H.xx.
+__device__ __forceinline__ double idealHinv(const TabArgs& t, double zeroth, double Hxx, double nPerh, double hmin,
+                                            double hmax) {
+  double nperhNow;
+  if (fuzzyEqual(zeroth, 0.0)) nperhNow = 0.5 * nPerh;
+  else nperhNow = equivalentNperh(t, zeroth);
+  const double s = dmin(4.0, dmax(0.25, nPerh / (nperhNow + 1.0e-30)));
+  const double aa = (s < 1.0) ? 0.4 * (1.0 + s * s) : 0.4 * (1.0 + 1.0 / (s * s * s));
+  const double hi0 = 1.0 / Hxx;
+  const double hi1 = dmin(hmax, dmax(hmin, hi0 * (1.0 - aa + aa * s)));
+  return 1.0 / hi1;
 }
 
 // H.v: with isotropic H the product is h*v componentwise (same bits as the
@@ -406,14 +422,7 @@ __device__ __forceinline__ void epilogue(int64_t k, int len, const Self<D>& me, 
   S<D> DHDt = sdivs(negH, double(D));
   for (int c = 0; c < NS; ++c) DHDt.c[c] = DHDt.c[c] * trace;
   // SPHSmoothingScale::idealSmoothingScale
-  double nperhNow;
-  if (fuzzyEqual(wNSo, 0.0)) nperhNow = 0.5 * p.nPerh;
-  else nperhNow = equivalentNperh(t, wNSo);
-  const double s = dmin(4.0, dmax(0.25, p.nPerh / (nperhNow + 1.0e-30)));
-  const double aa = (s < 1.0) ? 0.4 * (1.0 + s * s) : 0.4 * (1.0 + 1.0 / (s * s * s));
-  const double hi0 = 1.0 / Hi.c[0];
-  const double hi1 = dmin(p.hmax, dmax(p.hmin, hi0 * (1.0 - aa + aa * s)));
-  const double hinv = 1.0 / hi1;
+  const double hinv = idealHinv(t, wNSo, Hi.c[0], p.nPerh, p.hmin, p.hmax);
 
   // outputs
   *outRow(o, SPHGPU_D_RHO_SUM, k, 1) = rhoSum;
@@ -1130,6 +1139,95 @@ void omegaGradh(Sim& s) {
   applyGhostScalar(s, s.omega);
   s.recValid = false;         // records hold safeInv(omega)
   s.derivsValid = false;
+}
+
+// One iterateIdealH pass (src/Utilities/iterateIdealH.cc:139-230) with the
+// SPH smoothing scale: one lane per internal node; a node whose last change is
+// within tol keeps its H, the others take zeroth = rootnu(sum over the row of
+// |W.gradValue(|H_i x_ij|, 1)|) in row order, H1 = idealSmoothingScale, and the
+// change max|phi - 1| over the eigenvalues c/lambda(H_i) of H1^1/2 H_i^-1 H1^1/2
+// (H1 = c I).  Node i reads only its own H (neighbour positions come from
+// the records), so H is replaced in place.
+template <int D, bool ISO, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_ideal_h(const double* __restrict__ rec, const int32_t* __restrict__ rowCount,
+                                                    const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
+                                                    const int32_t* __restrict__ work, const int32_t* __restrict__ perm,
+                                                    TabArgs t, double nPerh, double hmin, double hmax, double tol,
+                                                    double* __restrict__ H, double* __restrict__ deltaH,
+                                                    unsigned long long* __restrict__ maxBits) {
+  using R = RL<ISO>;
+  constexpr int NS = NSym<D>::value;
+  extern __shared__ double lds[];
+  const double* tab = stageTable<LDS>(lds, t);
+  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  double d = 0.0;
+  if (tw < nInt) {
+    const int64_t k = work[tw];
+    const int64_t i = perm[k];
+    if (deltaH[i] > tol) {
+      S<D> Hi;
+      for (int c = 0; c < NS; ++c) Hi.c[c] = H[i * NS + c];
+      const double* ri_ = rec + k * R::STRIDE;
+      V<D> xi;
+      for (int c = 0; c < D; ++c) xi.c[c] = ri_[R::X + c];
+      double zeroth = 0.0;
+      const int len = rowCount[k];
+      for (int w = 0; w < len; ++w) {
+        const double* rj = rec + int64_t(nbr[ellIndex(w, n, k)]) * R::STRIDE;
+        V<D> xij;
+        for (int c = 0; c < D; ++c) xij.c[c] = xi.c[c] - rj[R::X + c];
+        const double etai = vmag(sdot(Hi, xij));
+        double Wv, gW;
+        lookupWG(tab, t, etai, 1.0, Wv, gW);
+        zeroth += 1.0 * dabs(gW);
+      }
+      zeroth = rootnu<D>(zeroth);
+      const double c1 = idealHinv(t, zeroth, Hi.c[0], nPerh, hmin, hmax);
+      const double phimin = c1 / maxEigen<D>(Hi.c), phimax = c1 / minEigen<D>(Hi.c);
+      d = dmax(dabs(phimin - 1.0), dabs(phimax - 1.0));
+      deltaH[i] = d;
+      int q = 0;
+      for (int a = 0; a < D; ++a)
+        for (int b = a; b < D; ++b, ++q) H[i * NS + q] = (a == b) ? c1 : 0.0;
+    }
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) d = dmax(d, __shfl_xor(d, sh));
+  if ((threadIdx.x & 63) == 0 && d > 0.0) atomicMax(maxBits, (unsigned long long)__double_as_longlong(d));
+}
+
+template <int D>
+double idealHPassD(Sim& s, const sphgpu_hydro_params& hp, double tol, double* deltaH, unsigned long long* bits) {
+  const int64_t n = s.nSlots;
+  TabArgs t = tabArgs(s);
+  const bool lds = s.table.n <= kLdsTablePoints;
+  const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(),
+                       s.perm.as<int32_t>(), t, hp.nPerh, hp.hmin, hp.hmax, tol, s.H.as<double>(), deltaH, bits);
+  };
+  if (s.isotropic) { if (lds) go(k_ideal_h<D, true, true>); else go(k_ideal_h<D, true, false>); }
+  else { if (lds) go(k_ideal_h<D, false, true>); else go(k_ideal_h<D, false, false>); }
+  SG_CHECK_HIP(hipGetLastError());
+  unsigned long long hb = 0;
+  SG_CHECK_HIP(hipMemcpyAsync(&hb, bits, sizeof(hb), hipMemcpyDeviceToHost, s.stream));
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+  double m;
+  std::memcpy(&m, &hb, sizeof(m));
+  return m;
+}
+
+double idealHPass(Sim& s, const sphgpu_hydro_params& hp, double tol, double* deltaH, unsigned long long* bits) {
+  SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "the ideal H pass needs a neighbour build");
+  SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
+  if (!s.recValid) packRecords(s);
+  double m;
+  if (s.dim == 1) m = idealHPassD<1>(s, hp, tol, deltaH, bits);
+  else if (s.dim == 2) m = idealHPassD<2>(s, hp, tol, deltaH, bits);
+  else m = idealHPassD<3>(s, hp, tol, deltaH, bits);
+  s.recValid = false;
+  s.derivsValid = false;
+  return m;
 }
 
 void packRecords(Sim& s) {

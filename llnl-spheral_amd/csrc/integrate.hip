@@ -190,6 +190,10 @@ __global__ void k_dt(const int32_t* __restrict__ work, const int32_t* __restrict
 }
 
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void k_fill_value(double* __restrict__ a, int64_t n, double v) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) a[i] = v;
+}
 
 // ---- SpecificThermalEnergyPolicy ------------------------------------------
 // DvDt and DepsDt of the internal slots to caller order; the ghost rows are
@@ -513,6 +517,41 @@ void ensembleRk2(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_
   if (s.dim == 1) ensembleRk2D<1>(s, hp, ip);
   else if (s.dim == 2) ensembleRk2D<2>(s, hp, ip);
   else ensembleRk2D<3>(s, hp, ip);
+}
+
+// iterateIdealH (src/Utilities/iterateIdealH.cc:30-310), SPH smoothing scale,
+// nPerhForIteration = 0, no spherical start, no fixed determinant.  Each
+// iteration makes the ghosts of the sim's boundaries (or only the
+// connectivity without boundaries) from the current H and passes over the
+// internal nodes; the loop ends after maxIterations or once the largest
+// change is within tol.  The ghosts are remade from the final H at the end.
+int iterateIdealH(Sim& s, const sphgpu_hydro_params& hp, int maxIterations, double tol, double ext, bool cull,
+                  double* maxDelta) {
+  SG_REQUIRE(s.nInt > 0, SPHGPU_ERR_STATE, "no nodes");
+  SG_REQUIRE(!s.ensemble, SPHGPU_ERR_STATE, "ensembles iterate H per member alone");
+  SG_REQUIRE(s.nTot == s.nInt || s.deviceGhosts, SPHGPU_ERR_STATE,
+             "iterateIdealH makes its own ghosts: set_nodes(n_internal, n_internal)");
+  SG_REQUIRE(maxIterations >= 0 && tol >= 0.0 && ext > 0.0, SPHGPU_ERR_ARG, "bad iteration parameters");
+  DBuf deltaH, bits;
+  deltaH.ensure(sizeof(double) * size_t(s.nInt));
+  bits.ensure(sizeof(unsigned long long));
+  const double init = 2.0 * tol;
+  hipLaunchKernelGGL(k_fill_value, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, deltaH.as<double>(), s.nInt, init);
+  double m = 2.0 * tol;
+  int it = 0;
+  while (it < maxIterations && m > tol) {
+    ++it;
+    if (!s.subs.empty()) setGhostNodes(s, ext, cull);
+    else buildNeighbors(s, ext);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s.stream, bits.as<unsigned long long>(), 0ull);
+    m = idealHPass(s, hp, tol, deltaH.as<double>(), bits.as<unsigned long long>());
+  }
+  // leave the ghosts made from the final H
+  if (!s.subs.empty()) setGhostNodes(s, ext, cull);
+  else s.neighborsValid = false;
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+  if (maxDelta) *maxDelta = m;
+  return it;
 }
 
 void eosRows(Sim& s, int64_t first, int64_t rows, double gamma, double Pmin, double Pmax, int floorType) {

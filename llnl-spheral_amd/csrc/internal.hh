@@ -235,6 +235,11 @@ int stateFieldComps(const Sim& s, int which);
 constexpr int kStateFields = 13;
 // hydro.hip
 void omegaGradh(Sim& s);          // computeSPHOmegaGradhCorrection into the omega field
+// one iterateIdealH pass over the current rows; returns the largest change
+double idealHPass(Sim& s, const sphgpu_hydro_params& hp, double tol, double* deltaH, unsigned long long* maxBits);
+// integrate.hip: iterateIdealH; returns the iterations done, *maxDelta the last largest change
+int iterateIdealH(Sim& s, const sphgpu_hydro_params& hp, int maxIterations, double tol, double kernelExtent,
+                  bool cull, double* maxDelta);
 void enforceBoundaries(Sim& s);   // internal nodes behind a plane are mapped back
 // integrate.hip
 void eosGammaLaw(Sim& s, double gamma, double Pmin, double Pmax, int floorType);

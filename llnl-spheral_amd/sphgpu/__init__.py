@@ -205,6 +205,8 @@ def lib():
             "sphgpu_apply_ghost_boundaries": (C.c_int, [vp]),
             "sphgpu_node_counts": (C.c_int, [vp, _i64p, _i64p]),
             "sphgpu_download_ghost_controls": (C.c_int, [vp, _i32p, _i32p]),
+            "sphgpu_iterate_ideal_h": (C.c_int, [vp, C.POINTER(HydroParams), C.c_int, C.c_double, C.c_double, C.c_int,
+                                                 C.POINTER(C.c_int), _dp]),
             "sphgpu_ensemble_begin": (C.c_int, [vp]),
             "sphgpu_ensemble_add": (C.c_int, [vp, C.POINTER(Member), C.POINTER(IntegratorParams), _i64p]),
             "sphgpu_ensemble_step": (C.c_int, [vp, C.POINTER(HydroParams), C.POINTER(IntegratorParams),
@@ -425,6 +427,15 @@ class Sim:
         self._chk(self.L.sphgpu_rk2_step(self.h, C.byref(hp), C.byref(ip), max_time, C.byref(dt)))
         self._sync_counts()
         return dt.value
+
+    def iterate_ideal_h(self, hydro=None, max_iterations=100, tolerance=1.0e-10, kernel_extent=2.0, cull=True):
+        """iterateIdealH on the resident state; returns (iterations, last largest change)."""
+        hp = hydro if hydro is not None else hydro_params()
+        it, md = C.c_int(), C.c_double()
+        self._chk(self.L.sphgpu_iterate_ideal_h(self.h, C.byref(hp), max_iterations, tolerance, kernel_extent,
+                                                 int(cull), C.byref(it), C.byref(md)))
+        self._sync_counts()
+        return it.value, md.value
 
     def clock(self):
         t, ldt, cyc = C.c_double(), C.c_double(), C.c_int64()

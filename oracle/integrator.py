@@ -232,3 +232,32 @@ class RK2:
         self.t = t + dt
         self.lastDt = dt
         return dt
+
+
+def iterate_ideal_h(dim, state, boundaries, max_iterations=100, tolerance=1.0e-10, params=None, table_points=1000,
+                    ext=2.0, cull=True):
+    """iterateIdealH (src/Utilities/iterateIdealH.cc:30-310) with the SPH
+    smoothing scale, nPerhForIteration = 0, no spherical start, no fixed
+    determinant: each iteration rebuilds the ghosts and the connectivity from
+    the current H, then passes over the internal nodes (oracle.ideal_h_pass);
+    the loop ends after max_iterations or when the largest change is within
+    the tolerance.  Returns (H of the internal nodes, iterations, last largest
+    change)."""
+    n = state["pos"].shape[0]
+    T = oracle.Table(dim, table_points)
+    p = params if params is not None else oracle.make_params()
+    st = {k: np.array(state[k], dtype=np.float64) for k in ("pos", "H", "mass", "rho")}
+    deltaH = np.full(n, 2.0 * tolerance)
+    maxDelta = 2.0 * tolerance
+    it = 0
+    while it < max_iterations and maxDelta > tolerance:
+        it += 1
+        if boundaries:
+            full, off, nbr, _ = oracle.ghost_pipeline(dim, st, boundaries, ext, cull)
+        else:
+            full = dict(st)
+            off, nbr = oracle.connectivity(dim, n, st["pos"], st["H"], ext)
+        H = np.ascontiguousarray(full["H"], dtype=np.float64).copy()
+        maxDelta = oracle.ideal_h_pass(T, p, n, full["pos"], H, deltaH, off, nbr, tolerance)
+        st["H"] = H[:n].copy()
+    return st["H"], it, maxDelta

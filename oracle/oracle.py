@@ -131,6 +131,9 @@ def lib(fast=False):
         L.orc_compatible_depsdt.argtypes = [C.c_int, C.c_int64, C.c_int64, _i64p, _i32p, _dp, _dp, _dp, _dp, _dp,
                                             C.c_double, _dp]
         L.orc_omega_gradh.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int64, _dp, _dp, _i64p, _i32p, _dp]
+        L.orc_ideal_h_pass.argtypes = [C.c_void_p, C.c_int, C.POINTER(Params), C.c_int64, _dp, _dp, _dp, _i64p, _i32p,
+                                       C.c_double]
+        L.orc_ideal_h_pass.restype = C.c_double
         L.orc_gamma_law.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     _dp, _dp]
         L.orc_num_threads.restype = C.c_int
@@ -262,6 +265,16 @@ def eval_derivs(table, params, n_internal, state, off, nbr, gather=False, pair_a
     if pair_acc:
         res["pairAcc"] = pa[:int(off[-1])]
     return res
+
+
+def ideal_h_pass(table, params, n_internal, pos, H, deltaH, off, nbr, tol):
+    """One iterateIdealH pass (SPH smoothing scale); H (all rows, internal ones
+    replaced) and deltaH (internal rows) are updated in place.  Returns the
+    largest change of the nodes computed."""
+    assert H.flags.c_contiguous and deltaH.flags.c_contiguous and H.dtype == np.float64
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    return table.L.orc_ideal_h_pass(table.h, table.dim, C.byref(params), n_internal, _p(pos), _p(H), _p(deltaH),
+                                    _p(off, _i64p), _p(nbr, _i32p), tol)
 
 
 def compatible_depsdt(dim, n_internal, off, nbr, mass, vel, acc, DepsDt0, pair_acc, multiplier):

@@ -545,9 +545,61 @@ template <int D> static NodeState<D> toNS(const orc_state* st) {
   return s;
 }
 
+// One iteration of iterateIdealH (src/Utilities/iterateIdealH.cc:139-230) with
+// the SPH smoothing scale, over the internal nodes with the current
+// connectivity (rows in index order, ghosts included): nodes whose last change
+// is within `tol` keep their H; the others take
+//   zeroth = rootnu(sum_j |W.gradValue(|H_i x_ij|, 1)|)   (fweight 1, one NodeList)
+//   H1 = SPHSmoothingScale::idealSmoothingScale(H_i, zeroth)
+// and the change max|phi - 1| over the eigenvalues phi of H1^1/2 H_i^-1 H1^1/2;
+// with H1 = c I these are c/lambda(H_i), lambda from minEig/maxEig.  H is
+// replaced for every internal node after the pass; returns the largest change
+// of the nodes computed.
+template <int D>
+double idealHPass(const Table& W, const Params& p, int64_t nInt, const double* pos, double* H, double* deltaH,
+                  const int64_t* off, const int32_t* nbr, double tol, double (*minEig)(int, const double*),
+                  double (*maxEig)(int, const double*)) {
+  constexpr int NS = nsym<D>();
+  std::vector<double> H1(static_cast<size_t>(nInt) * NS);
+  double maxDelta = 0.0;
+  for (int64_t i = 0; i < nInt; ++i) {
+    Sym<D> Hi;
+    for (int k = 0; k < NS; ++k) sptr(Hi)[k] = H[i * NS + k];
+    if (deltaH[i] <= tol) {
+      for (int k = 0; k < NS; ++k) H1[size_t(i) * NS + k] = H[i * NS + k];
+      continue;
+    }
+    Vec<D> ri;
+    for (int k = 0; k < D; ++k) ri.c[k] = pos[i * D + k];
+    double zeroth = 0.0;
+    for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+      const int64_t j = nbr[e];
+      Vec<D> rj;
+      for (int k = 0; k < D; ++k) rj.c[k] = pos[j * D + k];
+      const Vec<D> xij = sub(ri, rj);
+      const double etai = mag(sdot(Hi, xij));
+      const double Wi = std::fabs(W.gradValue(etai, 1.0));
+      zeroth += 1.0 * Wi;
+    }
+    zeroth = rootnu<D>(zeroth);
+    const Sym<D> Hn = idealH<D>(W, Hi, zeroth, p);
+    for (int k = 0; k < NS; ++k) H1[size_t(i) * NS + k] = sptr(Hn)[k];
+    const double c = Hn.xx;
+    const double phimin = c / maxEig(D, sptr(Hi)), phimax = c / minEig(D, sptr(Hi));
+    const double d = std::max(std::fabs(phimin - 1.0), std::fabs(phimax - 1.0));
+    deltaH[i] = d;
+    maxDelta = std::max(maxDelta, d);
+  }
+  std::copy(H1.begin(), H1.end(), H);
+  return maxDelta;
+}
+
 }  // namespace orc
 
 using namespace orc;
+
+extern "C" double orc_min_eigen(int dim, const double* h);
+extern "C" double orc_max_eigen(int dim, const double* h);
 
 extern "C" {
 
@@ -602,6 +654,14 @@ void orc_sum_density(void* tp, int dim, int gather, int64_t nInt, int64_t nTot, 
   if (dim == 1) sumDensity<1>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
   else if (dim == 2) sumDensity<2>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
   else sumDensity<3>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
+}
+
+double orc_ideal_h_pass(void* tp, int dim, const Params* p, int64_t nInt, const double* pos, double* H, double* deltaH,
+                        const int64_t* off, const int32_t* nbr, double tol) {
+  const Table& W = *static_cast<Table*>(tp);
+  if (dim == 1) return orc::idealHPass<1>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen);
+  if (dim == 2) return orc::idealHPass<2>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen);
+  return orc::idealHPass<3>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen);
 }
 
 void orc_omega_gradh(void* tp, int dim, int64_t nInt, int64_t nTot, const double* pos, const double* H,

@@ -232,3 +232,25 @@ def test_config1_converging_32cube_rk2(sphgpu, oracle):
         a = s.download(f, rows=N).reshape(ref.st[key].shape)
         b = ref.st[key]
         assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), key
+
+
+@pytest.mark.parametrize("dim,n,kind", [(1, 80, "reflecting"), (2, 14, "reflecting"), (3, 7, "periodic")])
+def test_iterate_ideal_h_matches_oracle(sphgpu, oracle, dim, n, kind):
+    """iterateIdealH on the device against the oracle (reference row order):
+    same iteration count, H equal value for value in 1-D/2-D, 1e-12 in 3-D
+    (cube root of the zeroth moment)."""
+    import integrator
+    st, _ = problems.random_state(dim, n, seed=90 + dim, jitter=0.2, h_var=0.3)
+    N = st["pos"].shape[0]
+    print(f"[ideal H] dim={dim} N={N} {kind}", flush=True)
+    b_or = oracle.box_boundaries(dim, kind=kind)
+    Hr, itr, mdr = integrator.iterate_ideal_h(dim, {k: st[k] for k in ("pos", "mass", "rho", "H")}, b_or)
+    s = device(sphgpu, dim, st, kind, 1.4)
+    it, md = s.iterate_ideal_h(sphgpu.hydro_params(), 100, 1.0e-10)
+    assert it == itr, (it, itr)
+    Hd = s.download("H", rows=N).reshape(Hr.shape)
+    if dim < 3:
+        assert np.array_equal(Hd, Hr)
+        assert md == mdr
+    else:
+        assert np.abs(Hd - Hr).max() <= 1e-12 * np.abs(Hr).max()

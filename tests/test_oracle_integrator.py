@@ -116,3 +116,43 @@ def test_compatible_energy_conserved(oracle, integ, kind):
         drift[compatible] = abs(E1 - E0) / E0
     assert drift[True] <= 1e-14
     assert drift[False] > 10 * drift[True]
+
+
+def test_iterate_ideal_h_fixed_point(oracle, integ):
+    """iterateIdealH on a periodic lattice: from a smoothing scale 30% too
+    large and from one 20% too small the iteration reaches the same H, equal on
+    every node, and there the lattice's equivalent nodes per smoothing scale
+    (TableKernel::equivalentNodesPerSmoothingScale of the root-nu sum of
+    |W'|) is nPerh, the fixed point of SPHSmoothingScale::idealSmoothingScale."""
+    dim, n = 2, 20
+    pos, mass, H, dx = problems.lattice(dim, n)
+    N = pos.shape[0]
+    b = oracle.box_boundaries(dim, kind="periodic")
+    res = []
+    for f in (1.3, 0.8):
+        st = dict(pos=pos, mass=mass, rho=np.ones(N), H=H * f)
+        H1, it, md = integ.iterate_ideal_h(dim, st, b)
+        assert md <= 1.0e-10 and it < 100
+        assert np.abs(H1[:, 0] - H1[0, 0]).max() <= 1e-13 * H1[0, 0]
+        res.append(H1[0, 0])
+    assert res[0] == pytest.approx(res[1], rel=1e-9)
+    # fixed point: sum |W'(|H x_ij|)| over the lattice neighbours of one node
+    T = oracle.Table(dim, 1000)
+    h = res[0]
+    d = (pos - pos[0] + 0.5) % 1.0 - 0.5
+    eta = h * np.sqrt((d ** 2).sum(1))
+    eta = eta[(eta > 0) & (eta < 2.0)]
+    w = sum(abs(T.L.orc_table_grad(T.h, float(e), 1.0)) for e in eta)
+    nperh = T.L.orc_table_nperh_from_wsum(T.h, float(np.sqrt(w)))
+    assert nperh == pytest.approx(1.51, rel=1e-8)
+
+
+def test_iterate_ideal_h_jittered_box(oracle, integ):
+    """A jittered lattice in a reflecting box with a spread of smoothing
+    scales converges below the tolerance, every change computed with the
+    ghosts of the current H."""
+    st, meta = problems.random_state(2, 14, seed=77, jitter=0.2, h_var=0.3)
+    H1, it, md = integ.iterate_ideal_h(2, {k: st[k] for k in ("pos", "mass", "rho", "H")},
+                                       oracle.box_boundaries(2), max_iterations=100, tolerance=1e-10)
+    assert md <= 1e-10 and 2 < it < 100
+    assert np.all(H1[:, 1] == 0.0) and np.all(H1[:, 0] == H1[:, 2])
