@@ -285,6 +285,14 @@ int sphgpu_rk2_step(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, const sph
  * from the final H.  *iterations and *max_delta report the loop (may be null). */
 int sphgpu_iterate_ideal_h(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, int max_iterations, double tolerance,
                            double kernel_extent, int cull, int* iterations, double* max_delta);
+/* Conservation totals of the internal nodes, as SpheralConservation takes them
+ * (src/SimulationControl/SpheralConservation.py:66-150): out[9] = total mass,
+ * linear momentum (x, y, z), angular momentum about the origin (x, y, z; z only
+ * in 2-D, zero in 1-D), kinetic energy sum m v^2 / 2, thermal energy sum m eps.
+ * Summed on the device in a fixed order (deterministic). */
+int sphgpu_conservation(sphgpu_sim* sim, double* out);
+/* The same totals over one ensemble member's rows. */
+int sphgpu_ensemble_conservation(sphgpu_sim* sim, int64_t id, double* out);
 /* Integrator clock: current time, last step, cycle count (and setter). */
 int sphgpu_get_time(sphgpu_sim* sim, double* time, double* last_dt, int64_t* cycle);
 int sphgpu_set_time(sphgpu_sim* sim, double time, double last_dt, int64_t cycle);

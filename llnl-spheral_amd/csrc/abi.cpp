@@ -467,6 +467,24 @@ int sphgpu_iterate_ideal_h(sphgpu_sim* s, const sphgpu_hydro_params* hp, int max
   });
 }
 
+int sphgpu_conservation(sphgpu_sim* s, double* out) {
+  return guarded(s, [&] {
+    SG_REQUIRE(out, SPHGPU_ERR_ARG, "null output");
+    SG_REQUIRE(s->nInt > 0, SPHGPU_ERR_STATE, "no nodes");
+    conservationTotals(*s, 0, s->nInt, out);
+  });
+}
+
+int sphgpu_ensemble_conservation(sphgpu_sim* s, int64_t id, double* out) {
+  return guarded(s, [&] {
+    SG_REQUIRE(out, SPHGPU_ERR_ARG, "null output");
+    SG_REQUIRE(s->ensemble, SPHGPU_ERR_STATE, "not an ensemble");
+    auto it = std::find_if(s->members.begin(), s->members.end(), [&](const Member& mb) { return mb.id == id; });
+    SG_REQUIRE(it != s->members.end(), SPHGPU_ERR_ARG, "no member with that id");
+    conservationTotals(*s, it->first, it->first + it->rows, out);
+  });
+}
+
 int sphgpu_ensemble_begin(sphgpu_sim* s) {
   return guarded(s, [&] {
     SG_REQUIRE(s->nTot == 0 && !s->ensemble, SPHGPU_ERR_STATE, "ensemble mode needs a sim with no nodes");

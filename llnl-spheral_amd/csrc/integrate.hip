@@ -519,6 +519,77 @@ void ensembleRk2(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_
   else ensembleRk2D<3>(s, hp, ip);
 }
 
+// ---- SpheralConservation totals ---------------------------------------------
+// Per block, in a fixed order, over rows [r0, r1): mass, linear momentum (3),
+// angular momentum about the origin (3), 2 x kinetic energy, thermal energy.
+constexpr int kCons = 9;
+template <int D>
+__global__ void k_conserve_partial(const double* __restrict__ pos, const double* __restrict__ vel,
+                                   const double* __restrict__ mass, const double* __restrict__ eps, int64_t r0,
+                                   int64_t r1, double* __restrict__ partial) {
+  __shared__ double sh[kCons][kBlock];
+  double a[kCons] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = r0 + blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < r1; i += int64_t(gridDim.x) * blockDim.x) {
+    const double m = mass[i];
+    double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    for (int c = 0; c < D; ++c) { x[c] = pos[i * D + c]; v[c] = vel[i * D + c]; }
+    a[0] += m;
+    for (int c = 0; c < D; ++c) a[1 + c] += m * v[c];
+    if (D == 2) a[6] += m * (x[0] * v[1] - x[1] * v[0]);
+    if (D == 3) {
+      a[4] += m * (x[1] * v[2] - x[2] * v[1]);
+      a[5] += m * (x[2] * v[0] - x[0] * v[2]);
+      a[6] += m * (x[0] * v[1] - x[1] * v[0]);
+    }
+    double v2 = v[0] * v[0];
+    for (int c = 1; c < D; ++c) v2 = v2 + v[c] * v[c];
+    a[7] += m * v2;
+    a[8] += m * eps[i];
+  }
+  for (int q = 0; q < kCons; ++q) sh[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int st = kBlock / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < kCons; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < kCons) partial[blockIdx.x * kCons + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void k_conserve_final(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= kCons) return;
+  double t = 0.0;
+  for (int b = 0; b < nb; ++b) t += partial[b * kCons + q];
+  out[q] = t;
+}
+
+// Totals of rows [r0, r1) (internal nodes of the sim, or of one ensemble
+// member): [mass, p (3), L (3), kinetic, thermal], as SpheralConservation
+// (src/SimulationControl/SpheralConservation.py:66-150) takes them.
+void conservationTotals(Sim& s, int64_t r0, int64_t r1, double* out) {
+  const int nb = int(std::min<int64_t>(256, std::max<int64_t>(1, (r1 - r0 + kBlock - 1) / kBlock)));
+  DBuf tmp;
+  tmp.ensure(sizeof(double) * size_t(nb + 1) * kCons);
+  double* partial = tmp.as<double>();
+  double* res = partial + size_t(nb) * kCons;
+  if (s.dim == 1)
+    hipLaunchKernelGGL(k_conserve_partial<1>, dim3(nb), dim3(kBlock), 0, s.stream, s.pos.as<double>(),
+                       s.vel.as<double>(), s.mass.as<double>(), s.eps.as<double>(), r0, r1, partial);
+  else if (s.dim == 2)
+    hipLaunchKernelGGL(k_conserve_partial<2>, dim3(nb), dim3(kBlock), 0, s.stream, s.pos.as<double>(),
+                       s.vel.as<double>(), s.mass.as<double>(), s.eps.as<double>(), r0, r1, partial);
+  else
+    hipLaunchKernelGGL(k_conserve_partial<3>, dim3(nb), dim3(kBlock), 0, s.stream, s.pos.as<double>(),
+                       s.vel.as<double>(), s.mass.as<double>(), s.eps.as<double>(), r0, r1, partial);
+  hipLaunchKernelGGL(k_conserve_final, dim3(1), dim3(64), 0, s.stream, partial, nb, res);
+  SG_CHECK_HIP(hipGetLastError());
+  double h[kCons];
+  SG_CHECK_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s.stream));
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+  for (int q = 0; q < kCons; ++q) out[q] = h[q];
+  out[7] = 0.5 * h[7];
+}
+
 // iterateIdealH (src/Utilities/iterateIdealH.cc:30-310), SPH smoothing scale,
 // nPerhForIteration = 0, no spherical start, no fixed determinant.  Each
 // iteration makes the ghosts of the sim's boundaries (or only the

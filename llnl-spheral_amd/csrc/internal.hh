@@ -237,6 +237,7 @@ constexpr int kStateFields = 13;
 void omegaGradh(Sim& s);          // computeSPHOmegaGradhCorrection into the omega field
 // one iterateIdealH pass over the current rows; returns the largest change
 double idealHPass(Sim& s, const sphgpu_hydro_params& hp, double tol, double* deltaH, unsigned long long* maxBits);
+void conservationTotals(Sim& s, int64_t r0, int64_t r1, double* out);   // integrate.hip, 9 values
 // integrate.hip: iterateIdealH; returns the iterations done, *maxDelta the last largest change
 int iterateIdealH(Sim& s, const sphgpu_hydro_params& hp, int maxIterations, double tol, double kernelExtent,
                   bool cull, double* maxDelta);

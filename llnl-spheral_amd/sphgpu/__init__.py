@@ -207,6 +207,8 @@ def lib():
             "sphgpu_download_ghost_controls": (C.c_int, [vp, _i32p, _i32p]),
             "sphgpu_iterate_ideal_h": (C.c_int, [vp, C.POINTER(HydroParams), C.c_int, C.c_double, C.c_double, C.c_int,
                                                  C.POINTER(C.c_int), _dp]),
+            "sphgpu_conservation": (C.c_int, [vp, _dp]),
+            "sphgpu_ensemble_conservation": (C.c_int, [vp, C.c_int64, _dp]),
             "sphgpu_ensemble_begin": (C.c_int, [vp]),
             "sphgpu_ensemble_add": (C.c_int, [vp, C.POINTER(Member), C.POINTER(IntegratorParams), _i64p]),
             "sphgpu_ensemble_step": (C.c_int, [vp, C.POINTER(HydroParams), C.POINTER(IntegratorParams),
@@ -436,6 +438,15 @@ class Sim:
                                                  int(cull), C.byref(it), C.byref(md)))
         self._sync_counts()
         return it.value, md.value
+
+    def conservation(self, member=None):
+        """Totals [mass, p(3), L(3), KE, TE] of the internal nodes (or one ensemble member)."""
+        out = np.zeros(9)
+        if member is None:
+            self._chk(self.L.sphgpu_conservation(self.h, _p(out)))
+        else:
+            self._chk(self.L.sphgpu_ensemble_conservation(self.h, member, _p(out)))
+        return out
 
     def clock(self):
         t, ldt, cyc = C.c_double(), C.c_double(), C.c_int64()

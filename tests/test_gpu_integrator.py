@@ -254,3 +254,36 @@ def test_iterate_ideal_h_matches_oracle(sphgpu, oracle, dim, n, kind):
         assert md == mdr
     else:
         assert np.abs(Hd - Hr).max() <= 1e-12 * np.abs(Hr).max()
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_conservation_totals(sphgpu, dim):
+    """sphgpu_conservation: SpheralConservation's totals (mass, linear and
+    angular momentum about the origin, kinetic and thermal energy) of the
+    internal nodes against numpy sums, 1e-13 relative; an ensemble member's
+    totals equal the same state's as a plain sim."""
+    n = {1: 90, 2: 15, 3: 7}[dim]
+    st, _ = problems.random_state(dim, n, seed=120 + dim, vel_scale=0.7)
+    N = st["pos"].shape[0]
+    s = device(sphgpu, dim, st, "reflecting", 1.4)
+    t = s.conservation()
+    m, x, v, e = st["mass"], st["pos"].reshape(N, dim), st["vel"].reshape(N, dim), st["eps"]
+    X, V = np.zeros((N, 3)), np.zeros((N, 3))
+    X[:, :dim], V[:, :dim] = x, v
+    L = (m[:, None] * np.cross(X, V)).sum(0)
+    if dim == 1:
+        L[:] = 0.0
+    if dim == 2:
+        L[:2] = 0.0
+    ref = np.concatenate([[m.sum()], (m[:, None] * V).sum(0), L, [0.5 * (m * (V * V).sum(1)).sum(), (m * e).sum()]])
+    scale = np.array([m.sum()] + [np.abs(m[:, None] * V).sum()] * 3 + [np.abs(m[:, None] * np.cross(X, V)).sum() + 1e-300] * 3
+                     + [ref[7], ref[8]])
+    assert np.all(np.abs(t - ref) <= 1e-13 * scale), (t, ref)
+    e_ = sphgpu.Sim(dim)
+    e_.set_table_bspline(1000)
+    e_.set_boundaries(sphgpu.box_boundaries(dim))
+    e_.ensemble_begin()
+    other, _ = problems.random_state(dim, n, seed=7, vel_scale=0.2)
+    e_.ensemble_add(other, 1.4, 1.0)
+    mid = e_.ensemble_add(st, 1.4, 1.0)
+    assert np.array_equal(e_.conservation(mid), t)
