@@ -346,6 +346,10 @@ int sphgpu_ensemble_members(sphgpu_sim* sim, int cap, int64_t* ids, double* time
                             int64_t* first_row, int64_t* rows, int* count);
 /* A member's internal rows of one state field (SPHGPU_F_*). */
 int sphgpu_ensemble_download(sphgpu_sim* sim, int64_t id, int field, double* host);
+/* Set a member's clock (time, last dt, cycle), e.g. when it is restored from a
+ * snapshot (the reference restores the integrator's clock with its restart
+ * state, src/Integrator/Integrator.cc:794-823). */
+int sphgpu_ensemble_set_clock(sphgpu_sim* sim, int64_t id, double time, double last_dt, int64_t cycle);
 /* Remove members; the others keep their state and results. */
 int sphgpu_ensemble_remove(sphgpu_sim* sim, int n_ids, const int64_t* ids);
 /* Device bytes per node the library holds for a member of this dimension

@@ -546,6 +546,25 @@ int sphgpu_ensemble_download(sphgpu_sim* s, int64_t id, int field, double* host)
   });
 }
 
+int sphgpu_ensemble_set_clock(sphgpu_sim* s, int64_t id, double time, double lastDt, int64_t cycle) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->ensemble, SPHGPU_ERR_STATE, "not an ensemble");
+    auto it = std::find_if(s->members.begin(), s->members.end(), [&](const Member& mb) { return mb.id == id; });
+    SG_REQUIRE(it != s->members.end(), SPHGPU_ERR_ARG, "no member with that id");
+    SG_REQUIRE(lastDt > 0.0 && cycle >= 0, SPHGPU_ERR_ARG, "bad clock");
+    const size_t q = size_t(it - s->members.begin());
+    SG_CHECK_HIP(hipMemcpyAsync(s->segHost.data(), s->segState.p, sizeof(SegState) * s->segHost.size(),
+                                hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+    s->segHost[q].t = time;
+    s->segHost[q].lastDt = lastDt;
+    s->segHost[q].cycle = double(cycle);
+    SG_CHECK_HIP(hipMemcpyAsync(static_cast<SegState*>(s->segState.p) + q, &s->segHost[q], sizeof(SegState),
+                                hipMemcpyHostToDevice, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
 int sphgpu_ensemble_remove(sphgpu_sim* s, int nIds, const int64_t* ids) {
   return guarded(s, [&] { ensembleRemove(*s, nIds, ids); });
 }

@@ -217,6 +217,7 @@ def lib():
                                                   C.POINTER(C.c_int)]),
             "sphgpu_ensemble_download": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
             "sphgpu_ensemble_remove": (C.c_int, [vp, C.c_int, _i64p]),
+            "sphgpu_ensemble_set_clock": (C.c_int, [vp, C.c_int64, C.c_double, C.c_double, C.c_int64]),
             "sphgpu_ensemble_bytes_per_node": (C.c_int64, [C.c_int]),
             "sphgpu_serve": (C.c_int, [vp, C.POINTER(Request), C.c_int, C.POINTER(HydroParams),
                                        C.POINTER(IntegratorParams), C.POINTER(ServeParams)]),
@@ -509,6 +510,9 @@ class Sim:
         a = np.zeros((self._members[member_id], c))
         self._chk(self.L.sphgpu_ensemble_download(self.h, member_id, STATE_FIELDS[field], _p(a)))
         return a[:, 0] if c == 1 else a
+
+    def ensemble_set_clock(self, member_id, t, last_dt, cycle):
+        self._chk(self.L.sphgpu_ensemble_set_clock(self.h, member_id, t, last_dt, cycle))
 
     def ensemble_remove(self, ids):
         ids = np.ascontiguousarray(list(ids), dtype=np.int64)

@@ -1,0 +1,61 @@
+"""Restart snapshots of a simulation or of one ensemble member.
+
+The reference's restart files (RestartRegistrar, src/DataOutput/RestartRegistrar.hh:28-66)
+hold every registered object's dumpState: the NodeList fields and the
+integrator's lastDt, currentTime and currentCycle
+(src/Integrator/Integrator.cc:794-823).  A snapshot here holds the same state
+of the internal nodes (position, velocity, mass, mass density, specific
+thermal energy, H, the grad-h correction) and the clock, in a numpy .npz file
+(loaded without pickles); the derived fields (pressure, sound speed, ghosts,
+neighbours) are remade on restore and by the next step.
+"""
+import numpy as np
+
+FIELDS = {"pos": "position", "vel": "velocity", "mass": "mass", "rho": "mass density",
+          "eps": "specific thermal energy", "H": "H", "omega": "grad h corrections"}
+
+
+def save(sim, path, member=None, gamma=None, t_end=None):
+    """Write the internal-node state and clock of `sim` (or of ensemble member
+    `member`, with its gamma and end time) to `path`."""
+    out = {"dim": np.int64(sim.dim)}
+    if member is None:
+        n = sim.node_counts()[0]
+        for k, f in FIELDS.items():
+            out[k] = sim.download(f, rows=n)
+        t, ldt, cyc = sim.clock()
+    else:
+        for k, f in FIELDS.items():
+            out[k] = sim.ensemble_download(member, f)
+        mb = [m for m in sim.ensemble_members() if m["id"] == member][0]
+        t, ldt, cyc = mb["t"], mb["last_dt"], mb["cycle"]
+        out["gamma"] = np.float64(gamma)
+        out["t_end"] = np.float64(t_end)
+    out["clock"] = np.array([t, ldt, float(cyc)])
+    np.savez(path, **out)
+
+
+def load(path):
+    """The snapshot as a dict (state arrays, clock, optional gamma/t_end)."""
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def restore(sim, path, gamma, Pmin=-1.0e200, Pmax=1.0e200, floor=True):
+    """Restore a plain simulation: n internal nodes, state, EOS and clock."""
+    z = load(path)
+    n = z["mass"].shape[0]
+    sim.set_nodes(n, n)
+    sim.upload_state({k: z[k] for k in FIELDS})
+    sim.eos_gamma_law(gamma, Pmin, Pmax, floor)
+    t, ldt, cyc = z["clock"]
+    sim.set_clock(float(t), float(ldt), int(cyc))
+
+
+def restore_member(sim, path, integ=None):
+    """Add a snapshot to an ensemble as a member with its clock; returns its id."""
+    z = load(path)
+    mid = sim.ensemble_add({k: z[k] for k in FIELDS}, float(z["gamma"]), float(z["t_end"]), integ)
+    t, ldt, cyc = z["clock"]
+    sim.ensemble_set_clock(mid, float(t), float(ldt), int(cyc))
+    return mid

@@ -287,3 +287,56 @@ def test_conservation_totals(sphgpu, dim):
     e_.ensemble_add(other, 1.4, 1.0)
     mid = e_.ensemble_add(st, 1.4, 1.0)
     assert np.array_equal(e_.conservation(mid), t)
+
+
+def test_restart_snapshot_continues_bitwise(sphgpu, tmp_path):
+    """A snapshot restored into a fresh sim (or as a member of an ensemble)
+    continues on exactly the trajectory of the run it was taken from."""
+    from sphgpu import restart
+    st, _ = problems.random_state(2, 12, seed=131, jitter=0.1, vel_scale=0.5)
+    N = st["pos"].shape[0]
+    hp, ip = sphgpu.hydro_params(), sphgpu.integrator_params(gamma=1.4)
+    a = device(sphgpu, 2, st, "reflecting", 1.4)
+    for _ in range(3):
+        a.rk2_step(hp, ip)
+    p = str(tmp_path / "snap.npz")
+    restart.save(a, p)
+    for _ in range(3):
+        a.rk2_step(hp, ip)
+    b = sphgpu.Sim(2)
+    b.set_row_order(sphgpu.ROW_ORDER_REFERENCE)
+    b.set_table_bspline(1000)
+    b.set_boundaries(sphgpu.box_boundaries(2))
+    restart.restore(b, p, 1.4)
+    for _ in range(3):
+        b.rk2_step(hp, ip)
+    assert a.clock() == b.clock()
+    for f in ("position", "velocity", "specific thermal energy", "mass density", "H"):
+        assert np.array_equal(a.download(f, rows=N), b.download(f, rows=N)), f
+    # ensemble member: snapshot after 2 steps, restore in another ensemble
+    def ens():
+        e = sphgpu.Sim(2)
+        e.set_row_order(sphgpu.ROW_ORDER_REFERENCE)
+        e.set_table_bspline(1000)
+        e.set_boundaries(sphgpu.box_boundaries(2))
+        e.ensemble_begin()
+        return e
+    e1 = ens()
+    m1 = e1.ensemble_add(st, 1.4, 1.0)
+    for _ in range(2):
+        e1.ensemble_step(hp, ip)
+    q = str(tmp_path / "member.npz")
+    restart.save(e1, q, member=m1, gamma=1.4, t_end=1.0)
+    for _ in range(2):
+        e1.ensemble_step(hp, ip)
+    e2 = ens()
+    other, _ = problems.random_state(2, 9, seed=5)
+    e2.ensemble_add(other, 5.0 / 3.0, 1.0)
+    m2 = restart.restore_member(e2, q)
+    for _ in range(2):
+        e2.ensemble_step(hp, ip)
+    c1 = [m for m in e1.ensemble_members() if m["id"] == m1][0]
+    c2 = [m for m in e2.ensemble_members() if m["id"] == m2][0]
+    assert (c1["t"], c1["last_dt"], c1["cycle"]) == (c2["t"], c2["last_dt"], c2["cycle"])
+    for f in ("position", "velocity", "specific thermal energy", "mass density", "H"):
+        assert np.array_equal(e1.ensemble_download(m1, f), e2.ensemble_download(m2, f)), f
