@@ -293,6 +293,13 @@ int sphgpu_iterate_ideal_h(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, in
 int sphgpu_conservation(sphgpu_sim* sim, double* out);
 /* The same totals over one ensemble member's rows. */
 int sphgpu_ensemble_conservation(sphgpu_sim* sim, int64_t id, double* out);
+/* The previous step's derivatives the dt vote reads (per internal row:
+ * maxViscousPressure, tr DvDx, DvDt; 2 + dim doubles), rows [first, first +
+ * rows): zero before the first step.  With the state and the clock they let a
+ * restored simulation continue on the same trajectory (the reference dumps its
+ * derivative fields with the restart state, src/SPH/SPHHydroBase.cc:1283-1318). */
+int sphgpu_get_dt_inputs(sphgpu_sim* sim, int64_t first, int64_t rows, double* out);
+int sphgpu_set_dt_inputs(sphgpu_sim* sim, int64_t first, int64_t rows, const double* in);
 /* Integrator clock: current time, last step, cycle count (and setter). */
 int sphgpu_get_time(sphgpu_sim* sim, double* time, double* last_dt, int64_t* cycle);
 int sphgpu_set_time(sphgpu_sim* sim, double time, double last_dt, int64_t cycle);

@@ -485,6 +485,35 @@ int sphgpu_ensemble_conservation(sphgpu_sim* s, int64_t id, double* out) {
   });
 }
 
+int sphgpu_get_dt_inputs(sphgpu_sim* s, int64_t first, int64_t rows, double* out) {
+  return guarded(s, [&] {
+    SG_REQUIRE(out && first >= 0 && rows >= 0 && first + rows <= s->nInt, SPHGPU_ERR_ARG, "bad row range");
+    const size_t per = size_t(2 + s->dim);
+    if (!s->haveDtIn) {
+      std::fill(out, out + per * size_t(rows), 0.0);
+      return;
+    }
+    SG_CHECK_HIP(hipMemcpyAsync(out, s->dtIn.as<double>() + size_t(first) * per, sizeof(double) * per * size_t(rows),
+                                hipMemcpyDeviceToHost, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int sphgpu_set_dt_inputs(sphgpu_sim* s, int64_t first, int64_t rows, const double* in) {
+  return guarded(s, [&] {
+    SG_REQUIRE(in && first >= 0 && rows >= 0 && first + rows <= s->nInt, SPHGPU_ERR_ARG, "bad row range");
+    const size_t per = size_t(2 + s->dim);
+    if (!s->haveDtIn) {
+      s->dtIn.ensure(sizeof(double) * per * size_t(std::max<int64_t>(s->nInt, 1)));
+      SG_CHECK_HIP(hipMemsetAsync(s->dtIn.p, 0, sizeof(double) * per * size_t(s->nInt), s->stream));
+      s->haveDtIn = true;
+    }
+    SG_CHECK_HIP(hipMemcpyAsync(s->dtIn.as<double>() + size_t(first) * per, in, sizeof(double) * per * size_t(rows),
+                                hipMemcpyHostToDevice, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
 int sphgpu_ensemble_begin(sphgpu_sim* s) {
   return guarded(s, [&] {
     SG_REQUIRE(s->nTot == 0 && !s->ensemble, SPHGPU_ERR_STATE, "ensemble mode needs a sim with no nodes");

@@ -853,19 +853,14 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
     const double* ri_ = rec + k * R::STRIDE;
     double xi[3];
     Real vi[3];
-    Self<D> me;
     for (int c = 0; c < 3; ++c) {
       xi[c] = c < D ? ri_[R::X + c] : 0.0;
       vi[c] = c < D ? Real(ri_[R::V + c]) : Real(0);
     }
-    for (int c = 0; c < D; ++c) me.v.c[c] = ri_[R::V + c];
-    const double hinvD = ri_[RL<true>::HINV];
-    for (int c = 0; c < NSym<D>::value; ++c) me.H.c[c] = 0.0;
-    for (int c = 0; c < D; ++c) me.H.c[sdiag<D>(c)] = hinvD;
-    me.Hdet = ri_[R::HDET]; me.m = ri_[R::M]; me.rho = ri_[R::RHO]; me.mor = ri_[R::MOR];
-    const Real hi = Real(hinvD), hdeti = Real(me.Hdet), ci = Real(ri_[R::CS]);
-    const Real invRhoi = Real(1.0 / me.rho), rhoi2 = Real(me.rho * me.rho);
-    const Real Prhoi = Real(ri_[R::PRHO]), mori = Real(me.mor);
+    const double rho0 = ri_[R::RHO];
+    const Real hi = Real(ri_[RL<true>::HINV]), hdeti = Real(ri_[R::HDET]), ci = Real(ri_[R::CS]);
+    const Real invRhoi = Real(1.0 / rho0), rhoi2 = Real(rho0 * rho0);
+    const Real Prhoi = Real(ri_[R::PRHO]), mori = Real(ri_[R::MOR]);
     const Real Cl = Real(p.Cl), Cq = Real(p.Cq), eps2 = Real(p.eps2);
 
     Acc<D> a;
@@ -946,6 +941,14 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
     for (int c = 0; c < D; ++c)
       for (int b = 0; b < c; ++b) a.M.c[c * D + b] = a.M.c[b * D + c];
     a.maxVP = double(maxVP);
+    // the node's own state for the epilogue, read again after the loop so it
+    // holds no registers during it
+    Self<D> me;
+    for (int c = 0; c < D; ++c) me.v.c[c] = ri_[R::V + c];
+    const double hinvD = ri_[RL<true>::HINV];
+    for (int c = 0; c < NSym<D>::value; ++c) me.H.c[c] = 0.0;
+    for (int c = 0; c < D; ++c) me.H.c[sdiag<D>(c)] = hinvD;
+    me.Hdet = ri_[R::HDET]; me.m = ri_[R::M]; me.rho = ri_[R::RHO]; me.mor = ri_[R::MOR];
     epilogue<D>(k, len, me, a, t, p, o);
   }
   for (int sh = 32; sh > 0; sh >>= 1) pairs += __shfl_xor(pairs, sh);

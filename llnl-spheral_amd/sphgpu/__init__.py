@@ -208,6 +208,8 @@ def lib():
             "sphgpu_iterate_ideal_h": (C.c_int, [vp, C.POINTER(HydroParams), C.c_int, C.c_double, C.c_double, C.c_int,
                                                  C.POINTER(C.c_int), _dp]),
             "sphgpu_conservation": (C.c_int, [vp, _dp]),
+            "sphgpu_get_dt_inputs": (C.c_int, [vp, C.c_int64, C.c_int64, _dp]),
+            "sphgpu_set_dt_inputs": (C.c_int, [vp, C.c_int64, C.c_int64, _dp]),
             "sphgpu_ensemble_conservation": (C.c_int, [vp, C.c_int64, _dp]),
             "sphgpu_ensemble_begin": (C.c_int, [vp]),
             "sphgpu_ensemble_add": (C.c_int, [vp, C.POINTER(Member), C.POINTER(IntegratorParams), _i64p]),
@@ -448,6 +450,17 @@ class Sim:
         else:
             self._chk(self.L.sphgpu_ensemble_conservation(self.h, member, _p(out)))
         return out
+
+    def dt_inputs(self, first=0, rows=None):
+        """The dt vote's inputs of rows [first, first + rows): (rows, 2 + dim)."""
+        n = (self.node_counts()[0] - first) if rows is None else rows
+        a = np.zeros((n, 2 + self.dim))
+        self._chk(self.L.sphgpu_get_dt_inputs(self.h, first, n, _p(a)))
+        return a
+
+    def set_dt_inputs(self, a, first=0):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(self.L.sphgpu_set_dt_inputs(self.h, first, a.shape[0], _p(a)))
 
     def clock(self):
         t, ldt, cyc = C.c_double(), C.c_double(), C.c_int64()

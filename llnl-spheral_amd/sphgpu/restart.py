@@ -5,9 +5,11 @@ hold every registered object's dumpState: the NodeList fields and the
 integrator's lastDt, currentTime and currentCycle
 (src/Integrator/Integrator.cc:794-823).  A snapshot here holds the same state
 of the internal nodes (position, velocity, mass, mass density, specific
-thermal energy, H, the grad-h correction) and the clock, in a numpy .npz file
-(loaded without pickles); the derived fields (pressure, sound speed, ghosts,
-neighbours) are remade on restore and by the next step.
+thermal energy, H, the grad-h correction), the clock and the derivatives the
+next dt vote reads (the reference dumps its derivative fields too,
+src/SPH/SPHHydroBase.cc:1283-1318), in a numpy .npz file (loaded without
+pickles); the derived fields (pressure, sound speed, ghosts, neighbours) are
+remade on restore and by the next step.
 """
 import numpy as np
 
@@ -23,11 +25,13 @@ def save(sim, path, member=None, gamma=None, t_end=None):
         n = sim.node_counts()[0]
         for k, f in FIELDS.items():
             out[k] = sim.download(f, rows=n)
+        out["dtin"] = sim.dt_inputs(0, n)
         t, ldt, cyc = sim.clock()
     else:
         for k, f in FIELDS.items():
             out[k] = sim.ensemble_download(member, f)
         mb = [m for m in sim.ensemble_members() if m["id"] == member][0]
+        out["dtin"] = sim.dt_inputs(mb["first"], mb["rows"])
         t, ldt, cyc = mb["t"], mb["last_dt"], mb["cycle"]
         out["gamma"] = np.float64(gamma)
         out["t_end"] = np.float64(t_end)
@@ -50,6 +54,8 @@ def restore(sim, path, gamma, Pmin=-1.0e200, Pmax=1.0e200, floor=True):
     sim.eos_gamma_law(gamma, Pmin, Pmax, floor)
     t, ldt, cyc = z["clock"]
     sim.set_clock(float(t), float(ldt), int(cyc))
+    if int(cyc) > 0:
+        sim.set_dt_inputs(z["dtin"], 0)
 
 
 def restore_member(sim, path, integ=None):
@@ -58,4 +64,7 @@ def restore_member(sim, path, integ=None):
     mid = sim.ensemble_add({k: z[k] for k in FIELDS}, float(z["gamma"]), float(z["t_end"]), integ)
     t, ldt, cyc = z["clock"]
     sim.ensemble_set_clock(mid, float(t), float(ldt), int(cyc))
+    if int(cyc) > 0:
+        mb = [m for m in sim.ensemble_members() if m["id"] == mid][0]
+        sim.set_dt_inputs(z["dtin"], mb["first"])
     return mid
