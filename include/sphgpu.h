@@ -351,8 +351,9 @@ int sphgpu_ensemble_step(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, cons
  * length cap, any may be null); *count = number of members. */
 int sphgpu_ensemble_members(sphgpu_sim* sim, int cap, int64_t* ids, double* time, double* last_dt, int64_t* cycle,
                             int64_t* first_row, int64_t* rows, int* count);
-/* A member's internal rows of one state field (SPHGPU_F_*). */
+/* A member's internal rows of one state field (SPHGPU_F_*), read or written. */
 int sphgpu_ensemble_download(sphgpu_sim* sim, int64_t id, int field, double* host);
+int sphgpu_ensemble_upload(sphgpu_sim* sim, int64_t id, int field, const double* host);
 /* Set a member's clock (time, last dt, cycle), e.g. when it is restored from a
  * snapshot (the reference restores the integrator's clock with its restart
  * state, src/Integrator/Integrator.cc:794-823). */

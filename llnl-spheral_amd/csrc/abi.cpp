@@ -594,6 +594,23 @@ int sphgpu_ensemble_set_clock(sphgpu_sim* s, int64_t id, double time, double las
   });
 }
 
+int sphgpu_ensemble_upload(sphgpu_sim* s, int64_t id, int field, const double* host) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->ensemble, SPHGPU_ERR_STATE, "not an ensemble");
+    SG_REQUIRE(host, SPHGPU_ERR_ARG, "null buffer");
+    auto it = std::find_if(s->members.begin(), s->members.end(), [&](const Member& mb) { return mb.id == id; });
+    SG_REQUIRE(it != s->members.end(), SPHGPU_ERR_ARG, "no member with that id");
+    SG_REQUIRE(field >= 0 && field <= SPHGPU_F_OMEGA_GRADH, SPHGPU_ERR_ARG, "unknown state field id");
+    const int comps = stateComponents(s->dim, field);
+    DBuf& b = stateBuf(*s, field);
+    SG_CHECK_HIP(hipMemcpyAsync(b.as<double>() + it->first * comps, host, sizeof(double) * size_t(it->rows) * comps,
+                                hipMemcpyHostToDevice, s->stream));
+    SG_CHECK_HIP(hipStreamSynchronize(s->stream));
+    s->recValid = false;
+    s->derivsValid = false;
+  });
+}
+
 int sphgpu_ensemble_remove(sphgpu_sim* s, int nIds, const int64_t* ids) {
   return guarded(s, [&] { ensembleRemove(*s, nIds, ids); });
 }

@@ -305,6 +305,7 @@ struct SearchArgs {
   DGrid g;
   double ext, ext2;
   float lo32, hi32;         // ext2*(1 -+ 1e-5): FP32 classification margins
+  float prune2, cell32, invCell32;   // staged path: (search radius (1 + 1e-4))^2, cell edge and its inverse (FP32)
   int64_t n, nInt;
   int cap;
   int32_t* keepSlot;        // optional: set to 1 for every ghost slot some row holds
@@ -404,10 +405,26 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
     const int sx = c[0] - cx0, sy = c[1] - cy0, sz = c[2] - cz0;   // in [2, 5]
     RowWriter row(nbr, a.n, k, a.cap);
     for (int dz = -2; dz <= 2; ++dz) {
+      // distance from the node to the cell slab along z (staged cell t spans
+      // [(t - 2) c, (t - 1) c) in brick-origin offsets)
+      const float zlo = float(sz + dz - 2) * a.cell32, zhi = zlo + a.cell32;
+      const float dzm = fz < zlo ? zlo - fz : (fz > zhi ? fz - zhi : 0.f);
       for (int dy = -2; dy <= 2; ++dy) {
         const int base = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge + (sx - 2);
         if (staged) {
-          const int32_t t0 = sOff[base], t1 = sOff[base + 5];
+          // the cells of this (dz, dy) run that a sphere of the search radius
+          // (with a 1e-4 margin) reaches: farther cells hold no neighbour
+          const float ylo = float(sy + dy - 2) * a.cell32, yhi = ylo + a.cell32;
+          const float dym = fy < ylo ? ylo - fy : (fy > yhi ? fy - yhi : 0.f);
+          const float rem2 = a.prune2 - (dzm * dzm + dym * dym);
+          if (rem2 < 0.f) continue;
+          const float rem = __builtin_sqrtf(rem2);
+          int x0 = int(floorf((fx - rem) * a.invCell32)) + 2, x1 = int(floorf((fx + rem) * a.invCell32)) + 2;
+          x0 = x0 < sx - 2 ? sx - 2 : x0;
+          x1 = x1 > sx + 2 ? sx + 2 : x1;
+          if (x0 > x1) continue;
+          const int rowb = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge;
+          const int32_t t0 = sOff[rowb + x0], t1 = sOff[rowb + x1 + 1];
           for (int32_t t = t0; t < t1; ++t) {
             const float4 cj = sCand[t];
             const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
@@ -934,6 +951,12 @@ void buildNeighborsD(Sim& s, double ext) {
   const double band = std::max(1.0e-5, 4.0 * (1.44e-6 * ratio + 5.0e-7));
   sa.lo32 = float(sa.ext2 * (1.0 - band));
   sa.hi32 = float(sa.ext2 * (1.0 + band));
+  {
+    const double Rm = ext / hb[6] * (1.0 + 1.0e-6) * (1.0 + 1.0e-4);
+    sa.prune2 = float(Rm * Rm);
+    sa.cell32 = float(dg.cell);
+    sa.invCell32 = float(1.0 / dg.cell);
+  }
   sa.n = n;
   sa.nInt = s.nInt;
   sa.keepSlot = nullptr;

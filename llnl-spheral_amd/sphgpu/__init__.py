@@ -218,6 +218,7 @@ def lib():
             "sphgpu_ensemble_members": (C.c_int, [vp, C.c_int, _i64p, _dp, _dp, _i64p, _i64p, _i64p,
                                                   C.POINTER(C.c_int)]),
             "sphgpu_ensemble_download": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
+            "sphgpu_ensemble_upload": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
             "sphgpu_ensemble_remove": (C.c_int, [vp, C.c_int, _i64p]),
             "sphgpu_ensemble_set_clock": (C.c_int, [vp, C.c_int64, C.c_double, C.c_double, C.c_int64]),
             "sphgpu_ensemble_bytes_per_node": (C.c_int64, [C.c_int]),
@@ -526,6 +527,11 @@ class Sim:
 
     def ensemble_set_clock(self, member_id, t, last_dt, cycle):
         self._chk(self.L.sphgpu_ensemble_set_clock(self.h, member_id, t, last_dt, cycle))
+
+    def ensemble_upload(self, member_id, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.size == self._members[member_id] * state_components(field, self.dim), (field, a.shape)
+        self._chk(self.L.sphgpu_ensemble_upload(self.h, member_id, STATE_FIELDS[field], _p(a)))
 
     def ensemble_remove(self, ids):
         ids = np.ascontiguousarray(list(ids), dtype=np.int64)

@@ -5,16 +5,18 @@ hold every registered object's dumpState: the NodeList fields and the
 integrator's lastDt, currentTime and currentCycle
 (src/Integrator/Integrator.cc:794-823).  A snapshot here holds the same state
 of the internal nodes (position, velocity, mass, mass density, specific
-thermal energy, H, the grad-h correction), the clock and the derivatives the
-next dt vote reads (the reference dumps its derivative fields too,
-src/SPH/SPHHydroBase.cc:1283-1318), in a numpy .npz file (loaded without
-pickles); the derived fields (pressure, sound speed, ghosts, neighbours) are
-remade on restore and by the next step.
+thermal energy, H, the grad-h correction, pressure and sound speed, which the
+last update set before the finalize's density sum), the clock and the
+derivatives the next dt vote reads (the reference dumps pressure, sound speed
+and its derivative fields too, src/SPH/SPHHydroBase.cc:1283-1318), in a numpy
+.npz file (loaded without pickles); ghosts and neighbours are remade by the
+next step.
 """
 import numpy as np
 
 FIELDS = {"pos": "position", "vel": "velocity", "mass": "mass", "rho": "mass density",
-          "eps": "specific thermal energy", "H": "H", "omega": "grad h corrections"}
+          "eps": "specific thermal energy", "H": "H", "omega": "grad h corrections",
+          "P": "pressure", "cs": "sound speed"}
 
 
 def save(sim, path, member=None, gamma=None, t_end=None):
@@ -45,13 +47,12 @@ def load(path):
         return {k: z[k] for k in z.files}
 
 
-def restore(sim, path, gamma, Pmin=-1.0e200, Pmax=1.0e200, floor=True):
-    """Restore a plain simulation: n internal nodes, state, EOS and clock."""
+def restore(sim, path):
+    """Restore a plain simulation: n internal nodes, state and clock."""
     z = load(path)
     n = z["mass"].shape[0]
     sim.set_nodes(n, n)
     sim.upload_state({k: z[k] for k in FIELDS})
-    sim.eos_gamma_law(gamma, Pmin, Pmax, floor)
     t, ldt, cyc = z["clock"]
     sim.set_clock(float(t), float(ldt), int(cyc))
     if int(cyc) > 0:
@@ -62,6 +63,8 @@ def restore_member(sim, path, integ=None):
     """Add a snapshot to an ensemble as a member with its clock; returns its id."""
     z = load(path)
     mid = sim.ensemble_add({k: z[k] for k in FIELDS}, float(z["gamma"]), float(z["t_end"]), integ)
+    for k in ("P", "cs", "omega"):
+        sim.ensemble_upload(mid, FIELDS[k], z[k])
     t, ldt, cyc = z["clock"]
     sim.ensemble_set_clock(mid, float(t), float(ldt), int(cyc))
     if int(cyc) > 0:

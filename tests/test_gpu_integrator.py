@@ -307,7 +307,7 @@ def test_restart_snapshot_continues_bitwise(sphgpu, tmp_path):
     b.set_row_order(sphgpu.ROW_ORDER_REFERENCE)
     b.set_table_bspline(1000)
     b.set_boundaries(sphgpu.box_boundaries(2))
-    restart.restore(b, p, 1.4)
+    restart.restore(b, p)
     for _ in range(3):
         b.rk2_step(hp, ip)
     assert a.clock() == b.clock()
