@@ -439,7 +439,7 @@ int sphgpu_download_derivative(sphgpu_sim* s, int field, double* host, int64_t n
 int sphgpu_pair_count(sphgpu_sim* s, int64_t* n) {
   return guarded(s, [&] {
     SG_REQUIRE(n, SPHGPU_ERR_ARG, "null output");
-    SG_REQUIRE(s->derivsValid, SPHGPU_ERR_STATE, "no current derivatives");
+    SG_REQUIRE(s->pairsValid, SPHGPU_ERR_STATE, "no derivative pass on the current neighbour rows");
     SG_CHECK_HIP(hipStreamSynchronize(s->stream));
     *n = int64_t(*s->ncalcHost);
   });

@@ -1267,6 +1267,7 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
   else evalDerivsD<3>(s, p);
   s.timingPending |= 4u;
   s.derivsValid = true;
+  s.pairsValid = true;
 }
 
 int derivComponents(int dim, int f) {

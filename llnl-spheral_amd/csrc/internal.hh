@@ -210,6 +210,7 @@ struct Sim {
   // inside a step): bit 0 neighbours, 1 density, 2 derivatives, 3 ghosts.
   unsigned timingPending = 0;
   unsigned long long* ncalcHost = nullptr;   // pinned; written by the stream after each pass
+  bool pairsValid = false;                   // a pass ran on the current neighbour rows
 };
 void resolveTimings(Sim& s);
 

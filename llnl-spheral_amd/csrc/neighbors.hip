@@ -1024,6 +1024,7 @@ void buildNeighbors(Sim& s, double ext) {
   SG_REQUIRE(ext > 0.0, SPHGPU_ERR_ARG, "kernel extent must be positive");
   s.kernelExtent = ext;
   s.neighborsValid = false;
+  s.pairsValid = false;
   s.derivsValid = false;
   if (s.dim == 1) buildNeighborsD<1>(s, ext);
   else if (s.dim == 2) buildNeighborsD<2>(s, ext);
