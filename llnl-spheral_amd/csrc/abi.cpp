@@ -189,6 +189,8 @@ int sphgpu_set_nodes(sphgpu_sim* s, int64_t nInt, int64_t nTot) {
   return guarded(s, [&] {
     SG_REQUIRE(nInt >= 0 && nTot >= nInt && nTot < (int64_t(1) << 31) - 1, SPHGPU_ERR_ARG, "bad node counts");
     SG_REQUIRE(!s->ensemble, SPHGPU_ERR_STATE, "an ensemble takes its nodes with sphgpu_ensemble_add");
+    // the previous step's dt inputs are rows of the old node set
+    if (nInt != s->nInt) s->haveDtIn = false;
     s->nInt = nInt;
     s->nTot = nTot;
     s->deviceGhosts = false;

@@ -1,0 +1,207 @@
+"""The C++ host layer (llnl-spheral_amd/host/SpheralGPU.hh) against the oracle.
+
+tests/host/spheral_host_check drives SPHHydroBase<Dim> and SynchronousRK2<Dim>
+through the reference's method sequence (initializeProblemStartup,
+registerState, registerDerivatives, initialize, evaluateDerivatives, dt,
+finalize; Integrator::step) and writes what they produced; the same inputs go
+through the oracle.  Bars as for the C-ABI parity tests, in the reference's row
+order: value for value, except the 3-D fields that pass through the cube root
+of the weighted neighbour sum (device and host pow may differ in the last bit,
+held to 1e-14) and, in 3-D, the dt and RK2 state (1e-12 relative).
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from sphgpu import problems
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST_BIN = os.path.join(ROOT, "tests", "host", "_build", "spheral_host_check")
+POW_FIELDS = ("weightedNeighborSum", "Hideal")
+
+
+def write_records(path, recs):
+    with open(path, "wb") as f:
+        for name, arr in recs.items():
+            a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())
+            nb = name.encode()
+            f.write(struct.pack("<i", len(nb)) + nb + struct.pack("<q", a.size) + a.tobytes())
+
+
+def read_records(path):
+    out = {}
+    with open(path, "rb") as f:
+        data = f.read()
+    i = 0
+    while i < len(data):
+        (ln,) = struct.unpack_from("<i", data, i)
+        i += 4
+        name = data[i:i + ln].decode()
+        i += ln
+        (cnt,) = struct.unpack_from("<q", data, i)
+        i += 8
+        out[name] = np.frombuffer(data, dtype=np.float64, count=cnt, offset=i).copy()
+        i += 8 * cnt
+    return out
+
+
+def run_host(mode, recs, tmp_path):
+    if not os.path.exists(HOST_BIN):
+        pytest.fail(f"{HOST_BIN} is not built (make -C tests/host, after llnl-spheral_amd/build.py)")
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_records(fin, recs)
+    r = subprocess.run([HOST_BIN, mode, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return read_records(fout)
+
+
+def box_records(dim, kind):
+    """The unit box of oracle.box_boundaries: 13 doubles per boundary."""
+    rows = []
+    for a in range(dim):
+        e = np.zeros(3)
+        e[a] = 1.0
+        lo, hi = np.zeros(3), np.zeros(3)
+        hi[a] = 1.0
+        if kind == "reflecting":
+            rows.append(np.concatenate([[0.0], lo, e, lo, e]))
+            rows.append(np.concatenate([[0.0], hi, -e, hi, -e]))
+        else:
+            rows.append(np.concatenate([[1.0], lo, e, hi, -e]))
+    return np.concatenate(rows)
+
+
+def state_records(dim, st, n_int, n_tot, **kw):
+    rec = {"dim": [dim], "n_internal": [n_int], "n_total": [n_tot]}
+    for k in ("pos", "vel", "mass", "rho", "eps", "H"):
+        rec[k] = st[k][:n_tot]
+    rec.update({k: [v] if np.isscalar(v) else v for k, v in kw.items()})
+    return rec
+
+
+def derivs_cases():
+    out = []
+    for dim, n in ((1, 200), (2, 24), (3, 10)):
+        st, meta = problems.random_state(dim, n, seed=21 + dim, h_var=0.2)
+        N = meta["n_internal"]
+        out.append((f"nodelist_ghosts{dim}d", dim, st, N - N // 9, None, {}))
+    st, meta = problems.random_state(3, 8, seed=4, h_var=0.1)
+    st["P"] = st["P"] - 1.0
+    out.append(("tensile3d", 3, st, meta["n_internal"], None, dict(epsTensile=0.25)))
+    st, meta = problems.random_state(1, 100, seed=63, jitter=0.1, vel_scale=0.5)
+    out.append(("gradh1d", 1, st, meta["n_internal"] - 10, None, dict(gradh=1)))
+    st, meta = problems.random_state(2, 20, seed=61, jitter=0.1, vel_scale=0.5)
+    out.append(("reflecting2d", 2, st, meta["n_internal"], "reflecting", {}))
+    st, meta = problems.periodic_sinusoid(7)
+    out.append(("periodic3d", 3, st, meta["n_internal"], "periodic", {}))
+    return out
+
+
+DERIVS_CASES = derivs_cases()
+
+
+def close(field, dim, a, b, pow_tol=1e-14):
+    if dim == 3 and field in POW_FIELDS:
+        return np.abs(a - b).max() <= pow_tol * max(np.abs(b).max(), 1e-300)
+    return np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", DERIVS_CASES, ids=[c[0] for c in DERIVS_CASES])
+def test_host_class_derivatives(sphgpu, oracle, case, tmp_path):
+    import integrator
+    name, dim, st, n_int, bkind, kw = case
+    N = st["pos"].shape[0]
+    n_tot = n_int if bkind else N
+    print(f"[host class] derivs {name}", flush=True)
+    rec = state_records(dim, st, n_int, n_tot, gamma=1.4, **kw)
+    rec["P"], rec["cs"] = st["P"][:n_tot], st["cs"][:n_tot]
+    if bkind:
+        rec["boundaries"] = box_records(dim, bkind)
+    got = run_host("derivs", rec, tmp_path)
+
+    # the equation of state of initializeProblemStartup
+    P, cs = oracle.gamma_law(st["rho"][:n_tot], st["eps"][:n_tot], 1.4, floor=True)
+    assert np.array_equal(got["eos P"], P) and np.array_equal(got["eos cs"], cs)
+
+    T = oracle.Table(dim, 1000)
+    base = {k: st[k][:n_tot].copy() for k in ("pos", "vel", "mass", "rho", "eps", "H", "P", "cs")}
+    if bkind:
+        full, off, nbr, info = oracle.ghost_pipeline(dim, base, oracle.box_boundaries(dim, kind=bkind), 2.0, True)
+    else:
+        full = base
+        off, nbr = oracle.connectivity(dim, n_int, full["pos"], full["H"], 2.0)
+        info = None
+    full["omega"] = np.ones(full["pos"].shape[0])
+    if kw.get("gradh"):
+        om = oracle.omega_gradh(T, n_int, full["pos"], full["H"], off, nbr)
+        assert np.array_equal(got["omega"], om)
+        full["omega"][:n_int] = om
+    params = oracle.make_params(epsTensile=kw.get("epsTensile", 0.0))
+    ref = oracle.eval_derivs(T, params, n_int, full, off, nbr)
+    bad = []
+    for cname, f in zip(sphgpu.DERIV_FIELDS, oracle.OUTPUT_FIELDS):
+        a = got["derivs/" + cname]
+        b = ref[f].ravel()
+        if not close(f, dim, a, b):
+            bad.append((f, float(np.abs(a - b).max())))
+    assert not bad, bad
+    assert int(got["pair count"][0]) == int(ref["ncalc"])
+
+    # GenericHydro::dt with the derivatives just computed
+    prev = {"maxViscousPressure": ref["maxViscousPressure"], "DvDx": ref["DvDx"].reshape(n_int, -1),
+            "DvDt": ref["DvDt"].reshape(n_int, -1)}
+    dt = integrator.hydro_dt(dim, n_int, full, prev, off, nbr, 1.51, 0.25)
+    if dim < 3:
+        assert got["dt"][0] == dt
+    else:
+        assert got["dt"][0] == pytest.approx(dt, rel=1e-12)
+
+    # SPHHydroBase::finalize: the summed density replaces the state's
+    rho = oracle.sum_density(T, n_int, full["pos"], full["mass"], full["H"], off, nbr)
+    assert np.array_equal(got["finalized rho"], rho)
+
+
+def rk2_cases():
+    tube = dict(seed=61, jitter=0.1, vel_scale=0.5)
+    return [
+        ("converging3d", 3, problems.converging_flow(8)[0], "reflecting", 5.0 / 3.0, 4, {}),
+        ("periodic2d", 2, problems.periodic_sinusoid(16, dim=2, sigma=0.3)[0], "periodic", 1.4, 5, {}),
+        ("reflect1d", 1, problems.random_state(1, 100, **tube)[0], "reflecting", 1.4, 5, {}),
+        ("compatible1d", 1, problems.random_state(1, 100, **tube)[0], "reflecting", 1.4, 4, dict(compatible=1)),
+        ("gradh1d", 1, problems.random_state(1, 100, seed=63, jitter=0.1, vel_scale=0.5)[0], "reflecting", 1.4, 4,
+         dict(gradh=1)),
+    ]
+
+
+RK2_CASES = rk2_cases()
+
+
+@pytest.mark.parametrize("case", RK2_CASES, ids=[c[0] for c in RK2_CASES])
+def test_host_class_synchronous_rk2(oracle, case, tmp_path):
+    import integrator
+    name, dim, st, kind, gamma, steps, kw = case
+    N = st["pos"].shape[0]
+    print(f"[host class] rk2 {name}", flush=True)
+    rec = state_records(dim, st, N, N, gamma=gamma, steps=steps, boundaries=box_records(dim, kind), **kw)
+    got = run_host("rk2", rec, tmp_path)
+    ref = integrator.RK2(dim, {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")},
+                         oracle.box_boundaries(dim, kind=kind), gamma, compatible=bool(kw.get("compatible")),
+                         gradh=bool(kw.get("gradh")))
+    dts = [ref.step() for _ in range(steps)]
+    if dim < 3:
+        assert list(got["dt"]) == dts
+    else:
+        assert np.allclose(got["dt"], dts, rtol=1e-12, atol=0)
+    assert got["time"][0] == pytest.approx(ref.t, rel=1e-14)
+    for key in ("pos", "vel", "rho", "eps", "H", "P", "cs"):
+        a = got[key].reshape(ref.st[key].shape)
+        b = ref.st[key]
+        if dim < 3:
+            assert np.array_equal(a, b), key
+        else:
+            assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), key
