@@ -289,76 +289,6 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
   ri_[R::MOR] = mi / acc;
 }
 
-// Isotropic H: the same sum with records two neighbours ahead (seven doubles
-// each: x, 1/h, m, det H, caller index), within the register budget of eight
-// wavefronts per SIMD.
-struct DIso {
-  double x[3], hinv, m, hdet, orig;
-};
-__device__ __forceinline__ DIso loadDIso(const double* __restrict__ rec, int32_t j) {
-  using R = RL<true>;
-  const double* r = rec + int64_t(j) * R::STRIDE;
-  const double4 a = *reinterpret_cast<const double4*>(r);          // x y z 1/h
-  const double2 b = *reinterpret_cast<const double2*>(r + 4);      // m det H
-  DIso o;
-  o.x[0] = a.x; o.x[1] = a.y; o.x[2] = a.z; o.hinv = a.w;
-  o.m = b.x; o.hdet = b.y; o.orig = r[R::ORIG];
-  return o;
-}
-
-template <int D, bool LDS>
-__global__ void __launch_bounds__(kBlock) k_sum_density_iso(double* __restrict__ rec,
-                                                            const int32_t* __restrict__ rowCount,
-                                                            const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
-                                                            const int32_t* __restrict__ work, TabArgs t,
-                                                            double* __restrict__ rhoOut) {
-  using R = RL<true>;
-  extern __shared__ double lds[];
-  const double* tab = stageTable<LDS>(lds, t);
-  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
-  if (tw >= nInt) return;
-  const int64_t k = work[tw];
-  double* ri_ = rec + k * R::STRIDE;
-  const DIso me = loadDIso(rec, int32_t(k));
-  const double oi = me.orig, mi = me.m;
-  if (oi >= double(nInt)) return;
-  const double self = mi * lookupW(tab, t, 0.0, me.hdet);
-  double acc = 0.0;
-  bool selfDone = false;
-  const int len = rowCount[k];
-  DIso n1 = me, n2 = me;
-  if (len > 0) n1 = loadDIso(rec, nbr[ellIndex(0, n, k)]);
-  if (len > 1) n2 = loadDIso(rec, nbr[ellIndex(1, n, k)]);
-  for (int w = 0; w < len; ++w) {
-    const DIso nb = n1;
-    n1 = n2;
-    if (w + 2 < len) n2 = loadDIso(rec, nbr[ellIndex(w + 2, n, k)]);
-    if (!selfDone && nb.orig > oi) { acc += self; selfDone = true; }
-    // |H_j r_ij| with H_j = h_j I: h_j r componentwise, magnitude left to right
-    const double h0 = nb.hinv * (me.x[0] - nb.x[0]);
-    double etaj;
-    if constexpr (D == 1) {
-      etaj = dabs(h0);
-    } else {
-      double e2 = h0 * h0;
-      const double h1 = nb.hinv * (me.x[1] - nb.x[1]);
-      e2 = e2 + h1 * h1;
-      if constexpr (D == 3) {
-        const double h2 = nb.hinv * (me.x[2] - nb.x[2]);
-        e2 = e2 + h2 * h2;
-      }
-      etaj = __builtin_sqrt(e2);
-    }
-    acc += nb.m * lookupW(tab, t, etaj, nb.hdet);
-  }
-  if (!selfDone) acc += self;
-  rhoOut[int64_t(oi)] = acc;
-  const double som = ri_[R::SOM], p = ri_[R::P];
-  ri_[R::RHO] = acc;
-  ri_[R::PRHO] = som * p / (acc * acc);
-  ri_[R::MOR] = mi / acc;
-}
-
 // ---------------------------------------------------------------------------
 // computeSPHOmegaGradhCorrection (src/SPH/computeSPHOmegaGradhCorrection.cc:26-133)
 // Gather form of the owner loop: node i receives W(|H_i r_ij|) det H_i and
