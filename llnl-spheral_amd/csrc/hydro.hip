@@ -870,16 +870,8 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
     const int4* nb4 = reinterpret_cast<const int4*>(nbr);
     int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
     int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
-    INbr<Real> nx;
-    int32_t jn = cur.x;
-    if (len > 0) loadIso(recN, jn, nx);
-    for (int w = 0; w < len; ++w) {
-      const INbr<Real> nb = nx;
-      const int32_t j = jn;
-      if (w + 1 < len) {
-        jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
-        loadIso(recN, jn, nx);
-      }
+    // One neighbour's contribution, in row order.
+    auto pair = [&](const INbr<Real>& nb, int32_t j) {
       pairs += (nb.ghost || j > int32_t(k)) ? 1u : 0u;
       Real rij[3], vij[3];
       Real r2 = Real(0), vdotr = Real(0);
@@ -937,6 +929,18 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
         a.xsphW += double(w2);
         for (int c = 0; c < D; ++c) a.xsphDV.c[c] -= double(vij[c] * w2);
       }
+    };
+    INbr<Real> nx;
+    int32_t jn = cur.x;
+    if (len > 0) loadIso(recN, jn, nx);
+    for (int w = 0; w < len; ++w) {
+      const INbr<Real> nb = nx;
+      const int32_t j = jn;
+      if (w + 1 < len) {
+        jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
+        loadIso(recN, jn, nx);
+      }
+      pair(nb, j);
     }
     for (int c = 0; c < D; ++c)
       for (int b = 0; b < c; ++b) a.M.c[c * D + b] = a.M.c[b * D + c];

@@ -1,11 +1,13 @@
-"""Size-independent properties at the benchmark size (128^3 converging flow),
-where the oracle is too slow to run in full:
+"""Checks at the benchmark size (128^3 converging flow).  Properties that need
+no oracle run:
   * neighbour rows: sampled rows equal the oracle's exact rows for those nodes
     (checked by brute force over the sampled nodes' neighbourhoods);
   * interior rows hold 122 +- jitter-crossings entries; the CSR is symmetric;
   * sum_i m_i DvDt_i = 0 and sum_i m_i (v.DvDt + DepsDt) = 0 to roundoff (no
     ghosts: every pair term is antisymmetric);
-  * two identical passes give bitwise identical results (determinism).
+  * two identical passes give bitwise identical results (determinism);
+and the measured pass itself (ghosts, culling, sum density, evaluateDerivatives)
+against the oracle's gather form, which its OpenMP loops finish in seconds.
 """
 import numpy as np
 import pytest
@@ -75,3 +77,74 @@ def test_fullsize_deterministic(big, sphgpu):
     b = s.derivatives()
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+POW_FIELDS = ("weightedNeighborSum", "Hideal")
+
+
+@pytest.mark.parametrize("order", [0, 1], ids=["refo", "searcho"])
+def test_fullsize_pass_matches_oracle(sphgpu, oracle, order):
+    """The measured pass itself at its full size: 128^3 converging flow, the six
+    reflecting planes, ghosts made and culled on the device, sum density with
+    the ghost density refresh, then evaluateDerivatives.  The oracle runs the
+    same pipeline (its ghost restatement, exact connectivity, and the gather
+    form of the sum and the pass, per node in index order).  Reference row
+    order: ghost rows, controls and the neighbour CSR byte-identical, density
+    and every derivative field equal value for value (cube-root fields
+    1e-14).  Search row order: 1e-12 relative to each field's largest
+    magnitude."""
+    n = 128
+    st, meta = problems.converging_flow(n)
+    N = meta["n_internal"]
+    print(f"[gpu] fullsize pass N={N} order={order}", flush=True)
+    s = sphgpu.Sim(3)
+    s.set_row_order(order)
+    s.set_table_bspline(1000)
+    s.set_nodes(N, N)
+    s.upload_state(st)
+    s.set_boundaries(sphgpu.box_boundaries(3, kind="reflecting"))
+    s.set_ghost_nodes(2.0, cull=True)
+    s.sum_density()
+    s.evaluate_derivatives(sphgpu.hydro_params())
+
+    bo = oracle.box_boundaries(3, kind="reflecting")
+    full, off, nbr, info = oracle.ghost_pipeline(3, st, bo)
+    n_int, n_tot = s.node_counts()
+    assert n_int == N and n_tot == full["pos"].shape[0]
+    if order == 0:
+        control, sub = s.ghost_controls()
+        assert np.array_equal(control, info["control"]) and np.array_equal(sub, info["sub_of"])
+        o2, n2 = s.neighbors()
+        assert np.array_equal(o2, off) and np.array_equal(n2, nbr)
+        del o2, n2
+    T = oracle.Table(3, 1000)
+    rho = oracle.sum_density(T, N, full["pos"], full["mass"], full["H"], off, nbr, gather=True)
+    rho_all = np.array(full["rho"])
+    rho_all[:N] = rho
+    rho_all = oracle.apply_ghosts(3, N, bo, info["control"], info["sub_of"], rho_all, 0)
+    got = s.download("mass density")
+    if order == 0:
+        assert np.array_equal(got, rho_all)
+    else:
+        assert np.abs(got - rho_all).max() <= 1e-12 * np.abs(rho_all).max()
+    full["rho"] = rho_all
+    ref = oracle.eval_derivs(T, oracle.make_params(), N, full, off, nbr, gather=True)
+    bad = []
+    for f in sphgpu.DERIV_ALIASES:
+        a = s.derivative(f).reshape(ref[f].shape)
+        b = ref[f]
+        scale = max(np.abs(b).max(), 1e-300)
+        err = float(np.abs(a - b).max() / scale)
+        if order == 1:
+            ok = err <= 1e-12
+        elif f in POW_FIELDS:
+            ok = err <= 1e-14
+        else:
+            ok = np.array_equal(a, b)
+        if not ok:
+            bad.append((f, err))
+    assert not bad, bad
+    # unordered pairs (the reference's ncalc): each internal pair once, each
+    # internal-ghost entry once
+    rows = np.repeat(np.arange(N, dtype=np.int32), np.diff(off))
+    assert s.pair_count() == int(np.count_nonzero((nbr >= N) | (nbr > rows)))
