@@ -138,6 +138,10 @@ void sphgpu_destroy(sphgpu_sim* s) {
   for (DBuf* b : bufs) b->release();
   for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
   if (s->ncalcHost) (void)hipHostFree(s->ncalcHost);
+  if (s->pinnedOut) (void)hipHostFree(s->pinnedOut);
+  if (s->pinnedIn) (void)hipHostFree(s->pinnedIn);
+  if (s->pinnedInDone) (void)hipEventDestroy(s->pinnedInDone);
+  s->pick.release();
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }

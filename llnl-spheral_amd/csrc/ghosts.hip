@@ -507,9 +507,10 @@ template <int D> void generateD(Sim& s, double ext) {
     hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), segp, n, d, ext,
                        hbits, s.keep.as<int32_t>());
     exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), n, s);
-    int32_t count = 0;
-    SG_CHECK_HIP(hipMemcpyAsync(&count, s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    int32_t* hc = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t)));
+    SG_CHECK_HIP(hipMemcpyAsync(hc, s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
+    const int32_t count = *hc;
     SG_REQUIRE(n + int64_t(count) < (int64_t(1) << 31) - 1, SPHGPU_ERR_CAPACITY, "too many ghost nodes");
     if (count == 0) continue;
     ensureRows(s, n + count);
@@ -541,13 +542,23 @@ void cull(Sim& s) {
                      s.ghostControl.as<int32_t>());
   exclusiveScanI32(keep, s.keepScan.as<int32_t>(), n, s);
   // new total and the new first ghost of every sub-boundary
-  std::vector<int32_t> starts(s.subStart.size() + 1);
-  for (size_t b = 0; b < s.subStart.size(); ++b)
-    SG_CHECK_HIP(hipMemcpyAsync(&starts[b], s.keepScan.as<int32_t>() + nInt + s.subStart[b], sizeof(int32_t),
-                                hipMemcpyDeviceToHost, st));
-  SG_CHECK_HIP(hipMemcpyAsync(&starts.back(), s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost,
-                              st));
-  SG_CHECK_HIP(hipStreamSynchronize(st));
+  const size_t nsub = s.subStart.size();
+  std::vector<int32_t> starts(nsub + 1);
+  if (nsub < 16) {
+    int64_t at[16];
+    for (size_t b = 0; b < nsub; ++b) at[b] = nInt + s.subStart[b];
+    at[nsub] = n;
+    const int32_t* h = readInts(s, s.keepScan.as<int32_t>(), at, int(nsub + 1));
+    std::copy(h, h + nsub + 1, starts.begin());
+  } else {
+    int32_t* h = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t) * (nsub + 1)));
+    for (size_t b = 0; b < nsub; ++b)
+      SG_CHECK_HIP(hipMemcpyAsync(h + b, s.keepScan.as<int32_t>() + nInt + s.subStart[b], sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, st));
+    SG_CHECK_HIP(hipMemcpyAsync(h + nsub, s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SG_CHECK_HIP(hipStreamSynchronize(st));
+    std::copy(h, h + nsub + 1, starts.begin());
+  }
   const int64_t nNew = starts.back();
   if (nNew == n) return;
   const int32_t* scan = s.keepScan.as<int32_t>();

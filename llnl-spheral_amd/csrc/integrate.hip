@@ -74,12 +74,16 @@ struct Bounds {
 };
 // Ensemble members: the update's multiplier (dt or dt/2) and gamma per member.
 struct SegArgs {
-  const double* seg;        // null: the scalar multiplier and EOS apply to every row
-  const SegState* st;
+  const double* seg;        // member of each row; null: one simulation
+  const SegState* st;       // null: the scalar multiplier and EOS apply to every row
   int half;                 // 1: multiply by dt/2
 };
 __device__ __forceinline__ void rowMultEos(const SegArgs& sa, int64_t i, double& mult, Eos& e) {
-  if (!sa.seg) return;
+  if (!sa.st) return;
+  if (!sa.seg) {            // one simulation whose dt was chosen on the device
+    mult = sa.half ? sa.st->hdt : sa.st->dt;
+    return;
+  }
   const SegState& g = sa.st[int(sa.seg[i])];
   mult = sa.half ? g.hdt : g.dt;
   e.gamma = g.gamma;
@@ -382,6 +386,21 @@ __global__ void k_seg_advance(SegState* __restrict__ st, int n) {
   g.cycle = g.cycle + 1.0;
 }
 
+// Several device copies in one dispatch (the step's saved state of a small
+// node set: five launches cost more than the bytes).
+struct CopySet {
+  double* d[5];
+  const double* s[5];
+  int64_t n[5];
+  int m;
+};
+constexpr size_t kCopyManyMaxBytes = size_t(32) << 20;
+__global__ void k_copy_many(CopySet c) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int q = 0; q < c.m; ++q)
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < c.n[q]; i += stride) c.d[q][i] = c.s[q][i];
+}
+
 // Integrator::preStepInitialize: ghosts and connectivity
 void preStep(Sim& s, const sphgpu_integrator_params& ip) {
   if (!s.subs.empty()) setGhostNodes(s, ip.kernelExtent, ip.cullGhostNodes != 0);
@@ -410,9 +429,24 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
   const size_t bytes[5] = {sizeof(double) * n * D, sizeof(double) * nv * D, sizeof(double) * n, sizeof(double) * n,
                            sizeof(double) * n * NS};
   DBuf* src[5] = {&s.pos, &s.vel, &s.eps, &s.rho, &s.H};
+  CopySet cs{};
+  size_t total = 0;
   for (int q = 0; q < 5; ++q) {
     s.state0[q].ensure(std::max<size_t>(bytes[q], 8));
-    if (bytes[q]) SG_CHECK_HIP(hipMemcpyAsync(s.state0[q].p, src[q]->p, bytes[q], hipMemcpyDeviceToDevice, st));
+    cs.d[q] = s.state0[q].as<double>();
+    cs.s[q] = src[q]->as<double>();
+    cs.n[q] = int64_t(bytes[q] / sizeof(double));
+    total += bytes[q];
+  }
+  cs.m = 5;
+  if (total <= kCopyManyMaxBytes) {
+    // one dispatch for the five copies of a small node set
+    hipLaunchKernelGGL(k_copy_many, dim3(nblocks(std::max<int64_t>(1, int64_t(total / sizeof(double) / 5)))),
+                       dim3(kBlock), 0, st, cs);
+    SG_CHECK_HIP(hipGetLastError());
+  } else {
+    for (int q = 0; q < 5; ++q)
+      if (bytes[q]) SG_CHECK_HIP(hipMemcpyAsync(s.state0[q].p, src[q]->p, bytes[q], hipMemcpyDeviceToDevice, st));
   }
   const Base b0{s.state0[0].as<double>(), s.state0[1].as<double>(), s.state0[2].as<double>(), s.state0[3].as<double>()};
   // trial advance to the midpoint
@@ -433,25 +467,52 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
   enforceBoundaries(s);
 }
 
+// The clock of a plain simulation as one SegState, so that selectDt runs on
+// the device (k_seg_select_dt) and the step needs no host round trip before
+// its end.
+__global__ void k_plain_seg_init(SegState* __restrict__ g, double t, double lastDt, double tEnd, double gamma,
+                                 unsigned long long vote) {
+  g->t = t;
+  g->lastDt = lastDt;
+  g->tEnd = tEnd;
+  g->gamma = gamma;
+  g->dt = 0.0;
+  g->hdt = 0.0;
+  g->voteBits = vote;
+  g->cycle = 0.0;
+}
+
 template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
                              double maxTime) {
-  const double t = s.time;
+  hipStream_t st = s.stream;
   preStep(s, ip);
-  // selectDt
-  const double dtMin = std::min(ip.dtMin, maxTime - t), dtMax = std::min(ip.dtMax, maxTime - t);
-  const double vote = hydroDtD<D>(s, hp.nPerh, ip.cfl, ip.useVelocityMagnitudeForDt);
-  double dt = dtMax;
-  if (vote > 0.0 && vote < dt) dt = vote;
-  dt = std::min(dt, ip.dtGrowth * s.lastDt);
-  dt = std::min(dtMax, std::max(dtMin, dt));
-  const SegArgs none{nullptr, nullptr, 0};
-  stepBody<D>(s, hp, ip, dt, none, none);
-  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
-  s.time = t + dt;
-  s.lastDt = dt;
+  // GenericHydro::dt, then selectDt (dtMin/dtMax clamped to maxTime - t, dtGrowth)
+  s.plainSeg.ensure(sizeof(SegState));
+  SegState* sd = s.plainSeg.as<SegState>();
+  const double init = double(FLT_MAX);
+  unsigned long long ib;
+  std::memcpy(&ib, &init, sizeof(ib));
+  hipLaunchKernelGGL(k_plain_seg_init, dim3(1), dim3(1), 0, st, sd, s.time, s.lastDt, maxTime, ip.gamma, ib);
+  if (!s.haveDtIn) s.dtIn.ensure(sizeof(double) * 8);
+  hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
+                     s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
+                     s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(), s.dtIn.as<double>(),
+                     s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt, nullptr, &sd->voteBits, int64_t(0));
+  hipLaunchKernelGGL(k_seg_select_dt, dim3(1), dim3(kBlock), 0, st, sd, 1, ip.cfl, ip.dtMin, ip.dtMax, ip.dtGrowth);
+  SG_CHECK_HIP(hipGetLastError());
+  const SegArgs half{nullptr, sd, 1}, full{nullptr, sd, 0};
+  stepBody<D>(s, hp, ip, 0.0, half, full);
+  hipLaunchKernelGGL(k_seg_advance, dim3(1), dim3(kBlock), 0, st, sd, 1);
+  SG_CHECK_HIP(hipGetLastError());
+  SegState* hp2 = static_cast<SegState*>(hostStaging(s, 0, sizeof(SegState)));
+  SG_CHECK_HIP(hipMemcpyAsync(hp2, sd, sizeof(SegState), hipMemcpyDeviceToHost, st));
+  SG_CHECK_HIP(hipStreamSynchronize(st));
+  const SegState h = *hp2;
+  s.time = h.t;
+  s.lastDt = h.lastDt;
   s.cycle += 1;
   s.derivsValid = false;
-  return dt;
+  return h.dt;
 }
 
 // One SynchronousRK2 step of every ensemble member, each with its own dt.
@@ -478,9 +539,10 @@ template <int D> void ensembleRk2D(Sim& s, const sphgpu_hydro_params& hp, const 
   stepBody<D>(s, hp, ip, 0.0, half, full);
   hipLaunchKernelGGL(k_seg_advance, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns);
   SG_CHECK_HIP(hipGetLastError());
-  s.segHost.resize(size_t(ns));
-  SG_CHECK_HIP(hipMemcpyAsync(s.segHost.data(), sd, sizeof(SegState) * size_t(ns), hipMemcpyDeviceToHost, st));
+  SegState* hs = static_cast<SegState*>(hostStaging(s, 0, sizeof(SegState) * size_t(ns)));
+  SG_CHECK_HIP(hipMemcpyAsync(hs, sd, sizeof(SegState) * size_t(ns), hipMemcpyDeviceToHost, st));
   SG_CHECK_HIP(hipStreamSynchronize(st));
+  s.segHost.assign(hs, hs + ns);
   s.derivsValid = false;
 }
 

@@ -199,6 +199,7 @@ struct Sim {
   DBuf segBoundsBits;            // per member: 9 ordered-bit bounds of the neighbour build
   DBuf segHmaxBits;              // per member: ghost-selection hmax bits
   DBuf segState;                 // SegState per member
+  DBuf plainSeg;                 // SegState of a plain simulation's step (device-side dt)
   std::vector<SegState> segHost;
   std::vector<Member> members;
   int64_t nextMemberId = 0;
@@ -210,6 +211,15 @@ struct Sim {
   // inside a step): bit 0 neighbours, 1 density, 2 derivatives, 3 ghosts.
   unsigned timingPending = 0;
   unsigned long long* ncalcHost = nullptr;   // pinned; written by the stream after each pass
+  // Pinned host staging: small readbacks (copies into pageable memory would
+  // each wait for the stream) and uploads the stream reads after the call
+  // returns (the grid lists of a neighbour build).  Grown only after a sync.
+  void* pinnedOut = nullptr;
+  size_t pinnedOutBytes = 0;
+  void* pinnedIn = nullptr;
+  size_t pinnedInBytes = 0;
+  hipEvent_t pinnedInDone = nullptr;   // recorded after the copies that read pinnedIn
+  DBuf pick;                     // device side of gathered readback values
   bool pairsValid = false;                   // a pass ran on the current neighbour rows
 };
 void resolveTimings(Sim& s);
@@ -259,6 +269,23 @@ int ensembleStep(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_
 void ensembleRk2(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip);
 void eosRows(Sim& s, int64_t first, int64_t rows, double gamma, double Pmin, double Pmax, int floorType);
 // scans (neighbors.hip)
+// Up to four int32 buffers zeroed by one kernel (a graph-safe memset that
+// also saves dispatches).
+// Pinned staging of at least `bytes`: which = 0 readbacks, 1 uploads.
+void* hostStaging(Sim& s, int which, size_t bytes);
+// Gathers up to 16 int32 values of a device array into s.pick and copies them
+// to pinned host memory, one round trip: returns the host copy.
+const int32_t* readInts(Sim& s, const int32_t* src, const int64_t* at, int m);
+
+struct ZeroSet {
+  int32_t* p[4];
+  int64_t n[4];
+  int m;
+  void add(int32_t* ptr, int64_t count) {
+    if (m < 4 && count > 0) { p[m] = ptr; n[m] = count; ++m; }
+  }
+};
+void zeroMany(const ZeroSet& z, hipStream_t st);
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);
 

@@ -675,6 +675,23 @@ __global__ void __launch_bounds__(kBlock) k_row_sort(const int32_t* __restrict__
   }
 }
 
+// ---- gathering a few int32 values for one readback -------------------------
+struct Pick32 {
+  int64_t at[16];
+  int m;
+};
+__global__ void k_pick_i32(const int32_t* __restrict__ src, Pick32 p, int32_t* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q < p.m) out[q] = src[p.at[q]];
+}
+
+// ---- zeroing several int32 buffers in one dispatch ------------------------
+__global__ void k_zero_many(ZeroSet z) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int q = 0; q < z.m; ++q)
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < z.n[q]; i += stride) z.p[q][i] = 0;
+}
+
 // ---- exclusive scans -------------------------------------------------------
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -737,9 +754,55 @@ __global__ void k_scan_add(TO* __restrict__ out, const TO* __restrict__ sums, in
   if (i == n) out[n] = *total;
 }
 
+// Small scans in one workgroup: tiles of kScanOneTile items in sequence with a
+// running carry (one dispatch instead of three; small simulations run several
+// scans per step and are bound by dispatch latency).
+constexpr int kScanOneBlock = 1024;
+constexpr int kScanOneTile = kScanOneBlock * kScanItems;
+constexpr int64_t kScanOneMax = 4 * int64_t(kScanOneTile);
+
+template <class TI, class TO>
+__global__ void __launch_bounds__(kScanOneBlock) k_scan_one(const TI* __restrict__ in, TO* __restrict__ out,
+                                                            int64_t n) {
+  __shared__ TO sh[kScanOneBlock];
+  TO carry = 0;
+  for (int64_t tile = 0; tile < n; tile += kScanOneTile) {
+    const int64_t base = tile + int64_t(threadIdx.x) * kScanItems;
+    TO v[kScanItems];
+    TO local = 0;
+    for (int q = 0; q < kScanItems; ++q) {
+      const int64_t i = base + q;
+      v[q] = (i < n) ? TO(in[i]) : TO(0);
+      local += v[q];
+    }
+    sh[threadIdx.x] = local;
+    __syncthreads();
+    for (int s = 1; s < kScanOneBlock; s <<= 1) {
+      const TO t = (threadIdx.x >= s) ? sh[threadIdx.x - s] : TO(0);
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    TO run = carry + sh[threadIdx.x] - local;
+    for (int q = 0; q < kScanItems; ++q) {
+      const int64_t i = base + q;
+      if (i < n) out[i] = run;
+      run += v[q];
+    }
+    carry += sh[kScanOneBlock - 1];
+    __syncthreads();                               // sh is rewritten by the next tile
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
 template <class TI, class TO>
 void exclusiveScan(const TI* in, TO* out, int64_t n, Sim& s) {
   // out has n+1 entries; out[n] = total.
+  if (n <= kScanOneMax) {
+    hipLaunchKernelGGL((k_scan_one<TI, TO>), dim3(1), dim3(kScanOneBlock), 0, s.stream, in, out, n);
+    SG_CHECK_HIP(hipGetLastError());
+    return;
+  }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   s.scanTmp.ensure(sizeof(TO) * size_t(tiles + 2));
   TO* sums = s.scanTmp.as<TO>();
@@ -840,10 +903,19 @@ int64_t uploadGrids(Sim& s, const std::vector<DGrid>& grids, const std::vector<i
   s.segGrid.ensure(sizeof(DGrid) * ng);
   s.segCellBase.ensure(sizeof(int64_t) * (ng + 1));
   s.gridFirst.ensure(sizeof(int32_t) * first.size());
-  SG_CHECK_HIP(hipMemcpyAsync(s.segGrid.p, grids.data(), sizeof(DGrid) * ng, hipMemcpyHostToDevice, st));
-  SG_CHECK_HIP(hipMemcpyAsync(s.segCellBase.p, base.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st));
-  SG_CHECK_HIP(hipMemcpyAsync(s.gridFirst.p, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice, st));
-  SG_CHECK_HIP(hipStreamSynchronize(st));   // the host vectors must outlive the copies
+  // staged in pinned memory; the next use waits for pinnedInDone, so the
+  // copies need no wait here
+  const size_t bG = sizeof(DGrid) * ng, bB = sizeof(int64_t) * (ng + 1), bF = sizeof(int32_t) * first.size();
+  const size_t oB = (bG + 63) / 64 * 64, oF = oB + (bB + 63) / 64 * 64;
+  char* h = static_cast<char*>(hostStaging(s, 1, oF + bF));
+  std::memcpy(h, grids.data(), bG);
+  std::memcpy(h + oB, base.data(), bB);
+  std::memcpy(h + oF, first.data(), bF);
+  SG_CHECK_HIP(hipMemcpyAsync(s.segGrid.p, h, bG, hipMemcpyHostToDevice, st));
+  SG_CHECK_HIP(hipMemcpyAsync(s.segCellBase.p, h + oB, bB, hipMemcpyHostToDevice, st));
+  SG_CHECK_HIP(hipMemcpyAsync(s.gridFirst.p, h + oF, bF, hipMemcpyHostToDevice, st));
+  if (!s.pinnedInDone) SG_CHECK_HIP(hipEventCreateWithFlags(&s.pinnedInDone, hipEventDisableTiming));
+  SG_CHECK_HIP(hipEventRecord(s.pinnedInDone, st));
   return base.back();
 }
 
@@ -868,8 +940,9 @@ void buildNeighborsD(Sim& s, double ext) {
     hipLaunchKernelGGL((k_seg_bounds<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
                        s.H.as<double>(), s.seg.as<double>(), n, bits);
     SG_CHECK_HIP(hipGetLastError());
-    std::vector<unsigned long long> hbits(size_t(ns) * kNB);
-    SG_CHECK_HIP(hipMemcpyAsync(hbits.data(), bits, hbits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    const size_t nbits = size_t(ns) * kNB;
+    unsigned long long* hbits = static_cast<unsigned long long*>(hostStaging(s, 0, nbits * sizeof(unsigned long long)));
+    SG_CHECK_HIP(hipMemcpyAsync(hbits, bits, nbits * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
     std::vector<DGrid> grids;
     std::vector<int64_t> cells;
@@ -898,8 +971,10 @@ void buildNeighborsD(Sim& s, double ext) {
     hipLaunchKernelGGL((k_bounds<D>), dim3(nbB), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(), n, partials);
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, st, partials, nbB, bounds);
     SG_CHECK_HIP(hipGetLastError());
-    SG_CHECK_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
+    double* hbp = static_cast<double*>(hostStaging(s, 0, sizeof(hb)));
+    SG_CHECK_HIP(hipMemcpyAsync(hbp, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
+    std::memcpy(hb, hbp, sizeof(hb));
     s.isotropic = hb[7] > 0.5 && !s.hasMultipliers;
     std::vector<DGrid> grids;
     std::vector<int64_t> cells;
@@ -922,8 +997,18 @@ void buildNeighborsD(Sim& s, double ext) {
   s.cellStart.ensure(sizeof(int32_t) * size_t(G.ncells + 2));
   s.perm.ensure(sizeof(int32_t) * size_t(n));
   s.iperm.ensure(sizeof(int32_t) * size_t(n));
-  SG_CHECK_HIP(hipMemsetAsync(s.cellCount.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
-  SG_CHECK_HIP(hipMemsetAsync(s.cellFill.p, 0, sizeof(int32_t) * size_t(G.ncells + 1), st));
+  // zero the cell counters, the search flags and the ghost-use marks in one dispatch
+  const bool markUse = s.markGhostUse && s.nTot > s.nInt;
+  if (markUse) s.slotKeep.ensure(sizeof(int32_t) * size_t(n + 1));
+  s.flags.ensure(sizeof(int32_t) * 8 + sizeof(unsigned long long) * 2);
+  {
+    ZeroSet z{};
+    z.add(s.cellCount.as<int32_t>(), G.ncells + 1);
+    z.add(s.cellFill.as<int32_t>(), G.ncells + 1);
+    z.add(s.flags.as<int32_t>(), int64_t(s.flags.bytes / sizeof(int32_t)));
+    if (markUse) z.add(s.slotKeep.as<int32_t>(), n);
+    zeroMany(z, s.stream);
+  }
   hipLaunchKernelGGL((k_cell_count<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
                      n, dg, sgr,
                      s.cellOf.as<int32_t>(), s.cellCount.as<int32_t>());
@@ -960,19 +1045,20 @@ void buildNeighborsD(Sim& s, double ext) {
   sa.n = n;
   sa.nInt = s.nInt;
   sa.keepSlot = nullptr;
-  if (s.markGhostUse && s.nTot > s.nInt) {
-    s.slotKeep.ensure(sizeof(int32_t) * size_t(n + 1));
-    SG_CHECK_HIP(hipMemsetAsync(s.slotKeep.p, 0, sizeof(int32_t) * size_t(n), st));
-    sa.keepSlot = s.slotKeep.as<int32_t>();
-  }
+  if (markUse) sa.keepSlot = s.slotKeep.as<int32_t>();
   if (s.ellCap <= 0) s.ellCap = D == 3 ? 160 : (D == 2 ? 48 : 16);
   s.rowCount.ensure(sizeof(int32_t) * size_t(n));
-  s.flags.ensure(sizeof(int32_t) * 8 + sizeof(unsigned long long) * 2);
   unsigned long long* dTotal = reinterpret_cast<unsigned long long*>(s.flags.as<int32_t>() + 4);
   for (int attempt = 0; attempt < 2; ++attempt) {
     sa.cap = s.ellCap;
     s.nbr.ensure(ellBytes(n, s.ellCap));
-    SG_CHECK_HIP(hipMemsetAsync(s.flags.p, 0, s.flags.bytes, st));
+    if (attempt > 0) {
+      // the first attempt's flags were zeroed with the cell counters; ghost
+      // marks of the first attempt are a subset of the second's
+      ZeroSet z{};
+      z.add(s.flags.as<int32_t>(), int64_t(s.flags.bytes / sizeof(int32_t)));
+      zeroMany(z, s.stream);
+    }
     if (stagedPath) {
       hipLaunchKernelGGL(k_search_staged, dim3(int(xcdGrid(bricks))), dim3(kBlock), 0, st, s.rec.as<double>(),
                          s.cellStart.as<int32_t>(), sa, s.rowCount.as<int32_t>(), s.nbr.as<int4>(),
@@ -987,9 +1073,11 @@ void buildNeighborsD(Sim& s, double ext) {
     hipLaunchKernelGGL(k_row_stats, dim3(int(std::min<int64_t>(1024, nblocks(n)))), dim3(kBlock), 0, st,
                        s.rowCount.as<int32_t>(), n, dTotal, s.flags.as<int32_t>());
     SG_CHECK_HIP(hipGetLastError());
-    struct { int32_t f[4]; unsigned long long total; } hf;
-    SG_CHECK_HIP(hipMemcpyAsync(&hf, s.flags.p, sizeof(hf), hipMemcpyDeviceToHost, st));
+    struct Flags { int32_t f[4]; unsigned long long total; };
+    Flags* hfp = static_cast<Flags*>(hostStaging(s, 0, sizeof(Flags)));
+    SG_CHECK_HIP(hipMemcpyAsync(hfp, s.flags.p, sizeof(Flags), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
+    const Flags hf = *hfp;
     s.maxRow = hf.f[0];
     s.stagedFallbacks = hf.f[1];
     s.nEntries = int64_t(hf.total);
@@ -1018,6 +1106,44 @@ void buildNeighborsD(Sim& s, double ext) {
 
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s) { exclusiveScan<int32_t, int64_t>(in, out, n, s); }
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s) { exclusiveScan<int32_t, int32_t>(in, out, n, s); }
+void* hostStaging(Sim& s, int which, size_t bytes) {
+  void*& p = which ? s.pinnedIn : s.pinnedOut;
+  size_t& cap = which ? s.pinnedInBytes : s.pinnedOutBytes;
+  // uploads: the copies of the previous use must have read the buffer
+  if (which && s.pinnedInDone) SG_CHECK_HIP(hipEventSynchronize(s.pinnedInDone));
+  if (bytes > cap) {
+    SG_CHECK_HIP(hipStreamSynchronize(s.stream));   // no copy still uses the old buffer
+    if (p) SG_CHECK_HIP(hipHostFree(p));
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(4096, ((bytes + 4095) / 4096) * 4096);
+    SG_CHECK_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+    cap = want;
+  }
+  return p;
+}
+
+const int32_t* readInts(Sim& s, const int32_t* src, const int64_t* at, int m) {
+  SG_REQUIRE(m >= 1 && m <= 16, SPHGPU_ERR_ARG, "readInts takes 1 to 16 values");
+  Pick32 pk{};
+  for (int q = 0; q < m; ++q) pk.at[q] = at[q];
+  pk.m = m;
+  s.pick.ensure(sizeof(int32_t) * 16);
+  int32_t* h = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t) * 16));
+  hipLaunchKernelGGL(k_pick_i32, dim3(1), dim3(64), 0, s.stream, src, pk, s.pick.as<int32_t>());
+  SG_CHECK_HIP(hipGetLastError());
+  SG_CHECK_HIP(hipMemcpyAsync(h, s.pick.p, sizeof(int32_t) * size_t(m), hipMemcpyDeviceToHost, s.stream));
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+  return h;
+}
+
+void zeroMany(const ZeroSet& z, hipStream_t st) {
+  int64_t most = 0;
+  for (int q = 0; q < z.m; ++q) most = std::max(most, z.n[q]);
+  if (most == 0) return;
+  hipLaunchKernelGGL(k_zero_many, dim3(std::min(nblocks(most), 1024)), dim3(kBlock), 0, st, z);
+  SG_CHECK_HIP(hipGetLastError());
+}
 
 void buildNeighbors(Sim& s, double ext) {
   SG_REQUIRE(s.nTot > 0, SPHGPU_ERR_STATE, "no nodes");
