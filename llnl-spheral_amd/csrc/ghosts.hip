@@ -151,12 +151,17 @@ __global__ void k_hinv_min(const double* __restrict__ H, int64_t n, double* __re
 
 // hmax of the nodes near either plane, one value: grid-stride loop, a block
 // maximum and one 64-bit atomic max per block.
+// Row counts of the ghost kernels: `n` rows, or, with nDev, the count the
+// device holds (n is then the capacity the grid covers).
+__device__ __forceinline__ int64_t liveRows(int64_t n, const int64_t* nDev) { return nDev ? *nDev : n; }
+
 template <int D>
 __global__ void k_ghost_hmax_one(const double* __restrict__ pos, const double* __restrict__ hmaxOf, int64_t n, DSub b,
-                                 double ext, unsigned long long* __restrict__ hmaxBits) {
+                                 double ext, unsigned long long* __restrict__ hmaxBits, const int64_t* __restrict__ nDev) {
   __shared__ double wmax[kBlock / 64];
   double v = 0.0;
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+  const int64_t nn = liveRows(n, nDev);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nn; i += int64_t(gridDim.x) * blockDim.x) {
     const V<D> x = loadV<D>(pos, i);
     const double hmaxi = hmaxOf[i];
     const double d = dmin(dabs(signedDist<D>(b.exitP, b.exitN, x)), dabs(signedDist<D>(b.enterP, b.enterN, x)));
@@ -175,11 +180,11 @@ __global__ void k_ghost_hmax_one(const double* __restrict__ pos, const double* _
 template <int D>
 __global__ void k_ghost_hmax(const double* __restrict__ pos, const double* __restrict__ hmaxOf,
                              const double* __restrict__ seg, int64_t n, DSub b, double ext,
-                             unsigned long long* __restrict__ hmaxBits) {
+                             unsigned long long* __restrict__ hmaxBits, const int64_t* __restrict__ nDev) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   double v = 0.0;
   int m = 0;
-  if (i < n) {
+  if (i < liveRows(n, nDev)) {
     const V<D> x = loadV<D>(pos, i);
     const double hmaxi = hmaxOf[i];
     const double d = dmin(dabs(signedDist<D>(b.exitP, b.exitN, x)), dabs(signedDist<D>(b.enterP, b.enterN, x)));
@@ -197,9 +202,11 @@ __global__ void k_ghost_hmax(const double* __restrict__ pos, const double* __res
 
 template <int D>
 __global__ void k_ghost_flag(const double* __restrict__ pos, const double* __restrict__ seg, int64_t n, DSub b,
-                             double ext, const unsigned long long* __restrict__ hmaxBits, int32_t* __restrict__ flags) {
+                             double ext, const unsigned long long* __restrict__ hmaxBits, int32_t* __restrict__ flags,
+                             const int64_t* __restrict__ nDev) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
+  if (i >= liveRows(n, nDev)) { flags[i] = 0; return; }   // capacity rows beyond the live count
   const double hmax = __longlong_as_double((long long)hmaxBits[seg ? int(seg[i]) : 0]);
   int32_t f = 0;
   if (hmax > 0.0) {
@@ -213,11 +220,14 @@ template <int D>
 __global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, double* __restrict__ seg, int64_t n,
                              int64_t nInt, const int32_t* __restrict__ flags, const int32_t* __restrict__ offs, DSub b,
                              int subIdx, int32_t* __restrict__ control, int32_t* __restrict__ subOf,
-                             double* __restrict__ hmaxOf) {
+                             double* __restrict__ hmaxOf, const int64_t* __restrict__ nDev, int64_t cap,
+                             int64_t* __restrict__ overflow) {
   constexpr int NS = NSym<D>::value;
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i >= n || !flags[i]) return;
-  const int64_t g = n + offs[i];
+  const int64_t nn = liveRows(n, nDev);
+  if (i >= nn || !flags[i]) return;
+  const int64_t g = nn + offs[i];
+  if (nDev && g >= cap) { *overflow = 1; return; }   // the capacity guess was short: the build redoes it exactly
   const V<D> xg = ghostPosition<D>(b, loadV<D>(pos, i));
   for (int k = 0; k < D; ++k) pos[g * D + k] = xg.c[k];
   if (b.reflect) {
@@ -232,6 +242,32 @@ __global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, d
   control[g - nInt] = int32_t(i);
   subOf[g - nInt] = subIdx;
   if (seg) seg[g] = seg[i];
+}
+
+// After a plane with device-held counts: the plane's first ghost, then the new
+// live count (the count of flags is the scan's total over the capacity).
+struct GenDev {
+  int64_t n;            // live rows
+  int64_t overflow;     // set when a ghost fell beyond the capacity
+  int64_t start[16];    // first ghost (offset from nInt) of each sub-boundary
+};
+// Both also clear the hmax words of the next plane.
+__global__ void k_gen_init(GenDev* g, int64_t nInt, unsigned long long* __restrict__ hbits, int nh) {
+  for (int q = threadIdx.x; q < nh; q += blockDim.x) hbits[q] = 0ull;
+  if (threadIdx.x == 0) {
+    g->n = nInt;
+    g->overflow = 0;
+  }
+}
+__global__ void k_gen_advance(GenDev* g, int b, const int32_t* __restrict__ total, int64_t nInt, int64_t cap,
+                              unsigned long long* __restrict__ hbits, int nh) {
+  for (int q = threadIdx.x; q < nh; q += blockDim.x) hbits[q] = 0ull;
+  if (threadIdx.x == 0) {
+    g->start[b] = g->n - nInt;
+    const int64_t next = g->n + int64_t(*total);
+    if (next > cap) g->overflow = 1;
+    g->n = next > cap ? cap : next;
+  }
 }
 
 struct FieldPtrs {
@@ -474,6 +510,56 @@ template <int D> void applyFieldsD(Sim& s, bool geometry) {
   SG_CHECK_HIP(hipGetLastError());
 }
 
+// Every plane's ghosts with the running row count kept on the device: the
+// grids cover a capacity guessed from the previous generation, and one
+// readback at the end returns the count and each sub-boundary's first ghost.
+// Returns false when a plane's ghosts would pass the capacity (the caller then
+// generates again plane by plane; generation reads only internal rows).
+template <int D> bool generateFastD(Sim& s, double ext, unsigned long long* hbits, int nh) {
+  hipStream_t st = s.stream;
+  const int64_t nInt = s.nInt;
+  const int64_t est = s.ghostsGenerated + s.ghostsGenerated / 4 + 1024;
+  const int64_t cap = nInt + est;
+  SG_REQUIRE(cap < (int64_t(1) << 31) - 1, SPHGPU_ERR_CAPACITY, "too many ghost nodes");
+  ensureRows(s, cap);
+  s.hmaxOf.grow(sizeof(double) * size_t(cap), sizeof(double) * size_t(nInt), st);
+  s.ghostControl.grow(sizeof(int32_t) * size_t(est), 0, st);
+  s.ghostSub.grow(sizeof(int32_t) * size_t(est), 0, st);
+  s.keep.ensure(sizeof(int32_t) * size_t(cap + 1));
+  s.keepScan.ensure(sizeof(int32_t) * size_t(cap + 1));
+  s.genDev.ensure(sizeof(GenDev));
+  GenDev* g = s.genDev.as<GenDev>();
+  const int64_t* nDev = &g->n;
+  hipLaunchKernelGGL(k_gen_init, dim3(1), dim3(kBlock), 0, st, g, nInt, hbits, nh);
+  const double* segp = s.nSeg > 0 ? s.seg.as<double>() : nullptr;
+  for (size_t b = 0; b < s.subs.size(); ++b) {
+    const DSub d = toDevice(s.subs[b]);
+    if (segp)
+      hipLaunchKernelGGL(k_ghost_hmax<D>, dim3(nblocks(cap)), dim3(kBlock), 0, st, s.pos.as<double>(),
+                         s.hmaxOf.as<double>(), segp, cap, d, ext, hbits, nDev);
+    else
+      hipLaunchKernelGGL(k_ghost_hmax_one<D>, dim3(std::min(nblocks(cap), 1024)), dim3(kBlock), 0, st,
+                         s.pos.as<double>(), s.hmaxOf.as<double>(), cap, d, ext, hbits, nDev);
+    hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(cap)), dim3(kBlock), 0, st, s.pos.as<double>(), segp, cap, d,
+                       ext, hbits, s.keep.as<int32_t>(), nDev);
+    exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), cap, s);
+    hipLaunchKernelGGL(k_ghost_make<D>, dim3(nblocks(cap)), dim3(kBlock), 0, st, s.pos.as<double>(),
+                       s.H.as<double>(), s.nSeg > 0 ? s.seg.as<double>() : nullptr, cap, nInt, s.keep.as<int32_t>(),
+                       s.keepScan.as<int32_t>(), d, int(b), s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(),
+                       s.hmaxOf.as<double>(), nDev, cap, &g->overflow);
+    hipLaunchKernelGGL(k_gen_advance, dim3(1), dim3(kBlock), 0, st, g, int(b), s.keepScan.as<int32_t>() + cap, nInt,
+                       cap, hbits, nh);
+    SG_CHECK_HIP(hipGetLastError());
+  }
+  GenDev* h = static_cast<GenDev*>(hostStaging(s, 0, sizeof(GenDev)));
+  SG_CHECK_HIP(hipMemcpyAsync(h, g, sizeof(GenDev), hipMemcpyDeviceToHost, st));
+  SG_CHECK_HIP(hipStreamSynchronize(st));
+  if (h->overflow) return false;
+  s.nTot = h->n;
+  for (size_t b = 0; b < s.subs.size(); ++b) s.subStart[b] = h->start[b];
+  return true;
+}
+
 template <int D> void generateD(Sim& s, double ext) {
   hipStream_t st = s.stream;
   constexpr int NS = NSym<D>::value;
@@ -490,7 +576,13 @@ template <int D> void generateD(Sim& s, double ext) {
   if (s.nInt > 0)
     hipLaunchKernelGGL(k_hinv_min<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, st, s.H.as<double>(), s.nInt,
                        s.hmaxOf.as<double>());
-  for (size_t b = 0; b < s.subs.size(); ++b) {
+  // after a first generation, one readback for all planes
+  bool done = false;
+  if (s.ghostsGenerated > 0 && !s.subs.empty() && s.subs.size() < 16) {
+    done = generateFastD<D>(s, ext, hbits, nh);
+    if (!done) s.nTot = s.nInt;
+  }
+  for (size_t b = 0; !done && b < s.subs.size(); ++b) {
     const int64_t n = s.nTot;
     s.subStart[b] = n - s.nInt;
     const DSub d = toDevice(s.subs[b]);
@@ -500,12 +592,12 @@ template <int D> void generateD(Sim& s, double ext) {
     SG_CHECK_HIP(hipMemsetAsync(hbits, 0, sizeof(unsigned long long) * size_t(nh), st));
     if (segp)
       hipLaunchKernelGGL(k_ghost_hmax<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
-                         s.hmaxOf.as<double>(), segp, n, d, ext, hbits);
+                         s.hmaxOf.as<double>(), segp, n, d, ext, hbits, nullptr);
     else
       hipLaunchKernelGGL(k_ghost_hmax_one<D>, dim3(std::min(nblocks(n), 1024)), dim3(kBlock), 0, st,
-                         s.pos.as<double>(), s.hmaxOf.as<double>(), n, d, ext, hbits);
+                         s.pos.as<double>(), s.hmaxOf.as<double>(), n, d, ext, hbits, nullptr);
     hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), segp, n, d, ext,
-                       hbits, s.keep.as<int32_t>());
+                       hbits, s.keep.as<int32_t>(), nullptr);
     exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), n, s);
     int32_t* hc = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t)));
     SG_CHECK_HIP(hipMemcpyAsync(hc, s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -520,7 +612,8 @@ template <int D> void generateD(Sim& s, double ext) {
     s.ghostSub.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
     hipLaunchKernelGGL(k_ghost_make<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
                        s.nSeg > 0 ? s.seg.as<double>() : nullptr, n, s.nInt, s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), d, int(b),
-                       s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>());
+                       s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>(), nullptr,
+                       int64_t(0), nullptr);
     SG_CHECK_HIP(hipGetLastError());
     s.nTot = n + count;
   }
@@ -529,10 +622,18 @@ template <int D> void generateD(Sim& s, double ext) {
   applyFieldsD<D>(s, false);
 }
 
-void cull(Sim& s) {
+}  // namespace
+
+// Integrator::cullGhostNodes, first half: keep flags (ghosts some internal row
+// uses, and the controls they chain through), their scan, and the new total
+// and first ghost of every sub-boundary copied to pinned `hostOut` (nsub + 1
+// values, read after the caller's next wait).  The neighbour build issues it
+// before its row readback so that one wait serves both.
+void cullLaunch(Sim& s, int32_t* hostOut) {
   hipStream_t st = s.stream;
   const int64_t n = s.nTot, nInt = s.nInt;
-  if (n == nInt) return;
+  const size_t nsub = s.subStart.size();
+  SG_REQUIRE(n > nInt && nsub < 16, SPHGPU_ERR_STATE, "cullLaunch needs ghosts of at most 15 sub-boundaries");
   s.keep.ensure(sizeof(int32_t) * size_t(n + 1));
   s.keepScan.ensure(sizeof(int32_t) * size_t(n + 1));
   int32_t* keep = s.keep.as<int32_t>();
@@ -541,16 +642,39 @@ void cull(Sim& s) {
   hipLaunchKernelGGL(k_keep_controls, dim3(nblocks(n - nInt)), dim3(kBlock), 0, st, keep, nInt, n,
                      s.ghostControl.as<int32_t>());
   exclusiveScanI32(keep, s.keepScan.as<int32_t>(), n, s);
-  // new total and the new first ghost of every sub-boundary
+  int64_t at[16];
+  for (size_t b = 0; b < nsub; ++b) at[b] = nInt + s.subStart[b];
+  at[nsub] = n;
+  pickInts(s, s.keepScan.as<int32_t>(), at, int(nsub + 1), hostOut);
+}
+
+namespace {
+
+void cull(Sim& s) {
+  hipStream_t st = s.stream;
+  const int64_t n = s.nTot, nInt = s.nInt;
+  if (n == nInt) return;
   const size_t nsub = s.subStart.size();
   std::vector<int32_t> starts(nsub + 1);
-  if (nsub < 16) {
-    int64_t at[16];
-    for (size_t b = 0; b < nsub; ++b) at[b] = nInt + s.subStart[b];
-    at[nsub] = n;
-    const int32_t* h = readInts(s, s.keepScan.as<int32_t>(), at, int(nsub + 1));
+  if (s.cullStartsReady) {
+    // launched and read with the neighbour build's row readback
+    SG_REQUIRE(s.cullStarts.size() == nsub + 1, SPHGPU_ERR_STATE, "stale cull starts");
+    starts = s.cullStarts;
+    s.cullStartsReady = false;
+  } else if (nsub < 16) {
+    int32_t* h = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t) * 16));
+    cullLaunch(s, h);
+    SG_CHECK_HIP(hipStreamSynchronize(st));
     std::copy(h, h + nsub + 1, starts.begin());
   } else {
+    s.keep.ensure(sizeof(int32_t) * size_t(n + 1));
+    s.keepScan.ensure(sizeof(int32_t) * size_t(n + 1));
+    int32_t* keep = s.keep.as<int32_t>();
+    hipLaunchKernelGGL(k_keep_gather, dim3(nblocks(n)), dim3(kBlock), 0, st, keep, s.iperm.as<int32_t>(),
+                       s.slotKeep.as<int32_t>(), nInt, n);
+    hipLaunchKernelGGL(k_keep_controls, dim3(nblocks(n - nInt)), dim3(kBlock), 0, st, keep, nInt, n,
+                       s.ghostControl.as<int32_t>());
+    exclusiveScanI32(keep, s.keepScan.as<int32_t>(), n, s);
     int32_t* h = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t) * (nsub + 1)));
     for (size_t b = 0; b < nsub; ++b)
       SG_CHECK_HIP(hipMemcpyAsync(h + b, s.keepScan.as<int32_t>() + nInt + s.subStart[b], sizeof(int32_t),
@@ -559,6 +683,7 @@ void cull(Sim& s) {
     SG_CHECK_HIP(hipStreamSynchronize(st));
     std::copy(h, h + nsub + 1, starts.begin());
   }
+  int32_t* keep = s.keep.as<int32_t>();
   const int64_t nNew = starts.back();
   if (nNew == n) return;
   const int32_t* scan = s.keepScan.as<int32_t>();

@@ -169,6 +169,7 @@ struct Sim {
   std::vector<int64_t> subStart; // first ghost (offset from nInt) of each sub-boundary, +1 entry
   int64_t ghostsGenerated = 0;   // before culling
   DBuf keep, keepScan, slotKeep, ghostControlSpare, ghostSubSpare, ipermSpare, hmaxOf;
+  DBuf genDev;                   // device-held counts of a ghost generation
   DBuf spare[13];                // one per caller-order state field (culling compacts into it)
   bool markGhostUse = false;     // the next build marks the ghost slots rows hold (for culling)
 
@@ -220,6 +221,8 @@ struct Sim {
   size_t pinnedInBytes = 0;
   hipEvent_t pinnedInDone = nullptr;   // recorded after the copies that read pinnedIn
   DBuf pick;                     // device side of gathered readback values
+  std::vector<int32_t> cullStarts;  // new ghost starts read with the last build's row readback
+  bool cullStartsReady = false;
   bool pairsValid = false;                   // a pass ran on the current neighbour rows
 };
 void resolveTimings(Sim& s);
@@ -276,6 +279,12 @@ void* hostStaging(Sim& s, int which, size_t bytes);
 // Gathers up to 16 int32 values of a device array into s.pick and copies them
 // to pinned host memory, one round trip: returns the host copy.
 const int32_t* readInts(Sim& s, const int32_t* src, const int64_t* at, int m);
+// The same gather with the copy to pinned `hostOut` left in flight.
+void pickInts(Sim& s, const int32_t* src, const int64_t* at, int m, int32_t* hostOut);
+// First half of the ghost cull (keep flags, scan, new ghost starts to pinned
+// hostOut); the cull takes the starts from s.cullStarts when the neighbour
+// build has read them.
+void cullLaunch(Sim& s, int32_t* hostOut);
 
 struct ZeroSet {
   int32_t* p[4];

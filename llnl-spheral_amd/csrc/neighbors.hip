@@ -1074,10 +1074,19 @@ void buildNeighborsD(Sim& s, double ext) {
                        s.rowCount.as<int32_t>(), n, dTotal, s.flags.as<int32_t>());
     SG_CHECK_HIP(hipGetLastError());
     struct Flags { int32_t f[4]; unsigned long long total; };
-    Flags* hfp = static_cast<Flags*>(hostStaging(s, 0, sizeof(Flags)));
+    char* hp = static_cast<char*>(hostStaging(s, 0, 256 + sizeof(int32_t) * 16));
+    // the ghost cull's first half rides on the same wait
+    const bool cullHere = markUse && s.subStart.size() < 16;
+    if (cullHere) cullLaunch(s, reinterpret_cast<int32_t*>(hp + 256));
+    Flags* hfp = reinterpret_cast<Flags*>(hp);
     SG_CHECK_HIP(hipMemcpyAsync(hfp, s.flags.p, sizeof(Flags), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
     const Flags hf = *hfp;
+    if (cullHere) {
+      const int32_t* cs = reinterpret_cast<const int32_t*>(hp + 256);
+      s.cullStarts.assign(cs, cs + s.subStart.size() + 1);
+      s.cullStartsReady = true;
+    }
     s.maxRow = hf.f[0];
     s.stagedFallbacks = hf.f[1];
     s.nEntries = int64_t(hf.total);
@@ -1123,16 +1132,20 @@ void* hostStaging(Sim& s, int which, size_t bytes) {
   return p;
 }
 
-const int32_t* readInts(Sim& s, const int32_t* src, const int64_t* at, int m) {
-  SG_REQUIRE(m >= 1 && m <= 16, SPHGPU_ERR_ARG, "readInts takes 1 to 16 values");
+void pickInts(Sim& s, const int32_t* src, const int64_t* at, int m, int32_t* hostOut) {
+  SG_REQUIRE(m >= 1 && m <= 16, SPHGPU_ERR_ARG, "pickInts takes 1 to 16 values");
   Pick32 pk{};
   for (int q = 0; q < m; ++q) pk.at[q] = at[q];
   pk.m = m;
   s.pick.ensure(sizeof(int32_t) * 16);
-  int32_t* h = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t) * 16));
   hipLaunchKernelGGL(k_pick_i32, dim3(1), dim3(64), 0, s.stream, src, pk, s.pick.as<int32_t>());
   SG_CHECK_HIP(hipGetLastError());
-  SG_CHECK_HIP(hipMemcpyAsync(h, s.pick.p, sizeof(int32_t) * size_t(m), hipMemcpyDeviceToHost, s.stream));
+  SG_CHECK_HIP(hipMemcpyAsync(hostOut, s.pick.p, sizeof(int32_t) * size_t(m), hipMemcpyDeviceToHost, s.stream));
+}
+
+const int32_t* readInts(Sim& s, const int32_t* src, const int64_t* at, int m) {
+  int32_t* h = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t) * 16));
+  pickInts(s, src, at, m, h);
   SG_CHECK_HIP(hipStreamSynchronize(s.stream));
   return h;
 }
@@ -1146,6 +1159,7 @@ void zeroMany(const ZeroSet& z, hipStream_t st) {
 }
 
 void buildNeighbors(Sim& s, double ext) {
+  s.cullStartsReady = false;
   SG_REQUIRE(s.nTot > 0, SPHGPU_ERR_STATE, "no nodes");
   SG_REQUIRE(ext > 0.0, SPHGPU_ERR_ARG, "kernel extent must be positive");
   s.kernelExtent = ext;
