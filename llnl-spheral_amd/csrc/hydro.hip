@@ -40,12 +40,49 @@ inline int nblocks(int64_t n, int bs = kBlock) { return int(std::max<int64_t>(1,
 // ---------------------------------------------------------------------------
 // Records
 // ---------------------------------------------------------------------------
-template <int D, bool ISO>
+// The isotropic pair kernels' records, from an FP64 record r (isotropic
+// layout): FP32, 64 B, for the mixed kernels (positions keep their FP64 bits);
+// FP64, 128 B, for the FP64 kernel.  det H is negative for ghost rows.
+template <int D>
+__device__ __forceinline__ void writeMixedRec(const double* r, int64_t nInt, float4* __restrict__ p) {
+  using R = RL<true>;
+  const double x0 = r[R::X], x1 = D > 1 ? r[R::X + 1] : 0.0, x2 = D > 2 ? r[R::X + 2] : 0.0;
+  const bool ghost = r[R::ORIG] >= double(nInt);
+  const float hdet = float(r[R::HDET]);
+  p[0] = make_float4(__int_as_float(__double2loint(x0)), __int_as_float(__double2hiint(x0)),
+                     __int_as_float(__double2loint(x1)), __int_as_float(__double2hiint(x1)));
+  p[1] = make_float4(__int_as_float(__double2loint(x2)), __int_as_float(__double2hiint(x2)), float(r[R::HINV]),
+                     ghost ? -hdet : hdet);
+  const double rho = r[R::RHO];
+  p[2] = make_float4(float(r[R::M]), rho != 0.0 ? float(1.0 / rho) : 0.0f, float(r[R::V]),
+                     D > 1 ? float(r[R::V + 1]) : 0.0f);
+  p[3] = make_float4(D > 2 ? float(r[R::V + 2]) : 0.0f, float(r[R::CS]), float(r[R::PRHO]), float(r[R::MOR]));
+}
+template <int D>
+__device__ __forceinline__ void writeFast64Rec(const double* r, int64_t nInt, double2* __restrict__ p) {
+  using R = RL<true>;
+  const bool ghost = r[R::ORIG] >= double(nInt);
+  const double rho = r[R::RHO];
+  p[0] = make_double2(r[R::X], D > 1 ? r[R::X + 1] : 0.0);
+  p[1] = make_double2(D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
+  p[2] = make_double2(ghost ? -r[R::HDET] : r[R::HDET], r[R::M]);
+  p[3] = make_double2(rho != 0.0 ? 1.0 / rho : 0.0, r[R::V]);
+  p[4] = make_double2(D > 1 ? r[R::V + 1] : 0.0, D > 2 ? r[R::V + 2] : 0.0);
+  p[5] = make_double2(r[R::CS], r[R::PRHO]);
+  p[6] = make_double2(r[R::MOR], 0.0);
+  p[7] = make_double2(0.0, 0.0);
+}
+
+// SEC: also write the pair kernel's record from the values in registers
+// (0 none, 1 FP32 mixed, 2 FP64), saving a second pass over the records.
+template <int D, bool ISO, int SEC = 0>
 __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double* __restrict__ pos,
                        const double* __restrict__ vel, const double* __restrict__ mass, const double* __restrict__ rho,
                        const double* __restrict__ P, const double* __restrict__ cs, const double* __restrict__ H,
                        const double* __restrict__ omega, const double* __restrict__ fCl, const double* __restrict__ fCq,
-                       const double* __restrict__ fsh, double* __restrict__ rec) {
+                       const double* __restrict__ fsh, double* __restrict__ rec, int64_t nInt = 0,
+                       void* __restrict__ sec = nullptr) {
+  static_assert(SEC == 0 || ISO, "the pair kernels' records are isotropic");
   using R = RL<ISO>;
   constexpr int NS = NSym<D>::value;
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -59,6 +96,8 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
     double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
 #pragma unroll
     for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
+    if constexpr (SEC == 1) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
+    if constexpr (SEC == 2) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
     return;
   }
   for (int c = 0; c < D; ++c) { r[R::X + c] = pos[i * D + c]; r[R::V + c] = vel[i * D + c]; }
@@ -86,6 +125,8 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
   double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
 #pragma unroll
   for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
+  if constexpr (SEC == 1) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
+  if constexpr (SEC == 2) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
 }
 
 // Table: per point [Ak Bk Ck Ag Bg Cg].
@@ -126,6 +167,15 @@ __device__ __forceinline__ double lookupW(const double* __restrict__ tab, const 
 
 // Largest table that is staged in LDS (the 1000-point default takes 48 KB).
 constexpr int kLdsTablePoints = 1365;
+// Staging the table costs a full table read and a barrier per workgroup; it
+// pays when each node makes many lookups (2-D rows hold ~28 entries, 3-D rows
+// ~120), not for 1-D rows of ~6, which read the table through L1.
+constexpr double kLdsMinRow = 16.0;
+bool ldsTable(const Sim& s) {
+  if (s.table.n > kLdsTablePoints) return false;
+  const double perNode = s.nInt > 0 ? double(s.nEntries) / double(s.nInt) : 0.0;
+  return perNode >= kLdsMinRow;
+}
 
 template <bool LDS> __device__ __forceinline__ const double* stageTable(double* lds, const TabArgs& t) {
   if constexpr (LDS) {
@@ -643,41 +693,14 @@ __device__ __forceinline__ void loadIso(const double2* __restrict__ recD, int32_
 
 template <int D>
 __global__ void k_pack_mixed(const double* __restrict__ rec, int64_t n, int64_t nInt, float4* __restrict__ recM) {
-  using R = RL<true>;
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k >= n) return;
-  const double* r = rec + k * R::STRIDE;
-  const double x0 = r[R::X], x1 = D > 1 ? r[R::X + 1] : 0.0, x2 = D > 2 ? r[R::X + 2] : 0.0;
-  const bool ghost = r[R::ORIG] >= double(nInt);
-  const float hdet = float(r[R::HDET]);
-  float4* p = recM + k * 4;
-  p[0] = make_float4(__int_as_float(__double2loint(x0)), __int_as_float(__double2hiint(x0)),
-                     __int_as_float(__double2loint(x1)), __int_as_float(__double2hiint(x1)));
-  p[1] = make_float4(__int_as_float(__double2loint(x2)), __int_as_float(__double2hiint(x2)), float(r[R::HINV]),
-                     ghost ? -hdet : hdet);
-  const double rho = r[R::RHO];
-  p[2] = make_float4(float(r[R::M]), rho != 0.0 ? float(1.0 / rho) : 0.0f, float(r[R::V]),
-                     D > 1 ? float(r[R::V + 1]) : 0.0f);
-  p[3] = make_float4(D > 2 ? float(r[R::V + 2]) : 0.0f, float(r[R::CS]), float(r[R::PRHO]), float(r[R::MOR]));
+  if (k < n) writeMixedRec<D>(rec + k * RL<true>::STRIDE, nInt, recM + k * 4);
 }
 
 template <int D>
 __global__ void k_pack_fast64(const double* __restrict__ rec, int64_t n, int64_t nInt, double2* __restrict__ recD) {
-  using R = RL<true>;
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k >= n) return;
-  const double* r = rec + k * R::STRIDE;
-  const bool ghost = r[R::ORIG] >= double(nInt);
-  const double rho = r[R::RHO];
-  double2* p = recD + k * 8;
-  p[0] = make_double2(r[R::X], D > 1 ? r[R::X + 1] : 0.0);
-  p[1] = make_double2(D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
-  p[2] = make_double2(ghost ? -r[R::HDET] : r[R::HDET], r[R::M]);
-  p[3] = make_double2(rho != 0.0 ? 1.0 / rho : 0.0, r[R::V]);
-  p[4] = make_double2(D > 1 ? r[R::V + 1] : 0.0, D > 2 ? r[R::V + 2] : 0.0);
-  p[5] = make_double2(r[R::CS], r[R::PRHO]);
-  p[6] = make_double2(r[R::MOR], 0.0);
-  p[7] = make_double2(0.0, 0.0);
+  if (k < n) writeFast64Rec<D>(rec + k * RL<true>::STRIDE, nInt, recD + k * 8);
 }
 
 // FP32 table in LDS: 8 floats per point.
@@ -964,21 +987,34 @@ TabArgs tabArgs(const Sim& s) {
                  s.dNperh.as<double>(), s.dWsum.as<double>()};
 }
 
+// Records from the caller-order fields; sec 1 / 2 also writes the mixed /
+// FP64 isotropic pair kernel's record in the same pass.
 template <int D>
-void packD(Sim& s) {
+void packD(Sim& s, int sec = 0) {
   const int64_t n = s.nSlots;
   const bool iso = s.isotropic;
+  SG_REQUIRE(sec == 0 || iso, SPHGPU_ERR_STATE, "the pair kernels' records need H = h I");
   const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
   s.rec.ensure(sizeof(double) * size_t(n) * stride);
+  void* secp = nullptr;
+  if (sec == 1) {
+    s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(n, 1)));
+    secp = s.recM.p;
+  } else if (sec == 2) {
+    s.recD.ensure(sizeof(double) * 16 * size_t(std::max<int64_t>(n, 1)));
+    secp = s.recD.p;
+  }
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.perm.as<int32_t>(), n, s.pos.as<double>(),
                        s.vel.as<double>(), s.mass.as<double>(), s.rho.as<double>(), s.P.as<double>(),
                        s.cs.as<double>(), s.H.as<double>(), s.omega.as<double>(),
                        s.hasMultipliers ? s.fCl.as<double>() : nullptr, s.hasMultipliers ? s.fCq.as<double>() : nullptr,
-                       s.hasMultipliers ? s.fshear.as<double>() : nullptr, s.rec.as<double>());
+                       s.hasMultipliers ? s.fshear.as<double>() : nullptr, s.rec.as<double>(), s.nInt, secp);
   };
-  if (iso) go(k_pack<D, true>);
-  else go(k_pack<D, false>);
+  if (!iso) go(k_pack<D, false, 0>);
+  else if (sec == 1) go(k_pack<D, true, 1>);
+  else if (sec == 2) go(k_pack<D, true, 2>);
+  else go(k_pack<D, true, 0>);
   SG_CHECK_HIP(hipGetLastError());
 }
 
@@ -1002,16 +1038,17 @@ void packFast64(Sim& s) {
 
 Tab64 tab64(const Sim& s) { return Tab64{1.0 / s.table.step, s.table.extent, s.table.n}; }
 
+// secFresh: the mixed record was written with the records just now.
 template <int D>
-void sumDensityD(Sim& s) {
+void sumDensityD(Sim& s, bool secFresh) {
   const int64_t n = s.nSlots;
   TabArgs t = tabArgs(s);
-  const bool lds = s.table.n <= kLdsTablePoints;
+  const bool lds = ldsTable(s);
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
   if (s.precision == SPHGPU_PRECISION_MIXED) {
     SG_REQUIRE(s.isotropic, SPHGPU_ERR_UNSUPPORTED, "mixed precision needs H = h I and no Q multipliers");
-    packMixed<D>(s);
+    if (!secFresh) packMixed<D>(s);
     const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
     const double W0 = s.table.kernelValue(0.0, 1.0);
     auto go = [&](auto kern) {
@@ -1048,13 +1085,32 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   s.reduceTmp.ensure(sizeof(double) * 16);
   unsigned long long* dncalc = reinterpret_cast<unsigned long long*>(s.reduceTmp.as<double>() + 8);
 
+  // the pair kernel decides the record it reads
+  const bool tensile = hp.epsTensile != 0.0;
+  const bool compat = hp.compatibleEnergyEvolution != 0;
+  const bool mixed = hp.precision == SPHGPU_PRECISION_MIXED;
+  if (mixed)
+    SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
+               "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
+  // search row order is held to round-off of the reference sums, not to
+  // their bits: the collapsed isotropic algebra in FP64
+  const bool fast64 = !mixed && s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile && !compat;
+  const int sec = mixed ? 1 : (fast64 ? 2 : 0);
+
   SG_CHECK_HIP(hipEventRecord(s.ev[4], st));
-  if (!s.recValid) { packD<D>(s); s.recValid = true; }
+  if (!s.recValid) {
+    packD<D>(s, sec);
+    s.recValid = true;
+  } else if (sec == 1) {
+    packMixed<D>(s);
+  } else if (sec == 2) {
+    packFast64<D>(s);
+  }
   SG_CHECK_HIP(hipMemsetAsync(dncalc, 0, sizeof(unsigned long long), st));
   SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
 
   TabArgs t = tabArgs(s);
-  const bool lds = s.table.n <= kLdsTablePoints;
+  const bool lds = ldsTable(s);
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   PassArgs p{};
   p.Cl = hp.Cl; p.Cq = hp.Cq; p.eps2 = hp.eps2; p.epsTensile = hp.epsTensile; p.nPerh = hp.nPerh;
@@ -1067,19 +1123,13 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   o.base = s.out.as<double>();
   o.n = n;
   for (int f = 0; f < SPHGPU_NUM_DERIV_FIELDS; ++f) o.off[f] = derivOffsetDoubles(s.dim, f);
-  const bool tensile = hp.epsTensile != 0.0;
-  const bool compat = hp.compatibleEnergyEvolution != 0;
   if (compat) {
     // pairAccelerations: the own-frame pair term of every row entry
     o.plane = n * int64_t(s.ellCap);
     s.pairDv.ensure(sizeof(double) * size_t(D) * size_t(std::max<int64_t>(o.plane, 1)));
     o.pairDv = s.pairDv.as<double>();
   }
-  if (hp.precision == SPHGPU_PRECISION_MIXED) {
-    SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
-               "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
-    packMixed<D>(s);
-    SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
+  if (mixed) {
     const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), lds32, st, s.rec.as<double>(),
@@ -1087,11 +1137,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
                          s.work.as<int32_t>(), t, tab32(s), p, o, dncalc);
     };
     if (lds) go(k_derivs_iso<float, D, true>); else go(k_derivs_iso<float, D, false>);
-  } else if (s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile && !compat) {
-    // search row order is held to round-off of the reference sums, not to
-    // their bits: the collapsed isotropic algebra in FP64
-    packFast64<D>(s);
-    SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
+  } else if (fast64) {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
                          s.recD.as<double2>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
@@ -1123,7 +1169,7 @@ template <int D>
 void omegaGradhD(Sim& s) {
   const int64_t n = s.nSlots;
   TabArgs t = tabArgs(s);
-  const bool lds = s.table.n <= kLdsTablePoints;
+  const bool lds = ldsTable(s);
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
@@ -1206,7 +1252,7 @@ template <int D>
 double idealHPassD(Sim& s, const sphgpu_hydro_params& hp, double tol, double* deltaH, unsigned long long* bits) {
   const int64_t n = s.nSlots;
   TabArgs t = tabArgs(s);
-  const bool lds = s.table.n <= kLdsTablePoints;
+  const bool lds = ldsTable(s);
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
@@ -1247,10 +1293,22 @@ void packRecords(Sim& s) {
 void sumDensity(Sim& s) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "sum density needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
-  if (!s.recValid) packRecords(s);
-  if (s.dim == 1) sumDensityD<1>(s);
-  else if (s.dim == 2) sumDensityD<2>(s);
-  else sumDensityD<3>(s);
+  bool secFresh = false;
+  if (!s.recValid) {
+    if (s.precision == SPHGPU_PRECISION_MIXED && s.isotropic) {
+      // records and the mixed kernel's record in one pass
+      if (s.dim == 1) packD<1>(s, 1);
+      else if (s.dim == 2) packD<2>(s, 1);
+      else packD<3>(s, 1);
+      s.recValid = true;
+      secFresh = true;
+    } else {
+      packRecords(s);
+    }
+  }
+  if (s.dim == 1) sumDensityD<1>(s, secFresh);
+  else if (s.dim == 2) sumDensityD<2>(s, secFresh);
+  else sumDensityD<3>(s, secFresh);
   s.timingPending |= 2u;
   s.derivsValid = false;
 }
