@@ -276,6 +276,9 @@ __global__ void k_iperm(const int32_t* __restrict__ perm, int32_t* __restrict__ 
 }
 
 // Row writer: keeps four entries in registers and stores them as one int4.
+// Each entry is shifted in at w, so after every fourth the buffer holds them
+// oldest first with no branch on the position; the last partial chunk is
+// aligned once per row.  Entries past the row length are zero.
 struct RowWriter {
   int4 buf;
   int cnt;
@@ -284,13 +287,19 @@ struct RowWriter {
   int4* out;
   __device__ __forceinline__ RowWriter(int4* o, int64_t n_, int64_t k_, int cap_) : buf{0, 0, 0, 0}, cnt(0), cap(cap_), n(n_), k(k_), out(o) {}
   __device__ __forceinline__ void push(int32_t v) {
-    const int l = cnt & 3;
-    if (l == 0) buf.x = v; else if (l == 1) buf.y = v; else if (l == 2) buf.z = v; else buf.w = v;
-    if (l == 3 && cnt < cap) out[ellChunk(cnt >> 2, n, k)] = buf;
+    buf.x = buf.y;
+    buf.y = buf.z;
+    buf.z = buf.w;
+    buf.w = v;
     ++cnt;
+    if ((cnt & 3) == 0 && cnt <= cap) out[ellChunk((cnt >> 2) - 1, n, k)] = buf;
   }
   __device__ __forceinline__ void finish() {
-    if ((cnt & 3) != 0 && cnt <= cap) out[ellChunk(cnt >> 2, n, k)] = buf;
+    const int r = cnt & 3;
+    if (r == 0 || cnt > cap) return;
+    const int4 o = r == 1 ? make_int4(buf.w, 0, 0, 0)
+                          : (r == 2 ? make_int4(buf.z, buf.w, 0, 0) : make_int4(buf.y, buf.z, buf.w, 0));
+    out[ellChunk(cnt >> 2, n, k)] = o;
   }
 };
 
