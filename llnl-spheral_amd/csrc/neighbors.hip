@@ -437,9 +437,11 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
           for (int32_t t = t0; t < t1; ++t) {
             const float4 cj = sCand[t];
             const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
-            const float q = rx * rx + ry * ry + rz * rz;
-            const float ti = hi2 * q, tj = cj.w * q;
-            const float tmin = ti < tj ? ti : tj;
+            // FP32 estimate of min(h_i^2, h_j^2) |r|^2: fused products (any
+            // rounding change stays inside the band that takes the exact test);
+            // q min(a, b) = min(q a, q b) exactly for q >= 0
+            const float q = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+            const float tmin = q * fminf(hi2, cj.w);
             bool in = tmin < a.lo32;
             if (!in && tmin <= a.hi32) {
               const int32_t sj = sSlot[t] & 0x7fffffff;
