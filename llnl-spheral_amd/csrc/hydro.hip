@@ -73,16 +73,28 @@ __device__ __forceinline__ void writeFast64Rec(const double* r, int64_t nInt, do
   p[7] = make_double2(0.0, 0.0);
 }
 
-// SEC: also write the pair kernel's record from the values in registers
-// (0 none, 1 FP32 mixed, 2 FP64), saving a second pass over the records.
+// The density sum's neighbour record (DensRec: half the FP64 record, so that
+// the gathered array fits the Infinity Cache).
+template <int D>
+__device__ __forceinline__ void writeDensRec(const double* r, double4* __restrict__ p) {
+  using R = RL<true>;
+  static_assert(R::X == 0 && R::HINV == 3 && R::M == 4 && R::HDET == 5 && R::ORIG == 6, "DensRec layout");
+  p[0] = make_double4(r[R::X], D > 1 ? r[R::X + 1] : 0.0, D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
+  p[1] = make_double4(r[R::M], r[R::HDET], r[R::ORIG], 0.0);
+}
+
+// SEC (bits): also write, from the values in registers, the mixed pair
+// kernel's record (1), the FP64 isotropic pair kernel's record (2) and the
+// density sum's record (4), saving passes over the records.
 template <int D, bool ISO, int SEC = 0>
 __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double* __restrict__ pos,
                        const double* __restrict__ vel, const double* __restrict__ mass, const double* __restrict__ rho,
                        const double* __restrict__ P, const double* __restrict__ cs, const double* __restrict__ H,
                        const double* __restrict__ omega, const double* __restrict__ fCl, const double* __restrict__ fCq,
                        const double* __restrict__ fsh, double* __restrict__ rec, int64_t nInt = 0,
-                       void* __restrict__ sec = nullptr) {
+                       void* __restrict__ sec = nullptr, void* __restrict__ sec2 = nullptr) {
   static_assert(SEC == 0 || ISO, "the pair kernels' records are isotropic");
+  double4* const dens = static_cast<double4*>(sec2);
   using R = RL<ISO>;
   constexpr int NS = NSym<D>::value;
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -96,8 +108,9 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
     double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
 #pragma unroll
     for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-    if constexpr (SEC == 1) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
-    if constexpr (SEC == 2) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
+    if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
+    if constexpr ((SEC & 2) != 0) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
+    if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, dens + k * 2);
     return;
   }
   for (int c = 0; c < D; ++c) { r[R::X + c] = pos[i * D + c]; r[R::V + c] = vel[i * D + c]; }
@@ -125,8 +138,9 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
   double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
 #pragma unroll
   for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-  if constexpr (SEC == 1) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
-  if constexpr (SEC == 2) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
+  if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
+  if constexpr ((SEC & 2) != 0) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
+  if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, dens + k * 2);
 }
 
 // Table: per point [Ak Bk Ck Ag Bg Cg].
@@ -280,9 +294,13 @@ template <int D, bool ISO> struct DNbr {
   S<D> H;
   double hdet, m, orig;
 };
-template <int D, bool ISO> __device__ __forceinline__ DNbr<D, ISO> loadDNbr(const double* __restrict__ rec, int32_t j) {
+// stride: RL<ISO>::STRIDE for the full records, DensRec::STRIDE for the density
+// sum's (isotropic, same offsets)
+template <int D, bool ISO>
+__device__ __forceinline__ DNbr<D, ISO> loadDNbr(const double* __restrict__ rec, int32_t j,
+                                                 int stride = RL<ISO>::STRIDE) {
   using R = RL<ISO>;
-  const double* r = rec + int64_t(j) * R::STRIDE;
+  const double* r = rec + int64_t(j) * stride;
   DNbr<D, ISO> o;
   for (int c = 0; c < D; ++c) o.x.c[c] = r[R::X + c];
   if constexpr (ISO) {
@@ -296,11 +314,14 @@ template <int D, bool ISO> __device__ __forceinline__ DNbr<D, ISO> loadDNbr(cons
   return o;
 }
 
+// Neighbours come from nrec with stride nstride (the density records for
+// isotropic H, else the full records).
 template <int D, bool ISO, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec, const int32_t* __restrict__ rowCount,
                                                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
                                                         const int32_t* __restrict__ work, TabArgs t,
-                                                        double* __restrict__ rhoOut) {
+                                                        double* __restrict__ rhoOut, const double* __restrict__ nrec,
+                                                        int nstride) {
   using R = RL<ISO>;
   extern __shared__ double lds[];
   const double* tab = stageTable<LDS>(lds, t);
@@ -320,10 +341,10 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
   int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
   int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
   DNbr<D, ISO> nx;
-  if (len > 0) nx = loadDNbr<D, ISO>(rec, cur.x);
+  if (len > 0) nx = loadDNbr<D, ISO>(nrec, cur.x, nstride);
   for (int w = 0; w < len; ++w) {
     const DNbr<D, ISO> nb = nx;
-    if (w + 1 < len) nx = loadDNbr<D, ISO>(rec, nextIndex(nb4, n, k, len, w + 1, cur, nxt));
+    if (w + 1 < len) nx = loadDNbr<D, ISO>(nrec, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nstride);
     if (!selfDone && nb.orig > oi) { acc += self; selfDone = true; }
     const V<D> rij = vsub(me.x, nb.x);
     const double etaj = vmag(hdot<D, ISO>(nb.H, rij));
@@ -703,6 +724,12 @@ __global__ void k_pack_fast64(const double* __restrict__ rec, int64_t n, int64_t
   if (k < n) writeFast64Rec<D>(rec + k * RL<true>::STRIDE, nInt, recD + k * 8);
 }
 
+template <int D>
+__global__ void k_pack_dens(const double* __restrict__ rec, int64_t n, double4* __restrict__ recS) {
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k < n) writeDensRec<D>(rec + k * RL<true>::STRIDE, recS + k * 2);
+}
+
 // FP32 table in LDS: 8 floats per point.
 template <bool LDS> __device__ __forceinline__ const float* stageTable32(float* lds, const TabArgs& t) {
   if constexpr (LDS) {
@@ -987,8 +1014,9 @@ TabArgs tabArgs(const Sim& s) {
                  s.dNperh.as<double>(), s.dWsum.as<double>()};
 }
 
-// Records from the caller-order fields; sec 1 / 2 also writes the mixed /
-// FP64 isotropic pair kernel's record in the same pass.
+// Records from the caller-order fields; sec bits 1 / 2 / 4 also write the
+// mixed or FP64 isotropic pair kernel's record and the density sum's record
+// in the same pass.
 template <int D>
 void packD(Sim& s, int sec = 0) {
   const int64_t n = s.nSlots;
@@ -997,24 +1025,39 @@ void packD(Sim& s, int sec = 0) {
   const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
   s.rec.ensure(sizeof(double) * size_t(n) * stride);
   void* secp = nullptr;
-  if (sec == 1) {
+  void* densp = nullptr;
+  SG_REQUIRE((sec & 3) != 3, SPHGPU_ERR_STATE, "one pair-kernel record per pack");
+  if (sec & 1) {
     s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(n, 1)));
     secp = s.recM.p;
-  } else if (sec == 2) {
+  } else if (sec & 2) {
     s.recD.ensure(sizeof(double) * 16 * size_t(std::max<int64_t>(n, 1)));
     secp = s.recD.p;
   }
+  if (sec & 4) {
+    s.recS.ensure(sizeof(double) * DensRec::STRIDE * size_t(std::max<int64_t>(n, 1)));
+    densp = s.recS.p;
+  }
+  s.recSValid = (sec & 4) != 0;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.perm.as<int32_t>(), n, s.pos.as<double>(),
                        s.vel.as<double>(), s.mass.as<double>(), s.rho.as<double>(), s.P.as<double>(),
                        s.cs.as<double>(), s.H.as<double>(), s.omega.as<double>(),
                        s.hasMultipliers ? s.fCl.as<double>() : nullptr, s.hasMultipliers ? s.fCq.as<double>() : nullptr,
-                       s.hasMultipliers ? s.fshear.as<double>() : nullptr, s.rec.as<double>(), s.nInt, secp);
+                       s.hasMultipliers ? s.fshear.as<double>() : nullptr, s.rec.as<double>(), s.nInt, secp, densp);
   };
-  if (!iso) go(k_pack<D, false, 0>);
-  else if (sec == 1) go(k_pack<D, true, 1>);
-  else if (sec == 2) go(k_pack<D, true, 2>);
-  else go(k_pack<D, true, 0>);
+  if (!iso) {
+    go(k_pack<D, false, 0>);
+  } else {
+    switch (sec) {
+      case 1: go(k_pack<D, true, 1>); break;
+      case 2: go(k_pack<D, true, 2>); break;
+      case 4: go(k_pack<D, true, 4>); break;
+      case 5: go(k_pack<D, true, 5>); break;
+      case 6: go(k_pack<D, true, 6>); break;
+      default: go(k_pack<D, true, 0>); break;
+    }
+  }
   SG_CHECK_HIP(hipGetLastError());
 }
 
@@ -1062,10 +1105,20 @@ void sumDensityD(Sim& s, bool secFresh) {
     SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
     return;
   }
+  // isotropic H: neighbours from the 64-B density records
+  if (s.isotropic && !s.recSValid) {
+    s.recS.ensure(sizeof(double) * DensRec::STRIDE * size_t(std::max<int64_t>(n, 1)));
+    hipLaunchKernelGGL(k_pack_dens<D>, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), n,
+                       s.recS.as<double4>());
+    SG_CHECK_HIP(hipGetLastError());
+    s.recSValid = true;
+  }
+  const double* nrec = s.isotropic ? s.recS.as<double>() : s.rec.as<double>();
+  const int nstride = s.isotropic ? DensRec::STRIDE : RL<false>::STRIDE;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
-                       s.rho.as<double>());
+                       s.rho.as<double>(), nrec, nstride);
   };
   if (s.isotropic) { if (lds) go(k_sum_density<D, true, true>); else go(k_sum_density<D, true, false>); }
   else { if (lds) go(k_sum_density<D, false, true>); else go(k_sum_density<D, false, false>); }
@@ -1295,13 +1348,14 @@ void sumDensity(Sim& s) {
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   bool secFresh = false;
   if (!s.recValid) {
-    if (s.precision == SPHGPU_PRECISION_MIXED && s.isotropic) {
-      // records and the mixed kernel's record in one pass
-      if (s.dim == 1) packD<1>(s, 1);
-      else if (s.dim == 2) packD<2>(s, 1);
-      else packD<3>(s, 1);
+    if (s.isotropic) {
+      // the records and the density kernel's own record in one pass
+      const int sec = s.precision == SPHGPU_PRECISION_MIXED ? 1 : 4;
+      if (s.dim == 1) packD<1>(s, sec);
+      else if (s.dim == 2) packD<2>(s, sec);
+      else packD<3>(s, sec);
       s.recValid = true;
-      secFresh = true;
+      secFresh = sec == 1;
     } else {
       packRecords(s);
     }

@@ -27,4 +27,11 @@ template <> struct RL<false> {
                        P = 19, SOM = 20, FCL = 21, FCQ = 22, FSH = 23;
 };
 
+// The FP64 density sum's neighbour record (isotropic H): the first eight
+// doubles of RL<true> at the same offsets (x, 1/h, m, det H, caller index,
+// then 0), 64 B, so the gathered array is half the size.
+struct DensRec {
+  static constexpr int STRIDE = 8;
+};
+
 }  // namespace sphgpu
