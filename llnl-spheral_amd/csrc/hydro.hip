@@ -58,19 +58,22 @@ __device__ __forceinline__ void writeMixedRec(const double* r, int64_t nInt, flo
                      D > 1 ? float(r[R::V + 1]) : 0.0f);
   p[3] = make_float4(D > 2 ? float(r[R::V + 2]) : 0.0f, float(r[R::CS]), float(r[R::PRHO]), float(r[R::MOR]));
 }
+// The FP64 isotropic pair kernel's record, 96 B per slot in two planes so that
+// the gathered arrays (230 MB at 2.4 M slots) fit the 256 MB Infinity Cache:
+// plane a (64 B): x y z, 1/h (negative for ghosts), m, 1/rho, c, P/rho^2/omega;
+// plane b (32 B): v, 0.  det H and m/rho are recomputed by the loader.
 template <int D>
-__device__ __forceinline__ void writeFast64Rec(const double* r, int64_t nInt, double2* __restrict__ p) {
+__device__ __forceinline__ void writeFast64Rec(const double* r, int64_t nInt, double2* __restrict__ a,
+                                               double2* __restrict__ b) {
   using R = RL<true>;
   const bool ghost = r[R::ORIG] >= double(nInt);
   const double rho = r[R::RHO];
-  p[0] = make_double2(r[R::X], D > 1 ? r[R::X + 1] : 0.0);
-  p[1] = make_double2(D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
-  p[2] = make_double2(ghost ? -r[R::HDET] : r[R::HDET], r[R::M]);
-  p[3] = make_double2(rho != 0.0 ? 1.0 / rho : 0.0, r[R::V]);
-  p[4] = make_double2(D > 1 ? r[R::V + 1] : 0.0, D > 2 ? r[R::V + 2] : 0.0);
-  p[5] = make_double2(r[R::CS], r[R::PRHO]);
-  p[6] = make_double2(r[R::MOR], 0.0);
-  p[7] = make_double2(0.0, 0.0);
+  a[0] = make_double2(r[R::X], D > 1 ? r[R::X + 1] : 0.0);
+  a[1] = make_double2(D > 2 ? r[R::X + 2] : 0.0, ghost ? -r[R::HINV] : r[R::HINV]);
+  a[2] = make_double2(r[R::M], rho != 0.0 ? 1.0 / rho : 0.0);
+  a[3] = make_double2(r[R::CS], r[R::PRHO]);
+  b[0] = make_double2(r[R::V], D > 1 ? r[R::V + 1] : 0.0);
+  b[1] = make_double2(D > 2 ? r[R::V + 2] : 0.0, 0.0);
 }
 
 // The density sum's neighbour record (DensRec: half the FP64 record, so that
@@ -109,7 +112,8 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
 #pragma unroll
     for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
     if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
-    if constexpr ((SEC & 2) != 0) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
+    if constexpr ((SEC & 2) != 0)
+      writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 4, static_cast<double2*>(sec) + 4 * n + k * 2);
     if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, dens + k * 2);
     return;
   }
@@ -139,7 +143,8 @@ __global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double
 #pragma unroll
   for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
   if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
-  if constexpr ((SEC & 2) != 0) writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 8);
+  if constexpr ((SEC & 2) != 0)
+    writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 4, static_cast<double2*>(sec) + 4 * n + k * 2);
   if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, dens + k * 2);
 }
 
@@ -690,7 +695,8 @@ template <typename Real> struct INbr {
   Real hinv, hdet, m, invRho, v[3], cs, prho, mor;
   bool ghost;
 };
-__device__ __forceinline__ void loadIso(const float4* __restrict__ recM, int32_t j, INbr<float>& o) {
+template <int D>
+__device__ __forceinline__ void loadIso(const float4* __restrict__ recM, int64_t, int32_t j, INbr<float>& o) {
   const float4* p = recM + int64_t(j) * 4;
   const float4 f0 = p[0], f1 = p[1], f2 = p[2], f3 = p[3];
   o.x[0] = __hiloint2double(__float_as_int(f0.y), __float_as_int(f0.x));
@@ -702,14 +708,22 @@ __device__ __forceinline__ void loadIso(const float4* __restrict__ recM, int32_t
   o.m = f2.x; o.invRho = f2.y; o.v[0] = f2.z; o.v[1] = f2.w;
   o.v[2] = f3.x; o.cs = f3.y; o.prho = f3.z; o.mor = f3.w;
 }
-__device__ __forceinline__ void loadIso(const double2* __restrict__ recD, int32_t j, INbr<double>& o) {
-  const double2* p = recD + int64_t(j) * 8;
-  const double2 d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3], d4 = p[4], d5 = p[5], d6 = p[6];
-  o.x[0] = d0.x; o.x[1] = d0.y; o.x[2] = d1.x; o.hinv = d1.y;
-  o.ghost = d2.x < 0.0;
-  o.hdet = fabs(d2.x);
-  o.m = d2.y; o.invRho = d3.x; o.v[0] = d3.y; o.v[1] = d4.x; o.v[2] = d4.y;
-  o.cs = d5.x; o.prho = d5.y; o.mor = d6.x;
+// det H of an exactly isotropic H is (1/h)^D evaluated as sdet evaluates it
+// (the off-diagonal products are exact zeros), so the value is bitwise the
+// packed one; m/rho is m * (1/rho), within an ulp.
+template <int D>
+__device__ __forceinline__ void loadIso(const double2* __restrict__ recD, int64_t n, int32_t j, INbr<double>& o) {
+  const double2* a = recD + int64_t(j) * 4;
+  const double2* b = recD + 4 * n + int64_t(j) * 2;
+  const double2 d0 = a[0], d1 = a[1], d2 = a[2], d3 = a[3], e0 = b[0], e1 = b[1];
+  o.x[0] = d0.x; o.x[1] = d0.y; o.x[2] = d1.x;
+  o.ghost = d1.y < 0.0;
+  const double hinv = fabs(d1.y);
+  o.hinv = hinv;
+  o.hdet = D == 1 ? hinv : (D == 2 ? hinv * hinv : (hinv * hinv) * hinv);
+  o.m = d2.x; o.invRho = d2.y; o.cs = d3.x; o.prho = d3.y;
+  o.mor = d2.x * d2.y;
+  o.v[0] = e0.x; o.v[1] = e0.y; o.v[2] = e1.x;
 }
 
 template <int D>
@@ -721,7 +735,7 @@ __global__ void k_pack_mixed(const double* __restrict__ rec, int64_t n, int64_t 
 template <int D>
 __global__ void k_pack_fast64(const double* __restrict__ rec, int64_t n, int64_t nInt, double2* __restrict__ recD) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k < n) writeFast64Rec<D>(rec + k * RL<true>::STRIDE, nInt, recD + k * 8);
+  if (k < n) writeFast64Rec<D>(rec + k * RL<true>::STRIDE, nInt, recD + k * 4, recD + 4 * n + k * 2);
 }
 
 template <int D>
@@ -840,10 +854,10 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
   int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
   int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
   INbr<float> nx;
-  if (len > 0) loadIso(recM, cur.x, nx);
+  if (len > 0) loadIso<D>(recM, n, cur.x, nx);
   for (int w = 0; w < len; ++w) {
     const INbr<float> nb = nx;
-    if (w + 1 < len) loadIso(recM, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nx);
+    if (w + 1 < len) loadIso<D>(recM, n, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nx);
     float r2 = 0.0f;
     for (int c = 0; c < D; ++c) {
       const float rc = float(xi[c] - nb.x[c]);
@@ -982,13 +996,13 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
     };
     INbr<Real> nx;
     int32_t jn = cur.x;
-    if (len > 0) loadIso(recN, jn, nx);
+    if (len > 0) loadIso<D>(recN, n, jn, nx);
     for (int w = 0; w < len; ++w) {
       const INbr<Real> nb = nx;
       const int32_t j = jn;
       if (w + 1 < len) {
         jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
-        loadIso(recN, jn, nx);
+        loadIso<D>(recN, n, jn, nx);
       }
       pair(nb, j);
     }
@@ -1031,7 +1045,7 @@ void packD(Sim& s, int sec = 0) {
     s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(n, 1)));
     secp = s.recM.p;
   } else if (sec & 2) {
-    s.recD.ensure(sizeof(double) * 16 * size_t(std::max<int64_t>(n, 1)));
+    s.recD.ensure(sizeof(double) * 12 * size_t(std::max<int64_t>(n, 1)));
     secp = s.recD.p;
   }
   if (sec & 4) {
@@ -1073,7 +1087,7 @@ void packMixed(Sim& s) {
 
 template <int D>
 void packFast64(Sim& s) {
-  s.recD.ensure(sizeof(double) * 16 * size_t(std::max<int64_t>(s.nSlots, 1)));
+  s.recD.ensure(sizeof(double) * 12 * size_t(std::max<int64_t>(s.nSlots, 1)));
   hipLaunchKernelGGL(k_pack_fast64<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, s.stream, s.rec.as<double>(),
                      s.nSlots, s.nInt, s.recD.as<double2>());
   SG_CHECK_HIP(hipGetLastError());
