@@ -431,6 +431,7 @@ struct OutArgs {
   double* base;      // field-major blocks, each n rows
   int64_t n;         // rows per block (sorted slots)
   int off[SPHGPU_NUM_DERIV_FIELDS];
+  int updateOnly;    // store only the fields an integrator update reads
   double* pairDv;    // compatible energy: the pair term of every row entry in the node's own
                      // frame, component c at pairDv[c*plane + ellIndex(w, n, k)]; null otherwise
   int64_t plane;
@@ -501,20 +502,28 @@ __device__ __forceinline__ void epilogue(int64_t k, int len, const Self<D>& me, 
   const double hinv = idealHinv(t, wNSo, Hi.c[0], p.nPerh, p.hmin, p.hmax);
 
   // outputs
-  *outRow(o, SPHGPU_D_RHO_SUM, k, 1) = rhoSum;
-  *outRow(o, SPHGPU_D_NORMALIZATION, k, 1) = norm;
+  // the fields an integrator update reads
   *outRow(o, SPHGPU_D_DRHODT, k, 1) = DrhoDt;
   *outRow(o, SPHGPU_D_DEPSDT, k, 1) = DepsDt;
+  {
+    double* dx = outRow(o, SPHGPU_D_DXDT, k, D);
+    double* dv = outRow(o, SPHGPU_D_DVDT, k, D);
+    for (int c = 0; c < D; ++c) { dx[c] = DxDt.c[c]; dv[c] = DvDt.c[c]; }
+    double* hid = outRow(o, SPHGPU_D_HIDEAL, k, NS);
+    for (int a2 = 0, s2 = 0; a2 < D; ++a2)
+      for (int b2 = a2; b2 < D; ++b2, ++s2) hid[s2] = (a2 == b2 ? 1.0 : 0.0) * hinv;
+  }
+  if (o.updateOnly) return;
+  *outRow(o, SPHGPU_D_RHO_SUM, k, 1) = rhoSum;
+  *outRow(o, SPHGPU_D_NORMALIZATION, k, 1) = norm;
   *outRow(o, SPHGPU_D_MAX_VISCOUS_PRESSURE, k, 1) = a.maxVP;
   *outRow(o, SPHGPU_D_EFF_VISCOUS_PRESSURE, k, 1) = a.effVP;
   *outRow(o, SPHGPU_D_VISCOUS_WORK, k, 1) = a.viscW;
   *outRow(o, SPHGPU_D_XSPH_WEIGHT_SUM, k, 1) = xsphW;
   *outRow(o, SPHGPU_D_WEIGHTED_NEIGHBOR_SUM, k, 1) = wNSo;
   {
-    double* dx = outRow(o, SPHGPU_D_DXDT, k, D);
-    double* dv = outRow(o, SPHGPU_D_DVDT, k, D);
     double* xv = outRow(o, SPHGPU_D_XSPH_DELTA_V, k, D);
-    for (int c = 0; c < D; ++c) { dx[c] = DxDt.c[c]; dv[c] = DvDt.c[c]; xv[c] = xsphDV.c[c]; }
+    for (int c = 0; c < D; ++c) xv[c] = xsphDV.c[c];
   }
   {
     double* g = outRow(o, SPHGPU_D_DVDX, k, D * D);
@@ -525,15 +534,8 @@ __device__ __forceinline__ void epilogue(int64_t k, int len, const Self<D>& me, 
   }
   {
     double* dh = outRow(o, SPHGPU_D_DHDT, k, NS);
-    double* hid = outRow(o, SPHGPU_D_HIDEAL, k, NS);
     double* m2 = outRow(o, SPHGPU_D_MASS_SECOND_MOMENT, k, NS);
-    int s2 = 0;
-    for (int a2 = 0; a2 < D; ++a2)
-      for (int b2 = a2; b2 < D; ++b2, ++s2) {
-        dh[s2] = DHDt.c[s2];
-        hid[s2] = (a2 == b2 ? 1.0 : 0.0) * hinv;
-        m2[s2] = m2nd.c[s2];
-      }
+    for (int c = 0; c < NS; ++c) { dh[c] = DHDt.c[c]; m2[c] = m2nd.c[c]; }
   }
 }
 
@@ -1143,7 +1145,7 @@ void sumDensityD(Sim& s, bool secFresh) {
 }
 
 template <int D>
-void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
+void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   const int64_t n = s.nSlots;
   hipStream_t st = s.stream;
   const int total = derivTotalDoubles(s.dim);
@@ -1189,6 +1191,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp) {
   OutArgs o{};
   o.base = s.out.as<double>();
   o.n = n;
+  o.updateOnly = updateOnly ? 1 : 0;
   for (int f = 0; f < SPHGPU_NUM_DERIV_FIELDS; ++f) o.off[f] = derivOffsetDoubles(s.dim, f);
   if (compat) {
     // pairAccelerations: the own-frame pair term of every row entry
@@ -1381,7 +1384,7 @@ void sumDensity(Sim& s) {
   s.derivsValid = false;
 }
 
-void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "evaluateDerivatives needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   SG_REQUIRE(p.compatibleEnergyEvolution == 0 || p.precision == SPHGPU_PRECISION_FP64, SPHGPU_ERR_UNSUPPORTED,
@@ -1392,9 +1395,9 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p) {
              "unknown precision");
   SG_REQUIRE(p.nPerh > 0.0, SPHGPU_ERR_ARG, "nPerh must be positive");
   s.derivsValid = false;
-  if (s.dim == 1) evalDerivsD<1>(s, p);
-  else if (s.dim == 2) evalDerivsD<2>(s, p);
-  else evalDerivsD<3>(s, p);
+  if (s.dim == 1) evalDerivsD<1>(s, p, updateOnly);
+  else if (s.dim == 2) evalDerivsD<2>(s, p, updateOnly);
+  else evalDerivsD<3>(s, p, updateOnly);
   s.timingPending |= 4u;
   s.derivsValid = true;
   s.pairsValid = true;

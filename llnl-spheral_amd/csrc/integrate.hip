@@ -31,8 +31,6 @@
 
 namespace sphgpu {
 
-void evalDerivs(Sim& s, const sphgpu_hydro_params& p);
-
 namespace {
 
 constexpr int kBlock = 256;
@@ -421,9 +419,11 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
   const Bounds bd{ip.rhoMin, ip.rhoMax, 1.0 / hp.hmax, 1.0 / hp.hmin};
   const Eos e = makeEos(ip.gamma, ip.Pmin, ip.Pmax, ip.pressureFloor);
 
-  // derivatives at t (SPHHydroBase::initialize first: the grad-h correction)
+  // derivatives at t (SPHHydroBase::initialize first: the grad-h correction); the
+  // midpoint pass below rewrites every field before anything but the trial
+  // update reads one, so this pass stores only the fields the update reads
   if (ip.gradhCorrection) omegaGradh(s);
-  evalDerivs(s, hp);
+  evalDerivs(s, hp, true);
   // state0 (the policy reads every row's step-start velocity, ghosts included)
   const size_t n = size_t(s.nInt), nv = compat ? size_t(s.nTot) : n;
   const size_t bytes[5] = {sizeof(double) * n * D, sizeof(double) * nv * D, sizeof(double) * n, sizeof(double) * n,

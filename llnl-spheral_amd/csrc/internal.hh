@@ -234,7 +234,9 @@ void buildNeighbors(Sim& s, double kernelExtent);
 // hydro.hip
 void packRecords(Sim& s);
 void sumDensity(Sim& s);
-void evalDerivs(Sim& s, const sphgpu_hydro_params& p);
+// updateOnly: store only DxDt, DvDt, DepsDt, DrhoDt and Hideal (the fields an
+// integrator update reads); the other fields keep what they held
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly = false);
 // probe.hip
 double copyProbe(Sim& s, int64_t bytes, int repeats);
 // ghosts.hip
