@@ -26,6 +26,7 @@
 #include "internal.hh"
 #include "dgeom.hh"
 #include "records.hh"
+#include "pack.hh"
 #include "ell.hh"
 #include "eigen.hh"
 #include <cstring>
@@ -38,114 +39,13 @@ constexpr int kBlock = 256;
 inline int nblocks(int64_t n, int bs = kBlock) { return int(std::max<int64_t>(1, (n + bs - 1) / bs)); }
 
 // ---------------------------------------------------------------------------
-// Records
+// Record pack: every slot, or the `count` slots listed in `slots`.
 // ---------------------------------------------------------------------------
-// The isotropic pair kernels' records, from an FP64 record r (isotropic
-// layout): FP32, 64 B, for the mixed kernels (positions keep their FP64 bits);
-// FP64, 128 B, for the FP64 kernel.  det H is negative for ghost rows.
-template <int D>
-__device__ __forceinline__ void writeMixedRec(const double* r, int64_t nInt, float4* __restrict__ p) {
-  using R = RL<true>;
-  const double x0 = r[R::X], x1 = D > 1 ? r[R::X + 1] : 0.0, x2 = D > 2 ? r[R::X + 2] : 0.0;
-  const bool ghost = r[R::ORIG] >= double(nInt);
-  const float hdet = float(r[R::HDET]);
-  p[0] = make_float4(__int_as_float(__double2loint(x0)), __int_as_float(__double2hiint(x0)),
-                     __int_as_float(__double2loint(x1)), __int_as_float(__double2hiint(x1)));
-  p[1] = make_float4(__int_as_float(__double2loint(x2)), __int_as_float(__double2hiint(x2)), float(r[R::HINV]),
-                     ghost ? -hdet : hdet);
-  const double rho = r[R::RHO];
-  p[2] = make_float4(float(r[R::M]), rho != 0.0 ? float(1.0 / rho) : 0.0f, float(r[R::V]),
-                     D > 1 ? float(r[R::V + 1]) : 0.0f);
-  p[3] = make_float4(D > 2 ? float(r[R::V + 2]) : 0.0f, float(r[R::CS]), float(r[R::PRHO]), float(r[R::MOR]));
-}
-// The FP64 isotropic pair kernel's record, 96 B per slot in two planes so that
-// the gathered arrays (230 MB at 2.4 M slots) fit the 256 MB Infinity Cache:
-// plane a (64 B): x y z, 1/h (negative for ghosts), m, 1/rho, c, P/rho^2/omega;
-// plane b (32 B): v, 0.  det H and m/rho are recomputed by the loader.
-template <int D>
-__device__ __forceinline__ void writeFast64Rec(const double* r, int64_t nInt, double2* __restrict__ a,
-                                               double2* __restrict__ b) {
-  using R = RL<true>;
-  const bool ghost = r[R::ORIG] >= double(nInt);
-  const double rho = r[R::RHO];
-  a[0] = make_double2(r[R::X], D > 1 ? r[R::X + 1] : 0.0);
-  a[1] = make_double2(D > 2 ? r[R::X + 2] : 0.0, ghost ? -r[R::HINV] : r[R::HINV]);
-  a[2] = make_double2(r[R::M], rho != 0.0 ? 1.0 / rho : 0.0);
-  a[3] = make_double2(r[R::CS], r[R::PRHO]);
-  b[0] = make_double2(r[R::V], D > 1 ? r[R::V + 1] : 0.0);
-  b[1] = make_double2(D > 2 ? r[R::V + 2] : 0.0, 0.0);
-}
-
-// The density sum's neighbour record (DensRec: half the FP64 record, so that
-// the gathered array fits the Infinity Cache).
-template <int D>
-__device__ __forceinline__ void writeDensRec(const double* r, double4* __restrict__ p) {
-  using R = RL<true>;
-  static_assert(R::X == 0 && R::HINV == 3 && R::M == 4 && R::HDET == 5 && R::ORIG == 6, "DensRec layout");
-  p[0] = make_double4(r[R::X], D > 1 ? r[R::X + 1] : 0.0, D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
-  p[1] = make_double4(r[R::M], r[R::HDET], r[R::ORIG], 0.0);
-}
-
-// SEC (bits): also write, from the values in registers, the mixed pair
-// kernel's record (1), the FP64 isotropic pair kernel's record (2) and the
-// density sum's record (4), saving passes over the records.
 template <int D, bool ISO, int SEC = 0>
-__global__ void k_pack(const int32_t* __restrict__ perm, int64_t n, const double* __restrict__ pos,
-                       const double* __restrict__ vel, const double* __restrict__ mass, const double* __restrict__ rho,
-                       const double* __restrict__ P, const double* __restrict__ cs, const double* __restrict__ H,
-                       const double* __restrict__ omega, const double* __restrict__ fCl, const double* __restrict__ fCq,
-                       const double* __restrict__ fsh, double* __restrict__ rec, int64_t nInt = 0,
-                       void* __restrict__ sec = nullptr, void* __restrict__ sec2 = nullptr) {
-  static_assert(SEC == 0 || ISO, "the pair kernels' records are isotropic");
-  double4* const dens = static_cast<double4*>(sec2);
-  using R = RL<ISO>;
-  constexpr int NS = NSym<D>::value;
-  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k >= n) return;
-  const int64_t i = perm[k];
-  double r[R::STRIDE];
-#pragma unroll
-  for (int c = 0; c < R::STRIDE; ++c) r[c] = 0.0;
-  if (i < 0) {                      // dead slot (culled ghost): no row holds it
-    r[R::ORIG] = 2147483648.0;
-    double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
-#pragma unroll
-    for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-    if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
-    if constexpr ((SEC & 2) != 0)
-      writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 4, static_cast<double2*>(sec) + 4 * n + k * 2);
-    if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, dens + k * 2);
-    return;
-  }
-  for (int c = 0; c < D; ++c) { r[R::X + c] = pos[i * D + c]; r[R::V + c] = vel[i * D + c]; }
-  S<D> h;
-  for (int c = 0; c < NS; ++c) h.c[c] = H[i * NS + c];
-  const double rh = rho[i], m = mass[i], p = P[i];
-  const double som = safeInv(omega[i], 1.0e-30);
-  if constexpr (ISO) {
-    r[RL<true>::HINV] = h.c[0];
-  } else {
-    for (int c = 0; c < NS; ++c) r[RL<false>::H + c] = h.c[c];
-    r[RL<false>::FCL] = fCl ? fCl[i] : 1.0;
-    r[RL<false>::FCQ] = fCq ? fCq[i] : 1.0;
-    r[RL<false>::FSH] = fsh ? fsh[i] : 1.0;
-  }
-  r[R::HDET] = sdet(h);
-  r[R::M] = m;
-  r[R::ORIG] = double(i);
-  r[R::RHO] = rh;
-  r[R::CS] = cs[i];
-  r[R::P] = p;
-  r[R::SOM] = som;
-  r[R::PRHO] = som * p / (rh * rh);
-  r[R::MOR] = m / rh;
-  double2* dst = reinterpret_cast<double2*>(rec + k * R::STRIDE);
-#pragma unroll
-  for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-  if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, nInt, static_cast<float4*>(sec) + k * 4);
-  if constexpr ((SEC & 2) != 0)
-    writeFast64Rec<D>(r, nInt, static_cast<double2*>(sec) + k * 4, static_cast<double2*>(sec) + 4 * n + k * 2);
-  if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, dens + k * 2);
+__global__ void k_pack(PackIn in, PackOut o, const int32_t* __restrict__ slots, int64_t count) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= count) return;
+  packSlot<D, ISO, SEC>(slots ? int64_t(slots[t]) : t, in, o);
 }
 
 // Table: per point [Ak Bk Ck Ag Bg Cg].
@@ -1030,39 +930,47 @@ TabArgs tabArgs(const Sim& s) {
                  s.dNperh.as<double>(), s.dWsum.as<double>()};
 }
 
-// Records from the caller-order fields; sec bits 1 / 2 / 4 also write the
-// mixed or FP64 isotropic pair kernel's record and the density sum's record
-// in the same pass.
-template <int D>
-void packD(Sim& s, int sec = 0) {
+}  // namespace
+
+PackIn packInputs(Sim& s) {
+  return PackIn{s.perm.as<int32_t>(), s.pos.as<double>(), s.vel.as<double>(), s.mass.as<double>(),
+                s.rho.as<double>(), s.P.as<double>(), s.cs.as<double>(), s.H.as<double>(), s.omega.as<double>(),
+                s.hasMultipliers ? s.fCl.as<double>() : nullptr, s.hasMultipliers ? s.fCq.as<double>() : nullptr,
+                s.hasMultipliers ? s.fshear.as<double>() : nullptr};
+}
+
+PackOut packOutputs(Sim& s, int sec) {
   const int64_t n = s.nSlots;
   const bool iso = s.isotropic;
   SG_REQUIRE(sec == 0 || iso, SPHGPU_ERR_STATE, "the pair kernels' records need H = h I");
+  SG_REQUIRE((sec & 3) != 3, SPHGPU_ERR_STATE, "one pair-kernel record per pack");
   const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
   s.rec.ensure(sizeof(double) * size_t(n) * stride);
-  void* secp = nullptr;
-  void* densp = nullptr;
-  SG_REQUIRE((sec & 3) != 3, SPHGPU_ERR_STATE, "one pair-kernel record per pack");
+  PackOut o{s.rec.as<double>(), nullptr, nullptr, n, s.nInt};
   if (sec & 1) {
     s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(n, 1)));
-    secp = s.recM.p;
+    o.sec = s.recM.p;
   } else if (sec & 2) {
     s.recD.ensure(sizeof(double) * 12 * size_t(std::max<int64_t>(n, 1)));
-    secp = s.recD.p;
+    o.sec = s.recD.p;
   }
   if (sec & 4) {
     s.recS.ensure(sizeof(double) * DensRec::STRIDE * size_t(std::max<int64_t>(n, 1)));
-    densp = s.recS.p;
+    o.sec2 = s.recS.p;
   }
   s.recSValid = (sec & 4) != 0;
+  return o;
+}
+
+template <int D>
+void launchPack(Sim& s, int sec, const int32_t* slots, int64_t count) {
+  if (count <= 0) return;
+  const PackOut o = packOutputs(s, sec);
+  const PackIn in = packInputs(s);
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.perm.as<int32_t>(), n, s.pos.as<double>(),
-                       s.vel.as<double>(), s.mass.as<double>(), s.rho.as<double>(), s.P.as<double>(),
-                       s.cs.as<double>(), s.H.as<double>(), s.omega.as<double>(),
-                       s.hasMultipliers ? s.fCl.as<double>() : nullptr, s.hasMultipliers ? s.fCq.as<double>() : nullptr,
-                       s.hasMultipliers ? s.fshear.as<double>() : nullptr, s.rec.as<double>(), s.nInt, secp, densp);
+    hipLaunchKernelGGL(kern, dim3(nblocks(count)), dim3(kBlock), 0, s.stream, in, o, slots, count);
   };
-  if (!iso) {
+  if (!s.isotropic) {
     go(k_pack<D, false, 0>);
   } else {
     switch (sec) {
@@ -1076,6 +984,38 @@ void packD(Sim& s, int sec = 0) {
   }
   SG_CHECK_HIP(hipGetLastError());
 }
+
+// Records from the caller-order fields; sec bits 1 / 2 / 4 also write the
+// mixed or FP64 isotropic pair kernel's record and the density sum's record
+// in the same pass.
+template <int D>
+void packD(Sim& s, int sec = 0) {
+  launchPack<D>(s, sec, nullptr, s.nSlots);
+}
+
+void packGhostRecords(Sim& s, int sec) {
+  if (s.nTot == s.nInt) { (void)packOutputs(s, sec); return; }
+  const int32_t* slots = s.iperm.as<int32_t>() + s.nInt;
+  if (s.dim == 1) launchPack<1>(s, sec, slots, s.nTot - s.nInt);
+  else if (s.dim == 2) launchPack<2>(s, sec, slots, s.nTot - s.nInt);
+  else launchPack<3>(s, sec, slots, s.nTot - s.nInt);
+}
+
+int pairRecordSec(const Sim& s, const sphgpu_hydro_params& hp) {
+  const bool tensile = hp.epsTensile != 0.0;
+  const bool compat = hp.compatibleEnergyEvolution != 0;
+  if (hp.precision == SPHGPU_PRECISION_MIXED) return 1;
+  // search row order is held to round-off of the reference sums, not to
+  // their bits: the collapsed isotropic algebra in FP64
+  return (s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile && !compat) ? 2 : 0;
+}
+
+int densityRecordSec(const Sim& s) {
+  if (!s.isotropic) return 0;
+  return s.precision == SPHGPU_PRECISION_MIXED ? 1 : 4;
+}
+
+namespace {
 
 Tab32 tab32(const Sim& s) { return Tab32{float(1.0 / s.table.step), float(s.table.extent), s.table.n}; }
 
@@ -1145,7 +1085,7 @@ void sumDensityD(Sim& s, bool secFresh) {
 }
 
 template <int D>
-void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
+void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly, int fresh) {
   const int64_t n = s.nSlots;
   hipStream_t st = s.stream;
   const int total = derivTotalDoubles(s.dim);
@@ -1161,18 +1101,16 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   if (mixed)
     SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
                "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
-  // search row order is held to round-off of the reference sums, not to
-  // their bits: the collapsed isotropic algebra in FP64
-  const bool fast64 = !mixed && s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile && !compat;
-  const int sec = mixed ? 1 : (fast64 ? 2 : 0);
+  const int sec = pairRecordSec(s, hp);
+  const bool fast64 = sec == 2;
 
   SG_CHECK_HIP(hipEventRecord(s.ev[4], st));
   if (!s.recValid) {
     packD<D>(s, sec);
     s.recValid = true;
-  } else if (sec == 1) {
+  } else if (sec == 1 && fresh != 1) {
     packMixed<D>(s);
-  } else if (sec == 2) {
+  } else if (sec == 2 && fresh != 2) {
     packFast64<D>(s);
   }
   SG_CHECK_HIP(hipMemsetAsync(dncalc, 0, sizeof(unsigned long long), st));
@@ -1360,10 +1298,10 @@ void packRecords(Sim& s) {
   s.recValid = true;
 }
 
-void sumDensity(Sim& s) {
+void sumDensity(Sim& s, int fresh) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "sum density needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
-  bool secFresh = false;
+  bool secFresh = s.recValid && fresh == 1;
   if (!s.recValid) {
     if (s.isotropic) {
       // the records and the density kernel's own record in one pass
@@ -1384,7 +1322,7 @@ void sumDensity(Sim& s) {
   s.derivsValid = false;
 }
 
-void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly) {
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly, int fresh) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "evaluateDerivatives needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   SG_REQUIRE(p.compatibleEnergyEvolution == 0 || p.precision == SPHGPU_PRECISION_FP64, SPHGPU_ERR_UNSUPPORTED,
@@ -1395,9 +1333,9 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly) {
              "unknown precision");
   SG_REQUIRE(p.nPerh > 0.0, SPHGPU_ERR_ARG, "nPerh must be positive");
   s.derivsValid = false;
-  if (s.dim == 1) evalDerivsD<1>(s, p, updateOnly);
-  else if (s.dim == 2) evalDerivsD<2>(s, p, updateOnly);
-  else evalDerivsD<3>(s, p, updateOnly);
+  if (s.dim == 1) evalDerivsD<1>(s, p, updateOnly, fresh);
+  else if (s.dim == 2) evalDerivsD<2>(s, p, updateOnly, fresh);
+  else evalDerivsD<3>(s, p, updateOnly, fresh);
   s.timingPending |= 4u;
   s.derivsValid = true;
   s.pairsValid = true;

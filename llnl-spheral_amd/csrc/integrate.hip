@@ -25,6 +25,7 @@
 #include "dgeom.hh"
 #include "eigen.hh"
 #include "ell.hh"
+#include "pack.hh"
 #include <cfloat>
 #include <cstring>
 #include <algorithm>
@@ -88,9 +89,12 @@ __device__ __forceinline__ void rowMultEos(const SegArgs& sa, int64_t i, double&
   e.gamma1 = g.gamma - 1.0;
 }
 
-template <int D>
+// REC >= 0: also write the slot's records (pack sec bits REC, isotropic
+// layout) from the values just stored, as the record pack would.
+template <int D, int REC>
 __global__ void k_update(const int32_t* __restrict__ work, const int32_t* __restrict__ perm, int64_t nInt, Fields f,
-                         Base b, DerivIn d, double mult, Bounds bd, Eos e, SegArgs sa) {
+                         Base b, DerivIn d, double mult, Bounds bd, Eos e, SegArgs sa, PackIn pin,
+                         PackOut pout) {
   constexpr int NS = NSym<D>::value;
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (t >= nInt) return;
@@ -113,6 +117,7 @@ __global__ void k_update(const int32_t* __restrict__ work, const int32_t* __rest
     for (int q = a; q < D; ++q, ++c) f.H[i * NS + c] = (a == q) ? dmin(bd.hminInv, dmax(bd.hmaxInv, hid[c])) : hid[c];
   f.P[i] = eosP(e, rho, eps);
   f.cs[i] = eosCs(e, eps);
+  if constexpr (REC >= 0) packSlot<D, true, REC>(k, pin, pout);
 }
 
 // maxViscousPressure, tr DvDx and DvDt of the last evaluation, by caller index.
@@ -312,15 +317,31 @@ template <int D> void compatD(Sim& s, const double* vel0, double mult, const Seg
   SG_CHECK_HIP(hipGetLastError());
 }
 
+// rec >= 0: the update also writes the internal slots' records (pack sec bits;
+// isotropic H only) and the caller packs the ghost slots' records after the
+// ghost rows are refreshed.
 template <int D> void updateD(Sim& s, const Base& b, double mult, const Bounds& bd, const Eos& e,
-                              const double* compDe, const SegArgs& sa) {
+                              const double* compDe, const SegArgs& sa, int rec = -1) {
   Fields f{s.pos.as<double>(), s.vel.as<double>(), s.eps.as<double>(), s.rho.as<double>(), s.H.as<double>(),
            s.P.as<double>(), s.cs.as<double>()};
   DerivIn d{s.out.as<double>(), s.outStride, derivOffsetDoubles(D, SPHGPU_D_DXDT), derivOffsetDoubles(D, SPHGPU_D_DVDT),
             derivOffsetDoubles(D, SPHGPU_D_DEPSDT), derivOffsetDoubles(D, SPHGPU_D_DRHODT),
             derivOffsetDoubles(D, SPHGPU_D_HIDEAL), compDe};
-  hipLaunchKernelGGL(k_update<D>, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
-                     s.perm.as<int32_t>(), s.nInt, f, b, d, mult, bd, e, sa);
+  SG_REQUIRE(rec < 0 || s.isotropic, SPHGPU_ERR_STATE, "records from the update need H = h I");
+  const PackIn pin = rec >= 0 ? packInputs(s) : PackIn{};
+  const PackOut pout = rec >= 0 ? packOutputs(s, rec) : PackOut{};
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.work.as<int32_t>(),
+                       s.perm.as<int32_t>(), s.nInt, f, b, d, mult, bd, e, sa, pin, pout);
+  };
+  switch (rec) {
+    case -1: go(k_update<D, -1>); break;
+    case 0: go(k_update<D, 0>); break;
+    case 1: go(k_update<D, 1>); break;
+    case 2: go(k_update<D, 2>); break;
+    case 4: go(k_update<D, 4>); break;
+    default: SG_REQUIRE(false, SPHGPU_ERR_STATE, "unsupported record kinds for the update");
+  }
   SG_CHECK_HIP(hipGetLastError());
   s.recValid = false;
 }
@@ -449,20 +470,32 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
       if (bytes[q]) SG_CHECK_HIP(hipMemcpyAsync(s.state0[q].p, src[q]->p, bytes[q], hipMemcpyDeviceToDevice, st));
   }
   const Base b0{s.state0[0].as<double>(), s.state0[1].as<double>(), s.state0[2].as<double>(), s.state0[3].as<double>()};
+  // The updates write the records the next pass reads (isotropic H; grad-h
+  // recomputes omega, which the records hold, before the midpoint pass).
+  const bool recs = s.isotropic && !ip.gradhCorrection;
+  const int recMid = recs ? pairRecordSec(s, hp) : -1;
+  const int recEnd = recs ? densityRecordSec(s) : -1;
+  auto ghostsAndRecords = [&](int rec) {
+    applyGhostBoundaries(s);
+    if (rec >= 0) {
+      packGhostRecords(s, rec);
+      s.recValid = true;
+    }
+  };
   // trial advance to the midpoint
   if (compat) compatD<D>(s, b0.vel, hdt, half);
-  updateD<D>(s, b0, hdt, bd, e, compat ? s.compDe.as<double>() : nullptr, half);
-  applyGhostBoundaries(s);
+  updateD<D>(s, b0, hdt, bd, e, compat ? s.compDe.as<double>() : nullptr, half, recMid);
+  ghostsAndRecords(recMid);
   // derivatives at t + dt/2 with the step's neighbour rows
   if (ip.gradhCorrection) omegaGradh(s);
-  evalDerivs(s, hp);
+  evalDerivs(s, hp, false, recMid);
   // advance from state0 with the midpoint derivatives
   if (compat) compatD<D>(s, b0.vel, dt, full);
-  updateD<D>(s, b0, dt, bd, e, compat ? s.compDe.as<double>() : nullptr, full);
-  applyGhostBoundaries(s);
+  updateD<D>(s, b0, dt, bd, e, compat ? s.compDe.as<double>() : nullptr, full, recEnd);
+  ghostsAndRecords(recEnd);
   dtInputsD<D>(s);
   // SPHHydroBase::finalize: sum density
-  sumDensity(s);
+  sumDensity(s, recEnd);
   // Integrator::enforceBoundaries
   enforceBoundaries(s);
 }

@@ -233,10 +233,14 @@ void resolveTimings(Sim& s);
 void buildNeighbors(Sim& s, double kernelExtent);
 // hydro.hip
 void packRecords(Sim& s);
-void sumDensity(Sim& s);
+// fresh: the record kinds (pack sec bits) the integrator update just wrote, or -1
+void sumDensity(Sim& s, int fresh = -1);
 // updateOnly: store only DxDt, DvDt, DepsDt, DrhoDt and Hideal (the fields an
 // integrator update reads); the other fields keep what they held
-void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly = false);
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly = false, int fresh = -1);
+// The pack sec bits of the records the pair kernel and the density sum read.
+int pairRecordSec(const Sim& s, const sphgpu_hydro_params& hp);
+int densityRecordSec(const Sim& s);
 // probe.hip
 double copyProbe(Sim& s, int64_t bytes, int repeats);
 // ghosts.hip
