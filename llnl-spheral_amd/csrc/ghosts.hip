@@ -718,7 +718,7 @@ void cull(Sim& s) {
   const bool dens = recs && s.recSValid;
   hipLaunchKernelGGL(k_perm_remap, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), keep, scan,
                      s.nSlots, recs ? s.rec.as<double>() : nullptr, s.isotropic ? RL<true>::STRIDE : RL<false>::STRIDE,
-                     s.isotropic ? RL<true>::ORIG : RL<false>::ORIG, dens ? s.recS.as<double>() : nullptr, DensRec::STRIDE, RL<true>::ORIG);
+                     s.isotropic ? RL<true>::ORIG : RL<false>::ORIG, dens ? s.recS.as<double>() : nullptr, DensRec::STRIDE, DensRec::ORIG);
   s.ipermSpare.ensure(s.iperm.bytes);
   hipLaunchKernelGGL(k_iperm_remap, dim3(nblocks(n)), dim3(kBlock), 0, st, s.iperm.as<int32_t>(), keep, scan, n,
                      s.ipermSpare.as<int32_t>());

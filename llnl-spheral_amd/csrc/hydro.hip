@@ -199,8 +199,6 @@ template <int D, bool ISO> struct DNbr {
   S<D> H;
   double hdet, m, orig;
 };
-// stride: RL<ISO>::STRIDE for the full records, DensRec::STRIDE for the density
-// sum's (isotropic, same offsets)
 template <int D, bool ISO>
 __device__ __forceinline__ DNbr<D, ISO> loadDNbr(const double* __restrict__ rec, int32_t j,
                                                  int stride = RL<ISO>::STRIDE) {
@@ -218,15 +216,33 @@ __device__ __forceinline__ DNbr<D, ISO> loadDNbr(const double* __restrict__ rec,
   o.hdet = r[R::HDET]; o.m = r[R::M]; o.orig = r[R::ORIG];
   return o;
 }
+// A density record (DensRec).  det H of an exactly isotropic H is (1/h)^D as
+// sdet evaluates it (the off-diagonal products are exact zeros): bitwise the
+// packed value.
+template <int D>
+__device__ __forceinline__ DNbr<D, true> loadDensRec(const double* __restrict__ recS, int32_t j) {
+  const double2* p = reinterpret_cast<const double2*>(recS) + int64_t(j) * 3;
+  const double2 a = p[0], b = p[1], c = p[2];
+  DNbr<D, true> o;
+  o.x.c[0] = a.x;
+  if constexpr (D > 1) o.x.c[1] = a.y;
+  if constexpr (D > 2) o.x.c[2] = b.x;
+  const double hinv = b.y;
+  o.H.c[0] = hinv;
+  for (int c2 = 1; c2 < NSym<D>::value; ++c2) o.H.c[c2] = 0.0;
+  for (int c2 = 1; c2 < D; ++c2) o.H.c[sdiag<D>(c2)] = hinv;
+  o.hdet = D == 1 ? hinv : (D == 2 ? hinv * hinv : (hinv * hinv) * hinv);
+  o.m = c.x; o.orig = c.y;
+  return o;
+}
 
-// Neighbours come from nrec with stride nstride (the density records for
-// isotropic H, else the full records).
+// Neighbours come from nrec: the density records (DensRec) for isotropic H, else
+// the full records.
 template <int D, bool ISO, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec, const int32_t* __restrict__ rowCount,
                                                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
                                                         const int32_t* __restrict__ work, TabArgs t,
-                                                        double* __restrict__ rhoOut, const double* __restrict__ nrec,
-                                                        int nstride) {
+                                                        double* __restrict__ rhoOut, const double* __restrict__ nrec) {
   using R = RL<ISO>;
   extern __shared__ double lds[];
   const double* tab = stageTable<LDS>(lds, t);
@@ -246,10 +262,14 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
   int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
   int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
   DNbr<D, ISO> nx;
-  if (len > 0) nx = loadDNbr<D, ISO>(nrec, cur.x, nstride);
+  auto load = [&](int32_t j) {
+    if constexpr (ISO) return loadDensRec<D>(nrec, j);
+    else return loadDNbr<D, false>(nrec, j);
+  };
+  if (len > 0) nx = load(cur.x);
   for (int w = 0; w < len; ++w) {
     const DNbr<D, ISO> nb = nx;
-    if (w + 1 < len) nx = loadDNbr<D, ISO>(nrec, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nstride);
+    if (w + 1 < len) nx = load(nextIndex(nb4, n, k, len, w + 1, cur, nxt));
     if (!selfDone && nb.orig > oi) { acc += self; selfDone = true; }
     const V<D> rij = vsub(me.x, nb.x);
     const double etaj = vmag(hdot<D, ISO>(nb.H, rij));
@@ -641,9 +661,9 @@ __global__ void k_pack_fast64(const double* __restrict__ rec, int64_t n, int64_t
 }
 
 template <int D>
-__global__ void k_pack_dens(const double* __restrict__ rec, int64_t n, double4* __restrict__ recS) {
+__global__ void k_pack_dens(const double* __restrict__ rec, int64_t n, double2* __restrict__ recS) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k < n) writeDensRec<D>(rec + k * RL<true>::STRIDE, recS + k * 2);
+  if (k < n) writeDensRec<D>(rec + k * RL<true>::STRIDE, recS + k * 3);
 }
 
 // FP32 table in LDS: 8 floats per point.
@@ -1065,16 +1085,15 @@ void sumDensityD(Sim& s, bool secFresh) {
   if (s.isotropic && !s.recSValid) {
     s.recS.ensure(sizeof(double) * DensRec::STRIDE * size_t(std::max<int64_t>(n, 1)));
     hipLaunchKernelGGL(k_pack_dens<D>, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), n,
-                       s.recS.as<double4>());
+                       s.recS.as<double2>());
     SG_CHECK_HIP(hipGetLastError());
     s.recSValid = true;
   }
   const double* nrec = s.isotropic ? s.recS.as<double>() : s.rec.as<double>();
-  const int nstride = s.isotropic ? DensRec::STRIDE : RL<false>::STRIDE;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
-                       s.rho.as<double>(), nrec, nstride);
+                       s.rho.as<double>(), nrec);
   };
   if (s.isotropic) { if (lds) go(k_sum_density<D, true, true>); else go(k_sum_density<D, true, false>); }
   else { if (lds) go(k_sum_density<D, false, true>); else go(k_sum_density<D, false, false>); }

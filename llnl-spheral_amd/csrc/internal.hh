@@ -153,7 +153,7 @@ struct Sim {
   bool recValid = false;    // records match the caller-order state
   DBuf recM;                // mixed-precision neighbour records (64 B per slot)
   DBuf recD;                // FP64 records of the collapsed isotropic pair loop (128 B per slot)
-  DBuf recS;                // FP64 density-sum neighbour records, isotropic H (64 B per slot)
+  DBuf recS;                // FP64 density-sum neighbour records, isotropic H (48 B per slot)
   bool recSValid = false;    // recS matches rec (written by the record pack, renumbered by the cull)
   DBuf pairDv;              // compatible energy: own-frame pair term per row entry (D planes)
   DBuf compDe;              // compatible energy: DepsDt of the policy, per slot

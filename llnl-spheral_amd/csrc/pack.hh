@@ -48,14 +48,13 @@ __device__ __forceinline__ void writeFast64Rec(const double* r, int64_t nInt, do
   b[1] = make_double2(D > 2 ? r[R::V + 2] : 0.0, 0.0);
 }
 
-// The density sum's neighbour record (DensRec: half the FP64 record, so that
-// the gathered array fits the Infinity Cache).
+// The density sum's neighbour record (DensRec).
 template <int D>
-__device__ __forceinline__ void writeDensRec(const double* r, double4* __restrict__ p) {
+__device__ __forceinline__ void writeDensRec(const double* r, double2* __restrict__ p) {
   using R = RL<true>;
-  static_assert(R::X == 0 && R::HINV == 3 && R::M == 4 && R::HDET == 5 && R::ORIG == 6, "DensRec layout");
-  p[0] = make_double4(r[R::X], D > 1 ? r[R::X + 1] : 0.0, D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
-  p[1] = make_double4(r[R::M], r[R::HDET], r[R::ORIG], 0.0);
+  p[0] = make_double2(r[R::X], D > 1 ? r[R::X + 1] : 0.0);
+  p[1] = make_double2(D > 2 ? r[R::X + 2] : 0.0, r[R::HINV]);
+  p[2] = make_double2(r[R::M], r[R::ORIG]);
 }
 
 // The caller-order fields a record is made from.
@@ -93,7 +92,7 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
     if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * 4);
     if constexpr ((SEC & 2) != 0)
       writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
-    if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double4*>(o.sec2) + k * 2);
+    if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
     return;
   }
   for (int c = 0; c < D; ++c) { r[R::X + c] = in.pos[i * D + c]; r[R::V + c] = in.vel[i * D + c]; }
@@ -124,7 +123,7 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
   if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * 4);
   if constexpr ((SEC & 2) != 0)
     writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
-  if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double4*>(o.sec2) + k * 2);
+  if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
 }
 
 struct Sim;

@@ -27,11 +27,12 @@ template <> struct RL<false> {
                        P = 19, SOM = 20, FCL = 21, FCQ = 22, FSH = 23;
 };
 
-// The FP64 density sum's neighbour record (isotropic H): the first eight
-// doubles of RL<true> at the same offsets (x, 1/h, m, det H, caller index,
-// then 0), 64 B, so the gathered array is half the size.
+// The FP64 density sum's neighbour record (isotropic H), 48 B: x, 1/h, m and the
+// caller index; three 16-B loads per neighbour, and the gathered array (115 MB
+// at 2.4 M slots) fits the Infinity Cache.  det H = (1/h)^D is recomputed.
 struct DensRec {
-  static constexpr int STRIDE = 8;
+  static constexpr int STRIDE = 6;
+  static constexpr int X = 0, HINV = 3, M = 4, ORIG = 5;
 };
 
 }  // namespace sphgpu
