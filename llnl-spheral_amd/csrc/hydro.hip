@@ -661,6 +661,12 @@ __global__ void k_pack_fast64(const double* __restrict__ rec, int64_t n, int64_t
 }
 
 template <int D>
+__global__ void k_pack_dens_mixed(const double* __restrict__ rec, int64_t n, float4* __restrict__ recSM) {
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k < n) writeDensRecMixed<D>(rec + k * RL<true>::STRIDE, recSM + k * 2);
+}
+
+template <int D>
 __global__ void k_pack_dens(const double* __restrict__ rec, int64_t n, double2* __restrict__ recS) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (k < n) writeDensRec<D>(rec + k * RL<true>::STRIDE, recS + k * 3);
@@ -756,7 +762,7 @@ __device__ __forceinline__ Real muQR(Real mu, Real Cl, Real Cq, Real cs, int lin
 }
 
 template <int D, bool LDS>
-__global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict__ rec, const float4* __restrict__ recM,
+__global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict__ rec, const float4* __restrict__ recSM,
                                                               const int32_t* __restrict__ rowCount,
                                                               const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
                                                               const int32_t* __restrict__ work, TabArgs t, Tab32 t32,
@@ -775,18 +781,29 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
   const int4* nb4 = reinterpret_cast<const int4*>(nbr);
   int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
   int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
-  INbr<float> nx;
-  if (len > 0) loadIso<D>(recM, n, cur.x, nx);
+  // the mixed density record: x (FP64 bits), 1/h, m
+  auto load = [&](int32_t j, float4& a, float4& b) {
+    const float4* p = recSM + int64_t(j) * 2;
+    a = p[0];
+    b = p[1];
+  };
+  float4 na, nb2;
+  if (len > 0) load(cur.x, na, nb2);
   for (int w = 0; w < len; ++w) {
-    const INbr<float> nb = nx;
-    if (w + 1 < len) loadIso<D>(recM, n, nextIndex(nb4, n, k, len, w + 1, cur, nxt), nx);
+    const float4 a = na, b = nb2;
+    if (w + 1 < len) load(nextIndex(nb4, n, k, len, w + 1, cur, nxt), na, nb2);
+    const double xj[3] = {__hiloint2double(__float_as_int(a.y), __float_as_int(a.x)),
+                          __hiloint2double(__float_as_int(a.w), __float_as_int(a.z)),
+                          __hiloint2double(__float_as_int(b.y), __float_as_int(b.x))};
     float r2 = 0.0f;
     for (int c = 0; c < D; ++c) {
-      const float rc = float(xi[c] - nb.x[c]);
+      const float rc = float(xi[c] - xj[c]);
       r2 = __builtin_fmaf(rc, rc, r2);
     }
-    const float eta = nb.hinv * __builtin_sqrtf(r2);
-    acc += double(nb.m * lookupW32(tab, t32, eta, nb.hdet));
+    const float hj = b.z;
+    const float hdetj = D == 1 ? hj : (D == 2 ? hj * hj : hj * hj * hj);
+    const float eta = hj * __builtin_sqrtf(r2);
+    acc += double(b.w * lookupW32(tab, t32, eta, hdetj));
   }
   rhoOut[int64_t(ri_[R::ORIG])] = acc;
   const double som = ri_[R::SOM], p = ri_[R::P];
@@ -966,7 +983,7 @@ PackOut packOutputs(Sim& s, int sec) {
   SG_REQUIRE((sec & 3) != 3, SPHGPU_ERR_STATE, "one pair-kernel record per pack");
   const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
   s.rec.ensure(sizeof(double) * size_t(n) * stride);
-  PackOut o{s.rec.as<double>(), nullptr, nullptr, n, s.nInt};
+  PackOut o{s.rec.as<double>(), nullptr, nullptr, n, s.nInt, nullptr};
   if (sec & 1) {
     s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(n, 1)));
     o.sec = s.recM.p;
@@ -978,7 +995,12 @@ PackOut packOutputs(Sim& s, int sec) {
     s.recS.ensure(sizeof(double) * DensRec::STRIDE * size_t(std::max<int64_t>(n, 1)));
     o.sec2 = s.recS.p;
   }
+  if (sec & 8) {
+    s.recSM.ensure(sizeof(float) * 8 * size_t(std::max<int64_t>(n, 1)));
+    o.sec3 = s.recSM.p;
+  }
   s.recSValid = (sec & 4) != 0;
+  s.recSMValid = (sec & 8) != 0;
   return o;
 }
 
@@ -999,6 +1021,8 @@ void launchPack(Sim& s, int sec, const int32_t* slots, int64_t count) {
       case 4: go(k_pack<D, true, 4>); break;
       case 5: go(k_pack<D, true, 5>); break;
       case 6: go(k_pack<D, true, 6>); break;
+      case 8: go(k_pack<D, true, 8>); break;
+      case 9: go(k_pack<D, true, 9>); break;
       default: go(k_pack<D, true, 0>); break;
     }
   }
@@ -1032,7 +1056,7 @@ int pairRecordSec(const Sim& s, const sphgpu_hydro_params& hp) {
 
 int densityRecordSec(const Sim& s) {
   if (!s.isotropic) return 0;
-  return s.precision == SPHGPU_PRECISION_MIXED ? 1 : 4;
+  return s.precision == SPHGPU_PRECISION_MIXED ? 8 : 4;
 }
 
 namespace {
@@ -1057,9 +1081,8 @@ void packFast64(Sim& s) {
 
 Tab64 tab64(const Sim& s) { return Tab64{1.0 / s.table.step, s.table.extent, s.table.n}; }
 
-// secFresh: the mixed record was written with the records just now.
 template <int D>
-void sumDensityD(Sim& s, bool secFresh) {
+void sumDensityD(Sim& s) {
   const int64_t n = s.nSlots;
   TabArgs t = tabArgs(s);
   const bool lds = ldsTable(s);
@@ -1067,12 +1090,18 @@ void sumDensityD(Sim& s, bool secFresh) {
   SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
   if (s.precision == SPHGPU_PRECISION_MIXED) {
     SG_REQUIRE(s.isotropic, SPHGPU_ERR_UNSUPPORTED, "mixed precision needs H = h I and no Q multipliers");
-    if (!secFresh) packMixed<D>(s);
+    if (!s.recSMValid) {
+      s.recSM.ensure(sizeof(float) * 8 * size_t(std::max<int64_t>(n, 1)));
+      hipLaunchKernelGGL(k_pack_dens_mixed<D>, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), n,
+                         s.recSM.as<float4>());
+      SG_CHECK_HIP(hipGetLastError());
+      s.recSMValid = true;
+    }
     const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
     const double W0 = s.table.kernelValue(0.0, 1.0);
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), lds32, s.stream, s.rec.as<double>(),
-                         s.recM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
+                         s.recSM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
                          s.work.as<int32_t>(), t, tab32(s), W0, s.rho.as<double>());
     };
     if (lds) go(k_sum_density_mixed<D, true>); else go(k_sum_density_mixed<D, false>);
@@ -1317,26 +1346,24 @@ void packRecords(Sim& s) {
   s.recValid = true;
 }
 
-void sumDensity(Sim& s, int fresh) {
+void sumDensity(Sim& s) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "sum density needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
-  bool secFresh = s.recValid && fresh == 1;
   if (!s.recValid) {
     if (s.isotropic) {
       // the records and the density kernel's own record in one pass
-      const int sec = s.precision == SPHGPU_PRECISION_MIXED ? 1 : 4;
+      const int sec = densityRecordSec(s);
       if (s.dim == 1) packD<1>(s, sec);
       else if (s.dim == 2) packD<2>(s, sec);
       else packD<3>(s, sec);
       s.recValid = true;
-      secFresh = sec == 1;
     } else {
       packRecords(s);
     }
   }
-  if (s.dim == 1) sumDensityD<1>(s, secFresh);
-  else if (s.dim == 2) sumDensityD<2>(s, secFresh);
-  else sumDensityD<3>(s, secFresh);
+  if (s.dim == 1) sumDensityD<1>(s);
+  else if (s.dim == 2) sumDensityD<2>(s);
+  else sumDensityD<3>(s);
   s.timingPending |= 2u;
   s.derivsValid = false;
 }

@@ -340,6 +340,7 @@ template <int D> void updateD(Sim& s, const Base& b, double mult, const Bounds& 
     case 1: go(k_update<D, 1>); break;
     case 2: go(k_update<D, 2>); break;
     case 4: go(k_update<D, 4>); break;
+    case 8: go(k_update<D, 8>); break;
     default: SG_REQUIRE(false, SPHGPU_ERR_STATE, "unsupported record kinds for the update");
   }
   SG_CHECK_HIP(hipGetLastError());
@@ -495,7 +496,7 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
   ghostsAndRecords(recEnd);
   dtInputsD<D>(s);
   // SPHHydroBase::finalize: sum density
-  sumDensity(s, recEnd);
+  sumDensity(s);
   // Integrator::enforceBoundaries
   enforceBoundaries(s);
 }

@@ -155,6 +155,8 @@ struct Sim {
   DBuf recD;                // FP64 records of the collapsed isotropic pair loop (128 B per slot)
   DBuf recS;                // FP64 density-sum neighbour records, isotropic H (48 B per slot)
   bool recSValid = false;    // recS matches rec (written by the record pack, renumbered by the cull)
+  DBuf recSM;               // mixed density-sum neighbour records (32 B per slot)
+  bool recSMValid = false;   // recSM matches rec (written by the record pack; the cull keeps it)
   DBuf pairDv;              // compatible energy: own-frame pair term per row entry (D planes)
   DBuf compDe;              // compatible energy: DepsDt of the policy, per slot
   DBuf compAcc, compDe0;    // compatible energy: DvDt and DepsDt by caller index, ghost rows from the boundaries
@@ -233,8 +235,7 @@ void resolveTimings(Sim& s);
 void buildNeighbors(Sim& s, double kernelExtent);
 // hydro.hip
 void packRecords(Sim& s);
-// fresh: the record kinds (pack sec bits) the integrator update just wrote, or -1
-void sumDensity(Sim& s, int fresh = -1);
+void sumDensity(Sim& s);
 // updateOnly: store only DxDt, DvDt, DepsDt, DrhoDt and Hideal (the fields an
 // integrator update reads); the other fields keep what they held
 void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly = false, int fresh = -1);

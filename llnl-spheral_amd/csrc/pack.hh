@@ -57,6 +57,18 @@ __device__ __forceinline__ void writeDensRec(const double* r, double2* __restric
   p[2] = make_double2(r[R::M], r[R::ORIG]);
 }
 
+// The mixed density sum's neighbour record, 32 B: x (FP64 bits), 1/h and m
+// (FP32); det H = (1/h)^D is formed in FP32 by the reader.
+template <int D>
+__device__ __forceinline__ void writeDensRecMixed(const double* r, float4* __restrict__ p) {
+  using R = RL<true>;
+  const double x0 = r[R::X], x1 = D > 1 ? r[R::X + 1] : 0.0, x2 = D > 2 ? r[R::X + 2] : 0.0;
+  p[0] = make_float4(__int_as_float(__double2loint(x0)), __int_as_float(__double2hiint(x0)),
+                     __int_as_float(__double2loint(x1)), __int_as_float(__double2hiint(x1)));
+  p[1] = make_float4(__int_as_float(__double2loint(x2)), __int_as_float(__double2hiint(x2)), float(r[R::HINV]),
+                     float(r[R::M]));
+}
+
 // The caller-order fields a record is made from.
 struct PackIn {
   const int32_t* perm;     // slot -> caller row (-1: dead slot)
@@ -64,17 +76,20 @@ struct PackIn {
   const double *fCl, *fCq, *fsh;   // Q multipliers (anisotropic records), or null
 };
 // Where the records go: rec (FP64 record), sec (SEC bit 1: mixed, bit 2: FP64
-// isotropic pair record), sec2 (SEC bit 4: density record); n slots.
+// isotropic pair record), sec2 (SEC bit 4: FP64 density record), sec3 (SEC
+// bit 8: mixed density record); n slots.
 struct PackOut {
   double* rec;
   void* sec;
   void* sec2;
   int64_t n, nInt;
+  void* sec3;
 };
 
 // The records of slot k.  SEC (bits): also write, from the values in
 // registers, the mixed pair kernel's record (1), the FP64 isotropic pair
-// kernel's record (2) and the density sum's record (4).
+// kernel's record (2), the FP64 density sum's record (4) and the mixed density
+// sum's record (8).
 template <int D, bool ISO, int SEC>
 __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const PackOut& o) {
   static_assert(SEC == 0 || ISO, "the pair kernels' records are isotropic");
@@ -93,6 +108,7 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
     if constexpr ((SEC & 2) != 0)
       writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
     if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
+    if constexpr ((SEC & 8) != 0) writeDensRecMixed<D>(r, static_cast<float4*>(o.sec3) + k * 2);
     return;
   }
   for (int c = 0; c < D; ++c) { r[R::X + c] = in.pos[i * D + c]; r[R::V + c] = in.vel[i * D + c]; }
@@ -124,11 +140,12 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
   if constexpr ((SEC & 2) != 0)
     writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
   if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
+  if constexpr ((SEC & 8) != 0) writeDensRecMixed<D>(r, static_cast<float4*>(o.sec3) + k * 2);
 }
 
 struct Sim;
 PackIn packInputs(Sim& s);
-// allocates the records that sec asks for; sets recSValid from bit 4
+// allocates the records that sec asks for; sets recSValid and recSMValid from bits 4 and 8
 PackOut packOutputs(Sim& s, int sec);
 // The records of the ghost slots only (after the ghost rows' fields changed).
 void packGhostRecords(Sim& s, int sec);
