@@ -619,16 +619,30 @@ template <typename Real> struct INbr {
 };
 template <int D>
 __device__ __forceinline__ void loadIso(const float4* __restrict__ recM, int64_t, int32_t j, INbr<float>& o) {
-  const float4* p = recM + int64_t(j) * 4;
-  const float4 f0 = p[0], f1 = p[1], f2 = p[2], f3 = p[3];
-  o.x[0] = __hiloint2double(__float_as_int(f0.y), __float_as_int(f0.x));
-  o.x[1] = __hiloint2double(__float_as_int(f0.w), __float_as_int(f0.z));
-  o.x[2] = __hiloint2double(__float_as_int(f1.y), __float_as_int(f1.x));
-  o.hinv = f1.z;
-  o.ghost = f1.w < 0.0f;
-  o.hdet = fabsf(f1.w);
-  o.m = f2.x; o.invRho = f2.y; o.v[0] = f2.z; o.v[1] = f2.w;
-  o.v[2] = f3.x; o.cs = f3.y; o.prho = f3.z; o.mor = f3.w;
+  const float4* p = recM + int64_t(j) * MixedRec<D>::F4;
+  auto dbl = [](float lo, float hi) { return __hiloint2double(__float_as_int(hi), __float_as_int(lo)); };
+  float h;
+  if constexpr (D == 1) {
+    const float4 f0 = p[0], f1 = p[1];
+    o.x[0] = dbl(f0.x, f0.y); o.x[1] = 0.0; o.x[2] = 0.0;
+    h = f0.z; o.m = f0.w;
+    o.invRho = f1.x; o.v[0] = f1.y; o.v[1] = 0.0f; o.v[2] = 0.0f; o.cs = f1.z; o.prho = f1.w;
+    o.mor = o.m * o.invRho;
+  } else if constexpr (D == 2) {
+    const float4 f0 = p[0], f1 = p[1], f2 = p[2];
+    o.x[0] = dbl(f0.x, f0.y); o.x[1] = dbl(f0.z, f0.w); o.x[2] = 0.0;
+    h = f1.x; o.m = f1.y; o.invRho = f1.z; o.v[0] = f1.w;
+    o.v[1] = f2.x; o.v[2] = 0.0f; o.cs = f2.y; o.prho = f2.z; o.mor = f2.w;
+  } else {
+    const float4 f0 = p[0], f1 = p[1], f2 = p[2], f3 = p[3];
+    o.x[0] = dbl(f0.x, f0.y); o.x[1] = dbl(f0.z, f0.w); o.x[2] = dbl(f1.x, f1.y);
+    h = f1.z; o.m = f1.w;
+    o.invRho = f2.x; o.v[0] = f2.y; o.v[1] = f2.z; o.v[2] = f2.w;
+    o.cs = f3.x; o.prho = f3.y; o.mor = f3.z;
+  }
+  o.ghost = h < 0.0f;
+  o.hinv = fabsf(h);
+  o.hdet = D == 1 ? o.hinv : (D == 2 ? o.hinv * o.hinv : o.hinv * o.hinv * o.hinv);
 }
 // det H of an exactly isotropic H is (1/h)^D evaluated as sdet evaluates it
 // (the off-diagonal products are exact zeros), so the value is bitwise the
@@ -651,7 +665,7 @@ __device__ __forceinline__ void loadIso(const double2* __restrict__ recD, int64_
 template <int D>
 __global__ void k_pack_mixed(const double* __restrict__ rec, int64_t n, int64_t nInt, float4* __restrict__ recM) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k < n) writeMixedRec<D>(rec + k * RL<true>::STRIDE, nInt, recM + k * 4);
+  if (k < n) writeMixedRec<D>(rec + k * RL<true>::STRIDE, nInt, recM + k * MixedRec<D>::F4);
 }
 
 template <int D>
@@ -663,7 +677,7 @@ __global__ void k_pack_fast64(const double* __restrict__ rec, int64_t n, int64_t
 template <int D>
 __global__ void k_pack_dens_mixed(const double* __restrict__ rec, int64_t n, float4* __restrict__ recSM) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k < n) writeDensRecMixed<D>(rec + k * RL<true>::STRIDE, recSM + k * 2);
+  if (k < n) writeDensRecMixed<D>(rec + k * RL<true>::STRIDE, recSM + k * DensMixedRec<D>::F4);
 }
 
 template <int D>
@@ -783,9 +797,10 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
   int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
   // the mixed density record: x (FP64 bits), 1/h, m
   auto load = [&](int32_t j, float4& a, float4& b) {
-    const float4* p = recSM + int64_t(j) * 2;
+    const float4* p = recSM + int64_t(j) * DensMixedRec<D>::F4;
     a = p[0];
-    b = p[1];
+    if constexpr (D > 1) b = p[1];
+    else b = make_float4(0.0f, 0.0f, a.z, a.w);
   };
   float4 na, nb2;
   if (len > 0) load(cur.x, na, nb2);

@@ -15,20 +15,38 @@ namespace sphgpu {
 // layout): FP32, 64 B, for the mixed kernels (positions keep their FP64 bits);
 // FP64, 96 B in two planes, for the FP64 kernel.  Ghost rows are marked by a
 // negative det H (mixed) or 1/h (FP64).
+// The mixed pair kernel's record (FP32 but for the FP64 bits of x), float4s per
+// slot by dimension: 1-D (x, +-1/h, m | 1/rho, v, c, P/rho^2/omega), 2-D
+// (x | +-1/h, m, 1/rho, vx | vy, c, P/rho^2/omega, m/rho), 3-D (x, y |
+// z, +-1/h, m | 1/rho, v | c, P/rho^2/omega, m/rho, 0); 1/h is negative for
+// ghost rows, det H = (1/h)^D and in 1-D m/rho = m * (1/rho) are formed by the
+// reader.
+template <int D> struct MixedRec {
+  static constexpr int F4 = D == 1 ? 2 : (D == 2 ? 3 : 4);
+};
+__device__ __forceinline__ float dlo(double x) { return __int_as_float(__double2loint(x)); }
+__device__ __forceinline__ float dhi(double x) { return __int_as_float(__double2hiint(x)); }
 template <int D>
 __device__ __forceinline__ void writeMixedRec(const double* r, int64_t nInt, float4* __restrict__ p) {
   using R = RL<true>;
-  const double x0 = r[R::X], x1 = D > 1 ? r[R::X + 1] : 0.0, x2 = D > 2 ? r[R::X + 2] : 0.0;
   const bool ghost = r[R::ORIG] >= double(nInt);
-  const float hdet = float(r[R::HDET]);
-  p[0] = make_float4(__int_as_float(__double2loint(x0)), __int_as_float(__double2hiint(x0)),
-                     __int_as_float(__double2loint(x1)), __int_as_float(__double2hiint(x1)));
-  p[1] = make_float4(__int_as_float(__double2loint(x2)), __int_as_float(__double2hiint(x2)), float(r[R::HINV]),
-                     ghost ? -hdet : hdet);
+  const float hinv = ghost ? -float(r[R::HINV]) : float(r[R::HINV]);
   const double rho = r[R::RHO];
-  p[2] = make_float4(float(r[R::M]), rho != 0.0 ? float(1.0 / rho) : 0.0f, float(r[R::V]),
-                     D > 1 ? float(r[R::V + 1]) : 0.0f);
-  p[3] = make_float4(D > 2 ? float(r[R::V + 2]) : 0.0f, float(r[R::CS]), float(r[R::PRHO]), float(r[R::MOR]));
+  const float invRho = rho != 0.0 ? float(1.0 / rho) : 0.0f;
+  const float m = float(r[R::M]), cs = float(r[R::CS]), prho = float(r[R::PRHO]), mor = float(r[R::MOR]);
+  if constexpr (D == 1) {
+    p[0] = make_float4(dlo(r[R::X]), dhi(r[R::X]), hinv, m);
+    p[1] = make_float4(invRho, float(r[R::V]), cs, prho);
+  } else if constexpr (D == 2) {
+    p[0] = make_float4(dlo(r[R::X]), dhi(r[R::X]), dlo(r[R::X + 1]), dhi(r[R::X + 1]));
+    p[1] = make_float4(hinv, m, invRho, float(r[R::V]));
+    p[2] = make_float4(float(r[R::V + 1]), cs, prho, mor);
+  } else {
+    p[0] = make_float4(dlo(r[R::X]), dhi(r[R::X]), dlo(r[R::X + 1]), dhi(r[R::X + 1]));
+    p[1] = make_float4(dlo(r[R::X + 2]), dhi(r[R::X + 2]), hinv, m);
+    p[2] = make_float4(invRho, float(r[R::V]), float(r[R::V + 1]), float(r[R::V + 2]));
+    p[3] = make_float4(cs, prho, mor, 0.0f);
+  }
 }
 // The FP64 isotropic pair kernel's record, 96 B per slot in two planes so that
 // the gathered arrays (230 MB at 2.4 M slots) fit the 256 MB Infinity Cache:
@@ -57,16 +75,21 @@ __device__ __forceinline__ void writeDensRec(const double* r, double2* __restric
   p[2] = make_double2(r[R::M], r[R::ORIG]);
 }
 
-// The mixed density sum's neighbour record, 32 B: x (FP64 bits), 1/h and m
-// (FP32); det H = (1/h)^D is formed in FP32 by the reader.
+// The mixed density sum's neighbour record: x (FP64 bits), 1/h and m (FP32),
+// one float4 in 1-D, two otherwise; det H = (1/h)^D is formed by the reader.
+template <int D> struct DensMixedRec {
+  static constexpr int F4 = D == 1 ? 1 : 2;
+};
 template <int D>
 __device__ __forceinline__ void writeDensRecMixed(const double* r, float4* __restrict__ p) {
   using R = RL<true>;
-  const double x0 = r[R::X], x1 = D > 1 ? r[R::X + 1] : 0.0, x2 = D > 2 ? r[R::X + 2] : 0.0;
-  p[0] = make_float4(__int_as_float(__double2loint(x0)), __int_as_float(__double2hiint(x0)),
-                     __int_as_float(__double2loint(x1)), __int_as_float(__double2hiint(x1)));
-  p[1] = make_float4(__int_as_float(__double2loint(x2)), __int_as_float(__double2hiint(x2)), float(r[R::HINV]),
-                     float(r[R::M]));
+  if constexpr (D == 1) {
+    p[0] = make_float4(dlo(r[R::X]), dhi(r[R::X]), float(r[R::HINV]), float(r[R::M]));
+  } else {
+    const double x1 = r[R::X + 1], x2 = D > 2 ? r[R::X + 2] : 0.0;
+    p[0] = make_float4(dlo(r[R::X]), dhi(r[R::X]), dlo(x1), dhi(x1));
+    p[1] = make_float4(dlo(x2), dhi(x2), float(r[R::HINV]), float(r[R::M]));
+  }
 }
 
 // The caller-order fields a record is made from.
@@ -104,11 +127,11 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
     double2* dst = reinterpret_cast<double2*>(o.rec + k * R::STRIDE);
 #pragma unroll
     for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-    if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * 4);
+    if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * MixedRec<D>::F4);
     if constexpr ((SEC & 2) != 0)
       writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
     if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
-    if constexpr ((SEC & 8) != 0) writeDensRecMixed<D>(r, static_cast<float4*>(o.sec3) + k * 2);
+    if constexpr ((SEC & 8) != 0) writeDensRecMixed<D>(r, static_cast<float4*>(o.sec3) + k * DensMixedRec<D>::F4);
     return;
   }
   for (int c = 0; c < D; ++c) { r[R::X + c] = in.pos[i * D + c]; r[R::V + c] = in.vel[i * D + c]; }
@@ -136,11 +159,11 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
   double2* dst = reinterpret_cast<double2*>(o.rec + k * R::STRIDE);
 #pragma unroll
   for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-  if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * 4);
+  if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * MixedRec<D>::F4);
   if constexpr ((SEC & 2) != 0)
     writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
   if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
-  if constexpr ((SEC & 8) != 0) writeDensRecMixed<D>(r, static_cast<float4*>(o.sec3) + k * 2);
+  if constexpr ((SEC & 8) != 0) writeDensRecMixed<D>(r, static_cast<float4*>(o.sec3) + k * DensMixedRec<D>::F4);
 }
 
 struct Sim;
