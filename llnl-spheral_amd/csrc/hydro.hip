@@ -338,6 +338,8 @@ struct PassArgs {
   double Cl, Cq, eps2, epsTensile, nPerh, hmin, hmax;
   double W0, WnPerh;
   int XSPH, correctVelocityGradient, evolveTotalEnergy, linearInExpansion, quadraticInExpansion;
+  double qInv;   // 1/u and u of the 3-D mixed record's positions
+  float qUnit;
 };
 
 __device__ __forceinline__ double muQ(double mu, double Cl, double Cq, double cs, const PassArgs& p) {
@@ -614,6 +616,7 @@ __global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict
 // ---------------------------------------------------------------------------
 template <typename Real> struct INbr {
   double x[3];
+  uint32_t q[3];    // 3-D mixed: quantised position words
   Real hinv, hdet, m, invRho, v[3], cs, prho, mor;
   bool ghost;
 };
@@ -634,11 +637,11 @@ __device__ __forceinline__ void loadIso(const float4* __restrict__ recM, int64_t
     h = f1.x; o.m = f1.y; o.invRho = f1.z; o.v[0] = f1.w;
     o.v[1] = f2.x; o.v[2] = 0.0f; o.cs = f2.y; o.prho = f2.z; o.mor = f2.w;
   } else {
-    const float4 f0 = p[0], f1 = p[1], f2 = p[2], f3 = p[3];
-    o.x[0] = dbl(f0.x, f0.y); o.x[1] = dbl(f0.z, f0.w); o.x[2] = dbl(f1.x, f1.y);
-    h = f1.z; o.m = f1.w;
-    o.invRho = f2.x; o.v[0] = f2.y; o.v[1] = f2.z; o.v[2] = f2.w;
-    o.cs = f3.x; o.prho = f3.y; o.mor = f3.z;
+    const float4 f0 = p[0], f1 = p[1], f2 = p[2];
+    o.q[0] = __float_as_uint(f0.x); o.q[1] = __float_as_uint(f0.y); o.q[2] = __float_as_uint(f0.z);
+    h = f0.w;
+    o.m = f1.x; o.invRho = f1.y; o.v[0] = f1.z; o.v[1] = f1.w;
+    o.v[2] = f2.x; o.cs = f2.y; o.prho = f2.z; o.mor = f2.w;
   }
   o.ghost = h < 0.0f;
   o.hinv = fabsf(h);
@@ -663,9 +666,10 @@ __device__ __forceinline__ void loadIso(const double2* __restrict__ recD, int64_
 }
 
 template <int D>
-__global__ void k_pack_mixed(const double* __restrict__ rec, int64_t n, int64_t nInt, float4* __restrict__ recM) {
+__global__ void k_pack_mixed(const double* __restrict__ rec, int64_t n, int64_t nInt, double qInv,
+                             float4* __restrict__ recM) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (k < n) writeMixedRec<D>(rec + k * RL<true>::STRIDE, nInt, recM + k * MixedRec<D>::F4);
+  if (k < n) writeMixedRec<D>(rec + k * RL<true>::STRIDE, nInt, qInv, recM + k * MixedRec<D>::F4);
 }
 
 template <int D>
@@ -866,13 +870,17 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
   }
   const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
   unsigned pairs = 0;
+  // the 3-D mixed record holds quantised positions
+  constexpr bool QPOS = std::is_same<Real, float>::value && D == 3;
   if (tw < nInt) {
     const int64_t k = work[tw];
     const double* ri_ = rec + k * R::STRIDE;
     double xi[3];
+    uint32_t qi[3];
     Real vi[3];
     for (int c = 0; c < 3; ++c) {
       xi[c] = c < D ? ri_[R::X + c] : 0.0;
+      if constexpr (QPOS) qi[c] = quantPos(xi[c], p.qInv);
       vi[c] = c < D ? Real(ri_[R::V + c]) : Real(0);
     }
     const double rho0 = ri_[R::RHO];
@@ -894,7 +902,8 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
       Real rij[3], vij[3];
       Real r2 = Real(0), vdotr = Real(0);
       for (int c = 0; c < D; ++c) {
-        rij[c] = Real(xi[c] - nb.x[c]);
+        if constexpr (QPOS) rij[c] = Real(int32_t(qi[c] - nb.q[c])) * p.qUnit;
+        else rij[c] = Real(xi[c] - nb.x[c]);
         vij[c] = vi[c] - nb.v[c];
         r2 = fmaR(rij[c], rij[c], r2);
         vdotr = fmaR(vij[c], rij[c], vdotr);
@@ -984,6 +993,12 @@ TabArgs tabArgs(const Sim& s) {
 
 }  // namespace
 
+double quantInv(const Sim& s) {
+  const int e = std::ilogb(s.rMax) + 1 - 30;     // rMax < 2^30 u
+  SG_REQUIRE(s.rMax > 0.0 && e > -120 && e < 120, SPHGPU_ERR_STATE, "search radius out of range");
+  return std::ldexp(1.0, -e);
+}
+
 PackIn packInputs(Sim& s) {
   return PackIn{s.perm.as<int32_t>(), s.pos.as<double>(), s.vel.as<double>(), s.mass.as<double>(),
                 s.rho.as<double>(), s.P.as<double>(), s.cs.as<double>(), s.H.as<double>(), s.omega.as<double>(),
@@ -998,7 +1013,7 @@ PackOut packOutputs(Sim& s, int sec) {
   SG_REQUIRE((sec & 3) != 3, SPHGPU_ERR_STATE, "one pair-kernel record per pack");
   const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
   s.rec.ensure(sizeof(double) * size_t(n) * stride);
-  PackOut o{s.rec.as<double>(), nullptr, nullptr, n, s.nInt, nullptr};
+  PackOut o{s.rec.as<double>(), nullptr, nullptr, n, s.nInt, nullptr, quantInv(s)};
   if (sec & 1) {
     s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(n, 1)));
     o.sec = s.recM.p;
@@ -1082,7 +1097,7 @@ template <int D>
 void packMixed(Sim& s) {
   s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(s.nSlots, 1)));
   hipLaunchKernelGGL(k_pack_mixed<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), s.nSlots,
-                     s.nInt, s.recM.as<float4>());
+                     s.nInt, quantInv(s), s.recM.as<float4>());
   SG_CHECK_HIP(hipGetLastError());
 }
 
@@ -1189,6 +1204,8 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly, int fre
   p.WnPerh = s.table.kernelValue(1.0 / hp.nPerh, 1.0);
   p.XSPH = hp.XSPH; p.correctVelocityGradient = hp.correctVelocityGradient; p.evolveTotalEnergy = hp.evolveTotalEnergy;
   p.linearInExpansion = hp.linearInExpansion; p.quadraticInExpansion = hp.quadraticInExpansion;
+  p.qInv = quantInv(s);
+  p.qUnit = float(1.0 / p.qInv);
   OutArgs o{};
   o.base = s.out.as<double>();
   o.n = n;

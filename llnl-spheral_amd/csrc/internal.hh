@@ -134,6 +134,7 @@ struct Sim {
   // entry w of slot k at nbr[w * nTot + k], rowCount[k] entries)
   Grid grid;
   double kernelExtent = 2.0;
+  double rMax = 1.0;         // largest search radius of the last neighbour build
   int rowOrder = SPHGPU_ROW_ORDER_REFERENCE;
   bool neighborsValid = false;
   bool isotropic = false;   // every H is h*I (off-diagonals exactly zero)

@@ -970,6 +970,7 @@ void buildNeighborsD(Sim& s, double ext) {
       hb[8] = std::max(hb[8], hm[8]);
     }
     s.isotropic = iso && !s.hasMultipliers;
+    s.rMax = ext / hb[6];
     G = Grid{};
     G.ncells = uploadGrids(s, grids, cells, first);
     sgr = SegGrids{s.seg.as<double>(), s.segGrid.as<DGrid>(), s.segCellBase.as<int64_t>(),
@@ -987,6 +988,7 @@ void buildNeighborsD(Sim& s, double ext) {
     SG_CHECK_HIP(hipStreamSynchronize(st));
     std::memcpy(hb, hbp, sizeof(hb));
     s.isotropic = hb[7] > 0.5 && !s.hasMultipliers;
+    s.rMax = ext / hb[6];
     std::vector<DGrid> grids;
     std::vector<int64_t> cells;
     levelGrids<D>(hb, ext, grids, cells);

@@ -15,19 +15,26 @@ namespace sphgpu {
 // layout): FP32, 64 B, for the mixed kernels (positions keep their FP64 bits);
 // FP64, 96 B in two planes, for the FP64 kernel.  Ghost rows are marked by a
 // negative det H (mixed) or 1/h (FP64).
-// The mixed pair kernel's record (FP32 but for the FP64 bits of x), float4s per
-// slot by dimension: 1-D (x, +-1/h, m | 1/rho, v, c, P/rho^2/omega), 2-D
-// (x | +-1/h, m, 1/rho, vx | vy, c, P/rho^2/omega, m/rho), 3-D (x, y |
-// z, +-1/h, m | 1/rho, v | c, P/rho^2/omega, m/rho, 0); 1/h is negative for
-// ghost rows, det H = (1/h)^D and in 1-D m/rho = m * (1/rho) are formed by the
-// reader.
+// The mixed pair kernel's record (FP32), float4s per slot by dimension: 1-D
+// (x, +-1/h, m | 1/rho, v, c, P/rho^2/omega), 2-D (x, y | +-1/h, m, 1/rho, vx |
+// vy, c, P/rho^2/omega, m/rho) with x, y as FP64 bits, 3-D (qx, qy, qz, +-1/h |
+// m, 1/rho, vx, vy | vz, c, P/rho^2/omega, m/rho).  1/h is negative for ghost
+// rows; det H = (1/h)^D and in 1-D m/rho = m * (1/rho) are formed by the reader.
+// In 3-D the position is the low 32 bits of round(x / u) for a power of two u
+// with the largest search radius below 2^30 u: the 32-bit difference of two
+// neighbours' words is the exact difference of their rounded positions, so
+// r_ij carries an error of at most u (1e-9 of the search radius), far below
+// the FP32 rounding of r_ij itself.
+__device__ __forceinline__ uint32_t quantPos(double x, double qInv) {
+  return uint32_t(uint64_t(static_cast<long long>(rint(x * qInv))));
+}
 template <int D> struct MixedRec {
-  static constexpr int F4 = D == 1 ? 2 : (D == 2 ? 3 : 4);
+  static constexpr int F4 = D == 1 ? 2 : 3;
 };
 __device__ __forceinline__ float dlo(double x) { return __int_as_float(__double2loint(x)); }
 __device__ __forceinline__ float dhi(double x) { return __int_as_float(__double2hiint(x)); }
 template <int D>
-__device__ __forceinline__ void writeMixedRec(const double* r, int64_t nInt, float4* __restrict__ p) {
+__device__ __forceinline__ void writeMixedRec(const double* r, int64_t nInt, double qInv, float4* __restrict__ p) {
   using R = RL<true>;
   const bool ghost = r[R::ORIG] >= double(nInt);
   const float hinv = ghost ? -float(r[R::HINV]) : float(r[R::HINV]);
@@ -42,10 +49,10 @@ __device__ __forceinline__ void writeMixedRec(const double* r, int64_t nInt, flo
     p[1] = make_float4(hinv, m, invRho, float(r[R::V]));
     p[2] = make_float4(float(r[R::V + 1]), cs, prho, mor);
   } else {
-    p[0] = make_float4(dlo(r[R::X]), dhi(r[R::X]), dlo(r[R::X + 1]), dhi(r[R::X + 1]));
-    p[1] = make_float4(dlo(r[R::X + 2]), dhi(r[R::X + 2]), hinv, m);
-    p[2] = make_float4(invRho, float(r[R::V]), float(r[R::V + 1]), float(r[R::V + 2]));
-    p[3] = make_float4(cs, prho, mor, 0.0f);
+    p[0] = make_float4(__uint_as_float(quantPos(r[R::X], qInv)), __uint_as_float(quantPos(r[R::X + 1], qInv)),
+                       __uint_as_float(quantPos(r[R::X + 2], qInv)), hinv);
+    p[1] = make_float4(m, invRho, float(r[R::V]), float(r[R::V + 1]));
+    p[2] = make_float4(float(r[R::V + 2]), cs, prho, mor);
   }
 }
 // The FP64 isotropic pair kernel's record, 96 B per slot in two planes so that
@@ -107,6 +114,7 @@ struct PackOut {
   void* sec2;
   int64_t n, nInt;
   void* sec3;
+  double qInv;     // 1/u of the 3-D mixed record's positions
 };
 
 // The records of slot k.  SEC (bits): also write, from the values in
@@ -127,7 +135,7 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
     double2* dst = reinterpret_cast<double2*>(o.rec + k * R::STRIDE);
 #pragma unroll
     for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-    if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * MixedRec<D>::F4);
+    if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, o.qInv, static_cast<float4*>(o.sec) + k * MixedRec<D>::F4);
     if constexpr ((SEC & 2) != 0)
       writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
     if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
@@ -159,7 +167,7 @@ __device__ __forceinline__ void packSlot(int64_t k, const PackIn& in, const Pack
   double2* dst = reinterpret_cast<double2*>(o.rec + k * R::STRIDE);
 #pragma unroll
   for (int c = 0; c < R::STRIDE / 2; ++c) dst[c] = make_double2(r[2 * c], r[2 * c + 1]);
-  if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, static_cast<float4*>(o.sec) + k * MixedRec<D>::F4);
+  if constexpr ((SEC & 1) != 0) writeMixedRec<D>(r, o.nInt, o.qInv, static_cast<float4*>(o.sec) + k * MixedRec<D>::F4);
   if constexpr ((SEC & 2) != 0)
     writeFast64Rec<D>(r, o.nInt, static_cast<double2*>(o.sec) + k * 4, static_cast<double2*>(o.sec) + 4 * o.n + k * 2);
   if constexpr ((SEC & 4) != 0) writeDensRec<D>(r, static_cast<double2*>(o.sec2) + k * 3);
@@ -170,6 +178,9 @@ struct Sim;
 PackIn packInputs(Sim& s);
 // allocates the records that sec asks for; sets recSValid and recSMValid from bits 4 and 8
 PackOut packOutputs(Sim& s, int sec);
+// 1/u for the 3-D mixed record's positions (a power of two; from the last
+// neighbour build's largest search radius)
+double quantInv(const Sim& s);
 // The records of the ghost slots only (after the ghost rows' fields changed).
 void packGhostRecords(Sim& s, int sec);
 
