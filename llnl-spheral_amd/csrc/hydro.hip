@@ -607,12 +607,9 @@ __global__ void __launch_bounds__(kBlock) k_derivs_fp64(const double* __restrict
 // kernel to round-off (the bar for search row order is 1e-12), in FP32 to the
 // mixed-precision bar (1e-5).
 //
-// Neighbour records:
-//  FP32, 64 B (16 floats): [0..5] x, y, z (FP64)  [6] 1/h  [7] det H (negative
-//    for ghost nodes)  [8] m  [9] 1/rho  [10..12] v  [13] c  [14] P/rho^2/omega
-//    [15] m/rho
-//  FP64, 128 B (16 doubles): [0..2] x  [3] 1/h  [4] det H (negative for
-//    ghosts)  [5] m  [6] 1/rho  [7..9] v  [10] c  [11] P/rho^2/omega  [12] m/rho
+// Neighbour records (written in pack.hh): the mixed record (MixedRec, 32 / 48 /
+// 48 B in 1-D / 2-D / 3-D) and the FP64 record (96 B in two planes); both mark
+// ghosts by a negative 1/h, and the reader forms det H = (1/h)^D.
 // ---------------------------------------------------------------------------
 template <typename Real> struct INbr {
   double x[3];
@@ -1140,7 +1137,7 @@ void sumDensityD(Sim& s) {
     SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
     return;
   }
-  // isotropic H: neighbours from the 64-B density records
+  // isotropic H: neighbours from the 48-B density records
   if (s.isotropic && !s.recSValid) {
     s.recS.ensure(sizeof(double) * DensRec::STRIDE * size_t(std::max<int64_t>(n, 1)));
     hipLaunchKernelGGL(k_pack_dens<D>, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), n,

@@ -152,8 +152,8 @@ struct Sim {
   // packed per-node records (sorted order) and outputs (sorted order)
   DBuf rec;
   bool recValid = false;    // records match the caller-order state
-  DBuf recM;                // mixed-precision neighbour records (64 B per slot)
-  DBuf recD;                // FP64 records of the collapsed isotropic pair loop (128 B per slot)
+  DBuf recM;                // mixed-precision neighbour records (32 / 48 / 48 B per slot in 1 / 2 / 3-D)
+  DBuf recD;                // FP64 records of the collapsed isotropic pair loop (96 B per slot, two planes)
   DBuf recS;                // FP64 density-sum neighbour records, isotropic H (48 B per slot)
   bool recSValid = false;    // recS matches rec (written by the record pack, renumbered by the cull)
   DBuf recSM;               // mixed density-sum neighbour records (32 B per slot)

@@ -11,10 +11,10 @@ namespace sphgpu {
 // ---------------------------------------------------------------------------
 // Records
 // ---------------------------------------------------------------------------
-// The isotropic pair kernels' records, from an FP64 record r (isotropic
-// layout): FP32, 64 B, for the mixed kernels (positions keep their FP64 bits);
-// FP64, 96 B in two planes, for the FP64 kernel.  Ghost rows are marked by a
-// negative det H (mixed) or 1/h (FP64).
+// The neighbour records of the isotropic pair and density kernels, each made
+// from an FP64 record r (isotropic layout).  The kernels' time is set by the
+// number of 16-B loads per neighbour gather, so each record holds only what
+// its kernel reads.
 // The mixed pair kernel's record (FP32), float4s per slot by dimension: 1-D
 // (x, +-1/h, m | 1/rho, v, c, P/rho^2/omega), 2-D (x, y | +-1/h, m, 1/rho, vx |
 // vy, c, P/rho^2/omega, m/rho) with x, y as FP64 bits, 3-D (qx, qy, qz, +-1/h |

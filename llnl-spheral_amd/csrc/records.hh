@@ -1,7 +1,9 @@
 // Per-node records in the neighbour build's sorted order.
 //
 // Every pass gathers the state of its neighbours; packing what one side of a
-// pair needs into one record keeps each gather to whole 128-byte lines.
+// pair needs into one record keeps each gather to whole 128-byte lines.  The
+// isotropic pair and density kernels gather smaller records made from these
+// (pack.hh).
 //
 //  isotropic-H record (every H = h I, no Q multipliers): 16 doubles, one line
 //    [0..2] x  [3] 1/h=H.xx  [4] m  [5] det H  [6] caller index  [7] rho
