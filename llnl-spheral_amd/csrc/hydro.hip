@@ -1028,6 +1028,7 @@ PackOut packOutputs(Sim& s, int sec) {
   }
   s.recSValid = (sec & 4) != 0;
   s.recSMValid = (sec & 8) != 0;
+  s.pairRec = sec & 3;
   return o;
 }
 
@@ -1095,6 +1096,7 @@ void packMixed(Sim& s) {
   s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(s.nSlots, 1)));
   hipLaunchKernelGGL(k_pack_mixed<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), s.nSlots,
                      s.nInt, quantInv(s), s.recM.as<float4>());
+  s.pairRec = 1;
   SG_CHECK_HIP(hipGetLastError());
 }
 
@@ -1103,6 +1105,7 @@ void packFast64(Sim& s) {
   s.recD.ensure(sizeof(double) * 12 * size_t(std::max<int64_t>(s.nSlots, 1)));
   hipLaunchKernelGGL(k_pack_fast64<D>, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, s.stream, s.rec.as<double>(),
                      s.nSlots, s.nInt, s.recD.as<double2>());
+  s.pairRec = 2;
   SG_CHECK_HIP(hipGetLastError());
 }
 
@@ -1160,7 +1163,7 @@ void sumDensityD(Sim& s) {
 }
 
 template <int D>
-void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly, int fresh) {
+void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   const int64_t n = s.nSlots;
   hipStream_t st = s.stream;
   const int total = derivTotalDoubles(s.dim);
@@ -1183,9 +1186,9 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly, int fre
   if (!s.recValid) {
     packD<D>(s, sec);
     s.recValid = true;
-  } else if (sec == 1 && fresh != 1) {
+  } else if (sec == 1 && s.pairRec != 1) {
     packMixed<D>(s);
-  } else if (sec == 2 && fresh != 2) {
+  } else if (sec == 2 && s.pairRec != 2) {
     packFast64<D>(s);
   }
   SG_CHECK_HIP(hipMemsetAsync(dncalc, 0, sizeof(unsigned long long), st));
@@ -1368,10 +1371,10 @@ double idealHPass(Sim& s, const sphgpu_hydro_params& hp, double tol, double* del
   return m;
 }
 
-void packRecords(Sim& s) {
-  if (s.dim == 1) packD<1>(s);
-  else if (s.dim == 2) packD<2>(s);
-  else packD<3>(s);
+void packRecords(Sim& s, int sec) {
+  if (s.dim == 1) packD<1>(s, sec);
+  else if (s.dim == 2) packD<2>(s, sec);
+  else packD<3>(s, sec);
   s.recValid = true;
 }
 
@@ -1393,11 +1396,12 @@ void sumDensity(Sim& s) {
   if (s.dim == 1) sumDensityD<1>(s);
   else if (s.dim == 2) sumDensityD<2>(s);
   else sumDensityD<3>(s);
+  s.pairRec = 0;      // the density changed rho in the records
   s.timingPending |= 2u;
   s.derivsValid = false;
 }
 
-void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly, int fresh) {
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "evaluateDerivatives needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   SG_REQUIRE(p.compatibleEnergyEvolution == 0 || p.precision == SPHGPU_PRECISION_FP64, SPHGPU_ERR_UNSUPPORTED,
@@ -1408,9 +1412,9 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly, int fresh
              "unknown precision");
   SG_REQUIRE(p.nPerh > 0.0, SPHGPU_ERR_ARG, "nPerh must be positive");
   s.derivsValid = false;
-  if (s.dim == 1) evalDerivsD<1>(s, p, updateOnly, fresh);
-  else if (s.dim == 2) evalDerivsD<2>(s, p, updateOnly, fresh);
-  else evalDerivsD<3>(s, p, updateOnly, fresh);
+  if (s.dim == 1) evalDerivsD<1>(s, p, updateOnly);
+  else if (s.dim == 2) evalDerivsD<2>(s, p, updateOnly);
+  else evalDerivsD<3>(s, p, updateOnly);
   s.timingPending |= 4u;
   s.derivsValid = true;
   s.pairsValid = true;

@@ -421,10 +421,16 @@ __global__ void k_copy_many(CopySet c) {
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < c.n[q]; i += stride) c.d[q][i] = c.s[q][i];
 }
 
-// Integrator::preStepInitialize: ghosts and connectivity
-void preStep(Sim& s, const sphgpu_integrator_params& ip) {
+// Integrator::preStepInitialize: ghosts and connectivity.  The build's record
+// pack also writes the pair kernel's record, which the step reads first (grad-h
+// recomputes omega, which the records hold, before that).
+void preStep(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip) {
+  const bool mixed = hp.precision == SPHGPU_PRECISION_MIXED;
+  const bool fast64 = s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && hp.epsTensile == 0.0 && !hp.compatibleEnergyEvolution;
+  s.buildRecHint = ip.gradhCorrection ? 0 : (mixed ? 1 : (fast64 ? 2 : 0));
   if (!s.subs.empty()) setGhostNodes(s, ip.kernelExtent, ip.cullGhostNodes != 0);
   else buildNeighbors(s, ip.kernelExtent);
+  s.buildRecHint = -1;
 }
 
 // The two derivative passes and updates of SynchronousRK2::step, then
@@ -489,7 +495,7 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
   ghostsAndRecords(recMid);
   // derivatives at t + dt/2 with the step's neighbour rows
   if (ip.gradhCorrection) omegaGradh(s);
-  evalDerivs(s, hp, false, recMid);
+  evalDerivs(s, hp);
   // advance from state0 with the midpoint derivatives
   if (compat) compatD<D>(s, b0.vel, dt, full);
   updateD<D>(s, b0, dt, bd, e, compat ? s.compDe.as<double>() : nullptr, full, recEnd);
@@ -519,7 +525,7 @@ __global__ void k_plain_seg_init(SegState* __restrict__ g, double t, double last
 template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
                              double maxTime) {
   hipStream_t st = s.stream;
-  preStep(s, ip);
+  preStep(s, hp, ip);
   // GenericHydro::dt, then selectDt (dtMin/dtMax clamped to maxTime - t, dtGrowth)
   s.plainSeg.ensure(sizeof(SegState));
   SegState* sd = s.plainSeg.as<SegState>();
@@ -554,7 +560,7 @@ template <int D> void ensembleRk2D(Sim& s, const sphgpu_hydro_params& hp, const 
   hipStream_t st = s.stream;
   const int ns = s.nSeg;
   SegState* sd = s.segState.as<SegState>();
-  preStep(s, ip);
+  preStep(s, hp, ip);
   // GenericHydro::dt per member, then selectDt per member
   const double init = double(FLT_MAX);
   unsigned long long ib;

@@ -155,6 +155,9 @@ struct Sim {
   DBuf recM;                // mixed-precision neighbour records (32 / 48 / 48 B per slot in 1 / 2 / 3-D)
   DBuf recD;                // FP64 records of the collapsed isotropic pair loop (96 B per slot, two planes)
   DBuf recS;                // FP64 density-sum neighbour records, isotropic H (48 B per slot)
+  int pairRec = 0;           // pair-kernel record current with rec: 1 mixed, 2 FP64 isotropic, 0 none
+  int buildRecHint = -1;     // record kinds (pack sec bits, isotropic H) the neighbour build's pack also
+                             // writes: -1 the density sum's, else what the caller will read first
   bool recSValid = false;    // recS matches rec (written by the record pack, renumbered by the cull)
   DBuf recSM;               // mixed density-sum neighbour records (32 B per slot)
   bool recSMValid = false;   // recSM matches rec (written by the record pack; the cull keeps it)
@@ -235,11 +238,12 @@ void resolveTimings(Sim& s);
 // neighbors.hip
 void buildNeighbors(Sim& s, double kernelExtent);
 // hydro.hip
-void packRecords(Sim& s);
+// sec: pack sec bits of the records written with the FP64 record (isotropic H)
+void packRecords(Sim& s, int sec = 0);
 void sumDensity(Sim& s);
 // updateOnly: store only DxDt, DvDt, DepsDt, DrhoDt and Hideal (the fields an
 // integrator update reads); the other fields keep what they held
-void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly = false, int fresh = -1);
+void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly = false);
 // The pack sec bits of the records the pair kernel and the density sum read.
 int pairRecordSec(const Sim& s, const sphgpu_hydro_params& hp);
 int densityRecordSec(const Sim& s);

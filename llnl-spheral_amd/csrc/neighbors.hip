@@ -1033,7 +1033,9 @@ void buildNeighborsD(Sim& s, double ext) {
   hipLaunchKernelGGL(k_iperm, dim3(nblocks(n)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), s.iperm.as<int32_t>(), n);
   SG_CHECK_HIP(hipGetLastError());
   s.nSlots = n;
-  packRecords(s);
+  // the records the next kernel reads, written with the FP64 record: the
+  // caller's hint, else the density sum's
+  packRecords(s, s.isotropic ? (s.buildRecHint >= 0 ? s.buildRecHint : densityRecordSec(s)) : 0);
 
   // 3. rows
   SearchArgs sa{};
