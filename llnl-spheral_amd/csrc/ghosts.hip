@@ -447,7 +447,7 @@ __global__ void k_compact_ghosts(const int32_t* __restrict__ control, const int3
 // record depends on the numbering, so they stay valid.
 __global__ void k_perm_remap(int32_t* __restrict__ perm, const int32_t* __restrict__ keep,
                              const int32_t* __restrict__ scan, int64_t nSlots, double* __restrict__ rec, int stride,
-                             int origOff, double* __restrict__ rec2, int stride2, int origOff2) {
+                             int origOff) {
   const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (k >= nSlots) return;
   const int32_t c = perm[k];
@@ -456,7 +456,6 @@ __global__ void k_perm_remap(int32_t* __restrict__ perm, const int32_t* __restri
   perm[k] = nc;
   const double o = nc >= 0 ? double(nc) : 2147483648.0;
   if (rec) rec[k * stride + origOff] = o;
-  if (rec2) rec2[k * stride2 + origOff2] = o;
 }
 
 __global__ void k_iperm_remap(const int32_t* __restrict__ iperm, const int32_t* __restrict__ keep,
@@ -714,11 +713,13 @@ void cull(Sim& s) {
   s.ghostSub.swap(s.ghostSubSpare);
   // slot <-> caller maps
   const bool recs = s.recValid;
-  // the density sum's records renumber too
-  const bool dens = recs && s.recSValid;
+  // The density records keep the pre-cull caller index of their ghosts: the
+  // density sum compares a neighbour's index only with an internal node's
+  // (to place the self term), and culling keeps every internal index and
+  // every ghost's index above them, so no comparison changes.
   hipLaunchKernelGGL(k_perm_remap, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), keep, scan,
                      s.nSlots, recs ? s.rec.as<double>() : nullptr, s.isotropic ? RL<true>::STRIDE : RL<false>::STRIDE,
-                     s.isotropic ? RL<true>::ORIG : RL<false>::ORIG, dens ? s.recS.as<double>() : nullptr, DensRec::STRIDE, DensRec::ORIG);
+                     s.isotropic ? RL<true>::ORIG : RL<false>::ORIG);
   s.ipermSpare.ensure(s.iperm.bytes);
   hipLaunchKernelGGL(k_iperm_remap, dim3(nblocks(n)), dim3(kBlock), 0, st, s.iperm.as<int32_t>(), keep, scan, n,
                      s.ipermSpare.as<int32_t>());

@@ -158,7 +158,7 @@ struct Sim {
   int pairRec = 0;           // pair-kernel record current with rec: 1 mixed, 2 FP64 isotropic, 0 none
   int buildRecHint = -1;     // record kinds (pack sec bits, isotropic H) the neighbour build's pack also
                              // writes: -1 the density sum's, else what the caller will read first
-  bool recSValid = false;    // recS matches rec (written by the record pack, renumbered by the cull)
+  bool recSValid = false;    // recS matches rec (written by the record pack; valid through a cull)
   DBuf recSM;               // mixed density-sum neighbour records (32 B per slot)
   bool recSMValid = false;   // recSM matches rec (written by the record pack; the cull keeps it)
   DBuf pairDv;              // compatible energy: own-frame pair term per row entry (D planes)
