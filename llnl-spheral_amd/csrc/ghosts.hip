@@ -201,23 +201,32 @@ __global__ void k_ghost_hmax(const double* __restrict__ pos, const double* __res
 }
 
 template <int D>
+// blockCount (optional): each workgroup's count of flags, for k_ghost_make<BLK>.
 __global__ void k_ghost_flag(const double* __restrict__ pos, const double* __restrict__ seg, int64_t n, DSub b,
                              double ext, const unsigned long long* __restrict__ hmaxBits, int32_t* __restrict__ flags,
-                             const int64_t* __restrict__ nDev) {
+                             const int64_t* __restrict__ nDev, int32_t* __restrict__ blockCount) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  if (i >= liveRows(n, nDev)) { flags[i] = 0; return; }   // capacity rows beyond the live count
-  const double hmax = __longlong_as_double((long long)hmaxBits[seg ? int(seg[i]) : 0]);
   int32_t f = 0;
-  if (hmax > 0.0) {
-    const double disti = signedDist<D>(b.exitP, b.exitN, loadV<D>(pos, i)) / hmax;
-    f = (disti >= 0.0 && disti <= ext) ? 1 : 0;
+  if (i < liveRows(n, nDev)) {                    // capacity rows beyond the live count take no flag
+    const double hmax = __longlong_as_double((long long)hmaxBits[seg ? int(seg[i]) : 0]);
+    if (hmax > 0.0) {
+      const double disti = signedDist<D>(b.exitP, b.exitN, loadV<D>(pos, i)) / hmax;
+      f = (disti >= 0.0 && disti <= ext) ? 1 : 0;
+    }
   }
-  flags[i] = f;
+  if (i < n) flags[i] = f;
+  if (blockCount) {
+    const int c = __syncthreads_count(f);        // every thread of the block reaches it
+    if (threadIdx.x == 0) blockCount[blockIdx.x] = c;
+  }
 }
 
-template <int D>
-__global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, double* __restrict__ seg, int64_t n,
+// BLK: offs holds the exclusive scan of the flag kernel's per-workgroup counts
+// (same grid and workgroup size), and a row's offset is its workgroup's plus
+// its rank among the workgroup's flags: the global exclusive scan of flags.
+// Otherwise offs is that scan itself.
+template <int D, bool BLK>
+__global__ void __launch_bounds__(kBlock) k_ghost_make(double* __restrict__ pos, double* __restrict__ H, double* __restrict__ seg, int64_t n,
                              int64_t nInt, const int32_t* __restrict__ flags, const int32_t* __restrict__ offs, DSub b,
                              int subIdx, int32_t* __restrict__ control, int32_t* __restrict__ subOf,
                              double* __restrict__ hmaxOf, const int64_t* __restrict__ nDev, int64_t cap,
@@ -225,8 +234,23 @@ __global__ void k_ghost_make(double* __restrict__ pos, double* __restrict__ H, d
   constexpr int NS = NSym<D>::value;
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   const int64_t nn = liveRows(n, nDev);
-  if (i >= nn || !flags[i]) return;
-  const int64_t g = nn + offs[i];
+  int64_t off;
+  if constexpr (BLK) {
+    const bool f = i < nn && flags[i] != 0;
+    const unsigned long long m = __ballot(f);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ int waveCount[kBlock / 64];
+    if (lane == 0) waveCount[wave] = __popcll(m);
+    __syncthreads();
+    if (!f) return;
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += waveCount[w];
+    off = int64_t(offs[blockIdx.x]) + before + __popcll(m & ((1ull << lane) - 1ull));
+  } else {
+    if (i >= nn || !flags[i]) return;
+    off = offs[i];
+  }
+  const int64_t g = nn + off;
   if (nDev && g >= cap) { *overflow = 1; return; }   // the capacity guess was short: the build redoes it exactly
   const V<D> xg = ghostPosition<D>(b, loadV<D>(pos, i));
   for (int k = 0; k < D; ++k) pos[g * D + k] = xg.c[k];
@@ -527,7 +551,11 @@ template <int D> bool generateFastD(Sim& s, double ext, unsigned long long* hbit
   s.ghostControl.grow(sizeof(int32_t) * size_t(est), 0, st);
   s.ghostSub.grow(sizeof(int32_t) * size_t(est), 0, st);
   s.keep.ensure(sizeof(int32_t) * size_t(cap + 1));
-  s.keepScan.ensure(sizeof(int32_t) * size_t(cap + 1));
+  // per-workgroup flag counts (at nb + 1) and their exclusive scan (at 0)
+  const int64_t nb = nblocks(cap);
+  s.keepScan.ensure(sizeof(int32_t) * size_t(std::max<int64_t>(cap + 1, 2 * (nb + 1))));
+  int32_t* blkScan = s.keepScan.as<int32_t>();
+  int32_t* blkCount = blkScan + (nb + 1);
   s.genDev.ensure(sizeof(GenDev));
   GenDev* g = s.genDev.as<GenDev>();
   const int64_t* nDev = &g->n;
@@ -541,15 +569,14 @@ template <int D> bool generateFastD(Sim& s, double ext, unsigned long long* hbit
     else
       hipLaunchKernelGGL(k_ghost_hmax_one<D>, dim3(std::min(nblocks(cap), 1024)), dim3(kBlock), 0, st,
                          s.pos.as<double>(), s.hmaxOf.as<double>(), cap, d, ext, hbits, nDev);
-    hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(cap)), dim3(kBlock), 0, st, s.pos.as<double>(), segp, cap, d,
-                       ext, hbits, s.keep.as<int32_t>(), nDev);
-    exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), cap, s);
-    hipLaunchKernelGGL(k_ghost_make<D>, dim3(nblocks(cap)), dim3(kBlock), 0, st, s.pos.as<double>(),
-                       s.H.as<double>(), s.nSeg > 0 ? s.seg.as<double>() : nullptr, cap, nInt, s.keep.as<int32_t>(),
-                       s.keepScan.as<int32_t>(), d, int(b), s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(),
-                       s.hmaxOf.as<double>(), nDev, cap, &g->overflow);
-    hipLaunchKernelGGL(k_gen_advance, dim3(1), dim3(kBlock), 0, st, g, int(b), s.keepScan.as<int32_t>() + cap, nInt,
-                       cap, hbits, nh);
+    hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nb), dim3(kBlock), 0, st, s.pos.as<double>(), segp, cap, d, ext, hbits,
+                       s.keep.as<int32_t>(), nDev, blkCount);
+    exclusiveScanI32(blkCount, blkScan, nb, s);
+    hipLaunchKernelGGL((k_ghost_make<D, true>), dim3(nb), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
+                       s.nSeg > 0 ? s.seg.as<double>() : nullptr, cap, nInt, s.keep.as<int32_t>(), blkScan, d, int(b),
+                       s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>(), nDev, cap,
+                       &g->overflow);
+    hipLaunchKernelGGL(k_gen_advance, dim3(1), dim3(kBlock), 0, st, g, int(b), blkScan + nb, nInt, cap, hbits, nh);
     SG_CHECK_HIP(hipGetLastError());
   }
   GenDev* h = static_cast<GenDev*>(hostStaging(s, 0, sizeof(GenDev)));
@@ -598,7 +625,7 @@ template <int D> void generateD(Sim& s, double ext) {
       hipLaunchKernelGGL(k_ghost_hmax_one<D>, dim3(std::min(nblocks(n), 1024)), dim3(kBlock), 0, st,
                          s.pos.as<double>(), s.hmaxOf.as<double>(), n, d, ext, hbits, nullptr);
     hipLaunchKernelGGL(k_ghost_flag<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), segp, n, d, ext,
-                       hbits, s.keep.as<int32_t>(), nullptr);
+                       hbits, s.keep.as<int32_t>(), nullptr, nullptr);
     exclusiveScanI32(s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), n, s);
     int32_t* hc = static_cast<int32_t*>(hostStaging(s, 0, sizeof(int32_t)));
     SG_CHECK_HIP(hipMemcpyAsync(hc, s.keepScan.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -611,7 +638,7 @@ template <int D> void generateD(Sim& s, double ext) {
     const int64_t ng = n + count - s.nInt;
     s.ghostControl.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
     s.ghostSub.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
-    hipLaunchKernelGGL(k_ghost_make<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
+    hipLaunchKernelGGL((k_ghost_make<D, false>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
                        s.nSeg > 0 ? s.seg.as<double>() : nullptr, n, s.nInt, s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), d, int(b),
                        s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>(), nullptr,
                        int64_t(0), nullptr);
