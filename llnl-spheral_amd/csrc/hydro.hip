@@ -30,6 +30,7 @@
 #include "ell.hh"
 #include "eigen.hh"
 #include <cstring>
+#include <cmath>
 
 namespace sphgpu {
 
@@ -991,8 +992,12 @@ TabArgs tabArgs(const Sim& s) {
 }  // namespace
 
 double quantInv(const Sim& s) {
+  // only the 3-D mixed record holds quantised positions; with no finite search
+  // radius there is no pair to difference and any unit serves
+  if (s.dim != 3 || !(s.rMax > 0.0) || !std::isfinite(s.rMax)) return 1.0;
   const int e = std::ilogb(s.rMax) + 1 - 30;     // rMax < 2^30 u
-  SG_REQUIRE(s.rMax > 0.0 && e > -120 && e < 120, SPHGPU_ERR_STATE, "search radius out of range");
+  SG_REQUIRE(e > -120 && e < 120, SPHGPU_ERR_UNSUPPORTED,
+             "mixed precision: the search radius is outside the FP32 exponent range");
   return std::ldexp(1.0, -e);
 }
 
@@ -1010,7 +1015,7 @@ PackOut packOutputs(Sim& s, int sec) {
   SG_REQUIRE((sec & 3) != 3, SPHGPU_ERR_STATE, "one pair-kernel record per pack");
   const int stride = iso ? RL<true>::STRIDE : RL<false>::STRIDE;
   s.rec.ensure(sizeof(double) * size_t(n) * stride);
-  PackOut o{s.rec.as<double>(), nullptr, nullptr, n, s.nInt, nullptr, quantInv(s)};
+  PackOut o{s.rec.as<double>(), nullptr, nullptr, n, s.nInt, nullptr, (sec & 1) ? quantInv(s) : 1.0};
   if (sec & 1) {
     s.recM.ensure(sizeof(float) * 16 * size_t(std::max<int64_t>(n, 1)));
     o.sec = s.recM.p;
@@ -1204,7 +1209,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   p.WnPerh = s.table.kernelValue(1.0 / hp.nPerh, 1.0);
   p.XSPH = hp.XSPH; p.correctVelocityGradient = hp.correctVelocityGradient; p.evolveTotalEnergy = hp.evolveTotalEnergy;
   p.linearInExpansion = hp.linearInExpansion; p.quadraticInExpansion = hp.quadraticInExpansion;
-  p.qInv = quantInv(s);
+  p.qInv = mixed ? quantInv(s) : 1.0;
   p.qUnit = float(1.0 / p.qInv);
   OutArgs o{};
   o.base = s.out.as<double>();

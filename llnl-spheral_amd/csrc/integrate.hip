@@ -427,10 +427,13 @@ __global__ void k_copy_many(CopySet c) {
 void preStep(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip) {
   const bool mixed = hp.precision == SPHGPU_PRECISION_MIXED;
   const bool fast64 = s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && hp.epsTensile == 0.0 && !hp.compatibleEnergyEvolution;
+  struct Hint {     // the hint holds for this build only, also when it throws
+    Sim& s;
+    ~Hint() { s.buildRecHint = -1; }
+  } hint{s};
   s.buildRecHint = ip.gradhCorrection ? 0 : (mixed ? 1 : (fast64 ? 2 : 0));
   if (!s.subs.empty()) setGhostNodes(s, ip.kernelExtent, ip.cullGhostNodes != 0);
   else buildNeighbors(s, ip.kernelExtent);
-  s.buildRecHint = -1;
 }
 
 // The two derivative passes and updates of SynchronousRK2::step, then
