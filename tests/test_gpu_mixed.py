@@ -28,6 +28,12 @@ def cases():
     out.append(("random1d", 1, st, "reflecting", dict(evolveTotalEnergy=1)))
     st, _ = problems.random_state(3, 8, seed=94, h_var=0.1)
     out.append(("noxsph3d", 3, st, None, dict(XSPH=0, correctVelocityGradient=0)))
+    # far from the origin, negative coordinates too: the 3-D mixed record's
+    # positions are the low 32 bits of round(x / u), here past 2^32 and wrapped
+    st, _ = problems.random_state(3, 9, seed=95, h_var=0.15)
+    st = dict(st)
+    st["pos"] = st["pos"] + np.array([1000.0, -1000.0, 250.5])
+    out.append(("shifted3d", 3, st, None, {}))
     return out
 
 
