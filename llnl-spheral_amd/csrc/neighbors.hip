@@ -159,42 +159,67 @@ __global__ void k_seg_bounds_init(unsigned long long* __restrict__ bits, int nSe
   if (i < nSeg * kNB) bits[i] = isMaxSlot(i % kNB) ? 0ull : ~0ull;
 }
 
-// The bounds of k_bounds per member: a wavefront whose nodes share one member
-// reduces in registers and makes one atomic per value.
+// The bounds of k_bounds per member.  A wavefront walks kSegGroups consecutive
+// groups of 64 rows (members are contiguous runs of rows) and keeps a running
+// reduction in registers while its groups stay in one member, with one atomic
+// per value when the member changes or the walk ends; a group that spans
+// members makes its atomics per lane.  Min and max are order-free, so the
+// result does not depend on the grouping.
+constexpr int kSegGroups = 16;
 template <int D>
 __global__ void k_seg_bounds(const double* __restrict__ pos, const double* __restrict__ H,
                              const double* __restrict__ seg, int64_t n, unsigned long long* __restrict__ bits) {
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
-  int m = -1;
-  if (i < n) {
-    m = int(seg[i]);
-    for (int k = 0; k < D; ++k) { const double x = pos[i * D + k]; r[k] = x; r[3 + k] = x; }
-    const double* h = H + i * NSym<D>::value;
-    r[6] = gershgorinMin<D>(h);
-    r[7] = isIsotropic<D>(h) ? 1.0 : 0.0;
-    r[8] = maxDiag<D>(h);
-    r[9] = r[6];
-  }
-  const int m0 = __shfl(m, 0);
-  if (m0 < 0) return;                      // lanes past n come last: the whole wavefront is idle
-  if (__all(m == m0 || m < 0)) {
-    for (int k = 0; k < kNB; ++k)
-      for (int sh = 32; sh > 0; sh >>= 1) {
-        const double o = __shfl_xor(r[k], sh);
-        r[k] = isMaxSlot(k) ? dmax(r[k], o) : dmin(r[k], o);
-      }
-    if ((threadIdx.x & 63) == 0)
-      for (int k = 0; k < kNB; ++k) {
-        if (isMaxSlot(k)) atomicMax(bits + int64_t(m0) * kNB + k, orderedBits(r[k]));
-        else atomicMin(bits + int64_t(m0) * kNB + k, orderedBits(r[k]));
-      }
-  } else if (m >= 0) {
+  const int lane = threadIdx.x & 63;
+  const int64_t base = ((blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6) * int64_t(64 * kSegGroups);
+  const double ident[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
+  double acc[kNB];
+  for (int k = 0; k < kNB; ++k) acc[k] = ident[k];
+  int mAcc = -1;                           // member of the running reduction (wave-uniform)
+  auto atomicOne = [&](int m, int k, double v) {
+    if (isMaxSlot(k)) atomicMax(bits + int64_t(m) * kNB + k, orderedBits(v));
+    else atomicMin(bits + int64_t(m) * kNB + k, orderedBits(v));
+  };
+  auto flush = [&]() {                     // called by the whole wavefront
     for (int k = 0; k < kNB; ++k) {
-      if (isMaxSlot(k)) atomicMax(bits + int64_t(m) * kNB + k, orderedBits(r[k]));
-      else atomicMin(bits + int64_t(m) * kNB + k, orderedBits(r[k]));
+      double v = acc[k];
+      for (int sh = 32; sh > 0; sh >>= 1) {
+        const double o = __shfl_xor(v, sh);
+        v = isMaxSlot(k) ? dmax(v, o) : dmin(v, o);
+      }
+      if (lane == 0) atomicOne(mAcc, k, v);
+      acc[k] = ident[k];
+    }
+  };
+  for (int it = 0; it < kSegGroups; ++it) {
+    const int64_t i = base + int64_t(it) * 64 + lane;
+    double r[kNB];
+    for (int k = 0; k < kNB; ++k) r[k] = ident[k];
+    int m = -1;
+    if (i < n) {
+      m = int(seg[i]);
+      for (int k = 0; k < D; ++k) { const double x = pos[i * D + k]; r[k] = x; r[3 + k] = x; }
+      const double* h = H + i * NSym<D>::value;
+      r[6] = gershgorinMin<D>(h);
+      r[7] = isIsotropic<D>(h) ? 1.0 : 0.0;
+      r[8] = maxDiag<D>(h);
+      r[9] = r[6];
+    }
+    const int m0 = __shfl(m, 0);
+    if (m0 < 0) break;                     // lane 0 holds the group's first row: the rest are past n too
+    if (__all(m == m0 || m < 0)) {
+      if (m0 != mAcc) {
+        if (mAcc >= 0) flush();
+        mAcc = m0;
+      }
+      for (int k = 0; k < kNB; ++k) acc[k] = isMaxSlot(k) ? dmax(acc[k], r[k]) : dmin(acc[k], r[k]);
+    } else {
+      if (mAcc >= 0) flush();
+      mAcc = -1;
+      if (m >= 0)
+        for (int k = 0; k < kNB; ++k) atomicOne(m, k, r[k]);
     }
   }
+  if (mAcc >= 0) flush();
 }
 
 template <int D> __device__ __forceinline__ void cellCoords(const DGrid& g, const double* x, int* c) {
@@ -948,8 +973,8 @@ void buildNeighborsD(Sim& s, double ext) {
     s.segBoundsBits.ensure(sizeof(unsigned long long) * size_t(ns) * kNB);
     unsigned long long* bits = s.segBoundsBits.as<unsigned long long>();
     hipLaunchKernelGGL(k_seg_bounds_init, dim3(nblocks(int64_t(ns) * kNB)), dim3(kBlock), 0, st, bits, ns);
-    hipLaunchKernelGGL((k_seg_bounds<D>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
-                       s.H.as<double>(), s.seg.as<double>(), n, bits);
+    hipLaunchKernelGGL((k_seg_bounds<D>), dim3(nblocks((n + kSegGroups - 1) / kSegGroups)), dim3(kBlock), 0, st,
+                       s.pos.as<double>(), s.H.as<double>(), s.seg.as<double>(), n, bits);
     SG_CHECK_HIP(hipGetLastError());
     const size_t nbits = size_t(ns) * kNB;
     unsigned long long* hbits = static_cast<unsigned long long*>(hostStaging(s, 0, nbits * sizeof(unsigned long long)));
