@@ -203,8 +203,10 @@ struct Sim {
   bool ensemble = false;
   int nSeg = 0;
   DBuf seg;                      // member index per caller row (as a double)
-  DBuf segGrid, segCellBase;     // neighbour grid list (levels, members): grid, first global cell (+1 entry)
-  DBuf gridFirst;                // first grid of each member (+1 entry)
+  DBuf segGrid;                  // neighbour grid list (levels, members), in one buffer:
+  void* gridList = nullptr;      //   the grids (DGrid, neighbors.hip),
+  int64_t* gridCellBase = nullptr;  // their first global cells (+1 entry),
+  int32_t* gridFirst = nullptr;  //   the first grid of each member (+1 entry)
   DBuf segBoundsBits;            // per member: 9 ordered-bit bounds of the neighbour build
   DBuf segHmaxBits;              // per member: ghost-selection hmax bits
   DBuf segState;                 // SegState per member

@@ -936,20 +936,21 @@ int64_t uploadGrids(Sim& s, const std::vector<DGrid>& grids, const std::vector<i
   std::vector<int64_t> base(ng + 1, 0);
   for (size_t q = 0; q < ng; ++q) base[q + 1] = base[q] + cells[q];
   SG_REQUIRE(base.back() < (int64_t(1) << 31) - 2, SPHGPU_ERR_CAPACITY, "neighbour grid too large");
-  s.segGrid.ensure(sizeof(DGrid) * ng);
-  s.segCellBase.ensure(sizeof(int64_t) * (ng + 1));
-  s.gridFirst.ensure(sizeof(int32_t) * first.size());
-  // staged in pinned memory; the next use waits for pinnedInDone, so the
-  // copies need no wait here
+  // the grids, their first global cells and each member's first grid in one
+  // buffer, staged in pinned memory at the same offsets and sent in one copy;
+  // the next use of the staging waits for pinnedInDone, so no wait here
   const size_t bG = sizeof(DGrid) * ng, bB = sizeof(int64_t) * (ng + 1), bF = sizeof(int32_t) * first.size();
   const size_t oB = (bG + 63) / 64 * 64, oF = oB + (bB + 63) / 64 * 64;
+  s.segGrid.ensure(oF + bF);
+  char* d = static_cast<char*>(s.segGrid.p);
+  s.gridList = d;
+  s.gridCellBase = reinterpret_cast<int64_t*>(d + oB);
+  s.gridFirst = reinterpret_cast<int32_t*>(d + oF);
   char* h = static_cast<char*>(hostStaging(s, 1, oF + bF));
   std::memcpy(h, grids.data(), bG);
   std::memcpy(h + oB, base.data(), bB);
   std::memcpy(h + oF, first.data(), bF);
-  SG_CHECK_HIP(hipMemcpyAsync(s.segGrid.p, h, bG, hipMemcpyHostToDevice, st));
-  SG_CHECK_HIP(hipMemcpyAsync(s.segCellBase.p, h + oB, bB, hipMemcpyHostToDevice, st));
-  SG_CHECK_HIP(hipMemcpyAsync(s.gridFirst.p, h + oF, bF, hipMemcpyHostToDevice, st));
+  SG_CHECK_HIP(hipMemcpyAsync(d, h, oF + bF, hipMemcpyHostToDevice, st));
   if (!s.pinnedInDone) SG_CHECK_HIP(hipEventCreateWithFlags(&s.pinnedInDone, hipEventDisableTiming));
   SG_CHECK_HIP(hipEventRecord(s.pinnedInDone, st));
   return base.back();
@@ -998,8 +999,7 @@ void buildNeighborsD(Sim& s, double ext) {
     s.rMax = ext / hb[6];
     G = Grid{};
     G.ncells = uploadGrids(s, grids, cells, first);
-    sgr = SegGrids{s.seg.as<double>(), s.segGrid.as<DGrid>(), s.segCellBase.as<int64_t>(),
-                   s.gridFirst.as<int32_t>(), ext};
+    sgr = SegGrids{s.seg.as<double>(), static_cast<const DGrid*>(s.gridList), s.gridCellBase, s.gridFirst, ext};
   } else {
     const int nbB = int(std::min<int64_t>(1024, nblocks(n)));
     s.reduceTmp.ensure(sizeof(double) * (size_t(nbB) * kNB + 16));
@@ -1024,7 +1024,7 @@ void buildNeighborsD(Sim& s, double ext) {
       std::vector<int32_t> first = {0, int32_t(grids.size())};
       G = Grid{};
       G.ncells = uploadGrids(s, grids, cells, first);
-      sgr = SegGrids{nullptr, s.segGrid.as<DGrid>(), s.segCellBase.as<int64_t>(), s.gridFirst.as<int32_t>(), ext};
+      sgr = SegGrids{nullptr, static_cast<const DGrid*>(s.gridList), s.gridCellBase, s.gridFirst, ext};
     }
   }
 
