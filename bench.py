@@ -41,6 +41,13 @@ NBR_BYTES_PER_ENTRY = 4      # one int32 neighbour index per directed entry
 # HBM bytes per pair-kernel dispatch, from calibrated FETCH_SIZE / WRITE_SIZE runs
 PROFILES = os.path.join(ROOT, "profiles")
 VALU_ISSUE_HZ = 1024 * 2.4e9 / 4     # wavefront instructions per second: 1024 SIMDs, 4 cycles each
+# Neighbour gathers: each 16-B load of a random row is one L1 line access per
+# lane, and a CU's L1 serves about one access per cycle.  This is a model of the
+# pair kernel's gather limit, not a published peak.
+L1_LINE_RATE = 256 * 2.4e9
+# 16-B loads per neighbour of the isotropic pair kernels (96-B FP64 record,
+# 48-B 3-D mixed record)
+GATHER_LOADS = {("fp64", "search"): 6, ("mixed", "search"): 3}
 
 
 def measured_counters(n, precision, row_order, boundaries):
@@ -476,14 +483,23 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": counters_src,
             "algorithmic_bytes": alg_bytes,
-            "note": "the pair kernel is bound by the latency of its neighbour gathers at 2-3 waves/SIMD, "
-                    "not by HBM bandwidth: see valu for the issue-time fraction",
+            "note": "the pair kernel is bound by its neighbour gathers (L1 line accesses: see gather) and "
+                    "by VALU issue (see valu), not by HBM bandwidth",
         },
     }
     if counters and counters.get("valu_wave_instructions"):
         issue_ms = 1e3 * counters["valu_wave_instructions"] / VALU_ISSUE_HZ
         line["roofline"]["valu"] = {"wave_instructions": counters["valu_wave_instructions"], "issue_ms": issue_ms,
                                     "kernel_ms": derivs_ms, "busy_frac": issue_ms / derivs_ms}
+    loads = GATHER_LOADS.get((args.precision, args.row_order)) if args.n >= 64 else None
+    if loads:
+        acc = entries * loads
+        rate = acc / (derivs_ms * 1e-3)
+        line["roofline"]["gather"] = {
+            "loads_per_neighbour": loads, "line_accesses": acc, "achieved_per_s": rate,
+            "model_peak_per_s": L1_LINE_RATE, "frac": rate / L1_LINE_RATE,
+            "note": "model: one L1 line access per lane per 16-B load of a neighbour record, "
+                    "one access per CU per cycle (256 CUs at 2.4 GHz)"}
     if args.rk2_steps > 0:
         # full time steps on the same resident state: ghosts + neighbours, dt
         # vote, two derivative passes, updates, sum density, enforcement
