@@ -201,10 +201,9 @@ template <int D, bool ISO> struct DNbr {
   double hdet, m, orig;
 };
 template <int D, bool ISO>
-__device__ __forceinline__ DNbr<D, ISO> loadDNbr(const double* __restrict__ rec, int32_t j,
-                                                 int stride = RL<ISO>::STRIDE) {
+__device__ __forceinline__ DNbr<D, ISO> loadDNbr(const double* __restrict__ rec, int32_t j) {
   using R = RL<ISO>;
-  const double* r = rec + int64_t(j) * stride;
+  const double* r = rec + int64_t(j) * R::STRIDE;
   DNbr<D, ISO> o;
   for (int c = 0; c < D; ++c) o.x.c[c] = r[R::X + c];
   if constexpr (ISO) {
