@@ -1066,7 +1066,11 @@ void launchPack(Sim& s, int sec, const int32_t* slots, int64_t count) {
 // in the same pass.
 template <int D>
 void packD(Sim& s, int sec = 0) {
-  launchPack<D>(s, sec, nullptr, s.nSlots);
+  // With every slot live (no culled ghost), walk the caller rows so that the
+  // field reads are contiguous and each slot's record is written whole; else
+  // walk the slots, which also writes the dead ones.
+  if (s.nSlots == s.nTot) launchPack<D>(s, sec, s.iperm.as<int32_t>(), s.nTot);
+  else launchPack<D>(s, sec, nullptr, s.nSlots);
 }
 
 void packGhostRecords(Sim& s, int sec) {
