@@ -344,9 +344,33 @@ int sphgpu_ensemble_add(sphgpu_sim* sim, const sphgpu_member* member, const sphg
                         int64_t* id);
 /* One SynchronousRK2 step of every member (each with its own dt, clamped to
  * its end time).  *finished = number of members that reached their end time;
- * they must be downloaded and removed before the next step. */
+ * they must be downloaded and removed before the next step.
+ * A member that fails is evicted by the step itself and the others are
+ * stepped as if it had never been there (their results are unchanged): a
+ * NaN or infinity in its position or H at the step start, or in its state
+ * after the step's final update, or a dt vote below the dt threshold.
+ * sphgpu_ensemble_failed lists the members the last step evicted.  The
+ * reference has no per-simulation failure: Process::haltAll ends the run
+ * (src/Utilities/Process.cc:74-87) and a step below
+ * Integrator::dtThreshold is only reported (src/Integrator/Integrator.cc:190-197). */
 int sphgpu_ensemble_step(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, const sphgpu_integrator_params* integ,
                          int* finished);
+/* Reasons a member fails (bits). */
+enum {
+  SPHGPU_MEMBER_OK = 0,
+  SPHGPU_MEMBER_NONFINITE = 1,    /* NaN or infinity in its state                   */
+  SPHGPU_MEMBER_BAD_H = 2,        /* an H without a positive lower eigenvalue bound */
+  SPHGPU_MEMBER_DT_COLLAPSE = 4   /* its dt vote fell below the dt threshold        */
+};
+/* Members evicted by the last sphgpu_ensemble_step: id, reason
+ * (SPHGPU_MEMBER_*), steps taken (arrays of length cap, any may be null);
+ * *count = number evicted. */
+int sphgpu_ensemble_failed(sphgpu_sim* sim, int cap, int64_t* ids, int32_t* reasons, int64_t* cycles, int* count);
+/* A member whose dt vote (cfl times GenericHydro::dt's minimum) falls below
+ * dtThreshold fails with SPHGPU_MEMBER_DT_COLLAPSE; 0 (the default) never.
+ * Like Integrator::dtThreshold (default 1e-10, src/Integrator/Integrator.cc:56),
+ * which only reports such a step. */
+int sphgpu_ensemble_set_dt_threshold(sphgpu_sim* sim, double dtThreshold);
 /* Members in row order: id, time, last dt, cycle, first row, rows (arrays of
  * length cap, any may be null); *count = number of members. */
 int sphgpu_ensemble_members(sphgpu_sim* sim, int cap, int64_t* ids, double* time, double* last_dt, int64_t* cycle,
@@ -372,8 +396,9 @@ int64_t sphgpu_ensemble_bytes_per_node(int dim);
  * ghostFactor per member) fits hbmBudget and fewer than maxMembers are
  * resident; all resident members advance together (sphgpu_ensemble_step) and
  * each leaves when it reaches tEnd, with its final state copied to the out
- * arrays.  Returns when every request has finished.  The sim must be an empty
- * ensemble (sphgpu_ensemble_begin). */
+ * arrays, or when it fails (status set, the others unaffected).  Returns when
+ * every request has finished or failed.  The sim must be an empty ensemble
+ * (sphgpu_ensemble_begin). */
 typedef struct sphgpu_request {
   sphgpu_member member;        /* the simulation                                          */
   double arrival;              /* seconds after the start of the call                     */
@@ -383,13 +408,17 @@ typedef struct sphgpu_request {
   double* outEps;              /* n, or null                                              */
   double* outH;                /* n*dim(dim+1)/2, or null                                 */
   double admitted;             /* out: seconds after the start when it joined             */
-  double finished;             /* out: seconds after the start when it reached tEnd       */
+  double finished;             /* out: seconds after the start when it reached tEnd or
+                                       failed                                             */
   int64_t steps;               /* out: RK2 steps taken                                    */
+  int32_t status;              /* out: SPHGPU_MEMBER_OK, or the reason it failed (its out
+                                       arrays are then left untouched)                    */
 } sphgpu_request;
 typedef struct sphgpu_serve_params {
   int64_t hbmBudget;           /* bytes; <= 0: 80% of the device memory free at the start */
   int maxMembers;              /* resident members at most                                */
   double ghostFactor;          /* footprint rows per internal node (>= 1)                 */
+  double dtThreshold;          /* sphgpu_ensemble_set_dt_threshold for the call (0: off)  */
 } sphgpu_serve_params;
 int sphgpu_serve(sphgpu_sim* sim, sphgpu_request* requests, int n_requests, const sphgpu_hydro_params* hydro,
                  const sphgpu_integrator_params* integ, const sphgpu_serve_params* serve);

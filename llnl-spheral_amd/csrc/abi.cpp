@@ -25,6 +25,10 @@ template <class F> int guarded(sphgpu_sim* s, F&& f) {
     return SPHGPU_OK;
   } catch (const Error& e) {
     return fail(s, e.code, e.what());
+  } catch (const MemberFailure& e) {
+    // only sphgpu_ensemble_step evicts failing members; other calls report them
+    return fail(s, SPHGPU_ERR_ARG, "ensemble member " + std::to_string(e.members.front()) +
+                                       " (row order) has a non-finite position or H, or H without a positive bound");
   } catch (const std::exception& e) {
     return fail(s, SPHGPU_ERR_HIP, e.what());
   }
@@ -543,6 +547,28 @@ int sphgpu_ensemble_step(sphgpu_sim* s, const sphgpu_hydro_params* hp, const sph
     SG_REQUIRE(hp && ip, SPHGPU_ERR_ARG, "null parameters");
     const int done = ensembleStep(*s, *hp, *ip);
     if (finished) *finished = done;
+  });
+}
+
+int sphgpu_ensemble_set_dt_threshold(sphgpu_sim* s, double dtThreshold) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->ensemble, SPHGPU_ERR_STATE, "not an ensemble");
+    SG_REQUIRE(dtThreshold >= 0.0, SPHGPU_ERR_ARG, "dt threshold must be >= 0");
+    s->dtThreshold = dtThreshold;
+  });
+}
+
+int sphgpu_ensemble_failed(sphgpu_sim* s, int cap, int64_t* ids, int32_t* reasons, int64_t* cycles, int* count) {
+  return guarded(s, [&] {
+    SG_REQUIRE(s->ensemble, SPHGPU_ERR_STATE, "not an ensemble");
+    SG_REQUIRE(cap >= 0, SPHGPU_ERR_ARG, "negative capacity");
+    const int nf = int(s->failedIds.size());
+    if (count) *count = nf;
+    for (int q = 0; q < std::min(cap, nf); ++q) {
+      if (ids) ids[q] = s->failedIds[size_t(q)];
+      if (reasons) reasons[q] = s->failedReasons[size_t(q)];
+      if (cycles) cycles[q] = s->failedCycles[size_t(q)];
+    }
   });
 }
 

@@ -184,6 +184,30 @@ void ensembleRemove(Sim& s, int nIds, const int64_t* ids) {
   invalidate(s);
 }
 
+// Failure isolation.  A member fails alone when its position or H holds a
+// NaN or infinity at the step start (found by the neighbour build's
+// per-member bounds, before anything of the step is written), when its state
+// turns non-finite in the step's final update, or when its dt vote falls
+// below the ensemble's dtThreshold.  It is evicted (rows compacted away,
+// recorded in failedIds with its reason) and the others are unaffected: their
+// rows, grids, ghosts, dt and sums never involve another member.  The
+// reference has no per-simulation failure: Process::haltAll ends the run
+// (src/Utilities/Process.cc:74-87) and a step below Integrator::dtThreshold is
+// only reported (src/Integrator/Integrator.cc:190-197).
+namespace {
+void evictFailed(Sim& s, const std::vector<int>& rowsOrder, const std::vector<int>& reasons) {
+  std::vector<int64_t> ids;
+  for (size_t q = 0; q < rowsOrder.size(); ++q) {
+    const int64_t id = s.members[size_t(rowsOrder[q])].id;
+    ids.push_back(id);
+    s.failedIds.push_back(id);
+    s.failedReasons.push_back(reasons[q]);
+    s.failedCycles.push_back(int64_t(s.segHost[size_t(rowsOrder[q])].cycle));
+  }
+  ensembleRemove(s, int(ids.size()), ids.data());
+}
+}  // namespace
+
 int ensembleStep(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip) {
   SG_REQUIRE(s.ensemble && s.nSeg > 0, SPHGPU_ERR_STATE, "the ensemble has no members");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
@@ -192,7 +216,24 @@ int ensembleStep(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_
   for (size_t q = 0; q < s.segHost.size(); ++q)
     SG_REQUIRE(s.segHost[q].t < s.segHost[q].tEnd, SPHGPU_ERR_STATE,
                "a member reached its end time: download and remove it before the next step");
-  ensembleRk2(s, hp, ip);
+  s.failedIds.clear();
+  s.failedReasons.clear();
+  s.failedCycles.clear();
+  for (;;) {
+    try {
+      ensembleRk2(s, hp, ip);
+      break;
+    } catch (MemberFailure& f) {
+      evictFailed(s, f.members, f.reasons);
+      if (s.nSeg == 0) return 0;
+    }
+  }
+  MemberFailure late;
+  for (size_t q = 0; q < s.segHost.size(); ++q) {
+    const unsigned f = s.segHost[q].fail;
+    if (f) late.add(int(q), (f & SPHGPU_MEMBER_NONFINITE) ? SPHGPU_MEMBER_NONFINITE : int(f));
+  }
+  if (!late.members.empty()) evictFailed(s, late.members, late.reasons);
   int done = 0;
   for (const SegState& g : s.segHost) done += (g.t >= g.tEnd) ? 1 : 0;
   return done;
@@ -206,7 +247,7 @@ int64_t ensembleBytesPerNode(int dim) {
   const int64_t out = 8 * int64_t(derivTotalDoubles(D));
   const int64_t ell = 4 * (D == 3 ? 160 : (D == 2 ? 48 : 16)) + 4;
   const int64_t sortMaps = 4 * 8;                      // cellOf, perm, iperm, work, flags, scans
-  const int64_t step = 8 * (2 * D + 2 + NS) + 8 * (2 + D);   // state0 and dt inputs
+  const int64_t step = 8 * (2 * D + 2 + NS) + 8 * (2 + D) + 8;   // state0, dt inputs, grad-h omega0
   return state + spare + rec + out + ell + sortMaps + step;
 }
 

@@ -997,7 +997,13 @@ double quantInv(const Sim& s) {
   const int e = std::ilogb(s.rMax) + 1 - 30;     // rMax < 2^30 u
   SG_REQUIRE(e > -120 && e < 120, SPHGPU_ERR_UNSUPPORTED,
              "mixed precision: the search radius is outside the FP32 exponent range");
-  return std::ldexp(1.0, -e);
+  const double q = std::ldexp(1.0, -e);
+  // round(x / u) must fit a 64-bit integer; 2^61 leaves room for the motion
+  // of one step beyond the positions the neighbour build saw
+  SG_REQUIRE(s.posAbsMax * q < std::ldexp(1.0, 61), SPHGPU_ERR_UNSUPPORTED,
+             "mixed precision: positions too far from the origin for the 3-D fixed-point record (|x| >= 2^61 u, "
+             "u = largest search radius / 2^30)");
+  return q;
 }
 
 PackIn packInputs(Sim& s) {

@@ -74,7 +74,7 @@ struct Bounds {
 // Ensemble members: the update's multiplier (dt or dt/2) and gamma per member.
 struct SegArgs {
   const double* seg;        // member of each row; null: one simulation
-  const SegState* st;       // null: the scalar multiplier and EOS apply to every row
+  SegState* st;             // null: the scalar multiplier and EOS apply to every row
   int half;                 // 1: multiply by dt/2
 };
 __device__ __forceinline__ void rowMultEos(const SegArgs& sa, int64_t i, double& mult, Eos& e) {
@@ -115,9 +115,18 @@ __global__ void k_update(const int32_t* __restrict__ work, const int32_t* __rest
   int c = 0;
   for (int a = 0; a < D; ++a)
     for (int q = a; q < D; ++q, ++c) f.H[i * NS + c] = (a == q) ? dmin(bd.hminInv, dmax(bd.hmaxInv, hid[c])) : hid[c];
-  f.P[i] = eosP(e, rho, eps);
-  f.cs[i] = eosCs(e, eps);
+  const double P = eosP(e, rho, eps), cs = eosCs(e, eps);
+  f.P[i] = P;
+  f.cs[i] = cs;
   if constexpr (REC >= 0) packSlot<D, true, REC>(k, pin, pout);
+  // an ensemble member whose state turns non-finite fails alone (the step's
+  // final update; NaN at the midpoint reaches it)
+  if (sa.seg && !sa.half) {
+    bool ok = isfinite(eps) && isfinite(rho) && isfinite(P) && isfinite(cs);
+    for (int c = 0; c < D; ++c) ok = ok && isfinite(f.pos[i * D + c]) && isfinite(f.vel[i * D + c]);
+    for (int c = 0; c < NS; ++c) ok = ok && isfinite(f.H[i * NS + c]);
+    if (!ok) atomicOr(&sa.st[int(sa.seg[i])].fail, unsigned(SPHGPU_MEMBER_NONFINITE));
+  }
 }
 
 // maxViscousPressure, tr DvDx and DvDt of the last evaluation, by caller index.
@@ -381,15 +390,21 @@ template <int D> double hydroDtD(Sim& s, double nPerh, double cfl, int useVelMag
 // Per-member dt selection, the same expressions as rk2D's host selection.
 __global__ void k_seg_vote_init(SegState* __restrict__ st, int n, unsigned long long v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) st[i].voteBits = v;
+  if (i < n) {
+    st[i].voteBits = v;
+    st[i].fail = 0u;
+  }
 }
+// dtThreshold > 0: a member whose vote falls below it (a collapsing step) is
+// flagged; its step still runs and it is evicted afterwards.
 __global__ void k_seg_select_dt(SegState* __restrict__ st, int n, double cfl, double dtMinIn, double dtMaxIn,
-                                double dtGrowth) {
+                                double dtGrowth, double dtThreshold) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   SegState& g = st[i];
   const double dtMin = dmin(dtMinIn, g.tEnd - g.t), dtMax = dmin(dtMaxIn, g.tEnd - g.t);
   const double vote = cfl * __longlong_as_double((long long)g.voteBits);
+  if (dtThreshold > 0.0 && !(vote >= dtThreshold)) g.fail |= unsigned(SPHGPU_MEMBER_DT_COLLAPSE);
   double dt = dtMax;
   if (vote > 0.0 && vote < dt) dt = vote;
   dt = dmin(dt, dtGrowth * g.lastDt);
@@ -480,6 +495,13 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
       if (bytes[q]) SG_CHECK_HIP(hipMemcpyAsync(s.state0[q].p, src[q]->p, bytes[q], hipMemcpyDeviceToDevice, st));
   }
   const Base b0{s.state0[0].as<double>(), s.state0[1].as<double>(), s.state0[2].as<double>(), s.state0[3].as<double>()};
+  // omega has no update policy (SPHHydroBase::registerState enrolls it bare):
+  // state.assign(state0) gives the final update the step-start value again
+  const size_t omegaBytes = sizeof(double) * size_t(s.nTot);
+  if (ip.gradhCorrection && omegaBytes) {
+    s.omega0.ensure(omegaBytes);
+    SG_CHECK_HIP(hipMemcpyAsync(s.omega0.p, s.omega.p, omegaBytes, hipMemcpyDeviceToDevice, st));
+  }
   // The updates write the records the next pass reads (isotropic H; grad-h
   // recomputes omega, which the records hold, before the midpoint pass).
   const bool recs = s.isotropic && !ip.gradhCorrection;
@@ -499,6 +521,8 @@ template <int D> void stepBody(Sim& s, const sphgpu_hydro_params& hp, const sphg
   // derivatives at t + dt/2 with the step's neighbour rows
   if (ip.gradhCorrection) omegaGradh(s);
   evalDerivs(s, hp);
+  if (ip.gradhCorrection && omegaBytes)
+    SG_CHECK_HIP(hipMemcpyAsync(s.omega.p, s.omega0.p, omegaBytes, hipMemcpyDeviceToDevice, st));
   // advance from state0 with the midpoint derivatives
   if (compat) compatD<D>(s, b0.vel, dt, full);
   updateD<D>(s, b0, dt, bd, e, compat ? s.compDe.as<double>() : nullptr, full, recEnd);
@@ -523,6 +547,8 @@ __global__ void k_plain_seg_init(SegState* __restrict__ g, double t, double last
   g->hdt = 0.0;
   g->voteBits = vote;
   g->cycle = 0.0;
+  g->fail = 0u;
+  g->failPad = 0u;
 }
 
 template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
@@ -541,7 +567,8 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
                      s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
                      s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(), s.dtIn.as<double>(),
                      s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt, nullptr, &sd->voteBits, int64_t(0));
-  hipLaunchKernelGGL(k_seg_select_dt, dim3(1), dim3(kBlock), 0, st, sd, 1, ip.cfl, ip.dtMin, ip.dtMax, ip.dtGrowth);
+  hipLaunchKernelGGL(k_seg_select_dt, dim3(1), dim3(kBlock), 0, st, sd, 1, ip.cfl, ip.dtMin, ip.dtMax, ip.dtGrowth,
+                     0.0);
   SG_CHECK_HIP(hipGetLastError());
   const SegArgs half{nullptr, sd, 1}, full{nullptr, sd, 0};
   stepBody<D>(s, hp, ip, 0.0, half, full);
@@ -576,7 +603,7 @@ template <int D> void ensembleRk2D(Sim& s, const sphgpu_hydro_params& hp, const 
                      s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt, s.seg.as<double>(), &sd->voteBits,
                      int64_t(sizeof(SegState) / sizeof(unsigned long long)));
   hipLaunchKernelGGL(k_seg_select_dt, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns, ip.cfl, ip.dtMin, ip.dtMax,
-                     ip.dtGrowth);
+                     ip.dtGrowth, s.dtThreshold);
   SG_CHECK_HIP(hipGetLastError());
   const SegArgs half{s.seg.as<double>(), sd, 1}, full{s.seg.as<double>(), sd, 0};
   stepBody<D>(s, hp, ip, 0.0, half, full);

@@ -15,6 +15,14 @@ struct Error : std::runtime_error {
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// Ensemble members (indices in row order) that cannot take the step, with
+// their SPHGPU_MEMBER_* reasons: thrown by the neighbour build before any
+// state of the step is written, so the others can be stepped without them.
+struct MemberFailure {
+  std::vector<int> members, reasons;
+  void add(int m, int reason) { members.push_back(m); reasons.push_back(reason); }
+};
+
 #define SG_CHECK_HIP(expr)                                                                     \
   do {                                                                                         \
     hipError_t e_ = (expr);                                                                    \
@@ -105,6 +113,8 @@ struct SegState {
   double dt, hdt;           // this step's dt and dt/2
   unsigned long long voteBits;   // GenericHydro::dt minimum (bits of a positive double)
   double cycle;
+  unsigned int fail;        // SPHGPU_MEMBER_* of this step (0: healthy)
+  unsigned int failPad;
 };
 
 // One member of an ensemble: a contiguous range of internal caller rows.
@@ -135,6 +145,7 @@ struct Sim {
   Grid grid;
   double kernelExtent = 2.0;
   double rMax = 1.0;         // largest search radius of the last neighbour build
+  double posAbsMax = 0.0;    // largest |position component| of the last neighbour build
   int rowOrder = SPHGPU_ROW_ORDER_REFERENCE;
   bool neighborsValid = false;
   bool isotropic = false;   // every H is h*I (off-diagonals exactly zero)
@@ -192,6 +203,7 @@ struct Sim {
   double lastDt = 1.0e-5;        // Integrator::mLastDt default
   int64_t cycle = 0;
   DBuf state0[5];                // pos, vel, eps, rho, H of the internal rows at the step start
+  DBuf omega0;                   // grad-h: every row's step-start omega (state.assign restores it)
   DBuf dtIn;                     // per internal node: maxViscousPressure, tr DvDx, DvDt (caller order)
   bool haveDtIn = false;
 
@@ -214,6 +226,10 @@ struct Sim {
   std::vector<SegState> segHost;
   std::vector<Member> members;
   int64_t nextMemberId = 0;
+  double dtThreshold = 0.0;      // a member whose dt vote falls below this fails (0: never)
+  std::vector<int64_t> failedIds;   // members evicted by the last step, with their reasons
+  std::vector<int32_t> failedReasons;
+  std::vector<int64_t> failedCycles;   // steps each had taken
   DBuf segSpare, dtInSpare;
 
   hipEvent_t ev[14] = {};          // 0-11 stage timings, 12-13 the copy probe

@@ -60,9 +60,31 @@ template <int D> __device__ bool isIsotropic(const double* h) {
 }
 
 // reduced values: lo[3] hi[3] min Gershgorin bound, isotropic flag, max diagonal
-// of H, max Gershgorin bound
-constexpr int kNB = 10;
-__host__ __device__ __forceinline__ bool isMaxSlot(int k) { return (k >= 3 && k < 6) || k == 8 || k == 9; }
+// of H, max Gershgorin bound, non-finite flag (1 when a position or H entry is
+// NaN or infinite: min and max would pass a NaN over)
+constexpr int kNB = 11;
+constexpr int kNonFinite = 10;
+__host__ __device__ __forceinline__ bool isMaxSlot(int k) { return (k >= 3 && k < 6) || k >= 8; }
+#define SG_BOUNDS_IDENT {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300, 0.0}
+
+// Bounds a grid can be made from: finite positions and H, and a positive
+// Gershgorin bound (1: ok; 0: a non-finite entry; -1: no positive bound).
+int boundsUsable(const double* hb) {
+  if (hb[kNonFinite] > 0.5) return 0;
+  return (hb[6] > 0.0 && std::isfinite(hb[6]) && hb[9] > 0.0 && std::isfinite(hb[9])) ? 1 : -1;
+}
+void requireUsableBounds(const double* hb) {
+  const int u = boundsUsable(hb);
+  SG_REQUIRE(u != 0, SPHGPU_ERR_ARG, "non-finite node position or H");
+  SG_REQUIRE(u > 0, SPHGPU_ERR_UNSUPPORTED, "neighbour search needs H with a positive Gershgorin bound on every node");
+}
+
+template <int D> __device__ __forceinline__ bool rowFinite(const double* x, const double* h) {
+  bool ok = true;
+  for (int k = 0; k < D; ++k) ok = ok && isfinite(x[k]);
+  for (int k = 0; k < NSym<D>::value; ++k) ok = ok && isfinite(h[k]);
+  return ok;
+}
 
 template <int D> __device__ double maxDiag(const double* h) {
   double m = h[0];
@@ -73,7 +95,7 @@ template <int D> __device__ double maxDiag(const double* h) {
 template <int D>
 __global__ void k_bounds(const double* __restrict__ pos, const double* __restrict__ H, int64_t n, double* partials) {
   __shared__ double sh[kNB][kBlock];
-  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
+  double r[kNB] = SG_BOUNDS_IDENT;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
     for (int k = 0; k < D; ++k) { const double x = pos[i * D + k]; r[k] = dmin(r[k], x); r[3 + k] = dmax(r[3 + k], x); }
     const double* h = H + i * NSym<D>::value;
@@ -82,6 +104,7 @@ __global__ void k_bounds(const double* __restrict__ pos, const double* __restric
     if (!isIsotropic<D>(h)) r[7] = 0.0;
     r[8] = dmax(r[8], maxDiag<D>(h));
     r[9] = dmax(r[9], gm);
+    if (!rowFinite<D>(pos + i * D, h)) r[kNonFinite] = 1.0;
   }
   for (int k = 0; k < kNB; ++k) sh[k][threadIdx.x] = r[k];
   __syncthreads();
@@ -98,7 +121,7 @@ __global__ void k_bounds(const double* __restrict__ pos, const double* __restric
 
 __global__ void k_bounds_final(const double* partials, int nb, double* out) {
   __shared__ double sh[kNB][kBlock];
-  double r[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
+  double r[kNB] = SG_BOUNDS_IDENT;
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int k = 0; k < kNB; ++k) {
       const double v = partials[b * kNB + k];
@@ -171,7 +194,7 @@ __global__ void k_seg_bounds(const double* __restrict__ pos, const double* __res
                              const double* __restrict__ seg, int64_t n, unsigned long long* __restrict__ bits) {
   const int lane = threadIdx.x & 63;
   const int64_t base = ((blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6) * int64_t(64 * kSegGroups);
-  const double ident[kNB] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 1.0, -1e300, -1e300};
+  const double ident[kNB] = SG_BOUNDS_IDENT;
   double acc[kNB];
   for (int k = 0; k < kNB; ++k) acc[k] = ident[k];
   int mAcc = -1;                           // member of the running reduction (wave-uniform)
@@ -203,6 +226,7 @@ __global__ void k_seg_bounds(const double* __restrict__ pos, const double* __res
       r[7] = isIsotropic<D>(h) ? 1.0 : 0.0;
       r[8] = maxDiag<D>(h);
       r[9] = r[6];
+      r[kNonFinite] = rowFinite<D>(pos + i * D, h) ? 0.0 : 1.0;
     }
     const int m0 = __shfl(m, 0);
     if (m0 < 0) break;                     // lane 0 holds the group's first row: the rest are past n too
@@ -875,10 +899,7 @@ void gridWithCell(const double* hb, double cell, Grid& G, DGrid& dg, int64_t& br
 template <int D>
 void gridFromBounds(const double* hb, double ext, bool allowStaged, Grid& G, DGrid& dg, int64_t& bricks,
                     bool& stagedPath) {
-  SG_REQUIRE(hb[6] > 0.0 && std::isfinite(hb[6]), SPHGPU_ERR_UNSUPPORTED,
-             "neighbour search needs H with a positive Gershgorin bound on every node");
-  for (int k = 0; k < D; ++k)
-    SG_REQUIRE(std::isfinite(hb[k]) && std::isfinite(hb[3 + k]), SPHGPU_ERR_ARG, "non-finite node position");
+  requireUsableBounds(hb);
   const double R = ext / hb[6] * (1.0 + 1.0e-6);
   double spanX = 0.0;
   for (int k = 0; k < D; ++k) spanX = std::max(spanX, hb[3 + k] - hb[k]);
@@ -900,6 +921,7 @@ void gridFromBounds(const double* hb, double ext, bool allowStaged, Grid& G, DGr
 // One grid when the radii are within a factor 2.
 template <int D>
 void levelGrids(const double* hb, double ext, std::vector<DGrid>& out, std::vector<int64_t>& cells) {
+  requireUsableBounds(hb);
   const double R = ext / hb[6] * (1.0 + 1.0e-6);
   const double Rmin = ext / hb[9] * (1.0 + 1.0e-6);
   Grid G;
@@ -984,10 +1006,23 @@ void buildNeighborsD(Sim& s, double ext) {
     std::vector<DGrid> grids;
     std::vector<int64_t> cells;
     std::vector<int32_t> first(size_t(ns) + 1, 0);
-    bool iso = true;
+    // a member with a non-finite position or H (or no positive bound of its
+    // H) fails alone: the step is abandoned before anything of it is kept,
+    // and the caller evicts the member and steps the others
+    MemberFailure bad;
     for (int m = 0; m < ns; ++m) {
       double hm[kNB];
       for (int k = 0; k < kNB; ++k) hm[k] = fromOrderedBits(hbits[size_t(m) * kNB + k]);
+      const int u = boundsUsable(hm);
+      if (u <= 0) bad.add(m, u == 0 ? SPHGPU_MEMBER_NONFINITE : SPHGPU_MEMBER_BAD_H);
+    }
+    if (!bad.members.empty()) throw bad;
+    bool iso = true;
+    double absMax = 0.0;
+    for (int m = 0; m < ns; ++m) {
+      double hm[kNB];
+      for (int k = 0; k < kNB; ++k) hm[k] = fromOrderedBits(hbits[size_t(m) * kNB + k]);
+      for (int k = 0; k < D; ++k) absMax = std::max(absMax, std::max(std::fabs(hm[k]), std::fabs(hm[3 + k])));
       iso = iso && hm[7] > 0.5;
       levelGrids<D>(hm, ext, grids, cells);
       first[size_t(m) + 1] = int32_t(grids.size());
@@ -997,6 +1032,7 @@ void buildNeighborsD(Sim& s, double ext) {
     }
     s.isotropic = iso && !s.hasMultipliers;
     s.rMax = ext / hb[6];
+    s.posAbsMax = absMax;
     G = Grid{};
     G.ncells = uploadGrids(s, grids, cells, first);
     sgr = SegGrids{s.seg.as<double>(), static_cast<const DGrid*>(s.gridList), s.gridCellBase, s.gridFirst, ext};
@@ -1012,8 +1048,11 @@ void buildNeighborsD(Sim& s, double ext) {
     SG_CHECK_HIP(hipMemcpyAsync(hbp, bounds, sizeof(hb), hipMemcpyDeviceToHost, st));
     SG_CHECK_HIP(hipStreamSynchronize(st));
     std::memcpy(hb, hbp, sizeof(hb));
+    requireUsableBounds(hb);
     s.isotropic = hb[7] > 0.5 && !s.hasMultipliers;
     s.rMax = ext / hb[6];
+    s.posAbsMax = 0.0;
+    for (int k = 0; k < D; ++k) s.posAbsMax = std::max(s.posAbsMax, std::max(std::fabs(hb[k]), std::fabs(hb[3 + k])));
     std::vector<DGrid> grids;
     std::vector<int64_t> cells;
     levelGrids<D>(hb, ext, grids, cells);

@@ -29,12 +29,21 @@ void serve(Sim& s, sphgpu_request* req, int n, const sphgpu_hydro_params& hp, co
     SG_CHECK_HIP(hipMemGetInfo(&freeB, &totalB));
     budget = int64_t(0.8 * double(freeB));
   }
+  SG_REQUIRE(sp.dtThreshold >= 0.0, SPHGPU_ERR_ARG, "dt threshold must be >= 0");
   std::vector<int> order(static_cast<size_t>(n));
   for (int q = 0; q < n; ++q) {
     order[size_t(q)] = q;
     req[q].admitted = req[q].finished = -1.0;
     req[q].steps = 0;
+    req[q].status = SPHGPU_MEMBER_OK;
   }
+  const double thresholdBefore = s.dtThreshold;
+  struct Restore {      // the ensemble's own threshold again, also when a step throws
+    Sim& s;
+    double v;
+    ~Restore() { s.dtThreshold = v; }
+  } restore{s, thresholdBefore};
+  s.dtThreshold = sp.dtThreshold;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return req[a].arrival < req[b].arrival; });
   using clock = std::chrono::steady_clock;
   const auto t0 = clock::now();
@@ -66,6 +75,18 @@ void serve(Sim& s, sphgpu_request* req, int n, const sphgpu_hydro_params& hp, co
       continue;
     }
     const int finished = ensembleStep(s, hp, ip);
+    // members the step evicted: failed, the others go on unaffected
+    for (size_t q = 0; q < s.failedIds.size(); ++q) {
+      auto it = std::find_if(active.begin(), active.end(), [&](const Active& x) { return x.id == s.failedIds[q]; });
+      SG_REQUIRE(it != active.end(), SPHGPU_ERR_STATE, "failed member not tracked");
+      sphgpu_request& r = req[it->q];
+      r.status = s.failedReasons[q];
+      r.finished = elapsed();
+      r.steps = s.failedCycles[q];
+      used -= it->bytes;
+      active.erase(it);
+      ++done;
+    }
     if (finished == 0) continue;
     // completions: copy out, then remove together
     std::vector<int64_t> gone;

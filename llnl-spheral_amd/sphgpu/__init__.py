@@ -75,11 +75,16 @@ class Request(C.Structure):
     """sphgpu_request: a simulation arriving at a serving loop."""
     _fields_ = [("member", Member), ("arrival", C.c_double), ("outPos", _dp), ("outVel", _dp), ("outRho", _dp),
                 ("outEps", _dp), ("outH", _dp), ("admitted", C.c_double), ("finished", C.c_double),
-                ("steps", C.c_int64)]
+                ("steps", C.c_int64), ("status", C.c_int32)]
 
 
 class ServeParams(C.Structure):
-    _fields_ = [("hbmBudget", C.c_int64), ("maxMembers", C.c_int), ("ghostFactor", C.c_double)]
+    _fields_ = [("hbmBudget", C.c_int64), ("maxMembers", C.c_int), ("ghostFactor", C.c_double),
+                ("dtThreshold", C.c_double)]
+
+
+# reasons an ensemble member fails (bits, SPHGPU_MEMBER_*)
+MEMBER_OK, MEMBER_NONFINITE, MEMBER_BAD_H, MEMBER_DT_COLLAPSE = 0, 1, 2, 4
 
 
 class Plane(C.Structure):
@@ -217,6 +222,8 @@ def lib():
                                                C.POINTER(C.c_int)]),
             "sphgpu_ensemble_members": (C.c_int, [vp, C.c_int, _i64p, _dp, _dp, _i64p, _i64p, _i64p,
                                                   C.POINTER(C.c_int)]),
+            "sphgpu_ensemble_failed": (C.c_int, [vp, C.c_int, _i64p, _i32p, _i64p, C.POINTER(C.c_int)]),
+            "sphgpu_ensemble_set_dt_threshold": (C.c_int, [vp, C.c_double]),
             "sphgpu_ensemble_download": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
             "sphgpu_ensemble_upload": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
             "sphgpu_ensemble_remove": (C.c_int, [vp, C.c_int, _i64p]),
@@ -507,6 +514,22 @@ class Sim:
         self._chk(self.L.sphgpu_ensemble_step(self.h, C.byref(hp), C.byref(ip), C.byref(done)))
         return done.value
 
+    def ensemble_failed(self):
+        """Members the last ensemble step evicted: list of dicts (id, reason, cycle)."""
+        cnt = C.c_int()
+        self._chk(self.L.sphgpu_ensemble_failed(self.h, 0, None, None, None, C.byref(cnt)))
+        n = cnt.value
+        ids, cyc = np.zeros(max(n, 1), dtype=np.int64), np.zeros(max(n, 1), dtype=np.int64)
+        why = np.zeros(max(n, 1), dtype=np.int32)
+        self._chk(self.L.sphgpu_ensemble_failed(self.h, n, _p(ids, _i64p), _p(why, _i32p), _p(cyc, _i64p),
+                                                C.byref(cnt)))
+        for q in range(n):
+            self._members.pop(int(ids[q]), None)
+        return [dict(id=int(ids[q]), reason=int(why[q]), cycle=int(cyc[q])) for q in range(n)]
+
+    def ensemble_set_dt_threshold(self, dt_threshold):
+        self._chk(self.L.sphgpu_ensemble_set_dt_threshold(self.h, float(dt_threshold)))
+
     def ensemble_members(self):
         """List of dicts (id, t, last_dt, cycle, first, rows) in row order."""
         cnt = C.c_int()
@@ -539,11 +562,11 @@ class Sim:
         for q in ids:
             self._members.pop(int(q), None)
 
-    def serve(self, requests, hydro=None, integ=None, budget=0, max_members=1024, ghost_factor=1.5):
+    def serve(self, requests, hydro=None, integ=None, budget=0, max_members=1024, ghost_factor=1.5, dt_threshold=0.0):
         """Run the request loop (sphgpu_serve) on this empty ensemble.
         requests: dicts with state, gamma, t_end, arrival (seconds).  Returns
-        per request: admitted, finished, steps and the final pos, vel, rho,
-        eps, H."""
+        per request: admitted, finished, steps, status (MEMBER_*) and the final
+        pos, vel, rho, eps, H."""
         hp = hydro if hydro is not None else hydro_params()
         ip = integ if integ is not None else integrator_params()
         dim, ns = self.dim, self.dim * (self.dim + 1) // 2
@@ -561,11 +584,13 @@ class Sim:
             reqs[q].arrival = r.get("arrival", 0.0)
             reqs[q].outPos, reqs[q].outVel = _p(o["pos"]), _p(o["vel"])
             reqs[q].outRho, reqs[q].outEps, reqs[q].outH = _p(o["rho"]), _p(o["eps"]), _p(o["H"])
-        sp = ServeParams(hbmBudget=int(budget), maxMembers=int(max_members), ghostFactor=float(ghost_factor))
+        sp = ServeParams(hbmBudget=int(budget), maxMembers=int(max_members), ghostFactor=float(ghost_factor),
+                         dtThreshold=float(dt_threshold))
         self._chk(self.L.sphgpu_serve(self.h, reqs, len(requests), C.byref(hp), C.byref(ip), C.byref(sp)))
         res = []
         for q, o in enumerate(outs):
-            o.update(admitted=reqs[q].admitted, finished=reqs[q].finished, steps=int(reqs[q].steps))
+            o.update(admitted=reqs[q].admitted, finished=reqs[q].finished, steps=int(reqs[q].steps),
+                     status=int(reqs[q].status))
             res.append(o)
         return res
 

@@ -46,9 +46,10 @@ def ensemble(sg, dim, row_order, precision=0):
     return e
 
 
-def run_ensemble(sg, e, members, hp, ip_kw, late=()):
+def run_ensemble(sg, e, members, hp, ip_kw, late=(), failed=None):
     """Step until every member finished; members in `late` join after the
-    first step.  Returns {index: (fields, cycle)}."""
+    first step.  Returns {index: (fields, cycle)}; members the steps evicted
+    go to `failed` ({index: (reason, cycle)})."""
     ip = sg.integrator_params(**ip_kw)
     ids = {}
     for q, m in enumerate(members):
@@ -58,6 +59,10 @@ def run_ensemble(sg, e, members, hp, ip_kw, late=()):
     first = True
     while ids:
         done = e.ensemble_step(hp, ip)
+        for f in e.ensemble_failed():
+            q = ids.pop(f["id"])
+            assert failed is not None, (q, f)
+            failed[q] = (f["reason"], f["cycle"])
         if first:
             for q in late:
                 m = members[q]
@@ -194,3 +199,126 @@ def test_ensemble_errors(sphgpu):
         e.ensemble_remove([mid + 7])
     e.ensemble_remove([mid])
     e.close()
+
+
+def _poisoned_tubes():
+    members = tubes(4, 60, 200, 0.015, seed=17)
+    bad = {k: np.array(v, dtype=np.float64) for k, v in members[1]["state"].items()}
+    bad["eps"][bad["eps"].shape[0] // 3] = np.nan       # a value, not a fault
+    members[1] = dict(members[1], state=bad)
+    pos = {k: np.array(v, dtype=np.float64) for k, v in members[2]["state"].items()}
+    pos["pos"][5, 0] = np.nan
+    members[2] = dict(members[2], state=pos)
+    return members
+
+
+@pytest.mark.parametrize("row_order", ["reference", "search"])
+def test_failed_members_evicted_others_unchanged(sphgpu, row_order):
+    """One tube carries a NaN specific energy, another a NaN position: the
+    first fails in the step whose final update turns its state non-finite,
+    the second at the first step's neighbour build; both are evicted with
+    MEMBER_NONFINITE and every other member equals its solo run bit for bit."""
+    order = sphgpu.ROW_ORDER_REFERENCE if row_order == "reference" else sphgpu.ROW_ORDER_SEARCH
+    members = _poisoned_tubes()
+    print(f"[ensemble failures] {row_order}", flush=True)
+    hp = sphgpu.hydro_params()
+    e = ensemble(sphgpu, 1, order)
+    failed = {}
+    got = run_ensemble(sphgpu, e, members, hp, {}, failed=failed)
+    e.close()
+    assert sorted(failed) == [1, 2], failed
+    assert failed[1] == (sphgpu.MEMBER_NONFINITE, 1), failed     # after its first step
+    assert failed[2] == (sphgpu.MEMBER_NONFINITE, 0), failed     # before any step
+    assert sorted(got) == [0, 3]
+    for q in (0, 3):
+        m = members[q]
+        ref, steps = plain_run(sphgpu, 1, m["state"], m["gamma"], m["t_end"], hp, {}, order)
+        assert got[q][1] == steps
+        check_same(got[q][0], ref, q)
+
+
+def test_collapsing_dt_member_evicted(sphgpu):
+    """A tube 1e8 times hotter than the others votes a dt near 1e-8; with the
+    dt threshold at 1e-6 it fails with MEMBER_DT_COLLAPSE on the first step
+    and the others finish as they would alone."""
+    members = tubes(3, 60, 120, 0.01, seed=23)
+    hot = {k: np.array(v, dtype=np.float64) for k, v in members[0]["state"].items()}
+    hot["eps"] *= 1.0e8
+    members[0] = dict(members[0], state=hot)
+    hp = sphgpu.hydro_params()
+    e = ensemble(sphgpu, 1, sphgpu.ROW_ORDER_SEARCH)
+    e.ensemble_set_dt_threshold(1.0e-6)
+    failed = {}
+    got = run_ensemble(sphgpu, e, members, hp, {}, failed=failed)
+    e.close()
+    assert failed == {0: (sphgpu.MEMBER_DT_COLLAPSE, 1)}, failed
+    for q in (1, 2):
+        m = members[q]
+        ref, steps = plain_run(sphgpu, 1, m["state"], m["gamma"], m["t_end"], hp, {}, sphgpu.ROW_ORDER_SEARCH)
+        assert got[q][1] == steps
+        check_same(got[q][0], ref, q)
+
+
+def test_serve_failed_request_isolated(sphgpu):
+    """The request loop with a poisoned request: its status says why it
+    failed, and every other request equals its lone solve."""
+    reqs = []
+    for q, n in enumerate((10, 13, 11)):
+        reqs.append(dict(state=problems.point_blast(n, seed=400 + q), gamma=5.0 / 3.0, t_end=0.003,
+                         arrival=0.01 * q))
+    bad = {k: np.array(v, dtype=np.float64) for k, v in reqs[1]["state"].items()}
+    bad["eps"][7] = np.inf
+    reqs[1] = dict(reqs[1], state=bad)
+    e = ensemble(sphgpu, 2, sphgpu.ROW_ORDER_SEARCH)
+    hp = sphgpu.hydro_params()
+    out = e.serve(reqs, hp, sphgpu.integrator_params(), max_members=3)
+    e.close()
+    assert out[1]["status"] == sphgpu.MEMBER_NONFINITE
+    for q in (0, 2):
+        o = out[q]
+        assert o["status"] == sphgpu.MEMBER_OK
+        r = reqs[q]
+        ref, steps = plain_run(sphgpu, 2, r["state"], r["gamma"], r["t_end"], hp, {}, sphgpu.ROW_ORDER_SEARCH)
+        assert o["steps"] == steps
+        got = {"position": o["pos"], "velocity": o["vel"], "mass density": o["rho"],
+               "specific thermal energy": o["eps"], "H": o["H"]}
+        check_same({k: np.asarray(v).reshape(np.asarray(ref[k]).shape) for k, v in got.items()}, ref, q)
+
+
+@pytest.mark.parametrize("field", ["pos", "H"])
+def test_plain_sim_rejects_nonfinite_nodes(sphgpu, field):
+    """Outside an ensemble a NaN position or H makes the neighbour build fail
+    with SPHGPU_ERR_ARG (min/max reductions would pass a NaN over), on the
+    single-grid and the multi-level path."""
+    for spread in (0.0, 0.6):
+        st, _ = problems.random_state(2, 12, seed=31, h_var=spread)
+        st = {k: np.array(v, dtype=np.float64) for k, v in st.items()}
+        st[field][4, 0] = np.nan
+        N = st["pos"].shape[0]
+        s = sphgpu.Sim(2)
+        s.set_table_bspline(1000)
+        s.set_nodes(N, N)
+        s.upload_state(st)
+        with pytest.raises(sphgpu.SphGpuError, match="non-finite"):
+            s.build_neighbors(2.0)
+        s.close()
+
+
+def test_mixed_3d_rejects_positions_beyond_the_fixed_point_range(sphgpu):
+    """The 3-D mixed record holds round(x / u) with u = search radius / 2^30:
+    a lattice 1e10 from the origin would overflow it and is refused; the FP64
+    pass takes it."""
+    st, _ = problems.converging_flow(6)
+    st = {k: np.array(v, dtype=np.float64) for k, v in st.items()}
+    st["pos"] = st["pos"] + 1.0e10
+    N = st["pos"].shape[0]
+    s = sphgpu.Sim(3)
+    s.set_row_order(sphgpu.ROW_ORDER_SEARCH)
+    s.set_table_bspline(1000)
+    s.set_nodes(N, N)
+    s.upload_state(st)
+    s.build_neighbors(2.0)
+    s.evaluate_derivatives(sphgpu.hydro_params())
+    with pytest.raises(sphgpu.SphGpuError, match="fixed-point"):
+        s.evaluate_derivatives(sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED))
+    s.close()
