@@ -2,7 +2,9 @@
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 module.  It loads oracle/_build/liboracle.so (parity build, no FMA contraction) or
-liboracle_fast.so (timed baseline build) produced by oracle/Makefile.
+liboracle_fast.so (timed baseline build) produced by oracle/Makefile; the
+environment variable SPHGPU_ORACLE_LIB names another parity build to load
+instead (the sanitized one).
 
 Array conventions follow Spheral's Field storage: one row per node, Vector =
 D doubles, SymTensor = D(D+1)/2 doubles (xx,xy,xz,yy,yz,zz), Tensor = D*D doubles
@@ -106,6 +108,9 @@ def lib(fast=False):
     name = "liboracle_fast.so" if fast else "liboracle.so"
     if name not in _libs:
         path = os.path.join(BUILD, name)
+        # SPHGPU_ORACLE_LIB: another build of the parity library (the sanitized one)
+        if not fast and os.environ.get("SPHGPU_ORACLE_LIB"):
+            path = os.environ["SPHGPU_ORACLE_LIB"]
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
