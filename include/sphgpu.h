@@ -293,6 +293,17 @@ int sphgpu_iterate_ideal_h(sphgpu_sim* sim, const sphgpu_hydro_params* hydro, in
 int sphgpu_conservation(sphgpu_sim* sim, double* out);
 /* The same totals over one ensemble member's rows. */
 int sphgpu_ensemble_conservation(sphgpu_sim* sim, int64_t id, double* out);
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops).  The totals of
+ * sphgpu_conservation over internal rows [first, first + rows) (all of them,
+ * or one ensemble member's), bitwise the same, written to out_device[9] in
+ * device memory asynchronously on the sim's stream (sphgpu_synchronize before
+ * reading).  Allocates nothing: the partial sums live in a static device array
+ * of 256 x 9 doubles per GPU, enough for any row count.  Every call on a GPU
+ * shares that array, so calls that overlap on one GPU (sims on other streams
+ * of the same device) race; a row range outside the internal rows is rejected
+ * with SPHGPU_ERR_ARG. */
+int sphgpu_conservation_noalloc(sphgpu_sim* sim, int64_t first, int64_t rows, double* out_device);
 /* The previous step's derivatives the dt vote reads (per internal row:
  * maxViscousPressure, tr DvDx, DvDt; 2 + dim doubles), rows [first, first +
  * rows): zero before the first step.  With the state and the clock they let a

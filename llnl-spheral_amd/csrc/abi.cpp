@@ -120,6 +120,13 @@ int sphgpu_create(int dim, int device, sphgpu_sim** out) {
   for (auto& e : s->ev) {
     if (hipEventCreate(&e) != hipSuccess) { sphgpu_destroy(s); return SPHGPU_ERR_HIP; }
   }
+  // the allocation-free entry point's scratch array on this device (once)
+  try {
+    resolveScratch(device);
+  } catch (const Error& e) {
+    sphgpu_destroy(s);
+    return e.code;
+  }
   void* pinned = nullptr;
   if (hipHostMalloc(&pinned, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
     sphgpu_destroy(s);
@@ -492,6 +499,15 @@ int sphgpu_ensemble_conservation(sphgpu_sim* s, int64_t id, double* out) {
     auto it = std::find_if(s->members.begin(), s->members.end(), [&](const Member& mb) { return mb.id == id; });
     SG_REQUIRE(it != s->members.end(), SPHGPU_ERR_ARG, "no member with that id");
     conservationTotals(*s, it->first, it->first + it->rows, out);
+  });
+}
+
+int sphgpu_conservation_noalloc(sphgpu_sim* s, int64_t first, int64_t rows, double* out_device) {
+  return guarded(s, [&] {
+    SG_REQUIRE(out_device, SPHGPU_ERR_ARG, "null output");
+    SG_REQUIRE(first >= 0 && rows > 0 && first + rows <= s->nInt, SPHGPU_ERR_ARG,
+               "row range outside the internal rows");
+    conservationTotalsNoAlloc(*s, first, first + rows, out_device);
   });
 }
 

@@ -29,6 +29,7 @@
 #include <cfloat>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 
 namespace sphgpu {
 
@@ -695,15 +696,22 @@ __global__ void k_conserve_final(const double* __restrict__ partial, int nb, dou
   out[q] = t;
 }
 
-// Totals of rows [r0, r1) (internal nodes of the sim, or of one ensemble
-// member): [mass, p (3), L (3), kinetic, thermal], as SpheralConservation
-// (src/SimulationControl/SpheralConservation.py:66-150) takes them.
-void conservationTotals(Sim& s, int64_t r0, int64_t r1, double* out) {
-  const int nb = int(std::min<int64_t>(256, std::max<int64_t>(1, (r1 - r0 + kBlock - 1) / kBlock)));
-  DBuf tmp;
-  tmp.ensure(sizeof(double) * size_t(nb + 1) * kCons);
-  double* partial = tmp.as<double>();
-  double* res = partial + size_t(nb) * kCons;
+// The totals as reported: the kinetic energy is half the summed m v^2.
+__global__ void k_conserve_report(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= kCons) return;
+  double t = 0.0;
+  for (int b = 0; b < nb; ++b) t += partial[b * kCons + q];
+  out[q] = q == 7 ? 0.5 * t : t;
+}
+
+// Workgroups of the partial sums (the same for every row count, so the
+// summation order depends only on the rows).
+constexpr int kConserveBlocks = 256;
+inline int conserveBlocks(int64_t rows) {
+  return int(std::min<int64_t>(kConserveBlocks, std::max<int64_t>(1, (rows + kBlock - 1) / kBlock)));
+}
+void conservePartials(Sim& s, int64_t r0, int64_t r1, int nb, double* partial) {
   if (s.dim == 1)
     hipLaunchKernelGGL(k_conserve_partial<1>, dim3(nb), dim3(kBlock), 0, s.stream, s.pos.as<double>(),
                        s.vel.as<double>(), s.mass.as<double>(), s.eps.as<double>(), r0, r1, partial);
@@ -713,6 +721,18 @@ void conservationTotals(Sim& s, int64_t r0, int64_t r1, double* out) {
   else
     hipLaunchKernelGGL(k_conserve_partial<3>, dim3(nb), dim3(kBlock), 0, s.stream, s.pos.as<double>(),
                        s.vel.as<double>(), s.mass.as<double>(), s.eps.as<double>(), r0, r1, partial);
+}
+
+// Totals of rows [r0, r1) (internal nodes of the sim, or of one ensemble
+// member): [mass, p (3), L (3), kinetic, thermal], as SpheralConservation
+// (src/SimulationControl/SpheralConservation.py:66-150) takes them.
+void conservationTotals(Sim& s, int64_t r0, int64_t r1, double* out) {
+  const int nb = conserveBlocks(r1 - r0);
+  DBuf tmp;
+  tmp.ensure(sizeof(double) * size_t(nb + 1) * kCons);
+  double* partial = tmp.as<double>();
+  double* res = partial + size_t(nb) * kCons;
+  conservePartials(s, r0, r1, nb, partial);
   hipLaunchKernelGGL(k_conserve_final, dim3(1), dim3(64), 0, s.stream, partial, nb, res);
   SG_CHECK_HIP(hipGetLastError());
   double h[kCons];
@@ -720,6 +740,47 @@ void conservationTotals(Sim& s, int64_t r0, int64_t r1, double* out) {
   SG_CHECK_HIP(hipStreamSynchronize(s.stream));
   for (int q = 0; q < kCons; ++q) out[q] = h[q];
   out[7] = 0.5 * h[7];
+}
+
+// ---- the allocation-free totals ---------------------------------------------
+// Partial sums of sphgpu_conservation_noalloc live in a file-scope device
+// array instead of a per-call allocation: kConserveBlocks x kCons doubles
+// (18 KB), held by every GPU the library runs on until the process ends.
+// Each device has its own copy at its own address, resolved once per device
+// with hipGetSymbolAddress (host code never passes the array's name itself).
+// The partial kernel writes every slot it is read back from, so nothing needs
+// clearing; calls that overlap on one GPU share the array and race.
+constexpr size_t kScratchDoubles = size_t(kConserveBlocks) * kCons;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+namespace {
+constexpr int kMaxDevices = 64;
+void* gScratch[kMaxDevices] = {};   // the array's address on each device
+std::mutex gScratchMu;
+}  // namespace
+
+void resolveScratch(int device) {
+  SG_REQUIRE(device >= 0 && device < kMaxDevices, SPHGPU_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(gScratchMu);
+  if (gScratch[device]) return;
+  void* p = nullptr;
+  SG_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));   // the device is current
+  gScratch[device] = p;
+}
+
+void conservationTotalsNoAlloc(Sim& s, int64_t r0, int64_t r1, double* outDevice) {
+  void* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(gScratchMu);
+    p = (s.device >= 0 && s.device < kMaxDevices) ? gScratch[s.device] : nullptr;
+  }
+  SG_REQUIRE(p, SPHGPU_ERR_STATE, "the device's scratch array was not resolved");
+  const int nb = conserveBlocks(r1 - r0);
+  SG_REQUIRE(size_t(nb) * kCons <= kScratchDoubles, SPHGPU_ERR_CAPACITY,
+             "conservation partial sums exceed the scratch array");
+  double* partial = static_cast<double*>(p);
+  conservePartials(s, r0, r1, nb, partial);
+  hipLaunchKernelGGL(k_conserve_report, dim3(1), dim3(64), 0, s.stream, partial, nb, outDevice);
+  SG_CHECK_HIP(hipGetLastError());
 }
 
 // iterateIdealH (src/Utilities/iterateIdealH.cc:30-310), SPH smoothing scale,

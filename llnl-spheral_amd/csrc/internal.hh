@@ -284,6 +284,11 @@ void omegaGradh(Sim& s);          // computeSPHOmegaGradhCorrection into the ome
 // one iterateIdealH pass over the current rows; returns the largest change
 double idealHPass(Sim& s, const sphgpu_hydro_params& hp, double tol, double* deltaH, unsigned long long* maxBits);
 void conservationTotals(Sim& s, int64_t r0, int64_t r1, double* out);   // integrate.hip, 9 values
+// The same totals into device memory, asynchronously, with the partial sums in
+// the library's static scratch array (no allocation); resolveScratch(device)
+// must have run once on the device (sphgpu_create does it).
+void conservationTotalsNoAlloc(Sim& s, int64_t r0, int64_t r1, double* outDevice);
+void resolveScratch(int device);
 // integrate.hip: iterateIdealH; returns the iterations done, *maxDelta the last largest change
 int iterateIdealH(Sim& s, const sphgpu_hydro_params& hp, int maxIterations, double tol, double kernelExtent,
                   bool cull, double* maxDelta);

@@ -216,6 +216,7 @@ def lib():
             "sphgpu_get_dt_inputs": (C.c_int, [vp, C.c_int64, C.c_int64, _dp]),
             "sphgpu_set_dt_inputs": (C.c_int, [vp, C.c_int64, C.c_int64, _dp]),
             "sphgpu_ensemble_conservation": (C.c_int, [vp, C.c_int64, _dp]),
+            "sphgpu_conservation_noalloc": (C.c_int, [vp, C.c_int64, C.c_int64, C.c_void_p]),
             "sphgpu_ensemble_begin": (C.c_int, [vp]),
             "sphgpu_ensemble_add": (C.c_int, [vp, C.POINTER(Member), C.POINTER(IntegratorParams), _i64p]),
             "sphgpu_ensemble_step": (C.c_int, [vp, C.POINTER(HydroParams), C.POINTER(IntegratorParams),
@@ -486,6 +487,12 @@ class Sim:
 
     def synchronize(self):
         self._chk(self.L.sphgpu_synchronize(self.h))
+
+    def conservation_noalloc(self, first, rows, out_device_ptr):
+        """sphgpu_conservation_noalloc: the totals of rows [first, first + rows)
+        into 9 doubles of device memory at out_device_ptr (an integer address),
+        asynchronously on the sim's stream."""
+        self._chk(self.L.sphgpu_conservation_noalloc(self.h, int(first), int(rows), C.c_void_p(int(out_device_ptr))))
 
     # ensembles -------------------------------------------------------------------
     def ensemble_begin(self):
