@@ -35,6 +35,17 @@ import numpy as np  # noqa: E402
 METRIC = "SPH pair-interactions/s (neighbors+derivs) at 128³ 3D; ensemble p50/p99 solve time"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+# Floating-point operations of the collapsed isotropic pair body per directed
+# neighbour entry (the FP64 kernel in search row order, and the mixed kernel),
+# counted term by term from its source (csrc/hydro.hip, k_derivs_iso): a
+# reciprocal and a reciprocal square root count as one operation each, an
+# FMA as two; compares, selects and conversions count none.  The FP64 kernel
+# also executes two Newton steps per reciprocal / reciprocal square root
+# (36 FLOPs more per entry).  Mixed: the pair terms and their block sums in
+# FP32, and each block of 16 entries adds its 34 partial sums into the FP64
+# per-node accumulators (34 / 16 FP64 additions per entry).
+PAIR_FLOPS = {"fp64": {"fp64": 174}, "mixed": {"fp32": 174, "fp64": 34 / 16}}
 IN_BYTES_PER_NODE = 136      # x(3) v(3) m rho P c omega H(6), FP64
 OUT_BYTES_PER_NODE = 576     # the 72 doubles of the derivative fields
 NBR_BYTES_PER_ENTRY = 4      # one int32 neighbour index per directed entry
@@ -62,6 +73,65 @@ def measured_counters(n, precision, row_order, boundaries):
         if t.get("key") == key:
             return t, os.path.join("profiles", name)
     return None, None
+
+
+def pair_roofline(precision, row_order, n, boundaries, entries, n_int, n_tot, kernel_ms):
+    """Roofline of the pair kernel (evaluateDerivatives pair loop + epilogue)
+    for one pass: the FLOP roofline from the exact per-entry count (the
+    binding one: the larger fraction), the HBM roofline from the algorithmic
+    bytes, the VALU issue fraction and the L2 hit rate from the committed
+    counters of this workload."""
+    alg_bytes = n_int * OUT_BYTES_PER_NODE + n_tot * IN_BYTES_PER_NODE + NBR_BYTES_PER_ENTRY * entries
+    t = kernel_ms * 1e-3
+    counters, src = measured_counters(n, precision, row_order, boundaries)
+    if counters and counters.get("kernel"):
+        kernel = counters["kernel"]
+    elif precision == "mixed":
+        kernel = "k_derivs_iso<float, 3, true, false>"
+    else:
+        kernel = ("k_derivs_iso<double, 3, true, true>" if row_order == "search"
+                  else "k_derivs_fp64<3, true, false, true>")
+    hbm = {"achieved": alg_bytes / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": alg_bytes / t / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": alg_bytes,
+           "traffic": counters["traffic_bytes"] if counters else None}
+    out = {"kernel": f"{kernel} (evaluateDerivatives pair loop + epilogue)", "traffic_source": src}
+    fl = PAIR_FLOPS.get(precision) if row_order == "search" else None
+    if fl:
+        rates = {k: v * entries / t / 1e12 for k, v in fl.items()}
+        peaks = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS}
+        frac = sum(rates[k] / peaks[k] for k in rates)
+        total = sum(rates.values())
+        out.update({"bound": "valu", "achieved": total, "peak": total / frac if frac else None, "unit": "TFLOP/s",
+                    "frac": frac, "traffic": hbm["traffic"],
+                    "flops_per_entry": fl, "flops_per_launch": {k: v * entries for k, v in fl.items()},
+                    "flop_rate_tflops": rates,
+                    "flop_peaks_tflops": {k: peaks[k] for k in rates},
+                    "note": "FLOP roofline (vector ALU; no MFMA on a sparse gather): exact FLOPs per directed "
+                            "entry x entries / the kernel's event time, against the published FP64 / FP32 peaks; "
+                            "peak is their FLOP-weighted combination"})
+        out["hbm"] = hbm
+    else:
+        out.update({"bound": "hbm", "achieved": hbm["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm["frac"], "traffic": hbm["traffic"], "algorithmic_bytes": alg_bytes})
+    if counters and counters.get("valu_wave_instructions"):
+        issue_ms = 1e3 * counters["valu_wave_instructions"] / VALU_ISSUE_HZ
+        out["issue"] = {"valu_wave_instructions": counters["valu_wave_instructions"], "issue_ms": issue_ms,
+                        "kernel_ms": kernel_ms, "busy_frac": issue_ms / kernel_ms,
+                        "note": "VALU wavefront instructions (SQ_INSTS_VALU) at one per 4 cycles per SIMD, "
+                                "1024 SIMDs at 2.4 GHz, over the kernel time"}
+    if counters and counters.get("tcc_hit") is not None:
+        h, m = counters["tcc_hit"], counters["tcc_miss"]
+        out["l2_hit_rate"] = h / (h + m) if h + m else None
+    loads = GATHER_LOADS.get((precision, row_order)) if n >= 64 else None
+    if loads:
+        acc = entries * loads
+        rate = acc / t
+        out["gather"] = {
+            "loads_per_neighbour": loads, "line_accesses": acc, "achieved_per_s": rate,
+            "model_peak_per_s": L1_LINE_RATE, "frac": rate / L1_LINE_RATE,
+            "note": "model, not a published peak: one L1 line access per lane per 16-B load of a neighbour "
+                    "record, one access per CU per cycle (256 CUs at 2.4 GHz)"}
+    return out
 
 
 def cpu_info():
@@ -360,6 +430,8 @@ def main():
                          "every step (the benchmark configuration); none: free surface")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-pass", type=int, default=1,
+                    help="1: also time the same pass in mixed precision (reported under mixed_pass)")
     ap.add_argument("--rk2-steps", type=int, default=5,
                     help="SynchronousRK2 steps timed after the pass measurement (reported under rk2; 0 = off)")
     ap.add_argument("--probe-mib", type=int, default=256,
@@ -371,11 +443,11 @@ def main():
     ap.add_argument("--serve", type=int, default=32,
                     help="serving loop with this many blast requests (0 = off)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-check", type=int, default=3, help="requests re-solved alone in FP64 for the error")
+    ap.add_argument("--serve-check", type=int, default=8, help="requests re-solved alone in FP64 for the error")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--only-batch", action="store_true", help="skip the 128^3 pass measurement")
-    ap.add_argument("--periodic", type=int, default=0,
-                    help="also run the periodic mixed-vs-FP64 comparison at this lattice size (256 = the benchmark)")
+    ap.add_argument("--periodic", type=int, default=256,
+                    help="periodic mixed-vs-FP64 comparison at this lattice size (256 = the configuration; 0 = off)")
     ap.add_argument("--periodic-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -415,43 +487,39 @@ def main():
     sim.set_precision(prec)
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s N={N}", file=sys.stderr, flush=True)
 
-    def step():
-        if ghosts:
-            sim.set_ghost_nodes(2.0, cull=True)   # ghosts + neighbour build + culling
-        else:
-            sim.build_neighbors(2.0)
-        sim.sum_density()
-        sim.evaluate_derivatives(params)
+    def measure(params_now):
+        def step():
+            if ghosts:
+                sim.set_ghost_nodes(2.0, cull=True)   # ghosts + neighbour build + culling
+            else:
+                sim.build_neighbors(2.0)
+            sim.sum_density()
+            sim.evaluate_derivatives(params_now)
 
-    for _ in range(args.warmup):
-        step()
-    sim.synchronize()
-    stage = np.zeros(6)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        t = sim.timings()
-        stage += [t["neighbors_ms"], t["density_ms"], t["derivs_ms"], t["pack_ms"],
-                  t["ghosts_ms"] if ghosts else 0.0, t["cull_ms"] if ghosts else 0.0]
-    sim.synchronize()
-    wall = time.perf_counter() - t0
-    stage /= args.steps
+        for _ in range(args.warmup):
+            step()
+        sim.synchronize()
+        stage = np.zeros(6)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            t = sim.timings()
+            stage += [t["neighbors_ms"], t["density_ms"], t["derivs_ms"], t["pack_ms"],
+                      t["ghosts_ms"] if ghosts else 0.0, t["cull_ms"] if ghosts else 0.0]
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        stage /= args.steps
+        return wall, stage
+
+    def stage_dict(stage):
+        return {"ghosts": stage[4], "neighbors": stage[0], "cull": stage[5], "sum_density": stage[1],
+                "derivatives": stage[2], "pack": stage[3]}
+
+    wall, stage = measure(params)
     pairs = sim.pair_count()
     entries = sim.neighbor_count()
     ms_step = 1e3 * wall / args.steps
-
-    derivs_ms = stage[2]
     n_tot = sim.node_counts()[1]
-    alg_bytes = N * OUT_BYTES_PER_NODE + n_tot * IN_BYTES_PER_NODE + NBR_BYTES_PER_ENTRY * entries
-    achieved = alg_bytes / (derivs_ms * 1e-3) / 1e9
-    counters, counters_src = measured_counters(args.n, args.precision, args.row_order, args.boundaries)
-    traffic = counters["traffic_bytes"] if counters else None
-    if counters and counters.get("kernel"):
-        kernel = counters["kernel"]
-    elif args.precision == "mixed":
-        kernel = "k_derivs_iso<float, 3, true>"
-    else:
-        kernel = "k_derivs_iso<double, 3, true>" if args.row_order == "search" else "k_derivs_fp64<3, true, false, true>"
     line = {
         "metric": METRIC,
         "value": pairs * args.steps / wall,
@@ -474,32 +542,25 @@ def main():
             "options": "XSPH, correctVelocityGradient, no grad-h, no compatible energy",
             "precision": args.precision,
             "row_order": args.row_order,
-            "stage_ms": {"ghosts": stage[4], "neighbors": stage[0], "cull": stage[5], "sum_density": stage[1],
-                         "derivatives": stage[2], "pack": stage[3]},
+            "stage_ms": stage_dict(stage),
         },
-        "roofline": {
-            "kernel": f"{kernel} (evaluateDerivatives pair loop + epilogue)",
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "traffic_source": counters_src,
-            "algorithmic_bytes": alg_bytes,
-            "note": "the pair kernel is bound by its neighbour gathers (L1 line accesses: see gather) and "
-                    "by VALU issue (see valu), not by HBM bandwidth",
-        },
+        "roofline": pair_roofline(args.precision, args.row_order, args.n, args.boundaries, entries, N, n_tot,
+                                  stage[2]),
     }
-    if counters and counters.get("valu_wave_instructions"):
-        issue_ms = 1e3 * counters["valu_wave_instructions"] / VALU_ISSUE_HZ
-        line["roofline"]["valu"] = {"wave_instructions": counters["valu_wave_instructions"], "issue_ms": issue_ms,
-                                    "kernel_ms": derivs_ms, "busy_frac": issue_ms / derivs_ms}
-    loads = GATHER_LOADS.get((args.precision, args.row_order)) if args.n >= 64 else None
-    if loads:
-        acc = entries * loads
-        rate = acc / (derivs_ms * 1e-3)
-        line["roofline"]["gather"] = {
-            "loads_per_neighbour": loads, "line_accesses": acc, "achieved_per_s": rate,
-            "model_peak_per_s": L1_LINE_RATE, "frac": rate / L1_LINE_RATE,
-            "note": "model: one L1 line access per lane per 16-B load of a neighbour record, "
-                    "one access per CU per cycle (256 CUs at 2.4 GHz)"}
+    if args.precision == "fp64" and args.mixed_pass and args.row_order == "search":
+        # the same pass on the same state in mixed precision (FP32 pair
+        # arithmetic, FP64 positions and sums)
+        sim.set_precision(sphgpu.PRECISION_MIXED)
+        wall_m, stage_m = measure(sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED))
+        line["mixed_pass"] = {
+            "value": sim.pair_count() * args.steps / wall_m, "unit": "pair-interactions/s",
+            "ms_per_step": 1e3 * wall_m / args.steps, "steps": args.steps, "warmup": args.warmup,
+            "speedup_vs_fp64": wall / wall_m, "dtype": "f32+f64", "stage_ms": stage_dict(stage_m),
+            "roofline": pair_roofline("mixed", "search", args.n, args.boundaries, sim.neighbor_count(), N,
+                                      sim.node_counts()[1], stage_m[2]),
+        }
+        sim.set_precision(prec)
+        sim.sum_density()
     if args.rk2_steps > 0:
         # full time steps on the same resident state: ghosts + neighbours, dt
         # vote, two derivative passes, updates, sum density, enforcement
@@ -536,6 +597,12 @@ def main():
                                           args.serve_check)
         except Exception as e:
             line["serving"] = {"error": str(e)}
+    if args.periodic:
+        sim.close()       # the 256^3 periodic box needs the memory
+        try:
+            line["periodic"] = periodic_bench(sphgpu, problems, args.periodic, args.periodic_steps)
+        except Exception as e:
+            line["periodic"] = {"error": str(e)}
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args.cpu_budget, boundaries=args.boundaries)

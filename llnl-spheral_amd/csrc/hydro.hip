@@ -844,7 +844,47 @@ template <> struct IsoTab<double> {
   static constexpr int PER = 6;
 };
 
-template <typename Real, int D, bool LDS>
+// Mixed precision: the pair terms of up to kSumBlock consecutive neighbours
+// are summed in FP32 (fused into the term's last product where it has one),
+// and each block's partial sums are added once to the FP64 per-node
+// accumulators: one conversion and one FP64 add per block instead of per
+// entry.  A block's FP32 sum of 16 terms is within 16 u32 (1e-6) of its
+// absolute sum; the per-node sums stay FP64.
+constexpr int kSumBlock = 16;
+template <int D> struct SumsF {
+  float DvDt[D], xsphDV[D], DvDx[D * D], M[D * (D + 1) / 2], m2nd[D * (D + 1) / 2];
+  float DepsDt, viscW, effVP, rhoSum, norm, wNS, xsphW;
+  __device__ __forceinline__ void zero() {
+    for (int c = 0; c < D; ++c) DvDt[c] = xsphDV[c] = 0.0f;
+    for (int c = 0; c < D * D; ++c) DvDx[c] = 0.0f;
+    for (int c = 0; c < D * (D + 1) / 2; ++c) M[c] = m2nd[c] = 0.0f;
+    DepsDt = viscW = effVP = rhoSum = norm = wNS = xsphW = 0.0f;
+  }
+  // add into the FP64 accumulators (M and m2nd as upper triangles) and restart
+  __device__ __forceinline__ void flushInto(Acc<D>& a) {
+    for (int c = 0; c < D; ++c) {
+      a.DvDt.c[c] += double(DvDt[c]);
+      a.xsphDV.c[c] += double(xsphDV[c]);
+    }
+    for (int c = 0; c < D * D; ++c) a.DvDx.c[c] += double(DvDx[c]);
+    int s2 = 0;
+    for (int c = 0; c < D; ++c)
+      for (int b = c; b < D; ++b, ++s2) {
+        a.M.c[c * D + b] += double(M[s2]);
+        a.m2nd.c[s2] += double(m2nd[s2]);
+      }
+    a.DepsDt += double(DepsDt);
+    a.viscW += double(viscW);
+    a.effVP += double(effVP);
+    a.rhoSum += double(rhoSum);
+    a.norm += double(norm);
+    a.wNS += double(wNS);
+    a.xsphW += double(xsphW);
+    zero();
+  }
+};
+
+template <typename Real, int D, bool LDS, bool PREFETCH>
 __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict__ rec,
                                                        const typename IsoTab<Real>::Rec* __restrict__ recN,
                                                        const int32_t* __restrict__ rowCount,
@@ -888,6 +928,9 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
 
     Acc<D> a;
     a.zero();
+    constexpr bool F32 = std::is_same<Real, float>::value;
+    SumsF<F32 ? D : 1> af;
+    if constexpr (F32) af.zero();
     Real maxVP = Real(0);
     const int len = rowCount[k];
     const int4* nb4 = reinterpret_cast<const int4*>(nbr);
@@ -925,20 +968,45 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
       const Real pj = fmaR(gj, nb.prho, Real(0.5) * QPj * gj);
       const Real mj = nb.m;
       const Real fsr = mj * (pi + pj) * rinv;        // acceleration along r_ij
-      for (int c = 0; c < D; ++c) a.DvDt.c[c] -= double(rij[c] * fsr);
-      a.DepsDt += double(mj * vdr * pi);
-      a.viscW += double(mj * vdr * halfQi);
-      a.effVP += double(nb.mor * Qi * Wi);
-      a.rhoSum += double(mj * Wi);
-      a.norm += double(mori * Wi);
-      a.wNS += double(absR(gWi));
       const Real wg = mj * gi * rinv;               // grad W_i * m_j = r_ij * wg
-      for (int c = 0; c < D; ++c) {
-        const Real u = vij[c] * wg;
-        for (int b = 0; b < D; ++b) a.DvDx.c[c * D + b] -= double(u * rij[b]);
-      }
       const Real f2 = gi * gi * rcpR(maxR(Real(1.0e-30), r2 * r2 * r2));
-      {
+      if constexpr (F32) {
+        for (int c = 0; c < D; ++c) af.DvDt[c] = fmaR(-rij[c], fsr, af.DvDt[c]);
+        af.DepsDt = fmaR(mj * vdr, pi, af.DepsDt);
+        af.viscW = fmaR(mj * vdr, halfQi, af.viscW);
+        af.effVP = fmaR(nb.mor * Qi, Wi, af.effVP);
+        af.rhoSum = fmaR(mj, Wi, af.rhoSum);
+        af.norm = fmaR(mori, Wi, af.norm);
+        af.wNS += absR(gWi);
+        for (int c = 0; c < D; ++c) {
+          const Real u = vij[c] * wg;
+          for (int b = 0; b < D; ++b) af.DvDx[c * D + b] = fmaR(-u, rij[b], af.DvDx[c * D + b]);
+        }
+        int s2 = 0;
+        for (int c = 0; c < D; ++c) {
+          const Real rw = rij[c] * wg, rf = rij[c] * f2;
+          for (int b = c; b < D; ++b, ++s2) {
+            af.M[s2] = fmaR(-rw, rij[b], af.M[s2]);
+            af.m2nd[s2] = fmaR(rf, rij[b], af.m2nd[s2]);
+          }
+        }
+        if (p.XSPH) {
+          const Real w2 = Real(0.5) * fmaR(mori, Wi, nb.mor * Wj);
+          af.xsphW += w2;
+          for (int c = 0; c < D; ++c) af.xsphDV[c] = fmaR(-vij[c], w2, af.xsphDV[c]);
+        }
+      } else {
+        for (int c = 0; c < D; ++c) a.DvDt.c[c] -= double(rij[c] * fsr);
+        a.DepsDt += double(mj * vdr * pi);
+        a.viscW += double(mj * vdr * halfQi);
+        a.effVP += double(nb.mor * Qi * Wi);
+        a.rhoSum += double(mj * Wi);
+        a.norm += double(mori * Wi);
+        a.wNS += double(absR(gWi));
+        for (int c = 0; c < D; ++c) {
+          const Real u = vij[c] * wg;
+          for (int b = 0; b < D; ++b) a.DvDx.c[c * D + b] -= double(u * rij[b]);
+        }
         int s2 = 0;
         for (int c = 0; c < D; ++c) {
           const Real rw = rij[c] * wg, rf = rij[c] * f2;
@@ -947,25 +1015,38 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
             a.m2nd.c[s2] += double(rf * rij[b]);
           }
         }
-      }
-      if (p.XSPH) {
-        const Real w2 = Real(0.5) * fmaR(mori, Wi, nb.mor * Wj);
-        a.xsphW += double(w2);
-        for (int c = 0; c < D; ++c) a.xsphDV.c[c] -= double(vij[c] * w2);
+        if (p.XSPH) {
+          const Real w2 = Real(0.5) * fmaR(mori, Wi, nb.mor * Wj);
+          a.xsphW += double(w2);
+          for (int c = 0; c < D; ++c) a.xsphDV.c[c] -= double(vij[c] * w2);
+        }
       }
     };
     INbr<Real> nx;
     int32_t jn = cur.x;
-    if (len > 0) loadIso<D>(recN, n, jn, nx);
+    if (PREFETCH && len > 0) loadIso<D>(recN, n, jn, nx);
     for (int w = 0; w < len; ++w) {
-      const INbr<Real> nb = nx;
-      const int32_t j = jn;
-      if (w + 1 < len) {
-        jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
-        loadIso<D>(recN, n, jn, nx);
+      INbr<Real> nb;
+      int32_t j;
+      if constexpr (PREFETCH) {
+        // the next record is loaded while this pair is evaluated
+        nb = nx;
+        j = jn;
+        if (w + 1 < len) {
+          jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
+          loadIso<D>(recN, n, jn, nx);
+        }
+      } else {
+        // one record in flight per lane (fewer registers, more waves)
+        j = w == 0 ? cur.x : nextIndex(nb4, n, k, len, w, cur, nxt);
+        loadIso<D>(recN, n, j, nb);
       }
       pair(nb, j);
+      if constexpr (F32) {
+        if ((w & (kSumBlock - 1)) == kSumBlock - 1) af.flushInto(a);
+      }
     }
+    if constexpr (F32) af.flushInto(a);
     for (int c = 0; c < D; ++c)
       for (int b = 0; b < c; ++b) a.M.c[c * D + b] = a.M.c[b * D + c];
     a.maxVP = double(maxVP);
@@ -1231,6 +1312,10 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
     s.pairDv.ensure(sizeof(double) * size_t(D) * size_t(std::max<int64_t>(o.plane, 1)));
     o.pairDv = s.pairDv.as<double>();
   }
+  // Mixed: one record in flight per lane keeps the kernel at 167 VGPRs, three
+  // wavefronts per SIMD (2.11 ms at 128^3 against 2.53 ms with the next
+  // record prefetched at 179 VGPRs, two wavefronts).  FP64: the prefetch wins
+  // (3.44 ms at 194 VGPRs against 3.51 ms at 168).
   if (mixed) {
     const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
     auto go = [&](auto kern) {
@@ -1238,14 +1323,14 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
                          s.recM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
                          s.work.as<int32_t>(), t, tab32(s), p, o, dncalc);
     };
-    if (lds) go(k_derivs_iso<float, D, true>); else go(k_derivs_iso<float, D, false>);
+    if (lds) go(k_derivs_iso<float, D, true, false>); else go(k_derivs_iso<float, D, false, false>);
   } else if (fast64) {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
                          s.recD.as<double2>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
                          s.work.as<int32_t>(), t, tab64(s), p, o, dncalc);
     };
-    if (lds) go(k_derivs_iso<double, D, true>); else go(k_derivs_iso<double, D, false>);
+    if (lds) go(k_derivs_iso<double, D, true, true>); else go(k_derivs_iso<double, D, false, true>);
   } else {
     auto launch = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
