@@ -190,12 +190,13 @@ def cpu_baseline(budget_s=10.0, n=48, boundaries="reflecting"):
     }
 
 
-def plain_solve(sphgpu, dim, st, gamma, t_end, hp, prec, order, max_steps=None):
+def plain_solve(sphgpu, dim, st, gamma, t_end, hp, prec, order, max_steps=None, graphs=False):
     """One problem alone as a plain simulation (sphgpu_rk2_step loop) to its
     end time, or for max_steps steps; returns (wall seconds, steps, final
     fields)."""
     N = st["mass"].shape[0]
     s = sphgpu.Sim(dim)
+    s.set_graphs(graphs)
     s.set_row_order(order)
     s.set_table_bspline(1000)
     s.set_precision(prec)
@@ -220,7 +221,7 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else None
 
 
-def ensemble_bench(sphgpu, problems, count, precision, sample):
+def ensemble_bench(sphgpu, problems, count, precision, sample, graphs=False):
     """Batched shock tubes (count tubes, 200-20000 nodes, to t = 0.2): every
     member advanced by one launch sequence per step; time-to-solution of each
     tube from the common start.  The one-at-a-time baseline solves `sample`
@@ -237,6 +238,7 @@ def ensemble_bench(sphgpu, problems, count, precision, sample):
     e.set_precision(prec)
     e.set_boundaries(sphgpu.box_boundaries(1))
     e.ensemble_begin()
+    e.set_graphs(graphs)
     print(f"[bench] ensemble of {count} tubes, {sum(m['n'] for m in members)} nodes", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     ids = {}
@@ -263,6 +265,7 @@ def ensemble_bench(sphgpu, problems, count, precision, sample):
             last_report = time.perf_counter()
             print(f"[bench] ensemble step {steps}: {len(ids)} tubes left", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    graph_counts = e.graph_counts()
     e.close()
     tts = [finish[q] for q in range(count)]
     # one-at-a-time plain solves of a sample spread over the sizes
@@ -271,7 +274,8 @@ def ensemble_bench(sphgpu, problems, count, precision, sample):
     samp = []
     for q in pick:
         m = members[q]
-        w, s, _ = plain_solve(sphgpu, 1, m["state"], m["gamma"], m["t_end"], hp, prec, order, max_steps=300)
+        w, s, _ = plain_solve(sphgpu, 1, m["state"], m["gamma"], m["t_end"], hp, prec, order, max_steps=300,
+                              graphs=graphs)
         samp.append((m["n"], w / s, s))
     est = None
     if samp:
@@ -282,7 +286,8 @@ def ensemble_bench(sphgpu, problems, count, precision, sample):
     return {
         "workload": f"{count} planar shock tubes, n log-uniform in [200, 20000] (total {sum(m['n'] for m in members)}), "
                     "rho_L,P_L~U[0.5,2], rho_R,P_R~U[0.05,0.5], gamma in {1.4,5/3}, reflecting walls, t_end 0.2",
-        "precision": precision,
+        "precision": precision, "hip_graphs": {"on": bool(graphs), "captures": graph_counts[0],
+                                               "replays": graph_counts[1]},
         "wall_s": wall, "steps": steps, "node_steps": node_steps, "node_steps_per_s": node_steps / wall,
         "time_to_solution_s": {"p50": pct(tts, 50), "p99": pct(tts, 99), "max": max(tts)},
         "member_steps": {"p50": pct(list(cycles.values()), 50), "max": max(cycles.values())},
@@ -294,7 +299,7 @@ def ensemble_bench(sphgpu, problems, count, precision, sample):
     }
 
 
-def serve_bench(sphgpu, problems, count, rate, precision, check):
+def serve_bench(sphgpu, problems, count, rate, precision, check, graphs=False):
     """Poisson arrivals of 2-D point blasts through the request loop
     (continuous batching); time-to-solution = finish - arrival."""
     prec = sphgpu.PRECISION_FP64 if precision == "fp64" else sphgpu.PRECISION_MIXED
@@ -312,6 +317,7 @@ def serve_bench(sphgpu, problems, count, rate, precision, check):
         e.set_precision(prec)
         e.set_boundaries(sphgpu.box_boundaries(2))
         e.ensemble_begin()
+        e.set_graphs(graphs)
         t0 = time.perf_counter()
         out = e.serve(reqs, hp, ip, max_members=max_members)
         wall = time.perf_counter() - t0
@@ -327,7 +333,7 @@ def serve_bench(sphgpu, problems, count, rate, precision, check):
     _, single = run(1)
     res = {"workload": f"{count} 2-D point blasts (E=1, rho=1, P=1e-6, gamma 5/3, n log-uniform in [1k, 250k], "
                        f"t_end~U[0.05,0.2], reflecting box), Poisson arrivals at {rate}/s",
-           "precision": precision, "batched": batched, "one_at_a_time": single}
+           "precision": precision, "hip_graphs": bool(graphs), "batched": batched, "one_at_a_time": single}
     if check > 0 and prec == sphgpu.PRECISION_MIXED:
         # error of the mixed-precision results vs FP64 solves alone, smallest requests
         errs = {}
@@ -445,6 +451,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-check", type=int, default=8, help="requests re-solved alone in FP64 for the error")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="1: ensemble, serving and their lone baselines replay the device part of each step from a "
+                         "captured HIP graph")
     ap.add_argument("--only-batch", action="store_true", help="skip the 128^3 pass measurement")
     ap.add_argument("--periodic", type=int, default=256,
                     help="periodic mixed-vs-FP64 comparison at this lattice size (256 = the configuration; 0 = off)")
@@ -464,10 +473,10 @@ def main():
             line["periodic"] = periodic_bench(sphgpu, problems, args.periodic, args.periodic_steps)
         if args.ensemble:
             line["ensemble"] = ensemble_bench(sphgpu, problems, args.ensemble, args.batch_precision,
-                                              args.ensemble_sample)
+                                              args.ensemble_sample, graphs=args.graphs)
         if args.serve:
             line["serving"] = serve_bench(sphgpu, problems, args.serve, args.serve_rate, args.batch_precision,
-                                          args.serve_check)
+                                          args.serve_check, graphs=args.graphs)
         print(json.dumps(line), flush=True)
         return
 
@@ -588,13 +597,13 @@ def main():
     if args.ensemble:
         try:
             line["ensemble"] = ensemble_bench(sphgpu, problems, args.ensemble, args.batch_precision,
-                                              args.ensemble_sample)
+                                              args.ensemble_sample, graphs=args.graphs)
         except Exception as e:
             line["ensemble"] = {"error": str(e)}
     if args.serve:
         try:
             line["serving"] = serve_bench(sphgpu, problems, args.serve, args.serve_rate, args.batch_precision,
-                                          args.serve_check)
+                                          args.serve_check, graphs=args.graphs)
         except Exception as e:
             line["serving"] = {"error": str(e)}
     if args.periodic:

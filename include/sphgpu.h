@@ -315,6 +315,18 @@ int sphgpu_set_dt_inputs(sphgpu_sim* sim, int64_t first, int64_t rows, const dou
 int sphgpu_get_time(sphgpu_sim* sim, double* time, double* last_dt, int64_t* cycle);
 int sphgpu_set_time(sphgpu_sim* sim, double time, double last_dt, int64_t cycle);
 
+/* Run the device part of each time step (sphgpu_rk2_step, sphgpu_ensemble_step:
+ * dt vote and selection, both derivative passes, updates, ghost refreshes,
+ * the finalize density sum, boundary enforcement, clock advance) as a HIP
+ * graph: the first step with a new launch key runs eagerly, the second is
+ * captured, later ones replay it (the key: node and slot counts, device
+ * buffers, parameters, record state).  Results are bitwise those of the eager
+ * step.  Stage timings are not updated by graph steps.  No counterpart in the
+ * reference; a launch-overhead measure for small node sets (default off). */
+int sphgpu_set_graphs(sphgpu_sim* sim, int on);
+/* Steps captured as graphs and steps replayed from one, since creation. */
+int sphgpu_graph_counts(sphgpu_sim* sim, int64_t* captures, int64_t* replays);
+
 /* Device times in milliseconds of the last call of each stage, measured with
  * HIP events on the sim's stream: [neighbours, sum density, derivatives, pack,
  * ghost generation, ghost culling]. */

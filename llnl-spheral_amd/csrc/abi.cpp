@@ -62,6 +62,7 @@ DBuf& stateBuf(Sim& s, int field) {
 
 void installTable(Sim& s) {
   const int n = s.table.n;
+  ++s.tableVersion;
   std::vector<double> inter(size_t(6) * n);
   for (int i = 0; i < n; ++i) {
     inter[6 * i + 0] = s.table.Ak[i]; inter[6 * i + 1] = s.table.Bk[i]; inter[6 * i + 2] = s.table.Ck[i];
@@ -148,6 +149,7 @@ void sphgpu_destroy(sphgpu_sim* s) {
                   &s->scanTmp, &s->reduceTmp, &s->flags, &s->rec, &s->out};
   for (DBuf* b : bufs) b->release();
   for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
+  for (auto& g : s->stepGraphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (s->ncalcHost) (void)hipHostFree(s->ncalcHost);
   if (s->pinnedOut) (void)hipHostFree(s->pinnedOut);
   if (s->pinnedIn) (void)hipHostFree(s->pinnedIn);
@@ -499,6 +501,17 @@ int sphgpu_ensemble_conservation(sphgpu_sim* s, int64_t id, double* out) {
     auto it = std::find_if(s->members.begin(), s->members.end(), [&](const Member& mb) { return mb.id == id; });
     SG_REQUIRE(it != s->members.end(), SPHGPU_ERR_ARG, "no member with that id");
     conservationTotals(*s, it->first, it->first + it->rows, out);
+  });
+}
+
+int sphgpu_set_graphs(sphgpu_sim* s, int on) {
+  return guarded(s, [&] { s->useGraphs = on != 0; });
+}
+
+int sphgpu_graph_counts(sphgpu_sim* s, int64_t* captures, int64_t* replays) {
+  return guarded(s, [&] {
+    if (captures) *captures = s->graphCaptures;
+    if (replays) *replays = s->graphReplays;
   });
 }
 

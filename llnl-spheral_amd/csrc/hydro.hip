@@ -1212,7 +1212,7 @@ void sumDensityD(Sim& s) {
   TabArgs t = tabArgs(s);
   const bool lds = ldsTable(s);
   const size_t ldsBytes = lds ? sizeof(double) * 6 * size_t(s.table.n) : 0;
-  SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
+  if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
   if (s.precision == SPHGPU_PRECISION_MIXED) {
     SG_REQUIRE(s.isotropic, SPHGPU_ERR_UNSUPPORTED, "mixed precision needs H = h I and no Q multipliers");
     if (!s.recSMValid) {
@@ -1232,7 +1232,7 @@ void sumDensityD(Sim& s) {
     if (lds) go(k_sum_density_mixed<D, true>); else go(k_sum_density_mixed<D, false>);
     SG_CHECK_HIP(hipGetLastError());
     applyGhostDensity(s);
-    SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
+    if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
     return;
   }
   // isotropic H: neighbours from the 48-B density records
@@ -1254,7 +1254,7 @@ void sumDensityD(Sim& s) {
   SG_CHECK_HIP(hipGetLastError());
   // SPHHydroBase::finalize then applies the boundaries to the mass density
   applyGhostDensity(s);
-  SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
+  if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
 }
 
 template <int D>
@@ -1277,7 +1277,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   const int sec = pairRecordSec(s, hp);
   const bool fast64 = sec == 2;
 
-  SG_CHECK_HIP(hipEventRecord(s.ev[4], st));
+  if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[4], st));
   if (!s.recValid) {
     packD<D>(s, sec);
     s.recValid = true;
@@ -1286,8 +1286,13 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   } else if (sec == 2 && s.pairRec != 2) {
     packFast64<D>(s);
   }
-  SG_CHECK_HIP(hipMemsetAsync(dncalc, 0, sizeof(unsigned long long), st));
-  SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
+  {
+    // a kernel, not a memset node, when the step is captured as a graph
+    ZeroSet z{};
+    z.add(reinterpret_cast<int32_t*>(dncalc), 2);
+    zeroMany(z, st);
+  }
+  if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[5], st));
 
   TabArgs t = tabArgs(s);
   const bool lds = ldsTable(s);
@@ -1346,7 +1351,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
     }
   }
   SG_CHECK_HIP(hipGetLastError());
-  SG_CHECK_HIP(hipEventRecord(s.ev[6], st));
+  if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[6], st));
   SG_CHECK_HIP(hipMemcpyAsync(s.ncalcHost, dncalc, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 }
 
@@ -1496,7 +1501,7 @@ void sumDensity(Sim& s) {
   else if (s.dim == 2) sumDensityD<2>(s);
   else sumDensityD<3>(s);
   s.pairRec = 0;      // the density changed rho in the records
-  s.timingPending |= 2u;
+  if (!s.capturing) s.timingPending |= 2u;
   s.derivsValid = false;
 }
 
@@ -1514,7 +1519,7 @@ void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly) {
   if (s.dim == 1) evalDerivsD<1>(s, p, updateOnly);
   else if (s.dim == 2) evalDerivsD<2>(s, p, updateOnly);
   else evalDerivsD<3>(s, p, updateOnly);
-  s.timingPending |= 4u;
+  if (!s.capturing) s.timingPending |= 4u;
   s.derivsValid = true;
   s.pairsValid = true;
 }

@@ -552,6 +552,128 @@ __global__ void k_plain_seg_init(SegState* __restrict__ g, double t, double last
   g->failPad = 0u;
 }
 
+// ---- the device part of a step as a HIP graph ---------------------------------
+// Everything between the neighbour build and the clock read-back (dt vote,
+// dt selection, both passes, updates, ghost refreshes, finalize, boundary
+// enforcement, clock advance) is launched without a host round trip.  Its
+// launches depend on the node and slot counts, the device buffers, the
+// parameters and the host-side record flags at its start: that is the key.
+// The first step with a new key runs eagerly (sizing every buffer), the next
+// step with the same key is captured and instantiated, and later steps with
+// that key replay the graph; the host flags the step leaves are saved at the
+// capture and restored after each replay.  An ensemble whose membership and
+// ghost counts hold still (most steps of a tube ensemble) replays one graph
+// per step.  Graphs hold no memset node (the pair count is cleared by a
+// kernel) and no stage-timing events.
+void keyBuf(std::vector<uint64_t>& k, const DBuf& b) {
+  k.push_back(reinterpret_cast<uint64_t>(b.p));
+  k.push_back(uint64_t(b.bytes));
+}
+template <class T> void keyBytes(std::vector<uint64_t>& k, const T& v) {
+  const unsigned char* c = reinterpret_cast<const unsigned char*>(&v);
+  for (size_t i = 0; i < sizeof(T); i += 8) {
+    uint64_t w = 0;
+    std::memcpy(&w, c + i, std::min<size_t>(8, sizeof(T) - i));
+    k.push_back(w);
+  }
+}
+std::vector<uint64_t> stepKey(const Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip) {
+  std::vector<uint64_t> k;
+  k.reserve(256);
+  for (int64_t v : {int64_t(s.dim), s.nInt, s.nTot, s.nSlots, int64_t(s.ellCap), int64_t(s.nSeg), s.tableVersion,
+                    int64_t(s.isotropic), int64_t(s.rowOrder), int64_t(s.precision), int64_t(s.deviceGhosts),
+                    int64_t(s.hasMultipliers), int64_t(s.recValid), int64_t(s.recSValid), int64_t(s.recSMValid),
+                    int64_t(s.pairRec), int64_t(s.haveDtIn), int64_t(s.derivsValid), int64_t(s.subs.size())})
+    k.push_back(uint64_t(v));
+  for (int64_t v : s.subStart) k.push_back(uint64_t(v));
+  for (const SubBoundary& b : s.subs) keyBytes(k, b);
+  keyBytes(k, hp);
+  keyBytes(k, ip);
+  keyBytes(k, s.dtThreshold);
+  k.push_back(reinterpret_cast<uint64_t>(s.ncalcHost));
+  for (const DBuf* b : {&s.pos, &s.vel, &s.mass, &s.rho, &s.eps, &s.P, &s.cs, &s.H, &s.omega, &s.fCl, &s.fCq,
+                        &s.fshear, &s.cellOf, &s.cellCount, &s.cellStart, &s.cellFill, &s.perm, &s.iperm, &s.xs,
+                        &s.Hs, &s.rowCount, &s.nbr, &s.scanTmp, &s.reduceTmp, &s.flags, &s.rec, &s.recM, &s.recD,
+                        &s.recS, &s.recSM, &s.pairDv, &s.compDe, &s.compAcc, &s.compDe0, &s.compRec, &s.out,
+                        &s.ghostControl, &s.ghostSub, &s.keep, &s.keepScan, &s.slotKeep, &s.hmaxOf, &s.genDev,
+                        &s.work, &s.workFlag, &s.workScan, &s.omega0, &s.dtIn, &s.seg, &s.segState, &s.plainSeg,
+                        &s.dTable, &s.dTable32, &s.dNperh, &s.dWsum, &s.pick})
+    keyBuf(k, *b);
+  for (const DBuf& b : s.state0) keyBuf(k, b);
+  return k;
+}
+
+Sim::StepFlags stepFlagsOf(const Sim& s) {
+  return Sim::StepFlags{s.recValid, s.recSValid, s.recSMValid, s.derivsValid, s.pairsValid, s.haveDtIn,
+                        s.pairRec, s.outStride, s.timingPending};
+}
+void restoreStepFlags(Sim& s, const Sim::StepFlags& f) {
+  s.recValid = f.recValid;
+  s.recSValid = f.recSValid;
+  s.recSMValid = f.recSMValid;
+  s.derivsValid = f.derivsValid;
+  s.pairsValid = f.pairsValid;
+  s.haveDtIn = f.haveDtIn;
+  s.pairRec = f.pairRec;
+  s.outStride = f.outStride;
+  s.timingPending = f.timingPending;
+}
+
+// A few graphs and recent keys are kept: culling swaps the state fields with
+// their spares, so a step's buffers alternate between two sets.
+constexpr size_t kStepGraphs = 4;
+
+template <class F> void runStep(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip, F&& launches) {
+  if (!s.useGraphs) {
+    launches();
+    return;
+  }
+  std::vector<uint64_t> key = stepKey(s, hp, ip);
+  for (Sim::StepGraph& g : s.stepGraphs)
+    if (g.key == key) {
+      SG_CHECK_HIP(hipGraphLaunch(g.exec, s.stream));
+      restoreStepFlags(s, g.flags);
+      ++s.graphReplays;
+      return;
+    }
+  if (std::find(s.recentKeys.begin(), s.recentKeys.end(), key) == s.recentKeys.end()) {
+    // a new key: run eagerly, which also sizes every buffer the step uses
+    launches();
+    s.recentKeys.push_back(std::move(key));
+    if (s.recentKeys.size() > kStepGraphs) s.recentKeys.erase(s.recentKeys.begin());
+    return;
+  }
+  SG_CHECK_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+  s.capturing = true;
+  try {
+    launches();
+  } catch (...) {
+    s.capturing = false;
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  s.capturing = false;
+  hipGraph_t g = nullptr;
+  SG_CHECK_HIP(hipStreamEndCapture(s.stream, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  SG_CHECK_HIP(e);
+  if (s.stepGraphs.size() >= kStepGraphs) {
+    SG_CHECK_HIP(hipGraphExecDestroy(s.stepGraphs.front().exec));
+    s.stepGraphs.erase(s.stepGraphs.begin());
+  }
+  Sim::StepGraph sg;
+  sg.key = std::move(key);
+  sg.exec = exec;
+  sg.flags = stepFlagsOf(s);
+  s.stepGraphs.push_back(std::move(sg));
+  SG_CHECK_HIP(hipGraphLaunch(exec, s.stream));
+  ++s.graphCaptures;
+}
+
 template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu_integrator_params& ip,
                              double maxTime) {
   hipStream_t st = s.stream;
@@ -564,17 +686,20 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
   std::memcpy(&ib, &init, sizeof(ib));
   hipLaunchKernelGGL(k_plain_seg_init, dim3(1), dim3(1), 0, st, sd, s.time, s.lastDt, maxTime, ip.gamma, ib);
   if (!s.haveDtIn) s.dtIn.ensure(sizeof(double) * 8);
-  hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
-                     s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
-                     s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(), s.dtIn.as<double>(),
-                     s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt, nullptr, &sd->voteBits, int64_t(0));
-  hipLaunchKernelGGL(k_seg_select_dt, dim3(1), dim3(kBlock), 0, st, sd, 1, ip.cfl, ip.dtMin, ip.dtMax, ip.dtGrowth,
-                     0.0);
-  SG_CHECK_HIP(hipGetLastError());
-  const SegArgs half{nullptr, sd, 1}, full{nullptr, sd, 0};
-  stepBody<D>(s, hp, ip, 0.0, half, full);
-  hipLaunchKernelGGL(k_seg_advance, dim3(1), dim3(kBlock), 0, st, sd, 1);
-  SG_CHECK_HIP(hipGetLastError());
+  runStep(s, hp, ip, [&] {
+    hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
+                       s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
+                       s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(),
+                       s.dtIn.as<double>(), s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt, nullptr,
+                       &sd->voteBits, int64_t(0));
+    hipLaunchKernelGGL(k_seg_select_dt, dim3(1), dim3(kBlock), 0, st, sd, 1, ip.cfl, ip.dtMin, ip.dtMax, ip.dtGrowth,
+                       0.0);
+    SG_CHECK_HIP(hipGetLastError());
+    const SegArgs half{nullptr, sd, 1}, full{nullptr, sd, 0};
+    stepBody<D>(s, hp, ip, 0.0, half, full);
+    hipLaunchKernelGGL(k_seg_advance, dim3(1), dim3(kBlock), 0, st, sd, 1);
+    SG_CHECK_HIP(hipGetLastError());
+  });
   SegState* hp2 = static_cast<SegState*>(hostStaging(s, 0, sizeof(SegState)));
   SG_CHECK_HIP(hipMemcpyAsync(hp2, sd, sizeof(SegState), hipMemcpyDeviceToHost, st));
   SG_CHECK_HIP(hipStreamSynchronize(st));
@@ -596,20 +721,22 @@ template <int D> void ensembleRk2D(Sim& s, const sphgpu_hydro_params& hp, const 
   const double init = double(FLT_MAX);
   unsigned long long ib;
   std::memcpy(&ib, &init, sizeof(ib));
-  hipLaunchKernelGGL(k_seg_vote_init, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns, ib);
   if (!s.haveDtIn) s.dtIn.ensure(sizeof(double) * 8);
-  hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
-                     s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
-                     s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(), s.dtIn.as<double>(),
-                     s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt, s.seg.as<double>(), &sd->voteBits,
-                     int64_t(sizeof(SegState) / sizeof(unsigned long long)));
-  hipLaunchKernelGGL(k_seg_select_dt, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns, ip.cfl, ip.dtMin, ip.dtMax,
-                     ip.dtGrowth, s.dtThreshold);
-  SG_CHECK_HIP(hipGetLastError());
-  const SegArgs half{s.seg.as<double>(), sd, 1}, full{s.seg.as<double>(), sd, 0};
-  stepBody<D>(s, hp, ip, 0.0, half, full);
-  hipLaunchKernelGGL(k_seg_advance, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns);
-  SG_CHECK_HIP(hipGetLastError());
+  runStep(s, hp, ip, [&] {
+    hipLaunchKernelGGL(k_seg_vote_init, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns, ib);
+    hipLaunchKernelGGL(k_dt<D>, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), 0, st, s.work.as<int32_t>(),
+                       s.perm.as<int32_t>(), s.nInt, s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), s.nSlots,
+                       s.H.as<double>(), s.cs.as<double>(), s.rho.as<double>(), s.vel.as<double>(),
+                       s.dtIn.as<double>(), s.haveDtIn ? 1 : 0, hp.nPerh, ip.useVelocityMagnitudeForDt,
+                       s.seg.as<double>(), &sd->voteBits, int64_t(sizeof(SegState) / sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_seg_select_dt, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns, ip.cfl, ip.dtMin, ip.dtMax,
+                       ip.dtGrowth, s.dtThreshold);
+    SG_CHECK_HIP(hipGetLastError());
+    const SegArgs half{s.seg.as<double>(), sd, 1}, full{s.seg.as<double>(), sd, 0};
+    stepBody<D>(s, hp, ip, 0.0, half, full);
+    hipLaunchKernelGGL(k_seg_advance, dim3(nblocks(ns)), dim3(kBlock), 0, st, sd, ns);
+    SG_CHECK_HIP(hipGetLastError());
+  });
   SegState* hs = static_cast<SegState*>(hostStaging(s, 0, sizeof(SegState) * size_t(ns)));
   SG_CHECK_HIP(hipMemcpyAsync(hs, sd, sizeof(SegState) * size_t(ns), hipMemcpyDeviceToHost, st));
   SG_CHECK_HIP(hipStreamSynchronize(st));

@@ -131,6 +131,7 @@ struct Sim {
   int64_t nInt = 0, nTot = 0;
 
   TableHost table;
+  int64_t tableVersion = 0;   // counts table installs (kernel arguments derive from the table)
   DBuf dTable;              // 6*n interleaved [Ak Bk Ck Ag Bg Cg] per point
   DBuf dTable32;            // FP32 copy, 8 floats per point [Ak Bk Ck 0 Ag Bg Cg 0]
   int precision = SPHGPU_PRECISION_FP64;   // of sum density (evaluate_derivatives takes its own)
@@ -232,6 +233,25 @@ struct Sim {
   std::vector<int64_t> failedCycles;   // steps each had taken
   DBuf segSpare, dtInSpare;
 
+  // HIP graph of the device part of a time step (dt vote, the two passes and
+  // updates, finalize, enforcement, clock advance): captured once its launch
+  // key (extents, buffers, parameters, host flags) repeats, then replayed.
+  bool useGraphs = false;
+  bool capturing = false;          // the step is being captured: no stage events
+  struct StepFlags {
+    bool recValid, recSValid, recSMValid, derivsValid, pairsValid, haveDtIn;
+    int pairRec;
+    int64_t outStride;
+    unsigned timingPending;
+  };
+  struct StepGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    StepFlags flags{};             // the host flags the step leaves
+  };
+  std::vector<StepGraph> stepGraphs;              // most recent last (a few: culling alternates buffers)
+  std::vector<std::vector<uint64_t>> recentKeys;  // keys of the last eager steps
+  int64_t graphCaptures = 0, graphReplays = 0;
   hipEvent_t ev[14] = {};          // 0-11 stage timings, 12-13 the copy probe
   Timing timing;
   // Stage timings and the pair count are read back lazily (no host wait

@@ -217,6 +217,8 @@ def lib():
             "sphgpu_set_dt_inputs": (C.c_int, [vp, C.c_int64, C.c_int64, _dp]),
             "sphgpu_ensemble_conservation": (C.c_int, [vp, C.c_int64, _dp]),
             "sphgpu_conservation_noalloc": (C.c_int, [vp, C.c_int64, C.c_int64, C.c_void_p]),
+            "sphgpu_set_graphs": (C.c_int, [vp, C.c_int]),
+            "sphgpu_graph_counts": (C.c_int, [vp, _i64p, _i64p]),
             "sphgpu_ensemble_begin": (C.c_int, [vp]),
             "sphgpu_ensemble_add": (C.c_int, [vp, C.POINTER(Member), C.POINTER(IntegratorParams), _i64p]),
             "sphgpu_ensemble_step": (C.c_int, [vp, C.POINTER(HydroParams), C.POINTER(IntegratorParams),
@@ -487,6 +489,16 @@ class Sim:
 
     def synchronize(self):
         self._chk(self.L.sphgpu_synchronize(self.h))
+
+    def set_graphs(self, on=True):
+        """Replay the device part of each time step from a captured HIP graph."""
+        self._chk(self.L.sphgpu_set_graphs(self.h, 1 if on else 0))
+
+    def graph_counts(self):
+        """(steps captured as graphs, steps replayed from a graph)."""
+        c, r = C.c_int64(), C.c_int64()
+        self._chk(self.L.sphgpu_graph_counts(self.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def conservation_noalloc(self, first, rows, out_device_ptr):
         """sphgpu_conservation_noalloc: the totals of rows [first, first + rows)
