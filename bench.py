@@ -451,6 +451,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-check", type=int, default=8, help="requests re-solved alone in FP64 for the error")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: launch shapes of the 1-D/2-D pair kernel from the committed tuning table "
+                         "(llnl-spheral_amd/tuning/), timing any workload class it lacks")
     ap.add_argument("--graphs", type=int, default=1,
                     help="1: ensemble, serving and their lone baselines replay the device part of each step from a "
                          "captured HIP graph")
@@ -467,6 +470,25 @@ def main():
     import sphgpu
     from sphgpu import problems
 
+    tuning = None
+    if args.autotune:
+        tuning = {"table_file": None, "imported": 0}
+        path = os.path.join(ROOT, "llnl-spheral_amd", "tuning", "mi355x_gfx950.txt")
+        if os.path.exists(path):
+            tuning["table_file"] = os.path.relpath(path, ROOT)
+            tuning["imported"] = sphgpu.autotune_load(open(path).read())
+        sphgpu.autotune(True)
+
+    def tuning_report():
+        if tuning is None:
+            return None
+        rows = [l.split() for l in sphgpu.autotune_table().splitlines() if l and not l.startswith("#")]
+        return dict(tuning, decisions=[{"kernel": "k_derivs_iso", "dim": int(r[1]),
+                                        "precision": "mixed" if r[2] == "1" else "fp64",
+                                        "row_bucket": int(r[3]), "size_bucket": int(r[4]), "variant": int(r[5]),
+                                        "ms": [float(x) for x in r[6:]]} for r in rows if r[0] == "0"],
+                    variants="v = 2 * log2(workgroup / 64) + (table in LDS); 4 variants when the table does not fit")
+
     if args.only_batch:
         line = {"metric": METRIC, "n_gpus": 1}
         if args.periodic:
@@ -477,6 +499,7 @@ def main():
         if args.serve:
             line["serving"] = serve_bench(sphgpu, problems, args.serve, args.serve_rate, args.batch_precision,
                                           args.serve_check, graphs=args.graphs)
+        line["autotune"] = tuning_report()
         print(json.dumps(line), flush=True)
         return
 
@@ -612,6 +635,7 @@ def main():
             line["periodic"] = periodic_bench(sphgpu, problems, args.periodic, args.periodic_steps)
         except Exception as e:
             line["periodic"] = {"error": str(e)}
+    line["autotune"] = tuning_report()
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args.cpu_budget, boundaries=args.boundaries)

@@ -327,6 +327,21 @@ int sphgpu_set_graphs(sphgpu_sim* sim, int on);
 /* Steps captured as graphs and steps replayed from one, since creation. */
 int sphgpu_graph_counts(sphgpu_sim* sim, int64_t* captures, int64_t* replays);
 
+/* Launch-shape autotuner (process-wide; no counterpart in the reference).
+ * With tuning on, the first pass of a workload class (kernel, dimension,
+ * arithmetic, log2 buckets of row length and node count) times every launch
+ * variant on the live inputs and keeps the fastest; variants compute the same
+ * values in the same order, so results do not change.  Tuned now: the 1-D
+ * and 2-D isotropic pair kernel (workgroups of 64-512 threads, table in LDS
+ * or read through L1).  Decisions export as text lines and import again, so
+ * that measured choices are reused without timing. */
+int sphgpu_autotune(int on);
+/* Copy the decision table into buf (NUL-terminated, at most cap bytes);
+ * *len = its full length. */
+int sphgpu_autotune_export(char* buf, int64_t cap, int64_t* len);
+/* Load decisions (the export format); *entries = lines taken. */
+int sphgpu_autotune_import(const char* text, int* entries);
+
 /* Device times in milliseconds of the last call of each stage, measured with
  * HIP events on the sim's stream: [neighbours, sum density, derivatives, pack,
  * ghost generation, ghost culling]. */

@@ -16,7 +16,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libsphgpu.so")
 
-SOURCES = ["abi.cpp", "table.cpp", "neighbors.hip", "hydro.hip", "ghosts.hip", "integrate.hip", "ensemble.hip", "serve.cpp", "probe.hip"]
+SOURCES = ["abi.cpp", "table.cpp", "neighbors.hip", "hydro.hip", "ghosts.hip", "integrate.hip", "ensemble.hip", "serve.cpp",
+           "probe.hip", "tune.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPHGPU_ARCH", "gfx950")

@@ -684,6 +684,37 @@ int sphgpu_serve(sphgpu_sim* s, sphgpu_request* req, int n, const sphgpu_hydro_p
   });
 }
 
+int sphgpu_autotune(int on) {
+  tuneEnable(on != 0);
+  return SPHGPU_OK;
+}
+
+int sphgpu_autotune_export(char* buf, int64_t cap, int64_t* len) {
+  try {
+    const std::string t = tuneExport();
+    if (len) *len = int64_t(t.size());
+    if (buf && cap > 0) {
+      const size_t n = std::min<size_t>(t.size(), size_t(cap - 1));
+      std::memcpy(buf, t.data(), n);
+      buf[n] = '\0';
+    }
+    return SPHGPU_OK;
+  } catch (...) {
+    return SPHGPU_ERR_ARG;
+  }
+}
+
+int sphgpu_autotune_import(const char* text, int* entries) {
+  if (!text) return SPHGPU_ERR_ARG;
+  try {
+    const int n = tuneImport(text);
+    if (entries) *entries = n;
+    return SPHGPU_OK;
+  } catch (...) {
+    return SPHGPU_ERR_ARG;
+  }
+}
+
 int64_t sphgpu_ensemble_bytes_per_node(int dim) {
   if (dim < 1 || dim > 3) return -1;
   return ensembleBytesPerNode(dim);

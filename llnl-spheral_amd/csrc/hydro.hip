@@ -884,8 +884,8 @@ template <int D> struct SumsF {
   }
 };
 
-template <typename Real, int D, bool LDS, bool PREFETCH>
-__global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict__ rec,
+template <typename Real, int D, bool LDS, bool PREFETCH, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ rec,
                                                        const typename IsoTab<Real>::Rec* __restrict__ recN,
                                                        const int32_t* __restrict__ rowCount,
                                                        const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
@@ -905,7 +905,7 @@ __global__ void __launch_bounds__(kBlock) k_derivs_iso(const double* __restrict_
   } else {
     tab = std::is_same<Real, float>::value ? reinterpret_cast<const E*>(t.g32) : reinterpret_cast<const E*>(t.g);
   }
-  const int64_t tw = xcdBlock((nInt + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+  const int64_t tw = xcdBlock((nInt + BS - 1) / BS) * BS + threadIdx.x;
   unsigned pairs = 0;
   // the 3-D mixed record holds quantised positions
   constexpr bool QPOS = std::is_same<Real, float>::value && D == 3;
@@ -1257,6 +1257,46 @@ void sumDensityD(Sim& s) {
   if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[3], s.stream));
 }
 
+// The collapsed isotropic pair kernel with BS-thread workgroups, its table in
+// LDS or read through L1.  Tuning variant v: with a table that fits in LDS,
+// BS = 64 << (v / 2) and the table in LDS when v is odd; otherwise BS = 64 << v.
+template <typename Real> typename IsoTab<Real>::T isoTable(const Sim& s);
+template <> Tab32 isoTable<float>(const Sim& s) { return tab32(s); }
+template <> Tab64 isoTable<double>(const Sim& s) { return tab64(s); }
+template <typename Real, int D, bool PF, int BS>
+void isoLaunch(Sim& s, bool inLds, hipStream_t st, int64_t n, const TabArgs& t, const PassArgs& p, const OutArgs& o,
+               unsigned long long* dncalc) {
+  using Rec = typename IsoTab<Real>::Rec;
+  const Rec* recN = reinterpret_cast<const Rec*>(std::is_same<Real, float>::value ? s.recM.p : s.recD.p);
+  const size_t lb = inLds ? sizeof(typename IsoTab<Real>::Elem) * IsoTab<Real>::PER * size_t(s.table.n) : 0;
+  const dim3 g(unsigned(xcdGrid(nblocks(s.nInt, BS)))), b(BS);
+  if (inLds)
+    hipLaunchKernelGGL((k_derivs_iso<Real, D, true, PF, BS>), g, b, lb, st, s.rec.as<double>(), recN,
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
+                       isoTable<Real>(s), p, o, dncalc);
+  else
+    hipLaunchKernelGGL((k_derivs_iso<Real, D, false, PF, BS>), g, b, 0, st, s.rec.as<double>(), recN,
+                       s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
+                       isoTable<Real>(s), p, o, dncalc);
+}
+template <typename Real, int D, bool PF>
+void isoVariant(Sim& s, int v, bool fits, hipStream_t st, int64_t n, const TabArgs& t, const PassArgs& p,
+                const OutArgs& o, unsigned long long* dncalc) {
+  const int bsLog = fits ? v >> 1 : v;
+  const bool inLds = fits && (v & 1);
+  if constexpr (D == 3) {
+    isoLaunch<Real, D, PF, 256>(s, inLds, st, n, t, p, o, dncalc);
+  } else {
+    switch (bsLog) {
+      case 0: isoLaunch<Real, D, PF, 64>(s, inLds, st, n, t, p, o, dncalc); break;
+      case 1: isoLaunch<Real, D, PF, 128>(s, inLds, st, n, t, p, o, dncalc); break;
+      case 2: isoLaunch<Real, D, PF, 256>(s, inLds, st, n, t, p, o, dncalc); break;
+      default: isoLaunch<Real, D, PF, 512>(s, inLds, st, n, t, p, o, dncalc); break;
+    }
+  }
+  SG_CHECK_HIP(hipGetLastError());
+}
+
 template <int D>
 void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   const int64_t n = s.nSlots;
@@ -1321,21 +1361,30 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   // wavefronts per SIMD (2.11 ms at 128^3 against 2.53 ms with the next
   // record prefetched at 179 VGPRs, two wavefronts).  FP64: the prefetch wins
   // (3.44 ms at 194 VGPRs against 3.51 ms at 168).
-  if (mixed) {
-    const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), lds32, st, s.rec.as<double>(),
-                         s.recM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
-                         s.work.as<int32_t>(), t, tab32(s), p, o, dncalc);
+  if (mixed || fast64) {
+    // 1-D and 2-D (ensembles, serving): the workgroup size and the table's
+    // place (LDS or read through L1) are autotuned per workload class.  3-D
+    // runs 256-thread workgroups with the table in LDS when it fits.
+    const bool fits = s.table.n <= kLdsTablePoints;
+    const int nv = fits ? 8 : 4;
+    auto run = [&](int v) {
+      if (mixed) isoVariant<float, D, false>(s, v, fits, st, n, t, p, o, dncalc);
+      else isoVariant<double, D, true>(s, v, fits, st, n, t, p, o, dncalc);
     };
-    if (lds) go(k_derivs_iso<float, D, true, false>); else go(k_derivs_iso<float, D, false, false>);
-  } else if (fast64) {
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),
-                         s.recD.as<double2>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
-                         s.work.as<int32_t>(), t, tab64(s), p, o, dncalc);
-    };
-    if (lds) go(k_derivs_iso<double, D, true, true>); else go(k_derivs_iso<double, D, false, true>);
+    const int def = fits ? (4 + (lds ? 1 : 0)) : 2;      // 256 threads, the row-length rule for the table
+    int v = def;
+    if (D < 3) {
+      bool timed = false;
+      const double perNode = s.nInt > 0 ? double(s.nEntries) / double(s.nInt) : 0.0;
+      v = tunedVariant(s, tuneKey(kTunePairIso, D, mixed ? 1 : 0, perNode, s.nInt), nv, def, run, &timed);
+      if (timed) {
+        // the trials added to the pair count
+        ZeroSet z{};
+        z.add(reinterpret_cast<int32_t*>(dncalc), 2);
+        zeroMany(z, st);
+      }
+    }
+    run(v);
   } else {
     auto launch = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, st, s.rec.as<double>(),

@@ -4,7 +4,9 @@
 #include <cstdint>
 #include <string>
 #include <vector>
+#include <functional>
 #include <stdexcept>
+#include <tuple>
 #include <utility>
 #include "../../include/sphgpu.h"
 
@@ -354,6 +356,22 @@ struct ZeroSet {
 void zeroMany(const ZeroSet& z, hipStream_t st);
 void exclusiveScanI32toI64(const int32_t* in, int64_t* out, int64_t n, Sim& s);
 void exclusiveScanI32(const int32_t* in, int32_t* out, int64_t n, Sim& s);
+
+// tune.cpp: launch-shape autotuner.  A workload class of one kernel family;
+// tunedVariant returns the variant to launch, timing every variant once (on
+// the live inputs, launched by `launch`) when tuning is on and the class is
+// new; otherwise the recorded choice, or `fallback`.
+struct TuneKey {
+  int kernel, dim, precision, rowBucket, sizeBucket;
+};
+bool operator<(const TuneKey& a, const TuneKey& b);
+enum { kTunePairIso = 0 };
+TuneKey tuneKey(int kernel, int dim, int precision, double rowLength, int64_t nodes);
+void tuneEnable(bool on);
+int tunedVariant(Sim& s, const TuneKey& key, int nVariants, int fallback, const std::function<void(int)>& launch,
+                 bool* timed = nullptr);
+std::string tuneExport();
+int tuneImport(const std::string& text);
 
 inline int nsymOf(int dim) { return dim * (dim + 1) / 2; }
 

@@ -218,6 +218,9 @@ def lib():
             "sphgpu_ensemble_conservation": (C.c_int, [vp, C.c_int64, _dp]),
             "sphgpu_conservation_noalloc": (C.c_int, [vp, C.c_int64, C.c_int64, C.c_void_p]),
             "sphgpu_set_graphs": (C.c_int, [vp, C.c_int]),
+            "sphgpu_autotune": (C.c_int, [C.c_int]),
+            "sphgpu_autotune_export": (C.c_int, [C.c_char_p, C.c_int64, _i64p]),
+            "sphgpu_autotune_import": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
             "sphgpu_graph_counts": (C.c_int, [vp, _i64p, _i64p]),
             "sphgpu_ensemble_begin": (C.c_int, [vp]),
             "sphgpu_ensemble_add": (C.c_int, [vp, C.POINTER(Member), C.POINTER(IntegratorParams), _i64p]),
@@ -265,6 +268,30 @@ def deriv_components(field, dim):
     if a in ("DHDt", "Hideal", "massSecondMoment"):
         return dim * (dim + 1) // 2
     return 1
+
+
+def autotune(on=True):
+    """Turn the launch-shape autotuner on or off (process-wide)."""
+    lib().sphgpu_autotune(1 if on else 0)
+
+
+def autotune_table():
+    """The autotuner's decisions as text (one workload class per line)."""
+    L = lib()
+    n = C.c_int64()
+    L.sphgpu_autotune_export(None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value + 1)
+    L.sphgpu_autotune_export(buf, n.value + 1, C.byref(n))
+    return buf.value.decode()
+
+
+def autotune_load(text):
+    """Load autotuner decisions (the autotune_table format); returns the lines taken."""
+    k = C.c_int()
+    rc = lib().sphgpu_autotune_import(text.encode(), C.byref(k))
+    if rc != 0:
+        raise SphGpuError(rc, "bad autotune table")
+    return k.value
 
 
 class Sim:

@@ -123,3 +123,35 @@ def test_ensemble_graph_steps_equal_eager_steps(sphgpu):
             assert np.array_equal(ra[q][0][f], rb[q][0][f]), (q, f)
     print(f"[graphs] ensemble: {sb} steps, {cap} captures, {rep} replays", flush=True)
     assert cap >= 1 and rep >= 1, (cap, rep, sb)
+
+
+def test_autotuned_pair_kernel_same_results(sphgpu):
+    """With the autotuner on, a 1-D tube ensemble times every launch variant
+    of the pair kernel on its first pass and keeps the fastest; the members'
+    results are bitwise those of the untuned run."""
+    members = problems.tube_ensemble(4, 100, 400, t_end=0.004, seed=57)
+    hp = sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED)
+    ip = sphgpu.integrator_params()
+    out = []
+    for tune in (False, True):
+        sphgpu.autotune(tune)
+        e = sphgpu.Sim(1)
+        e.set_row_order(sphgpu.ROW_ORDER_SEARCH)
+        e.set_table_bspline(1000)
+        e.set_precision(sphgpu.PRECISION_MIXED)
+        e.set_boundaries(sphgpu.box_boundaries(1))
+        e.ensemble_begin()
+        ids = [e.ensemble_add(m["state"], m["gamma"], 1.0, sphgpu.integrator_params(gamma=m["gamma"]))
+               for m in members]
+        for _ in range(6):
+            e.ensemble_step(hp, ip)
+        out.append({(i, f): e.ensemble_download(i, f) for i in ids
+                    for f in ("position", "velocity", "mass density", "specific thermal energy", "H")})
+        e.close()
+    sphgpu.autotune(False)
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+    rows = [l.split() for l in sphgpu.autotune_table().splitlines() if l and not l.startswith("#")]
+    tuned = [r for r in rows if r[0] == "0" and r[1] == "1" and r[2] == "1" and len(r) == 6 + 8]
+    assert tuned, rows
+    print("[autotune]", tuned, flush=True)

@@ -40,3 +40,15 @@ def test_binding_field_tables(sphgpu):
     assert sphgpu.deriv_components("DvDx", 3) == 9
     assert sphgpu.deriv_components("new H", 2) == 3
     assert sphgpu.state_components("H", 3) == 6
+
+
+def test_autotune_table_round_trip(sphgpu):
+    """The autotuner's decision table exports what it imported (no GPU call)."""
+    text = ("# kernel dim precision rowBucket sizeBucket variant [ms per variant]\n"
+            "0 1 1 2 17 3 0.5000 0.4000 0.3000 0.2000 0.2500 0.3500 0.4500 0.5500\n"
+            "0 2 0 4 12 5\n")
+    assert sphgpu.autotune_load(text) == 2
+    out = sphgpu.autotune_table()
+    rows = [l.split() for l in out.splitlines() if l and not l.startswith("#")]
+    assert ["0", "1", "1", "2", "17", "3"] == rows[0][:6] and len(rows[0]) == 14
+    assert ["0", "2", "0", "4", "12", "5"] in [r[:6] for r in rows]
