@@ -140,6 +140,27 @@ int sphgpu_table_values(sphgpu_sim* sim, double eta, double Hdet, double* W, dou
 /* Size the node arrays (NodeList::numInternalNodes / numGhostNodes,
  * src/NodeList/NodeListInline.hh:28-42).  Invalidates neighbours. */
 int sphgpu_set_nodes(sphgpu_sim* sim, int64_t n_internal, int64_t n_total);
+/* Several NodeLists in one sim (a DataBase of fluid NodeLists,
+ * src/DataBase/DataBaseInline.hh:230-258): NodeList L owns the next
+ * internal_counts[L] internal rows, NodeLists in order (the counts add up to
+ * n_internal; call after sphgpu_set_nodes with n_total = n_internal, the
+ * device boundaries make each NodeList's ghosts with its own hmax,
+ * PlanarBoundary::setGhostNodes(nodeList), src/Boundary/PlanarBoundary.cc:
+ * 112-182).  Pairs form across NodeLists as within one, each owned as
+ * ConnectivityMap::calculatePairInteraction decides
+ * (src/Neighbor/ConnectivityMapInline.hh:169-186); REFERENCE row order walks
+ * each row in the order the reference's loops deliver the pair terms, so the
+ * FP64 pass is bitwise the reference's.  evaluateDerivatives sums rhoSum and
+ * normalization over same-NodeList pairs only (src/SPH/SPHHydroBase.cc:746-751);
+ * the density sum counts a pair across NodeLists only with
+ * sum_over_all_node_lists (the SPH() default, src/SPH/SPHHydros.py:332), each
+ * end then taking its own mass times the other end's kernel
+ * (src/SPH/computeSPHSumMassDensity.cc:77-100).  gamma (n values, may be
+ * null) gives each NodeList its gamma-law EOS for sphgpu_eos_gamma_law and the
+ * time step.  n = 1 returns to one NodeList.  Not with mixed precision,
+ * compatible energy, grad-h, iterateIdealH or ensembles (SPHGPU_ERR_UNSUPPORTED). */
+int sphgpu_set_node_lists(sphgpu_sim* sim, int n, const int64_t* internal_counts, const double* gamma,
+                          int sum_over_all_node_lists);
 /* Copy one state field host -> device: n_total rows, or, when the ghosts were
  * made by sphgpu_set_ghost_nodes, n_internal rows (ghost rows then keep their
  * values until sphgpu_apply_ghost_boundaries). */

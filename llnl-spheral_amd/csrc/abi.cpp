@@ -214,6 +214,9 @@ int sphgpu_set_nodes(sphgpu_sim* s, int64_t nInt, int64_t nTot) {
     s->neighborsValid = false;
     s->derivsValid = false;
     s->hasMultipliers = false;
+    s->nLists = 0;             // one NodeList until sphgpu_set_node_lists
+    s->listStart.clear();
+    s->listGammaHost.clear();
     const size_t n = size_t(std::max<int64_t>(nTot, 1));
     const int D = s->dim, NS = nsymOf(D);
     s->pos.ensure(sizeof(double) * n * D);
@@ -504,6 +507,11 @@ int sphgpu_ensemble_conservation(sphgpu_sim* s, int64_t id, double* out) {
   });
 }
 
+int sphgpu_set_node_lists(sphgpu_sim* s, int n, const int64_t* internal_counts, const double* gamma,
+                          int sum_over_all_node_lists) {
+  return guarded(s, [&] { setNodeLists(*s, n, internal_counts, gamma, sum_over_all_node_lists); });
+}
+
 int sphgpu_set_graphs(sphgpu_sim* s, int on) {
   return guarded(s, [&] { s->useGraphs = on != 0; });
 }
@@ -556,6 +564,7 @@ int sphgpu_set_dt_inputs(sphgpu_sim* s, int64_t first, int64_t rows, const doubl
 int sphgpu_ensemble_begin(sphgpu_sim* s) {
   return guarded(s, [&] {
     SG_REQUIRE(s->nTot == 0 && !s->ensemble, SPHGPU_ERR_STATE, "ensemble mode needs a sim with no nodes");
+    SG_REQUIRE(s->nLists <= 1, SPHGPU_ERR_STATE, "an ensemble holds independent simulations, not NodeLists");
     SG_REQUIRE(!s->hasMultipliers, SPHGPU_ERR_UNSUPPORTED, "ensembles take no Q multiplier fields");
     s->ensemble = true;
     s->nSeg = 0;

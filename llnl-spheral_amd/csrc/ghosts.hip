@@ -494,7 +494,7 @@ std::vector<DBuf*> stateBufs(Sim& s) {
 // fields 9-11 exist with Q multipliers, field 12 in an ensemble
 bool stateUsed(const Sim& s, int which) {
   if (which >= 9 && which < 12) return s.hasMultipliers;
-  if (which == 12) return s.ensemble;
+  if (which == 12) return s.ensemble || s.nLists > 1;
   return true;
 }
 int stateComps(const Sim& s, int which) {
@@ -560,7 +560,7 @@ template <int D> bool generateFastD(Sim& s, double ext, unsigned long long* hbit
   GenDev* g = s.genDev.as<GenDev>();
   const int64_t* nDev = &g->n;
   hipLaunchKernelGGL(k_gen_init, dim3(1), dim3(kBlock), 0, st, g, nInt, hbits, nh);
-  const double* segp = s.nSeg > 0 ? s.seg.as<double>() : nullptr;
+  const double* segp = groupCount(s) > 0 ? s.seg.as<double>() : nullptr;
   for (size_t b = 0; b < s.subs.size(); ++b) {
     const DSub d = toDevice(s.subs[b]);
     if (segp)
@@ -573,7 +573,7 @@ template <int D> bool generateFastD(Sim& s, double ext, unsigned long long* hbit
                        s.keep.as<int32_t>(), nDev, blkCount);
     exclusiveScanI32(blkCount, blkScan, nb, s);
     hipLaunchKernelGGL((k_ghost_make<D, true>), dim3(nb), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
-                       s.nSeg > 0 ? s.seg.as<double>() : nullptr, cap, nInt, s.keep.as<int32_t>(), blkScan, d, int(b),
+                       groupCount(s) > 0 ? s.seg.as<double>() : nullptr, cap, nInt, s.keep.as<int32_t>(), blkScan, d, int(b),
                        s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>(), nDev, cap,
                        &g->overflow);
     hipLaunchKernelGGL(k_gen_advance, dim3(1), dim3(kBlock), 0, st, g, int(b), blkScan + nb, nInt, cap, hbits, nh);
@@ -595,8 +595,8 @@ template <int D> void generateD(Sim& s, double ext) {
   s.subStart.assign(s.subs.size() + 1, 0);
   s.flags.ensure(sizeof(int32_t) * 8 + sizeof(unsigned long long) * 2);
   unsigned long long* hbits = reinterpret_cast<unsigned long long*>(s.flags.as<int32_t>() + 4);
-  const int nh = std::max(1, s.nSeg);
-  if (s.nSeg > 0) {
+  const int nh = std::max(1, groupCount(s));
+  if (groupCount(s) > 0) {
     s.segHmaxBits.ensure(sizeof(unsigned long long) * size_t(nh));
     hbits = s.segHmaxBits.as<unsigned long long>();
   }
@@ -616,7 +616,7 @@ template <int D> void generateD(Sim& s, double ext) {
     const DSub d = toDevice(s.subs[b]);
     s.keep.ensure(sizeof(int32_t) * size_t(n + 1));
     s.keepScan.ensure(sizeof(int32_t) * size_t(n + 1));
-    const double* segp = s.nSeg > 0 ? s.seg.as<double>() : nullptr;
+    const double* segp = groupCount(s) > 0 ? s.seg.as<double>() : nullptr;
     SG_CHECK_HIP(hipMemsetAsync(hbits, 0, sizeof(unsigned long long) * size_t(nh), st));
     if (segp)
       hipLaunchKernelGGL(k_ghost_hmax<D>, dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(),
@@ -639,7 +639,7 @@ template <int D> void generateD(Sim& s, double ext) {
     s.ghostControl.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
     s.ghostSub.grow(sizeof(int32_t) * size_t(ng + ng / 2), sizeof(int32_t) * size_t(n - s.nInt), st);
     hipLaunchKernelGGL((k_ghost_make<D, false>), dim3(nblocks(n)), dim3(kBlock), 0, st, s.pos.as<double>(), s.H.as<double>(),
-                       s.nSeg > 0 ? s.seg.as<double>() : nullptr, n, s.nInt, s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), d, int(b),
+                       groupCount(s) > 0 ? s.seg.as<double>() : nullptr, n, s.nInt, s.keep.as<int32_t>(), s.keepScan.as<int32_t>(), d, int(b),
                        s.ghostControl.as<int32_t>(), s.ghostSub.as<int32_t>(), s.hmaxOf.as<double>(), nullptr,
                        int64_t(0), nullptr);
     SG_CHECK_HIP(hipGetLastError());
@@ -755,6 +755,9 @@ void cull(Sim& s) {
   for (size_t b = 0; b < s.subStart.size(); ++b) s.subStart[b] = int64_t(starts[b]) - nInt;
   s.nTot = nNew;
   s.recValid = recs;
+  // with several NodeLists the density sum also looks up a neighbour's
+  // NodeList by that index: the density records are packed again
+  if (s.nLists > 1) s.recSValid = false;
   s.derivsValid = false;
 }
 
@@ -889,6 +892,52 @@ void applyGhostScalar(Sim& s, DBuf& field) {
 
 void ensureStateRows(Sim& s, int64_t rows) { ensureRows(s, rows); }
 bool stateFieldUsed(const Sim& s, int which) { return stateUsed(s, which); }
+
+namespace {
+__global__ void k_fill_value(double* __restrict__ a, int64_t first, int64_t rows, double v) {
+  const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (r < rows) a[first + r] = v;
+}
+}  // namespace
+
+// Several NodeLists in one node set: NodeList L owns `counts[L]` consecutive
+// internal rows, in NodeList order; their ghosts are made by the device
+// boundaries, each NodeList with its own (PlanarBoundary::setGhostNodes per
+// NodeList).  gamma (may be null) gives each NodeList its gamma-law EOS.
+void setNodeLists(Sim& s, int n, const int64_t* counts, const double* gamma, int sumAll) {
+  SG_REQUIRE(!s.ensemble, SPHGPU_ERR_STATE, "an ensemble holds independent simulations, not NodeLists");
+  SG_REQUIRE(n >= 1 && counts, SPHGPU_ERR_ARG, "need at least one NodeList");
+  SG_REQUIRE(s.nTot == s.nInt, SPHGPU_ERR_STATE,
+             "set the NodeLists on internal rows only (n_total = n_internal); the device makes their ghosts");
+  std::vector<int64_t> start(size_t(n) + 1, 0);
+  for (int L = 0; L < n; ++L) {
+    SG_REQUIRE(counts[L] >= 0, SPHGPU_ERR_ARG, "negative NodeList size");
+    start[size_t(L) + 1] = start[size_t(L)] + counts[L];
+  }
+  SG_REQUIRE(start.back() == s.nInt, SPHGPU_ERR_ARG, "the NodeList sizes must add up to n_internal");
+  if (gamma)
+    for (int L = 0; L < n; ++L) SG_REQUIRE(gamma[L] > 1.0, SPHGPU_ERR_ARG, "a NodeList gamma must exceed 1");
+  s.nLists = n > 1 ? n : 0;
+  s.listStart = start;
+  s.sumAllLists = sumAll != 0;
+  s.listGammaHost.assign(gamma ? gamma : nullptr, gamma ? gamma + n : nullptr);
+  if (gamma) {
+    s.listGamma.ensure(sizeof(double) * size_t(n));
+    SG_CHECK_HIP(hipMemcpyAsync(s.listGamma.p, gamma, sizeof(double) * size_t(n), hipMemcpyHostToDevice, s.stream));
+  }
+  if (s.nLists > 1) {
+    ensureRows(s, std::max<int64_t>(s.nTot, 1));
+    for (int L = 0; L < n; ++L)
+      if (counts[L] > 0)
+        hipLaunchKernelGGL(k_fill_value, dim3(nblocks(counts[L])), dim3(kBlock), 0, s.stream, s.seg.as<double>(),
+                           start[size_t(L)], counts[L], double(L));
+    SG_CHECK_HIP(hipGetLastError());
+  }
+  SG_CHECK_HIP(hipStreamSynchronize(s.stream));   // `gamma` may be released on return
+  s.neighborsValid = false;
+  s.recValid = false;
+  s.derivsValid = false;
+}
 DBuf& stateField(Sim& s, int which) { return *stateBufs(s)[size_t(which)]; }
 int stateFieldComps(const Sim& s, int which) { return stateComps(s, which); }
 

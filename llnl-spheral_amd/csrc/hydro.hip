@@ -242,7 +242,8 @@ template <int D, bool ISO, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec, const int32_t* __restrict__ rowCount,
                                                         const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
                                                         const int32_t* __restrict__ work, TabArgs t,
-                                                        double* __restrict__ rhoOut, const double* __restrict__ nrec) {
+                                                        double* __restrict__ rhoOut, const double* __restrict__ nrec,
+                                                        const double* __restrict__ listOf, int sumAll) {
   using R = RL<ISO>;
   extern __shared__ double lds[];
   const double* tab = stageTable<LDS>(lds, t);
@@ -271,9 +272,13 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
     const DNbr<D, ISO> nb = nx;
     if (w + 1 < len) nx = load(nextIndex(nb4, n, k, len, w + 1, cur, nxt));
     if (!selfDone && nb.orig > oi) { acc += self; selfDone = true; }
+    // several NodeLists: a pair across them counts with sumAll only, the node
+    // taking its own mass (computeSPHSumMassDensity.cc:77-100)
+    const bool same = !listOf || listOf[int64_t(nb.orig)] == listOf[int64_t(oi)];
+    if (!same && !sumAll) continue;
     const V<D> rij = vsub(me.x, nb.x);
     const double etaj = vmag(hdot<D, ISO>(nb.H, rij));
-    acc += nb.m * lookupW(tab, t, etaj, nb.hdet);
+    acc += (same ? nb.m : mi) * lookupW(tab, t, etaj, nb.hdet);
   }
   if (!selfDone) acc += self;
   rhoOut[int64_t(oi)] = acc;
@@ -340,6 +345,7 @@ struct PassArgs {
   int XSPH, correctVelocityGradient, evolveTotalEnergy, linearInExpansion, quadraticInExpansion;
   double qInv;   // 1/u and u of the 3-D mixed record's positions
   float qUnit;
+  const double* listOf;   // NodeList of every caller row (several NodeLists), or null
 };
 
 __device__ __forceinline__ double muQ(double mu, double Cl, double Cq, double cs, const PassArgs& p) {
@@ -517,8 +523,10 @@ __device__ __forceinline__ unsigned derivsNodeFP64(int64_t k, const double* __re
       for (int a = 0; a < D; ++a)
         for (int b = a; b < D; ++b, ++s) m2nd.c[s] += (rij.c[a] * rij.c[b]) * sinv * g2;
     }
-    rhoSum += mj * Wi;
-    norm += mori * Wi;
+    if (!p.listOf || p.listOf[int64_t(nb.orig)] == p.listOf[int64_t(oi)]) {   // SPHHydroBase.cc:746-751
+      rhoSum += mj * Wi;
+      norm += mori * Wi;
+    }
 
     // Monaghan-Gingold Q (MonaghanGingoldViscosity.cc:83-132)
     double Cl = p.Cl, Cq = p.Cq;
@@ -1173,8 +1181,8 @@ int pairRecordSec(const Sim& s, const sphgpu_hydro_params& hp) {
   const bool compat = hp.compatibleEnergyEvolution != 0;
   if (hp.precision == SPHGPU_PRECISION_MIXED) return 1;
   // search row order is held to round-off of the reference sums, not to
-  // their bits: the collapsed isotropic algebra in FP64
-  return (s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile && !compat) ? 2 : 0;
+  // their bits: the collapsed isotropic algebra in FP64 (one NodeList)
+  return (s.rowOrder == SPHGPU_ROW_ORDER_SEARCH && s.isotropic && !tensile && !compat && s.nLists <= 1) ? 2 : 0;
 }
 
 int densityRecordSec(const Sim& s) {
@@ -1215,6 +1223,7 @@ void sumDensityD(Sim& s) {
   if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
   if (s.precision == SPHGPU_PRECISION_MIXED) {
     SG_REQUIRE(s.isotropic, SPHGPU_ERR_UNSUPPORTED, "mixed precision needs H = h I and no Q multipliers");
+    SG_REQUIRE(s.nLists <= 1, SPHGPU_ERR_UNSUPPORTED, "mixed precision takes one NodeList");
     if (!s.recSMValid) {
       s.recSM.ensure(sizeof(float) * 8 * size_t(std::max<int64_t>(n, 1)));
       hipLaunchKernelGGL(k_pack_dens_mixed<D>, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), n,
@@ -1244,10 +1253,11 @@ void sumDensityD(Sim& s) {
     s.recSValid = true;
   }
   const double* nrec = s.isotropic ? s.recS.as<double>() : s.rec.as<double>();
+  const double* listOf = s.nLists > 1 ? s.seg.as<double>() : nullptr;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(), t,
-                       s.rho.as<double>(), nrec);
+                       s.rho.as<double>(), nrec, listOf, s.sumAllLists ? 1 : 0);
   };
   if (s.isotropic) { if (lds) go(k_sum_density<D, true, true>); else go(k_sum_density<D, true, false>); }
   else { if (lds) go(k_sum_density<D, false, true>); else go(k_sum_density<D, false, false>); }
@@ -1314,6 +1324,8 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   if (mixed)
     SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
                "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
+  SG_REQUIRE(s.nLists <= 1 || (!mixed && !compat), SPHGPU_ERR_UNSUPPORTED,
+             "several NodeLists take the FP64 pass without compatible energy");
   const int sec = pairRecordSec(s, hp);
   const bool fast64 = sec == 2;
 
@@ -1346,6 +1358,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   p.linearInExpansion = hp.linearInExpansion; p.quadraticInExpansion = hp.quadraticInExpansion;
   p.qInv = mixed ? quantInv(s) : 1.0;
   p.qUnit = float(1.0 / p.qInv);
+  p.listOf = s.nLists > 1 ? s.seg.as<double>() : nullptr;
   OutArgs o{};
   o.base = s.out.as<double>();
   o.n = n;
@@ -1424,6 +1437,7 @@ void omegaGradhD(Sim& s) {
 
 void omegaGradh(Sim& s) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "the grad-h correction needs a neighbour build");
+  SG_REQUIRE(s.nLists <= 1, SPHGPU_ERR_UNSUPPORTED, "the grad-h correction takes one NodeList");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   if (!s.recValid) packRecords(s);
   if (s.dim == 1) omegaGradhD<1>(s);

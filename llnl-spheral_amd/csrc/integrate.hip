@@ -77,8 +77,15 @@ struct SegArgs {
   const double* seg;        // member of each row; null: one simulation
   SegState* st;             // null: the scalar multiplier and EOS apply to every row
   int half;                 // 1: multiply by dt/2
+  const double* listOf = nullptr;      // several NodeLists with their own gamma-law EOS:
+  const double* listGamma = nullptr;   //   each row's NodeList, and the NodeLists' gammas
 };
 __device__ __forceinline__ void rowMultEos(const SegArgs& sa, int64_t i, double& mult, Eos& e) {
+  if (sa.listGamma) {
+    const double g = sa.listGamma[int(sa.listOf[i])];
+    e.gamma = g;
+    e.gamma1 = g - 1.0;
+  }
   if (!sa.st) return;
   if (!sa.seg) {            // one simulation whose dt was chosen on the device
     mult = sa.half ? sa.st->hdt : sa.st->dt;
@@ -583,7 +590,8 @@ std::vector<uint64_t> stepKey(const Sim& s, const sphgpu_hydro_params& hp, const
   for (int64_t v : {int64_t(s.dim), s.nInt, s.nTot, s.nSlots, int64_t(s.ellCap), int64_t(s.nSeg), s.tableVersion,
                     int64_t(s.isotropic), int64_t(s.rowOrder), int64_t(s.precision), int64_t(s.deviceGhosts),
                     int64_t(s.hasMultipliers), int64_t(s.recValid), int64_t(s.recSValid), int64_t(s.recSMValid),
-                    int64_t(s.pairRec), int64_t(s.haveDtIn), int64_t(s.derivsValid), int64_t(s.subs.size())})
+                    int64_t(s.pairRec), int64_t(s.haveDtIn), int64_t(s.derivsValid), int64_t(s.subs.size()),
+                    int64_t(s.nLists), int64_t(s.sumAllLists), int64_t(s.listGammaHost.size())})
     k.push_back(uint64_t(v));
   for (int64_t v : s.subStart) k.push_back(uint64_t(v));
   for (const SubBoundary& b : s.subs) keyBytes(k, b);
@@ -597,7 +605,7 @@ std::vector<uint64_t> stepKey(const Sim& s, const sphgpu_hydro_params& hp, const
                         &s.recS, &s.recSM, &s.pairDv, &s.compDe, &s.compAcc, &s.compDe0, &s.compRec, &s.out,
                         &s.ghostControl, &s.ghostSub, &s.keep, &s.keepScan, &s.slotKeep, &s.hmaxOf, &s.genDev,
                         &s.work, &s.workFlag, &s.workScan, &s.omega0, &s.dtIn, &s.seg, &s.segState, &s.plainSeg,
-                        &s.dTable, &s.dTable32, &s.dNperh, &s.dWsum, &s.pick})
+                        &s.dTable, &s.dTable32, &s.dNperh, &s.dWsum, &s.pick, &s.listGamma})
     keyBuf(k, *b);
   for (const DBuf& b : s.state0) keyBuf(k, b);
   return k;
@@ -695,7 +703,10 @@ template <int D> double rk2D(Sim& s, const sphgpu_hydro_params& hp, const sphgpu
     hipLaunchKernelGGL(k_seg_select_dt, dim3(1), dim3(kBlock), 0, st, sd, 1, ip.cfl, ip.dtMin, ip.dtMax, ip.dtGrowth,
                        0.0);
     SG_CHECK_HIP(hipGetLastError());
-    const SegArgs half{nullptr, sd, 1}, full{nullptr, sd, 0};
+    const bool perList = s.nLists > 1 && !s.listGammaHost.empty();
+    const double* lo = perList ? s.seg.as<double>() : nullptr;
+    const double* lg = perList ? s.listGamma.as<double>() : nullptr;
+    const SegArgs half{nullptr, sd, 1, lo, lg}, full{nullptr, sd, 0, lo, lg};
     stepBody<D>(s, hp, ip, 0.0, half, full);
     hipLaunchKernelGGL(k_seg_advance, dim3(1), dim3(kBlock), 0, st, sd, 1);
     SG_CHECK_HIP(hipGetLastError());
@@ -757,6 +768,17 @@ __global__ void k_eos_rows(const double* __restrict__ rho, const double* __restr
 
 void eosGammaLaw(Sim& s, double gamma, double Pmin, double Pmax, int floorType) {
   if (s.nInt == 0) return;
+  if (s.nLists > 1 && !s.listGammaHost.empty()) {
+    // each NodeList with its own gamma-law gas
+    for (int L = 0; L < s.nLists; ++L) {
+      const int64_t first = s.listStart[size_t(L)], rows = s.listStart[size_t(L) + 1] - first;
+      if (rows > 0) eosRows(s, first, rows, s.listGammaHost[size_t(L)], Pmin, Pmax, floorType);
+    }
+    SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+    s.recValid = false;
+    s.derivsValid = false;
+    return;
+  }
   hipLaunchKernelGGL(k_eos, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, s.rho.as<double>(), s.eps.as<double>(),
                      s.P.as<double>(), s.cs.as<double>(), s.nInt, makeEos(gamma, Pmin, Pmax, floorType));
   SG_CHECK_HIP(hipGetLastError());
@@ -920,6 +942,7 @@ int iterateIdealH(Sim& s, const sphgpu_hydro_params& hp, int maxIterations, doub
                   double* maxDelta) {
   SG_REQUIRE(s.nInt > 0, SPHGPU_ERR_STATE, "no nodes");
   SG_REQUIRE(!s.ensemble, SPHGPU_ERR_STATE, "ensembles iterate H per member alone");
+  SG_REQUIRE(s.nLists <= 1, SPHGPU_ERR_UNSUPPORTED, "iterateIdealH takes one NodeList");
   SG_REQUIRE(s.nTot == s.nInt || s.deviceGhosts, SPHGPU_ERR_STATE,
              "iterateIdealH makes its own ghosts: set_nodes(n_internal, n_internal)");
   SG_REQUIRE(maxIterations >= 0 && tol >= 0.0 && ext > 0.0, SPHGPU_ERR_ARG, "bad iteration parameters");

@@ -217,6 +217,17 @@ struct Sim {
   // over all members in one launch.  nSeg = 0: a plain simulation.
   bool ensemble = false;
   int nSeg = 0;
+  // Several NodeLists in one node set (not an ensemble): NodeList L owns the
+  // internal caller rows [listStart[L], listStart[L+1]); `seg` holds every
+  // row's NodeList (ghosts inherit their control's).  Pairs form across
+  // NodeLists; each NodeList makes its own ghosts; rhoSum and normalization
+  // take same-NodeList pairs; the density sum follows sumAllLists
+  // (sumMassDensityOverAllNodeLists); listGamma gives each its gamma-law EOS.
+  int nLists = 0;                // 0: one NodeList
+  std::vector<int64_t> listStart;
+  std::vector<double> listGammaHost;
+  DBuf listGamma;                // per NodeList gamma (device), empty: the integrator's gamma
+  bool sumAllLists = true;
   DBuf seg;                      // member index per caller row (as a double)
   DBuf segGrid;                  // neighbour grid list (levels, members), in one buffer:
   void* gridList = nullptr;      //   the grids (DGrid, neighbors.hip),
@@ -374,6 +385,9 @@ std::string tuneExport();
 int tuneImport(const std::string& text);
 
 inline int nsymOf(int dim) { return dim * (dim + 1) / 2; }
+// Row groups with their own ghost selection: ensemble members, or NodeLists.
+inline int groupCount(const Sim& s) { return s.nSeg > 0 ? s.nSeg : (s.nLists > 1 ? s.nLists : 0); }
+void setNodeLists(Sim& s, int n, const int64_t* counts, const double* gamma, int sumAll);   // ghosts.hip
 
 // Number of doubles per node of each derivative output field, in the order of
 // the SPHGPU_D_* ids (see sphgpu.h).

@@ -659,8 +659,8 @@ __global__ void k_row_stats(const int32_t* __restrict__ rowCount, int64_t n, uns
 // K keys per lane (rows up to 256 entries), bitonic network in registers;
 // longer rows are sorted by a single lane.
 // ---------------------------------------------------------------------------
-template <int K>
-__device__ __forceinline__ void waveBitonic(int32_t (&key)[K], int lane) {
+template <int K, typename T>
+__device__ __forceinline__ void waveBitonic(T (&key)[K], int lane) {
   constexpr int N = 64 * K;
 #pragma unroll
   for (int kk = 2; kk <= N; kk <<= 1) {
@@ -674,7 +674,7 @@ __device__ __forceinline__ void waveBitonic(int32_t (&key)[K], int lane) {
           if (p > q) {
             const int e = q * 64 + lane;
             const bool asc = (e & kk) == 0;
-            const int32_t a = key[q], b = key[p];
+            const T a = key[q], b = key[p];
             const bool sw = asc ? (a > b) : (a < b);
             key[q] = sw ? b : a;
             key[p] = sw ? a : b;
@@ -684,11 +684,11 @@ __device__ __forceinline__ void waveBitonic(int32_t (&key)[K], int lane) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
           const int e = q * 64 + lane;
-          const int32_t other = __shfl_xor(key[q], jj);
+          const T other = __shfl_xor(key[q], jj);
           const bool asc = (e & kk) == 0;
           const bool lower = (lane & jj) == 0;
-          const int32_t mn = key[q] < other ? key[q] : other;
-          const int32_t mx = key[q] < other ? other : key[q];
+          const T mn = key[q] < other ? key[q] : other;
+          const T mx = key[q] < other ? other : key[q];
           key[q] = (lower == asc) ? mn : mx;
         }
       }
@@ -713,14 +713,69 @@ __device__ void sortRowWave(int32_t* __restrict__ nbr, int64_t n, int64_t k, int
   }
 }
 
+// Several NodeLists: the order in which the reference's scatter delivers node
+// x's pair terms (calculatePairInteraction, ConnectivityMapInline.hh:169-186):
+// first the internal neighbours that own their pair (lower caller index, in
+// that order), then x's own pairs in ConnectivityMap order, by (NodeList,
+// index within the NodeList), a NodeList's ghosts after its internal nodes.
+// With one NodeList this is ascending caller index.
+__device__ __forceinline__ int64_t receiveKey(int32_t c, int32_t x, const double* __restrict__ seg, int64_t nInt) {
+  if (c < nInt && c < x) return int64_t(c);
+  const int64_t local = c < nInt ? int64_t(c) : (int64_t(1) << 31) + c;
+  return (int64_t(1) << 62) | (int64_t(seg[c]) << 32) | local;
+}
+__device__ __forceinline__ int32_t callerOfKey(int64_t key) {
+  if (key < (int64_t(1) << 62)) return int32_t(key);
+  const int64_t low = key & 0xffffffffll;
+  return int32_t(low >= (int64_t(1) << 31) ? low - (int64_t(1) << 31) : low);
+}
+template <int K>
+__device__ void sortRowWaveLists(int32_t* __restrict__ nbr, int64_t n, int64_t k, int len,
+                                 const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
+                                 const double* __restrict__ seg, int64_t nInt, int lane) {
+  const int32_t x = perm[k];
+  int64_t key[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int e = q * 64 + lane;
+    key[q] = e < len ? receiveKey(perm[nbr[ellIndex(e, n, k)]], x, seg, nInt) : INT64_MAX;
+  }
+  waveBitonic<K>(key, lane);
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int e = q * 64 + lane;
+    if (e < len) nbr[ellIndex(e, n, k)] = iperm[callerOfKey(key[q])];
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_row_sort(const int32_t* __restrict__ rowCount, int32_t* __restrict__ nbr,
                                                     int64_t n, const int32_t* __restrict__ perm,
-                                                    const int32_t* __restrict__ iperm) {
+                                                    const int32_t* __restrict__ iperm,
+                                                    const double* __restrict__ seg, int64_t nInt) {
   const int lane = threadIdx.x & 63;
   const int64_t k = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6;
   if (k >= n) return;                              // uniform per wavefront
   const int len = rowCount[k];
   if (len <= 1) return;
+  if (seg) {
+    if (len <= 64) sortRowWaveLists<1>(nbr, n, k, len, perm, iperm, seg, nInt, lane);
+    else if (len <= 128) sortRowWaveLists<2>(nbr, n, k, len, perm, iperm, seg, nInt, lane);
+    else if (len <= 256) sortRowWaveLists<4>(nbr, n, k, len, perm, iperm, seg, nInt, lane);
+    else if (lane == 0) {
+      const int32_t x = perm[k];
+      for (int a2 = 1; a2 < len; ++a2) {
+        const int32_t v = nbr[ellIndex(a2, n, k)];
+        const int64_t pv = receiveKey(perm[v], x, seg, nInt);
+        int q = a2 - 1;
+        while (q >= 0 && receiveKey(perm[nbr[ellIndex(q, n, k)]], x, seg, nInt) > pv) {
+          nbr[ellIndex(q + 1, n, k)] = nbr[ellIndex(q, n, k)];
+          --q;
+        }
+        nbr[ellIndex(q + 1, n, k)] = v;
+      }
+    }
+    return;
+  }
   if (len <= 64) sortRowWave<1>(nbr, n, k, len, perm, iperm, lane);
   else if (len <= 128) sortRowWave<2>(nbr, n, k, len, perm, iperm, lane);
   else if (len <= 256) sortRowWave<4>(nbr, n, k, len, perm, iperm, lane);
@@ -1176,7 +1231,8 @@ void buildNeighborsD(Sim& s, double ext) {
   // 4. reference row order
   if (s.rowOrder == SPHGPU_ROW_ORDER_REFERENCE && n > 0)
     hipLaunchKernelGGL(k_row_sort, dim3(nblocks(n * 64)), dim3(kBlock), 0, st, s.rowCount.as<int32_t>(),
-                       s.nbr.as<int32_t>(), n, s.perm.as<int32_t>(), s.iperm.as<int32_t>());
+                       s.nbr.as<int32_t>(), n, s.perm.as<int32_t>(), s.iperm.as<int32_t>(),
+                       s.nLists > 1 ? s.seg.as<double>() : nullptr, s.nInt);
   // 5. work list of internal slots
   s.workFlag.ensure(sizeof(int32_t) * size_t(n + 1));
   s.workScan.ensure(sizeof(int32_t) * size_t(n + 1));

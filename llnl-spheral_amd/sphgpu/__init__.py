@@ -218,6 +218,7 @@ def lib():
             "sphgpu_ensemble_conservation": (C.c_int, [vp, C.c_int64, _dp]),
             "sphgpu_conservation_noalloc": (C.c_int, [vp, C.c_int64, C.c_int64, C.c_void_p]),
             "sphgpu_set_graphs": (C.c_int, [vp, C.c_int]),
+            "sphgpu_set_node_lists": (C.c_int, [vp, C.c_int, _i64p, _dp, C.c_int]),
             "sphgpu_autotune": (C.c_int, [C.c_int]),
             "sphgpu_autotune_export": (C.c_int, [C.c_char_p, C.c_int64, _i64p]),
             "sphgpu_autotune_import": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
@@ -516,6 +517,15 @@ class Sim:
 
     def synchronize(self):
         self._chk(self.L.sphgpu_synchronize(self.h))
+
+    def set_node_lists(self, counts, gamma=None, sum_all=True):
+        """Several NodeLists: NodeList L owns the next counts[L] internal rows;
+        gamma (one per NodeList, optional) sets each its gamma-law EOS;
+        sum_all = sumMassDensityOverAllNodeLists."""
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        g = None if gamma is None else np.ascontiguousarray(gamma, dtype=np.float64)
+        self._chk(self.L.sphgpu_set_node_lists(self.h, len(c), _p(c, _i64p), _p(g) if g is not None else None,
+                                               1 if sum_all else 0))
 
     def set_graphs(self, on=True):
         """Replay the device part of each time step from a captured HIP graph."""

@@ -199,29 +199,40 @@ template <int D> Sym<D> reflectSym(const Ten<D>& R, const Sym<D>& H) {
   return o;
 }
 
+// Several NodeLists (list != null, one entry per node and room for the
+// ghosts): each NodeList finds its own hmax and controls among its own nodes
+// (PlanarBoundary::setGhostNodes(nodeList), src/Boundary/PlanarBoundary.cc:
+// 112-182), and a ghost belongs to its control's NodeList.  The ghosts of one
+// sub-boundary are appended in node order, so each NodeList's own ghosts keep
+// the reference's per-NodeList order.
 template <int D>
 int64_t setGhosts(int64_t nInt, double* pos, double* H, int nb, const orc_boundary* b, double ext, int64_t maxGhost,
-                  int32_t* control, int32_t* subOf, int32_t* subStart) {
+                  int32_t* control, int32_t* subOf, int32_t* subStart, int32_t* list = nullptr) {
   constexpr int NS = nsym<D>();
   const auto subs = expand<D>(nb, b);
   int64_t n = nInt;
   auto X = [&](int64_t i) { Vec<D> v; for (int k = 0; k < D; ++k) v.c[k] = pos[i * D + k]; return v; };
   auto HH = [&](int64_t i) { Sym<D> s; double* p = sptr(s); for (int k = 0; k < NS; ++k) p[k] = H[i * NS + k]; return s; };
+  int nLists = 1;
+  if (list)
+    for (int64_t i = 0; i < nInt; ++i) nLists = std::max(nLists, list[i] + 1);
+  auto L = [&](int64_t i) { return list ? list[i] : 0; };
   for (size_t si = 0; si < subs.size(); ++si) {
     const Sub<D>& S = subs[si];
     subStart[si] = int32_t(n - nInt);
-    double hmax = 0.0;
+    std::vector<double> hmax(size_t(nLists), 0.0);
     for (int64_t i = 0; i < n; ++i) {
       const Vec<D> ri = X(i);
       const double hmaxi = 1.0 / minEigen(HH(i));
-      if (hmaxi > hmax && std::min(minimumDistance(S.exit, ri), minimumDistance(S.enter, ri)) < ext * hmaxi) hmax = hmaxi;
+      double& hm = hmax[size_t(L(i))];
+      if (hmaxi > hm && std::min(minimumDistance(S.exit, ri), minimumDistance(S.enter, ri)) < ext * hmaxi) hm = hmaxi;
     }
     std::vector<int64_t> ctrl;
-    if (hmax > 0.0) {
-      for (int64_t i = 0; i < n; ++i) {
-        const double disti = signedDistance(S.exit, X(i)) / hmax;
-        if (disti >= 0.0 && disti <= ext) ctrl.push_back(i);
-      }
+    for (int64_t i = 0; i < n; ++i) {
+      const double hm = hmax[size_t(L(i))];
+      if (!(hm > 0.0)) continue;
+      const double disti = signedDistance(S.exit, X(i)) / hm;
+      if (disti >= 0.0 && disti <= ext) ctrl.push_back(i);
     }
     if (n - nInt + int64_t(ctrl.size()) > maxGhost) return -1;   // later boundaries need these rows
     for (size_t k = 0; k < ctrl.size(); ++k) {
@@ -234,6 +245,7 @@ int64_t setGhosts(int64_t nInt, double* pos, double* H, int nb, const orc_bounda
       for (int d = 0; d < NS; ++d) H[g * NS + d] = hp[d];
       control[g - nInt] = int32_t(c);
       subOf[g - nInt] = int32_t(si);
+      if (list) list[g] = list[c];
     }
     n += int64_t(ctrl.size());
   }
@@ -348,6 +360,13 @@ int64_t orc_set_ghosts(int dim, int64_t nInt, double* pos, double* H, int nb, co
   if (dim == 1) return orc::setGhosts<1>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart);
   if (dim == 2) return orc::setGhosts<2>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart);
   return orc::setGhosts<3>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart);
+}
+
+int64_t orc_set_ghosts_lists(int dim, int64_t nInt, double* pos, double* H, int nb, const orc_boundary* b, double ext,
+                             int64_t maxGhost, int32_t* control, int32_t* subOf, int32_t* subStart, int32_t* list) {
+  if (dim == 1) return orc::setGhosts<1>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart, list);
+  if (dim == 2) return orc::setGhosts<2>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart, list);
+  return orc::setGhosts<3>(nInt, pos, H, nb, b, ext, maxGhost, control, subOf, subStart, list);
 }
 
 // SymTensor::eigenValues().minElement() of one packed SymTensor.

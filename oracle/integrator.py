@@ -101,8 +101,15 @@ class RK2:
 
     def __init__(self, dim, state, boundaries, gamma, params=None, table_points=1000, cfl=0.25, dtMin=0.0,
                  dtMax=FLT_MAX, dtGrowth=2.0, rhoMin=1.0e-10, rhoMax=1.0e10, hmin=1.0e-20, hmax=1.0e20,
-                 ext=2.0, cull=True, Pmin=-1.0e200, Pmax=1.0e200, gradh=False, compatible=False):
+                 ext=2.0, cull=True, Pmin=-1.0e200, Pmax=1.0e200, gradh=False, compatible=False,
+                 lists=None, gammas=None, sum_all=True):
+        """lists: NodeList of every internal row (several NodeLists, each a
+        contiguous block in NodeList order), gammas: each NodeList's gamma,
+        sum_all: sumMassDensityOverAllNodeLists."""
         self.dim = dim
+        self.lists = None if lists is None else np.asarray(lists, dtype=np.int32)
+        self.gammas = gammas
+        self.sum_all = sum_all
         self.n = state["pos"].shape[0]
         self.st = {k: np.array(v, dtype=np.float64) for k, v in state.items() if v is not None}
         self.st.setdefault("omega", np.ones(self.n))
@@ -117,7 +124,7 @@ class RK2:
         self.Pmin, self.Pmax = Pmin, Pmax
         self.gradh = gradh
         self.compatible = compatible
-        self.st["P"], self.st["cs"] = gamma_law_eos(self.st["rho"], self.st["eps"], gamma, Pmin, Pmax)
+        self.st["P"], self.st["cs"] = self._eos(self.st["rho"], self.st["eps"])
         self.t = 0.0
         self.lastDt = 1.0e-5
         self.cycle = 0
@@ -127,13 +134,26 @@ class RK2:
         self.nsym = nsym
 
     # -- pieces ----------------------------------------------------------------
+    def _eos(self, rho, eps):
+        """GammaLawGas of the internal rows (each NodeList its own gamma)."""
+        if self.lists is None or self.gammas is None:
+            return gamma_law_eos(rho, eps, self.gamma, self.Pmin, self.Pmax)
+        P, cs = np.zeros_like(rho), np.zeros_like(rho)
+        for L, g in enumerate(self.gammas):
+            sel = self.lists == L
+            P[sel], cs[sel] = gamma_law_eos(rho[sel], eps[sel], g, self.Pmin, self.Pmax)
+        return P, cs
+
     def _ghosts(self):
         if self.b:
-            full, off, nbr, info = oracle.ghost_pipeline(self.dim, self.st, self.b, self.ext, self.cull)
+            full, off, nbr, info = oracle.ghost_pipeline(self.dim, self.st, self.b, self.ext, self.cull,
+                                                         lists=self.lists)
         else:
             full = {k: v.copy() for k, v in self.st.items()}
             off, nbr = oracle.connectivity(self.dim, self.n, full["pos"], full["H"], self.ext)
             info = None
+            if self.lists is not None:
+                full["list"] = self.lists.copy()
         return full, off, nbr, info
 
     def _apply_ghosts(self, full, info):
@@ -163,7 +183,7 @@ class RK2:
         full["rho"][:n] = np.minimum(self.rhoMax, np.maximum(self.rhoMin, base["rho"] + mult * d["DrhoDt"]))
         hid = d["Hideal"].reshape(n, self.nsym)
         full["H"][:n] = np.minimum(self.hminInv, np.maximum(self.hmaxInv, hid)) * (hid != 0.0) + hid * (hid == 0.0)
-        full["P"][:n], full["cs"][:n] = gamma_law_eos(full["rho"][:n], full["eps"][:n], self.gamma, self.Pmin, self.Pmax)
+        full["P"][:n], full["cs"][:n] = self._eos(full["rho"][:n], full["eps"][:n])
 
     def _derivs(self, full, off, nbr, info=None):
         if self.gradh:
@@ -172,7 +192,8 @@ class RK2:
             if info is not None:
                 full["omega"] = oracle.apply_ghosts(self.dim, self.n, self.b, info["control"], info["sub_of"],
                                                     full["omega"], 0)
-        d = oracle.eval_derivs(self.T, self.p, self.n, full, off, nbr, pair_acc=self.compatible)
+        d = oracle.eval_derivs(self.T, self.p, self.n, full, off, nbr, pair_acc=self.compatible,
+                               lists=full.get("list"))
         if self.compatible:
             # SPHHydroBase::finalizeDerivatives: boundaries on DvDt and DepsDt
             n, dim = self.n, self.dim
@@ -221,7 +242,8 @@ class RK2:
         self._update(full, {k: v[:n] for k, v in state0.items()}, d2, dt)
         self._apply_ghosts(full, info)
         # SPHHydroBase::finalize: sum density with the step's connectivity
-        full["rho"][:n] = oracle.sum_density(self.T, n, full["pos"], full["mass"], full["H"], off, nbr)
+        full["rho"][:n] = oracle.sum_density(self.T, n, full["pos"], full["mass"], full["H"], off, nbr,
+                                             lists=full.get("list"), sum_all=self.sum_all)
         for k in ("pos", "vel", "mass", "rho", "eps", "H", "P", "cs", "omega"):
             self.st[k] = full[k][:n].copy()
         if self.b:

@@ -36,7 +36,7 @@ class Params(C.Structure):
 
 class State(C.Structure):
     _fields_ = [(n, _dp) for n in ("pos", "vel", "mass", "rho", "eps", "P", "cs", "H", "omega",
-                                   "fCl", "fCq", "fshear")]
+                                   "fCl", "fCq", "fshear")] + [("list", _i32p)]
 
 
 OUTPUT_FIELDS = ["rhoSum", "normalization", "DxDt", "DrhoDt", "DvDt", "DepsDt", "DvDx",
@@ -142,6 +142,11 @@ def lib(fast=False):
         L.orc_gamma_law.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     _dp, _dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_sum_density_lists.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int64, _dp, _dp, _dp,
+                                            _i64p, _i32p, _i32p, C.c_int, _dp]
+        L.orc_set_ghosts_lists.restype = C.c_int64
+        L.orc_set_ghosts_lists.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int, C.POINTER(Boundary), C.c_double,
+                                           C.c_int64, _i32p, _i32p, _i32p, _i32p]
         L.orc_set_ghosts.restype = C.c_int64
         L.orc_set_ghosts.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int, C.POINTER(Boundary), C.c_double,
                                      C.c_int64, _i32p, _i32p, _i32p]
@@ -216,14 +221,50 @@ def connectivity(dim, n_internal, pos, H, ext=2.0, brute=False, fast=False):
     return off, nbr[:total]
 
 
-def sum_density(table, n_internal, pos, mass, H, off, nbr, gather=False):
+def list_order(n_internal, lists, off, nbr, receive):
+    """Rows of several NodeLists in the reference's order.  lists: NodeList of
+    every node (internal rows of each NodeList contiguous and in NodeList
+    order, ghosts after all internal rows).  receive=False: ConnectivityMap's
+    order, by (NodeList, index within the NodeList) (the scatter walks it);
+    receive=True: the order in which the scatter delivers node i's terms, the
+    internal neighbours that own their pair (lower index) first, then i's own
+    pairs in ConnectivityMap order (the gather walks it)."""
+    lists = np.asarray(lists, dtype=np.int64)
+    n_tot = lists.shape[0]
+    idx = np.arange(n_tot)
+    # rank in (NodeList, index within the NodeList) order: a NodeList's ghosts
+    # follow its internal rows
+    ghost = idx >= n_internal
+    rank = np.lexsort((idx, ghost, lists))
+    pos_of = np.empty(n_tot, dtype=np.int64)
+    pos_of[rank] = idx
+    out = nbr.copy()
+    for i in range(n_internal):
+        row = nbr[off[i]:off[i + 1]].astype(np.int64)
+        key = pos_of[row]
+        if receive:
+            owned_by_j = (row < n_internal) & (row < i)
+            key = np.where(owned_by_j, row, n_tot + key)
+        out[off[i]:off[i + 1]] = row[np.argsort(key, kind="stable")]
+    return out
+
+
+def sum_density(table, n_internal, pos, mass, H, off, nbr, gather=False, lists=None, sum_all=True):
+    """computeSPHSumMassDensity restated.  lists (NodeList of every node) and
+    sum_all (sumMassDensityOverAllNodeLists) for several NodeLists; the rows
+    are reordered as the reference walks them."""
     L = table.L
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     rho = np.zeros(n_internal)
-    L.orc_sum_density(table.h, table.dim, int(gather), n_internal, pos.shape[0], _p(pos),
-                      _p(np.ascontiguousarray(mass, dtype=np.float64)),
-                      _p(np.ascontiguousarray(H, dtype=np.float64)), _p(off, _i64p),
-                      _p(nbr, _i32p), _p(rho))
+    args = (_p(np.ascontiguousarray(mass, dtype=np.float64)), _p(np.ascontiguousarray(H, dtype=np.float64)))
+    if lists is None:
+        L.orc_sum_density(table.h, table.dim, int(gather), n_internal, pos.shape[0], _p(pos), *args,
+                          _p(off, _i64p), _p(nbr, _i32p), _p(rho))
+        return rho
+    la = np.ascontiguousarray(lists, dtype=np.int32)
+    rows = np.ascontiguousarray(list_order(n_internal, la, off, nbr, receive=gather), dtype=np.int32)
+    L.orc_sum_density_lists(table.h, table.dim, int(gather), n_internal, pos.shape[0], _p(pos), *args,
+                            _p(off, _i64p), _p(rows, _i32p), _p(la, _i32p), int(bool(sum_all)), _p(rho))
     return rho
 
 
@@ -244,19 +285,30 @@ def make_params(**kw):
     return Params(**d)
 
 
-def eval_derivs(table, params, n_internal, state, off, nbr, gather=False, pair_acc=False):
+def eval_derivs(table, params, n_internal, state, off, nbr, gather=False, pair_acc=False, lists=None):
     """SPHHydroBase::evaluateDerivatives restated.  state: dict of numpy arrays
     (pos, vel, mass, rho, eps, P, cs, H, omega, optional fCl, fCq, fshear) over all
-    nodes.  Returns dict of per-internal-node outputs (+ 'ncalc' for scatter)."""
+    nodes.  Returns dict of per-internal-node outputs (+ 'ncalc' for scatter).
+    lists: NodeList of every node (several NodeLists); the rows are reordered
+    as the reference walks them."""
     dim = table.dim
     keep = {}
     st = State()
     for name, _ in State._fields_:
+        if name == "list":
+            continue
         a = state.get(name)
         if a is not None:
             a = np.ascontiguousarray(a, dtype=np.float64)
             keep[name] = a
         setattr(st, name, _p(a))
+    if lists is not None:
+        keep["list"] = np.ascontiguousarray(lists, dtype=np.int32)
+        st.list = _p(keep["list"], _i32p)
+        nbr = np.ascontiguousarray(list_order(n_internal, keep["list"], off, nbr, receive=gather), dtype=np.int32)
+        assert not pair_acc, "pairAccelerations with several NodeLists are not restated"
+    else:
+        st.list = C.cast(None, _i32p)
     outs = {f: np.zeros((n_internal, ncomp(f, dim))) for f in OUTPUT_FIELDS}
     ncalc = np.zeros(1, dtype=np.int64)
     pa = np.zeros((max(int(off[-1]), 1), dim)) if pair_acc else None
@@ -314,9 +366,11 @@ def n_sub(boundaries):
     return sum(1 if b.kind == 0 else 2 for b in boundaries)
 
 
-def set_ghosts(dim, pos, H, boundaries, ext=2.0):
+def set_ghosts(dim, pos, H, boundaries, ext=2.0, lists=None):
     """PlanarBoundary::setGhostNodes for each boundary in order over internal
-    rows pos/H.  Returns (pos_all, H_all, control, sub_of, sub_start)."""
+    rows pos/H.  Returns (pos_all, H_all, control, sub_of, sub_start), and with
+    lists (NodeList of every internal row: each NodeList selects its own ghosts)
+    also the NodeList of every row."""
     L = lib()
     nsym = dim * (dim + 1) // 2
     n = pos.shape[0]
@@ -330,10 +384,18 @@ def set_ghosts(dim, pos, H, boundaries, ext=2.0):
         control = np.zeros(cap, dtype=np.int32)
         sub_of = np.zeros(cap, dtype=np.int32)
         sub_start = np.zeros(n_sub(boundaries) + 1, dtype=np.int32)
-        ng = L.orc_set_ghosts(dim, n, _p(P), _p(Hh), len(boundaries), barr, ext, cap, _p(control, _i32p),
-                              _p(sub_of, _i32p), _p(sub_start, _i32p))
+        if lists is None:
+            ng = L.orc_set_ghosts(dim, n, _p(P), _p(Hh), len(boundaries), barr, ext, cap, _p(control, _i32p),
+                                  _p(sub_of, _i32p), _p(sub_start, _i32p))
+        else:
+            la = np.zeros(n + cap, dtype=np.int32)
+            la[:n] = lists
+            ng = L.orc_set_ghosts_lists(dim, n, _p(P), _p(Hh), len(boundaries), barr, ext, cap, _p(control, _i32p),
+                                        _p(sub_of, _i32p), _p(sub_start, _i32p), _p(la, _i32p))
         if ng >= 0:
-            return P[:n + ng], Hh[:n + ng], control[:ng], sub_of[:ng], sub_start
+            if lists is None:
+                return P[:n + ng], Hh[:n + ng], control[:ng], sub_of[:ng], sub_start
+            return P[:n + ng], Hh[:n + ng], control[:ng], sub_of[:ng], sub_start, la[:n + ng]
         cap *= 2
 
 
@@ -380,14 +442,21 @@ STATE_KINDS = {"vel": 1, "mass": 0, "rho": 0, "eps": 0, "P": 0, "cs": 0, "omega"
                "fCl": 0, "fCq": 0, "fshear": 0}
 
 
-def ghost_pipeline(dim, state, boundaries, ext=2.0, cull=True, fast=False):
+def ghost_pipeline(dim, state, boundaries, ext=2.0, cull=True, fast=False, lists=None):
     """Integrator::setGhostNodes + ConnectivityMap + culling + applyGhostBoundaries
     on internal-node state.  Returns (state_all, off, nbr, info) with ghosts
-    appended after the internal rows and rows in the culled numbering."""
+    appended after the internal rows and rows in the culled numbering.  With
+    lists (NodeList of every internal row), each NodeList makes its own ghosts
+    and state_all["list"] holds the NodeList of every row."""
     n = state["pos"].shape[0]
-    pos, H, control, sub_of, sub_start = set_ghosts(dim, state["pos"], state["H"], boundaries, ext)
+    if lists is None:
+        pos, H, control, sub_of, sub_start = set_ghosts(dim, state["pos"], state["H"], boundaries, ext)
+    else:
+        pos, H, control, sub_of, sub_start, la = set_ghosts(dim, state["pos"], state["H"], boundaries, ext, lists)
     off, nbr = connectivity(dim, n, pos, H, ext, fast=fast)
     out = {"pos": pos, "H": H}
+    if lists is not None:
+        out["list"] = la
     for k, kind in STATE_KINDS.items():
         if state.get(k) is None:
             continue

@@ -60,6 +60,8 @@ template <int D> struct Acc {
 
 template <int D> struct NodeState {
   const double *pos, *vel, *mass, *rho, *eps, *P, *cs, *H, *omega, *fCl, *fCq, *fshear;
+  const int32_t* list = nullptr;   // NodeList of every node (ghosts included), or null: one NodeList
+  bool sameList(int64_t i, int64_t j) const { return !list || list[i] == list[j]; }
   Vec<D> r(int64_t i) const { Vec<D> v; for (int k = 0; k < D; ++k) v.c[k] = pos[i * D + k]; return v; }
   Vec<D> v(int64_t i) const { Vec<D> w; for (int k = 0; k < D; ++k) w.c[k] = vel[i * D + k]; return w; }
   Sym<D> h(int64_t i) const { Sym<D> s; double* p = sptr(s); for (int k = 0; k < nsym<D>(); ++k) p[k] = H[i * nsym<D>() + k]; return s; }
@@ -150,10 +152,17 @@ int64_t connectivity(int64_t nInt, int64_t nTot, const double* pos, const double
 // ---------------------------------------------------------------------------
 // computeSPHSumMassDensity (scatter form, reference order)
 // ---------------------------------------------------------------------------
+//
+// Several NodeLists (list != null): the pairs are those of every list (the
+// rows hold every neighbour, in the reference's (NodeList, index) order), each
+// once by its owner as before; a pair across NodeLists counts only with
+// sumOverAllNodeLists, and then each end takes its own mass times the other
+// end's kernel (computeSPHSumMassDensity.cc:77-100).
 template <int D>
 void sumDensity(const Table& W, int64_t nInt, int64_t nTot, const double* pos, const double* mass, const double* H,
-                const int64_t* off, const int32_t* nbr, double* rhoOut) {
-  NodeState<D> s{}; s.pos = pos; s.H = H; s.mass = mass;
+                const int64_t* off, const int32_t* nbr, double* rhoOut, const int32_t* list = nullptr,
+                int sumAll = 1) {
+  NodeState<D> s{}; s.pos = pos; s.H = H; s.mass = mass; s.list = list;
   std::vector<double> rho(static_cast<size_t>(nTot), 0.0);
   for (int64_t i = 0; i < nInt; ++i) {
     const Vec<D> ri = s.r(i); const double mi = mass[i]; const Sym<D> Hi = s.h(i);
@@ -163,6 +172,8 @@ void sumDensity(const Table& W, int64_t nInt, int64_t nTot, const double* pos, c
     for (int64_t k = off[i]; k < off[i + 1]; ++k) {
       const int64_t j = nbr[k];
       if (!(j > i)) continue;
+      const bool same = s.sameList(i, j);
+      if (!same && !sumAll) continue;
       const Vec<D> rj = s.r(j); const double mj = mass[j]; const Sym<D> Hj = s.h(j);
       const double Hdetj = sdet(Hj);
       const Vec<D> rij = sub(ri, rj);
@@ -170,8 +181,8 @@ void sumDensity(const Table& W, int64_t nInt, int64_t nTot, const double* pos, c
       const double etaj = mag(sdot(Hj, rij));
       const double Wi = W.kernelValue(etai, Hdeti);
       const double Wj = W.kernelValue(etaj, Hdetj);
-      rho[i] += mj * Wj;
-      rho[j] += mi * Wi;
+      rho[i] += (same ? mj : mi) * Wj;
+      rho[j] += (same ? mi : mj) * Wi;
     }
   }
   for (int64_t i = 0; i < nInt; ++i) rhoOut[i] = rho[i];
@@ -213,10 +224,14 @@ void omegaGradh(const Table& W, int64_t nInt, int64_t nTot, const double* pos, c
 
 // Gather form: node i walks its row; the self term is inserted where the
 // reference adds it (after all j < i, before all j > i).
+// With several NodeLists the rows are in the order node i receives its terms
+// from the scatter: first the internal neighbours that own the pair (lower
+// index), then i's own pairs in (NodeList, index) order.
 template <int D>
 void sumDensityGather(const Table& W, int64_t nInt, const double* pos, const double* mass, const double* H,
-                      const int64_t* off, const int32_t* nbr, double* rhoOut) {
-  NodeState<D> s{}; s.pos = pos; s.H = H; s.mass = mass;
+                      const int64_t* off, const int32_t* nbr, double* rhoOut, const int32_t* list = nullptr,
+                      int sumAll = 1) {
+  NodeState<D> s{}; s.pos = pos; s.H = H; s.mass = mass; s.list = list;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < nInt; ++i) {
     const Vec<D> ri = s.r(i); const double mi = mass[i];
@@ -226,9 +241,11 @@ void sumDensityGather(const Table& W, int64_t nInt, const double* pos, const dou
     for (int64_t k = off[i]; k < off[i + 1]; ++k) {
       const int64_t j = nbr[k];
       if (!selfDone && j > i) { acc += self; selfDone = true; }
+      const bool same = s.sameList(i, j);
+      if (!same && !sumAll) continue;
       const Sym<D> Hj = s.h(j);
       const double etaj = mag(sdot(Hj, sub(ri, s.r(j))));
-      acc += mass[j] * W.kernelValue(etaj, sdet(Hj));
+      acc += (same ? mass[j] : mi) * W.kernelValue(etaj, sdet(Hj));
     }
     if (!selfDone) acc += self;
     rhoOut[i] = acc;
@@ -376,10 +393,12 @@ void evalScatter(const Table& W, const Params& p, int64_t nInt, int64_t nTot, co
       saddTo(ai.m2nd, sscale(thpt, fweightij * mag2(gradWi)));
       saddTo(aj.m2nd, sscale(thpt, 1.0 / fweightij * mag2(gradWj)));
 
-      ai.rhoSum += mj * Wi;
-      aj.rhoSum += mi * Wj;
-      ai.norm += mi / rhoi * Wi;
-      aj.norm += mj / rhoj * Wj;
+      if (s.sameList(i, j)) {     // SPHHydroBase.cc:746-751
+        ai.rhoSum += mj * Wi;
+        aj.rhoSum += mi * Wj;
+        ai.norm += mi / rhoi * Wi;
+        aj.norm += mj / rhoj * Wj;
+      }
 
       const Vec<D> vij = sub(vi, vj);
       double Cl = p.Cl, Cq = p.Cq;
@@ -491,8 +510,10 @@ void evalGather(const Table& W, const Params& p, int64_t nInt, const NodeState<D
       const Sym<D> thpt = sscale(selfdyad(rij), safeInvVar(rij2 * rij2 * rij2));
       a.wNS += std::fabs(gWi);
       saddTo(a.m2nd, sscale(thpt, mag2(gradWi)));
-      a.rhoSum += mj * Wi;
-      a.norm += mi / rhoi * Wi;
+      if (s.sameList(i, j)) {
+        a.rhoSum += mj * Wi;
+        a.norm += mi / rhoi * Wi;
+      }
 
       const Vec<D> vij = sub(vi, vj);
       double Cl = p.Cl, Cq = p.Cq;
@@ -536,12 +557,14 @@ void evalGather(const Table& W, const Params& p, int64_t nInt, const NodeState<D
 
 struct orc_state {
   const double *pos, *vel, *mass, *rho, *eps, *P, *cs, *H, *omega, *fCl, *fCq, *fshear;
+  const int32_t* list;      // NodeList of every node, or null (one NodeList)
 };
 
 template <int D> static NodeState<D> toNS(const orc_state* st) {
   NodeState<D> s;
   s.pos = st->pos; s.vel = st->vel; s.mass = st->mass; s.rho = st->rho; s.eps = st->eps; s.P = st->P;
   s.cs = st->cs; s.H = st->H; s.omega = st->omega; s.fCl = st->fCl; s.fCq = st->fCq; s.fshear = st->fshear;
+  s.list = st->list;
   return s;
 }
 
@@ -654,6 +677,23 @@ void orc_sum_density(void* tp, int dim, int gather, int64_t nInt, int64_t nTot, 
   if (dim == 1) sumDensity<1>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
   else if (dim == 2) sumDensity<2>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
   else sumDensity<3>(W, nInt, nTot, pos, mass, H, off, nbr, rho);
+}
+
+// Several NodeLists: list[node] for every node, sumAll = SPHHydroBase's
+// sumMassDensityOverAllNodeLists.
+void orc_sum_density_lists(void* tp, int dim, int gather, int64_t nInt, int64_t nTot, const double* pos,
+                           const double* mass, const double* H, const int64_t* off, const int32_t* nbr,
+                           const int32_t* list, int sumAll, double* rho) {
+  const Table& W = *static_cast<Table*>(tp);
+  if (gather) {
+    if (dim == 1) sumDensityGather<1>(W, nInt, pos, mass, H, off, nbr, rho, list, sumAll);
+    else if (dim == 2) sumDensityGather<2>(W, nInt, pos, mass, H, off, nbr, rho, list, sumAll);
+    else sumDensityGather<3>(W, nInt, pos, mass, H, off, nbr, rho, list, sumAll);
+    return;
+  }
+  if (dim == 1) sumDensity<1>(W, nInt, nTot, pos, mass, H, off, nbr, rho, list, sumAll);
+  else if (dim == 2) sumDensity<2>(W, nInt, nTot, pos, mass, H, off, nbr, rho, list, sumAll);
+  else sumDensity<3>(W, nInt, nTot, pos, mass, H, off, nbr, rho, list, sumAll);
 }
 
 double orc_ideal_h_pass(void* tp, int dim, const Params* p, int64_t nInt, const double* pos, double* H, double* deltaH,

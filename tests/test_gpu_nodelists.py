@@ -1,0 +1,182 @@
+"""Several NodeLists in one device node set against the oracle.
+
+Two NodeLists of different masses (and, in the time steps, different gamma)
+in a reflecting or periodic box: each NodeList makes its own ghosts, the
+neighbour rows come in the order the reference's scatter delivers each
+node's terms, rhoSum and normalization take same-NodeList pairs only, and
+the summed density takes a pair across NodeLists only with
+sumMassDensityOverAllNodeLists, each end with its own mass.  Bars as the
+one-NodeList parity tests: reference row order bitwise (3-D cube-root
+fields 1e-14), search row order 1e-12 relative to each field's largest
+magnitude."""
+import numpy as np
+import pytest
+
+from sphgpu import problems
+
+pytestmark = pytest.mark.gpu
+
+POW_FIELDS = ("weightedNeighborSum", "Hideal")
+STATE = {"pos": "position", "vel": "velocity", "mass": "mass", "rho": "mass density",
+         "eps": "specific thermal energy", "H": "H", "P": "pressure", "cs": "sound speed"}
+STEP_FIELDS = {"pos": "position", "vel": "velocity", "eps": "specific thermal energy", "rho": "mass density",
+               "H": "H", "P": "pressure", "cs": "sound speed"}
+
+
+def two_lists(dim, n, seed, mass_ratio, gamma=(1.4, 1.4)):
+    """NodeList 0 the nodes left of x = 0.5, NodeList 1 the others (heavier
+    by mass_ratio, with larger smoothing scales), in that order."""
+    st, _ = problems.random_state(dim, n, seed=seed, jitter=0.15, h_var=0.15, vel_scale=0.3)
+    st = {k: np.array(v, dtype=np.float64) for k, v in st.items()}
+    right = st["pos"][:, 0] > 0.5
+    order = np.concatenate([np.flatnonzero(~right), np.flatnonzero(right)])
+    st = {k: v[order] for k, v in st.items()}
+    counts = [int((~right).sum()), int(right.sum())]
+    lists = np.repeat(np.arange(2, dtype=np.int32), counts)
+    st["mass"] = st["mass"] * np.where(lists == 1, mass_ratio, 1.0)
+    st["H"][lists == 1] *= 0.85
+    P, cs = np.zeros(lists.size), np.zeros(lists.size)
+    for L, g in enumerate(gamma):
+        P[lists == L], cs[lists == L] = problems.gamma_law(st["rho"][lists == L], st["eps"][lists == L], g)
+    st.update(P=P, cs=cs, omega=np.ones(lists.size))
+    return st, lists, counts
+
+
+def device(sg, dim, st, counts, kind, order, gamma=None, sum_all=True):
+    N = st["pos"].shape[0]
+    s = sg.Sim(dim)
+    s.set_row_order(order)
+    s.set_table_bspline(1000)
+    s.set_nodes(N, N)
+    s.upload_state(st)
+    s.set_node_lists(counts, gamma, sum_all)
+    s.set_boundaries(sg.box_boundaries(dim, kind=kind))
+    return s
+
+
+def close(a, b, order, dim, f, tol=1e-12):
+    scale = max(np.abs(b).max(), 1e-300)
+    if order == 1:
+        return np.abs(a - b).max() <= tol * scale
+    if dim == 3 and f in POW_FIELDS:
+        return np.abs(a - b).max() <= 1e-14 * scale
+    return np.array_equal(a, b)
+
+
+CASES = [(1, 80, "reflecting"), (2, 14, "reflecting"), (2, 14, "periodic"), (3, 8, "reflecting"),
+         (3, 8, "periodic")]
+
+
+@pytest.mark.parametrize("sum_all", [True, False], ids=["sumall", "ownlist"])
+@pytest.mark.parametrize("order", [0, 1], ids=["refo", "searcho"])
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[2]}{c[0]}d" for c in CASES])
+def test_pass_with_nodelists(sphgpu, oracle, case, order, sum_all):
+    dim, n, kind = case
+    print(f"[nodelists-pass] dim={dim} n={n} {kind} order={order} sum_all={sum_all}", flush=True)
+    st, lists, counts = two_lists(dim, n, seed=70 + dim, mass_ratio=2.5)
+    N = lists.size
+    bo = oracle.box_boundaries(dim, kind=kind)
+    full, off, nbr, info = oracle.ghost_pipeline(dim, st, bo, 2.0, True, lists=lists)
+    s = device(sphgpu, dim, st, counts, kind, order, sum_all=sum_all)
+    s.set_ghost_nodes(2.0, cull=True)
+    n_int, n_tot = s.node_counts()
+    assert n_int == N and n_tot == full["pos"].shape[0]
+    for k, f in STATE.items():
+        assert np.array_equal(s.download(f).reshape(full[k].shape), full[k]), k
+    control, sub = s.ghost_controls()
+    assert np.array_equal(control, info["control"]) and np.array_equal(sub, info["sub_of"])
+    # the ConnectivityMap download: rows ascending in caller index, whatever
+    # order the device walks them in (the scatter's delivery order with
+    # reference row order: the bitwise sums below hold it to that)
+    o2, n2 = s.neighbors()
+    assert np.array_equal(o2, off) and np.array_equal(n2, nbr)
+    T = oracle.Table(dim, 1000)
+    rho = oracle.sum_density(T, N, full["pos"], full["mass"], full["H"], off, nbr, lists=full["list"],
+                             sum_all=sum_all)
+    s.sum_density()
+    rho_all = np.array(full["rho"])
+    rho_all[:N] = rho
+    rho_all = oracle.apply_ghosts(dim, N, bo, info["control"], info["sub_of"], rho_all, 0)
+    assert close(s.download("mass density"), rho_all, order, dim, "rho")
+    full["rho"] = rho_all
+    ref = oracle.eval_derivs(T, oracle.make_params(), N, full, off, nbr, lists=full["list"])
+    s.evaluate_derivatives(sphgpu.hydro_params())
+    for f in sphgpu.DERIV_ALIASES:
+        assert close(s.derivative(f).reshape(ref[f].shape), ref[f], order, dim, f), f
+
+
+def test_nodelist_rules_change_the_result(sphgpu):
+    """The NodeList rules act on the device: rhoSum and the summed density of
+    the same nodes differ between one and two NodeLists, and between the
+    two density rules."""
+    dim = 2
+    st, lists, counts = two_lists(dim, 14, seed=3, mass_ratio=3.0)
+    N = lists.size
+    out = {}
+    for name, c, sa in (("one", [N], True), ("all", counts, True), ("own", counts, False)):
+        s = device(sphgpu, dim, st, c, "reflecting", 0, sum_all=sa)
+        s.set_ghost_nodes(2.0, cull=True)
+        s.sum_density()
+        rho = s.download("mass density", rows=N)
+        s.evaluate_derivatives(sphgpu.hydro_params())
+        out[name] = (rho, s.derivative("rhoSum").copy())
+    assert not np.array_equal(out["one"][0], out["all"][0])
+    assert not np.array_equal(out["all"][0], out["own"][0])
+    assert not np.array_equal(out["one"][1], out["all"][1])
+    assert np.array_equal(out["all"][1], out["own"][1])   # rhoSum does not depend on the density rule
+
+
+STEP_CASES = [(1, 80, "reflecting", 5), (2, 14, "periodic", 4), (3, 7, "reflecting", 3)]
+
+
+@pytest.mark.parametrize("sum_all", [True, False], ids=["sumall", "ownlist"])
+@pytest.mark.parametrize("case", STEP_CASES, ids=[f"{c[2]}{c[0]}d" for c in STEP_CASES])
+def test_rk2_with_nodelists(sphgpu, oracle, case, sum_all):
+    """SynchronousRK2 over two NodeLists with their own gamma: the same dt
+    sequence and state as the oracle's integrator."""
+    import integrator
+    dim, n, kind, steps = case
+    print(f"[nodelists-rk2] dim={dim} n={n} {kind} sum_all={sum_all}", flush=True)
+    gammas = (1.4, 5.0 / 3.0)
+    st, lists, counts = two_lists(dim, n, seed=90 + dim, mass_ratio=2.0, gamma=gammas)
+    N = lists.size
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(dim, base, oracle.box_boundaries(dim, kind=kind), gammas[0], lists=lists,
+                         gammas=gammas, sum_all=sum_all)
+    s = device(sphgpu, dim, base, counts, kind, sphgpu.ROW_ORDER_REFERENCE, gamma=gammas, sum_all=sum_all)
+    s.eos_gamma_law(gammas[0])
+    hp = sphgpu.hydro_params()
+    ip = sphgpu.integrator_params(gamma=gammas[0])
+    for k in range(steps):
+        dref = ref.step()
+        ddev = s.rk2_step(hp, ip)
+        if dim < 3:
+            assert ddev == dref, (k, ddev, dref)
+        else:
+            assert ddev == pytest.approx(dref, rel=1e-12), (k, ddev, dref)
+    for key, f in STEP_FIELDS.items():
+        a = s.download(f, rows=N).reshape(ref.st[key].shape)
+        b = ref.st[key]
+        if dim < 3:
+            assert np.array_equal(a, b), key
+        else:
+            assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), key
+
+
+def test_nodelist_argument_errors(sphgpu):
+    dim = 2
+    st, lists, counts = two_lists(dim, 10, seed=5, mass_ratio=1.0)
+    N = lists.size
+    s = device(sphgpu, dim, st, counts, "reflecting", 0)
+    with pytest.raises(sphgpu.SphGpuError):
+        s.set_node_lists([N, 1])                 # sizes do not add up
+    with pytest.raises(sphgpu.SphGpuError):
+        s.set_node_lists([N - 1, 1], gamma=[1.4, 1.0])
+    s.set_ghost_nodes(2.0, cull=True)
+    with pytest.raises(sphgpu.SphGpuError):
+        s.set_node_lists(counts)                 # ghosts exist: internal rows only
+    m = device(sphgpu, dim, st, counts, "reflecting", 1)
+    m.set_precision(sphgpu.PRECISION_MIXED)
+    m.set_ghost_nodes(2.0, cull=True)
+    with pytest.raises(sphgpu.SphGpuError):
+        m.sum_density()                          # mixed precision takes one NodeList
