@@ -327,28 +327,46 @@ __global__ void k_iperm(const int32_t* __restrict__ perm, int32_t* __restrict__ 
 // Row writer: keeps four entries in registers and stores them as one int4.
 // Each entry is shifted in at w, so after every fourth the buffer holds them
 // oldest first with no branch on the position; the last partial chunk is
-// aligned once per row.  Entries past the row length are zero.
+// aligned once per row.  Entries past the row length are zero.  `at` walks
+// the row's chunks (ellChunk(c, n, k) = c n + k).
 struct RowWriter {
   int4 buf;
   int cnt;
   int cap;
-  int64_t n, k;
-  int4* out;
-  __device__ __forceinline__ RowWriter(int4* o, int64_t n_, int64_t k_, int cap_) : buf{0, 0, 0, 0}, cnt(0), cap(cap_), n(n_), k(k_), out(o) {}
+  int64_t n;
+  int4* at;
+  __device__ __forceinline__ RowWriter(int4* o, int64_t n_, int64_t k_, int cap_)
+      : buf{0, 0, 0, 0}, cnt(0), cap(cap_), n(n_), at(o + ellChunk(0, n_, k_)) {}
   __device__ __forceinline__ void push(int32_t v) {
     buf.x = buf.y;
     buf.y = buf.z;
     buf.z = buf.w;
     buf.w = v;
     ++cnt;
-    if ((cnt & 3) == 0 && cnt <= cap) out[ellChunk((cnt >> 2) - 1, n, k)] = buf;
+    if ((cnt & 3) == 0 && cnt <= cap) {
+      *at = buf;
+      at += n;
+    }
+  }
+  // push(v) when acc, with the entry shifted in by selects: only the chunk
+  // store is conditional
+  __device__ __forceinline__ void pushIf(bool acc, int32_t v) {
+    buf.x = acc ? buf.y : buf.x;
+    buf.y = acc ? buf.z : buf.y;
+    buf.z = acc ? buf.w : buf.z;
+    buf.w = acc ? v : buf.w;
+    cnt += acc ? 1 : 0;
+    if (acc && (cnt & 3) == 0 && cnt <= cap) {
+      *at = buf;
+      at += n;
+    }
   }
   __device__ __forceinline__ void finish() {
     const int r = cnt & 3;
     if (r == 0 || cnt > cap) return;
     const int4 o = r == 1 ? make_int4(buf.w, 0, 0, 0)
                           : (r == 2 ? make_int4(buf.z, buf.w, 0, 0) : make_int4(buf.y, buf.z, buf.w, 0));
-    out[ellChunk(cnt >> 2, n, k)] = o;
+    *at = o;
   }
 };
 
@@ -358,6 +376,7 @@ struct RowWriter {
 constexpr int kStageEdge = 8;                       // 4-cell brick + 2 cells each side
 constexpr int kStaged = kStageEdge * kStageEdge * kStageEdge;
 constexpr int kCandCap = 2048;
+constexpr int kRunRows = 29;
 
 struct SearchArgs {
   DGrid g;
@@ -388,6 +407,9 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
   __shared__ float4 sCand[kCandCap];               // x,y,z offsets from the brick origin, (1/h)^2 (FP32)
   __shared__ int32_t sSlot[kCandCap];
   __shared__ int32_t sc[kBlock];
+  // per lane: its runs (t0 | t1 << 16), lane-strided; up to 25 runs, three
+  // sentinels and the row the reader reads ahead of them
+  __shared__ uint32_t sRun[kRunRows * kBlock];
   const DGrid& g = a.g;
   const int b = int(xcdBlock(int64_t(g.nb[0]) * g.nb[1] * g.nb[2]));
   if (b >= g.nb[0] * g.nb[1] * g.nb[2]) return;   // uniform over the block
@@ -447,9 +469,9 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
   }
   __syncthreads();
 
-  // One lane per node; the node's 25 (dz, dy) runs of 5 staged cells are
-  // contiguous candidate ranges in LDS.  Rows leave as int4 chunks, one per
-  // lane, contiguous across a wavefront's slots.  Bricks whose candidates
+  // One lane per node; the node's 25 (dz, dy) runs of up to 5 staged cells
+  // are contiguous candidate ranges in LDS.  Rows leave as int4 chunks, one
+  // per lane, contiguous across a wavefront's slots.  Bricks whose candidates
   // exceed the stage read them from the records in global memory.
   for (int64_t k = h0 + threadIdx.x; k < h1; k += blockDim.x) {
     const double* ri = rec + k * R::STRIDE;
@@ -462,16 +484,17 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
     cellCoords<3>(g, xv, c);
     const int sx = c[0] - cx0, sy = c[1] - cy0, sz = c[2] - cz0;   // in [2, 5]
     RowWriter row(nbr, a.n, k, a.cap);
-    for (int dz = -2; dz <= 2; ++dz) {
-      // distance from the node to the cell slab along z (staged cell t spans
-      // [(t - 2) c, (t - 1) c) in brick-origin offsets)
-      const float zlo = float(sz + dz - 2) * a.cell32, zhi = zlo + a.cell32;
-      const float dzm = fz < zlo ? zlo - fz : (fz > zhi ? fz - zhi : 0.f);
-      for (int dy = -2; dy <= 2; ++dy) {
-        const int base = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge + (sx - 2);
-        if (staged) {
-          // the cells of this (dz, dy) run that a sphere of the search radius
-          // (with a 1e-4 margin) reaches: farther cells hold no neighbour
+    if (staged) {
+      // 1. the node's non-empty runs, in (dz, dy) order, into its column of
+      //    the run table: the cells of each (dz, dy) row of staged cells that
+      //    a sphere of the search radius (with a 1e-4 margin) reaches
+      int nr = 0, nCand = 0;
+      for (int dz = -2; dz <= 2; ++dz) {
+        // distance from the node to the cell slab along z (staged cell t spans
+        // [(t - 2) c, (t - 1) c) in brick-origin offsets)
+        const float zlo = float(sz + dz - 2) * a.cell32, zhi = zlo + a.cell32;
+        const float dzm = fz < zlo ? zlo - fz : (fz > zhi ? fz - zhi : 0.f);
+        for (int dy = -2; dy <= 2; ++dy) {
           const float ylo = float(sy + dy - 2) * a.cell32, yhi = ylo + a.cell32;
           const float dym = fy < ylo ? ylo - fy : (fy > yhi ? fy - yhi : 0.f);
           const float rem2 = a.prune2 - (dzm * dzm + dym * dym);
@@ -483,30 +506,87 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
           if (x0 > x1) continue;
           const int rowb = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge;
           const int32_t t0 = sOff[rowb + x0], t1 = sOff[rowb + x1 + 1];
-          for (int32_t t = t0; t < t1; ++t) {
-            const float4 cj = sCand[t];
-            const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
-            // FP32 estimate of min(h_i^2, h_j^2) |r|^2: fused products (any
-            // rounding change stays inside the band that takes the exact test);
-            // q min(a, b) = min(q a, q b) exactly for q >= 0
-            const float q = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-            const float tmin = q * fminf(hi2, cj.w);
-            bool in = tmin < a.lo32;
-            if (!in && tmin <= a.hi32) {
-              const int32_t sj = sSlot[t] & 0x7fffffff;
+          if (t1 > t0) {
+            sRun[nr++ * kBlock + threadIdx.x] = uint32_t(t0) | (uint32_t(t1) << 16);
+            nCand += t1 - t0;
+          }
+        }
+      }
+      // 2. one walk over the runs' candidates: a wavefront iterates as often
+      //    as its longest candidate list, not its longest run in every
+      //    (dz, dy) row.  Candidates are read two ahead of their test (three
+      //    register sets in rotation); the reader steps through the runs with
+      //    no branch and, past the last candidate, through three sentinel runs
+      //    (candidate 0), so every index it forms stays inside the tables.
+      //    The rare paths (the exact FP64 test, ghost marks) are entered only
+      //    when some lane of the wavefront needs them.
+      if (nr > 0) {
+        for (int q = 0; q < 3; ++q) sRun[(nr + q) * kBlock + threadIdx.x] = 0u;
+        const uint32_t first = sRun[threadIdx.x];
+        int32_t t = int32_t(first & 0xffffu), tEnd = int32_t(first >> 16);
+        uint32_t nxt = sRun[kBlock + threadIdx.x];
+        int r = 0;
+        auto advance = [&]() {
+          const bool same = t + 1 < tEnd;
+          r = same ? r : r + 1;
+          t = same ? t + 1 : int32_t(nxt & 0xffffu);
+          tEnd = same ? tEnd : int32_t(nxt >> 16);
+          nxt = sRun[(r + 1) * kBlock + threadIdx.x];
+        };
+        auto read = [&](float4& c, int32_t& raw) {
+          c = sCand[t];
+          raw = sSlot[t];
+        };
+        auto test = [&](const float4 cj, const int32_t raw) {
+          const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
+          // FP32 estimate of min(h_i^2, h_j^2) |r|^2: fused products (any
+          // rounding change stays inside the band that takes the exact test);
+          // q min(a, b) = min(q a, q b) exactly for q >= 0
+          const float q = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+          const float tmin = q * (cj.w < hi2 ? cj.w : hi2);
+          bool in = tmin < a.lo32;
+          const int32_t sj = raw & 0x7fffffff;
+          const bool band = !in && tmin <= a.hi32;
+          if (__ballot(band)) {
+            if (band) {
               const double* rj = rec + int64_t(sj) * R::STRIDE;
               in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
             }
-            if (in) {
-              const int32_t raw = sSlot[t];
-              const int32_t sj = raw & 0x7fffffff;
-              if (sj != int32_t(k)) {
-                row.push(sj);
-                if (raw < 0 && a.keepSlot) a.keepSlot[sj] = 1;
-              }
+          }
+          const bool acc = in && sj != int32_t(k);
+          row.pushIf(acc, sj);
+          if (a.keepSlot) {
+            const bool mark = acc && raw < 0;
+            if (__ballot(mark)) {
+              if (mark) a.keepSlot[sj] = 1;
             }
           }
-        } else {
+        };
+        float4 c0, c1, c2;
+        int32_t s0, s1, s2;
+        read(c0, s0);
+        advance();
+        read(c1, s1);
+        advance();
+        for (int left = nCand;;) {
+          read(c2, s2);
+          advance();
+          test(c0, s0);
+          if (--left == 0) break;
+          read(c0, s0);
+          advance();
+          test(c1, s1);
+          if (--left == 0) break;
+          read(c1, s1);
+          advance();
+          test(c2, s2);
+          if (--left == 0) break;
+        }
+      }
+    } else {
+      for (int dz = -2; dz <= 2; ++dz) {
+        for (int dy = -2; dy <= 2; ++dy) {
+          const int base = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge + (sx - 2);
           for (int dx = 0; dx < 5; ++dx) {
             const int cidx = base + dx;
             const int32_t st = sStart[cidx], ncell = sOff[cidx + 1] - sOff[cidx];
