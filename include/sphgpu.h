@@ -108,9 +108,12 @@ typedef struct sphgpu_hydro_params {
   int evolveTotalEnergy;       /* 0 */
   int linearInExpansion;       /* 0 */
   int quadraticInExpansion;    /* 0 */
-  int compatibleEnergyEvolution; /* 0; 1 (FP64 only): the pass keeps every row
-                                    entry's pair term for the RK2 step's
-                                    SpecificThermalEnergyPolicy */
+  int compatibleEnergyEvolution; /* 0; 1: the pass keeps every row entry's
+                                    pair term for the RK2 step's
+                                    SpecificThermalEnergyPolicy (FP64, or
+                                    mixed precision with the pair terms
+                                    computed in FP32, exact negatives at the
+                                    two ends of a pair) */
   int precision;               /* SPHGPU_PRECISION_* */
 } sphgpu_hydro_params;
 

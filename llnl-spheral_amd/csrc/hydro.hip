@@ -929,7 +929,13 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
       vi[c] = c < D ? Real(ri_[R::V + c]) : Real(0);
     }
     const double rho0 = ri_[R::RHO];
-    const Real hi = Real(ri_[RL<true>::HINV]), hdeti = Real(ri_[R::HDET]), ci = Real(ri_[R::CS]);
+    const Real hi = Real(ri_[RL<true>::HINV]), ci = Real(ri_[R::CS]);
+    // det H as a neighbour's record gives it (from the same 1/h), so that the
+    // two ends of a pair evaluate the same kernel gradient and their pair
+    // terms are exact negatives of each other (FP64: the packed det H is
+    // already that value)
+    const Real hdeti = std::is_same<Real, float>::value ? (D == 1 ? hi : (D == 2 ? hi * hi : hi * hi * hi))
+                                                        : Real(ri_[R::HDET]);
     const Real invRhoi = Real(1.0 / rho0), rhoi2 = Real(rho0 * rho0);
     const Real Prhoi = Real(ri_[R::PRHO]), mori = Real(ri_[R::MOR]);
     const Real Cl = Real(p.Cl), Cq = Real(p.Cq), eps2 = Real(p.eps2);
@@ -944,8 +950,8 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
     const int4* nb4 = reinterpret_cast<const int4*>(nbr);
     int4 cur = len > 0 ? nb4[ellChunk(0, n, k)] : make_int4(0, 0, 0, 0);
     int4 nxt = len > 4 ? nb4[ellChunk(1, n, k)] : make_int4(0, 0, 0, 0);
-    // One neighbour's contribution, in row order.
-    auto pair = [&](const INbr<Real>& nb, int32_t j) {
+    // One neighbour's contribution, in row order (w: its place in the row).
+    auto pair = [&](const INbr<Real>& nb, int32_t j, int w) {
       pairs += (nb.ghost || j > int32_t(k)) ? 1u : 0u;
       Real rij[3], vij[3];
       Real r2 = Real(0), vdotr = Real(0);
@@ -978,6 +984,11 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
       const Real fsr = mj * (pi + pj) * rinv;        // acceleration along r_ij
       const Real wg = mj * gi * rinv;               // grad W_i * m_j = r_ij * wg
       const Real f2 = gi * gi * rcpR(maxR(Real(1.0e-30), r2 * r2 * r2));
+      if (o.pairDv) {
+        // compatible energy: the own-frame pair term, DvDt_i += -m_j dv
+        const Real dsc = (pi + pj) * rinv;
+        for (int c = 0; c < D; ++c) o.pairDv[int64_t(c) * o.plane + ellIndex(w, n, k)] = double(rij[c] * dsc);
+      }
       if constexpr (F32) {
         for (int c = 0; c < D; ++c) af.DvDt[c] = fmaR(-rij[c], fsr, af.DvDt[c]);
         af.DepsDt = fmaR(mj * vdr, pi, af.DepsDt);
@@ -1049,7 +1060,7 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
         j = w == 0 ? cur.x : nextIndex(nb4, n, k, len, w, cur, nxt);
         loadIso<D>(recN, n, j, nb);
       }
-      pair(nb, j);
+      pair(nb, j, w);
       if constexpr (F32) {
         if ((w & (kSumBlock - 1)) == kSumBlock - 1) af.flushInto(a);
       }
@@ -1571,8 +1582,6 @@ void sumDensity(Sim& s) {
 void evalDerivs(Sim& s, const sphgpu_hydro_params& p, bool updateOnly) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "evaluateDerivatives needs a neighbour build");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
-  SG_REQUIRE(p.compatibleEnergyEvolution == 0 || p.precision == SPHGPU_PRECISION_FP64, SPHGPU_ERR_UNSUPPORTED,
-             "compatibleEnergyEvolution (pairAccelerations) needs the FP64 pass");
   SG_REQUIRE(!(p.compatibleEnergyEvolution && p.evolveTotalEnergy), SPHGPU_ERR_ARG,
              "compatibleEnergyEvolution and evolveTotalEnergy exclude each other");
   SG_REQUIRE(p.precision == SPHGPU_PRECISION_FP64 || p.precision == SPHGPU_PRECISION_MIXED, SPHGPU_ERR_ARG,

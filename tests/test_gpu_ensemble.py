@@ -131,7 +131,8 @@ def test_blast_members_match_plain_runs(sphgpu):
 def test_mixed_and_compatible_members(sphgpu):
     """Mixed precision and compatible energy take the same per-member path."""
     members = tubes(3, 80, 160, 0.01, seed=9)
-    for prec, kw in ((sphgpu.PRECISION_MIXED, {}), (sphgpu.PRECISION_FP64, dict(compatibleEnergyEvolution=1))):
+    for prec, kw in ((sphgpu.PRECISION_MIXED, {}), (sphgpu.PRECISION_FP64, dict(compatibleEnergyEvolution=1)),
+                     (sphgpu.PRECISION_MIXED, dict(compatibleEnergyEvolution=1))):
         print(f"[ensemble variant] precision={prec} {kw}", flush=True)
         hp = sphgpu.hydro_params(precision=prec, **kw)
         e = ensemble(sphgpu, 1, sphgpu.ROW_ORDER_SEARCH, precision=prec)

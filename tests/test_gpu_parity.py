@@ -180,8 +180,8 @@ def test_errors_are_reported(sphgpu):
     s.set_nodes(meta["n_internal"], meta["n_internal"])
     s.upload_state(st)
     s.build_neighbors(2.0)
-    with pytest.raises(sphgpu.SphGpuError) as e:   # pairAccelerations need the FP64 pass
-        s.evaluate_derivatives(sphgpu.hydro_params(compatibleEnergyEvolution=1, precision=sphgpu.PRECISION_MIXED))
+    with pytest.raises(sphgpu.SphGpuError) as e:   # the tensile term needs the FP64 pass
+        s.evaluate_derivatives(sphgpu.hydro_params(epsTensile=0.3, precision=sphgpu.PRECISION_MIXED))
     assert e.value.code == -4
     with pytest.raises(sphgpu.SphGpuError):
         s.derivative("DvDt")                # no valid derivatives after the failed call
