@@ -154,7 +154,59 @@ def test_rk2_with_gradh(sphgpu, oracle):
         assert np.array_equal(s.download(fields[key], rows=N).reshape(ref.st[key].shape), ref.st[key]), key
 
 
+GRADH = [c for c in CASES if c[0] in ("converging3d", "periodic2d", "reflect1d")]
+
+
+@pytest.mark.parametrize("case", GRADH, ids=[c[0] for c in GRADH])
+def test_rk2_with_gradh_mixed(sphgpu, oracle, case):
+    """Mixed precision with the grad-h correction: omega from the FP64 pass
+    (k_omega), carried into the pair terms through the records' P/(rho^2
+    omega); against the oracle's FP64 grad-h step, dt and every field within
+    1e-5 (L-infinity over the field's largest magnitude, and L2)."""
+    import integrator
+    name, dim, st, kind, gamma, steps = case
+    print(f"[rk2 gradh mixed] {name}", flush=True)
+    N = st["pos"].shape[0]
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(dim, base, oracle.box_boundaries(dim, kind=kind), gamma, gradh=True)
+    s = device(sphgpu, dim, st, kind, gamma, precision=sphgpu.PRECISION_MIXED)
+    hp = sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED)
+    ip = sphgpu.integrator_params(gamma=gamma, gradhCorrection=1)
+    for k in range(min(steps, 4)):
+        dref = ref.step()
+        ddev = s.rk2_step(hp, ip)
+        assert ddev == pytest.approx(dref, rel=1e-5), (k, ddev, dref)
+    fields = dict(FIELDS, omega="grad h corrections")
+    for key in ("pos", "vel", "eps", "rho", "H", "P", "cs", "omega"):
+        a = s.download(fields[key], rows=N).reshape(ref.st[key].shape)
+        b = ref.st[key]
+        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-300), key
+        assert np.linalg.norm(np.ravel(a - b)) <= 1e-5 * max(np.linalg.norm(np.ravel(b)), 1e-300), key
+
+
 COMPAT = [c for c in CASES if c[0] in ("converging3d", "periodic2d", "reflect1d")]
+
+
+def test_rk2_sph_defaults_mixed(sphgpu, oracle):
+    """The SPH() factory's defaults together in mixed precision (grad-h,
+    compatible energy, XSPH, corrected velocity gradient, sum density, ideal
+    H) on the 3-D converging flow: within 1e-5 of the oracle's FP64 step."""
+    import integrator
+    name, dim, st, kind, gamma, steps = [c for c in CASES if c[0] == "converging3d"][0]
+    N = st["pos"].shape[0]
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(dim, base, oracle.box_boundaries(dim, kind=kind), gamma, gradh=True, compatible=True)
+    s = device(sphgpu, dim, st, kind, gamma, precision=sphgpu.PRECISION_MIXED)
+    hp = sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED, compatibleEnergyEvolution=1)
+    ip = sphgpu.integrator_params(gamma=gamma, gradhCorrection=1)
+    for k in range(steps):
+        dref = ref.step()
+        ddev = s.rk2_step(hp, ip)
+        assert ddev == pytest.approx(dref, rel=1e-5), (k, ddev, dref)
+    for key, f in FIELDS.items():
+        a = s.download(f, rows=N).reshape(ref.st[key].shape)
+        b = ref.st[key]
+        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-300), key
 
 
 @pytest.mark.parametrize("case", COMPAT, ids=[c[0] for c in COMPAT])
