@@ -10,13 +10,15 @@
 //
 // The containers are minimal stand-ins with the reference's names: a NodeList
 // (FluidNodeList, src/NodeList/FluidNodeLists.py:12-35) owns its fields, State
-// and StateDerivatives hold named fields (the reference's FieldLists of one
-// NodeList) under the reference's names (src/Hydro/HydroFieldNames.cc:10-69)
+// and StateDerivatives hold named fields (the reference's FieldLists)
+// under the reference's names (src/Hydro/HydroFieldNames.cc:10-69)
 // with the "delta " / "new " prefixes of IncrementFieldList / ReplaceFieldList
 // (src/DataBase/IncrementFieldList.hh:48, ReplaceFieldList.hh:52).  A field is
 // a flat std::vector<double> in the reference's element layout: Vector = nDim,
 // SymTensor = nDim(nDim+1)/2 (xx, xy, .., yy, ..), Tensor = nDim^2 (row-major)
 // doubles per node; internal nodes first, then the NodeList's own ghost nodes.
+// With several NodeLists a field holds each NodeList's (internal) rows in turn,
+// in the DataBase's order.
 // Failed checks and device errors throw VERIFYError, as the reference's
 // VERIFY2 does (src/Utilities/DBC.hh:160-169).
 #pragma once
@@ -81,7 +83,7 @@ inline const std::string M_SPHCorrection = "M SPH gradient correction";
 inline const std::string IncrementPrefix = "delta ";  // IncrementFieldList::prefix()
 inline const std::string ReplacePrefix = "new ";      // ReplaceFieldList::prefix()
 
-// Named flat fields: the FieldLists of one NodeList.
+// Named flat fields: the FieldLists of the DataBase's NodeLists.
 class FieldStore {
  public:
   std::vector<double>& field(const std::string& name) {
@@ -208,23 +210,69 @@ class NodeList {
   std::vector<double> mMass, mPos, mVel, mH, mRho, mEps;
 };
 
-// DataBase with one fluid NodeList: each device context serves one NodeList.
+// DataBase of fluid NodeLists (src/DataBase/DataBase.hh:68-75, 161-173).  A
+// State field of several NodeLists is the concatenation of each NodeList's
+// rows in the order they were appended; with more than one NodeList the ghost
+// rows are the device's (the hydro's appended boundaries), so each NodeList
+// holds internal rows only.
 template <typename Dimension>
 class DataBase {
  public:
   void appendNodeList(NodeList<Dimension>& nl) {
-    VERIFY2(mNodeList == nullptr, "this DataBase holds one fluid NodeList");
-    mNodeList = &nl;
+    for (const NodeList<Dimension>* p : mNodeLists) VERIFY2(p != &nl, "NodeList already in the DataBase");
+    mNodeLists.push_back(&nl);
   }
-  int numNodeLists() const { return mNodeList ? 1 : 0; }
+  int numNodeLists() const { return int(mNodeLists.size()); }
   int numFluidNodeLists() const { return numNodeLists(); }
+  const std::vector<NodeList<Dimension>*>& nodeListPtrs() const { return mNodeLists; }
+  const std::vector<NodeList<Dimension>*>& fluidNodeListPtrs() const { return mNodeLists; }
+  // The first NodeList; with several, the one whose scalar settings (nPerh,
+  // hmin, hmax, hminratio, rhoMin, rhoMax, pressure limits) all of them share.
   NodeList<Dimension>& nodeList() const {
-    VERIFY2(mNodeList != nullptr, "no NodeList in the DataBase");
-    return *mNodeList;
+    VERIFY2(!mNodeLists.empty(), "no NodeList in the DataBase");
+    return *mNodeLists.front();
+  }
+  int64_t numInternalNodes() const {
+    int64_t n = 0;
+    for (const NodeList<Dimension>* p : mNodeLists) n += p->numInternalNodes();
+    return n;
+  }
+  int64_t numGhostNodes() const {
+    int64_t n = 0;
+    for (const NodeList<Dimension>* p : mNodeLists) n += p->numGhostNodes();
+    return n;
+  }
+  int64_t numNodes() const { return numInternalNodes() + numGhostNodes(); }
+
+  // The rows of one NodeList field (picked by `get`, `comps` doubles a row),
+  // concatenated over the NodeLists.
+  template <typename Get>
+  std::vector<double> gather(Get get, int comps) const {
+    std::vector<double> v;
+    v.reserve(size_t(numNodes()) * size_t(comps));
+    for (NodeList<Dimension>* p : mNodeLists) {
+      const std::vector<double>& f = get(*p);
+      v.insert(v.end(), f.begin(), f.begin() + std::ptrdiff_t(size_t(p->numNodes()) * size_t(comps)));
+    }
+    return v;
+  }
+  // The inverse of gather over the internal rows: each NodeList's internal
+  // rows from `v`, which holds the internal rows of all of them.
+  template <typename Get>
+  void scatter(Get get, int comps, const std::vector<double>& v) const {
+    VERIFY2(numGhostNodes() == 0 || mNodeLists.size() == 1, "several NodeLists hold internal rows only");
+    size_t at = 0;
+    for (NodeList<Dimension>* p : mNodeLists) {
+      std::vector<double>& f = get(*p);
+      const size_t k = size_t(p->numInternalNodes()) * size_t(comps);
+      VERIFY2(at + k <= v.size() && f.size() >= k, "field sizes do not match the NodeLists");
+      std::copy(v.begin() + std::ptrdiff_t(at), v.begin() + std::ptrdiff_t(at + k), f.begin());
+      at += k;
+    }
   }
 
  private:
-  NodeList<Dimension>* mNodeList = nullptr;
+  std::vector<NodeList<Dimension>*> mNodeLists;
 };
 
 template <typename Dimension> struct BSplineKernel {};
@@ -319,9 +367,9 @@ class SPHHydroBase {
                int device = 0, int precision = SPHGPU_PRECISION_FP64, int rowOrder = SPHGPU_ROW_ORDER_REFERENCE)
       : mQ(Q), mW(W), mCfl(cfl), mUseVelMag(useVelocityMagnitudeForDt), mCompatible(compatibleEnergyEvolution),
         mTotalEnergy(evolveTotalEnergy), mGradh(gradhCorrection), mXSPH(XSPH), mCorrectVG(correctVelocityGradient),
-        mDensityUpdate(densityUpdate), mEpsTensile(epsTensile), mPrecision(precision), mXmin(xmin), mXmax(xmax) {
+        mSumAll(sumMassDensityOverAllNodeLists), mDensityUpdate(densityUpdate), mEpsTensile(epsTensile),
+        mPrecision(precision), mXmin(xmin), mXmax(xmax) {
     (void)smoothingScaleMethod;
-    (void)sumMassDensityOverAllNodeLists;   // one NodeList: both choices sum the same nodes
     VERIFY2(WPi.numPoints() == W.numPoints(), "the device pass takes WPi = W");
     VERIFY2(filter == 0.0, "the device pass takes filter = 0");
     VERIFY2(nTensile == 4.0, "the device pass takes nTensile = 4");
@@ -366,19 +414,22 @@ class SPHHydroBase {
   size_t numBoundaries() const { return mBoundaries.size(); }
 
   // SPHHydroBase::initializeProblemStartup (src/SPH/SPHHydroBase.cc:162-200):
-  // the hydro's own pressure, sound speed (from the NodeList's equation of
+  // the hydro's own pressure, sound speed (from each NodeList's equation of
   // state) and grad-h correction (1) fields.
   virtual void initializeProblemStartup(DataBase<Dimension>& db) {
-    const NodeList<Dimension>& nl = db.nodeList();
-    const int64_t n = nl.numNodes();
+    const NodeList<Dimension>& nl = settings(db);
+    const int64_t n = db.numNodes();
     const auto& eos = nl.equationOfState();
     mPressure.assign(size_t(n), 0.0);
     mSoundSpeed.assign(size_t(n), 0.0);
     mOmegaGradh.assign(size_t(n), 1.0);
     // every row, ghosts included, goes through the equation of state
     check(sphgpu_set_nodes(mSim, n, n));
-    check(sphgpu_upload(mSim, SPHGPU_F_MASS_DENSITY, nl.massDensity().data(), n));
-    check(sphgpu_upload(mSim, SPHGPU_F_SPECIFIC_THERMAL_ENERGY, nl.specificThermalEnergy().data(), n));
+    const auto rho = db.gather([](NodeList<Dimension>& l) -> auto& { return l.massDensity(); }, 1);
+    const auto eps = db.gather([](NodeList<Dimension>& l) -> auto& { return l.specificThermalEnergy(); }, 1);
+    check(sphgpu_upload(mSim, SPHGPU_F_MASS_DENSITY, rho.data(), n));
+    check(sphgpu_upload(mSim, SPHGPU_F_SPECIFIC_THERMAL_ENERGY, eps.data(), n));
+    setLists(db);
     check(sphgpu_eos_gamma_law(mSim, eos.gamma(), eos.minimumPressure(), eos.maximumPressure(),
                                eos.minimumPressureType() == MaterialPressureMinType::PressureFloor ? 1 : 0));
     check(sphgpu_download(mSim, SPHGPU_F_PRESSURE, mPressure.data(), n));
@@ -386,17 +437,18 @@ class SPHHydroBase {
   }
 
   // SPHHydroBase::registerState (src/SPH/SPHHydroBase.cc:238-336): the
-  // NodeList's fields and the hydro's pressure, sound speed and grad-h
+  // NodeLists' fields and the hydro's pressure, sound speed and grad-h
   // correction, under their reference names.
   virtual void registerState(DataBase<Dimension>& db, State<Dimension>& state) {
-    const NodeList<Dimension>& nl = db.nodeList();
-    VERIFY2(mPressure.size() == size_t(nl.numNodes()), "initializeProblemStartup first");
-    state.enroll(HydroFieldNames::position, nl.positions());
-    state.enroll(HydroFieldNames::velocity, nl.velocity());
-    state.enroll(HydroFieldNames::mass, nl.mass());
-    state.enroll(HydroFieldNames::massDensity, nl.massDensity());
-    state.enroll(HydroFieldNames::specificThermalEnergy, nl.specificThermalEnergy());
-    state.enroll(HydroFieldNames::H, nl.Hfield());
+    using NL = NodeList<Dimension>;
+    VERIFY2(mPressure.size() == size_t(db.numNodes()), "initializeProblemStartup first");
+    state.enroll(HydroFieldNames::position, db.gather([](NL& l) -> auto& { return l.positions(); }, nDim));
+    state.enroll(HydroFieldNames::velocity, db.gather([](NL& l) -> auto& { return l.velocity(); }, nDim));
+    state.enroll(HydroFieldNames::mass, db.gather([](NL& l) -> auto& { return l.mass(); }, 1));
+    state.enroll(HydroFieldNames::massDensity, db.gather([](NL& l) -> auto& { return l.massDensity(); }, 1));
+    state.enroll(HydroFieldNames::specificThermalEnergy,
+                 db.gather([](NL& l) -> auto& { return l.specificThermalEnergy(); }, 1));
+    state.enroll(HydroFieldNames::H, db.gather([](NL& l) -> auto& { return l.Hfield(); }, Dimension::nSym));
     state.enroll(HydroFieldNames::pressure, mPressure);
     state.enroll(HydroFieldNames::soundSpeed, mSoundSpeed);
     state.enroll(HydroFieldNames::omegaGradh, mOmegaGradh);
@@ -405,7 +457,7 @@ class SPHHydroBase {
   // SPHHydroBase::registerDerivatives (src/SPH/SPHHydroBase.cc:367-425): the
   // fields evaluateDerivatives writes, one value per internal node.
   virtual void registerDerivatives(DataBase<Dimension>& db, StateDerivatives<Dimension>& derivs) {
-    const size_t n = size_t(db.nodeList().numInternalNodes());
+    const size_t n = size_t(db.numInternalNodes());
     for (const auto& d : derivFields()) derivs.enroll(d.name, n, derivComps(d.id));
   }
 
@@ -427,9 +479,9 @@ class SPHHydroBase {
   virtual void evaluateDerivatives(Scalar, Scalar, const DataBase<Dimension>& db, const State<Dimension>& state,
                                    StateDerivatives<Dimension>& derivs) const {
     pushState(db, state);
-    sphgpu_hydro_params p = params(db.nodeList());
+    sphgpu_hydro_params p = params(settings(db));
     check(sphgpu_evaluate_derivatives(mSim, &p));
-    const int64_t n = db.nodeList().numInternalNodes();
+    const int64_t n = db.numInternalNodes();
     for (const auto& d : derivFields()) {
       auto& f = derivs.enroll(d.name, size_t(n), derivComps(d.id));
       check(sphgpu_download_derivative(mSim, d.id, f.data(), n));
@@ -444,7 +496,7 @@ class SPHHydroBase {
   virtual TimeStepType dt(const DataBase<Dimension>& db, const State<Dimension>& state,
                           const StateDerivatives<Dimension>& derivs, Scalar) const {
     pushState(db, state);
-    const int64_t n = db.nodeList().numInternalNodes();
+    const int64_t n = db.numInternalNodes();
     const auto& mvp = derivs.field(HydroFieldNames::maxViscousPressure);
     const auto& DvDx = derivs.field(HydroFieldNames::velocityGradient);
     const auto& DvDt = derivs.field(HydroFieldNames::hydroAcceleration);
@@ -462,7 +514,7 @@ class SPHHydroBase {
     }
     check(sphgpu_set_dt_inputs(mSim, 0, n, in.data()));
     double v = 0.0;
-    check(sphgpu_hydro_dt(mSim, db.nodeList().nodesPerSmoothingScale(), mCfl, mUseVelMag ? 1 : 0, &v));
+    check(sphgpu_hydro_dt(mSim, settings(db).nodesPerSmoothingScale(), mCfl, mUseVelMag ? 1 : 0, &v));
     return {v, "SPHHydroBase: minimum over the nodes of the sound-speed, viscous, divergence, pairwise-velocity "
                "and acceleration limits"};
   }
@@ -490,6 +542,7 @@ class SPHHydroBase {
   bool gradhCorrection() const { return mGradh; }
   bool XSPH() const { return mXSPH; }
   bool correctVelocityGradient() const { return mCorrectVG; }
+  bool sumMassDensityOverAllNodeLists() const { return mSumAll; }
   MassDensityType densityUpdate() const { return mDensityUpdate; }
   HEvolutionType HEvolution() const { return HEvolutionType::IdealH; }
   double epsilonTensile() const { return mEpsTensile; }
@@ -555,14 +608,49 @@ class SPHHydroBase {
     return p;
   }
 
+  // The NodeList whose scalar settings the device takes: with several
+  // NodeLists all of them agree on nPerh, hmin, hmax, hminratio, rhoMin, rhoMax
+  // and the pressure limits (the device holds one of each), and hold internal
+  // rows only (their ghosts are the device's, from the appended boundaries).
+  static const NodeList<Dimension>& settings(const DataBase<Dimension>& db) {
+    const NodeList<Dimension>& a = db.nodeList();
+    const auto& ea = a.equationOfState();
+    for (const NodeList<Dimension>* p : db.nodeListPtrs()) {
+      if (p == &a) continue;
+      const auto& e = p->equationOfState();
+      VERIFY2(a.numGhostNodes() == 0 && p->numGhostNodes() == 0,
+              "with several NodeLists the ghost nodes are the hydro boundaries' (none in the NodeLists)");
+      VERIFY2(p->nodesPerSmoothingScale() == a.nodesPerSmoothingScale() && p->hmin() == a.hmin() &&
+                  p->hmax() == a.hmax() && p->hminratio() == a.hminratio() && p->rhoMin() == a.rhoMin() &&
+                  p->rhoMax() == a.rhoMax(),
+              "the device takes one nPerh, hmin, hmax, hminratio, rhoMin and rhoMax for all NodeLists");
+      VERIFY2(e.minimumPressure() == ea.minimumPressure() && e.maximumPressure() == ea.maximumPressure() &&
+                  e.minimumPressureType() == ea.minimumPressureType(),
+              "the device takes one pressure range and floor type for all NodeLists");
+    }
+    return a;
+  }
+  // Each NodeList's internal rows and gamma to the device (after
+  // sphgpu_set_nodes, which returns it to one NodeList, and before the ghosts).
+  void setLists(const DataBase<Dimension>& db) const {
+    if (db.numNodeLists() == 1) return;
+    std::vector<int64_t> counts;
+    std::vector<double> gammas;
+    for (const NodeList<Dimension>* p : db.nodeListPtrs()) {
+      counts.push_back(p->numInternalNodes());
+      gammas.push_back(p->equationOfState().gamma());
+    }
+    check(sphgpu_set_node_lists(mSim, int(counts.size()), counts.data(), gammas.data(), mSumAll ? 1 : 0));
+  }
+
   // The state's fields to the device, then the connectivity.  With appended
   // boundaries the device makes the ghost rows (and culls the unused ones, as
   // the Integrator does by default); otherwise the NodeList's ghost rows are
   // taken as they are.
   void pushState(const DataBase<Dimension>& db, const State<Dimension>& state) const {
-    const NodeList<Dimension>& nl = db.nodeList();
-    const int64_t nInt = nl.numInternalNodes();
-    const int64_t rows = mBoundaries.empty() ? nl.numNodes() : nInt;
+    settings(db);
+    const int64_t nInt = db.numInternalNodes();
+    const int64_t rows = mBoundaries.empty() ? db.numNodes() : nInt;
     check(sphgpu_set_nodes(mSim, nInt, rows));
     struct F { const std::string* name; int id; int comps; };
     const F fields[] = {
@@ -580,13 +668,14 @@ class SPHHydroBase {
       VERIFY2(v.size() >= size_t(rows) * size_t(f.comps), "state field \"" + *f.name + "\" is too short");
       check(sphgpu_upload(mSim, f.id, v.data(), rows));
     }
+    setLists(db);
     if (!mBoundaries.empty()) check(sphgpu_set_ghost_nodes(mSim, mW.kernelExtent(), 1));
     else check(sphgpu_build_neighbors(mSim, mW.kernelExtent()));
   }
   // The internal rows of one resident field into the state (its ghost rows stay).
   void pullInternal(const DataBase<Dimension>& db, State<Dimension>& state, const std::string& name, int id,
                     int comps) const {
-    const int64_t nInt = db.nodeList().numInternalNodes();
+    const int64_t nInt = db.numInternalNodes();
     auto& v = state.field(name);
     VERIFY2(v.size() >= size_t(nInt) * size_t(comps), "state field \"" + name + "\" is too short");
     check(sphgpu_download(mSim, id, v.data(), nInt));
@@ -602,7 +691,7 @@ class SPHHydroBase {
   MonaghanGingoldViscosity<Dimension> mQ;
   TableKernel<Dimension> mW;
   double mCfl;
-  bool mUseVelMag, mCompatible, mTotalEnergy, mGradh, mXSPH, mCorrectVG;
+  bool mUseVelMag, mCompatible, mTotalEnergy, mGradh, mXSPH, mCorrectVG, mSumAll;
   MassDensityType mDensityUpdate;
   double mEpsTensile;
   int mPrecision;
@@ -623,6 +712,7 @@ template <typename Dimension>
 class SynchronousRK2 {
  public:
   using Scalar = double;
+  static constexpr int nDim = Dimension::nDim;
 
   SynchronousRK2(DataBase<Dimension>& dataBase, const std::vector<SPHHydroBase<Dimension>*>& physicsPackages)
       : mDB(dataBase) {
@@ -669,7 +759,7 @@ class SynchronousRK2 {
   void check(int rc) const { mHydro->check(rc); }
 
   sphgpu_integrator_params integratorParams() const {
-    const NodeList<Dimension>& nl = mDB.nodeList();
+    const NodeList<Dimension>& nl = SPHHydroBase<Dimension>::settings(mDB);
     const auto& eos = nl.equationOfState();
     sphgpu_integrator_params ip{};
     ip.cfl = mHydro->mCfl;
@@ -684,22 +774,22 @@ class SynchronousRK2 {
     return ip;
   }
 
+  // The NodeLists' fields, concatenated in the DataBase's order, with the
+  // hydro's pressure, sound speed and grad-h correction.
   void pushState() {
-    NodeList<Dimension>& nl = mDB.nodeList();
-    VERIFY2(nl.numGhostNodes() == 0, "the device integrator makes the ghost nodes of the hydro's boundaries");
-    VERIFY2(mHydro->mPressure.size() == size_t(nl.numNodes()), "initializeProblemStartup first");
+    VERIFY2(mDB.numGhostNodes() == 0, "the device integrator makes the ghost nodes of the hydro's boundaries");
+    VERIFY2(mHydro->mPressure.size() == size_t(mDB.numNodes()), "initializeProblemStartup first");
+    SPHHydroBase<Dimension>::settings(mDB);
     sphgpu_sim* s = mHydro->mSim;
-    const int64_t n = nl.numInternalNodes();
+    const int64_t n = mDB.numInternalNodes();
     check(sphgpu_set_nodes(s, n, n));
-    check(sphgpu_upload(s, SPHGPU_F_POSITION, nl.positions().data(), n));
-    check(sphgpu_upload(s, SPHGPU_F_VELOCITY, nl.velocity().data(), n));
-    check(sphgpu_upload(s, SPHGPU_F_MASS, nl.mass().data(), n));
-    check(sphgpu_upload(s, SPHGPU_F_MASS_DENSITY, nl.massDensity().data(), n));
-    check(sphgpu_upload(s, SPHGPU_F_SPECIFIC_THERMAL_ENERGY, nl.specificThermalEnergy().data(), n));
-    check(sphgpu_upload(s, SPHGPU_F_H, nl.Hfield().data(), n));
+    for (const NodeField& f : nodeFields()) check(sphgpu_upload(s, f.id, mDB.gather(f.get, f.comps).data(), n));
+    const auto mass = mDB.gather([](NodeList<Dimension>& l) -> auto& { return l.mass(); }, 1);
+    check(sphgpu_upload(s, SPHGPU_F_MASS, mass.data(), n));
     check(sphgpu_upload(s, SPHGPU_F_PRESSURE, mHydro->mPressure.data(), n));
     check(sphgpu_upload(s, SPHGPU_F_SOUND_SPEED, mHydro->mSoundSpeed.data(), n));
     check(sphgpu_upload(s, SPHGPU_F_OMEGA_GRADH, mHydro->mOmegaGradh.data(), n));
+    mHydro->setLists(mDB);
     syncClock();
     mResident = true;
   }
@@ -707,26 +797,44 @@ class SynchronousRK2 {
 
   void stepDevice(Scalar maxTime) {
     if (!mResident) pushState();
-    const sphgpu_hydro_params hp = mHydro->params(mDB.nodeList());
+    const sphgpu_hydro_params hp = mHydro->params(SPHHydroBase<Dimension>::settings(mDB));
     const sphgpu_integrator_params ip = integratorParams();
     double dt = 0.0;
     check(sphgpu_rk2_step(mHydro->mSim, &hp, &ip, maxTime, &dt));
     check(sphgpu_get_time(mHydro->mSim, &mTime, &mLastDt, &mCycle));
   }
 
+  // The evolved fields back into each NodeList (its internal rows), and the
+  // hydro's pressure, sound speed and grad-h correction.
   void pullState() {
     if (!mResident) return;
-    NodeList<Dimension>& nl = mDB.nodeList();
     sphgpu_sim* s = mHydro->mSim;
-    const int64_t n = nl.numInternalNodes();
-    check(sphgpu_download(s, SPHGPU_F_POSITION, nl.positions().data(), n));
-    check(sphgpu_download(s, SPHGPU_F_VELOCITY, nl.velocity().data(), n));
-    check(sphgpu_download(s, SPHGPU_F_MASS_DENSITY, nl.massDensity().data(), n));
-    check(sphgpu_download(s, SPHGPU_F_SPECIFIC_THERMAL_ENERGY, nl.specificThermalEnergy().data(), n));
-    check(sphgpu_download(s, SPHGPU_F_H, nl.Hfield().data(), n));
+    const int64_t n = mDB.numInternalNodes();
+    std::vector<double> v;
+    for (const NodeField& f : nodeFields()) {
+      v.resize(size_t(n) * size_t(f.comps));
+      check(sphgpu_download(s, f.id, v.data(), n));
+      mDB.scatter(f.get, f.comps, v);
+    }
     check(sphgpu_download(s, SPHGPU_F_PRESSURE, mHydro->mPressure.data(), n));
     check(sphgpu_download(s, SPHGPU_F_SOUND_SPEED, mHydro->mSoundSpeed.data(), n));
     check(sphgpu_download(s, SPHGPU_F_OMEGA_GRADH, mHydro->mOmegaGradh.data(), n));
+  }
+
+  // The NodeList fields the step evolves (the mass stays).
+  struct NodeField {
+    int id, comps;
+    std::vector<double>& (*get)(NodeList<Dimension>&);
+  };
+  static const std::vector<NodeField>& nodeFields() {
+    using NL = NodeList<Dimension>;
+    static const std::vector<NodeField> f = {
+        {SPHGPU_F_POSITION, nDim, [](NL& l) -> std::vector<double>& { return l.positions(); }},
+        {SPHGPU_F_VELOCITY, nDim, [](NL& l) -> std::vector<double>& { return l.velocity(); }},
+        {SPHGPU_F_MASS_DENSITY, 1, [](NL& l) -> std::vector<double>& { return l.massDensity(); }},
+        {SPHGPU_F_SPECIFIC_THERMAL_ENERGY, 1, [](NL& l) -> std::vector<double>& { return l.specificThermalEnergy(); }},
+        {SPHGPU_F_H, Dimension::nSym, [](NL& l) -> std::vector<double>& { return l.Hfield(); }}};
+    return f;
   }
 
   DataBase<Dimension>& mDB;

@@ -12,8 +12,12 @@
 // P, cs (derivs: replace the equation of state's values in the State),
 // optional boundaries (13 doubles each: kind, plane1 point(3), normal(3),
 // plane2 point(3), normal(3)), options epsTensile, gradh, compatible, steps.
+// Several NodeLists: list_counts (internal nodes of each, in row order), gammas
+// (one each), optional nperh (one each, default 1.51) and sum_all
+// (sumMassDensityOverAllNodeLists, default 1).
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <iostream>
 #include <map>
@@ -79,8 +83,7 @@ static typename Dimension::Vector vec(const double* p) {
 
 template <typename Dimension>
 struct Problem {
-  GammaLawGas<Dimension> eos;
-  NodeList<Dimension> nodes;
+  std::deque<NodeList<Dimension>> lists;   // stable addresses for the DataBase
   DataBase<Dimension> db;
   TableKernel<Dimension> W;
   MonaghanGingoldViscosity<Dimension> Q;
@@ -89,28 +92,42 @@ struct Problem {
   // The SPH() defaults for the B-spline kernel (src/SPH/SPHHydros.py:317-384):
   // cfl 0.25, XSPH, correctVelocityGradient, RigorousSumDensity, IdealH.
   Problem(const Records& in, int64_t nGhost)
-      : eos(scalar(in, "gamma", 5.0 / 3.0), 1.0),
-        nodes("nodes", eos, int64_t(scalar(in, "n_internal", 0)), nGhost, 1.0e-20, 1.0e20, 0.1, 1.51),
-        W(BSplineKernel<Dimension>(), 1000),
+      : W(BSplineKernel<Dimension>(), 1000),
         Q(1.0, 1.0),
         hydro(SPHSmoothingScale<Dimension>(), Q, W, W, 0.0, 0.25, false, scalar(in, "compatible", 0) != 0.0, false,
-              scalar(in, "gradh", 0) != 0.0, true, true, false, MassDensityType::RigorousSumDensity,
-              HEvolutionType::IdealH, scalar(in, "epsTensile", 0.0), 4.0, typename Dimension::Vector{},
-              typename Dimension::Vector{}) {
-    const size_t n = size_t(nodes.numNodes());
+              scalar(in, "gradh", 0) != 0.0, true, true, scalar(in, "sum_all", 1) != 0.0,
+              MassDensityType::RigorousSumDensity, HEvolutionType::IdealH, scalar(in, "epsTensile", 0.0), 4.0,
+              typename Dimension::Vector{}, typename Dimension::Vector{}) {
+    const std::vector<double> counts =
+        in.count("list_counts") ? array(in, "list_counts") : std::vector<double>{scalar(in, "n_internal", 0)};
+    const std::vector<double> gammas =
+        in.count("gammas") ? array(in, "gammas") : std::vector<double>{scalar(in, "gamma", 5.0 / 3.0)};
+    const std::vector<double> nPerh =
+        in.count("nperh") ? array(in, "nperh") : std::vector<double>(counts.size(), 1.51);
+    VERIFY2(counts.size() == gammas.size() && counts.size() == nPerh.size(), "one gamma and nPerh per NodeList");
     const int D = Dimension::nDim, S = Dimension::nSym;
-    auto take = [&](const char* key, std::vector<double>& dst, int comps) {
-      const auto& src = array(in, key);
-      VERIFY2(src.size() >= n * size_t(comps), std::string("input too short: ") + key);
-      std::copy(src.begin(), src.begin() + std::ptrdiff_t(n * size_t(comps)), dst.begin());
-    };
-    take("pos", nodes.positions(), D);
-    take("vel", nodes.velocity(), D);
-    take("mass", nodes.mass(), 1);
-    take("rho", nodes.massDensity(), 1);
-    take("eps", nodes.specificThermalEnergy(), 1);
-    take("H", nodes.Hfield(), S);
-    db.appendNodeList(nodes);
+    size_t at = 0;   // first row of the NodeList in the input arrays
+    for (size_t k = 0; k < counts.size(); ++k) {
+      // the caller's ghost rows (no boundaries) belong to the last NodeList
+      lists.emplace_back("nodes" + std::to_string(k), GammaLawGas<Dimension>(gammas[k], 1.0), int64_t(counts[k]),
+                         k + 1 == counts.size() ? nGhost : 0, 1.0e-20, 1.0e20, 0.1, nPerh[k]);
+      NodeList<Dimension>& nl = lists.back();
+      const size_t n = size_t(nl.numNodes());
+      auto take = [&](const char* key, std::vector<double>& dst, int comps) {
+        const auto& src = array(in, key);
+        VERIFY2(src.size() >= (at + n) * size_t(comps), std::string("input too short: ") + key);
+        std::copy(src.begin() + std::ptrdiff_t(at * size_t(comps)),
+                  src.begin() + std::ptrdiff_t((at + n) * size_t(comps)), dst.begin());
+      };
+      take("pos", nl.positions(), D);
+      take("vel", nl.velocity(), D);
+      take("mass", nl.mass(), 1);
+      take("rho", nl.massDensity(), 1);
+      take("eps", nl.specificThermalEnergy(), 1);
+      take("H", nl.Hfield(), S);
+      db.appendNodeList(nl);
+      at += n;
+    }
     auto b = in.find("boundaries");
     if (b != in.end()) {
       VERIFY2(b->second.size() % 13 == 0, "boundaries: 13 doubles each");
@@ -173,11 +190,12 @@ static Records runRK2(const Records& in) {
   VERIFY2(integrator.currentCycle() == steps, "cycle count");
   out["dt"] = dts;
   out["time"] = {integrator.currentTime()};
-  out["pos"] = pb.nodes.positions();
-  out["vel"] = pb.nodes.velocity();
-  out["rho"] = pb.nodes.massDensity();
-  out["eps"] = pb.nodes.specificThermalEnergy();
-  out["H"] = pb.nodes.Hfield();
+  using NL = NodeList<Dimension>;
+  out["pos"] = pb.db.gather([](NL& l) -> auto& { return l.positions(); }, Dimension::nDim);
+  out["vel"] = pb.db.gather([](NL& l) -> auto& { return l.velocity(); }, Dimension::nDim);
+  out["rho"] = pb.db.gather([](NL& l) -> auto& { return l.massDensity(); }, 1);
+  out["eps"] = pb.db.gather([](NL& l) -> auto& { return l.specificThermalEnergy(); }, 1);
+  out["H"] = pb.db.gather([](NL& l) -> auto& { return l.Hfield(); }, Dimension::nSym);
   out["P"] = pb.hydro.pressure();
   out["cs"] = pb.hydro.soundSpeed();
   return out;

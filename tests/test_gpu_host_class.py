@@ -205,3 +205,116 @@ def test_host_class_synchronous_rk2(oracle, case, tmp_path):
             assert np.array_equal(a, b), key
         else:
             assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), key
+
+
+def two_lists(dim, n, seed):
+    """NodeList 0 the nodes left of x = 0.5, NodeList 1 (twice as heavy) the
+    others, in that order."""
+    st, _ = problems.random_state(dim, n, seed=seed, jitter=0.15, h_var=0.15, vel_scale=0.3)
+    st = {k: np.array(v, dtype=np.float64) for k, v in st.items()}
+    right = st["pos"][:, 0] > 0.5
+    order = np.concatenate([np.flatnonzero(~right), np.flatnonzero(right)])
+    st = {k: v[order] for k, v in st.items()}
+    counts = [int((~right).sum()), int(right.sum())]
+    lists = np.repeat(np.arange(2, dtype=np.int32), counts)
+    st["mass"] = st["mass"] * np.where(lists == 1, 2.0, 1.0)
+    st["H"][lists == 1] *= 0.85
+    return st, lists, counts
+
+
+NODELIST_CASES = [(1, 80, "reflecting", True), (2, 14, "periodic", False), (3, 8, "reflecting", True)]
+
+
+@pytest.mark.parametrize("case", NODELIST_CASES, ids=[f"{c[2]}{c[0]}d_{'sumall' if c[3] else 'ownlist'}"
+                                                      for c in NODELIST_CASES])
+def test_host_class_nodelists_derivatives(sphgpu, oracle, case, tmp_path):
+    """Two NodeLists in one DataBase, each with its own gamma-law gas: the
+    hydro's equation of state, derivatives, dt and finalized density against
+    the oracle's several-NodeList pass."""
+    import integrator
+    dim, n, kind, sum_all = case
+    gammas = (1.4, 5.0 / 3.0)
+    st, lists, counts = two_lists(dim, n, 40 + dim)
+    N = lists.size
+    print(f"[host class] nodelists derivs dim={dim} {kind} sum_all={sum_all}", flush=True)
+    rec = state_records(dim, st, N, N, list_counts=counts, gammas=list(gammas), sum_all=int(sum_all),
+                        boundaries=box_records(dim, kind))
+    got = run_host("derivs", rec, tmp_path)
+
+    # each NodeList's own equation of state; the State then holds these values
+    base = {k: st[k].copy() for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    base["P"], base["cs"] = np.zeros(N), np.zeros(N)
+    for L, g in enumerate(gammas):
+        sel = lists == L
+        base["P"][sel], base["cs"][sel] = oracle.gamma_law(st["rho"][sel], st["eps"][sel], g, floor=True)
+    assert np.array_equal(got["eos P"], base["P"]) and np.array_equal(got["eos cs"], base["cs"])
+
+    T = oracle.Table(dim, 1000)
+    full, off, nbr, info = oracle.ghost_pipeline(dim, base, oracle.box_boundaries(dim, kind=kind), 2.0, True,
+                                                 lists=lists)
+    full["omega"] = np.ones(full["pos"].shape[0])
+    ref = oracle.eval_derivs(T, oracle.make_params(), N, full, off, nbr, lists=full["list"])
+    bad = []
+    for cname, f in zip(sphgpu.DERIV_FIELDS, oracle.OUTPUT_FIELDS):
+        a = got["derivs/" + cname]
+        b = ref[f].ravel()
+        if not close(f, dim, a, b):
+            bad.append((f, float(np.abs(a - b).max())))
+    assert not bad, bad
+    assert int(got["pair count"][0]) == int(ref["ncalc"])
+    prev = {"maxViscousPressure": ref["maxViscousPressure"], "DvDx": ref["DvDx"].reshape(N, -1),
+            "DvDt": ref["DvDt"].reshape(N, -1)}
+    dt = integrator.hydro_dt(dim, N, full, prev, off, nbr, 1.51, 0.25)
+    if dim < 3:
+        assert got["dt"][0] == dt
+    else:
+        assert got["dt"][0] == pytest.approx(dt, rel=1e-12)
+    rho = oracle.sum_density(T, N, full["pos"], full["mass"], full["H"], off, nbr, lists=full["list"],
+                             sum_all=sum_all)
+    assert np.array_equal(got["finalized rho"], rho)
+
+
+@pytest.mark.parametrize("case", NODELIST_CASES, ids=[f"{c[2]}{c[0]}d_{'sumall' if c[3] else 'ownlist'}"
+                                                      for c in NODELIST_CASES])
+def test_host_class_nodelists_synchronous_rk2(oracle, case, tmp_path):
+    """SynchronousRK2 over a DataBase of two NodeLists with their own gamma:
+    the dt sequence and every NodeList's state as the oracle's integrator."""
+    import integrator
+    dim, n, kind, sum_all = case
+    gammas = (1.4, 5.0 / 3.0)
+    st, lists, counts = two_lists(dim, n, 50 + dim)
+    N = lists.size
+    steps = 3
+    print(f"[host class] nodelists rk2 dim={dim} {kind} sum_all={sum_all}", flush=True)
+    rec = state_records(dim, st, N, N, list_counts=counts, gammas=list(gammas), sum_all=int(sum_all), steps=steps,
+                        boundaries=box_records(dim, kind))
+    got = run_host("rk2", rec, tmp_path)
+    ref = integrator.RK2(dim, {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")},
+                         oracle.box_boundaries(dim, kind=kind), gammas[0], lists=lists, gammas=gammas,
+                         sum_all=sum_all)
+    dts = [ref.step() for _ in range(steps)]
+    if dim < 3:
+        assert list(got["dt"]) == dts
+    else:
+        assert np.allclose(got["dt"], dts, rtol=1e-12, atol=0)
+    for key in ("pos", "vel", "rho", "eps", "H", "P", "cs"):
+        a = got[key].reshape(ref.st[key].shape)
+        b = ref.st[key]
+        if dim < 3:
+            assert np.array_equal(a, b), key
+        else:
+            assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), key
+
+
+def test_host_class_nodelists_must_share_settings(tmp_path):
+    """The device holds one nPerh (and hmin, hmax, ...) for all NodeLists: a
+    DataBase whose NodeLists differ there is refused with VERIFY2."""
+    dim = 2
+    st, lists, counts = two_lists(dim, 10, 7)
+    N = lists.size
+    rec = state_records(dim, st, N, N, list_counts=counts, gammas=[1.4, 1.4], nperh=[1.51, 2.01],
+                        boundaries=box_records(dim, "reflecting"))
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_records(fin, rec)
+    r = subprocess.run([HOST_BIN, "derivs", fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "one nPerh" in r.stderr, r.stdout + r.stderr
