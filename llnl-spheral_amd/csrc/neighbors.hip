@@ -453,6 +453,7 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
   __syncthreads();
   const int32_t total = sOff[kStaged];
   const bool staged = total <= kCandCap;
+  int ghosts = 0;
   if (staged) {
     for (int t = threadIdx.x; t < kStaged; t += blockDim.x) {
       const int32_t o = sOff[t], cnt = sOff[t + 1] - o, st = sStart[t];
@@ -461,6 +462,7 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
         sCand[o + q] = make_float4(float(r.x - ox), float(r.y - oy), float(r.z - oz), float(r.w) * float(r.w));
         // bit 31 flags a ghost (slots stay below 2^31)
         const bool ghost = rec[int64_t(st + q) * R::STRIDE + R::ORIG] >= double(a.nInt);
+        ghosts |= ghost ? 1 : 0;
         sSlot[o + q] = int32_t(uint32_t(st + q) | (ghost ? 0x80000000u : 0u));
       }
     }
@@ -468,6 +470,10 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
     atomicAdd(&flags[1], 1);                      // bricks that read candidates from global memory
   }
   __syncthreads();
+  // ghost marks only where the staged candidates hold a ghost (the outer
+  // bricks): a scalar flag, so interior bricks skip the mark code per candidate
+  const bool marks =
+      __builtin_amdgcn_readfirstlane(__syncthreads_or(ghosts)) != 0 && a.keepSlot != nullptr;
 
   // One lane per node; the node's 25 (dz, dy) runs of up to 5 staged cells
   // are contiguous candidate ranges in LDS.  Rows leave as int4 chunks, one
@@ -520,7 +526,9 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
       //    (candidate 0), so every index it forms stays inside the tables.
       //    The rare paths (the exact FP64 test, ghost marks) are entered only
       //    when some lane of the wavefront needs them.
-      if (nr > 0) {
+      // one instance of the walk with ghost marks and one without, chosen by
+      // the block-uniform flag
+      auto walk = [&](auto marksOn) {
         for (int q = 0; q < 3; ++q) sRun[(nr + q) * kBlock + threadIdx.x] = 0u;
         const uint32_t first = sRun[threadIdx.x];
         int32_t t = int32_t(first & 0xffffu), tEnd = int32_t(first >> 16);
@@ -545,21 +553,18 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
           const float q = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
           const float tmin = q * (cj.w < hi2 ? cj.w : hi2);
           bool in = tmin < a.lo32;
+          const bool band = (tmin >= a.lo32) & (tmin <= a.hi32);
           const int32_t sj = raw & 0x7fffffff;
-          const bool band = !in && tmin <= a.hi32;
-          if (__ballot(band)) {
-            if (band) {
-              const double* rj = rec + int64_t(sj) * R::STRIDE;
-              in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
-            }
+          // the exact test in the rare band (the branch is skipped when no
+          // lane of the wavefront takes it)
+          if (band) {
+            const double* rj = rec + int64_t(sj) * R::STRIDE;
+            in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
           }
           const bool acc = in && sj != int32_t(k);
           row.pushIf(acc, sj);
-          if (a.keepSlot) {
-            const bool mark = acc && raw < 0;
-            if (__ballot(mark)) {
-              if (mark) a.keepSlot[sj] = 1;
-            }
+          if constexpr (decltype(marksOn)::value) {
+            if (acc && raw < 0) a.keepSlot[sj] = 1;
           }
         };
         float4 c0, c1, c2;
@@ -582,6 +587,10 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
           test(c2, s2);
           if (--left == 0) break;
         }
+      };
+      if (nr > 0) {
+        if (marks) walk(std::true_type{});
+        else walk(std::false_type{});
       }
     } else {
       for (int dz = -2; dz <= 2; ++dz) {
