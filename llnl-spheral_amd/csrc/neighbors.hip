@@ -305,16 +305,33 @@ __global__ void k_cell_scatter(const int32_t* __restrict__ cellOf, const int32_t
   perm[slot] = int32_t(i);
 }
 
-// Order each cell's members by node index (insertion sort; cells hold a few to
-// tens of nodes).
-__global__ void k_cell_sort(const int32_t* __restrict__ cellStart, int32_t* __restrict__ perm, int64_t ncells) {
+// Order each cell's members by node index, or, with `xkey` (x of node i at
+// xkey[i * stride]), by x and then node index, so that the members of a row
+// of cells are in x order (insertion sort; cells hold a few to tens of nodes).
+__global__ void k_cell_sort(const int32_t* __restrict__ cellStart, int32_t* __restrict__ perm, int64_t ncells,
+                            const double* __restrict__ xkey, int stride) {
   const int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (c >= ncells) return;
   const int32_t b = cellStart[c], e = cellStart[c + 1];
+  if (!xkey) {
+    for (int32_t a = b + 1; a < e; ++a) {
+      const int32_t v = perm[a];
+      int32_t q = a - 1;
+      while (q >= b && perm[q] > v) { perm[q + 1] = perm[q]; --q; }
+      perm[q + 1] = v;
+    }
+    return;
+  }
   for (int32_t a = b + 1; a < e; ++a) {
     const int32_t v = perm[a];
+    const double xv = xkey[int64_t(v) * stride];
     int32_t q = a - 1;
-    while (q >= b && perm[q] > v) { perm[q + 1] = perm[q]; --q; }
+    for (; q >= b; --q) {
+      const int32_t u = perm[q];
+      const double xu = xkey[int64_t(u) * stride];
+      if (xu < xv || (xu == xv && u < v)) break;
+      perm[q + 1] = u;
+    }
     perm[q + 1] = v;
   }
 }
@@ -492,8 +509,9 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
     RowWriter row(nbr, a.n, k, a.cap);
     if (staged) {
       // 1. the node's non-empty runs, in (dz, dy) order, into its column of
-      //    the run table: the cells of each (dz, dy) row of staged cells that
-      //    a sphere of the search radius (with a 1e-4 margin) reaches
+      //    the run table: the candidates of each (dz, dy) row of staged cells
+      //    inside the x chord of a sphere of the search radius (with a 1e-4
+      //    margin) around the node
       int nr = 0, nCand = 0;
       for (int dz = -2; dz <= 2; ++dz) {
         // distance from the node to the cell slab along z (staged cell t spans
@@ -511,7 +529,31 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
           x1 = x1 > sx + 2 ? sx + 2 : x1;
           if (x0 > x1) continue;
           const int rowb = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge;
-          const int32_t t0 = sOff[rowb + x0], t1 = sOff[rowb + x1 + 1];
+          // the members of a cell are in x order: the chord [fx - rem, fx + rem]
+          // starts and ends inside the end cells, found by two bisections
+          int32_t t0 = sOff[rowb + x0], t1 = sOff[rowb + x1 + 1];
+          {
+            int32_t lo = t0, up = sOff[rowb + x0 + 1];
+            const float key = fx - rem;
+            while (lo < up) {
+              const int32_t mid = (lo + up) >> 1;
+              const bool below = sCand[mid].x < key;
+              lo = below ? mid + 1 : lo;
+              up = below ? up : mid;
+            }
+            t0 = lo;
+          }
+          {
+            int32_t lo = sOff[rowb + x1], up = t1;
+            const float key = fx + rem;
+            while (lo < up) {
+              const int32_t mid = (lo + up) >> 1;
+              const bool inside = sCand[mid].x <= key;
+              lo = inside ? mid + 1 : lo;
+              up = inside ? up : mid;
+            }
+            t1 = lo;
+          }
           if (t1 > t0) {
             sRun[nr++ * kBlock + threadIdx.x] = uint32_t(t0) | (uint32_t(t1) << 16);
             nCand += t1 - t0;
@@ -1211,7 +1253,7 @@ void buildNeighborsD(Sim& s, double ext) {
     }
   }
 
-  // 2. counting sort by cell, then by node index inside a cell
+  // 2. counting sort by cell, then by node index (staged search: by x) inside a cell
   s.cellOf.ensure(sizeof(int32_t) * size_t(n));
   s.cellCount.ensure(sizeof(int32_t) * size_t(G.ncells + 1));
   s.cellFill.ensure(sizeof(int32_t) * size_t(G.ncells + 1));
@@ -1236,8 +1278,9 @@ void buildNeighborsD(Sim& s, double ext) {
   exclusiveScan<int32_t, int32_t>(s.cellCount.as<int32_t>(), s.cellStart.as<int32_t>(), G.ncells, s);
   hipLaunchKernelGGL(k_cell_scatter, dim3(nblocks(n)), dim3(kBlock), 0, st, s.cellOf.as<int32_t>(),
                      s.cellStart.as<int32_t>(), s.cellFill.as<int32_t>(), s.perm.as<int32_t>(), n);
+  // the staged search cuts each row of cells to the chord of its sphere by x
   hipLaunchKernelGGL(k_cell_sort, dim3(nblocks(G.ncells)), dim3(kBlock), 0, st, s.cellStart.as<int32_t>(),
-                     s.perm.as<int32_t>(), G.ncells);
+                     s.perm.as<int32_t>(), G.ncells, stagedPath ? s.pos.as<double>() : nullptr, D);
   hipLaunchKernelGGL(k_iperm, dim3(nblocks(n)), dim3(kBlock), 0, st, s.perm.as<int32_t>(), s.iperm.as<int32_t>(), n);
   SG_CHECK_HIP(hipGetLastError());
   s.nSlots = n;
