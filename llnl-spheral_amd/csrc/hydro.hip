@@ -970,8 +970,19 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
       lookupWGR(tab, tt, hj * rmag, nb.hdet, Wj, gWj);
       const Real gi = hi * gWi, gj = hj * gWj;     // grad W = r_hat * g
       // Monaghan-Gingold Q: mu = v_ij . eta / (eta^2 + eps2), eta = h r_ij
-      const Real mui = hi * vdotr * rcpR(fmaR(hi * hi, r2, eps2));
-      const Real muj = hj * vdotr * rcpR(fmaR(hj * hj, r2, eps2));
+      // (FP64: one refined reciprocal of the product serves both ends)
+      const Real da = fmaR(hi * hi, r2, eps2), db = fmaR(hj * hj, r2, eps2);
+      Real ia, ib;
+      if constexpr (std::is_same<Real, double>::value) {
+        const Real inv = rcpR(da * db);
+        ia = db * inv;
+        ib = da * inv;
+      } else {
+        ia = rcpR(da);
+        ib = rcpR(db);
+      }
+      const Real mui = hi * vdotr * ia;
+      const Real muj = hj * vdotr * ib;
       const Real QPi = muQR(mui, Cl, Cq, ci, p.linearInExpansion, p.quadraticInExpansion) * invRhoi;
       const Real QPj = muQR(muj, Cl, Cq, nb.cs, p.linearInExpansion, p.quadraticInExpansion) * nb.invRho;
       const Real Qi = rhoi2 * absR(QPi);
@@ -983,7 +994,14 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
       const Real mj = nb.m;
       const Real fsr = mj * (pi + pj) * rinv;        // acceleration along r_ij
       const Real wg = mj * gi * rinv;               // grad W_i * m_j = r_ij * wg
-      const Real f2 = gi * gi * rcpR(maxR(Real(1.0e-30), r2 * r2 * r2));
+      // gi^2 / max(1e-30, r^6); FP64 takes 1/r^6 from 1/r
+      Real f2;
+      if constexpr (std::is_same<Real, double>::value) {
+        const Real ri2 = rinv * rinv;
+        f2 = r2 * r2 * r2 > Real(1.0e-30) ? gi * gi * (ri2 * ri2 * ri2) : gi * gi * Real(1.0e30);
+      } else {
+        f2 = gi * gi * rcpR(maxR(Real(1.0e-30), r2 * r2 * r2));
+      }
       if (o.pairDv) {
         // compatible energy: the own-frame pair term, DvDt_i += -m_j dv
         const Real dsc = (pi + pj) * rinv;
@@ -1015,29 +1033,31 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
           for (int c = 0; c < D; ++c) af.xsphDV[c] = fmaR(-vij[c], w2, af.xsphDV[c]);
         }
       } else {
-        for (int c = 0; c < D; ++c) a.DvDt.c[c] -= double(rij[c] * fsr);
-        a.DepsDt += double(mj * vdr * pi);
-        a.viscW += double(mj * vdr * halfQi);
-        a.effVP += double(nb.mor * Qi * Wi);
-        a.rhoSum += double(mj * Wi);
-        a.norm += double(mori * Wi);
-        a.wNS += double(absR(gWi));
+        // FP64, search row order: every accumulation fused
+        for (int c = 0; c < D; ++c) a.DvDt.c[c] = fmaR(-rij[c], fsr, a.DvDt.c[c]);
+        const Real mv = mj * vdr;
+        a.DepsDt = fmaR(mv, pi, a.DepsDt);
+        a.viscW = fmaR(mv, halfQi, a.viscW);
+        a.effVP = fmaR(nb.mor * Qi, Wi, a.effVP);
+        a.rhoSum = fmaR(mj, Wi, a.rhoSum);
+        a.norm = fmaR(mori, Wi, a.norm);
+        a.wNS += absR(gWi);
         for (int c = 0; c < D; ++c) {
           const Real u = vij[c] * wg;
-          for (int b = 0; b < D; ++b) a.DvDx.c[c * D + b] -= double(u * rij[b]);
+          for (int b = 0; b < D; ++b) a.DvDx.c[c * D + b] = fmaR(-u, rij[b], a.DvDx.c[c * D + b]);
         }
         int s2 = 0;
         for (int c = 0; c < D; ++c) {
           const Real rw = rij[c] * wg, rf = rij[c] * f2;
           for (int b = c; b < D; ++b, ++s2) {
-            a.M.c[c * D + b] -= double(rw * rij[b]);
-            a.m2nd.c[s2] += double(rf * rij[b]);
+            a.M.c[c * D + b] = fmaR(-rw, rij[b], a.M.c[c * D + b]);
+            a.m2nd.c[s2] = fmaR(rf, rij[b], a.m2nd.c[s2]);
           }
         }
         if (p.XSPH) {
           const Real w2 = Real(0.5) * fmaR(mori, Wi, nb.mor * Wj);
-          a.xsphW += double(w2);
-          for (int c = 0; c < D; ++c) a.xsphDV.c[c] -= double(vij[c] * w2);
+          a.xsphW += w2;
+          for (int c = 0; c < D; ++c) a.xsphDV.c[c] = fmaR(-vij[c], w2, a.xsphDV.c[c]);
         }
       }
     };
