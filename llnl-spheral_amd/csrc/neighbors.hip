@@ -322,6 +322,39 @@ __global__ void k_cell_sort(const int32_t* __restrict__ cellStart, int32_t* __re
     }
     return;
   }
+  constexpr int kSmall = 8;
+  if (e - b <= kSmall) {
+    // all keys loaded at once (independent loads), then an odd-even
+    // transposition network in registers; padding sorts last
+    int32_t id[kSmall];
+    double xv[kSmall];
+#pragma unroll
+    for (int q = 0; q < kSmall; ++q) {
+      id[q] = INT32_MAX;
+      xv[q] = HUGE_VAL;
+      if (b + q < e) {
+        id[q] = perm[b + q];
+        xv[q] = xkey[int64_t(id[q]) * stride];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kSmall; ++r) {
+#pragma unroll
+      for (int q = r & 1; q + 1 < kSmall; q += 2) {
+        const bool swap = xv[q] > xv[q + 1] || (xv[q] == xv[q + 1] && id[q] > id[q + 1]);
+        const double x0 = xv[q], x1 = xv[q + 1];
+        const int32_t i0 = id[q], i1 = id[q + 1];
+        xv[q] = swap ? x1 : x0;
+        xv[q + 1] = swap ? x0 : x1;
+        id[q] = swap ? i1 : i0;
+        id[q + 1] = swap ? i0 : i1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kSmall; ++q)
+      if (b + q < e) perm[b + q] = id[q];
+    return;
+  }
   for (int32_t a = b + 1; a < e; ++a) {
     const int32_t v = perm[a];
     const double xv = xkey[int64_t(v) * stride];

@@ -88,8 +88,10 @@ void serve(Sim& s, sphgpu_request* req, int n, const sphgpu_hydro_params& hp, co
       ++done;
     }
     if (finished == 0) continue;
-    // completions: copy out, then remove together
+    // completions: every finished member's copies queued, one wait for all of
+    // them, then the members removed together
     std::vector<int64_t> gone;
+    std::vector<std::pair<int, int64_t>> ended;   // (request, cycles)
     for (size_t a = 0; a < s.members.size(); ++a) {
       const SegState& g = s.segHost[a];
       if (g.t < g.tEnd) continue;
@@ -108,12 +110,16 @@ void serve(Sim& s, sphgpu_request* req, int n, const sphgpu_hydro_params& hp, co
       get(s.rho, r.outRho, 1);
       get(s.eps, r.outEps, 1);
       get(s.H, r.outH, NS);
-      SG_CHECK_HIP(hipStreamSynchronize(s.stream));
-      r.finished = elapsed();
-      r.steps = int64_t(g.cycle);
+      ended.emplace_back(it->q, int64_t(g.cycle));
       used -= it->bytes;
       gone.push_back(mb.id);
       active.erase(it);
+    }
+    SG_CHECK_HIP(hipStreamSynchronize(s.stream));
+    const double tEnd = elapsed();
+    for (const auto& e : ended) {
+      req[e.first].finished = tEnd;
+      req[e.first].steps = e.second;
       ++done;
     }
     ensembleRemove(s, int(gone.size()), gone.data());
