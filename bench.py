@@ -52,13 +52,11 @@ NBR_BYTES_PER_ENTRY = 4      # one int32 neighbour index per directed entry
 # HBM bytes per pair-kernel dispatch, from calibrated FETCH_SIZE / WRITE_SIZE runs
 PROFILES = os.path.join(ROOT, "profiles")
 VALU_ISSUE_HZ = 1024 * 2.4e9 / 4     # wavefront instructions per second: 1024 SIMDs, 4 cycles each
-# Neighbour gathers: each 16-B load of a random row is one L1 line access per
-# lane, and a CU's L1 serves about one access per cycle.  This is a model of the
-# pair kernel's gather limit, not a published peak.
-L1_LINE_RATE = 256 * 2.4e9
 # 16-B loads per neighbour of the isotropic pair kernels (96-B FP64 record,
-# 48-B 3-D mixed record)
+# 48-B 3-D mixed record) and of the density sums (48-B FP64, 32-B mixed
+# density record); their gather ceilings are measured on the pass's own rows
 GATHER_LOADS = {("fp64", "search"): 6, ("mixed", "search"): 3}
+DENSITY_LOADS = {"fp64": 3, "mixed": 2}
 
 
 def measured_counters(n, precision, row_order, boundaries):
@@ -75,7 +73,7 @@ def measured_counters(n, precision, row_order, boundaries):
     return None, None
 
 
-def pair_roofline(precision, row_order, n, boundaries, entries, n_int, n_tot, kernel_ms):
+def pair_roofline(precision, row_order, n, boundaries, entries, n_int, n_tot, kernel_ms, gather_floor_ms=None):
     """Roofline of the pair kernel (evaluateDerivatives pair loop + epilogue)
     for one pass: the FLOP roofline from the exact per-entry count (the
     binding one: the larger fraction), the HBM roofline from the algorithmic
@@ -122,15 +120,16 @@ def pair_roofline(precision, row_order, n, boundaries, entries, n_int, n_tot, ke
     if counters and counters.get("tcc_hit") is not None:
         h, m = counters["tcc_hit"], counters["tcc_miss"]
         out["l2_hit_rate"] = h / (h + m) if h + m else None
-    loads = GATHER_LOADS.get((precision, row_order)) if n >= 64 else None
-    if loads:
+    loads = GATHER_LOADS.get((precision, row_order))
+    if loads and gather_floor_ms:
         acc = entries * loads
-        rate = acc / t
         out["gather"] = {
-            "loads_per_neighbour": loads, "line_accesses": acc, "achieved_per_s": rate,
-            "model_peak_per_s": L1_LINE_RATE, "frac": rate / L1_LINE_RATE,
-            "note": "model, not a published peak: one L1 line access per lane per 16-B load of a neighbour "
-                    "record, one access per CU per cycle (256 CUs at 2.4 GHz)"}
+            "loads_per_neighbour": loads, "loads_per_launch": acc, "achieved_per_s": acc / t,
+            "floor_ms": gather_floor_ms, "kernel_ms": kernel_ms, "ceiling_per_s": acc / (gather_floor_ms * 1e-3),
+            "frac": gather_floor_ms / kernel_ms,
+            "note": "measured ceiling, not a published peak: sphgpu_gather_probe walks the same rows in the "
+                    "kernel's work order and only loads its records (loads_per_neighbour 16-B pieces per "
+                    "neighbour); frac = that time / the kernel's time"}
     return out
 
 
@@ -548,6 +547,16 @@ def main():
                 "derivatives": stage[2], "pack": stage[3]}
 
     wall, stage = measure(params)
+
+    def gather_floors(pname):
+        # the gather ceilings of the pair kernel and the density sum on this
+        # pass's rows (the probe walks the rows of the last build)
+        pl = GATHER_LOADS.get((pname, args.row_order))
+        dl = DENSITY_LOADS[pname]
+        return {"pair_ms": sim.gather_probe(pl) if pl else None, "pair_loads": pl,
+                "density_ms": sim.gather_probe(dl), "density_loads": dl}
+
+    floors = gather_floors(args.precision)
     pairs = sim.pair_count()
     entries = sim.neighbor_count()
     ms_step = 1e3 * wall / args.steps
@@ -577,19 +586,26 @@ def main():
             "stage_ms": stage_dict(stage),
         },
         "roofline": pair_roofline(args.precision, args.row_order, args.n, args.boundaries, entries, N, n_tot,
-                                  stage[2]),
+                                  stage[2], floors["pair_ms"]),
+        "density_gather": {"loads_per_neighbour": floors["density_loads"], "floor_ms": floors["density_ms"],
+                           "stage_ms": stage[1], "frac": floors["density_ms"] / stage[1],
+                           "note": "sum density stage (the gather kernel plus the ghost density refresh) "
+                                   "against the measured gather ceiling of its records on the same rows"},
     }
     if args.precision == "fp64" and args.mixed_pass and args.row_order == "search":
         # the same pass on the same state in mixed precision (FP32 pair
         # arithmetic, FP64 positions and sums)
         sim.set_precision(sphgpu.PRECISION_MIXED)
         wall_m, stage_m = measure(sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED))
+        floors_m = gather_floors("mixed")
         line["mixed_pass"] = {
             "value": sim.pair_count() * args.steps / wall_m, "unit": "pair-interactions/s",
             "ms_per_step": 1e3 * wall_m / args.steps, "steps": args.steps, "warmup": args.warmup,
             "speedup_vs_fp64": wall / wall_m, "dtype": "f32+f64", "stage_ms": stage_dict(stage_m),
             "roofline": pair_roofline("mixed", "search", args.n, args.boundaries, sim.neighbor_count(), N,
-                                      sim.node_counts()[1], stage_m[2]),
+                                      sim.node_counts()[1], stage_m[2], floors_m["pair_ms"]),
+            "density_gather": {"loads_per_neighbour": floors_m["density_loads"], "floor_ms": floors_m["density_ms"],
+                               "stage_ms": stage_m[1], "frac": floors_m["density_ms"] / stage_m[1]},
         }
         sim.set_precision(prec)
         sim.sum_density()

@@ -490,6 +490,13 @@ int sphgpu_serve(sphgpu_sim* sim, sphgpu_request* requests, int n_requests, cons
  * return the average kernel time (ms).  Used to calibrate HBM traffic counters
  * and to state the achievable copy bandwidth beside the roofline figures. */
 int sphgpu_copy_probe(sphgpu_sim* sim, int64_t bytes, int repeats, double* ms);
+/* Measurement helper with no counterpart in the reference: walk the rows of
+ * the last neighbour build as the pair and density kernels do (one lane per
+ * internal node, their work order) and only load loads_per_neighbour (2, 3 or
+ * 6) 16-B pieces of a record per neighbour; return the average kernel time
+ * (ms) over `repeats` launches: the gather ceiling of those kernels on these
+ * rows, stated beside their times. */
+int sphgpu_gather_probe(sphgpu_sim* sim, int loads_per_neighbour, int repeats, double* ms);
 
 #ifdef __cplusplus
 }

@@ -736,4 +736,11 @@ int sphgpu_copy_probe(sphgpu_sim* s, int64_t bytes, int repeats, double* ms) {
   });
 }
 
+int sphgpu_gather_probe(sphgpu_sim* s, int loads_per_neighbour, int repeats, double* ms) {
+  return guarded(s, [&] {
+    SG_REQUIRE(ms, SPHGPU_ERR_ARG, "null output");
+    *ms = gatherProbe(*s, loads_per_neighbour, repeats);
+  });
+}
+
 }  // extern "C"

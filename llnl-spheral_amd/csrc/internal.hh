@@ -300,6 +300,7 @@ int pairRecordSec(const Sim& s, const sphgpu_hydro_params& hp);
 int densityRecordSec(const Sim& s);
 // probe.hip
 double copyProbe(Sim& s, int64_t bytes, int repeats);
+double gatherProbe(Sim& s, int loads, int repeats);
 // ghosts.hip
 void setBoundaries(Sim& s, int n, const sphgpu_boundary* b);
 void setGhostNodes(Sim& s, double kernelExtent, bool cull);

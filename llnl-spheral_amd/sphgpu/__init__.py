@@ -199,6 +199,7 @@ def lib():
             "sphgpu_last_timings": (C.c_int, [vp, _dp]),
             "sphgpu_synchronize": (C.c_int, [vp]),
             "sphgpu_copy_probe": (C.c_int, [vp, C.c_int64, C.c_int, _dp]),
+            "sphgpu_gather_probe": (C.c_int, [vp, C.c_int, C.c_int, _dp]),
             "sphgpu_set_boundaries": (C.c_int, [vp, C.c_int, C.POINTER(Boundary)]),
             "sphgpu_eos_gamma_law": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int]),
             "sphgpu_omega_gradh": (C.c_int, [vp]),
@@ -654,4 +655,12 @@ class Sim:
         """Average time (ms) of a plain device-to-device streaming copy kernel."""
         ms = C.c_double()
         self._chk(self.L.sphgpu_copy_probe(self.h, nbytes, repeats, C.byref(ms)))
+        return ms.value
+
+    def gather_probe(self, loads, repeats=5):
+        """Average time (ms) of a kernel that walks the current neighbour rows
+        as the pair and density kernels do and only loads `loads` (2, 3 or 6)
+        16-B pieces of each neighbour's record: their gather ceiling."""
+        ms = C.c_double()
+        self._chk(self.L.sphgpu_gather_probe(self.h, loads, repeats, C.byref(ms)))
         return ms.value

@@ -185,3 +185,25 @@ def test_errors_are_reported(sphgpu):
     assert e.value.code == -4
     with pytest.raises(sphgpu.SphGpuError):
         s.derivative("DvDt")                # no valid derivatives after the failed call
+
+
+def test_gather_probe(sphgpu):
+    """The measurement probe walks the rows of a neighbour build and times
+    record gathers; it refuses before a build and for unsupported widths."""
+    print("[parity] gather probe", flush=True)
+    s = sphgpu.Sim(3)
+    s.set_table_bspline(1000)
+    st, meta = problems.converging_flow(12)
+    n = meta["n_internal"]
+    s.set_nodes(n, n)
+    s.upload_state(st)
+    with pytest.raises(sphgpu.SphGpuError):
+        s.gather_probe(6)                    # no rows yet
+    s.build_neighbors(2.0)
+    times = {L: s.gather_probe(L, repeats=2) for L in (2, 3, 6)}
+    assert all(t > 0.0 for t in times.values()), times
+    with pytest.raises(sphgpu.SphGpuError):
+        s.gather_probe(4)
+    # the rows are untouched: the pass still runs on them
+    s.sum_density()
+    s.evaluate_derivatives(sphgpu.hydro_params())
