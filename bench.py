@@ -334,16 +334,49 @@ def serve_bench(sphgpu, problems, count, rate, precision, check, graphs=False):
                        f"t_end~U[0.05,0.2], reflecting box), Poisson arrivals at {rate}/s",
            "precision": precision, "hip_graphs": bool(graphs), "batched": batched, "one_at_a_time": single}
     if check > 0 and prec == sphgpu.PRECISION_MIXED:
-        # error of the mixed-precision results vs FP64 solves alone, smallest requests
-        errs = {}
-        for q in sorted(range(count), key=lambda q: reqs[q]["n"])[:check]:
+        # error of the mixed-precision results of the batched run against
+        # FP64 solves of the same requests alone (the smallest `check`
+        # requests; all of them by default): relative L-inf (over the field's
+        # largest magnitude) and L2, worst over the requests
+        fields = (("mass density", "rho"), ("specific thermal energy", "eps"), ("velocity", "vel"))
+
+        def rel(a, b):
+            a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+            return (float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)),
+                    float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)))
+
+        linf, l2, per = {}, {}, []
+        picked = sorted(range(count), key=lambda q: reqs[q]["n"])[:check]
+        refs = {}
+        for q in picked:
             r = reqs[q]
-            _, _, ref = plain_solve(sphgpu, 2, r["state"], r["gamma"], r["t_end"], sphgpu.hydro_params(),
-                                    sphgpu.PRECISION_FP64, order)
-            for f, k in (("mass density", "rho"), ("specific thermal energy", "eps"), ("velocity", "vel")):
-                a, b = np.asarray(out[q][k]).ravel(), np.asarray(ref[f]).ravel()
-                errs[k] = max(errs.get(k, 0.0), float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)))
-        res["error_vs_fp64"] = {"rel_linf": errs, "requests": check}
+            _, steps, ref = plain_solve(sphgpu, 2, r["state"], r["gamma"], r["t_end"], sphgpu.hydro_params(),
+                                        sphgpu.PRECISION_FP64, order)
+            refs[q] = ref
+            row = {"n": r["n"], "steps": steps}
+            for f, k in fields:
+                li, ll = rel(out[q][k], ref[f])
+                linf[k] = max(linf.get(k, 0.0), li)
+                l2[k] = max(l2.get(k, 0.0), ll)
+                row[k] = [li, ll]
+            per.append(row)
+        res["error_vs_fp64"] = {"rel_linf": linf, "rel_l2": l2, "requests": len(picked), "bar": 1e-5,
+                                "within_bar": all(v <= 1e-5 for v in list(linf.values()) + list(l2.values())),
+                                "per_request": per,
+                                "note": "final states of the batched mixed run against FP64 solves alone; "
+                                        "per_request: n, steps of the FP64 solve, [L-inf, L2] per field"}
+        # control: the same FP64 solve with the other row order (only the
+        # summation order differs) on the largest checked requests, to show how
+        # much a blast amplifies rounding differences by its end time
+        ctrl = []
+        for q in picked[-2:]:
+            r = reqs[q]
+            _, _, alt = plain_solve(sphgpu, 2, r["state"], r["gamma"], r["t_end"], sphgpu.hydro_params(),
+                                    sphgpu.PRECISION_FP64, sphgpu.ROW_ORDER_REFERENCE)
+            ctrl.append({"n": r["n"], **{k: list(rel(alt[f], refs[q][f])) for f, k in fields}})
+        res["fp64_row_order_control"] = {"requests": ctrl,
+                                         "note": "FP64 reference row order against FP64 search row order, "
+                                                 "[L-inf, L2] per field at the end time"}
     return res
 
 
@@ -448,7 +481,8 @@ def main():
     ap.add_argument("--serve", type=int, default=32,
                     help="serving loop with this many blast requests (0 = off)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-check", type=int, default=8, help="requests re-solved alone in FP64 for the error")
+    ap.add_argument("--serve-check", type=int, default=1 << 30,
+                    help="requests (the smallest first) re-solved alone in FP64 for the error; default all")
     ap.add_argument("--batch-precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: launch shapes of the 1-D/2-D pair kernel from the committed tuning table "
