@@ -244,6 +244,12 @@ def ensemble_bench(sphgpu, problems, count, precision, sample, graphs=False):
     for q, m in enumerate(members):
         ids[e.ensemble_add(m["state"], m["gamma"], m["t_end"], sphgpu.integrator_params(gamma=m["gamma"]))] = q
     rows = {i: members[q]["n"] for i, q in ids.items()}
+    # tubes whose final state is checked against an FP64 solve alone: the
+    # one-at-a-time sample's tubes of up to 6000 nodes
+    by_n = sorted(range(count), key=lambda q: members[q]["n"])
+    pick = [by_n[int(round(k * (count - 1) / max(1, sample - 1)))] for k in range(sample)] if sample > 0 else []
+    check = {q for q in pick if members[q]["n"] <= 6000} if prec == sphgpu.PRECISION_MIXED else set()
+    final = {}
     finish, cycles = {}, {}
     steps = node_steps = 0
     last_report = t0
@@ -257,6 +263,9 @@ def ensemble_bench(sphgpu, problems, count, precision, sample, graphs=False):
             for mb in e.ensemble_members():
                 if mb["id"] in gone:
                     cycles[ids[mb["id"]]] = mb["cycle"]
+                    if ids[mb["id"]] in check:
+                        final[ids[mb["id"]]] = {f: e.ensemble_download(mb["id"], f)
+                                                for f in ("mass density", "specific thermal energy", "velocity")}
             e.ensemble_remove(gone)
             for g in gone:
                 finish[ids.pop(g)] = now
@@ -268,8 +277,6 @@ def ensemble_bench(sphgpu, problems, count, precision, sample, graphs=False):
     e.close()
     tts = [finish[q] for q in range(count)]
     # one-at-a-time plain solves of a sample spread over the sizes
-    by_n = sorted(range(count), key=lambda q: members[q]["n"])
-    pick = [by_n[int(round(k * (count - 1) / max(1, sample - 1)))] for k in range(sample)] if sample > 0 else []
     samp = []
     for q in pick:
         m = members[q]
@@ -282,6 +289,18 @@ def ensemble_bench(sphgpu, problems, count, precision, sample, graphs=False):
         lt = np.log([x[1] for x in samp])
         per_step = lambda n: float(np.exp(np.interp(np.log(n), ln, lt)))
         est = float(sum(cycles[q] * per_step(members[q]["n"]) for q in range(count)))
+    # final states of the checked members against FP64 solves alone
+    errs = []
+    for q in sorted(final, key=lambda q: members[q]["n"]):
+        m = members[q]
+        _, st64, ref = plain_solve(sphgpu, 1, m["state"], m["gamma"], m["t_end"], sphgpu.hydro_params(),
+                                   sphgpu.PRECISION_FP64, order)
+        row = {"n": m["n"], "steps": st64}
+        for f, k in (("mass density", "rho"), ("specific thermal energy", "eps"), ("velocity", "vel")):
+            a, b = np.asarray(final[q][f]).ravel(), np.asarray(ref[f]).ravel()
+            row[k] = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)),
+                      float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))]
+        errs.append(row)
     return {
         "workload": f"{count} planar shock tubes, n log-uniform in [200, 20000] (total {sum(m['n'] for m in members)}), "
                     "rho_L,P_L~U[0.5,2], rho_R,P_R~U[0.05,0.5], gamma in {1.4,5/3}, reflecting walls, t_end 0.2",
@@ -295,6 +314,9 @@ def ensemble_bench(sphgpu, problems, count, precision, sample, graphs=False):
                              "speedup": (est / wall) if est else None,
                              "method": "per-step time of the sampled tubes alone (first 300 steps), interpolated in "
                                        "log n, times each tube's step count from the ensemble run"},
+        "error_vs_fp64": {"per_member": errs,
+                          "note": "final states of sampled members (up to 6000 nodes) of the mixed batch against "
+                                  "FP64 solves of the same tubes alone: n, steps, [L-inf, L2] per field"},
     }
 
 
