@@ -189,6 +189,55 @@ def cpu_baseline(budget_s=10.0, n=48, boundaries="reflecting"):
     }
 
 
+def oracle_parity(sim, sphgpu, st, n_int, boundaries, row_order):
+    """The checker of the measured FP64 pass, untimed: the oracle's FP64
+    restatement runs the same pass on the same state on the host (ghosts of
+    the reflecting box, exact connectivity, sum density with the ghost
+    refresh, evaluateDerivatives) and the device's neighbour CSR and fields
+    are compared with it.  CSR byte for byte (rows ascending, the
+    ConnectivityMap layout); fields by relative L-inf and L2 against the
+    field's largest magnitude / norm, bar 1e-12 in search row order (value for
+    value in reference order, 1e-14 for the cube-root fields)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # test infrastructure, the checker here
+    t0 = time.perf_counter()
+    N = n_int
+    bo = oracle.box_boundaries(3, kind="reflecting") if boundaries == "reflecting" else []
+    if bo:
+        full, off, nbr, info = oracle.ghost_pipeline(3, st, bo)
+    else:
+        full = {k: np.array(v) for k, v in st.items()}
+        off, nbr = oracle.connectivity(3, N, st["pos"], st["H"], 2.0)
+        info = None
+    o2, n2 = sim.neighbors()
+    csr = bool(np.array_equal(o2, off) and np.array_equal(n2, nbr))
+    del o2, n2
+    T = oracle.Table(3, 1000)
+    rho = oracle.sum_density(T, N, full["pos"], full["mass"], full["H"], off, nbr, gather=True)
+    rho_all = np.array(full["rho"])
+    rho_all[:N] = rho
+    if bo:
+        rho_all = oracle.apply_ghosts(3, N, bo, info["control"], info["sub_of"], rho_all, 0)
+    full["rho"] = rho_all
+
+    def rel(a, b):
+        a, b = np.asarray(a, dtype=np.float64).ravel(), np.asarray(b, dtype=np.float64).ravel()
+        return [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)),
+                float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))]
+
+    fields = {"rho": rel(sim.download("mass density", rows=N), rho_all[:N])}
+    ref = oracle.eval_derivs(T, oracle.make_params(), N, full, off, nbr, gather=True)
+    for f in sphgpu.DERIV_ALIASES:
+        fields[f] = rel(sim.derivative(f).reshape(ref[f].shape), ref[f])
+    worst = max(max(v) for v in fields.values())
+    return {"csr_bitexact": csr, "directed_entries": int(nbr.size), "row_order": row_order,
+            "fields_rel_linf_l2": fields, "worst": worst,
+            "bar": 1e-12 if row_order == "search" else 0.0,
+            "within_bar": csr and (worst <= 1e-12 if row_order == "search" else None),
+            "check_s": time.perf_counter() - t0,
+            "oracle": "oracle/ FP64 restatement (-ffp-contract=off parity build), the checker; not timed"}
+
+
 def plain_solve(sphgpu, dim, st, gamma, t_end, hp, prec, order, max_steps=None, graphs=False):
     """One problem alone as a plain simulation (sphgpu_rk2_step loop) to its
     end time, or for max_steps steps; returns (wall seconds, steps, final
@@ -613,6 +662,13 @@ def main():
                 "density_ms": sim.gather_probe(dl), "density_loads": dl}
 
     floors = gather_floors(args.precision)
+    parity = None
+    if not args.no_cpu and args.precision == "fp64":
+        try:
+            parity = oracle_parity(sim, sphgpu, st, N, args.boundaries, args.row_order)
+        except Exception as e:  # the check is reported, never required
+            parity = {"error": str(e)}
+        print(f"[bench] oracle check: {parity.get('check_s', 0):.1f}s", file=sys.stderr, flush=True)
     pairs = sim.pair_count()
     entries = sim.neighbor_count()
     ms_step = 1e3 * wall / args.steps
@@ -648,6 +704,8 @@ def main():
                            "note": "sum density stage (the gather kernel plus the ghost density refresh) "
                                    "against the measured gather ceiling of its records on the same rows"},
     }
+    if parity is not None:
+        line["parity_vs_oracle"] = parity
     if args.precision == "fp64" and args.mixed_pass and args.row_order == "search":
         # the same pass on the same state in mixed precision (FP32 pair
         # arithmetic, FP64 positions and sums)
