@@ -230,12 +230,29 @@ def oracle_parity(sim, sphgpu, st, n_int, boundaries, row_order):
     for f in sphgpu.DERIV_ALIASES:
         fields[f] = rel(sim.derivative(f).reshape(ref[f].shape), ref[f])
     worst = max(max(v) for v in fields.values())
-    return {"csr_bitexact": csr, "directed_entries": int(nbr.size), "row_order": row_order,
-            "fields_rel_linf_l2": fields, "worst": worst,
-            "bar": 1e-12 if row_order == "search" else 0.0,
-            "within_bar": csr and (worst <= 1e-12 if row_order == "search" else None),
-            "check_s": time.perf_counter() - t0,
-            "oracle": "oracle/ FP64 restatement (-ffp-contract=off parity build), the checker; not timed"}
+    report = {"csr_bitexact": csr, "directed_entries": int(nbr.size), "row_order": row_order,
+              "fields_rel_linf_l2": fields, "worst": worst,
+              "bar": 1e-12 if row_order == "search" else 0.0,
+              "within_bar": csr and (worst <= 1e-12 if row_order == "search" else None),
+              "check_s": time.perf_counter() - t0,
+              "oracle": "oracle/ FP64 restatement (-ffp-contract=off parity build), the checker; not timed"}
+    ref["rho"] = rho_all[:N]
+    return report, ref
+
+
+def compare_to_ref(sim, sphgpu, ref, n_int, bar):
+    """The device's density and derivative fields against reference fields
+    kept from oracle_parity: relative L-inf and L2 per field."""
+    def rel(a, b):
+        a, b = np.asarray(a, dtype=np.float64).ravel(), np.asarray(b, dtype=np.float64).ravel()
+        return [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)),
+                float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))]
+    fields = {"rho": rel(sim.download("mass density", rows=n_int), ref["rho"])}
+    for f in sphgpu.DERIV_ALIASES:
+        fields[f] = rel(sim.derivative(f).reshape(ref[f].shape), ref[f])
+    worst = max(max(v) for v in fields.values())
+    return {"fields_rel_linf_l2": fields, "worst": worst, "bar": bar, "within_bar": worst <= bar,
+            "oracle": "the FP64 oracle's fields of the same pass (computed once for the FP64 check)"}
 
 
 def plain_solve(sphgpu, dim, st, gamma, t_end, hp, prec, order, max_steps=None, graphs=False):
@@ -662,10 +679,10 @@ def main():
                 "density_ms": sim.gather_probe(dl), "density_loads": dl}
 
     floors = gather_floors(args.precision)
-    parity = None
+    parity, oracle_ref = None, None
     if not args.no_cpu and args.precision == "fp64":
         try:
-            parity = oracle_parity(sim, sphgpu, st, N, args.boundaries, args.row_order)
+            parity, oracle_ref = oracle_parity(sim, sphgpu, st, N, args.boundaries, args.row_order)
         except Exception as e:  # the check is reported, never required
             parity = {"error": str(e)}
         print(f"[bench] oracle check: {parity.get('check_s', 0):.1f}s", file=sys.stderr, flush=True)
@@ -721,6 +738,9 @@ def main():
             "density_gather": {"loads_per_neighbour": floors_m["density_loads"], "floor_ms": floors_m["density_ms"],
                                "stage_ms": stage_m[1], "frac": floors_m["density_ms"] / stage_m[1]},
         }
+        if oracle_ref is not None:
+            line["mixed_pass"]["parity_vs_oracle"] = compare_to_ref(sim, sphgpu, oracle_ref, N, 1e-5)
+        oracle_ref = None
         sim.set_precision(prec)
         sim.sum_density()
     if args.rk2_steps > 0:
