@@ -296,7 +296,10 @@ __global__ void __launch_bounds__(kBlock) k_sum_density(double* __restrict__ rec
 // |H_i r_ij| W' from every neighbour in row order, its self term W(0) det H_i
 // after the lower-index neighbours, and omega_i = -sum(eta W')/(nu sum W); a
 // node without neighbours gets 1.  With index-ordered rows this is the
-// reference's sum order.
+// reference's sum order; with several NodeLists the reference-order rows come
+// in the order the owner loop delivers node i's terms (the internal owners of
+// lower index first, then i's own pairs), and the owner of a pair is still its
+// end of lower caller index, so the self term goes to the same place.
 // ---------------------------------------------------------------------------
 template <int D, bool ISO, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_omega(const double* __restrict__ rec, const int32_t* __restrict__ rowCount,
@@ -1468,7 +1471,6 @@ void omegaGradhD(Sim& s) {
 
 void omegaGradh(Sim& s) {
   SG_REQUIRE(s.neighborsValid, SPHGPU_ERR_STATE, "the grad-h correction needs a neighbour build");
-  SG_REQUIRE(s.nLists <= 1, SPHGPU_ERR_UNSUPPORTED, "the grad-h correction takes one NodeList");
   SG_REQUIRE(s.table.n > 0, SPHGPU_ERR_STATE, "no kernel table installed");
   if (!s.recValid) packRecords(s);
   if (s.dim == 1) omegaGradhD<1>(s);

@@ -188,7 +188,8 @@ class RK2:
     def _derivs(self, full, off, nbr, info=None):
         if self.gradh:
             # SPHHydroBase::initialize: grad-h correction, then its ghost boundaries
-            full["omega"][:self.n] = oracle.omega_gradh(self.T, self.n, full["pos"], full["H"], off, nbr)
+            full["omega"][:self.n] = oracle.omega_gradh(self.T, self.n, full["pos"], full["H"], off, nbr,
+                                                        lists=full.get("list"))
             if info is not None:
                 full["omega"] = oracle.apply_ghosts(self.dim, self.n, self.b, info["control"], info["sub_of"],
                                                     full["omega"], 0)

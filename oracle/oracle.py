@@ -268,10 +268,19 @@ def sum_density(table, n_internal, pos, mass, H, off, nbr, gather=False, lists=N
     return rho
 
 
-def omega_gradh(table, n_internal, pos, H, off, nbr):
-    """computeSPHOmegaGradhCorrection restated (internal nodes)."""
+def omega_gradh(table, n_internal, pos, H, off, nbr, lists=None):
+    """computeSPHOmegaGradhCorrection restated (internal nodes): the owner
+    loop over every pair once, all NodeLists alike
+    (src/SPH/computeSPHOmegaGradhCorrection.cc:55-130).  lists (NodeList of
+    every node): several NodeLists; node i then walks its own pairs in
+    ConnectivityMap order, by (NodeList, index within the NodeList).  With
+    each NodeList's internal rows contiguous in NodeList order and the ghosts
+    after all of them, the owner of a pair (ConnectivityMapInline.hh:182-184)
+    is its end of lower row index, as with one NodeList."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     om = np.zeros(n_internal)
+    if lists is not None:
+        nbr = np.ascontiguousarray(list_order(n_internal, lists, off, nbr, receive=False), dtype=np.int32)
     table.L.orc_omega_gradh(table.h, table.dim, n_internal, pos.shape[0], _p(pos),
                             _p(np.ascontiguousarray(H, dtype=np.float64)), _p(off, _i64p), _p(nbr, _i32p), _p(om))
     return om

@@ -180,3 +180,65 @@ def test_nodelist_argument_errors(sphgpu):
     m.set_ghost_nodes(2.0, cull=True)
     with pytest.raises(sphgpu.SphGpuError):
         m.sum_density()                          # mixed precision takes one NodeList
+
+
+GRADH_CASES = [(1, 80, "reflecting"), (2, 14, "periodic"), (3, 8, "reflecting")]
+
+
+@pytest.mark.parametrize("order", [0, 1], ids=["refo", "searcho"])
+@pytest.mark.parametrize("case", GRADH_CASES, ids=[f"{c[2]}{c[0]}d" for c in GRADH_CASES])
+def test_omega_gradh_with_nodelists(sphgpu, oracle, case, order):
+    """computeSPHOmegaGradhCorrection over two NodeLists: every pair alike,
+    node i's terms in the owner loop's delivery order.  Reference row order
+    value for value (ghost rows from their controls), search row order 1e-12;
+    the pass that follows uses it."""
+    dim, n, kind = case
+    print(f"[nodelists-omega] dim={dim} n={n} {kind} order={order}", flush=True)
+    st, lists, counts = two_lists(dim, n, seed=130 + dim, mass_ratio=2.0)
+    N = lists.size
+    bo = oracle.box_boundaries(dim, kind=kind)
+    full, off, nbr, info = oracle.ghost_pipeline(dim, st, bo, 2.0, True, lists=lists)
+    T = oracle.Table(dim, 1000)
+    om = oracle.omega_gradh(T, N, full["pos"], full["H"], off, nbr, lists=full["list"])
+    om_all = np.array(full["omega"])
+    om_all[:N] = om
+    om_all = oracle.apply_ghosts(dim, N, bo, info["control"], info["sub_of"], om_all, 0)
+    s = device(sphgpu, dim, st, counts, kind, order)
+    s.set_ghost_nodes(2.0, cull=True)
+    s.omega_gradh()
+    assert close(s.download("grad h corrections"), om_all, order, dim, "omega")
+    # the derivative pass with the correction
+    s.sum_density()
+    rho_all = np.array(full["rho"])
+    rho_all[:N] = oracle.sum_density(T, N, full["pos"], full["mass"], full["H"], off, nbr, lists=full["list"])
+    full["rho"] = oracle.apply_ghosts(dim, N, bo, info["control"], info["sub_of"], rho_all, 0)
+    full["omega"] = om_all
+    ref = oracle.eval_derivs(T, oracle.make_params(), N, full, off, nbr, lists=full["list"])
+    s.evaluate_derivatives(sphgpu.hydro_params())
+    for f in sphgpu.DERIV_ALIASES:
+        assert close(s.derivative(f).reshape(ref[f].shape), ref[f], order, dim, f), f
+
+
+@pytest.mark.parametrize("case", STEP_CASES[:2], ids=[f"{c[2]}{c[0]}d" for c in STEP_CASES[:2]])
+def test_rk2_gradh_with_nodelists(sphgpu, oracle, case):
+    """SynchronousRK2 with the grad-h correction over two NodeLists (gamma
+    1.4 and 5/3): the oracle's dt sequence and state, omega included."""
+    import integrator
+    dim, n, kind, steps = case
+    print(f"[nodelists-rk2-gradh] dim={dim} n={n} {kind}", flush=True)
+    gammas = (1.4, 5.0 / 3.0)
+    st, lists, counts = two_lists(dim, n, seed=140 + dim, mass_ratio=2.0, gamma=gammas)
+    N = lists.size
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(dim, base, oracle.box_boundaries(dim, kind=kind), gammas[0], lists=lists,
+                         gammas=gammas, gradh=True)
+    s = device(sphgpu, dim, base, counts, kind, sphgpu.ROW_ORDER_REFERENCE, gamma=gammas)
+    s.eos_gamma_law(gammas[0])
+    hp = sphgpu.hydro_params()
+    ip = sphgpu.integrator_params(gamma=gammas[0], gradhCorrection=1)
+    for k in range(steps):
+        assert s.rk2_step(hp, ip) == ref.step(), k
+    fields = dict(STEP_FIELDS, omega="grad h corrections")
+    for key, f in fields.items():
+        a = s.download(f, rows=N).reshape(ref.st[key].shape)
+        assert np.array_equal(a, ref.st[key]), key
