@@ -162,8 +162,9 @@ int sphgpu_set_nodes(sphgpu_sim* sim, int64_t n_internal, int64_t n_total);
  * null) gives each NodeList its gamma-law EOS for sphgpu_eos_gamma_law and the
  * time step.  The grad-h correction sums over every NodeList alike
  * (src/SPH/computeSPHOmegaGradhCorrection.cc:55-130).  n = 1 returns to one
- * NodeList.  Not with mixed precision, compatible energy, iterateIdealH or
- * ensembles (SPHGPU_ERR_UNSUPPORTED). */
+ * NodeList.  Compatible energy replays each node's pair terms in the order
+ * the owner loop delivers them (reference row order).  Not with mixed
+ * precision, iterateIdealH or ensembles (SPHGPU_ERR_UNSUPPORTED). */
 int sphgpu_set_node_lists(sphgpu_sim* sim, int n, const int64_t* internal_counts, const double* gamma,
                           int sum_over_all_node_lists);
 /* Copy one state field host -> device: n_total rows, or, when the ghosts were

@@ -1358,8 +1358,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   if (mixed)
     SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
                "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
-  SG_REQUIRE(s.nLists <= 1 || (!mixed && !compat), SPHGPU_ERR_UNSUPPORTED,
-             "several NodeLists take the FP64 pass without compatible energy");
+  SG_REQUIRE(s.nLists <= 1 || !mixed, SPHGPU_ERR_UNSUPPORTED, "several NodeLists take the FP64 pass");
   const int sec = pairRecordSec(s, hp);
   const bool fast64 = sec == 2;
 

@@ -207,7 +207,10 @@ class RK2:
                 acc = oracle.apply_ghosts(dim, n, self.b, info["control"], info["sub_of"], acc, 1)
                 de0 = oracle.apply_ghosts(dim, n, self.b, info["control"], info["sub_of"], de0, 0)
             d["_acc_all"], d["_depsdt0_all"] = acc, de0
-            d["_rows"] = (off, nbr)
+            # the rows the pair accelerations sit in (several NodeLists: the
+            # delivery order, whose owned entries come in ConnectivityMap
+            # order, as the policy walks them)
+            d["_rows"] = (off, d.get("nbr_order", nbr))
         return d
 
     def _compatible_eps(self, full, base, d, mult):

@@ -314,8 +314,8 @@ def eval_derivs(table, params, n_internal, state, off, nbr, gather=False, pair_a
     if lists is not None:
         keep["list"] = np.ascontiguousarray(lists, dtype=np.int32)
         st.list = _p(keep["list"], _i32p)
+        nbr_in = nbr
         nbr = np.ascontiguousarray(list_order(n_internal, keep["list"], off, nbr, receive=gather), dtype=np.int32)
-        assert not pair_acc, "pairAccelerations with several NodeLists are not restated"
     else:
         st.list = C.cast(None, _i32p)
     outs = {f: np.zeros((n_internal, ncomp(f, dim))) for f in OUTPUT_FIELDS}
@@ -329,7 +329,13 @@ def eval_derivs(table, params, n_internal, state, off, nbr, gather=False, pair_a
     if not gather:
         res["ncalc"] = int(ncalc[0])
     if pair_acc:
+        # each node's pair accelerations in the order they reach it (the
+        # scatter's pushes); with several NodeLists that is the delivery order,
+        # returned as rows in which they sit entry for entry
         res["pairAcc"] = pa[:int(off[-1])]
+        if lists is not None:
+            res["nbr_order"] = np.ascontiguousarray(list_order(n_internal, keep["list"], off, nbr_in, receive=True),
+                                                    dtype=np.int32)
     return res
 
 

@@ -318,3 +318,26 @@ def test_host_class_nodelists_must_share_settings(tmp_path):
     write_records(fin, rec)
     r = subprocess.run([HOST_BIN, "derivs", fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "one nPerh" in r.stderr, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("dim,n,kind", [(1, 80, "reflecting"), (2, 14, "periodic")], ids=["reflecting1d", "periodic2d"])
+def test_host_class_nodelists_sph_defaults(oracle, dim, n, kind, tmp_path):
+    """The SPH() defaults (grad-h correction and compatible energy) over a
+    DataBase of two NodeLists with their own gamma, through SynchronousRK2:
+    the oracle's dt sequence and every NodeList's state value for value."""
+    import integrator
+    gammas = (1.4, 5.0 / 3.0)
+    st, lists, counts = two_lists(dim, n, 60 + dim)
+    N = lists.size
+    steps = 3
+    print(f"[host class] nodelists SPH() defaults dim={dim} {kind}", flush=True)
+    rec = state_records(dim, st, N, N, list_counts=counts, gammas=list(gammas), steps=steps, gradh=1, compatible=1,
+                        boundaries=box_records(dim, kind))
+    got = run_host("rk2", rec, tmp_path)
+    ref = integrator.RK2(dim, {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")},
+                         oracle.box_boundaries(dim, kind=kind), gammas[0], lists=lists, gammas=gammas,
+                         gradh=True, compatible=True)
+    dts = [ref.step() for _ in range(steps)]
+    assert list(got["dt"]) == dts
+    for key in ("pos", "vel", "rho", "eps", "H", "P", "cs"):
+        assert np.array_equal(got[key].reshape(ref.st[key].shape), ref.st[key]), key

@@ -242,3 +242,36 @@ def test_rk2_gradh_with_nodelists(sphgpu, oracle, case):
     for key, f in fields.items():
         a = s.download(f, rows=N).reshape(ref.st[key].shape)
         assert np.array_equal(a, ref.st[key]), key
+
+
+@pytest.mark.parametrize("case", STEP_CASES[:2], ids=[f"{c[2]}{c[0]}d" for c in STEP_CASES[:2]])
+def test_rk2_compatible_with_nodelists(sphgpu, oracle, case):
+    """SynchronousRK2 with compatibleEnergyEvolution over two NodeLists (gamma
+    1.4 and 5/3): each node's pair terms replayed in the order the owner loop
+    delivers them; the oracle's dt sequence and state value for value, and in
+    the periodic box total energy to round-off."""
+    import integrator
+    dim, n, kind, steps = case
+    print(f"[nodelists-rk2-compatible] dim={dim} n={n} {kind}", flush=True)
+    gammas = (1.4, 5.0 / 3.0)
+    st, lists, counts = two_lists(dim, n, seed=150 + dim, mass_ratio=2.0, gamma=gammas)
+    N = lists.size
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(dim, base, oracle.box_boundaries(dim, kind=kind), gammas[0], lists=lists,
+                         gammas=gammas, compatible=True)
+    s = device(sphgpu, dim, base, counts, kind, sphgpu.ROW_ORDER_REFERENCE, gamma=gammas)
+    s.eos_gamma_law(gammas[0])
+    hp = sphgpu.hydro_params(compatibleEnergyEvolution=1)
+    ip = sphgpu.integrator_params(gamma=gammas[0])
+    for k in range(steps):
+        assert s.rk2_step(hp, ip) == ref.step(), k
+    for key, f in STEP_FIELDS.items():
+        a = s.download(f, rows=N).reshape(ref.st[key].shape)
+        assert np.array_equal(a, ref.st[key]), key
+    if kind == "periodic":
+        m = st["mass"]
+        v = s.download("velocity", rows=N).reshape(N, dim)
+        e = s.download("specific thermal energy", rows=N)
+        E0 = (0.5 * m * (st["vel"] ** 2).sum(1) + m * st["eps"]).sum()
+        E1 = (0.5 * m * (v ** 2).sum(1) + m * e).sum()
+        assert abs(E1 - E0) <= 1e-13 * E0

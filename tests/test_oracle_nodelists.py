@@ -206,3 +206,33 @@ def test_integrator_gradh_with_nodelists_runs_the_correction(oracle):
         assert a.step() == c.step()
     for k in ("pos", "vel", "eps", "rho", "H"):
         assert np.array_equal(a.st[k], c.st[k]), k
+
+
+def test_compatible_energy_with_nodelists_conserved(oracle):
+    """compatibleEnergyEvolution over two NodeLists (periodic box, gamma 1.4
+    and 5/3): every pair's work goes to the thermal energies of its two ends,
+    so sum m (v^2/2 + eps) holds to round-off; pairing an end with another
+    entry's pair acceleration (a wrong row order) breaks it.  One NodeList
+    label gives the plain compatible step bit for bit."""
+    import integrator
+    dim = 2
+    st, lists, N = _two_lists(dim, 10, seed=17, mass_ratio=2.0)
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    b = oracle.box_boundaries(dim, kind="periodic")
+    m = st["mass"]
+    E0 = (0.5 * m * (st["vel"] ** 2).sum(1) + m * st["eps"]).sum()
+    drift = {}
+    for compatible in (False, True):
+        r = integrator.RK2(dim, base, b, 1.4, compatible=compatible, lists=lists, gammas=(1.4, 5.0 / 3.0))
+        for _ in range(4):
+            r.step()
+        E1 = (0.5 * m * (r.st["vel"] ** 2).sum(1) + m * r.st["eps"]).sum()
+        drift[compatible] = abs(E1 - E0) / E0
+    assert drift[True] <= 1e-14, drift
+    assert drift[False] > 10 * drift[True], drift
+    a = integrator.RK2(dim, base, b, 1.4, compatible=True)
+    c = integrator.RK2(dim, base, b, 1.4, compatible=True, lists=np.zeros(N, dtype=np.int32), gammas=(1.4,))
+    for _ in range(3):
+        assert a.step() == c.step()
+    for k in ("pos", "vel", "eps", "rho", "H"):
+        assert np.array_equal(a.st[k], c.st[k]), k
