@@ -1494,7 +1494,8 @@ __global__ void __launch_bounds__(kBlock) k_ideal_h(const double* __restrict__ r
                                                     const int32_t* __restrict__ work, const int32_t* __restrict__ perm,
                                                     TabArgs t, double nPerh, double hmin, double hmax, double tol,
                                                     double* __restrict__ H, double* __restrict__ deltaH,
-                                                    unsigned long long* __restrict__ maxBits) {
+                                                    unsigned long long* __restrict__ maxBits,
+                                                    const double* __restrict__ listOf) {
   using R = RL<ISO>;
   constexpr int NS = NSym<D>::value;
   extern __shared__ double lds[];
@@ -1510,6 +1511,9 @@ __global__ void __launch_bounds__(kBlock) k_ideal_h(const double* __restrict__ r
       const double* ri_ = rec + k * R::STRIDE;
       V<D> xi;
       for (int c = 0; c < D; ++c) xi.c[c] = ri_[R::X + c];
+      // several NodeLists: a neighbour of another NodeList counts with
+      // m_j rho_i / (m_i rho_j) (iterateIdealH.cc:183-194)
+      const double li = listOf ? listOf[i] : 0.0, mi = ri_[R::M], rhoi = ri_[R::RHO];
       double zeroth = 0.0;
       const int len = rowCount[k];
       for (int w = 0; w < len; ++w) {
@@ -1519,7 +1523,9 @@ __global__ void __launch_bounds__(kBlock) k_ideal_h(const double* __restrict__ r
         const double etai = vmag(sdot(Hi, xij));
         double Wv, gW;
         lookupWG(tab, t, etai, 1.0, Wv, gW);
-        zeroth += 1.0 * dabs(gW);
+        double fw = 1.0;
+        if (listOf && listOf[int64_t(rj[R::ORIG])] != li) fw = rj[R::M] * rhoi / (mi * rj[R::RHO]);
+        zeroth += fw * dabs(gW);
       }
       zeroth = rootnu<D>(zeroth);
       const double c1 = idealHinv(t, zeroth, Hi.c[0], nPerh, hmin, hmax);
@@ -1544,7 +1550,8 @@ double idealHPassD(Sim& s, const sphgpu_hydro_params& hp, double tol, double* de
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), ldsBytes, s.stream, s.rec.as<double>(),
                        s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt, s.work.as<int32_t>(),
-                       s.perm.as<int32_t>(), t, hp.nPerh, hp.hmin, hp.hmax, tol, s.H.as<double>(), deltaH, bits);
+                       s.perm.as<int32_t>(), t, hp.nPerh, hp.hmin, hp.hmax, tol, s.H.as<double>(), deltaH, bits,
+                       s.nLists > 1 ? s.seg.as<double>() : nullptr);
   };
   if (s.isotropic) { if (lds) go(k_ideal_h<D, true, true>); else go(k_ideal_h<D, true, false>); }
   else { if (lds) go(k_ideal_h<D, false, true>); else go(k_ideal_h<D, false, false>); }

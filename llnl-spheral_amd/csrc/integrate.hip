@@ -942,7 +942,6 @@ int iterateIdealH(Sim& s, const sphgpu_hydro_params& hp, int maxIterations, doub
                   double* maxDelta) {
   SG_REQUIRE(s.nInt > 0, SPHGPU_ERR_STATE, "no nodes");
   SG_REQUIRE(!s.ensemble, SPHGPU_ERR_STATE, "ensembles iterate H per member alone");
-  SG_REQUIRE(s.nLists <= 1, SPHGPU_ERR_UNSUPPORTED, "iterateIdealH takes one NodeList");
   SG_REQUIRE(s.nTot == s.nInt || s.deviceGhosts, SPHGPU_ERR_STATE,
              "iterateIdealH makes its own ghosts: set_nodes(n_internal, n_internal)");
   SG_REQUIRE(maxIterations >= 0 && tol >= 0.0 && ext > 0.0, SPHGPU_ERR_ARG, "bad iteration parameters");
@@ -953,14 +952,24 @@ int iterateIdealH(Sim& s, const sphgpu_hydro_params& hp, int maxIterations, doub
   hipLaunchKernelGGL(k_fill_value, dim3(nblocks(s.nInt)), dim3(kBlock), 0, s.stream, deltaH.as<double>(), s.nInt, init);
   double m = 2.0 * tol;
   int it = 0;
-  while (it < maxIterations && m > tol) {
-    ++it;
-    if (!s.subs.empty()) setGhostNodes(s, ext, cull);
-    else buildNeighbors(s, ext);
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s.stream, bits.as<unsigned long long>(), 0ull);
-    m = idealHPass(s, hp, tol, deltaH.as<double>(), bits.as<unsigned long long>());
+  // several NodeLists: the pass walks each node's full connectivity, in
+  // ConnectivityMap order (reference row order)
+  struct MapOrder {
+    Sim& s;
+    explicit MapOrder(Sim& x) : s(x) { s.rowsMapOrder = s.nLists > 1; }
+    ~MapOrder() { s.rowsMapOrder = false; }
+  };
+  {
+    MapOrder mo(s);
+    while (it < maxIterations && m > tol) {
+      ++it;
+      if (!s.subs.empty()) setGhostNodes(s, ext, cull);
+      else buildNeighbors(s, ext);
+      hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s.stream, bits.as<unsigned long long>(), 0ull);
+      m = idealHPass(s, hp, tol, deltaH.as<double>(), bits.as<unsigned long long>());
+    }
   }
-  // leave the ghosts made from the final H
+  // leave the ghosts made from the final H (rows in the pass's order)
   if (!s.subs.empty()) setGhostNodes(s, ext, cull);
   else s.neighborsValid = false;
   SG_CHECK_HIP(hipStreamSynchronize(s.stream));

@@ -194,6 +194,8 @@ struct Sim {
   DBuf genDev;                   // device-held counts of a ghost generation
   DBuf spare[13];                // one per caller-order state field (culling compacts into it)
   bool markGhostUse = false;     // the next build marks the ghost slots rows hold (for culling)
+  bool rowsMapOrder = false;     // several NodeLists, reference order: rows in ConnectivityMap order
+                                 // (iterateIdealH) instead of the owner loop's delivery order
 
   // Slots of the current build.  Culling renumbers the caller-order rows but
   // keeps the slot space: culled ghosts become dead slots (perm = -1) that no

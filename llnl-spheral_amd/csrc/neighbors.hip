@@ -883,8 +883,12 @@ __device__ void sortRowWave(int32_t* __restrict__ nbr, int64_t n, int64_t k, int
 // that order), then x's own pairs in ConnectivityMap order, by (NodeList,
 // index within the NodeList), a NodeList's ghosts after its internal nodes.
 // With one NodeList this is ascending caller index.
-__device__ __forceinline__ int64_t receiveKey(int32_t c, int32_t x, const double* __restrict__ seg, int64_t nInt) {
-  if (c < nInt && c < x) return int64_t(c);
+// mapOrder: ConnectivityMap's own order, by (NodeList, index within the
+// NodeList), for the loops that walk a node's full connectivity
+// (iterateIdealH.cc:164-205) rather than receive from owners.
+__device__ __forceinline__ int64_t receiveKey(int32_t c, int32_t x, const double* __restrict__ seg, int64_t nInt,
+                                              bool mapOrder) {
+  if (!mapOrder && c < nInt && c < x) return int64_t(c);
   const int64_t local = c < nInt ? int64_t(c) : (int64_t(1) << 31) + c;
   return (int64_t(1) << 62) | (int64_t(seg[c]) << 32) | local;
 }
@@ -896,13 +900,13 @@ __device__ __forceinline__ int32_t callerOfKey(int64_t key) {
 template <int K>
 __device__ void sortRowWaveLists(int32_t* __restrict__ nbr, int64_t n, int64_t k, int len,
                                  const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
-                                 const double* __restrict__ seg, int64_t nInt, int lane) {
+                                 const double* __restrict__ seg, int64_t nInt, bool mapOrder, int lane) {
   const int32_t x = perm[k];
   int64_t key[K];
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     const int e = q * 64 + lane;
-    key[q] = e < len ? receiveKey(perm[nbr[ellIndex(e, n, k)]], x, seg, nInt) : INT64_MAX;
+    key[q] = e < len ? receiveKey(perm[nbr[ellIndex(e, n, k)]], x, seg, nInt, mapOrder) : INT64_MAX;
   }
   waveBitonic<K>(key, lane);
 #pragma unroll
@@ -915,23 +919,24 @@ __device__ void sortRowWaveLists(int32_t* __restrict__ nbr, int64_t n, int64_t k
 __global__ void __launch_bounds__(kBlock) k_row_sort(const int32_t* __restrict__ rowCount, int32_t* __restrict__ nbr,
                                                     int64_t n, const int32_t* __restrict__ perm,
                                                     const int32_t* __restrict__ iperm,
-                                                    const double* __restrict__ seg, int64_t nInt) {
+                                                    const double* __restrict__ seg, int64_t nInt, int mapOrder) {
   const int lane = threadIdx.x & 63;
   const int64_t k = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6;
   if (k >= n) return;                              // uniform per wavefront
   const int len = rowCount[k];
   if (len <= 1) return;
   if (seg) {
-    if (len <= 64) sortRowWaveLists<1>(nbr, n, k, len, perm, iperm, seg, nInt, lane);
-    else if (len <= 128) sortRowWaveLists<2>(nbr, n, k, len, perm, iperm, seg, nInt, lane);
-    else if (len <= 256) sortRowWaveLists<4>(nbr, n, k, len, perm, iperm, seg, nInt, lane);
+    const bool mo = mapOrder != 0;
+    if (len <= 64) sortRowWaveLists<1>(nbr, n, k, len, perm, iperm, seg, nInt, mo, lane);
+    else if (len <= 128) sortRowWaveLists<2>(nbr, n, k, len, perm, iperm, seg, nInt, mo, lane);
+    else if (len <= 256) sortRowWaveLists<4>(nbr, n, k, len, perm, iperm, seg, nInt, mo, lane);
     else if (lane == 0) {
       const int32_t x = perm[k];
       for (int a2 = 1; a2 < len; ++a2) {
         const int32_t v = nbr[ellIndex(a2, n, k)];
-        const int64_t pv = receiveKey(perm[v], x, seg, nInt);
+        const int64_t pv = receiveKey(perm[v], x, seg, nInt, mo);
         int q = a2 - 1;
-        while (q >= 0 && receiveKey(perm[nbr[ellIndex(q, n, k)]], x, seg, nInt) > pv) {
+        while (q >= 0 && receiveKey(perm[nbr[ellIndex(q, n, k)]], x, seg, nInt, mo) > pv) {
           nbr[ellIndex(q + 1, n, k)] = nbr[ellIndex(q, n, k)];
           --q;
         }
@@ -1397,7 +1402,7 @@ void buildNeighborsD(Sim& s, double ext) {
   if (s.rowOrder == SPHGPU_ROW_ORDER_REFERENCE && n > 0)
     hipLaunchKernelGGL(k_row_sort, dim3(nblocks(n * 64)), dim3(kBlock), 0, st, s.rowCount.as<int32_t>(),
                        s.nbr.as<int32_t>(), n, s.perm.as<int32_t>(), s.iperm.as<int32_t>(),
-                       s.nLists > 1 ? s.seg.as<double>() : nullptr, s.nInt);
+                       s.nLists > 1 ? s.seg.as<double>() : nullptr, s.nInt, s.rowsMapOrder ? 1 : 0);
   // 5. work list of internal slots
   s.workFlag.ensure(sizeof(int32_t) * size_t(n + 1));
   s.workScan.ensure(sizeof(int32_t) * size_t(n + 1));

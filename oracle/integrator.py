@@ -261,14 +261,16 @@ class RK2:
 
 
 def iterate_ideal_h(dim, state, boundaries, max_iterations=100, tolerance=1.0e-10, params=None, table_points=1000,
-                    ext=2.0, cull=True):
+                    ext=2.0, cull=True, lists=None):
     """iterateIdealH (src/Utilities/iterateIdealH.cc:30-310) with the SPH
     smoothing scale, nPerhForIteration = 0, no spherical start, no fixed
     determinant: each iteration rebuilds the ghosts and the connectivity from
     the current H, then passes over the internal nodes (oracle.ideal_h_pass);
     the loop ends after max_iterations or when the largest change is within
-    the tolerance.  Returns (H of the internal nodes, iterations, last largest
-    change)."""
+    the tolerance.  lists (NodeList of every internal node): several
+    NodeLists, each making its own ghosts; neighbours of another NodeList are
+    weighted by m_j rho_i / (m_i rho_j).  Returns (H of the internal nodes,
+    iterations, last largest change)."""
     n = state["pos"].shape[0]
     T = oracle.Table(dim, table_points)
     p = params if params is not None else oracle.make_params()
@@ -279,11 +281,17 @@ def iterate_ideal_h(dim, state, boundaries, max_iterations=100, tolerance=1.0e-1
     while it < max_iterations and maxDelta > tolerance:
         it += 1
         if boundaries:
-            full, off, nbr, _ = oracle.ghost_pipeline(dim, st, boundaries, ext, cull)
+            full, off, nbr, _ = oracle.ghost_pipeline(dim, st, boundaries, ext, cull, lists=lists)
         else:
             full = dict(st)
             off, nbr = oracle.connectivity(dim, n, st["pos"], st["H"], ext)
+            if lists is not None:
+                full["list"] = np.asarray(lists, dtype=np.int32)
         H = np.ascontiguousarray(full["H"], dtype=np.float64).copy()
-        maxDelta = oracle.ideal_h_pass(T, p, n, full["pos"], H, deltaH, off, nbr, tolerance)
+        if lists is None:
+            maxDelta = oracle.ideal_h_pass(T, p, n, full["pos"], H, deltaH, off, nbr, tolerance)
+        else:
+            maxDelta = oracle.ideal_h_pass(T, p, n, full["pos"], H, deltaH, off, nbr, tolerance, lists=full["list"],
+                                           mass=full["mass"], rho=full["rho"])
         st["H"] = H[:n].copy()
     return st["H"], it, maxDelta

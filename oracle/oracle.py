@@ -137,7 +137,7 @@ def lib(fast=False):
                                             C.c_double, _dp]
         L.orc_omega_gradh.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int64, _dp, _dp, _i64p, _i32p, _dp]
         L.orc_ideal_h_pass.argtypes = [C.c_void_p, C.c_int, C.POINTER(Params), C.c_int64, _dp, _dp, _dp, _i64p, _i32p,
-                                       C.c_double]
+                                       C.c_double, _i32p, _dp, _dp]
         L.orc_ideal_h_pass.restype = C.c_double
         L.orc_gamma_law.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     _dp, _dp]
@@ -339,14 +339,24 @@ def eval_derivs(table, params, n_internal, state, off, nbr, gather=False, pair_a
     return res
 
 
-def ideal_h_pass(table, params, n_internal, pos, H, deltaH, off, nbr, tol):
+def ideal_h_pass(table, params, n_internal, pos, H, deltaH, off, nbr, tol, lists=None, mass=None, rho=None):
     """One iterateIdealH pass (SPH smoothing scale); H (all rows, internal ones
     replaced) and deltaH (internal rows) are updated in place.  Returns the
-    largest change of the nodes computed."""
+    largest change of the nodes computed.  lists, mass, rho (every node):
+    several NodeLists; a neighbour of another NodeList then counts with
+    m_j rho_i / (m_i rho_j) (iterateIdealH.cc:183-194) and node i walks its
+    full connectivity in ConnectivityMap order."""
     assert H.flags.c_contiguous and deltaH.flags.c_contiguous and H.dtype == np.float64
     pos = np.ascontiguousarray(pos, dtype=np.float64)
+    la = ma = ra = None
+    if lists is not None:
+        la = np.ascontiguousarray(lists, dtype=np.int32)
+        ma = np.ascontiguousarray(mass, dtype=np.float64)
+        ra = np.ascontiguousarray(rho, dtype=np.float64)
+        nbr = np.ascontiguousarray(list_order(n_internal, la, off, nbr, receive=False), dtype=np.int32)
     return table.L.orc_ideal_h_pass(table.h, table.dim, C.byref(params), n_internal, _p(pos), _p(H), _p(deltaH),
-                                    _p(off, _i64p), _p(nbr, _i32p), tol)
+                                    _p(off, _i64p), _p(nbr, _i32p), tol,
+                                    _p(la, _i32p), _p(ma), _p(ra))
 
 
 def compatible_depsdt(dim, n_internal, off, nbr, mass, vel, acc, DepsDt0, pair_acc, multiplier):

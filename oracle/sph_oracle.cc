@@ -572,7 +572,10 @@ template <int D> static NodeState<D> toNS(const orc_state* st) {
 // the SPH smoothing scale, over the internal nodes with the current
 // connectivity (rows in index order, ghosts included): nodes whose last change
 // is within `tol` keep their H; the others take
-//   zeroth = rootnu(sum_j |W.gradValue(|H_i x_ij|, 1)|)   (fweight 1, one NodeList)
+//   zeroth = rootnu(sum_j fweight_ij |W.gradValue(|H_i x_ij|, 1)|)
+// with fweight_ij = 1 within a NodeList and m_j rho_i / (m_i rho_j) across
+// NodeLists (iterateIdealH.cc:183-194; list, mass, rho null: one NodeList),
+// the row walked in ConnectivityMap order
 //   H1 = SPHSmoothingScale::idealSmoothingScale(H_i, zeroth)
 // and the change max|phi - 1| over the eigenvalues phi of H1^1/2 H_i^-1 H1^1/2;
 // with H1 = c I these are c/lambda(H_i), lambda from minEig/maxEig.  H is
@@ -581,7 +584,8 @@ template <int D> static NodeState<D> toNS(const orc_state* st) {
 template <int D>
 double idealHPass(const Table& W, const Params& p, int64_t nInt, const double* pos, double* H, double* deltaH,
                   const int64_t* off, const int32_t* nbr, double tol, double (*minEig)(int, const double*),
-                  double (*maxEig)(int, const double*)) {
+                  double (*maxEig)(int, const double*), const int32_t* list = nullptr, const double* mass = nullptr,
+                  const double* rho = nullptr) {
   constexpr int NS = nsym<D>();
   std::vector<double> H1(static_cast<size_t>(nInt) * NS);
   double maxDelta = 0.0;
@@ -599,10 +603,12 @@ double idealHPass(const Table& W, const Params& p, int64_t nInt, const double* p
       const int64_t j = nbr[e];
       Vec<D> rj;
       for (int k = 0; k < D; ++k) rj.c[k] = pos[j * D + k];
+      double fweightij = 1.0;
+      if (list && list[j] != list[i]) fweightij = mass[j] * rho[i] / (mass[i] * rho[j]);
       const Vec<D> xij = sub(ri, rj);
       const double etai = mag(sdot(Hi, xij));
       const double Wi = std::fabs(W.gradValue(etai, 1.0));
-      zeroth += 1.0 * Wi;
+      zeroth += fweightij * Wi;
     }
     zeroth = rootnu<D>(zeroth);
     const Sym<D> Hn = idealH<D>(W, Hi, zeroth, p);
@@ -697,11 +703,14 @@ void orc_sum_density_lists(void* tp, int dim, int gather, int64_t nInt, int64_t 
 }
 
 double orc_ideal_h_pass(void* tp, int dim, const Params* p, int64_t nInt, const double* pos, double* H, double* deltaH,
-                        const int64_t* off, const int32_t* nbr, double tol) {
+                        const int64_t* off, const int32_t* nbr, double tol, const int32_t* list, const double* mass,
+                        const double* rho) {
   const Table& W = *static_cast<Table*>(tp);
-  if (dim == 1) return orc::idealHPass<1>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen);
-  if (dim == 2) return orc::idealHPass<2>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen);
-  return orc::idealHPass<3>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen);
+  if (dim == 1)
+    return orc::idealHPass<1>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen, list, mass, rho);
+  if (dim == 2)
+    return orc::idealHPass<2>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen, list, mass, rho);
+  return orc::idealHPass<3>(W, *p, nInt, pos, H, deltaH, off, nbr, tol, orc_min_eigen, orc_max_eigen, list, mass, rho);
 }
 
 void orc_omega_gradh(void* tp, int dim, int64_t nInt, int64_t nTot, const double* pos, const double* H,

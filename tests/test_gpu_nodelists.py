@@ -275,3 +275,31 @@ def test_rk2_compatible_with_nodelists(sphgpu, oracle, case):
         E0 = (0.5 * m * (st["vel"] ** 2).sum(1) + m * st["eps"]).sum()
         E1 = (0.5 * m * (v ** 2).sum(1) + m * e).sum()
         assert abs(E1 - E0) <= 1e-13 * E0
+
+
+@pytest.mark.parametrize("case", [(1, 80, "reflecting"), (2, 14, "periodic"), (3, 7, "reflecting")],
+                         ids=["reflecting1d", "periodic2d", "reflecting3d"])
+def test_iterate_ideal_h_with_nodelists(sphgpu, oracle, case):
+    """iterateIdealH over two NodeLists (the second twice as heavy): each
+    NodeList's own ghosts, every node's full connectivity in ConnectivityMap
+    order and neighbours of the other NodeList weighted by m_j rho_i /
+    (m_i rho_j).  Same iteration count; H value for value in 1-D and 2-D,
+    1e-12 in 3-D (cube root); the pass after it runs on the rows it leaves."""
+    import integrator
+    dim, n, kind = case
+    print(f"[nodelists-idealh] dim={dim} n={n} {kind}", flush=True)
+    st, lists, counts = two_lists(dim, n, seed=170 + dim, mass_ratio=2.0)
+    N = lists.size
+    Hr, itr, mdr = integrator.iterate_ideal_h(dim, {k: st[k] for k in ("pos", "mass", "rho", "H")},
+                                              oracle.box_boundaries(dim, kind=kind), lists=lists)
+    s = device(sphgpu, dim, st, counts, kind, sphgpu.ROW_ORDER_REFERENCE)
+    it, md = s.iterate_ideal_h(sphgpu.hydro_params(), 100, 1.0e-10)
+    assert it == itr, (it, itr)
+    Hd = s.download("H", rows=N).reshape(Hr.shape)
+    if dim < 3:
+        assert np.array_equal(Hd, Hr)
+        assert md == mdr
+    else:
+        assert np.abs(Hd - Hr).max() <= 1e-12 * np.abs(Hr).max()
+    s.sum_density()
+    s.evaluate_derivatives(sphgpu.hydro_params())

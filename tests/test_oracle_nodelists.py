@@ -236,3 +236,27 @@ def test_compatible_energy_with_nodelists_conserved(oracle):
         assert a.step() == c.step()
     for k in ("pos", "vel", "eps", "rho", "H"):
         assert np.array_equal(a.st[k], c.st[k]), k
+
+
+@pytest.mark.parametrize("dim,n", [(1, 60), (2, 12)])
+def test_ideal_h_weights_across_nodelists(oracle, dim, n):
+    """iterateIdealH's zeroth moment weights a neighbour of another NodeList
+    by m_j rho_i / (m_i rho_j) (iterateIdealH.cc:183-194): with one NodeList
+    label the pass is the one-NodeList pass bit for bit; with two NodeLists
+    whose m/rho agree the weight is 1 and the pass is the one-NodeList one up
+    to the order of the sum; doubling one NodeList's mass alone changes it."""
+    import integrator
+    st, lists, N = _two_lists(dim, n, seed=160 + dim, mass_ratio=1.0)
+    b = oracle.box_boundaries(dim, kind="reflecting")
+    base = {k: st[k] for k in ("pos", "mass", "rho", "H")}
+    one, it1, _ = integrator.iterate_ideal_h(dim, base, b, max_iterations=3)
+    lab, it2, _ = integrator.iterate_ideal_h(dim, base, b, max_iterations=3, lists=np.zeros(N, dtype=np.int32))
+    assert it1 == it2 and np.array_equal(one, lab)
+    # m and rho both doubled in NodeList 1: weight m_j rho_i / (m_i rho_j) = 1
+    same = dict(base, mass=base["mass"] * np.where(lists == 1, 2.0, 1.0),
+                rho=base["rho"] * np.where(lists == 1, 2.0, 1.0))
+    two, _, _ = integrator.iterate_ideal_h(dim, same, b, max_iterations=3, lists=lists)
+    assert np.abs(two - one).max() <= 1e-14 * np.abs(one).max()
+    heavy = dict(base, mass=base["mass"] * np.where(lists == 1, 2.0, 1.0))
+    diff, _, _ = integrator.iterate_ideal_h(dim, heavy, b, max_iterations=3, lists=lists)
+    assert np.abs(diff - one).max() > 1e-6 * np.abs(one).max()
