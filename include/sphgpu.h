@@ -163,8 +163,10 @@ int sphgpu_set_nodes(sphgpu_sim* sim, int64_t n_internal, int64_t n_total);
  * time step.  The grad-h correction sums over every NodeList alike
  * (src/SPH/computeSPHOmegaGradhCorrection.cc:55-130).  n = 1 returns to one
  * NodeList.  Compatible energy replays each node's pair terms in the order
- * the owner loop delivers them (reference row order).  Not with mixed
- * precision, iterateIdealH or ensembles (SPHGPU_ERR_UNSUPPORTED). */
+ * the owner loop delivers them (reference row order); sphgpu_iterate_ideal_h
+ * weights a neighbour of another NodeList by m_j rho_i / (m_i rho_j) and walks
+ * rows in ConnectivityMap order (src/Utilities/iterateIdealH.cc:164-205).  Not
+ * with mixed precision or ensembles (SPHGPU_ERR_UNSUPPORTED / SPHGPU_ERR_STATE). */
 int sphgpu_set_node_lists(sphgpu_sim* sim, int n, const int64_t* internal_counts, const double* gamma,
                           int sum_over_all_node_lists);
 /* Copy one state field host -> device: n_total rows, or, when the ghosts were
