@@ -165,8 +165,9 @@ int sphgpu_set_nodes(sphgpu_sim* sim, int64_t n_internal, int64_t n_total);
  * NodeList.  Compatible energy replays each node's pair terms in the order
  * the owner loop delivers them (reference row order); sphgpu_iterate_ideal_h
  * weights a neighbour of another NodeList by m_j rho_i / (m_i rho_j) and walks
- * rows in ConnectivityMap order (src/Utilities/iterateIdealH.cc:164-205).  Not
- * with mixed precision or ensembles (SPHGPU_ERR_UNSUPPORTED / SPHGPU_ERR_STATE). */
+ * rows in ConnectivityMap order (src/Utilities/iterateIdealH.cc:164-205).
+ * Mixed precision applies the same NodeList rules in its FP32 kernels.  Not
+ * in ensembles (SPHGPU_ERR_STATE). */
 int sphgpu_set_node_lists(sphgpu_sim* sim, int n, const int64_t* internal_counts, const double* gamma,
                           int sum_over_all_node_lists);
 /* Copy one state field host -> device: n_total rows, or, when the ghosts were

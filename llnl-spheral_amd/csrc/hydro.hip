@@ -349,6 +349,7 @@ struct PassArgs {
   double qInv;   // 1/u and u of the 3-D mixed record's positions
   float qUnit;
   const double* listOf;   // NodeList of every caller row (several NodeLists), or null
+  const int32_t* slotList;   // NodeList of every slot (several NodeLists, collapsed kernel), or null
 };
 
 __device__ __forceinline__ double muQ(double mu, double Cl, double Cq, double cs, const PassArgs& p) {
@@ -792,7 +793,8 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
                                                               const int32_t* __restrict__ rowCount,
                                                               const int32_t* __restrict__ nbr, int64_t n, int64_t nInt,
                                                               const int32_t* __restrict__ work, TabArgs t, Tab32 t32,
-                                                              double W0, double* __restrict__ rhoOut) {
+                                                              double W0, double* __restrict__ rhoOut,
+                                                              const int32_t* __restrict__ slotList, int sumAll) {
   using R = RL<true>;
   extern __shared__ float lds32[];
   const float* tab = stageTable32<LDS>(lds32, t);
@@ -814,11 +816,25 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
     if constexpr (D > 1) b = p[1];
     else b = make_float4(0.0f, 0.0f, a.z, a.w);
   };
+  // several NodeLists: a pair across NodeLists counts with sumAll only, the
+  // node taking its own mass (computeSPHSumMassDensity.cc:77-100)
+  const int32_t lk = slotList ? slotList[k] : 0;
+  const float mif = float(mi);
   float4 na, nb2;
-  if (len > 0) load(cur.x, na, nb2);
+  int32_t jn = cur.x;
+  if (len > 0) load(jn, na, nb2);
   for (int w = 0; w < len; ++w) {
     const float4 a = na, b = nb2;
-    if (w + 1 < len) load(nextIndex(nb4, n, k, len, w + 1, cur, nxt), na, nb2);
+    const int32_t j = jn;
+    if (w + 1 < len) {
+      jn = nextIndex(nb4, n, k, len, w + 1, cur, nxt);
+      load(jn, na, nb2);
+    }
+    float mj = b.w;
+    if (slotList && slotList[j] != lk) {
+      if (!sumAll) continue;
+      mj = mif;
+    }
     const double xj[3] = {__hiloint2double(__float_as_int(a.y), __float_as_int(a.x)),
                           __hiloint2double(__float_as_int(a.w), __float_as_int(a.z)),
                           __hiloint2double(__float_as_int(b.y), __float_as_int(b.x))};
@@ -830,7 +846,7 @@ __global__ void __launch_bounds__(kBlock) k_sum_density_mixed(double* __restrict
     const float hj = b.z;
     const float hdetj = D == 1 ? hj : (D == 2 ? hj * hj : hj * hj * hj);
     const float eta = hj * __builtin_sqrtf(r2);
-    acc += double(b.w * lookupW32(tab, t32, eta, hdetj));
+    acc += double(mj * lookupW32(tab, t32, eta, hdetj));
   }
   rhoOut[int64_t(ri_[R::ORIG])] = acc;
   const double som = ri_[R::SOM], p = ri_[R::P];
@@ -922,6 +938,7 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
   constexpr bool QPOS = std::is_same<Real, float>::value && D == 3;
   if (tw < nInt) {
     const int64_t k = work[tw];
+    const int32_t lk = p.slotList ? p.slotList[k] : 0;
     const double* ri_ = rec + k * R::STRIDE;
     double xi[3];
     uint32_t qi[3];
@@ -1010,13 +1027,18 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
         const Real dsc = (pi + pj) * rinv;
         for (int c = 0; c < D; ++c) o.pairDv[int64_t(c) * o.plane + ellIndex(w, n, k)] = double(rij[c] * dsc);
       }
+      // several NodeLists: rhoSum and the normalization take same-NodeList
+      // pairs only (SPHHydroBase.cc:746-751)
+      const bool sameList = !p.slotList || p.slotList[j] == lk;
       if constexpr (F32) {
         for (int c = 0; c < D; ++c) af.DvDt[c] = fmaR(-rij[c], fsr, af.DvDt[c]);
         af.DepsDt = fmaR(mj * vdr, pi, af.DepsDt);
         af.viscW = fmaR(mj * vdr, halfQi, af.viscW);
         af.effVP = fmaR(nb.mor * Qi, Wi, af.effVP);
-        af.rhoSum = fmaR(mj, Wi, af.rhoSum);
-        af.norm = fmaR(mori, Wi, af.norm);
+        if (sameList) {
+          af.rhoSum = fmaR(mj, Wi, af.rhoSum);
+          af.norm = fmaR(mori, Wi, af.norm);
+        }
         af.wNS += absR(gWi);
         for (int c = 0; c < D; ++c) {
           const Real u = vij[c] * wg;
@@ -1042,8 +1064,10 @@ __global__ void __launch_bounds__(BS) k_derivs_iso(const double* __restrict__ re
         a.DepsDt = fmaR(mv, pi, a.DepsDt);
         a.viscW = fmaR(mv, halfQi, a.viscW);
         a.effVP = fmaR(nb.mor * Qi, Wi, a.effVP);
-        a.rhoSum = fmaR(mj, Wi, a.rhoSum);
-        a.norm = fmaR(mori, Wi, a.norm);
+        if (sameList) {
+          a.rhoSum = fmaR(mj, Wi, a.rhoSum);
+          a.norm = fmaR(mori, Wi, a.norm);
+        }
         a.wNS += absR(gWi);
         for (int c = 0; c < D; ++c) {
           const Real u = vij[c] * wg;
@@ -1248,6 +1272,26 @@ void packFast64(Sim& s) {
 
 Tab64 tab64(const Sim& s) { return Tab64{1.0 / s.table.step, s.table.extent, s.table.n}; }
 
+__global__ void k_slot_lists(const int32_t* __restrict__ perm, const double* __restrict__ seg, int64_t nSlots,
+                             int32_t* __restrict__ out) {
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k < nSlots) {
+    const int32_t i = perm[k];
+    out[k] = i >= 0 ? int32_t(seg[i]) : -1;
+  }
+}
+
+// Several NodeLists: the NodeList of every slot of the current build (for the
+// collapsed kernels, whose records hold no caller index); null with one.
+const int32_t* slotLists(Sim& s) {
+  if (s.nLists <= 1) return nullptr;
+  s.slotList.ensure(sizeof(int32_t) * size_t(std::max<int64_t>(s.nSlots, 1)));
+  hipLaunchKernelGGL(k_slot_lists, dim3(nblocks(s.nSlots)), dim3(kBlock), 0, s.stream, s.perm.as<int32_t>(),
+                     s.seg.as<double>(), s.nSlots, s.slotList.as<int32_t>());
+  SG_CHECK_HIP(hipGetLastError());
+  return s.slotList.as<int32_t>();
+}
+
 template <int D>
 void sumDensityD(Sim& s) {
   const int64_t n = s.nSlots;
@@ -1257,7 +1301,6 @@ void sumDensityD(Sim& s) {
   if (!s.capturing) SG_CHECK_HIP(hipEventRecord(s.ev[2], s.stream));
   if (s.precision == SPHGPU_PRECISION_MIXED) {
     SG_REQUIRE(s.isotropic, SPHGPU_ERR_UNSUPPORTED, "mixed precision needs H = h I and no Q multipliers");
-    SG_REQUIRE(s.nLists <= 1, SPHGPU_ERR_UNSUPPORTED, "mixed precision takes one NodeList");
     if (!s.recSMValid) {
       s.recSM.ensure(sizeof(float) * 8 * size_t(std::max<int64_t>(n, 1)));
       hipLaunchKernelGGL(k_pack_dens_mixed<D>, dim3(nblocks(n)), dim3(kBlock), 0, s.stream, s.rec.as<double>(), n,
@@ -1267,10 +1310,11 @@ void sumDensityD(Sim& s) {
     }
     const size_t lds32 = lds ? sizeof(float) * 8 * size_t(s.table.n) : 0;
     const double W0 = s.table.kernelValue(0.0, 1.0);
+    const int32_t* lists = slotLists(s);
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(xcdGrid(nblocks(s.nInt))), dim3(kBlock), lds32, s.stream, s.rec.as<double>(),
                          s.recSM.as<float4>(), s.rowCount.as<int32_t>(), s.nbr.as<int32_t>(), n, s.nInt,
-                         s.work.as<int32_t>(), t, tab32(s), W0, s.rho.as<double>());
+                         s.work.as<int32_t>(), t, tab32(s), W0, s.rho.as<double>(), lists, s.sumAllLists ? 1 : 0);
     };
     if (lds) go(k_sum_density_mixed<D, true>); else go(k_sum_density_mixed<D, false>);
     SG_CHECK_HIP(hipGetLastError());
@@ -1358,7 +1402,6 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   if (mixed)
     SG_REQUIRE(s.isotropic && !tensile, SPHGPU_ERR_UNSUPPORTED,
                "mixed precision needs H = h I, no Q multipliers and epsTensile = 0");
-  SG_REQUIRE(s.nLists <= 1 || !mixed, SPHGPU_ERR_UNSUPPORTED, "several NodeLists take the FP64 pass");
   const int sec = pairRecordSec(s, hp);
   const bool fast64 = sec == 2;
 
@@ -1392,6 +1435,7 @@ void evalDerivsD(Sim& s, const sphgpu_hydro_params& hp, bool updateOnly) {
   p.qInv = mixed ? quantInv(s) : 1.0;
   p.qUnit = float(1.0 / p.qInv);
   p.listOf = s.nLists > 1 ? s.seg.as<double>() : nullptr;
+  p.slotList = mixed ? slotLists(s) : nullptr;
   OutArgs o{};
   o.base = s.out.as<double>();
   o.n = n;

@@ -231,6 +231,7 @@ struct Sim {
   DBuf listGamma;                // per NodeList gamma (device), empty: the integrator's gamma
   bool sumAllLists = true;
   DBuf seg;                      // member index per caller row (as a double)
+  DBuf slotList;                 // several NodeLists, mixed precision: NodeList of every slot (-1: dead)
   DBuf segGrid;                  // neighbour grid list (levels, members), in one buffer:
   void* gridList = nullptr;      //   the grids (DGrid, neighbors.hip),
   int64_t* gridCellBase = nullptr;  // their first global cells (+1 entry),

@@ -175,11 +175,6 @@ def test_nodelist_argument_errors(sphgpu):
     s.set_ghost_nodes(2.0, cull=True)
     with pytest.raises(sphgpu.SphGpuError):
         s.set_node_lists(counts)                 # ghosts exist: internal rows only
-    m = device(sphgpu, dim, st, counts, "reflecting", 1)
-    m.set_precision(sphgpu.PRECISION_MIXED)
-    m.set_ghost_nodes(2.0, cull=True)
-    with pytest.raises(sphgpu.SphGpuError):
-        m.sum_density()                          # mixed precision takes one NodeList
 
 
 GRADH_CASES = [(1, 80, "reflecting"), (2, 14, "periodic"), (3, 8, "reflecting")]
@@ -303,3 +298,70 @@ def test_iterate_ideal_h_with_nodelists(sphgpu, oracle, case):
         assert np.abs(Hd - Hr).max() <= 1e-12 * np.abs(Hr).max()
     s.sum_density()
     s.evaluate_derivatives(sphgpu.hydro_params())
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64).ravel(), np.asarray(b, dtype=np.float64).ravel()
+    return (float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)),
+            float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)))
+
+
+MIXED_CASES = [(1, 80, "reflecting"), (2, 14, "periodic"), (3, 8, "reflecting")]
+
+
+@pytest.mark.parametrize("sum_all", [True, False], ids=["sumall", "ownlist"])
+@pytest.mark.parametrize("case", MIXED_CASES, ids=[f"{c[2]}{c[0]}d" for c in MIXED_CASES])
+def test_mixed_pass_with_nodelists(sphgpu, oracle, case, sum_all):
+    """Mixed precision over two NodeLists: the density sum's cross-NodeList
+    rule and the same-NodeList rhoSum and normalization in the FP32 pair
+    kernels, each field within 1e-5 of the FP64 oracle (relative L-inf and
+    L2)."""
+    dim, n, kind = case
+    print(f"[nodelists-mixed] dim={dim} n={n} {kind} sum_all={sum_all}", flush=True)
+    st, lists, counts = two_lists(dim, n, seed=180 + dim, mass_ratio=2.5)
+    N = lists.size
+    bo = oracle.box_boundaries(dim, kind=kind)
+    full, off, nbr, info = oracle.ghost_pipeline(dim, st, bo, 2.0, True, lists=lists)
+    T = oracle.Table(dim, 1000)
+    rho = oracle.sum_density(T, N, full["pos"], full["mass"], full["H"], off, nbr, lists=full["list"],
+                             sum_all=sum_all)
+    rho_all = np.array(full["rho"])
+    rho_all[:N] = rho
+    full["rho"] = oracle.apply_ghosts(dim, N, bo, info["control"], info["sub_of"], rho_all, 0)
+    ref = oracle.eval_derivs(T, oracle.make_params(), N, full, off, nbr, lists=full["list"])
+    s = device(sphgpu, dim, st, counts, kind, sphgpu.ROW_ORDER_SEARCH, sum_all=sum_all)
+    s.set_precision(sphgpu.PRECISION_MIXED)
+    s.set_ghost_nodes(2.0, cull=True)
+    s.sum_density()
+    bad = [("rho",) + rel_err(s.download("mass density", rows=N), rho)]
+    s.evaluate_derivatives(sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED))
+    for f in sphgpu.DERIV_ALIASES:
+        bad.append((f,) + rel_err(s.derivative(f).reshape(ref[f].shape), ref[f]))
+    bad = [b for b in bad if max(b[1:]) > 1e-5]
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("case", STEP_CASES[:2], ids=[f"{c[2]}{c[0]}d" for c in STEP_CASES[:2]])
+def test_rk2_mixed_with_nodelists(sphgpu, oracle, case):
+    """Mixed-precision SynchronousRK2 steps over two NodeLists (gamma 1.4 and
+    5/3): dt and state within 1e-5 of the FP64 oracle's."""
+    import integrator
+    dim, n, kind, steps = case
+    print(f"[nodelists-rk2-mixed] dim={dim} n={n} {kind}", flush=True)
+    gammas = (1.4, 5.0 / 3.0)
+    st, lists, counts = two_lists(dim, n, seed=190 + dim, mass_ratio=2.0, gamma=gammas)
+    N = lists.size
+    base = {k: st[k] for k in ("pos", "vel", "mass", "rho", "eps", "H")}
+    ref = integrator.RK2(dim, base, oracle.box_boundaries(dim, kind=kind), gammas[0], lists=lists, gammas=gammas)
+    s = device(sphgpu, dim, base, counts, kind, sphgpu.ROW_ORDER_SEARCH, gamma=gammas)
+    s.set_precision(sphgpu.PRECISION_MIXED)
+    s.eos_gamma_law(gammas[0])
+    hp = sphgpu.hydro_params(precision=sphgpu.PRECISION_MIXED)
+    ip = sphgpu.integrator_params(gamma=gammas[0])
+    for k in range(steps):
+        dref = ref.step()
+        ddev = s.rk2_step(hp, ip)
+        assert ddev == pytest.approx(dref, rel=1e-5), (k, ddev, dref)
+    for key, f in STEP_FIELDS.items():
+        err = rel_err(s.download(f, rows=N).reshape(ref.st[key].shape), ref.st[key])
+        assert max(err) <= 1e-5, (key, err)
