@@ -426,7 +426,7 @@ struct RowWriter {
 constexpr int kStageEdge = 8;                       // 4-cell brick + 2 cells each side
 constexpr int kStaged = kStageEdge * kStageEdge * kStageEdge;
 constexpr int kCandCap = 2048;
-constexpr int kRunRows = 29;
+constexpr int kRunRows = 9;      // one z slab: 5 runs, 3 sentinels, the read-ahead row
 
 struct SearchArgs {
   DGrid g;
@@ -453,13 +453,17 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
                                                          int4* __restrict__ nbr, int32_t* __restrict__ flags) {
   using R = RL<true>;
   __shared__ int32_t sOff[kStaged + 1];
-  __shared__ int32_t sStart[kStaged];
   __shared__ float4 sCand[kCandCap];               // x,y,z offsets from the brick origin, (1/h)^2 (FP32)
   __shared__ int32_t sSlot[kCandCap];
-  __shared__ int32_t sc[kBlock];
-  // per lane: its runs (t0 | t1 << 16), lane-strided; up to 25 runs, three
-  // sentinels and the row the reader reads ahead of them
+  // per lane: the runs of one z slab of its stencil (t0 | t1 << 16),
+  // lane-strided; up to 5 runs, three sentinels and the row the reader reads
+  // ahead of them.  Before the runs are formed the same memory holds each
+  // staged cell's first slot and the scan's partial sums (52 KB in all: three
+  // workgroups per CU)
   __shared__ uint32_t sRun[kRunRows * kBlock];
+  static_assert(kRunRows * kBlock >= kStaged + kBlock, "sStart and sc alias the run table");
+  int32_t* const sStart = reinterpret_cast<int32_t*>(sRun);
+  int32_t* const sc = sStart + kStaged;
   const DGrid& g = a.g;
   const int b = int(xcdBlock(int64_t(g.nb[0]) * g.nb[1] * g.nb[2]));
   if (b >= g.nb[0] * g.nb[1] * g.nb[2]) return;   // uniform over the block
@@ -529,6 +533,8 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
   // are contiguous candidate ranges in LDS.  Rows leave as int4 chunks, one
   // per lane, contiguous across a wavefront's slots.  Bricks whose candidates
   // exceed the stage read them from the records in global memory.
+  // the cell table's first slots are read again only by bricks that are not
+  // staged, which write no runs
   for (int64_t k = h0 + threadIdx.x; k < h1; k += blockDim.x) {
     const double* ri = rec + k * R::STRIDE;
     if (int64_t(ri[R::ORIG]) >= a.nInt) { rowCount[k] = 0; continue; }
@@ -541,131 +547,133 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
     const int sx = c[0] - cx0, sy = c[1] - cy0, sz = c[2] - cz0;   // in [2, 5]
     RowWriter row(nbr, a.n, k, a.cap);
     if (staged) {
-      // 1. the node's non-empty runs, in (dz, dy) order, into its column of
-      //    the run table: the candidates of each (dz, dy) row of staged cells
-      //    inside the x chord of a sphere of the search radius (with a 1e-4
-      //    margin) around the node
-      int nr = 0, nCand = 0;
       for (int dz = -2; dz <= 2; ++dz) {
-        // distance from the node to the cell slab along z (staged cell t spans
-        // [(t - 2) c, (t - 1) c) in brick-origin offsets)
-        const float zlo = float(sz + dz - 2) * a.cell32, zhi = zlo + a.cell32;
-        const float dzm = fz < zlo ? zlo - fz : (fz > zhi ? fz - zhi : 0.f);
-        for (int dy = -2; dy <= 2; ++dy) {
-          const float ylo = float(sy + dy - 2) * a.cell32, yhi = ylo + a.cell32;
-          const float dym = fy < ylo ? ylo - fy : (fy > yhi ? fy - yhi : 0.f);
-          const float rem2 = a.prune2 - (dzm * dzm + dym * dym);
-          if (rem2 < 0.f) continue;
-          const float rem = __builtin_sqrtf(rem2);
-          int x0 = int(floorf((fx - rem) * a.invCell32)) + 2, x1 = int(floorf((fx + rem) * a.invCell32)) + 2;
-          x0 = x0 < sx - 2 ? sx - 2 : x0;
-          x1 = x1 > sx + 2 ? sx + 2 : x1;
-          if (x0 > x1) continue;
-          const int rowb = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge;
-          // the members of a cell are in x order: the chord [fx - rem, fx + rem]
-          // starts and ends inside the end cells, found by two bisections
-          int32_t t0 = sOff[rowb + x0], t1 = sOff[rowb + x1 + 1];
-          {
-            int32_t lo = t0, up = sOff[rowb + x0 + 1];
-            const float key = fx - rem;
-            while (lo < up) {
-              const int32_t mid = (lo + up) >> 1;
-              const bool below = sCand[mid].x < key;
-              lo = below ? mid + 1 : lo;
-              up = below ? up : mid;
+        // 1. the node's non-empty runs of this z slab, in dy order, into its
+        //    column of the run table: the candidates of each (dz, dy) row of
+        //    staged cells inside the x chord of a sphere of the search radius
+        //    (with a 1e-4 margin) around the node
+        int nr = 0, nCand = 0;
+        {
+          // distance from the node to the cell slab along z (staged cell t spans
+          // [(t - 2) c, (t - 1) c) in brick-origin offsets)
+          const float zlo = float(sz + dz - 2) * a.cell32, zhi = zlo + a.cell32;
+          const float dzm = fz < zlo ? zlo - fz : (fz > zhi ? fz - zhi : 0.f);
+          for (int dy = -2; dy <= 2; ++dy) {
+            const float ylo = float(sy + dy - 2) * a.cell32, yhi = ylo + a.cell32;
+            const float dym = fy < ylo ? ylo - fy : (fy > yhi ? fy - yhi : 0.f);
+            const float rem2 = a.prune2 - (dzm * dzm + dym * dym);
+            if (rem2 < 0.f) continue;
+            const float rem = __builtin_sqrtf(rem2);
+            int x0 = int(floorf((fx - rem) * a.invCell32)) + 2, x1 = int(floorf((fx + rem) * a.invCell32)) + 2;
+            x0 = x0 < sx - 2 ? sx - 2 : x0;
+            x1 = x1 > sx + 2 ? sx + 2 : x1;
+            if (x0 > x1) continue;
+            const int rowb = ((sz + dz) * kStageEdge + (sy + dy)) * kStageEdge;
+            // the members of a cell are in x order: the chord [fx - rem, fx + rem]
+            // starts and ends inside the end cells, found by two bisections
+            int32_t t0 = sOff[rowb + x0], t1 = sOff[rowb + x1 + 1];
+            {
+              int32_t lo = t0, up = sOff[rowb + x0 + 1];
+              const float key = fx - rem;
+              while (lo < up) {
+                const int32_t mid = (lo + up) >> 1;
+                const bool below = sCand[mid].x < key;
+                lo = below ? mid + 1 : lo;
+                up = below ? up : mid;
+              }
+              t0 = lo;
             }
-            t0 = lo;
-          }
-          {
-            int32_t lo = sOff[rowb + x1], up = t1;
-            const float key = fx + rem;
-            while (lo < up) {
-              const int32_t mid = (lo + up) >> 1;
-              const bool inside = sCand[mid].x <= key;
-              lo = inside ? mid + 1 : lo;
-              up = inside ? up : mid;
+            {
+              int32_t lo = sOff[rowb + x1], up = t1;
+              const float key = fx + rem;
+              while (lo < up) {
+                const int32_t mid = (lo + up) >> 1;
+                const bool inside = sCand[mid].x <= key;
+                lo = inside ? mid + 1 : lo;
+                up = inside ? up : mid;
+              }
+              t1 = lo;
             }
-            t1 = lo;
-          }
-          if (t1 > t0) {
-            sRun[nr++ * kBlock + threadIdx.x] = uint32_t(t0) | (uint32_t(t1) << 16);
-            nCand += t1 - t0;
+            if (t1 > t0) {
+              sRun[nr++ * kBlock + threadIdx.x] = uint32_t(t0) | (uint32_t(t1) << 16);
+              nCand += t1 - t0;
+            }
           }
         }
-      }
-      // 2. one walk over the runs' candidates: a wavefront iterates as often
-      //    as its longest candidate list, not its longest run in every
-      //    (dz, dy) row.  Candidates are read two ahead of their test (three
-      //    register sets in rotation); the reader steps through the runs with
-      //    no branch and, past the last candidate, through three sentinel runs
-      //    (candidate 0), so every index it forms stays inside the tables.
-      //    The rare paths (the exact FP64 test, ghost marks) are entered only
-      //    when some lane of the wavefront needs them.
-      // one instance of the walk with ghost marks and one without, chosen by
-      // the block-uniform flag
-      auto walk = [&](auto marksOn) {
-        for (int q = 0; q < 3; ++q) sRun[(nr + q) * kBlock + threadIdx.x] = 0u;
-        const uint32_t first = sRun[threadIdx.x];
-        int32_t t = int32_t(first & 0xffffu), tEnd = int32_t(first >> 16);
-        uint32_t nxt = sRun[kBlock + threadIdx.x];
-        int r = 0;
-        auto advance = [&]() {
-          const bool same = t + 1 < tEnd;
-          r = same ? r : r + 1;
-          t = same ? t + 1 : int32_t(nxt & 0xffffu);
-          tEnd = same ? tEnd : int32_t(nxt >> 16);
-          nxt = sRun[(r + 1) * kBlock + threadIdx.x];
-        };
-        auto read = [&](float4& c, int32_t& raw) {
-          c = sCand[t];
-          raw = sSlot[t];
-        };
-        auto test = [&](const float4 cj, const int32_t raw) {
-          const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
-          // FP32 estimate of min(h_i^2, h_j^2) |r|^2: fused products (any
-          // rounding change stays inside the band that takes the exact test);
-          // q min(a, b) = min(q a, q b) exactly for q >= 0
-          const float q = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-          const float tmin = q * (cj.w < hi2 ? cj.w : hi2);
-          bool in = tmin < a.lo32;
-          const bool band = (tmin >= a.lo32) & (tmin <= a.hi32);
-          const int32_t sj = raw & 0x7fffffff;
-          // the exact test in the rare band (the branch is skipped when no
-          // lane of the wavefront takes it)
-          if (band) {
-            const double* rj = rec + int64_t(sj) * R::STRIDE;
-            in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
-          }
-          const bool acc = in && sj != int32_t(k);
-          row.pushIf(acc, sj);
-          if constexpr (decltype(marksOn)::value) {
-            if (acc && raw < 0) a.keepSlot[sj] = 1;
-          }
-        };
-        float4 c0, c1, c2;
-        int32_t s0, s1, s2;
-        read(c0, s0);
-        advance();
-        read(c1, s1);
-        advance();
-        for (int left = nCand;;) {
-          read(c2, s2);
-          advance();
-          test(c0, s0);
-          if (--left == 0) break;
+        // 2. one walk over the slab's candidates: a wavefront iterates as often
+        //    as its longest candidate list in the slab, not its longest run in
+        //    every (dz, dy) row.  Candidates are read two ahead of their test (three
+        //    register sets in rotation); the reader steps through the runs with
+        //    no branch and, past the last candidate, through three sentinel runs
+        //    (candidate 0), so every index it forms stays inside the tables.
+        //    The rare paths (the exact FP64 test, ghost marks) are entered only
+        //    when some lane of the wavefront needs them.
+        // one instance of the walk with ghost marks and one without, chosen by
+        // the block-uniform flag
+        auto walk = [&](auto marksOn) {
+          for (int q = 0; q < 3; ++q) sRun[(nr + q) * kBlock + threadIdx.x] = 0u;
+          const uint32_t first = sRun[threadIdx.x];
+          int32_t t = int32_t(first & 0xffffu), tEnd = int32_t(first >> 16);
+          uint32_t nxt = sRun[kBlock + threadIdx.x];
+          int r = 0;
+          auto advance = [&]() {
+            const bool same = t + 1 < tEnd;
+            r = same ? r : r + 1;
+            t = same ? t + 1 : int32_t(nxt & 0xffffu);
+            tEnd = same ? tEnd : int32_t(nxt >> 16);
+            nxt = sRun[(r + 1) * kBlock + threadIdx.x];
+          };
+          auto read = [&](float4& c, int32_t& raw) {
+            c = sCand[t];
+            raw = sSlot[t];
+          };
+          auto test = [&](const float4 cj, const int32_t raw) {
+            const float rx = fx - cj.x, ry = fy - cj.y, rz = fz - cj.z;
+            // FP32 estimate of min(h_i^2, h_j^2) |r|^2: fused products (any
+            // rounding change stays inside the band that takes the exact test);
+            // q min(a, b) = min(q a, q b) exactly for q >= 0
+            const float q = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+            const float tmin = q * (cj.w < hi2 ? cj.w : hi2);
+            bool in = tmin < a.lo32;
+            const bool band = (tmin >= a.lo32) & (tmin <= a.hi32);
+            const int32_t sj = raw & 0x7fffffff;
+            // the exact test in the rare band (the branch is skipped when no
+            // lane of the wavefront takes it)
+            if (band) {
+              const double* rj = rec + int64_t(sj) * R::STRIDE;
+              in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
+            }
+            const bool acc = in && sj != int32_t(k);
+            row.pushIf(acc, sj);
+            if constexpr (decltype(marksOn)::value) {
+              if (acc && raw < 0) a.keepSlot[sj] = 1;
+            }
+          };
+          float4 c0, c1, c2;
+          int32_t s0, s1, s2;
           read(c0, s0);
           advance();
-          test(c1, s1);
-          if (--left == 0) break;
           read(c1, s1);
           advance();
-          test(c2, s2);
-          if (--left == 0) break;
+          for (int left = nCand;;) {
+            read(c2, s2);
+            advance();
+            test(c0, s0);
+            if (--left == 0) break;
+            read(c0, s0);
+            advance();
+            test(c1, s1);
+            if (--left == 0) break;
+            read(c1, s1);
+            advance();
+            test(c2, s2);
+            if (--left == 0) break;
+          }
+        };
+        if (nr > 0) {
+          if (marks) walk(std::true_type{});
+          else walk(std::false_type{});
         }
-      };
-      if (nr > 0) {
-        if (marks) walk(std::true_type{});
-        else walk(std::false_type{});
       }
     } else {
       for (int dz = -2; dz <= 2; ++dz) {
