@@ -400,13 +400,15 @@ struct RowWriter {
   }
   // push(v) when acc, with the entry shifted in by selects: only the chunk
   // store is conditional
+  // (ROOM: the caller has shown that the row stays within its capacity)
+  template <bool ROOM = false>
   __device__ __forceinline__ void pushIf(bool acc, int32_t v) {
     buf.x = acc ? buf.y : buf.x;
     buf.y = acc ? buf.z : buf.y;
     buf.z = acc ? buf.w : buf.z;
     buf.w = acc ? v : buf.w;
     cnt += acc ? 1 : 0;
-    if (acc && (cnt & 3) == 0 && cnt <= cap) {
+    if (acc && (cnt & 3) == 0 && (ROOM || cnt <= cap)) {
       *at = buf;
       at += n;
     }
@@ -610,7 +612,11 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
         //    when some lane of the wavefront needs them.
         // one instance of the walk with ghost marks and one without, chosen by
         // the block-uniform flag
-        auto walk = [&](auto marksOn) {
+        // plain: no lane of the wavefront can meet itself in this slab (only
+        // the dz = 0 slab holds the node's own cell) or pass its row capacity
+        // in it, so the walk skips both checks
+        auto walk = [&](auto marksOn, auto plainOn) {
+          constexpr bool plain = decltype(plainOn)::value;
           for (int q = 0; q < 3; ++q) sRun[(nr + q) * kBlock + threadIdx.x] = 0u;
           const uint32_t first = sRun[threadIdx.x];
           int32_t t = int32_t(first & 0xffffu), tEnd = int32_t(first >> 16);
@@ -643,8 +649,8 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
               const double* rj = rec + int64_t(sj) * R::STRIDE;
               in = exactIso(xi - rj[R::X], yi - rj[R::X + 1], zi - rj[R::X + 2], hi, rj[R::HINV], a.ext2);
             }
-            const bool acc = in && sj != int32_t(k);
-            row.pushIf(acc, sj);
+            const bool acc = in && (plain || sj != int32_t(k));
+            row.template pushIf<plain>(acc, sj);
             if constexpr (decltype(marksOn)::value) {
               if (acc && raw < 0) a.keepSlot[sj] = 1;
             }
@@ -671,8 +677,14 @@ __global__ void __launch_bounds__(kBlock) k_search_staged(const double* __restri
           }
         };
         if (nr > 0) {
-          if (marks) walk(std::true_type{});
-          else walk(std::false_type{});
+          const bool plain = dz != 0 && __all(row.cnt + nCand <= a.cap);
+          if (marks) {
+            if (plain) walk(std::true_type{}, std::true_type{});
+            else walk(std::true_type{}, std::false_type{});
+          } else {
+            if (plain) walk(std::false_type{}, std::true_type{});
+            else walk(std::false_type{}, std::false_type{});
+          }
         }
       }
     } else {
